@@ -1,0 +1,165 @@
+"""Headline benchmark: forward MPM sand substeps at 1M particles on one GPU.
+
+Workload: sand column collapse, 1,000,000 particles i.i.d. uniform in
+[0.3,0.7]^2 x [0.05,0.6] (seed 0), v ~ N(0, 0.1) per component (seed 1),
+128^3 grid on the unit cube, Drucker-Prager sand E=2e5 Pa, nu=0.3, 30 deg,
+density 2200, gravity -9.8 z, dt=1e-4, sticky floor at z=0.05 + bounding
+box, FP32.  A step is one substep (p2g2p) over all particles; the state is
+resident in HBM before the timed region starts.
+
+Prints one JSON line with the metric, the roofline of the dominant kernel
+(algorithmic bytes per launch / its average duration, HIP events on the
+launch stream) and the CPU baseline (the FP32 CPU restatement of the same
+substep, OpenMP over the host cores, timed on a bounded sample).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+METRIC = "particle-substeps/s (fwd; fwd+adjoint) at 1M particles; % HBM roofline"
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes per particle and launch (FP32 state that must cross HBM)
+ALG_BYTES = {
+    # whole substep: read x0,u,v,C,F,tau,m,vol (152) + write u,v,C,F,tau (132)
+    "step": 284,
+    # P2G: read x0,u,v,C,tau,m,vol
+    "p2g": 116,
+    # G2P: read x0,u,F; write u,v,C,F,tau
+    "g2p": 192,
+}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--particles", type=int, default=1_000_000)
+    ap.add_argument("--n-grid", type=int, default=128)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="budget of the CPU baseline sample (0 disables it)")
+    ap.add_argument("--stage-steps", type=int, default=20,
+                    help="substeps timed per stage for the roofline")
+    return ap.parse_args()
+
+
+def cpu_baseline(scene, budget_s):
+    """FP32 CPU restatement (oracle/, OpenMP) on the same workload for as many
+    substeps as fit in the budget (at least 1)."""
+    from oracle import oracle_lib
+    from reduced_mpm_stress_training_amd import scenes as S
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    cores = max(1, min(cores, 16))
+    oracle_lib.build()
+    o = oracle_lib.OracleScene(S.scene_params(scene), S.scene_state(scene, np.float32),
+                               S.scene_colliders(scene), np.float32)
+    o.step(scene.dt, nthreads=cores)          # first touch / warm-up
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        o.step(scene.dt, nthreads=cores)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or el / steps * (steps + 1) > 2 * budget_s:
+            break
+    return {"value": scene.n_particles * steps / el, "unit": "particle-substeps/s", "cores": cores,
+            "kind": "port",
+            "sample": f"{steps} substeps of the same {scene.n_particles}-particle, {scene.n_grid}^3 workload "
+                      f"({el:.1f} s), FP32 C restatement with OpenMP atomics"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the substep is not sharded "
+              "across GPUs in this build", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from reduced_mpm_stress_training_amd import scenes as S
+
+    scene = S.column_uniform(n_particles=args.particles, n_grid=args.n_grid)
+    sim = S.make_simulator(scene)
+    n = scene.n_particles
+    dt = scene.dt
+    print(f"[bench] workload ready: {n} particles, {scene.n_grid}^3 grid", file=sys.stderr, flush=True)
+
+    for k in range(args.warmup):
+        sim.p2g2p(k, dt)
+    torch.cuda.synchronize()
+    t_start = torch.cuda.Event(enable_timing=True)
+    t_end = torch.cuda.Event(enable_timing=True)
+    t_start.record()
+    for k in range(args.steps):
+        sim.p2g2p(k, dt)
+    t_end.record()
+    torch.cuda.synchronize()
+    ms_total = t_start.elapsed_time(t_end)
+    ms_step = ms_total / args.steps
+    value = n * args.steps / (ms_total / 1e3)
+    print(f"[bench] timed {args.steps} substeps: {ms_step:.4f} ms/substep", file=sys.stderr, flush=True)
+
+    # per-stage durations, HIP events on the launch stream
+    sim.stage_timer = []
+    for k in range(args.stage_steps):
+        sim.p2g2p(k, dt)
+    torch.cuda.synchronize()
+    names = ["zero_grid", "p2g", "grid_update", "g2p"]
+    dur = {nm: 0.0 for nm in names}
+    for ev in sim.stage_timer:
+        for i, nm in enumerate(names):
+            dur[nm] += ev[i].elapsed_time(ev[i + 1])
+    dur = {k: v / len(sim.stage_timer) for k, v in dur.items()}
+    sim.stage_timer = None
+    dom = max(("p2g", "g2p"), key=lambda k: dur[k])
+    achieved = ALG_BYTES[dom] * n / (dur[dom] / 1e3) / 1e9
+    step_gbs = ALG_BYTES["step"] * n / (ms_step / 1e3) / 1e9
+
+    u, v, Fd, tau = sim.export_uvFstress_from_warp()
+    finite = bool(torch.isfinite(u).all() and torch.isfinite(tau).all())
+
+    cpu = None
+    if args.cpu_seconds > 0:
+        print("[bench] CPU baseline ...", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(scene, args.cpu_seconds)
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "sand column collapse, forward substep", "n_particles": n,
+                   "n_grid": scene.n_grid, "box": "[0.3,0.7]^2x[0.05,0.6]", "dt": dt, "E": scene.E,
+                   "nu": scene.nu, "friction_angle": scene.friction_angle, "density": scene.density,
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_particle": ALG_BYTES[dom],
+                     "stage_ms": dur, "step_achieved_gbs": step_gbs, "step_frac": step_gbs / HBM_PEAK_GBS},
+        "cpu_baseline": cpu,
+        "finite": finite,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

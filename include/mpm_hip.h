@@ -1,0 +1,149 @@
+/*
+ * mpm_hip.h -- C-ABI of libmpm_hip.so, the MI355X (gfx950) explicit
+ * MLS/APIC-MPM sand substep.
+ *
+ * The reference (zeshunzong/reduced_mpm_stress_training) is Python + Warp:
+ * its "FFI" is `wp.launch(kernel, dim, inputs=[MPMStateStruct,
+ * MPMModelStruct, dt])` (engine_stress/mpm_solver_warp.py:156-203).  Each
+ * entry point below replaces one of those launches (or a fused run of them)
+ * and cites it.  The Python solver class in
+ * reduced_mpm_stress_training_amd/mpm_solver.py binds them with ctypes.
+ *
+ * Conventions
+ *  - All pointers are device pointers into caller-owned memory (torch
+ *    tensors); the library never allocates behind the caller's back except
+ *    where an entry point says so.
+ *  - Every call is stream-ordered on `stream` (a hipStream_t; NULL = the
+ *    legacy default stream) and does not synchronise the host.
+ *  - Return value: 0 = OK, negative = error; mpm_last_error() gives the
+ *    message of the last failing call on the calling host thread.
+ *
+ * Particle state layout ("planes"): one float32 buffer of MPM_NPLANES planes,
+ * plane k starting at planes + k*stride (stride >= n, in elements), indexed by
+ * solver slot.  Plane order: x0(3) u(3) v(3) C(9) F(9) tau(9) mass vol pid,
+ * with 3x3 matrices row-major (component a*3+b) like Warp mat33 and torch
+ * (N,3,3).  The pid plane holds the caller's particle index of each slot as
+ * int32 bits.
+ *
+ * Grid layout: n_grid^3 nodes in C order [ix][iy][iz] (as the reference's
+ * wp.array(ndim=3)), node i at position i*dx, one float4 per node:
+ *   accumulator  (mv.x, mv.y, mv.z, m)   -- grid_v_in and grid_m
+ *   velocity     (v.x,  v.y,  v.z,  m)   -- grid_v_out
+ */
+#ifndef MPM_HIP_H
+#define MPM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPM_ABI_VERSION 1
+
+/* plane indices */
+#define MPM_P_X0 0
+#define MPM_P_U 3
+#define MPM_P_V 6
+#define MPM_P_C 9
+#define MPM_P_F 18
+#define MPM_P_TAU 27
+#define MPM_P_MASS 36
+#define MPM_P_VOL 37
+#define MPM_P_PID 38
+#define MPM_NPLANES 39
+
+#define MPM_MAX_COLLIDERS 16
+
+/* Model scalars: MPMModelStruct (engine_stress/warp_utils.py:61-81).
+ * gravity generalises the reference's (0, -g, 0) (mpm_utils.py:236). */
+typedef struct MpmParams {
+    int32_t n_grid;
+    int32_t material; /* 2 = sand (Drucker-Prager); the only material on this path */
+    float dx, inv_dx;
+    float mu, lam, alpha;
+    float gravity[3];
+} MpmParams;
+
+/* Dirichlet_collider (engine_stress/warp_utils.py:126-151).
+ * kind 0: add_surface_collider (mpm_solver_warp.py:407-451) -- every surface
+ *         type writes v = 0 on the far side, as the reference does (:449);
+ * kind 1: add_bounding_box (mpm_solver_warp.py:499-525), padding 3. */
+typedef struct MpmCollider {
+    int32_t kind;
+    int32_t surface_type;
+    float point[3];
+    float normal[3];
+    float start_time, end_time;
+    float friction;
+} MpmCollider;
+
+/* Particle state: MPMStateStruct particle arrays (warp_utils.py:84-124). */
+typedef struct MpmState {
+    int64_t n;      /* particles */
+    int64_t stride; /* plane stride in elements */
+    float* planes;  /* MPM_NPLANES planes */
+} MpmState;
+
+int mpm_abi_version(void);
+const char* mpm_last_error(void);
+
+/* ---- stage entry points (one reference launch each) ------------------- */
+
+/* zero_grid (mpm_utils.py:115-123): clears n_grid^3 float4 nodes. */
+int mpm_zero_grid(const MpmParams* prm, float* grid, void* stream);
+
+/* update_x_from_u (mpm_utils.py:4-7) fused into p2g_apic_stress_given
+ * (mpm_utils.py:190-227): scatters m*(v + C dpos) + dt*(-vol tau grad w) and
+ * w*m of every particle into the accumulator grid (float atomics). */
+int mpm_p2g(const MpmParams* prm, const MpmState* st, float* grid_acc, float dt,
+            void* stream);
+
+/* grid_normalization_and_gravity (mpm_utils.py:229-237) followed by the
+ * colliders in registration order (mpm_solver_warp.py:182-185); `time` is the
+ * solver time before the step's increment (mpm_solver_warp.py:183,199).
+ * grid_v receives (v, m); nodes with m <= 1e-15 get v = 0. */
+int mpm_grid_update(const MpmParams* prm, const float* grid_acc, float* grid_v,
+                    const MpmCollider* colliders, int n_colliders, float time,
+                    float dt, void* stream);
+
+/* g2p_apic_and_stress (mpm_utils.py:241-306) with sand_return_mapping
+ * (:87-112) and kirchoff_stress_drucker_prager (:45-52), fused with
+ * update_x_from_u (:4-7) and update_u_from_x (:12-15): writes u, v, C, F,
+ * tau of every particle. */
+int mpm_g2p(const MpmParams* prm, const MpmState* st, const float* grid_v, float dt,
+            void* stream);
+
+/* ---- whole substep ----------------------------------------------------- */
+
+/* Bytes of device workspace mpm_step needs for this grid and particle count. */
+int64_t mpm_workspace_bytes(const MpmParams* prm, int64_t n_particles);
+
+/* One substep, MPM_Simulator_WARP.p2g2p (mpm_solver_warp.py:156-203):
+ * x = x0 + u, zero grid, P2G, grid update + colliders, G2P, u = x - x0.
+ * workspace: mpm_workspace_bytes() bytes, 256-byte aligned. */
+int mpm_step(const MpmParams* prm, const MpmState* st, const MpmCollider* colliders,
+             int n_colliders, void* workspace, int64_t workspace_bytes, float time,
+             float dt, void* stream);
+
+/* ---- state import / export (torch loaders and export_uvFstress_from_warp,
+ *      mpm_solver_warp.py:229-266 and :305-310) --------------------------- */
+
+/* Copies `ncomp` components of an (n, ncomp) row-major array indexed by the
+ * caller's particle index into planes [first_plane, first_plane+ncomp) of
+ * every slot (slot <- src[pid[slot]]).  mode 0: copy; mode 2: add the
+ * identity (F = F_disp + I, ncomp 9). */
+int mpm_import(const MpmState* st, int first_plane, int ncomp, const float* src,
+               int mode, void* stream);
+
+/* Inverse of mpm_import: dst[pid[slot]] <- planes (mode 0 copy; mode 1
+ * subtract the identity, F_disp = F - I, update_F_disp_from_F
+ * mpm_utils.py:16-19; mode 3: x = x0 + u, update_x_from_u, ncomp 3). */
+int mpm_export(const MpmState* st, int first_plane, int ncomp, float* dst, int mode,
+               void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MPM_HIP_H */

@@ -1,0 +1,178 @@
+"""Independent FP64 NumPy restatement of the MPM sand substep.
+
+TEST INFRASTRUCTURE ONLY: this module is a checker.  Only tests/,
+``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` may
+import it; the product package never does.
+
+It restates the same reference functions as ``oracle/mpm_oracle.c`` but by a
+different route, so that the two pin each other:
+
+* the 3x3 SVD is LAPACK's (``numpy.linalg.svd``), sign-fixed to rotations;
+* P2G is a vectorised scatter with ``numpy.add.at`` instead of a loop;
+* G2P is a vectorised gather (einsum) instead of a per-particle loop.
+
+Reference citations (zeshunzong/reduced_mpm_stress_training):
+  update_x_from_u            engine_stress/mpm_utils.py:4-7
+  B-spline weights           engine_stress/mpm_utils.py:196-209, 244-257
+  compute_dweight            engine_stress/mpm_utils.py:54-57
+  p2g_apic_stress_given      engine_stress/mpm_utils.py:190-227
+  grid_normalization_and_gravity  engine_stress/mpm_utils.py:229-237
+  surface collider           engine_stress/mpm_solver_warp.py:429-449
+  bounding box               engine_stress/mpm_solver_warp.py:505-523
+  g2p_apic_and_stress        engine_stress/mpm_utils.py:241-306
+  sand_return_mapping        engine_stress/mpm_utils.py:87-112
+  kirchoff_stress_drucker_prager engine_stress/mpm_utils.py:45-52
+  update_u_from_x            engine_stress/mpm_utils.py:12-15
+  p2g2p (order)              engine_stress/mpm_solver_warp.py:156-203
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def svd3_rot(A: np.ndarray):
+    """SVD of a stack of 3x3 matrices with U, V rotations (det +1) and the
+    sign of det A carried by the smallest singular value (the convention of
+    Warp's wp.svd3 for mpm_utils.py:92,297)."""
+    U, s, Vt = np.linalg.svd(A)
+    s = s.copy()
+    dU = np.linalg.det(U)
+    dV = np.linalg.det(Vt)
+    fixU = dU < 0
+    U[fixU, :, 2] *= -1.0
+    s[fixU, 2] *= -1.0
+    fixV = dV < 0
+    Vt[fixV, 2, :] *= -1.0
+    s[fixV, 2] *= -1.0
+    return U, s, np.swapaxes(Vt, -1, -2)
+
+
+def weights(x: np.ndarray, inv_dx: float):
+    """Quadratic B-spline weights (mpm_utils.py:196-209).  Returns base (N,3)
+    int, fx (N,3), w (N,3 axes,3 nodes), dw (N,3,3)."""
+    gp = x * inv_dx
+    base = np.trunc(gp - 0.5).astype(np.int64)
+    fx = gp - base
+    w = np.stack([0.5 * (1.5 - fx) ** 2, 0.75 - (fx - 1.0) ** 2, 0.5 * (fx - 0.5) ** 2], axis=-1)
+    dw = np.stack([fx - 1.5, -2.0 * (fx - 1.0), fx - 0.5], axis=-1)
+    return base, fx, w, dw
+
+
+def _stencil(w, dw, fx, inv_dx):
+    """Per-node weight (N,27), gradient (N,27,3), offset (N,27,3) in grid units."""
+    off = np.array([(i, j, k) for i in range(3) for j in range(3) for k in range(3)])
+    wi = w[:, 0, off[:, 0]]
+    wj = w[:, 1, off[:, 1]]
+    wk = w[:, 2, off[:, 2]]
+    weight = wi * wj * wk
+    grad = np.stack([dw[:, 0, off[:, 0]] * wj * wk,
+                     wi * dw[:, 1, off[:, 1]] * wk,
+                     wi * wj * dw[:, 2, off[:, 2]]], axis=-1) * inv_dx
+    dpos = off[None, :, :] - fx[:, None, :]
+    return off, weight, grad, dpos
+
+
+def sand_return_map(Ftr, mu, lam, alpha):
+    """sand_return_mapping, mpm_utils.py:87-112."""
+    U, s, V = svd3_rot(Ftr)
+    eps = np.log(np.maximum(np.abs(s), 1e-14))
+    tr = eps.sum(-1)
+    eh = eps - tr[:, None] / 3.0
+    ehn = np.linalg.norm(eh, axis=-1)
+    dg = ehn + (3.0 * lam + 2.0 * mu) / (2.0 * mu) * tr * alpha
+    Fe = Ftr.copy()
+    b2 = (dg > 0) & (tr > 0)
+    b3 = (dg > 0) & (tr <= 0)
+    Fe[b2] = U[b2] @ np.swapaxes(V[b2], -1, -2)
+    if b3.any():
+        H = eps[b3] - eh[b3] * (dg[b3] / ehn[b3])[:, None]
+        Fe[b3] = (U[b3] * np.exp(H)[:, None, :]) @ np.swapaxes(V[b3], -1, -2)
+    return Fe
+
+
+def dp_stress(F, mu, lam):
+    """kirchoff_stress_drucker_prager (mpm_utils.py:45-52) on a fresh SVD of F
+    (mpm_utils.py:297)."""
+    U, s, V = svd3_rot(F)
+    ls = np.log(s).sum(-1)
+    c = 2.0 * mu * np.log(s) / s + lam * ls[:, None] / s
+    return (U * c[:, None, :]) @ np.swapaxes(V, -1, -2) @ np.swapaxes(F, -1, -2)
+
+
+def p2g(params, x, v, C, tau, mass, vol, dt):
+    """p2g_apic_stress_given (mpm_utils.py:190-227) -> grid (m, mv)."""
+    ng = params["n_grid"]
+    dx, inv_dx = params["dx"], params["inv_dx"]
+    base, fx, w, dw = weights(x, inv_dx)
+    off, weight, grad, dpos = _stencil(w, dw, fx, inv_dx)
+    dpos = dpos * dx
+    nodes = base[:, None, :] + off[None, :, :]
+    flat = (nodes[..., 0] * ng + nodes[..., 1]) * ng + nodes[..., 2]
+    wm = weight * mass[:, None]
+    mom = wm[..., None] * (v[:, None, :] + np.einsum("pab,pnb->pna", C, dpos))
+    force = np.einsum("pab,pnb->pna", -vol[:, None, None] * tau, grad)
+    grid_m = np.zeros(ng ** 3)
+    grid_mv = np.zeros((ng ** 3, 3))
+    np.add.at(grid_m, flat.ravel(), wm.ravel())
+    np.add.at(grid_mv, flat.ravel(), (mom + dt * force).reshape(-1, 3))
+    return grid_m, grid_mv
+
+
+def grid_update(params, grid_m, grid_mv, colliders, time, dt):
+    """grid_normalization_and_gravity (mpm_utils.py:229-237) followed by the
+    colliders in registration order (mpm_solver_warp.py:182-185)."""
+    ng = params["n_grid"]
+    g = np.asarray(params["gravity"], dtype=np.float64)
+    gv = np.zeros((ng ** 3, 3))
+    act = grid_m > 1e-15
+    gv[act] = grid_mv[act] * (1.0 / grid_m[act])[:, None] + dt * g
+    gv = gv.reshape(ng, ng, ng, 3)
+    idx = np.indices((ng, ng, ng)).transpose(1, 2, 3, 0)
+    for c in colliders:
+        if not (time >= c["start_time"] and time < c["end_time"]):
+            continue
+        if c["kind"] == 0:
+            off = idx * params["dx"] - np.asarray(c["point"])
+            below = (off @ np.asarray(c["normal"])) < 0.0
+            gv[below] = 0.0
+        else:
+            pad = 3
+            for d in range(3):
+                lo = (idx[..., d] < pad) & (gv[..., d] < 0)
+                gv[..., d][lo] = 0.0
+                hi = (idx[..., d] >= ng - pad) & (gv[..., d] > 0)
+                gv[..., d][hi] = 0.0
+    return gv.reshape(-1, 3)
+
+
+def g2p(params, grid_v, x, F, dt):
+    """g2p_apic_and_stress (mpm_utils.py:241-306), sand branch."""
+    ng = params["n_grid"]
+    inv_dx = params["inv_dx"]
+    base, fx, w, dw = weights(x, inv_dx)
+    off, weight, grad, dpos = _stencil(w, dw, fx, inv_dx)
+    nodes = base[:, None, :] + off[None, :, :]
+    flat = (nodes[..., 0] * ng + nodes[..., 1]) * ng + nodes[..., 2]
+    g = grid_v[flat]                                          # (N,27,3)
+    v_new = np.einsum("pn,pna->pa", weight, g)
+    C_new = np.einsum("pna,pnb,pn->pab", g, dpos, weight * inv_dx * 4.0)
+    gradv = np.einsum("pna,pnb->pab", g, grad)
+    x_new = x + dt * v_new
+    Ftr = (np.eye(3)[None] + gradv * dt) @ F
+    Fe = sand_return_map(Ftr, params["mu"], params["lam"], params["alpha"])
+    tau = dp_stress(Fe, params["mu"], params["lam"])
+    return x_new, v_new, C_new, Fe, tau
+
+
+def step(params, state, colliders, time, dt):
+    """One p2g2p substep (mpm_solver_warp.py:156-203).  ``state`` is a dict of
+    float64 arrays x0,u,v (N,3), C,F,tau (N,3,3), mass,vol (N,); returns a new
+    dict.  ``colliders`` is a list of dicts (kind, point, normal, start_time,
+    end_time)."""
+    x = state["x0"] + state["u"]
+    gm, gmv = p2g(params, x, state["v"], state["C"], state["tau"], state["mass"], state["vol"], dt)
+    gv = grid_update(params, gm, gmv, colliders, time, dt)
+    x_new, v, C, F, tau = g2p(params, gv, x, state["F"], dt)
+    out = dict(state)
+    out.update(u=x_new - state["x0"], v=v, C=C, F=F, tau=tau)
+    return out

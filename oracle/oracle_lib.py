@@ -1,0 +1,162 @@
+"""ctypes wrapper of the C oracle (oracle/_build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, ``__graft_entry__.smoke()`` and
+the ``cpu_baseline`` leg of ``bench.py``; never by the product package.
+
+``OracleScene`` holds one scene's state as NumPy arrays of the chosen
+precision (float32 = the reference's Warp arithmetic, float64 = the FP64
+oracle) and advances it with ``oracle_step_{f32,f64}`` (a restatement of
+``MPM_Simulator_WARP.p2g2p``, engine_stress/mpm_solver_warp.py:156-203).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+
+
+class OracleParams(C.Structure):
+    _fields_ = [("n_grid", C.c_int32), ("material", C.c_int32),
+                ("dx", C.c_double), ("inv_dx", C.c_double),
+                ("mu", C.c_double), ("lam", C.c_double), ("alpha", C.c_double),
+                ("gravity", C.c_double * 3)]
+
+
+class OracleCollider(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("surface_type", C.c_int32),
+                ("point", C.c_double * 3), ("normal", C.c_double * 3),
+                ("start_time", C.c_double), ("end_time", C.c_double),
+                ("friction", C.c_double)]
+
+
+def build() -> str:
+    """Compile the oracle with its Makefile (gcc); returns the library path."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = C.CDLL(LIB_PATH)
+    return _lib
+
+
+def _p(a: np.ndarray):
+    assert a.flags.c_contiguous
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _sfx(dtype) -> str:
+    return "_f64" if np.dtype(dtype) == np.float64 else "_f32"
+
+
+def make_params(n_grid, dx, inv_dx, mu, lam, alpha, gravity, material=2) -> OracleParams:
+    p = OracleParams()
+    p.n_grid = int(n_grid)
+    p.material = int(material)
+    p.dx, p.inv_dx, p.mu, p.lam, p.alpha = float(dx), float(inv_dx), float(mu), float(lam), float(alpha)
+    for i in range(3):
+        p.gravity[i] = float(gravity[i])
+    return p
+
+
+def make_colliders(colliders):
+    arr = (OracleCollider * max(1, len(colliders)))()
+    for i, c in enumerate(colliders):
+        arr[i].kind = int(c["kind"])
+        arr[i].surface_type = int(c.get("surface_type", 0))
+        for d in range(3):
+            arr[i].point[d] = float(c.get("point", (0, 0, 0))[d])
+            arr[i].normal[d] = float(c.get("normal", (0, 0, 0))[d])
+        arr[i].start_time = float(c.get("start_time", 0.0))
+        arr[i].end_time = float(c.get("end_time", 999.0))
+        arr[i].friction = float(c.get("friction", 0.0))
+    return arr
+
+
+# ---- single-function entry points ---------------------------------------
+
+def svd3(A: np.ndarray):
+    """oracle_svd3 on one 3x3 matrix; returns U, sig, V."""
+    A = np.ascontiguousarray(A)
+    U = np.zeros((3, 3), A.dtype); V = np.zeros((3, 3), A.dtype); s = np.zeros(3, A.dtype)
+    getattr(lib(), "oracle_svd3" + _sfx(A.dtype))(_p(A), _p(U), _p(s), _p(V))
+    return U, s, V
+
+
+def weights(x: np.ndarray, inv_dx: float):
+    x = np.ascontiguousarray(x)
+    base = np.zeros(3, np.int32); fx = np.zeros(3, x.dtype)
+    w = np.zeros((3, 3), x.dtype); dw = np.zeros((3, 3), x.dtype)
+    ct = C.c_double if x.dtype == np.float64 else C.c_float
+    getattr(lib(), "oracle_weights" + _sfx(x.dtype))(_p(x), ct(inv_dx), _p(base), _p(fx), _p(w), _p(dw))
+    return base, fx, w, dw
+
+
+def sand_return_map(Ftr: np.ndarray, mu, lam, alpha):
+    Ftr = np.ascontiguousarray(Ftr)
+    Fe = np.zeros((3, 3), Ftr.dtype)
+    ct = C.c_double if Ftr.dtype == np.float64 else C.c_float
+    getattr(lib(), "oracle_sand_return_map" + _sfx(Ftr.dtype))(_p(Ftr), ct(mu), ct(lam), ct(alpha), _p(Fe))
+    return Fe
+
+
+def dp_stress(F: np.ndarray, mu, lam):
+    F = np.ascontiguousarray(F)
+    tau = np.zeros((3, 3), F.dtype)
+    ct = C.c_double if F.dtype == np.float64 else C.c_float
+    getattr(lib(), "oracle_dp_stress" + _sfx(F.dtype))(_p(F), ct(mu), ct(lam), _p(tau))
+    return tau
+
+
+# ---- scene state ----------------------------------------------------------
+
+class OracleScene:
+    """Particle state + grid scratch of one scene in float32 or float64."""
+
+    FIELDS = ("x0", "u", "v", "C", "F", "tau", "mass", "vol")
+
+    def __init__(self, params: dict, state: dict, colliders=(), dtype=np.float64):
+        self.dtype = np.dtype(dtype)
+        self.params = dict(params)
+        self.n = int(state["x0"].shape[0])
+        shapes = dict(x0=(self.n, 3), u=(self.n, 3), v=(self.n, 3), C=(self.n, 3, 3),
+                      F=(self.n, 3, 3), tau=(self.n, 3, 3), mass=(self.n,), vol=(self.n,))
+        self.s = {k: np.array(state[k], dtype=self.dtype, copy=True, order="C").reshape(shapes[k])
+                  for k in self.FIELDS}
+        ng = int(params["n_grid"])
+        self.x = np.zeros((self.n, 3), self.dtype)
+        self.grid_m = np.zeros(ng ** 3, self.dtype)
+        self.grid_mv = np.zeros((ng ** 3, 3), self.dtype)
+        self.grid_v = np.zeros((ng ** 3, 3), self.dtype)
+        self.colliders = list(colliders)
+        self._cp = make_params(ng, params["dx"], params["inv_dx"], params["mu"], params["lam"],
+                               params["alpha"], params["gravity"], params.get("material", 2))
+        self._cc = make_colliders(self.colliders)
+        self.time = float(params.get("time", 0.0))
+
+    def step(self, dt: float, nthreads: int = 1):
+        s = self.s
+        fn = getattr(lib(), "oracle_step" + _sfx(self.dtype))
+        fn.restype = C.c_int
+        fn(C.byref(self._cp), C.c_int64(self.n), _p(s["x0"]), _p(s["u"]), _p(s["v"]), _p(s["C"]),
+           _p(s["F"]), _p(s["tau"]), _p(s["mass"]), _p(s["vol"]), _p(self.x), _p(self.grid_m),
+           _p(self.grid_mv), _p(self.grid_v), self._cc, C.c_int(len(self.colliders)),
+           C.c_double(self.time), C.c_double(dt), C.c_int(nthreads))
+        self.time = self.time + dt
+
+    def run(self, n_steps: int, dt: float, nthreads: int = 1):
+        for _ in range(n_steps):
+            self.step(dt, nthreads)
+        return self

@@ -1,0 +1,89 @@
+"""ctypes binding of libmpm_hip.so (C-ABI in include/mpm_hip.h).
+
+The library is the only compute path: if it is missing this module raises
+instead of falling back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+from ._build import LIB_PATH
+
+MPM_ABI_VERSION = 1
+MAX_COLLIDERS = 16
+
+# plane indices of the particle state buffer
+P_X0, P_U, P_V, P_C, P_F, P_TAU, P_MASS, P_VOL, P_PID = 0, 3, 6, 9, 18, 27, 36, 37, 38
+NPLANES = 39
+
+
+class MpmParams(C.Structure):
+    _fields_ = [("n_grid", C.c_int32), ("material", C.c_int32),
+                ("dx", C.c_float), ("inv_dx", C.c_float),
+                ("mu", C.c_float), ("lam", C.c_float), ("alpha", C.c_float),
+                ("gravity", C.c_float * 3)]
+
+
+class MpmCollider(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("surface_type", C.c_int32),
+                ("point", C.c_float * 3), ("normal", C.c_float * 3),
+                ("start_time", C.c_float), ("end_time", C.c_float),
+                ("friction", C.c_float)]
+
+
+class MpmState(C.Structure):
+    _fields_ = [("n", C.c_int64), ("stride", C.c_int64), ("planes", C.c_void_p)]
+
+
+# exported symbol -> (restype, argtypes)
+_SIGNATURES = {
+    "mpm_abi_version": (C.c_int, []),
+    "mpm_last_error": (C.c_char_p, []),
+    "mpm_zero_grid": (C.c_int, [C.POINTER(MpmParams), C.c_void_p, C.c_void_p]),
+    "mpm_p2g": (C.c_int, [C.POINTER(MpmParams), C.POINTER(MpmState), C.c_void_p, C.c_float, C.c_void_p]),
+    "mpm_grid_update": (C.c_int, [C.POINTER(MpmParams), C.c_void_p, C.c_void_p, C.POINTER(MpmCollider),
+                                  C.c_int, C.c_float, C.c_float, C.c_void_p]),
+    "mpm_g2p": (C.c_int, [C.POINTER(MpmParams), C.POINTER(MpmState), C.c_void_p, C.c_float, C.c_void_p]),
+    "mpm_workspace_bytes": (C.c_int64, [C.POINTER(MpmParams), C.c_int64]),
+    "mpm_step": (C.c_int, [C.POINTER(MpmParams), C.POINTER(MpmState), C.POINTER(MpmCollider), C.c_int,
+                           C.c_void_p, C.c_int64, C.c_float, C.c_float, C.c_void_p]),
+    "mpm_import": (C.c_int, [C.POINTER(MpmState), C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p]),
+    "mpm_export": (C.c_int, [C.POINTER(MpmState), C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p]),
+}
+
+_lib = None
+
+
+class MpmError(RuntimeError):
+    pass
+
+
+def load():
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with "
+            "`python -m reduced_mpm_stress_training_amd._build` (hipcc, gfx950)")
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in _SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if lib.mpm_abi_version() != MPM_ABI_VERSION:
+        raise ImportError(f"{LIB_PATH}: ABI version {lib.mpm_abi_version()} != {MPM_ABI_VERSION}")
+    _lib = lib
+    return lib
+
+
+def check(rc: int, what: str):
+    if rc != 0:
+        msg = load().mpm_last_error().decode(errors="replace")
+        raise MpmError(f"{what} failed ({rc}): {msg}")
+
+
+def exported_symbols():
+    return list(_SIGNATURES)

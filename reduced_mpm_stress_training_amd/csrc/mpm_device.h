@@ -1,0 +1,253 @@
+// mpm_device.h -- per-particle FP32 math of the sand substep, in registers.
+//
+//   bspline_weights   quadratic B-spline w, dw     mpm_utils.py:196-209, 244-257
+//   svd3_rot          3x3 SVD (rotation U, V)       wp.svd3 at mpm_utils.py:92,297
+//   sand_return_map   Drucker-Prager projection     mpm_utils.py:87-112
+//   dp_stress_sym     Kirchhoff stress of F_e       mpm_utils.py:45-52
+//
+// All matrices are row-major float[9].  Loops have constant trip counts and
+// are fully unrolled so that every array lives in VGPRs.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace mpm {
+
+struct Weights {
+    int base[3];
+    float fx[3];
+    float w[3][3];   // w[axis][node]
+    float dw[3][3];  // dw[axis][node]
+};
+
+// mpm_utils.py:196-209: base = int(x*inv_dx - 0.5) (truncation), fx in
+// [0.5, 1.5); w = (0.5(1.5-fx)^2, 0.75-(fx-1)^2, 0.5(fx-0.5)^2),
+// dw = (fx-1.5, -2(fx-1), fx-0.5).
+__device__ __forceinline__ void bspline_weights(const float x[3], float inv_dx, Weights& W) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        float gp = x[d] * inv_dx;
+        int b = (int)(gp - 0.5f);
+        W.base[d] = b;
+        float fx = gp - (float)b;
+        W.fx[d] = fx;
+        float wa = 1.5f - fx, wb = fx - 1.0f, wc = fx - 0.5f;
+        W.w[d][0] = wa * wa * 0.5f;
+        W.w[d][1] = (0.0f - wb * wb) + 0.75f;
+        W.w[d][2] = wc * wc * 0.5f;
+        W.dw[d][0] = fx - 1.5f;
+        W.dw[d][1] = -2.0f * (fx - 1.0f);
+        W.dw[d][2] = fx - 0.5f;
+    }
+}
+
+// 1/sqrt(x) refined by one Newton step from the hardware estimate, so that
+// rotation cosines/sines built from it satisfy c^2 + s^2 = 1 to ~1 ulp.
+__device__ __forceinline__ float rsqrt_nr(float x) {
+    float y = rsqrtf(x);
+    float h = 0.5f * x * y;
+    return y * fmaf(-h, y, 1.5f);
+}
+
+// One Jacobi rotation zeroing a_pq of the symmetric matrix (app, aqq, apq,
+// arp, arq); V's columns p, q rotate with it.  t = tan of the rotation
+// angle, the smaller root of t^2 + 2 theta t - 1 = 0, theta = (aqq-app)/(2apq).
+template <int P, int Q>
+__device__ __forceinline__ void jacobi_rotate(float& app, float& aqq, float& apq, float& arp,
+                                              float& arq, float V[9]) {
+    float d = aqq - app;
+    float den = fabsf(d) + sqrtf(d * d + 4.0f * apq * apq);
+    float t = (den > 0.0f) ? ((d >= 0.0f ? 2.0f : -2.0f) * apq) / den : 0.0f;
+    float c = rsqrt_nr(1.0f + t * t);
+    float s = t * c;
+    app = app - t * apq;
+    aqq = aqq + t * apq;
+    apq = 0.0f;
+    float rp = arp, rq = arq;
+    arp = c * rp - s * rq;
+    arq = s * rp + c * rq;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        float vp = V[k * 3 + P], vq = V[k * 3 + Q];
+        V[k * 3 + P] = c * vp - s * vq;
+        V[k * 3 + Q] = s * vp + c * vq;
+    }
+}
+
+// Givens rotation on rows (P, Q) of B zeroing B[Q][COL]; U <- U G^T.
+template <int P, int Q, int COL>
+__device__ __forceinline__ void givens_qr(float B[9], float U[9]) {
+    float a = B[P * 3 + COL], b = B[Q * 3 + COL];
+    float r2 = a * a + b * b;
+    bool ok = r2 > 1e-36f;
+    float ir = ok ? rsqrt_nr(r2) : 0.0f;
+    float c = ok ? a * ir : 1.0f;
+    float s = ok ? b * ir : 0.0f;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        float bp = B[P * 3 + k], bq = B[Q * 3 + k];
+        B[P * 3 + k] = c * bp + s * bq;
+        B[Q * 3 + k] = -s * bp + c * bq;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        float up = U[k * 3 + P], uq = U[k * 3 + Q];
+        U[k * 3 + P] = c * up + s * uq;
+        U[k * 3 + Q] = -s * up + c * uq;
+    }
+}
+
+template <int I, int J>
+__device__ __forceinline__ void sort_swap(float B[9], float V[9]) {
+    float ni = B[I] * B[I] + B[3 + I] * B[3 + I] + B[6 + I] * B[6 + I];
+    float nj = B[J] * B[J] + B[3 + J] * B[3 + J] + B[6 + J] * B[6 + J];
+    bool sw = ni < nj;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        float bi = B[k * 3 + I], bj = B[k * 3 + J];
+        B[k * 3 + I] = sw ? bj : bi;
+        B[k * 3 + J] = sw ? -bi : bj;
+        float vi = V[k * 3 + I], vj = V[k * 3 + J];
+        V[k * 3 + I] = sw ? vj : vi;
+        V[k * 3 + J] = sw ? -vi : vj;
+    }
+}
+
+// SVD A = U diag(s) V^T after the structure of McAdams et al. 2011 (Jacobi
+// eigenanalysis of A^T A, column sort of AV by decreasing norm, Givens QR of
+// AV), with exact Jacobi rotations.  U and V are rotations; s0 >= s1 >= |s2|
+// and s2 < 0 iff det A < 0.
+__device__ __forceinline__ void svd3_rot(const float A[9], float U[9], float s[3], float V[9]) {
+    float s00 = A[0] * A[0] + A[3] * A[3] + A[6] * A[6];
+    float s11 = A[1] * A[1] + A[4] * A[4] + A[7] * A[7];
+    float s22 = A[2] * A[2] + A[5] * A[5] + A[8] * A[8];
+    float s01 = A[0] * A[1] + A[3] * A[4] + A[6] * A[7];
+    float s02 = A[0] * A[2] + A[3] * A[5] + A[6] * A[8];
+    float s12 = A[1] * A[2] + A[4] * A[5] + A[7] * A[8];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) V[k] = (k % 4 == 0) ? 1.0f : 0.0f;
+    for (int sweep = 0; sweep < 6; ++sweep) {
+        float off = s01 * s01 + s02 * s02 + s12 * s12;
+        float dia = s00 * s00 + s11 * s11 + s22 * s22;
+        if (!(off > dia * 1e-15f)) break;
+        jacobi_rotate<0, 1>(s00, s11, s01, s02, s12, V);
+        jacobi_rotate<0, 2>(s00, s22, s02, s01, s12, V);
+        jacobi_rotate<1, 2>(s11, s22, s12, s01, s02, V);
+    }
+    float B[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            B[i * 3 + j] = A[i * 3 + 0] * V[0 * 3 + j] + A[i * 3 + 1] * V[1 * 3 + j] +
+                           A[i * 3 + 2] * V[2 * 3 + j];
+    sort_swap<0, 1>(B, V);
+    sort_swap<1, 2>(B, V);
+    sort_swap<0, 1>(B, V);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) U[k] = (k % 4 == 0) ? 1.0f : 0.0f;
+    givens_qr<0, 1, 0>(B, U);
+    givens_qr<0, 2, 0>(B, U);
+    givens_qr<1, 2, 1>(B, U);
+    s[0] = B[0];
+    s[1] = B[4];
+    s[2] = B[8];
+}
+
+// sand_return_mapping (mpm_utils.py:87-112) on F_tr given its SVD.  Writes
+// F_e and ls = ln of the singular values of F_e = U diag(exp ls) V^T, the
+// logarithms the stress needs:
+//   branch 1 (dg <= 0): F_e = F_tr bitwise, ls = ln s (no abs/clamp, as the
+//                       stress at mpm_utils.py:47-50 takes it);
+//   branch 2 (dg > 0, tr > 0): F_e = U V^T, ls = 0;
+//   branch 3 (dg > 0, tr <= 0): ls = H = eps - eps_hat dg/|eps_hat|,
+//                       F_e = U diag(exp H) V^T.
+// Returns the branch.
+__device__ __forceinline__ int sand_return_map(const float Ftr[9], const float U[9],
+                                               const float s[3], const float V[9], float mu,
+                                               float lam, float alpha, float Fe[9],
+                                               float ls[3]) {
+    float eps[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) eps[d] = logf(fmaxf(fabsf(s[d]), 1e-14f));
+    float tr = eps[0] + eps[1] + eps[2];
+    float eh0 = eps[0] - tr / 3.0f, eh1 = eps[1] - tr / 3.0f, eh2 = eps[2] - tr / 3.0f;
+    float ehn = sqrtf(eh0 * eh0 + eh1 * eh1 + eh2 * eh2);
+    float dg = ehn + (3.0f * lam + 2.0f * mu) / (2.0f * mu) * tr * alpha;
+    if (dg <= 0.0f) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) Fe[k] = Ftr[k];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) ls[d] = (s[d] >= 1e-14f) ? eps[d] : logf(s[d]);
+        return 1;
+    }
+    float se[3];
+    int br;
+    if (tr > 0.0f) {
+        se[0] = se[1] = se[2] = 1.0f;
+        ls[0] = ls[1] = ls[2] = 0.0f;
+        br = 2;
+    } else {
+        float r = dg / ehn;
+        ls[0] = eps[0] - eh0 * r;
+        ls[1] = eps[1] - eh1 * r;
+        ls[2] = eps[2] - eh2 * r;
+        se[0] = expf(ls[0]);
+        se[1] = expf(ls[1]);
+        se[2] = expf(ls[2]);
+        br = 3;
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            Fe[i * 3 + j] = U[i * 3 + 0] * se[0] * V[j * 3 + 0] + U[i * 3 + 1] * se[1] * V[j * 3 + 1] +
+                            U[i * 3 + 2] * se[2] * V[j * 3 + 2];
+    return br;
+}
+
+// kirchoff_stress_drucker_prager (mpm_utils.py:45-52):
+//   tau = U diag((2 mu ln s_i + lam sum ln s)/s_i) V^T F_e^T.
+// With F_e = U diag(s) V^T this is U diag(2 mu ln s_i + lam sum ln s) U^T,
+// evaluated in that symmetric form from the SVD the return map already
+// holds (the reference recomputes the SVD of F_e, mpm_utils.py:297).
+// ln s_i is taken without abs or clamp, as the reference does.
+__device__ __forceinline__ void dp_stress_sym(const float U[9], const float ls[3], float mu,
+                                              float lam, float tau[9]) {
+    float lsum = ls[0] + ls[1] + ls[2];
+    float c0 = 2.0f * mu * ls[0] + lam * lsum;
+    float c1 = 2.0f * mu * ls[1] + lam * lsum;
+    float c2 = 2.0f * mu * ls[2] + lam * lsum;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = i; j < 3; ++j) {
+            float t = U[i * 3 + 0] * c0 * U[j * 3 + 0] + U[i * 3 + 1] * c1 * U[j * 3 + 1] +
+                      U[i * 3 + 2] * c2 * U[j * 3 + 2];
+            tau[i * 3 + j] = t;
+            tau[j * 3 + i] = t;
+        }
+}
+
+// Whole per-particle constitutive update of g2p_apic_and_stress
+// (mpm_utils.py:279-306), sand: F_tr = (I + dt gradv) F, return map, stress.
+__device__ __forceinline__ void sand_update(const float gradv[9], float dt, float F[9],
+                                            float mu, float lam, float alpha, float tau[9]) {
+    float A[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) A[k] = gradv[k] * dt;
+    A[0] += 1.0f; A[4] += 1.0f; A[8] += 1.0f;
+    float Ftr[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            Ftr[i * 3 + j] = A[i * 3 + 0] * F[0 * 3 + j] + A[i * 3 + 1] * F[1 * 3 + j] +
+                             A[i * 3 + 2] * F[2 * 3 + j];
+    float U[9], s[3], V[9];
+    svd3_rot(Ftr, U, s, V);
+    float ls[3];
+    sand_return_map(Ftr, U, s, V, mu, lam, alpha, F, ls);
+    dp_stress_sym(U, ls, mu, lam, tau);
+}
+
+}  // namespace mpm

@@ -1,0 +1,414 @@
+"""MPM_Simulator_HIP: drop-in for the reference's MPM_Simulator_WARP.
+
+Mirrors the Python API of engine_stress/mpm_solver_warp.py (class at :24) --
+method names, keyword arguments, argument meaning and exception types -- and
+runs every substep through the HIP library libmpm_hip.so (C-ABI in
+include/mpm_hip.h) on PyTorch-ROCm device memory.
+
+Deviations from the reference (each deliberate):
+  * state lives in solver-owned device memory ("planes", include/mpm_hip.h);
+    loaders copy from the given tensors and exports return fresh tensors in
+    the caller's particle order instead of aliasing solver memory
+    (mpm_solver_warp.py:229-266, 305-310 alias);
+  * F starts as the identity (the reference allocates it as zeros,
+    mpm_solver_warp.py:56, and sets it to I only in load_from_sampling :149),
+    and load_uvF_from_tensor sets F = F_disp + I (the reference sets F_disp
+    only, :245);
+  * extensions: set_parameters(E=, nu=, gravity=(gx,gy,gz)) and
+    load_vol_from_tensor (the reference has no volume loader);
+  * the per-step debug prints, host syncs and the blocking input() on a CFL
+    violation (:173-192) run only with debug=True, and the CFL violation then
+    raises RuntimeError; a missing sampling file raises FileNotFoundError
+    instead of exit() (:130-132);
+  * snapshots are written as .npz with the reference's h5 dataset names when
+    h5py is not importable.
+"""
+from __future__ import annotations
+
+import math
+import os
+import types
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from .scenes import alpha_from_friction_angle, lame_from_E_nu
+
+_MATERIALS = {"jelly": 0, "metal": 1, "sand": 2}
+
+
+def _stream_handle(device) -> int:
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def _check_float_tensor(t: torch.Tensor, ncols=None):
+    """Checks of torch2warp_vec3/mat33 (engine_stress/warp_utils.py:5-43)."""
+    assert t.is_contiguous()
+    if t.dtype != torch.float32 and t.dtype != torch.int32:
+        raise RuntimeError("Error aliasing Torch tensor to Warp array. Torch tensor must be float32 or int32 type")
+    if ncols is not None:
+        assert t.shape[1] == ncols
+
+
+class MPM_Simulator_HIP:
+    """Explicit MLS/APIC-MPM sand simulator on MI355X
+    (MPM_Simulator_WARP, engine_stress/mpm_solver_warp.py:24-720; the
+    substep path only)."""
+
+    def __init__(self, n_particles, n_grid=100, grid_lim=1.0, device="cuda:0"):
+        self.initialize(n_particles, n_grid, grid_lim, device=device)
+
+    # ------------------------------------------------------------------
+    # initialize (mpm_solver_warp.py:28-87)
+    # ------------------------------------------------------------------
+    def initialize(self, n_particles, n_grid=100, grid_lim=1.0, device="cuda:0"):
+        self.n_particles = int(n_particles)
+        self.device = torch.device(device)
+        m = types.SimpleNamespace()
+        m.grid_lim = grid_lim
+        m.n_grid = int(n_grid)
+        m.grid_dim_x = m.grid_dim_y = m.grid_dim_z = m.n_grid
+        m.dx, m.inv_dx = m.grid_lim / m.n_grid, float(m.n_grid / m.grid_lim)
+        m.E = 1000
+        m.nu = 0.3
+        m.mu = m.E / 2.0 / (1.0 + m.nu)
+        m.lam = m.E * m.nu / (1.0 + m.nu) / (1.0 - 2.0 * m.nu)
+        m.yield_stress = 0.0
+        m.hardening = 0
+        m.xi = 0.0
+        m.material = 0
+        m.friction_angle = 25.0
+        m.alpha = alpha_from_friction_angle(m.friction_angle)
+        m.gravitational_accelaration = 0.0
+        m.gravity = (0.0, 0.0, 0.0)
+        self.mpm_model = m
+
+        n = self.n_particles
+        self.planes = torch.zeros((L.NPLANES, max(n, 1)), dtype=torch.float32, device=self.device)
+        # F = I (deviation: the reference allocates zeros, :56)
+        for k in (0, 4, 8):
+            self.planes[L.P_F + k].fill_(1.0)
+        self.planes[L.P_PID].copy_(torch.arange(max(n, 1), dtype=torch.int32, device=self.device).view(torch.float32))
+        self.particle_density = torch.zeros(max(n, 1), dtype=torch.float32, device=self.device)
+        self.particle_color = np.ones(shape=n, dtype=float)
+        self.time = 0.0
+        self.debug = False
+        self.stage_timer = None      # list -> p2g2p records per-stage events into it
+        self.colliders = []          # list of dicts, registration order
+        self._alloc_grid()
+
+    def _alloc_grid(self):
+        prm = self._params()
+        nbytes = L.load().mpm_workspace_bytes(prm, self.n_particles)
+        if nbytes < 0:
+            L.check(-1, "mpm_workspace_bytes")
+        self.workspace = torch.zeros(int(nbytes) + 256, dtype=torch.uint8, device=self.device)
+
+    def _ws_ptr(self):
+        p = self.workspace.data_ptr()
+        return (p + 255) & ~255
+
+    def _params(self) -> L.MpmParams:
+        m = self.mpm_model
+        prm = L.MpmParams()
+        prm.n_grid = int(m.n_grid)
+        prm.material = int(m.material)
+        prm.dx = float(m.dx)
+        prm.inv_dx = float(m.inv_dx)
+        prm.mu = float(m.mu)
+        prm.lam = float(m.lam)
+        prm.alpha = float(m.alpha)
+        g = self._gravity_vec()
+        for i in range(3):
+            prm.gravity[i] = float(g[i])
+        return prm
+
+    def _gravity_vec(self):
+        m = self.mpm_model
+        if getattr(m, "gravity_override", None) is not None:
+            return m.gravity_override
+        return (0.0, -float(m.gravitational_accelaration), 0.0)
+
+    def _state(self) -> L.MpmState:
+        st = L.MpmState()
+        st.n = self.n_particles
+        st.stride = self.planes.shape[1]
+        st.planes = self.planes.data_ptr()
+        return st
+
+    def _colliders(self):
+        n = len(self.colliders)
+        if n > L.MAX_COLLIDERS:
+            raise ValueError(f"at most {L.MAX_COLLIDERS} colliders are supported")
+        arr = (L.MpmCollider * max(n, 1))()
+        for i, c in enumerate(self.colliders):
+            arr[i].kind = c["kind"]
+            arr[i].surface_type = c.get("surface_type", 0)
+            for d in range(3):
+                arr[i].point[d] = float(c.get("point", (0.0, 0.0, 0.0))[d])
+                arr[i].normal[d] = float(c.get("normal", (0.0, 0.0, 0.0))[d])
+            arr[i].start_time = float(c["start_time"])
+            arr[i].end_time = float(c["end_time"])
+            arr[i].friction = float(c.get("friction", 0.0))
+        return arr, n
+
+    # ------------------------------------------------------------------
+    # set_parameters (mpm_solver_warp.py:89-124)
+    # ------------------------------------------------------------------
+    def set_parameters(self, device="cuda:0", **kwargs):
+        m = self.mpm_model
+        if "material" in kwargs:
+            if kwargs["material"] in _MATERIALS:
+                m.material = _MATERIALS[kwargs["material"]]
+            else:
+                raise TypeError("Undefined material type")
+        if "grid_lim" in kwargs:
+            m.grid_lim = kwargs["grid_lim"]
+        if "n_grid" in kwargs:
+            m.n_grid = int(kwargs["n_grid"])
+        m.grid_dim_x = m.grid_dim_y = m.grid_dim_z = m.n_grid
+        m.dx, m.inv_dx = m.grid_lim / m.n_grid, float(m.n_grid / m.grid_lim)
+        self._alloc_grid()
+        if "mu" in kwargs:
+            m.mu = kwargs["mu"]
+        if "lam" in kwargs:
+            m.lam = kwargs["lam"]
+        if "E" in kwargs or "nu" in kwargs:          # extension: Young's modulus / Poisson ratio
+            m.E = kwargs.get("E", m.E)
+            m.nu = kwargs.get("nu", m.nu)
+            m.mu, m.lam = lame_from_E_nu(m.E, m.nu)
+        if "yield_stress" in kwargs:
+            m.yield_stress = kwargs["yield_stress"]
+        if "hardening" in kwargs:
+            m.hardening = kwargs["hardening"]
+        if "xi" in kwargs:
+            m.xi = kwargs["xi"]
+        if "friction_angle" in kwargs:
+            m.friction_angle = kwargs["friction_angle"]
+            m.alpha = alpha_from_friction_angle(m.friction_angle)
+        if "g" in kwargs:
+            m.gravitational_accelaration = kwargs["g"]
+        if "gravity" in kwargs:                       # extension: full gravity vector
+            g = kwargs["gravity"]
+            m.gravity_override = None if g is None else tuple(float(a) for a in g)
+        if "density" in kwargs:
+            n = self.n_particles
+            self.particle_density[:n].fill_(float(kwargs["density"]))
+            self.planes[L.P_MASS, :n] = self.particle_density[:n] * self.planes[L.P_VOL, :n]
+
+    # ------------------------------------------------------------------
+    # colliders (mpm_solver_warp.py:407-451, 499-525)
+    # ------------------------------------------------------------------
+    def add_surface_collider(self, point, normal, surface="sticky", friction=0.0, start_time=0.0,
+                             end_time=999.0):
+        point = list(point)
+        normal_scale = 1.0 / math.sqrt(sum(x ** 2 for x in normal))
+        normal = list(normal_scale * x for x in normal)
+        if surface == "sticky" and friction != 0:
+            raise ValueError("friction must be 0 on sticky surfaces.")
+        stype = 0 if surface == "sticky" else (1 if surface == "slip" else 2)
+        self.colliders.append(dict(kind=0, surface_type=stype, point=tuple(point), normal=tuple(normal),
+                                   start_time=start_time, end_time=end_time, friction=friction))
+
+    def add_bounding_box(self, start_time=0.0, end_time=999.0):
+        self.colliders.append(dict(kind=1, start_time=start_time, end_time=end_time))
+
+    # ------------------------------------------------------------------
+    # p2g2p (mpm_solver_warp.py:156-203)
+    # ------------------------------------------------------------------
+    def p2g2p(self, step, dt, print_norm=False, device="cuda:0"):
+        if self.mpm_model.material != 2:
+            raise NotImplementedError("only material='sand' is implemented by the HIP substep")
+        lib = L.load()
+        prm = self._params()
+        st = self._state()
+        cols, ncol = self._colliders()
+        stream = _stream_handle(self.device)
+        if self.stage_timer is None:
+            ws = self.workspace.numel() - 256
+            rc = lib.mpm_step(prm, st, cols, ncol, self._ws_ptr(), ws, float(self.time), float(dt), stream)
+            L.check(rc, "mpm_step")
+        else:
+            self._timed_stages(lib, prm, st, cols, ncol, float(dt), stream)
+        if self.debug:
+            v = self.planes[L.P_V:L.P_V + 3, :self.n_particles]
+            vmax = float(torch.max(torch.abs(v)))
+            print("max particle v: ", vmax)
+            print("max allowed  v: ", self.mpm_model.dx / dt)
+            if vmax > self.mpm_model.dx / dt:
+                raise RuntimeError(f"CFL violated: max |v| {vmax} > dx/dt {self.mpm_model.dx / dt}")
+        self.time = self.time + dt
+        if print_norm:
+            u, v, F_disp, stress = self.export_uvFstress_from_warp()
+            print("at mpm step", step, end=" ")
+            print("norm of u, v, F_disp, stress, respectively: ",
+                  "{:.4f}".format(float(torch.linalg.norm(u))), "{:.4f}".format(float(torch.linalg.norm(v))),
+                  "{:.4f}".format(float(torch.linalg.norm(F_disp))), "{:.4f}".format(float(torch.linalg.norm(stress))))
+
+    def _timed_stages(self, lib, prm, st, cols, ncol, dt, stream):
+        """The substep as its stage launches with a torch.cuda.Event between
+        stages (same kernels as mpm_step, on the same stream)."""
+        ng = self.mpm_model.n_grid
+        acc = self._ws_ptr()
+        gv = acc + 16 * ng ** 3
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
+        ev[0].record()
+        L.check(lib.mpm_zero_grid(prm, acc, stream), "mpm_zero_grid")
+        ev[1].record()
+        L.check(lib.mpm_p2g(prm, st, acc, dt, stream), "mpm_p2g")
+        ev[2].record()
+        L.check(lib.mpm_grid_update(prm, acc, gv, cols, ncol, float(self.time), dt, stream), "mpm_grid_update")
+        ev[3].record()
+        L.check(lib.mpm_g2p(prm, st, gv, dt, stream), "mpm_g2p")
+        ev[4].record()
+        self.stage_timer.append(ev)
+
+    def run(self, n_substeps, dt):
+        """n_substeps consecutive p2g2p calls without host synchronisation."""
+        for k in range(int(n_substeps)):
+            self.p2g2p(k, dt)
+
+    # ------------------------------------------------------------------
+    # torch loaders (mpm_solver_warp.py:229-266) -- copies, not aliases
+    # ------------------------------------------------------------------
+    def _import(self, plane, ncomp, t, mode=0):
+        if t.numel() != self.n_particles * ncomp:
+            raise ValueError(f"expected {self.n_particles} x {ncomp} values, got shape {tuple(t.shape)}")
+        t = t.to(device=self.device, dtype=torch.float32).contiguous()
+        lib = L.load()
+        L.check(lib.mpm_import(self._state(), plane, ncomp, t.data_ptr(), mode, _stream_handle(self.device)),
+                "mpm_import")
+
+    def load_x_initial_from_tensor(self, tensor_x_iniital, device="cuda:0"):
+        _check_float_tensor(tensor_x_iniital, 3)
+        self._import(L.P_X0, 3, tensor_x_iniital)
+
+    def load_mass_from_tensor(self, tensor_mass, device="cuda:0"):
+        _check_float_tensor(tensor_mass)
+        self._import(L.P_MASS, 1, tensor_mass.reshape(-1).contiguous())
+
+    def load_vol_from_tensor(self, tensor_vol, device="cuda:0"):
+        """Extension: the reference has no volume loader."""
+        _check_float_tensor(tensor_vol)
+        self._import(L.P_VOL, 1, tensor_vol.reshape(-1).contiguous())
+
+    def load_uvF_from_tensor(self, tensor_u, tensor_v, tensor_F, clone=True, device="cuda:0"):
+        tensor_F = torch.reshape(tensor_F, (-1, 3, 3)).contiguous()
+        _check_float_tensor(tensor_u, 3)
+        _check_float_tensor(tensor_v, 3)
+        _check_float_tensor(tensor_F, 3)
+        self._import(L.P_U, 3, tensor_u)
+        self._import(L.P_V, 3, tensor_v)
+        self._import(L.P_F, 9, tensor_F, mode=2)       # F = F_disp + I
+
+    def load_C_from_tensor(self, tensor_C, clone=True, device="cuda:0"):
+        tensor_C = torch.reshape(tensor_C, (-1, 3, 3)).contiguous()
+        _check_float_tensor(tensor_C, 3)
+        self._import(L.P_C, 9, tensor_C)
+
+    def load_uvstress_from_tensor(self, tensor_u, tensor_v, tensor_stress, clone=True, device="cuda:0"):
+        tensor_stress = torch.reshape(tensor_stress, (-1, 3, 3)).contiguous()
+        _check_float_tensor(tensor_u, 3)
+        _check_float_tensor(tensor_v, 3)
+        _check_float_tensor(tensor_stress, 3)
+        self._import(L.P_U, 3, tensor_u)
+        self._import(L.P_V, 3, tensor_v)
+        self._import(L.P_TAU, 9, tensor_stress)
+
+    def load_velocity_from_tensor(self, tensor_v, device="cuda:0"):
+        _check_float_tensor(tensor_v, 3)
+        self._import(L.P_V, 3, tensor_v)
+
+    def load_F_from_tensor(self, tensor_F, device="cuda:0"):
+        """Sets the elastic deformation gradient F itself ((N,9) or (N,3,3))."""
+        tensor_F = torch.reshape(tensor_F, (-1, 3, 3)).contiguous()
+        _check_float_tensor(tensor_F, 3)
+        self._import(L.P_F, 9, tensor_F)
+
+    # ------------------------------------------------------------------
+    # export (mpm_solver_warp.py:305-310)
+    # ------------------------------------------------------------------
+    def _export(self, plane, ncomp, mode=0):
+        out = torch.empty((self.n_particles, ncomp), dtype=torch.float32, device=self.device)
+        lib = L.load()
+        L.check(lib.mpm_export(self._state(), plane, ncomp, out.data_ptr(), mode, _stream_handle(self.device)),
+                "mpm_export")
+        return out
+
+    def export_uvFstress_from_warp(self, device="cuda:0"):
+        """(u (N,3), v (N,3), F_disp (N,3,3), stress (N,3,3)) in the caller's
+        particle order, as fresh tensors."""
+        u = self._export(L.P_U, 3)
+        v = self._export(L.P_V, 3)
+        F_disp = self._export(L.P_F, 9, mode=1).view(-1, 3, 3)
+        stress = self._export(L.P_TAU, 9).view(-1, 3, 3)
+        return u, v, F_disp, stress
+
+    export_uvFstress = export_uvFstress_from_warp
+
+    def export_x(self):
+        return self._export(L.P_X0, 3, mode=3)
+
+    def export_x_initial(self):
+        return self._export(L.P_X0, 3)
+
+    def export_C(self):
+        return self._export(L.P_C, 9).view(-1, 3, 3)
+
+    def export_F(self):
+        return self._export(L.P_F, 9).view(-1, 3, 3)
+
+    def export_mass(self):
+        return self._export(L.P_MASS, 1).view(-1)
+
+    def export_vol(self):
+        return self._export(L.P_VOL, 1).view(-1)
+
+    # grid views (reference: mpm_state.grid_m / grid_v_in / grid_v_out)
+    def _grid_views(self):
+        ng = self.mpm_model.n_grid
+        nn = ng ** 3
+        off = self._ws_ptr() - self.workspace.data_ptr()
+        f = self.workspace[off:off + 2 * nn * 16].view(torch.float32).view(2, ng, ng, ng, 4)
+        return f[0], f[1]
+
+    @property
+    def grid_m(self):
+        return self._grid_views()[0][..., 3]
+
+    @property
+    def grid_v_in(self):
+        return self._grid_views()[0][..., :3]
+
+    @property
+    def grid_v_out(self):
+        return self._grid_views()[1][..., :3]
+
+    # ------------------------------------------------------------------
+    # samplings and snapshots (mpm_solver_warp.py:129-153, 312-404)
+    # ------------------------------------------------------------------
+    def load_from_sampling(self, sampling_h5, n_grid=100, grid_lim=1.0, device="cuda:0"):
+        from . import snapshot
+        path = os.getcwd() + sampling_h5
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"h5 file cannot be found at {path}")
+        d = snapshot.read_sampling(path)
+        x, q = d["x"].T, d["q"].T
+        self.dim, n = q.shape[1], q.shape[0]
+        self.initialize(n, n_grid, grid_lim, device=device)
+        dev = self.device
+        self.load_x_initial_from_tensor(torch.from_numpy(np.ascontiguousarray(x, dtype=np.float32)).to(dev))
+        self.load_mass_from_tensor(torch.from_numpy(np.ascontiguousarray(np.squeeze(d["masses"], 0), dtype=np.float32)).to(dev))
+        self.load_vol_from_tensor(torch.from_numpy(np.ascontiguousarray(np.squeeze(d["particle_volume"], 0), dtype=np.float32)).to(dev))
+        self.particle_density[:n] = torch.from_numpy(np.ascontiguousarray(np.squeeze(d["particle_density"], 0), dtype=np.float32)).to(dev)
+
+    def save_data_at_frame(self, dir_name, frame, include_parameter="friction_angle", save_to_h5=True,
+                           save_to_obj=True, save_visual_F=False):
+        from . import snapshot
+        snapshot.save_frame(self, dir_name, frame, include_parameter=include_parameter, save_to_h5=save_to_h5,
+                            save_to_obj=save_to_obj, save_visual_F=save_visual_F)
+
+
+# drop-in name of the reference class
+MPM_Simulator_WARP = MPM_Simulator_HIP

@@ -1,0 +1,73 @@
+"""CPU-side checks of the C-ABI library and the host logic (no kernel runs)."""
+import ctypes as Ct
+import os
+import re
+
+import pytest
+
+from reduced_mpm_stress_training_amd import _lib as L
+from reduced_mpm_stress_training_amd import scenes as S
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _header_functions():
+    src = open(os.path.join(ROOT, "include", "mpm_hip.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(mpm_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = L.load()
+    declared = _header_functions()
+    assert declared, "no functions parsed from include/mpm_hip.h"
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert sorted(L.exported_symbols()) == declared
+
+
+def test_abi_version_and_errors():
+    lib = L.load()
+    assert lib.mpm_abi_version() == L.MPM_ABI_VERSION
+    prm = L.MpmParams()
+    prm.n_grid = 2
+    prm.material = 2
+    st = L.MpmState()
+    assert lib.mpm_p2g(prm, st, None, 1e-4, None) < 0
+    assert b"n_grid" in lib.mpm_last_error()
+    prm.n_grid = 32
+    prm.material = 0
+    assert lib.mpm_g2p(prm, st, None, 1e-4, None) < 0
+    assert b"sand" in lib.mpm_last_error()
+    prm.material = 2
+    assert lib.mpm_step(prm, st, None, 0, None, 0, 0.0, 1e-4, None) < 0
+    st.n, st.stride, st.planes = 10, 10, 1
+    assert lib.mpm_step(prm, st, None, 0, None, 0, 0.0, 1e-4, None) < 0
+    assert b"workspace" in lib.mpm_last_error()
+    cols = (L.MpmCollider * 1)()
+    assert lib.mpm_grid_update(prm, 1, 1, cols, 17, 0.0, 1e-4, None) < 0
+    assert lib.mpm_workspace_bytes(prm, 1000) >= 2 * 32 ** 3 * 16
+
+
+def test_material_parameters_follow_reference():
+    mu, lam = S.lame_from_E_nu(2e5, 0.3)
+    assert abs(mu - 2e5 / 2.6) < 1e-9 and abs(lam - 2e5 * 0.3 / 1.3 / 0.4) < 1e-6
+    a = S.alpha_from_friction_angle(30.0)
+    assert abs(a - (2.0 / 3.0) ** 0.5 * 2 * 0.5 / 2.5) < 1e-8
+
+
+def test_solver_host_logic_on_cpu_tensors():
+    from reduced_mpm_stress_training_amd.mpm_solver import MPM_Simulator_HIP, MPM_Simulator_WARP
+    assert MPM_Simulator_WARP is MPM_Simulator_HIP
+    sim = MPM_Simulator_HIP(10, n_grid=16, device="cpu")
+    with pytest.raises(TypeError):
+        sim.set_parameters(material="water")
+    with pytest.raises(ValueError):
+        sim.add_surface_collider((0, 0, 0), (0, 2, 0), "sticky", friction=0.1)
+    sim.set_parameters(material="sand", friction_angle=30, g=9.8, mu=1e5, lam=0.3)
+    assert sim.mpm_model.material == 2 and sim._gravity_vec() == (0.0, -9.8, 0.0)
+    sim.add_surface_collider((0, 0.58, 0), (0, 2, 0), "sticky", 0.0)
+    assert sim.colliders[0]["normal"] == (0.0, 1.0, 0.0)
+    sim.add_bounding_box()
+    with pytest.raises(FileNotFoundError):
+        sim.load_from_sampling("/does/not/exist.h5")
