@@ -31,9 +31,12 @@ HBM_PEAK_GBS = 8000.0
 ALG_BYTES = {
     # whole substep: read x0,u,v,C,F,tau,m,vol (152) + write u,v,C,F,tau (132)
     "step": 284,
-    # P2G: read x0,u,v,C,tau,m,vol
+    # the fused kernel does all particle work of a substep (G2P of this
+    # substep + P2G of the next), so it carries the whole substep's bytes
+    "g2p2g": 284,
+    # stage kernels of the unbinned path: P2G reads x0,u,v,C,tau,m,vol;
+    # G2P reads x0,u,F and writes u,v,C,F,tau
     "p2g": 116,
-    # G2P: read x0,u,F; write u,v,C,F,tau
     "g2p": 192,
 }
 
@@ -48,7 +51,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="budget of the CPU baseline sample (0 disables it)")
     ap.add_argument("--stage-steps", type=int, default=20,
-                    help="substeps timed per stage for the roofline")
+                    help="substeps timed per kernel for the roofline")
+    ap.add_argument("--unbinned", action="store_true",
+                    help="time the unbinned stage kernels (atomic P2G, grid update, G2P) instead")
     return ap.parse_args()
 
 
@@ -91,18 +96,17 @@ def main():
 
     scene = S.column_uniform(n_particles=args.particles, n_grid=args.n_grid)
     sim = S.make_simulator(scene)
+    sim.use_bins = not args.unbinned
     n = scene.n_particles
     dt = scene.dt
     print(f"[bench] workload ready: {n} particles, {scene.n_grid}^3 grid", file=sys.stderr, flush=True)
 
-    for k in range(args.warmup):
-        sim.p2g2p(k, dt)
+    sim.run(args.warmup, dt)
     torch.cuda.synchronize()
     t_start = torch.cuda.Event(enable_timing=True)
     t_end = torch.cuda.Event(enable_timing=True)
     t_start.record()
-    for k in range(args.steps):
-        sim.p2g2p(k, dt)
+    sim.run(args.steps, dt)
     t_end.record()
     torch.cuda.synchronize()
     ms_total = t_start.elapsed_time(t_end)
@@ -110,19 +114,19 @@ def main():
     value = n * args.steps / (ms_total / 1e3)
     print(f"[bench] timed {args.steps} substeps: {ms_step:.4f} ms/substep", file=sys.stderr, flush=True)
 
-    # per-stage durations, HIP events on the launch stream
+    # per-kernel durations, HIP events on the launch stream (torch's current
+    # stream, which the library launches on)
     sim.stage_timer = []
-    for k in range(args.stage_steps):
-        sim.p2g2p(k, dt)
+    sim.run(args.stage_steps, dt)
     torch.cuda.synchronize()
-    names = ["zero_grid", "p2g", "grid_update", "g2p"]
+    names = ["zero_grid", "p2g", "grid_update", "g2p"] if args.unbinned else ["grid_update", "g2p2g"]
     dur = {nm: 0.0 for nm in names}
     for ev in sim.stage_timer:
         for i, nm in enumerate(names):
             dur[nm] += ev[i].elapsed_time(ev[i + 1])
     dur = {k: v / len(sim.stage_timer) for k, v in dur.items()}
     sim.stage_timer = None
-    dom = max(("p2g", "g2p"), key=lambda k: dur[k])
+    dom = max((k for k in names if k in ALG_BYTES), key=lambda k: dur[k])
     achieved = ALG_BYTES[dom] * n / (dur[dom] / 1e3) / 1e9
     step_gbs = ALG_BYTES["step"] * n / (ms_step / 1e3) / 1e9
 
@@ -148,6 +152,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": "sand column collapse, forward substep", "n_particles": n,
+                   "path": "unbinned stage kernels" if args.unbinned else "binned G2P2G",
                    "n_grid": scene.n_grid, "box": "[0.3,0.7]^2x[0.05,0.6]", "dt": dt, "E": scene.E,
                    "nu": scene.nu, "friction_angle": scene.friction_angle, "density": scene.density,
                    "parallelism": "single GPU"},

@@ -127,6 +127,78 @@ int mpm_step(const MpmParams* prm, const MpmState* st, const MpmCollider* collid
              int n_colliders, void* workspace, int64_t workspace_bytes, float time,
              float dt, void* stream);
 
+/* ---- binned path: the solver's own particle layout --------------------
+ *
+ * Particles are kept in bins of 8^3 grid cells keyed by the base cell of
+ * their B-spline stencil (int(x*inv_dx - 0.5), mpm_utils.py:197-199).  Bin
+ * b owns slots [b*bin_cap, b*bin_cap + count[b]) of a plane set (same plane
+ * order as MpmState); slots from nb^3*bin_cap on are an overflow run for
+ * bins that are full.  Two plane sets alternate between substeps.
+ * A substep is the grid update of this substep, then one fused kernel doing
+ * G2P of this substep and P2G of the next.  The P2G lands in per-bin tiles
+ * of the workspace stamped with `epoch`, so between calls the workspace
+ * carries the pending P2G; mpm_bins_p2g computes it for a freshly inserted
+ * state.  Entry points that advance the state update `cur` and `epoch` in
+ * place. */
+typedef struct MpmBins {
+    int32_t nb;           /* bins per axis = ceil(n_grid / 8) */
+    int32_t bin_cap;      /* slots per bin */
+    int32_t overflow_cap; /* slots of the overflow run */
+    int32_t cur;          /* plane set holding the current state (0 or 1) */
+    int32_t epoch;        /* stamp of the pending P2G tiles; start at 0 */
+    int32_t reserved;
+    int64_t stride;       /* plane stride >= nb^3 * bin_cap + overflow_cap */
+    float* planes[2];     /* two sets of MPM_NPLANES planes */
+    int32_t* count;       /* [2][nb^3 + 1] particles per bin; last = overflow run */
+} MpmBins;
+
+/* Workspace of the binned path (zeroed once before first use, 256-byte
+ * aligned): spill and velocity grids (n_grid^3 float4 each), spill flags,
+ * 64 int32 status counters (0: particles dropped because bins and overflow
+ * were full, 1: particles scattered outside their bin's tile, 2: particles
+ * that left the grid), tile stamps and nb^3 P2G tiles of 12^3 float4. */
+int64_t mpm_bins_workspace_bytes(const MpmParams* prm);
+
+/* Byte offset of the status counters in the workspace. */
+int64_t mpm_bins_status_offset(const MpmParams* prm);
+
+/* Clears count[cur] and inserts every particle of a flat state (slot =
+ * caller index, as MpmState) into the bins of plane set `cur`. */
+int mpm_bins_insert(const MpmParams* prm, const MpmState* flat, const MpmBins* bins, void* workspace,
+                    void* stream);
+
+/* flat[pid] <- every particle of plane set `cur` (all planes). */
+int mpm_bins_extract(const MpmParams* prm, const MpmBins* bins, const MpmState* flat, void* stream);
+
+/* P2G (p2g_apic_stress_given, mpm_utils.py:190-227) of plane set `cur` into
+ * the tiles of epoch+1, with the given dt; advances epoch. */
+int mpm_bins_p2g(const MpmParams* prm, MpmBins* bins, void* workspace, float dt, void* stream);
+
+/* Discards a pending P2G (drains the spill grid, advances epoch). */
+int mpm_bins_clear_grid(const MpmParams* prm, MpmBins* bins, void* workspace, void* stream);
+
+/* The two launches of one substep, for per-kernel timing: the grid update
+ * (grid_normalization_and_gravity + colliders, mpm_utils.py:229-237 and
+ * mpm_solver_warp.py:429-449, 505-523, with `time` the solver time before
+ * the increment), which also resets the bin counters of plane set 1-cur; and
+ * the fused G2P (g2p_apic_and_stress with the sand return map and stress,
+ * mpm_utils.py:241-306) + next-substep P2G, which writes plane set 1-cur,
+ * then sets cur = 1-cur and advances epoch. */
+int mpm_bins_grid_update(const MpmParams* prm, const MpmBins* bins, const MpmCollider* colliders,
+                         int n_colliders, void* workspace, float time, float dt, void* stream);
+int mpm_bins_g2p2g(const MpmParams* prm, MpmBins* bins, void* workspace, float dt, void* stream);
+
+/* One substep, MPM_Simulator_WARP.p2g2p (mpm_solver_warp.py:156-203), on the
+ * binned state: mpm_bins_grid_update then mpm_bins_g2p2g. */
+int mpm_bins_substep(const MpmParams* prm, MpmBins* bins, const MpmCollider* colliders, int n_colliders,
+                     void* workspace, float time, float dt, void* stream);
+
+/* n_substeps consecutive mpm_bins_substep calls; the solver time of substep
+ * k is time + k*dt accumulated in double, as the Python solver accumulates
+ * self.time (mpm_solver_warp.py:199). */
+int mpm_bins_run(const MpmParams* prm, MpmBins* bins, const MpmCollider* colliders, int n_colliders,
+                 void* workspace, double time, float dt, int n_substeps, void* stream);
+
 /* ---- state import / export (torch loaders and export_uvFstress_from_warp,
  *      mpm_solver_warp.py:229-266 and :305-310) --------------------------- */
 

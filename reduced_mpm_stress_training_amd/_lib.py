@@ -36,6 +36,12 @@ class MpmState(C.Structure):
     _fields_ = [("n", C.c_int64), ("stride", C.c_int64), ("planes", C.c_void_p)]
 
 
+class MpmBins(C.Structure):
+    _fields_ = [("nb", C.c_int32), ("bin_cap", C.c_int32), ("overflow_cap", C.c_int32), ("cur", C.c_int32),
+                ("epoch", C.c_int32), ("reserved", C.c_int32),
+                ("stride", C.c_int64), ("planes", C.c_void_p * 2), ("count", C.c_void_p)]
+
+
 # exported symbol -> (restype, argtypes)
 _SIGNATURES = {
     "mpm_abi_version": (C.c_int, []),
@@ -48,6 +54,20 @@ _SIGNATURES = {
     "mpm_workspace_bytes": (C.c_int64, [C.POINTER(MpmParams), C.c_int64]),
     "mpm_step": (C.c_int, [C.POINTER(MpmParams), C.POINTER(MpmState), C.POINTER(MpmCollider), C.c_int,
                            C.c_void_p, C.c_int64, C.c_float, C.c_float, C.c_void_p]),
+    "mpm_bins_workspace_bytes": (C.c_int64, [C.POINTER(MpmParams)]),
+    "mpm_bins_insert": (C.c_int, [C.POINTER(MpmParams), C.POINTER(MpmState), C.POINTER(MpmBins), C.c_void_p,
+                                  C.c_void_p]),
+    "mpm_bins_extract": (C.c_int, [C.POINTER(MpmParams), C.POINTER(MpmBins), C.POINTER(MpmState), C.c_void_p]),
+    "mpm_bins_p2g": (C.c_int, [C.POINTER(MpmParams), C.POINTER(MpmBins), C.c_void_p, C.c_float, C.c_void_p]),
+    "mpm_bins_status_offset": (C.c_int64, [C.POINTER(MpmParams)]),
+    "mpm_bins_clear_grid": (C.c_int, [C.POINTER(MpmParams), C.POINTER(MpmBins), C.c_void_p, C.c_void_p]),
+    "mpm_bins_substep": (C.c_int, [C.POINTER(MpmParams), C.POINTER(MpmBins), C.POINTER(MpmCollider), C.c_int,
+                                   C.c_void_p, C.c_float, C.c_float, C.c_void_p]),
+    "mpm_bins_grid_update": (C.c_int, [C.POINTER(MpmParams), C.POINTER(MpmBins), C.POINTER(MpmCollider), C.c_int,
+                                       C.c_void_p, C.c_float, C.c_float, C.c_void_p]),
+    "mpm_bins_g2p2g": (C.c_int, [C.POINTER(MpmParams), C.POINTER(MpmBins), C.c_void_p, C.c_float, C.c_void_p]),
+    "mpm_bins_run": (C.c_int, [C.POINTER(MpmParams), C.POINTER(MpmBins), C.POINTER(MpmCollider), C.c_int,
+                               C.c_void_p, C.c_double, C.c_float, C.c_int, C.c_void_p]),
     "mpm_import": (C.c_int, [C.POINTER(MpmState), C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p]),
     "mpm_export": (C.c_int, [C.POINTER(MpmState), C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p]),
 }

@@ -1,0 +1,723 @@
+// mpm_binned.hip -- the MI355X substep on particles binned by grid block.
+//
+// Particles live in bins of 8^3 cells (keyed by the base cell of their
+// stencil, mpm_utils.py:197-199), each bin a fixed run of `bin_cap` slots in
+// a set of SoA planes, plus an overflow run.  Two plane sets alternate.
+//
+// One substep (MPM_Simulator_WARP.p2g2p, mpm_solver_warp.py:156-203) is two
+// launches:
+//   k_grid_bins   per 8^3 grid block: sums the P2G tiles of the (up to 27)
+//                 overlapping bins and the spill grid, then
+//                 grid_normalization_and_gravity (mpm_utils.py:229-237) and
+//                 the colliders (mpm_solver_warp.py:429-449, 505-523).
+//   k_g2p2g       per bin: G2P (mpm_utils.py:241-306) from an LDS tile of
+//                 grid velocities, the sand return map and stress in
+//                 registers, update_u_from_x, and the P2G of the next
+//                 substep (mpm_utils.py:190-227) into an FP64 LDS tile
+//                 (ds_add_f64; FP32 LDS atomics run ~16x slower on gfx950),
+//                 written out as one plain, coalesced tile per bin; every
+//                 particle is written to the slot of its new bin in the other
+//                 plane set.
+// Bins' tiles carry an epoch stamp; the grid update uses the tiles of the
+// current epoch, so nothing needs clearing between substeps (zero_grid,
+// mpm_utils.py:115-123, becomes the epoch bump).  Particles whose next
+// stencil leaves their bin's tile (or that sit in the overflow run) scatter
+// with float atomics into a spill grid that the grid update adds and clears.
+// The P2G of the first substep (or after the state is replaced) is
+// k_p2g_bins.  So between substeps the tiles hold the next substep's P2G,
+// computed with that substep's dt.
+#include <hip/hip_runtime.h>
+
+#include "mpm_common.h"
+#include "mpm_device.h"
+#include "mpm_hip.h"
+#include "mpm_stencil.h"
+
+using namespace mpm;
+
+namespace {
+
+constexpr int kWG = 256;
+constexpr int kBinEdge = 8;           // cells per bin edge
+constexpr int kTG = kBinEdge + 2;     // gather tile edge (nodes b0 .. b0+9)
+constexpr int kTS = kBinEdge + 4;     // scatter tile edge (nodes b0-1 .. b0+10)
+constexpr int kTG3 = kTG * kTG * kTG;
+constexpr int kTS3 = kTS * kTS * kTS;
+
+struct BinGeom {
+    int ng, nb, n_bins, cap, ovf_cap;
+    int64_t stride;
+};
+
+__device__ __forceinline__ int bin_of_base(const int b[3], int nb) {
+    return ((b[0] >> 3) * nb + (b[1] >> 3)) * nb + (b[2] >> 3);
+}
+
+__device__ __forceinline__ bool stencil_ok(const int b[3], int ng) {
+    return b[0] >= 0 && b[1] >= 0 && b[2] >= 0 && b[0] <= ng - 3 && b[1] <= ng - 3 && b[2] <= ng - 3;
+}
+
+// workgroup -> bin, keeping runs of spatially adjacent bins on one XCD
+// (workgroups are dealt to the 8 XCDs round-robin).
+__device__ __forceinline__ int xcd_bin(int wg, int n_bins) {
+    if ((n_bins & 7) != 0) return wg;
+    return (wg & 7) * (n_bins >> 3) + (wg >> 3);
+}
+
+__device__ __forceinline__ void mark_block(int* __restrict__ gflag, int nb, int x, int y, int z) {
+    gflag[((x >> 3) * nb + (y >> 3)) * nb + (z >> 3)] = 1;
+}
+
+// Scatter of one particle straight into the spill grid with float atomics
+// (overflow particles and particles that left their bin's scatter tile).
+__device__ void scatter_spill(float* __restrict__ acc, int* __restrict__ gflag, const BinGeom& g,
+                              const AxisWeights& A, float m, float vol, const float v[3], const float C[9],
+                              const float tau[9], float dx, float inv_dx, float dt) {
+    const int ng = g.ng;
+    p2g_contrib(A, m, vol, v, C, tau, dx, inv_dx, dt,
+                [&](int i, int j, int k, float mx, float my, float mz, float mm) {
+                    const int x = A.base[0] + i, y = A.base[1] + j, z = A.base[2] + k;
+                    float* a = acc + 4 * (((int64_t)x * ng + y) * ng + z);
+                    unsafeAtomicAdd(a + 0, mx);
+                    unsafeAtomicAdd(a + 1, my);
+                    unsafeAtomicAdd(a + 2, mz);
+                    unsafeAtomicAdd(a + 3, mm);
+                });
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int k = 0; k < 2; ++k)
+                mark_block(gflag, g.nb, A.base[0] + 2 * i, A.base[1] + 2 * j, A.base[2] + 2 * k);
+}
+
+// Scatter of one particle into a bin's FP64 LDS tile (SoA: mx, my, mz, m).
+__device__ __forceinline__ void scatter_tile(double* tile, int i0, const AxisWeights& A, float m, float vol,
+                                             const float v[3], const float C[9], const float tau[9], float dx,
+                                             float inv_dx, float dt) {
+    p2g_contrib(A, m, vol, v, C, tau, dx, inv_dx, dt,
+                [&](int i, int j, int k, float mx, float my, float mz, float mm) {
+                    const int t = i0 + (i * kTS + j) * kTS + k;
+                    atomicAdd(&tile[t], (double)mx);
+                    atomicAdd(&tile[kTS3 + t], (double)my);
+                    atomicAdd(&tile[2 * kTS3 + t], (double)mz);
+                    atomicAdd(&tile[3 * kTS3 + t], (double)mm);
+                });
+}
+
+// Writes a bin's LDS tile (nodes b0-1 .. b0+10) to its slot of the global
+// tile buffer as float4 (mv, m) and stamps it with the epoch.
+__device__ void write_tile(const double* s_acc, float4* __restrict__ tiles, int* __restrict__ stamps, int bin,
+                           int epoch) {
+    float4* out = tiles + (int64_t)bin * kTS3;
+    for (int t = threadIdx.x; t < kTS3; t += kWG)
+        out[t] = make_float4((float)s_acc[t], (float)s_acc[kTS3 + t], (float)s_acc[2 * kTS3 + t],
+                             (float)s_acc[3 * kTS3 + t]);
+    if (threadIdx.x == 0) stamps[bin] = epoch;
+}
+
+struct SubstepArgs {
+    MpmParams prm;
+    BinGeom g;
+    float dt;
+    int epoch_out;
+    const float* __restrict__ src;
+    float* __restrict__ dst;
+    const int* __restrict__ cnt_cur;
+    int* __restrict__ cnt_nxt;
+    const float4* __restrict__ vel;
+    float* __restrict__ acc;
+    int* __restrict__ gflag;
+    int* __restrict__ status;
+    float4* __restrict__ tiles;
+    int* __restrict__ stamps;
+};
+
+// Reserve the destination slot of every active lane: lanes staying in
+// `own_bin` share one atomic per wave; the others take one each.  Returns the
+// slot or -1 when both the bin and the overflow run are full.
+__device__ __forceinline__ int64_t reserve_slot(bool act, int new_bin, int own_bin, const SubstepArgs& a) {
+    const bool stay = act && new_bin == own_bin;
+    const unsigned long long mask = __ballot(stay);
+    const int lane = __lane_id();
+    int pos = 0;
+    if (mask) {
+        const int leader = __ffsll((long long)mask) - 1;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&a.cnt_nxt[own_bin], __popcll(mask));
+        base = __shfl(base, leader);
+        pos = base + __popcll(mask & ((1ull << lane) - 1ull));
+    }
+    if (act && !stay) pos = atomicAdd(&a.cnt_nxt[new_bin], 1);
+    if (!act) return -1;
+    if (pos < a.g.cap) return (int64_t)new_bin * a.g.cap + pos;
+    const int r = atomicAdd(&a.cnt_nxt[a.g.n_bins], 1);
+    if (r < a.g.ovf_cap) return (int64_t)a.g.n_bins * a.g.cap + r;
+    atomicAdd(&a.status[0], 1);
+    return -1;
+}
+
+__device__ __forceinline__ void store_particle(float* __restrict__ dst, int64_t s, int64_t stride,
+                                               const float x0[3], const float u[3], const float v[3],
+                                               const float C[9], const float F[9], const float tau[9],
+                                               float m, float vol, float pid) {
+    float* p = dst + s;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        p[(MPM_P_X0 + d) * stride] = x0[d];
+        p[(MPM_P_U + d) * stride] = u[d];
+        p[(MPM_P_V + d) * stride] = v[d];
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        p[(MPM_P_C + k) * stride] = C[k];
+        p[(MPM_P_F + k) * stride] = F[k];
+        p[(MPM_P_TAU + k) * stride] = tau[k];
+    }
+    p[MPM_P_MASS * stride] = m;
+    p[MPM_P_VOL * stride] = vol;
+    p[MPM_P_PID * stride] = pid;
+}
+
+// The per-particle body of k_g2p2g.  `tile_vel`/`tile_acc` are the bin's LDS
+// tiles (nullptr on the overflow path); b0 the bin origin in cells.
+__device__ __forceinline__ void g2p2g_particle(const SubstepArgs& a, bool act, int64_t slot, int own_bin,
+                                               const int b0[3], const float4* tile_vel, double* tile_acc) {
+    const int64_t S = a.g.stride;
+    const float* sp = a.src + slot;
+    const MpmParams& prm = a.prm;
+    const float dt = a.dt;
+    float x0[3], u[3], F[9], m = 0.f, vol = 0.f, pid = 0.f;
+    if (act) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            x0[d] = sp[(MPM_P_X0 + d) * S];
+            u[d] = sp[(MPM_P_U + d) * S];
+        }
+#pragma unroll
+        for (int k = 0; k < 9; ++k) F[k] = sp[(MPM_P_F + k) * S];
+        m = sp[MPM_P_MASS * S];
+        vol = sp[MPM_P_VOL * S];
+        pid = sp[MPM_P_PID * S];
+    } else {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) { x0[d] = 0.5f; u[d] = 0.f; }
+#pragma unroll
+        for (int k = 0; k < 9; ++k) F[k] = (k % 4 == 0) ? 1.f : 0.f;
+    }
+    float x[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) x[d] = x0[d] + u[d];
+    AxisWeights A;
+    axis_weights(x, prm.inv_dx, A);
+    const bool live = act && stencil_ok(A.base, a.g.ng);
+    float v[3], Cm[9], G[9], tau[9];
+    int new_bin = own_bin;
+    if (live) {
+        const int lx = A.base[0] - b0[0], ly = A.base[1] - b0[1], lz = A.base[2] - b0[2];
+        if (tile_vel != nullptr && lx >= 0 && ly >= 0 && lz >= 0 && lx < kBinEdge && ly < kBinEdge &&
+            lz < kBinEdge) {
+            gather_tile<kTG * kTG, kTG>(tile_vel, (lx * kTG + ly) * kTG + lz, A, prm.inv_dx, v, Cm, G);
+        } else {
+            // global gather: same contraction, nodes read from the grid
+            const int ng = a.g.ng;
+            const float4* vb = a.vel + (((int64_t)A.base[0] * ng + A.base[1]) * ng + A.base[2]);
+            float4 nodes[27];
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j)
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) nodes[(i * 3 + j) * 3 + k] = vb[((int64_t)i * ng + j) * ng + k];
+            gather_tile<9, 3>(nodes, 0, A, prm.inv_dx, v, Cm, G);
+        }
+        sand_update(G, dt, F, prm.mu, prm.lam, prm.alpha, tau);
+        // x += dt v ; u = x - x0 ; the next substep recomputes x = x0 + u
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const float xn = x[d] + dt * v[d];
+            u[d] = xn - x0[d];
+            x[d] = x0[d] + u[d];
+        }
+        AxisWeights An;
+        axis_weights(x, prm.inv_dx, An);
+        if (stencil_ok(An.base, a.g.ng)) {
+            new_bin = bin_of_base(An.base, a.g.nb);
+            const int sx = An.base[0] - (b0[0] - 1), sy = An.base[1] - (b0[1] - 1), sz = An.base[2] - (b0[2] - 1);
+            if (tile_acc != nullptr && sx >= 0 && sy >= 0 && sz >= 0 && sx <= kTS - 3 && sy <= kTS - 3 &&
+                sz <= kTS - 3) {
+                scatter_tile(tile_acc, (sx * kTS + sy) * kTS + sz, An, m, vol, v, Cm, tau, prm.dx, prm.inv_dx, dt);
+            } else {
+                if (tile_acc != nullptr) atomicAdd(&a.status[1], 1);
+                scatter_spill(a.acc, a.gflag, a.g, An, m, vol, v, Cm, tau, prm.dx, prm.inv_dx, dt);
+            }
+        } else {
+            atomicAdd(&a.status[2], 1);  // left the grid: frozen from here on
+            int bc[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) bc[d] = min(max(An.base[d], 0), a.g.ng - 3);
+            new_bin = bin_of_base(bc, a.g.nb);
+        }
+    } else if (act) {
+        // frozen particle (stencil outside the grid): copied unchanged
+#pragma unroll
+        for (int d = 0; d < 3; ++d) v[d] = sp[(MPM_P_V + d) * S];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            Cm[k] = sp[(MPM_P_C + k) * S];
+            tau[k] = sp[(MPM_P_TAU + k) * S];
+        }
+        int bc[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) bc[d] = min(max(A.base[d], 0), a.g.ng - 3);
+        new_bin = bin_of_base(bc, a.g.nb);
+    }
+    const int64_t dslot = reserve_slot(act, new_bin, own_bin, a);
+    if (dslot >= 0) store_particle(a.dst, dslot, S, x0, u, v, Cm, F, tau, m, vol, pid);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Grid update of one 8^3 grid block: gathers the P2G tiles of the current
+// epoch from the bins overlapping it (a node x is covered by the tiles of
+// bins (x+1)>>3 and, when x+1-8((x+1)>>3) <= 2, the bin before) and the
+// spill grid, sums in FP64, then normalization, gravity and colliders.
+// Also resets the next plane set's bin counters.  clear_only: only drains
+// the spill grid.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kWG)
+    k_grid_bins(float4* __restrict__ acc, float4* __restrict__ vel, int* __restrict__ gflag,
+                const int* __restrict__ stamps, const float4* __restrict__ tiles, int* __restrict__ cnt_nxt,
+                BinGeom g, MpmParams prm, ColliderSet cs, float time, float dt, int epoch, int clear_only) {
+    __shared__ int s_live[27];
+    __shared__ int s_any;
+    const int blk = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int nb = g.nb, ng = g.ng;
+    const int bz = blk % nb, by = (blk / nb) % nb, bx = blk / (nb * nb);
+    if (tid == 0 && cnt_nxt) {
+        cnt_nxt[blk] = 0;
+        if (blk == 0) cnt_nxt[g.n_bins] = 0;
+    }
+    if (tid == 0) s_any = 0;
+    __syncthreads();
+    if (tid < 27) {
+        const int x = bx + tid / 9 - 1, y = by + (tid / 3) % 3 - 1, z = bz + tid % 3 - 1;
+        int live = 0;
+        if (!clear_only && x >= 0 && y >= 0 && z >= 0 && x < nb && y < nb && z < nb)
+            live = stamps[(x * nb + y) * nb + z] == epoch;
+        s_live[tid] = live;
+        if (live) atomicOr(&s_any, 1);
+    }
+    const int spill = gflag[blk];
+    __syncthreads();
+    if (!spill && !s_any) return;
+    for (int t = tid; t < 512; t += kWG) {
+        const int x = bx * 8 + (t >> 6), y = by * 8 + ((t >> 3) & 7), z = bz * 8 + (t & 7);
+        if (x >= ng || y >= ng || z >= ng) continue;
+        const int64_t node = ((int64_t)x * ng + y) * ng + z;
+        double sx = 0.0, sy = 0.0, sz = 0.0, sm = 0.0;
+        if (spill) {
+            const float4 a4 = acc[node];
+            sx = a4.x; sy = a4.y; sz = a4.z; sm = a4.w;
+            acc[node] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        if (clear_only) continue;
+        const int p[3] = {x, y, z};
+        int hi[3], nlo[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            hi[d] = (p[d] + 1) >> 3;
+            nlo[d] = (p[d] + 1 - 8 * hi[d] <= 2 && hi[d] >= 1) ? 2 : 1;
+        }
+        const int bb[3] = {bx, by, bz};
+        for (int ix = 0; ix < nlo[0]; ++ix)
+            for (int iy = 0; iy < nlo[1]; ++iy)
+                for (int iz = 0; iz < nlo[2]; ++iz) {
+                    const int b[3] = {hi[0] - ix, hi[1] - iy, hi[2] - iz};
+                    if (b[0] >= nb || b[1] >= nb || b[2] >= nb) continue;
+                    const int li = ((b[0] - bb[0] + 1) * 3 + (b[1] - bb[1] + 1)) * 3 + (b[2] - bb[2] + 1);
+                    if (!s_live[li]) continue;
+                    const int lx = p[0] - 8 * b[0] + 1, ly = p[1] - 8 * b[1] + 1, lz = p[2] - 8 * b[2] + 1;
+                    const float4 tv = tiles[(int64_t)((b[0] * nb + b[1]) * nb + b[2]) * kTS3 +
+                                            (lx * kTS + ly) * kTS + lz];
+                    sx += tv.x; sy += tv.y; sz += tv.z; sm += tv.w;
+                }
+        const float4 a4 = make_float4((float)sx, (float)sy, (float)sz, (float)sm);
+        vel[node] = grid_node_update(a4, x, y, z, ng, prm.dx, dt, prm.gravity, cs, time);
+    }
+    __syncthreads();
+    if (tid == 0 && spill) gflag[blk] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Fused G2P (this substep) + P2G (next substep), one workgroup per bin plus
+// workgroups over the overflow run.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kWG) k_g2p2g(SubstepArgs a) {
+    __shared__ float4 s_vel[kTG3];
+    __shared__ double s_acc[4 * kTS3];
+    const int tid = threadIdx.x;
+    const int wg = blockIdx.x;
+    const BinGeom& g = a.g;
+    if (wg >= g.n_bins) {
+        // overflow run of the current plane set: global gather and spill scatter
+        const int n_ovf = min(a.cnt_cur[g.n_bins], g.ovf_cap);
+        const int i = (wg - g.n_bins) * kWG + tid;
+        const int b0[3] = {0, 0, 0};
+        if ((wg - g.n_bins) * kWG >= n_ovf) return;
+        g2p2g_particle(a, i < n_ovf, (int64_t)g.n_bins * g.cap + i, g.n_bins, b0, nullptr, nullptr);
+        return;
+    }
+    const int bin = xcd_bin(wg, g.n_bins);
+    const int n_in = min(a.cnt_cur[bin], g.cap);
+    if (n_in == 0) return;
+    const int nb = g.nb, ng = g.ng;
+    const int b0[3] = {(bin / (nb * nb)) * kBinEdge, ((bin / nb) % nb) * kBinEdge, (bin % nb) * kBinEdge};
+    for (int t = tid; t < kTG3; t += kWG) {
+        const int x = b0[0] + t / (kTG * kTG), y = b0[1] + (t / kTG) % kTG, z = b0[2] + t % kTG;
+        float4 val = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (x < ng && y < ng && z < ng) val = a.vel[((int64_t)x * ng + y) * ng + z];
+        s_vel[t] = val;
+    }
+    for (int t = tid; t < 4 * kTS3; t += kWG) s_acc[t] = 0.0;
+    __syncthreads();
+    const int64_t slot0 = (int64_t)bin * g.cap;
+    for (int it = 0; it < n_in; it += kWG) {
+        const int i = it + tid;
+        g2p2g_particle(a, i < n_in, slot0 + i, bin, b0, s_vel, s_acc);
+    }
+    __syncthreads();
+    write_tile(s_acc, a.tiles, a.stamps, bin, a.epoch_out);
+}
+
+// ---------------------------------------------------------------------------
+// P2G of the first substep (p2g_apic_stress_given, mpm_utils.py:190-227) from
+// the binned state: same tiles as k_g2p2g.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kWG)
+    k_p2g_bins(MpmParams prm, BinGeom g, float dt, const float* __restrict__ planes, const int* __restrict__ cnt,
+               float* __restrict__ acc, int* __restrict__ gflag, float4* __restrict__ tiles,
+               int* __restrict__ stamps, int epoch_out) {
+    __shared__ double s_acc[4 * kTS3];
+    const int tid = threadIdx.x;
+    const int wg = blockIdx.x;
+    const int64_t S = g.stride;
+    const bool ovf = wg >= g.n_bins;
+    int n_in, bin;
+    int64_t slot0;
+    if (ovf) {
+        n_in = min(cnt[g.n_bins], g.ovf_cap) - (wg - g.n_bins) * kWG;
+        n_in = min(max(n_in, 0), kWG);
+        bin = g.n_bins;
+        slot0 = (int64_t)g.n_bins * g.cap + (int64_t)(wg - g.n_bins) * kWG;
+    } else {
+        bin = xcd_bin(wg, g.n_bins);
+        n_in = min(cnt[bin], g.cap);
+        slot0 = (int64_t)bin * g.cap;
+    }
+    if (n_in == 0) return;
+    const int nb = g.nb;
+    const int b0[3] = {(bin / (nb * nb)) * kBinEdge, ((bin / nb) % nb) * kBinEdge, (bin % nb) * kBinEdge};
+    if (!ovf) {
+        for (int t = tid; t < 4 * kTS3; t += kWG) s_acc[t] = 0.0;
+        __syncthreads();
+    }
+    for (int it = 0; it < n_in; it += kWG) {
+        const int i = it + tid;
+        if (i >= n_in) continue;
+        const float* sp = planes + slot0 + i;
+        float x[3], v[3], Cm[9], tau[9];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            x[d] = sp[(MPM_P_X0 + d) * S] + sp[(MPM_P_U + d) * S];
+            v[d] = sp[(MPM_P_V + d) * S];
+        }
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            Cm[k] = sp[(MPM_P_C + k) * S];
+            tau[k] = sp[(MPM_P_TAU + k) * S];
+        }
+        const float m = sp[MPM_P_MASS * S], vol = sp[MPM_P_VOL * S];
+        AxisWeights A;
+        axis_weights(x, prm.inv_dx, A);
+        if (!stencil_ok(A.base, g.ng)) continue;
+        const int sx = A.base[0] - (b0[0] - 1), sy = A.base[1] - (b0[1] - 1), sz = A.base[2] - (b0[2] - 1);
+        if (!ovf && sx >= 0 && sy >= 0 && sz >= 0 && sx <= kTS - 3 && sy <= kTS - 3 && sz <= kTS - 3) {
+            scatter_tile(s_acc, (sx * kTS + sy) * kTS + sz, A, m, vol, v, Cm, tau, prm.dx, prm.inv_dx, dt);
+        } else {
+            scatter_spill(acc, gflag, g, A, m, vol, v, Cm, tau, prm.dx, prm.inv_dx, dt);
+        }
+    }
+    if (!ovf) {
+        __syncthreads();
+        write_tile(s_acc, tiles, stamps, bin, epoch_out);
+    }
+}
+
+// fill n ints with v (bin counters; a kernel rather than a memset so that the
+// sequence can be captured in a graph)
+__global__ void k_fill_int(int* __restrict__ p, int n, int v) {
+    const int i = blockIdx.x * kWG + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// Insert every particle of a flat state (slot = caller index) into the bins
+// of plane set `cur` by the base cell of x = x0 + u.
+__global__ void __launch_bounds__(kWG)
+    k_bins_insert(const float* __restrict__ flat, int64_t fstride, int n, float inv_dx, BinGeom g,
+                  float* __restrict__ planes, int* __restrict__ cnt, int* __restrict__ status) {
+    const int p = blockIdx.x * kWG + threadIdx.x;
+    if (p >= n) return;
+    float x[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) x[d] = flat[(MPM_P_X0 + d) * fstride + p] + flat[(MPM_P_U + d) * fstride + p];
+    int b[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) b[d] = min(max((int)(x[d] * inv_dx - 0.5f), 0), g.ng - 3);
+    const int bin = bin_of_base(b, g.nb);
+    const int pos = atomicAdd(&cnt[bin], 1);
+    int64_t s;
+    if (pos < g.cap) {
+        s = (int64_t)bin * g.cap + pos;
+    } else {
+        const int r = atomicAdd(&cnt[g.n_bins], 1);
+        if (r >= g.ovf_cap) {
+            atomicAdd(&status[0], 1);
+            return;
+        }
+        s = (int64_t)g.n_bins * g.cap + r;
+    }
+    for (int k = 0; k < MPM_NPLANES; ++k) planes[k * g.stride + s] = flat[k * fstride + p];
+}
+
+// flat[pid] <- every valid slot of plane set `cur`
+__global__ void __launch_bounds__(kWG)
+    k_bins_extract(const float* __restrict__ planes, const int* __restrict__ cnt, BinGeom g,
+                   float* __restrict__ flat, int64_t fstride, int n) {
+    const int wg = blockIdx.x;
+    int n_in;
+    int64_t slot0;
+    if (wg >= g.n_bins) {
+        n_in = min(max(min(cnt[g.n_bins], g.ovf_cap) - (wg - g.n_bins) * kWG, 0), kWG);
+        slot0 = (int64_t)g.n_bins * g.cap + (int64_t)(wg - g.n_bins) * kWG;
+    } else {
+        n_in = min(cnt[wg], g.cap);
+        slot0 = (int64_t)wg * g.cap;
+    }
+    for (int i = threadIdx.x; i < n_in; i += kWG) {
+        const int64_t s = slot0 + i;
+        const int pid = __float_as_int(planes[MPM_P_PID * g.stride + s]);
+        if (pid < 0 || pid >= n) continue;
+        for (int k = 0; k < MPM_NPLANES; ++k) flat[k * fstride + pid] = planes[k * g.stride + s];
+    }
+}
+
+// ===========================================================================
+// C-ABI of the binned path
+// ===========================================================================
+namespace {
+
+int check_bins(const MpmParams* prm, const MpmBins* b, BinGeom* g) {
+    int rc = check_grid(prm);
+    if (rc) return rc;
+    if (!b || !b->planes[0] || !b->planes[1] || !b->count) return set_err(-20, "bins: NULL pointer");
+    const int nb = (prm->n_grid + kBinEdge - 1) / kBinEdge;
+    if (b->nb != nb) return set_err(-21, "bins: nb must be ceil(n_grid / 8)");
+    if (b->bin_cap < 1 || b->overflow_cap < 0) return set_err(-22, "bins: capacity invalid");
+    if (b->cur != 0 && b->cur != 1) return set_err(-23, "bins: cur must be 0 or 1");
+    const int64_t need = (int64_t)nb * nb * nb * b->bin_cap + b->overflow_cap;
+    if (b->stride < need) return set_err(-24, "bins: stride < nb^3 * bin_cap + overflow_cap");
+    g->ng = prm->n_grid;
+    g->nb = nb;
+    g->n_bins = nb * nb * nb;
+    g->cap = b->bin_cap;
+    g->ovf_cap = b->overflow_cap;
+    g->stride = b->stride;
+    return 0;
+}
+
+struct WsLayout {
+    int64_t off_status;
+    float4* acc;     // spill grid
+    float4* vel;     // grid velocities (v, m)
+    int* gflag;      // spill flags per grid block
+    int* status;     // 64 counters
+    int* stamps;     // epoch of each bin's tile
+    float4* tiles;   // nb^3 tiles of kTS^3 nodes
+    int64_t bytes;
+};
+
+int64_t align256(int64_t b) { return ((b + 255) / 256) * 256; }
+
+WsLayout ws_layout(const MpmParams* prm, void* base) {
+    const int64_t ng = prm->n_grid;
+    const int64_t nn = ng * ng * ng;
+    const int64_t nb = (ng + kBinEdge - 1) / kBinEdge;
+    const int64_t nbins = nb * nb * nb;
+    char* p = (char*)base;
+    WsLayout w;
+    int64_t off = 0;
+    w.acc = (float4*)(p + off);
+    off += align256(nn * 16);
+    w.vel = (float4*)(p + off);
+    off += align256(nn * 16);
+    w.gflag = (int*)(p + off);
+    off += align256(nbins * 4);
+    w.status = (int*)(p + off);
+    w.off_status = off;
+    off += 256;
+    w.stamps = (int*)(p + off);
+    off += align256(nbins * 4);
+    w.tiles = (float4*)(p + off);
+    off += align256(nbins * kTS3 * 16);
+    w.bytes = off;
+    return w;
+}
+
+unsigned ovf_wgs(const BinGeom& g) { return (unsigned)((g.ovf_cap + kWG - 1) / kWG); }
+
+}  // namespace
+
+extern "C" {
+
+int64_t mpm_bins_workspace_bytes(const MpmParams* prm) {
+    if (check_grid(prm)) return -1;
+    return ws_layout(prm, nullptr).bytes;
+}
+
+int64_t mpm_bins_status_offset(const MpmParams* prm) {
+    if (check_grid(prm)) return -1;
+    return ws_layout(prm, nullptr).off_status;
+}
+
+int mpm_bins_insert(const MpmParams* prm, const MpmState* flat, const MpmBins* bins, void* workspace,
+                    void* stream) {
+    BinGeom g;
+    int rc = check_bins(prm, bins, &g);
+    if (rc) return rc;
+    if (!flat || !flat->planes || flat->n < 0 || flat->stride < flat->n) return set_err(-25, "flat state invalid");
+    if (!workspace) return set_err(-10, "workspace is NULL");
+    WsLayout w = ws_layout(prm, workspace);
+    hipStream_t s = (hipStream_t)stream;
+    int* cnt = bins->count + (int64_t)bins->cur * (g.n_bins + 1);
+    hipLaunchKernelGGL(k_fill_int, dim3((g.n_bins + 1 + kWG - 1) / kWG), dim3(kWG), 0, s, cnt, g.n_bins + 1, 0);
+    if ((rc = check_launch("k_fill_int"))) return rc;
+    if (flat->n == 0) return 0;
+    hipLaunchKernelGGL(k_bins_insert, dim3((unsigned)((flat->n + kWG - 1) / kWG)), dim3(kWG), 0, s, flat->planes,
+                       flat->stride, (int)flat->n, prm->inv_dx, g, bins->planes[bins->cur], cnt, w.status);
+    return check_launch("k_bins_insert");
+}
+
+int mpm_bins_extract(const MpmParams* prm, const MpmBins* bins, const MpmState* flat, void* stream) {
+    BinGeom g;
+    int rc = check_bins(prm, bins, &g);
+    if (rc) return rc;
+    if (!flat || !flat->planes || flat->n < 0 || flat->stride < flat->n) return set_err(-25, "flat state invalid");
+    const int* cnt = bins->count + (int64_t)bins->cur * (g.n_bins + 1);
+    hipLaunchKernelGGL(k_bins_extract, dim3(g.n_bins + ovf_wgs(g)), dim3(kWG), 0, (hipStream_t)stream,
+                       bins->planes[bins->cur], cnt, g, flat->planes, flat->stride, (int)flat->n);
+    return check_launch("k_bins_extract");
+}
+
+int mpm_bins_clear_grid(const MpmParams* prm, MpmBins* bins, void* workspace, void* stream) {
+    BinGeom g;
+    int rc = check_bins(prm, bins, &g);
+    if (rc) return rc;
+    if (!workspace) return set_err(-10, "workspace is NULL");
+    WsLayout w = ws_layout(prm, workspace);
+    ColliderSet cs;
+    make_collider_set(nullptr, 0, &cs);
+    hipLaunchKernelGGL(k_grid_bins, dim3(g.n_bins), dim3(kWG), 0, (hipStream_t)stream, w.acc, w.vel, w.gflag,
+                       (const int*)w.stamps, (const float4*)w.tiles, (int*)nullptr, g, *prm, cs, 0.f, 0.f,
+                       bins->epoch, 1);
+    if ((rc = check_launch("k_grid_bins(clear)"))) return rc;
+    bins->epoch += 1;  // the pending tiles are no longer current
+    return 0;
+}
+
+int mpm_bins_p2g(const MpmParams* prm, MpmBins* bins, void* workspace, float dt, void* stream) {
+    BinGeom g;
+    int rc = check_bins(prm, bins, &g);
+    if (rc) return rc;
+    if ((rc = check_params(prm))) return rc;
+    if (!workspace) return set_err(-10, "workspace is NULL");
+    WsLayout w = ws_layout(prm, workspace);
+    const int* cnt = bins->count + (int64_t)bins->cur * (g.n_bins + 1);
+    const int epoch_out = bins->epoch + 1;
+    hipLaunchKernelGGL(k_p2g_bins, dim3(g.n_bins + ovf_wgs(g)), dim3(kWG), 0, (hipStream_t)stream, *prm, g, dt,
+                       bins->planes[bins->cur], cnt, (float*)w.acc, w.gflag, w.tiles, w.stamps, epoch_out);
+    if ((rc = check_launch("k_p2g_bins"))) return rc;
+    bins->epoch = epoch_out;
+    return 0;
+}
+
+int mpm_bins_grid_update(const MpmParams* prm, const MpmBins* bins, const MpmCollider* colliders,
+                         int n_colliders, void* workspace, float time, float dt, void* stream) {
+    BinGeom g;
+    int rc = check_bins(prm, bins, &g);
+    if (rc) return rc;
+    if ((rc = check_params(prm))) return rc;
+    if (!workspace) return set_err(-10, "workspace is NULL");
+    ColliderSet cs;
+    if ((rc = make_collider_set(colliders, n_colliders, &cs))) return rc;
+    WsLayout w = ws_layout(prm, workspace);
+    int* cnt_nxt = bins->count + (int64_t)(1 - bins->cur) * (g.n_bins + 1);
+    hipLaunchKernelGGL(k_grid_bins, dim3(g.n_bins), dim3(kWG), 0, (hipStream_t)stream, w.acc, w.vel, w.gflag,
+                       (const int*)w.stamps, (const float4*)w.tiles, cnt_nxt, g, *prm, cs, time, dt, bins->epoch, 0);
+    return check_launch("k_grid_bins");
+}
+
+int mpm_bins_g2p2g(const MpmParams* prm, MpmBins* bins, void* workspace, float dt, void* stream) {
+    BinGeom g;
+    int rc = check_bins(prm, bins, &g);
+    if (rc) return rc;
+    if ((rc = check_params(prm))) return rc;
+    if (!workspace) return set_err(-10, "workspace is NULL");
+    WsLayout w = ws_layout(prm, workspace);
+    const int cur = bins->cur, nxt = 1 - cur;
+    SubstepArgs a;
+    a.prm = *prm;
+    a.g = g;
+    a.dt = dt;
+    a.epoch_out = bins->epoch + 1;
+    a.src = bins->planes[cur];
+    a.dst = bins->planes[nxt];
+    a.cnt_cur = bins->count + (int64_t)cur * (g.n_bins + 1);
+    a.cnt_nxt = bins->count + (int64_t)nxt * (g.n_bins + 1);
+    a.vel = w.vel;
+    a.acc = (float*)w.acc;
+    a.gflag = w.gflag;
+    a.status = w.status;
+    a.tiles = w.tiles;
+    a.stamps = w.stamps;
+    hipLaunchKernelGGL(k_g2p2g, dim3(g.n_bins + ovf_wgs(g)), dim3(kWG), 0, (hipStream_t)stream, a);
+    if ((rc = check_launch("k_g2p2g"))) return rc;
+    bins->epoch = a.epoch_out;
+    bins->cur = nxt;
+    return 0;
+}
+
+int mpm_bins_substep(const MpmParams* prm, MpmBins* bins, const MpmCollider* colliders, int n_colliders,
+                     void* workspace, float time, float dt, void* stream) {
+    int rc = mpm_bins_grid_update(prm, bins, colliders, n_colliders, workspace, time, dt, stream);
+    if (rc) return rc;
+    return mpm_bins_g2p2g(prm, bins, workspace, dt, stream);
+}
+
+int mpm_bins_run(const MpmParams* prm, MpmBins* bins, const MpmCollider* colliders, int n_colliders,
+                 void* workspace, double time, float dt, int n_substeps, void* stream) {
+    if (!bins) return set_err(-20, "bins: NULL pointer");
+    if (n_substeps < 0) return set_err(-26, "n_substeps < 0");
+    double t = time;
+    for (int k = 0; k < n_substeps; ++k) {
+        int rc = mpm_bins_substep(prm, bins, colliders, n_colliders, workspace, (float)t, dt, stream);
+        if (rc) return rc;
+        t = t + (double)dt;
+    }
+    return 0;
+}
+
+}  // extern "C"

@@ -14,14 +14,28 @@
 #include <stdio.h>
 #include <string.h>
 
+#include "mpm_common.h"
 #include "mpm_device.h"
 #include "mpm_hip.h"
 
 using namespace mpm;
 
 namespace {
-
 thread_local char g_err[512] = "";
+constexpr int kBlock = 256;
+inline unsigned grid_1d(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+// The reference indexes the grid without bounds checks (mpm_utils.py:215-217,
+// 264-266) and relies on the bounding box to keep particles inside.  Here a
+// particle whose 3x3x3 stencil leaves the grid is skipped (P2G) and left
+// unchanged (G2P) instead of touching memory outside the grid.
+__device__ __forceinline__ bool stencil_inside(const mpm::Weights& W, int ng) {
+    return W.base[0] >= 0 && W.base[1] >= 0 && W.base[2] >= 0 && W.base[0] <= ng - 3 &&
+           W.base[1] <= ng - 3 && W.base[2] <= ng - 3;
+}
+}  // namespace
+
+namespace mpm {
 
 int set_err(int code, const char* msg) {
     snprintf(g_err, sizeof(g_err), "%s", msg);
@@ -50,31 +64,24 @@ int check_params(const MpmParams* prm) {
     return 0;
 }
 
+int make_collider_set(const MpmCollider* cols, int n, ColliderSet* cs) {
+    if (n < 0 || n > MPM_MAX_COLLIDERS) return set_err(-8, "n_colliders out of range [0, 16]");
+    if (n > 0 && !cols) return set_err(-9, "colliders is NULL");
+    memset(cs, 0, sizeof(*cs));
+    cs->n = n;
+    for (int i = 0; i < n; ++i) cs->c[i] = cols[i];
+    return 0;
+}
+
+}  // namespace mpm
+
+namespace {
 int check_state(const MpmState* st) {
     if (!st || !st->planes) return set_err(-4, "state or planes is NULL");
     if (st->n < 0 || st->stride < st->n) return set_err(-5, "state n/stride invalid");
     if (st->n > 0x7fffffffLL) return set_err(-6, "more than 2^31-1 particles");
     return 0;
 }
-
-constexpr int kBlock = 256;
-
-inline unsigned grid_1d(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
-
-// The reference indexes the grid without bounds checks (mpm_utils.py:215-217,
-// 264-266) and relies on the bounding box to keep particles inside.  Here a
-// particle whose 3x3x3 stencil leaves the grid is skipped (P2G) and left
-// unchanged (G2P) instead of touching memory outside the grid.
-__device__ __forceinline__ bool stencil_inside(const mpm::Weights& W, int ng) {
-    return W.base[0] >= 0 && W.base[1] >= 0 && W.base[2] >= 0 && W.base[0] <= ng - 3 &&
-           W.base[1] <= ng - 3 && W.base[2] <= ng - 3;
-}
-
-struct ColliderSet {
-    MpmCollider c[MPM_MAX_COLLIDERS];
-    int n;
-};
-
 }  // namespace
 
 // --------------------------------------------------------------------------
@@ -144,42 +151,6 @@ __global__ void __launch_bounds__(kBlock)
                 unsafeAtomicAdd(gn + 2, add[2]);
                 unsafeAtomicAdd(gn + 3, wm);
             }
-}
-
-// --------------------------------------------------------------------------
-// Grid update of one node: grid_normalization_and_gravity (mpm_utils.py:
-// 229-237) then each active collider in registration order.
-// --------------------------------------------------------------------------
-__device__ __forceinline__ float4 grid_node_update(float4 acc, int gx, int gy, int gz, int ng, float dx,
-                                                   float dt, const float grav[3],
-                                                   const ColliderSet& cs, float time) {
-    float4 o = make_float4(0.f, 0.f, 0.f, acc.w);
-    if (acc.w > 1e-15f) {
-        float im = 1.0f / acc.w;
-        o.x = acc.x * im + dt * grav[0];
-        o.y = acc.y * im + dt * grav[1];
-        o.z = acc.z * im + dt * grav[2];
-    }
-    for (int c = 0; c < cs.n; ++c) {
-        const MpmCollider& cl = cs.c[c];
-        if (!(time >= cl.start_time && time < cl.end_time)) continue;
-        if (cl.kind == 0) {
-            float o0 = (float)gx * dx - cl.point[0];
-            float o1 = (float)gy * dx - cl.point[1];
-            float o2 = (float)gz * dx - cl.point[2];
-            float dot = o0 * cl.normal[0] + o1 * cl.normal[1] + o2 * cl.normal[2];
-            if (dot < 0.0f) { o.x = 0.f; o.y = 0.f; o.z = 0.f; }
-        } else {
-            const int pad = 3;
-            if (gx < pad && o.x < 0.f) o.x = 0.f;
-            if (gx >= ng - pad && o.x > 0.f) o.x = 0.f;
-            if (gy < pad && o.y < 0.f) o.y = 0.f;
-            if (gy >= ng - pad && o.y > 0.f) o.y = 0.f;
-            if (gz < pad && o.z < 0.f) o.z = 0.f;
-            if (gz >= ng - pad && o.z > 0.f) o.z = 0.f;
-        }
-    }
-    return o;
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -324,14 +295,6 @@ int mpm_p2g(const MpmParams* prm, const MpmState* st, float* grid_acc, float dt,
     return check_launch("k_p2g_atomic");
 }
 
-static int make_collider_set(const MpmCollider* cols, int n, ColliderSet* cs) {
-    if (n < 0 || n > MPM_MAX_COLLIDERS) return set_err(-8, "n_colliders out of range [0, 16]");
-    if (n > 0 && !cols) return set_err(-9, "colliders is NULL");
-    memset(cs, 0, sizeof(*cs));
-    cs->n = n;
-    for (int i = 0; i < n; ++i) cs->c[i] = cols[i];
-    return 0;
-}
 
 int mpm_grid_update(const MpmParams* prm, const float* grid_acc, float* grid_v,
                     const MpmCollider* colliders, int n_colliders, float time, float dt,

@@ -96,6 +96,14 @@ class MPM_Simulator_HIP:
         self.debug = False
         self.stage_timer = None      # list -> p2g2p records per-stage events into it
         self.colliders = []          # list of dicts, registration order
+        # binned path (default): state moves into per-block bins at the first
+        # substep and back to the flat caller-order planes when a loader runs
+        self.use_bins = True
+        self._bins = None            # dict of tensors + ctypes struct
+        self._loc = "flat"           # where the authoritative state lives
+        self._flat_valid = True      # flat planes mirror the binned state
+        self._pending_dt = None      # dt of the P2G held in the accumulator
+        self._since_check = 0
         self._alloc_grid()
 
     def _alloc_grid(self):
@@ -157,6 +165,7 @@ class MPM_Simulator_HIP:
     # set_parameters (mpm_solver_warp.py:89-124)
     # ------------------------------------------------------------------
     def set_parameters(self, device="cuda:0", **kwargs):
+        self._ensure_flat()
         m = self.mpm_model
         if "material" in kwargs:
             if kwargs["material"] in _MATERIALS:
@@ -222,23 +231,27 @@ class MPM_Simulator_HIP:
             raise NotImplementedError("only material='sand' is implemented by the HIP substep")
         lib = L.load()
         prm = self._params()
-        st = self._state()
         cols, ncol = self._colliders()
         stream = _stream_handle(self.device)
-        if self.stage_timer is None:
-            ws = self.workspace.numel() - 256
-            rc = lib.mpm_step(prm, st, cols, ncol, self._ws_ptr(), ws, float(self.time), float(dt), stream)
-            L.check(rc, "mpm_step")
+        if self.use_bins:
+            self._bins_run(1, float(dt))          # advances self.time
         else:
-            self._timed_stages(lib, prm, st, cols, ncol, float(dt), stream)
+            self._ensure_flat()
+            st = self._state()
+            if self.stage_timer is None:
+                ws = self.workspace.numel() - 256
+                rc = lib.mpm_step(prm, st, cols, ncol, self._ws_ptr(), ws, float(self.time), float(dt), stream)
+                L.check(rc, "mpm_step")
+            else:
+                self._timed_stages(lib, prm, st, cols, ncol, float(dt), stream)
+            self.time = self.time + dt
         if self.debug:
-            v = self.planes[L.P_V:L.P_V + 3, :self.n_particles]
+            v = self.export_uvFstress_from_warp()[1]
             vmax = float(torch.max(torch.abs(v)))
             print("max particle v: ", vmax)
             print("max allowed  v: ", self.mpm_model.dx / dt)
             if vmax > self.mpm_model.dx / dt:
                 raise RuntimeError(f"CFL violated: max |v| {vmax} > dx/dt {self.mpm_model.dx / dt}")
-        self.time = self.time + dt
         if print_norm:
             u, v, F_disp, stress = self.export_uvFstress_from_warp()
             print("at mpm step", step, end=" ")
@@ -265,14 +278,134 @@ class MPM_Simulator_HIP:
         self.stage_timer.append(ev)
 
     def run(self, n_substeps, dt):
-        """n_substeps consecutive p2g2p calls without host synchronisation."""
-        for k in range(int(n_substeps)):
-            self.p2g2p(k, dt)
+        """n_substeps consecutive substeps (p2g2p without prints) in one
+        library call, without host synchronisation."""
+        if self.mpm_model.material != 2:
+            raise NotImplementedError("only material='sand' is implemented by the HIP substep")
+        if self.use_bins and self.stage_timer is None and not self.debug:
+            self._bins_run(int(n_substeps), float(dt))
+        else:
+            for k in range(int(n_substeps)):
+                self.p2g2p(k, dt)
+
+    # ------------------------------------------------------------------
+    # binned path
+    # ------------------------------------------------------------------
+    def _flat_state(self) -> L.MpmState:
+        return self._state()
+
+    def _alloc_bins(self):
+        """Bins of 8^3 cells sized from the current positions: capacity twice
+        the fullest bin (at least 256 slots), overflow run n/16."""
+        ng = self.mpm_model.n_grid
+        nb = (ng + 7) // 8
+        n = self.n_particles
+        x = self.planes[L.P_X0:L.P_X0 + 3, :n] + self.planes[L.P_U:L.P_U + 3, :n]
+        b = torch.clamp((x * float(self.mpm_model.inv_dx) - 0.5).to(torch.int64), 0, ng - 3) >> 3
+        idx = (b[0] * nb + b[1]) * nb + b[2]
+        fullest = int(torch.bincount(idx, minlength=nb ** 3).max()) if n > 0 else 0
+        cap = max(256, ((2 * fullest + 63) // 64) * 64)
+        ovf = max(4096, ((n // 16 + 255) // 256) * 256)
+        stride = nb ** 3 * cap + ovf
+        old = self._bins
+        if old is not None and old["nb"] == nb and old["cap"] >= cap and old["ovf"] >= ovf:
+            return old
+        planes = torch.zeros((2, L.NPLANES, stride), dtype=torch.float32, device=self.device)
+        count = torch.zeros((2, nb ** 3 + 1), dtype=torch.int32, device=self.device)
+        st = L.MpmBins()
+        st.nb, st.bin_cap, st.overflow_cap, st.cur, st.epoch, st.stride = nb, cap, ovf, 0, 0, stride
+        st.planes[0] = planes[0].data_ptr()
+        st.planes[1] = planes[1].data_ptr()
+        st.count = count.data_ptr()
+        prm = self._params()
+        nbytes = L.load().mpm_bins_workspace_bytes(prm)
+        ws = torch.zeros(int(nbytes) + 256, dtype=torch.uint8, device=self.device)
+        self._bins = dict(nb=nb, cap=cap, ovf=ovf, stride=stride, planes=planes, count=count, struct=st, ws=ws,
+                          ng=ng)
+        return self._bins
+
+    def _bins_ws_ptr(self):
+        p = self._bins["ws"].data_ptr()
+        return (p + 255) & ~255
+
+    def _bins_status(self):
+        """Device status counters (host sync)."""
+        off = self._bins_ws_ptr() - self._bins["ws"].data_ptr()
+        start = off + int(L.load().mpm_bins_status_offset(self._params()))
+        return self._bins["ws"][start:start + 256].view(torch.int32).cpu()
+
+    def _check_bins_health(self):
+        st = self._bins_status()
+        if int(st[0]) > 0:
+            raise RuntimeError(f"{int(st[0])} particles were dropped: bins and overflow run full")
+        cnt = self._bins["count"][self._bins["struct"].cur, -1].item()
+        if cnt > self._bins["ovf"] // 2:
+            # re-bin with larger bins at the next substep
+            self._ensure_flat()
+        self._since_check = 0
+
+    def _bins_run(self, n_substeps, dt):
+        lib = L.load()
+        prm = self._params()
+        stream = _stream_handle(self.device)
+        if self._loc == "flat":
+            B = self._alloc_bins()
+            ws = self._bins_ws_ptr()
+            L.check(lib.mpm_bins_clear_grid(prm, B["struct"], ws, stream), "mpm_bins_clear_grid")
+            L.check(lib.mpm_bins_insert(prm, self._flat_state(), B["struct"], ws, stream), "mpm_bins_insert")
+            L.check(lib.mpm_bins_p2g(prm, B["struct"], ws, dt, stream), "mpm_bins_p2g")
+            self._pending_dt = dt
+            self._loc = "bins"
+        B = self._bins
+        ws = self._bins_ws_ptr()
+        if self._pending_dt != dt:
+            L.check(lib.mpm_bins_clear_grid(prm, B["struct"], ws, stream), "mpm_bins_clear_grid")
+            L.check(lib.mpm_bins_p2g(prm, B["struct"], ws, dt, stream), "mpm_bins_p2g")
+            self._pending_dt = dt
+        cols, ncol = self._colliders()
+        if self.stage_timer is None:
+            L.check(lib.mpm_bins_run(prm, B["struct"], cols, ncol, ws, float(self.time), dt, int(n_substeps),
+                                     stream), "mpm_bins_run")
+            for _ in range(int(n_substeps)):
+                self.time = self.time + dt
+        else:
+            # the two kernels of each substep with a torch.cuda.Event between them
+            for _ in range(int(n_substeps)):
+                ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+                ev[0].record()
+                L.check(lib.mpm_bins_grid_update(prm, B["struct"], cols, ncol, ws, float(self.time), dt, stream),
+                        "mpm_bins_grid_update")
+                ev[1].record()
+                L.check(lib.mpm_bins_g2p2g(prm, B["struct"], ws, dt, stream), "mpm_bins_g2p2g")
+                ev[2].record()
+                self.stage_timer.append(ev)
+                self.time = self.time + dt
+        self._flat_valid = False
+        self._since_check += int(n_substeps)
+        if self._since_check >= 256:
+            self._check_bins_health()
+
+    def _sync_flat(self):
+        """Make the flat planes mirror the binned state (no change of owner)."""
+        if self._loc == "bins" and not self._flat_valid:
+            lib = L.load()
+            L.check(lib.mpm_bins_extract(self._params(), self._bins["struct"], self._flat_state(),
+                                         _stream_handle(self.device)), "mpm_bins_extract")
+            self._flat_valid = True
+            self._check_bins_health()
+
+    def _ensure_flat(self):
+        """Move the authoritative state to the flat planes (before a loader,
+        a parameter change or a step on the stage path)."""
+        self._sync_flat()
+        self._loc = "flat"
+        self._pending_dt = None
 
     # ------------------------------------------------------------------
     # torch loaders (mpm_solver_warp.py:229-266) -- copies, not aliases
     # ------------------------------------------------------------------
     def _import(self, plane, ncomp, t, mode=0):
+        self._ensure_flat()
         if t.numel() != self.n_particles * ncomp:
             raise ValueError(f"expected {self.n_particles} x {ncomp} values, got shape {tuple(t.shape)}")
         t = t.to(device=self.device, dtype=torch.float32).contiguous()
@@ -330,6 +463,7 @@ class MPM_Simulator_HIP:
     # export (mpm_solver_warp.py:305-310)
     # ------------------------------------------------------------------
     def _export(self, plane, ncomp, mode=0):
+        self._sync_flat()
         out = torch.empty((self.n_particles, ncomp), dtype=torch.float32, device=self.device)
         lib = L.load()
         L.check(lib.mpm_export(self._state(), plane, ncomp, out.data_ptr(), mode, _stream_handle(self.device)),
@@ -367,15 +501,26 @@ class MPM_Simulator_HIP:
 
     # grid views (reference: mpm_state.grid_m / grid_v_in / grid_v_out)
     def _grid_views(self):
+        """(accumulator, velocity) grids, (ng, ng, ng, 4) each.  On the binned
+        path the accumulator holds the pending P2G of the next substep."""
         ng = self.mpm_model.n_grid
         nn = ng ** 3
-        off = self._ws_ptr() - self.workspace.data_ptr()
-        f = self.workspace[off:off + 2 * nn * 16].view(torch.float32).view(2, ng, ng, ng, 4)
-        return f[0], f[1]
+        if self._loc == "bins" and self._bins is not None:
+            ws = self._bins["ws"]
+            off = self._bins_ws_ptr() - ws.data_ptr()
+            vel_off = ((nn * 16 + 255) // 256) * 256      # workspace layout of the binned path
+        else:
+            ws = self.workspace
+            off = self._ws_ptr() - ws.data_ptr()
+            vel_off = nn * 16
+        acc = ws[off:off + nn * 16].view(torch.float32).view(ng, ng, ng, 4)
+        vel = ws[off + vel_off:off + vel_off + nn * 16].view(torch.float32).view(ng, ng, ng, 4)
+        return acc, vel
 
     @property
     def grid_m(self):
-        return self._grid_views()[0][..., 3]
+        """Node masses of the last substep's grid update."""
+        return self._grid_views()[1][..., 3]
 
     @property
     def grid_v_in(self):
