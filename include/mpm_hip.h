@@ -40,7 +40,7 @@
 extern "C" {
 #endif
 
-#define MPM_ABI_VERSION 1
+#define MPM_ABI_VERSION 2
 
 /* plane indices */
 #define MPM_P_X0 0
@@ -130,10 +130,13 @@ int mpm_step(const MpmParams* prm, const MpmState* st, const MpmCollider* collid
 /* ---- binned path: the solver's own particle layout --------------------
  *
  * Particles are kept in bins of 8^3 grid cells keyed by the base cell of
- * their B-spline stencil (int(x*inv_dx - 0.5), mpm_utils.py:197-199).  Bin
- * b owns slots [b*bin_cap, b*bin_cap + count[b]) of a plane set (same plane
- * order as MpmState); slots from nb^3*bin_cap on are an overflow run for
- * bins that are full.  Two plane sets alternate between substeps.
+ * their B-spline stencil (int(x*inv_dx - 0.5), mpm_utils.py:197-199).  A
+ * plane set is bin-major: bin b is a block of MPM_NPLANES planes of bin_cap
+ * floats (plane order as MpmState), slot i of bin b / plane p at
+ * (b*MPM_NPLANES + p)*bin_cap + i; bin b holds count[b] particles (slots
+ * with pid -1 are holes left by particles that moved).  The blocks after the
+ * nb^3 bins form an overflow run of overflow_cap slots for bins that are
+ * full.  Two plane sets alternate between substeps.
  * A substep is the grid update of this substep, then one fused kernel doing
  * G2P of this substep and P2G of the next.  The P2G lands in per-bin tiles
  * of the workspace stamped with `epoch`, so between calls the workspace
@@ -143,11 +146,11 @@ int mpm_step(const MpmParams* prm, const MpmState* st, const MpmCollider* collid
 typedef struct MpmBins {
     int32_t nb;           /* bins per axis = ceil(n_grid / 8) */
     int32_t bin_cap;      /* slots per bin */
-    int32_t overflow_cap; /* slots of the overflow run */
+    int32_t overflow_cap; /* slots of the overflow run, a multiple of bin_cap */
     int32_t cur;          /* plane set holding the current state (0 or 1) */
     int32_t epoch;        /* stamp of the pending P2G tiles; start at 0 */
     int32_t reserved;
-    int64_t stride;       /* plane stride >= nb^3 * bin_cap + overflow_cap */
+    int64_t stride;       /* floats per plane set >= (nb^3*bin_cap + overflow_cap) * MPM_NPLANES */
     float* planes[2];     /* two sets of MPM_NPLANES planes */
     int32_t* count;       /* [2][nb^3 + 1] particles per bin; last = overflow run */
 } MpmBins;

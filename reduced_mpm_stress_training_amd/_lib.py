@@ -10,7 +10,7 @@ import os
 
 from ._build import LIB_PATH
 
-MPM_ABI_VERSION = 1
+MPM_ABI_VERSION = 2
 MAX_COLLIDERS = 16
 
 # plane indices of the particle state buffer

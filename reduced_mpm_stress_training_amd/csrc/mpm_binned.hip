@@ -37,17 +37,25 @@ using namespace mpm;
 
 namespace {
 
-constexpr int kWG = 256;
+constexpr int kWG = 512;          // threads per bin workgroup
 constexpr int kBinEdge = 8;           // cells per bin edge
 constexpr int kTG = kBinEdge + 2;     // gather tile edge (nodes b0 .. b0+9)
 constexpr int kTS = kBinEdge + 4;     // scatter tile edge (nodes b0-1 .. b0+10)
 constexpr int kTG3 = kTG * kTG * kTG;
 constexpr int kTS3 = kTS * kTS * kTS;
 
+// Plane sets are bin-major: bin b (b >= n_bins: overflow run) is a block of
+// MPM_NPLANES planes of `cap` floats, so slot i of bin b, plane p, is at
+// (b * MPM_NPLANES + p) * cap + i.  A slot index is b * cap + i.
 struct BinGeom {
     int ng, nb, n_bins, cap, ovf_cap;
     int64_t stride;
 };
+
+__device__ __forceinline__ int64_t slot_off(const BinGeom& g, int64_t slot) {
+    const int64_t b = slot / g.cap, i = slot - b * g.cap;
+    return b * MPM_NPLANES * g.cap + i;
+}
 
 __device__ __forceinline__ int bin_of_base(const int b[3], int nb) {
     return ((b[0] >> 3) * nb + (b[1] >> 3)) * nb + (b[2] >> 3);
@@ -57,11 +65,13 @@ __device__ __forceinline__ bool stencil_ok(const int b[3], int ng) {
     return b[0] >= 0 && b[1] >= 0 && b[2] >= 0 && b[0] <= ng - 3 && b[1] <= ng - 3 && b[2] <= ng - 3;
 }
 
-// workgroup -> bin, keeping runs of spatially adjacent bins on one XCD
-// (workgroups are dealt to the 8 XCDs round-robin).
+// workgroup -> bin.  Workgroups are dealt to the 8 XCDs round-robin, so the
+// identity mapping spreads every row of bins (z fastest) over all XCDs.
+// Mapping contiguous slabs of bins to one XCD would leave XCDs idle for
+// scenes that fill only part of the domain (a sand column fills the middle).
 __device__ __forceinline__ int xcd_bin(int wg, int n_bins) {
-    if ((n_bins & 7) != 0) return wg;
-    return (wg & 7) * (n_bins >> 3) + (wg >> 3);
+    (void)n_bins;
+    return wg;
 }
 
 __device__ __forceinline__ void mark_block(int* __restrict__ gflag, int nb, int x, int y, int z) {
@@ -74,15 +84,10 @@ __device__ void scatter_spill(float* __restrict__ acc, int* __restrict__ gflag, 
                               const AxisWeights& A, float m, float vol, const float v[3], const float C[9],
                               const float tau[9], float dx, float inv_dx, float dt) {
     const int ng = g.ng;
-    p2g_contrib(A, m, vol, v, C, tau, dx, inv_dx, dt,
-                [&](int i, int j, int k, float mx, float my, float mz, float mm) {
-                    const int x = A.base[0] + i, y = A.base[1] + j, z = A.base[2] + k;
-                    float* a = acc + 4 * (((int64_t)x * ng + y) * ng + z);
-                    unsafeAtomicAdd(a + 0, mx);
-                    unsafeAtomicAdd(a + 1, my);
-                    unsafeAtomicAdd(a + 2, mz);
-                    unsafeAtomicAdd(a + 3, mm);
-                });
+    p2g_contrib(A, m, vol, v, C, tau, dx, inv_dx, dt, [&](int c, int i, int j, int k, float val) {
+        const int x = A.base[0] + i, y = A.base[1] + j, z = A.base[2] + k;
+        unsafeAtomicAdd(acc + 4 * (((int64_t)x * ng + y) * ng + z) + c, val);
+    });
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -96,14 +101,9 @@ __device__ void scatter_spill(float* __restrict__ acc, int* __restrict__ gflag, 
 __device__ __forceinline__ void scatter_tile(double* tile, int i0, const AxisWeights& A, float m, float vol,
                                              const float v[3], const float C[9], const float tau[9], float dx,
                                              float inv_dx, float dt) {
-    p2g_contrib(A, m, vol, v, C, tau, dx, inv_dx, dt,
-                [&](int i, int j, int k, float mx, float my, float mz, float mm) {
-                    const int t = i0 + (i * kTS + j) * kTS + k;
-                    atomicAdd(&tile[t], (double)mx);
-                    atomicAdd(&tile[kTS3 + t], (double)my);
-                    atomicAdd(&tile[2 * kTS3 + t], (double)mz);
-                    atomicAdd(&tile[3 * kTS3 + t], (double)mm);
-                });
+    p2g_contrib(A, m, vol, v, C, tau, dx, inv_dx, dt, [&](int c, int i, int j, int k, float val) {
+        atomicAdd(&tile[c * kTS3 + i0 + (i * kTS + j) * kTS + k], (double)val);
+    });
 }
 
 // Writes a bin's LDS tile (nodes b0-1 .. b0+10) to its slot of the global
@@ -134,35 +134,39 @@ struct SubstepArgs {
     int* __restrict__ stamps;
 };
 
-// Reserve the destination slot of every active lane: lanes staying in
-// `own_bin` share one atomic per wave; the others take one each.  Returns the
-// slot or -1 when both the bin and the overflow run are full.
-__device__ __forceinline__ int64_t reserve_slot(bool act, int new_bin, int own_bin, const SubstepArgs& a) {
-    const bool stay = act && new_bin == own_bin;
+// Destination slot of every active lane.  A bin's workgroup reserves
+// n_in slots of its own bin in the next plane set with one atomic before its
+// loop (`stay_base`, -1 on the overflow path); lanes that stay in the bin
+// take consecutive slots of that run through an LDS counter.  Lanes that move
+// take one atomic each on their new bin.  Returns the float offset of the
+// slot (slot_off), or -1 when both the bin and the overflow run are full.
+__device__ __forceinline__ int64_t reserve_slot(bool act, int new_bin, int own_bin, int stay_base,
+                                                int* s_stay, const SubstepArgs& a) {
+    const bool stay = act && new_bin == own_bin && stay_base >= 0;
     const unsigned long long mask = __ballot(stay);
     const int lane = __lane_id();
     int pos = 0;
     if (mask) {
         const int leader = __ffsll((long long)mask) - 1;
         int base = 0;
-        if (lane == leader) base = atomicAdd(&a.cnt_nxt[own_bin], __popcll(mask));
+        if (lane == leader) base = atomicAdd(s_stay, __popcll(mask));
         base = __shfl(base, leader);
-        pos = base + __popcll(mask & ((1ull << lane) - 1ull));
+        pos = stay_base + base + __popcll(mask & ((1ull << lane) - 1ull));
     }
     if (act && !stay) pos = atomicAdd(&a.cnt_nxt[new_bin], 1);
     if (!act) return -1;
-    if (pos < a.g.cap) return (int64_t)new_bin * a.g.cap + pos;
+    if (pos < a.g.cap) return (int64_t)new_bin * MPM_NPLANES * a.g.cap + pos;
     const int r = atomicAdd(&a.cnt_nxt[a.g.n_bins], 1);
-    if (r < a.g.ovf_cap) return (int64_t)a.g.n_bins * a.g.cap + r;
+    if (r < a.g.ovf_cap) return slot_off(a.g, (int64_t)a.g.n_bins * a.g.cap + r);
     atomicAdd(&a.status[0], 1);
     return -1;
 }
 
-__device__ __forceinline__ void store_particle(float* __restrict__ dst, int64_t s, int64_t stride,
+__device__ __forceinline__ void store_particle(float* __restrict__ dst, int64_t off, int64_t stride,
                                                const float x0[3], const float u[3], const float v[3],
                                                const float C[9], const float F[9], const float tau[9],
                                                float m, float vol, float pid) {
-    float* p = dst + s;
+    float* p = dst + off;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         p[(MPM_P_X0 + d) * stride] = x0[d];
@@ -182,30 +186,50 @@ __device__ __forceinline__ void store_particle(float* __restrict__ dst, int64_t 
 
 // The per-particle body of k_g2p2g.  `tile_vel`/`tile_acc` are the bin's LDS
 // tiles (nullptr on the overflow path); b0 the bin origin in cells.
-__device__ __forceinline__ void g2p2g_particle(const SubstepArgs& a, bool act, int64_t slot, int own_bin,
-                                               const int b0[3], const float4* tile_vel, double* tile_acc) {
-    const int64_t S = a.g.stride;
-    const float* sp = a.src + slot;
-    const MpmParams& prm = a.prm;
-    const float dt = a.dt;
-    float x0[3], u[3], F[9], m = 0.f, vol = 0.f, pid = 0.f;
+// The G2P inputs of one particle (x0, u, F, m, vol, pid), loaded ahead of use.
+struct ParticleIn {
+    float x0[3], u[3], F[9], m, vol, pid;
+};
+
+__device__ __forceinline__ void load_particle(const float* __restrict__ src, int64_t off, int64_t S, bool act,
+                                              ParticleIn& q) {
     if (act) {
+        const float* sp = src + off;
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            x0[d] = sp[(MPM_P_X0 + d) * S];
-            u[d] = sp[(MPM_P_U + d) * S];
+            q.x0[d] = sp[(MPM_P_X0 + d) * S];
+            q.u[d] = sp[(MPM_P_U + d) * S];
         }
 #pragma unroll
-        for (int k = 0; k < 9; ++k) F[k] = sp[(MPM_P_F + k) * S];
-        m = sp[MPM_P_MASS * S];
-        vol = sp[MPM_P_VOL * S];
-        pid = sp[MPM_P_PID * S];
+        for (int k = 0; k < 9; ++k) q.F[k] = sp[(MPM_P_F + k) * S];
+        q.m = sp[MPM_P_MASS * S];
+        q.vol = sp[MPM_P_VOL * S];
+        q.pid = sp[MPM_P_PID * S];
     } else {
 #pragma unroll
-        for (int d = 0; d < 3; ++d) { x0[d] = 0.5f; u[d] = 0.f; }
+        for (int d = 0; d < 3; ++d) { q.x0[d] = 0.5f; q.u[d] = 0.f; }
 #pragma unroll
-        for (int k = 0; k < 9; ++k) F[k] = (k % 4 == 0) ? 1.f : 0.f;
+        for (int k = 0; k < 9; ++k) q.F[k] = (k % 4 == 0) ? 1.f : 0.f;
+        q.m = q.vol = 0.f;
+        q.pid = __int_as_float(-1);
     }
+}
+
+__device__ __forceinline__ void g2p2g_particle(const SubstepArgs& a, bool act_slot, int64_t off, int own_bin,
+                                               int stay_base, int* s_stay, const int b0[3], const float* tile_vel,
+                                               double* tile_acc, const ParticleIn& q) {
+    const int64_t S = a.g.cap;
+    const float* sp = a.src + off;
+    const MpmParams& prm = a.prm;
+    const float dt = a.dt;
+    // slots left empty by particles that moved to another bin carry pid -1
+    const bool act = act_slot && __float_as_int(q.pid) >= 0;
+    float x0[3], u[3], F[9];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) { x0[d] = q.x0[d]; u[d] = q.u[d]; }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) F[k] = q.F[k];
+    const float m = q.m, vol = q.vol, pid = q.pid;
     float x[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) x[d] = x0[d] + u[d];
@@ -214,11 +238,14 @@ __device__ __forceinline__ void g2p2g_particle(const SubstepArgs& a, bool act, i
     const bool live = act && stencil_ok(A.base, a.g.ng);
     float v[3], Cm[9], G[9], tau[9];
     int new_bin = own_bin;
+    int mode = 0;  // next-substep scatter: 0 none, 1 LDS tile, 2 spill grid
+    int i0 = 0;
     if (live) {
         const int lx = A.base[0] - b0[0], ly = A.base[1] - b0[1], lz = A.base[2] - b0[2];
         if (tile_vel != nullptr && lx >= 0 && ly >= 0 && lz >= 0 && lx < kBinEdge && ly < kBinEdge &&
             lz < kBinEdge) {
-            gather_tile<kTG * kTG, kTG>(tile_vel, (lx * kTG + ly) * kTG + lz, A, prm.inv_dx, v, Cm, G);
+            gather_tile<kTG * kTG, kTG>(SoANodes{tile_vel, kTG3}, (lx * kTG + ly) * kTG + lz, A, prm.inv_dx, v, Cm,
+                                        G);
         } else {
             // global gather: same contraction, nodes read from the grid
             const int ng = a.g.ng;
@@ -230,7 +257,7 @@ __device__ __forceinline__ void g2p2g_particle(const SubstepArgs& a, bool act, i
                 for (int j = 0; j < 3; ++j)
 #pragma unroll
                     for (int k = 0; k < 3; ++k) nodes[(i * 3 + j) * 3 + k] = vb[((int64_t)i * ng + j) * ng + k];
-            gather_tile<9, 3>(nodes, 0, A, prm.inv_dx, v, Cm, G);
+            gather_tile<9, 3>(Float4Nodes{nodes}, 0, A, prm.inv_dx, v, Cm, G);
         }
         sand_update(G, dt, F, prm.mu, prm.lam, prm.alpha, tau);
         // x += dt v ; u = x - x0 ; the next substep recomputes x = x0 + u
@@ -240,23 +267,23 @@ __device__ __forceinline__ void g2p2g_particle(const SubstepArgs& a, bool act, i
             u[d] = xn - x0[d];
             x[d] = x0[d] + u[d];
         }
-        AxisWeights An;
-        axis_weights(x, prm.inv_dx, An);
-        if (stencil_ok(An.base, a.g.ng)) {
-            new_bin = bin_of_base(An.base, a.g.nb);
-            const int sx = An.base[0] - (b0[0] - 1), sy = An.base[1] - (b0[1] - 1), sz = An.base[2] - (b0[2] - 1);
+        axis_weights(x, prm.inv_dx, A);     // weights of the next substep's P2G
+        if (stencil_ok(A.base, a.g.ng)) {
+            new_bin = bin_of_base(A.base, a.g.nb);
+            const int sx = A.base[0] - (b0[0] - 1), sy = A.base[1] - (b0[1] - 1), sz = A.base[2] - (b0[2] - 1);
             if (tile_acc != nullptr && sx >= 0 && sy >= 0 && sz >= 0 && sx <= kTS - 3 && sy <= kTS - 3 &&
                 sz <= kTS - 3) {
-                scatter_tile(tile_acc, (sx * kTS + sy) * kTS + sz, An, m, vol, v, Cm, tau, prm.dx, prm.inv_dx, dt);
+                mode = 1;
+                i0 = (sx * kTS + sy) * kTS + sz;
             } else {
+                mode = 2;
                 if (tile_acc != nullptr) atomicAdd(&a.status[1], 1);
-                scatter_spill(a.acc, a.gflag, a.g, An, m, vol, v, Cm, tau, prm.dx, prm.inv_dx, dt);
             }
         } else {
             atomicAdd(&a.status[2], 1);  // left the grid: frozen from here on
             int bc[3];
 #pragma unroll
-            for (int d = 0; d < 3; ++d) bc[d] = min(max(An.base[d], 0), a.g.ng - 3);
+            for (int d = 0; d < 3; ++d) bc[d] = min(max(A.base[d], 0), a.g.ng - 3);
             new_bin = bin_of_base(bc, a.g.nb);
         }
     } else if (act) {
@@ -273,8 +300,14 @@ __device__ __forceinline__ void g2p2g_particle(const SubstepArgs& a, bool act, i
         for (int d = 0; d < 3; ++d) bc[d] = min(max(A.base[d], 0), a.g.ng - 3);
         new_bin = bin_of_base(bc, a.g.nb);
     }
-    const int64_t dslot = reserve_slot(act, new_bin, own_bin, a);
-    if (dslot >= 0) store_particle(a.dst, dslot, S, x0, u, v, Cm, F, tau, m, vol, pid);
+    const int64_t dslot = reserve_slot(act, new_bin, own_bin, stay_base, s_stay, a);
+    if (dslot >= 0) store_particle(a.dst, dslot, a.g.cap, x0, u, v, Cm, F, tau, m, vol, pid);
+    // P2G of the next substep, after the store so that x0, u, F are dead
+    if (mode == 1) {
+        scatter_tile(tile_acc, i0, A, m, vol, v, Cm, tau, prm.dx, prm.inv_dx, dt);
+    } else if (mode == 2) {
+        scatter_spill(a.acc, a.gflag, a.g, A, m, vol, v, Cm, tau, prm.dx, prm.inv_dx, dt);
+    }
 }
 
 }  // namespace
@@ -356,9 +389,11 @@ __global__ void __launch_bounds__(kWG)
 // Fused G2P (this substep) + P2G (next substep), one workgroup per bin plus
 // workgroups over the overflow run.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kWG) k_g2p2g(SubstepArgs a) {
-    __shared__ float4 s_vel[kTG3];
+__global__ void __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(2, 2))) k_g2p2g(SubstepArgs a) {
+    __shared__ float s_vel[3 * kTG3];
     __shared__ double s_acc[4 * kTS3];
+    __shared__ int s_stay;
+    __shared__ int s_base;
     const int tid = threadIdx.x;
     const int wg = blockIdx.x;
     const BinGeom& g = a.g;
@@ -368,7 +403,10 @@ __global__ void __launch_bounds__(kWG) k_g2p2g(SubstepArgs a) {
         const int i = (wg - g.n_bins) * kWG + tid;
         const int b0[3] = {0, 0, 0};
         if ((wg - g.n_bins) * kWG >= n_ovf) return;
-        g2p2g_particle(a, i < n_ovf, (int64_t)g.n_bins * g.cap + i, g.n_bins, b0, nullptr, nullptr);
+        const int64_t off = slot_off(g, (int64_t)g.n_bins * g.cap + i);
+        ParticleIn q;
+        load_particle(a.src, off, g.cap, i < n_ovf, q);
+        g2p2g_particle(a, i < n_ovf, off, g.n_bins, -1, nullptr, b0, nullptr, nullptr, q);
         return;
     }
     const int bin = xcd_bin(wg, g.n_bins);
@@ -376,20 +414,35 @@ __global__ void __launch_bounds__(kWG) k_g2p2g(SubstepArgs a) {
     if (n_in == 0) return;
     const int nb = g.nb, ng = g.ng;
     const int b0[3] = {(bin / (nb * nb)) * kBinEdge, ((bin / nb) % nb) * kBinEdge, (bin % nb) * kBinEdge};
+    if (tid == 0) {
+        s_base = atomicAdd(&a.cnt_nxt[bin], n_in);   // the run stayers will fill
+        s_stay = 0;
+    }
     for (int t = tid; t < kTG3; t += kWG) {
         const int x = b0[0] + t / (kTG * kTG), y = b0[1] + (t / kTG) % kTG, z = b0[2] + t % kTG;
         float4 val = make_float4(0.f, 0.f, 0.f, 0.f);
         if (x < ng && y < ng && z < ng) val = a.vel[((int64_t)x * ng + y) * ng + z];
-        s_vel[t] = val;
+        s_vel[t] = val.x;
+        s_vel[kTG3 + t] = val.y;
+        s_vel[2 * kTG3 + t] = val.z;
     }
     for (int t = tid; t < 4 * kTS3; t += kWG) s_acc[t] = 0.0;
+    const int64_t off0 = (int64_t)bin * MPM_NPLANES * g.cap;
+    ParticleIn q;
+    load_particle(a.src, off0 + tid, g.cap, tid < n_in, q);
     __syncthreads();
-    const int64_t slot0 = (int64_t)bin * g.cap;
+    const int stay_base = s_base;
     for (int it = 0; it < n_in; it += kWG) {
         const int i = it + tid;
-        g2p2g_particle(a, i < n_in, slot0 + i, bin, b0, s_vel, s_acc);
+        const ParticleIn cur = q;
+        if (it + kWG < n_in) load_particle(a.src, off0 + i + kWG, g.cap, i + kWG < n_in, q);  // prefetch
+        g2p2g_particle(a, i < n_in, off0 + i, bin, stay_base, &s_stay, b0, s_vel, s_acc, cur);
     }
     __syncthreads();
+    // reserved slots nobody stayed in become holes (pid -1)
+    const int hole_lo = stay_base + s_stay, hole_hi = min(stay_base + n_in, g.cap);
+    for (int h = hole_lo + tid; h < hole_hi; h += kWG)
+        a.dst[off0 + (int64_t)MPM_P_PID * g.cap + h] = __int_as_float(-1);
     write_tile(s_acc, a.tiles, a.stamps, bin, a.epoch_out);
 }
 
@@ -404,7 +457,7 @@ __global__ void __launch_bounds__(kWG)
     __shared__ double s_acc[4 * kTS3];
     const int tid = threadIdx.x;
     const int wg = blockIdx.x;
-    const int64_t S = g.stride;
+    const int64_t S = g.cap;
     const bool ovf = wg >= g.n_bins;
     int n_in, bin;
     int64_t slot0;
@@ -428,7 +481,7 @@ __global__ void __launch_bounds__(kWG)
     for (int it = 0; it < n_in; it += kWG) {
         const int i = it + tid;
         if (i >= n_in) continue;
-        const float* sp = planes + slot0 + i;
+        const float* sp = planes + slot_off(g, slot0 + i);
         float x[3], v[3], Cm[9], tau[9];
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
@@ -441,6 +494,7 @@ __global__ void __launch_bounds__(kWG)
             tau[k] = sp[(MPM_P_TAU + k) * S];
         }
         const float m = sp[MPM_P_MASS * S], vol = sp[MPM_P_VOL * S];
+        if (__float_as_int(sp[MPM_P_PID * S]) < 0) continue;  // hole
         AxisWeights A;
         axis_weights(x, prm.inv_dx, A);
         if (!stencil_ok(A.base, g.ng)) continue;
@@ -490,7 +544,8 @@ __global__ void __launch_bounds__(kWG)
         }
         s = (int64_t)g.n_bins * g.cap + r;
     }
-    for (int k = 0; k < MPM_NPLANES; ++k) planes[k * g.stride + s] = flat[k * fstride + p];
+    const int64_t off = slot_off(g, s);
+    for (int k = 0; k < MPM_NPLANES; ++k) planes[off + (int64_t)k * g.cap] = flat[k * fstride + p];
 }
 
 // flat[pid] <- every valid slot of plane set `cur`
@@ -508,10 +563,10 @@ __global__ void __launch_bounds__(kWG)
         slot0 = (int64_t)wg * g.cap;
     }
     for (int i = threadIdx.x; i < n_in; i += kWG) {
-        const int64_t s = slot0 + i;
-        const int pid = __float_as_int(planes[MPM_P_PID * g.stride + s]);
+        const int64_t off = slot_off(g, slot0 + i);
+        const int pid = __float_as_int(planes[off + (int64_t)MPM_P_PID * g.cap]);
         if (pid < 0 || pid >= n) continue;
-        for (int k = 0; k < MPM_NPLANES; ++k) flat[k * fstride + pid] = planes[k * g.stride + s];
+        for (int k = 0; k < MPM_NPLANES; ++k) flat[k * fstride + pid] = planes[off + (int64_t)k * g.cap];
     }
 }
 
@@ -526,10 +581,11 @@ int check_bins(const MpmParams* prm, const MpmBins* b, BinGeom* g) {
     if (!b || !b->planes[0] || !b->planes[1] || !b->count) return set_err(-20, "bins: NULL pointer");
     const int nb = (prm->n_grid + kBinEdge - 1) / kBinEdge;
     if (b->nb != nb) return set_err(-21, "bins: nb must be ceil(n_grid / 8)");
-    if (b->bin_cap < 1 || b->overflow_cap < 0) return set_err(-22, "bins: capacity invalid");
+    if (b->bin_cap < 1 || b->overflow_cap < 0 || b->overflow_cap % b->bin_cap != 0)
+        return set_err(-22, "bins: bin_cap < 1 or overflow_cap not a multiple of bin_cap");
     if (b->cur != 0 && b->cur != 1) return set_err(-23, "bins: cur must be 0 or 1");
-    const int64_t need = (int64_t)nb * nb * nb * b->bin_cap + b->overflow_cap;
-    if (b->stride < need) return set_err(-24, "bins: stride < nb^3 * bin_cap + overflow_cap");
+    const int64_t need = ((int64_t)nb * nb * nb * b->bin_cap + b->overflow_cap) * MPM_NPLANES;
+    if (b->stride < need) return set_err(-24, "bins: stride < (nb^3 * bin_cap + overflow_cap) * MPM_NPLANES");
     g->ng = prm->n_grid;
     g->nb = nb;
     g->n_bins = nb * nb * nb;

@@ -54,9 +54,11 @@ __device__ __forceinline__ float rsqrt_nr(float x) {
 template <int P, int Q>
 __device__ __forceinline__ void jacobi_rotate(float& app, float& aqq, float& apq, float& arp,
                                               float& arq, float V[9]) {
+    // t only needs to define a rotation (c, s below are normalised from it),
+    // so the hardware sqrt and reciprocal are enough here
     float d = aqq - app;
-    float den = fabsf(d) + sqrtf(d * d + 4.0f * apq * apq);
-    float t = (den > 0.0f) ? ((d >= 0.0f ? 2.0f : -2.0f) * apq) / den : 0.0f;
+    float den = fabsf(d) + __builtin_sqrtf(d * d + 4.0f * apq * apq);
+    float t = (den > 0.0f) ? ((d >= 0.0f ? 2.0f : -2.0f) * apq) * __builtin_amdgcn_rcpf(den) : 0.0f;
     float c = rsqrt_nr(1.0f + t * t);
     float s = t * c;
     app = app - t * apq;

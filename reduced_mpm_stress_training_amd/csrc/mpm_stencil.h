@@ -45,59 +45,57 @@ __device__ __forceinline__ void axis_weights(const float x[3], float inv_dx, Axi
 }
 
 // G2P gather (mpm_utils.py:261-273) from a node-velocity tile: node
-// (i,j,k) of the stencil is tile[idx0 + i*SX + j*SY + k] (float4, .xyz = v).
+// (i,j,k) of the stencil is entry idx0 + i*SX + j*SY + k of the tile.
 // Returns v, C = 4 inv_dx sum v_i (x) (x_i - x_p) w (grid units) and
 // grad v = sum v_i (x) grad w.
-template <int SX, int SY, typename TileT>
-__device__ __forceinline__ void gather_tile(const TileT* tile, int idx0, const AxisWeights& A,
-                                            float inv_dx, float v[3], float C[9], float G[9]) {
-    // accumulators over (x-axis factor) of sums over y, z
-    float yww[3][3], yew[3][3], ydw[3][3], ywe[3][3], ywd[3][3];  // [i][component]
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            yww[i][c] = 0.f; yew[i][c] = 0.f; ydw[i][c] = 0.f; ywe[i][c] = 0.f; ywd[i][c] = 0.f;
-        }
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-            float zw[3], ze[3], zd[3];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) { zw[c] = 0.f; ze[c] = 0.f; zd[c] = 0.f; }
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                const float4 g = tile[idx0 + i * SX + j * SY + k];
-                const float gv[3] = {g.x, g.y, g.z};
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    zw[c] = fmaf(A.w[2][k], gv[c], zw[c]);
-                    ze[c] = fmaf(A.e[2][k], gv[c], ze[c]);
-                    zd[c] = fmaf(A.d[2][k], gv[c], zd[c]);
-                }
-            }
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                yww[i][c] = fmaf(A.w[1][j], zw[c], yww[i][c]);
-                yew[i][c] = fmaf(A.e[1][j], zw[c], yew[i][c]);
-                ydw[i][c] = fmaf(A.d[1][j], zw[c], ydw[i][c]);
-                ywe[i][c] = fmaf(A.w[1][j], ze[c], ywe[i][c]);
-                ywd[i][c] = fmaf(A.w[1][j], zd[c], ywd[i][c]);
-            }
-        }
+// A gather source: component c of stencil node (idx0 + i*SX + j*SY + k).
+struct Float4Nodes {  // float4 array, .xyz = v
+    const float4* p;
+    __device__ __forceinline__ float comp(int c, int t) const {
+        const float4 g = p[t];
+        return c == 0 ? g.x : (c == 1 ? g.y : g.z);
     }
+};
+struct SoANodes {     // three float arrays of stride `n` (bank-conflict friendly in LDS)
+    const float* p;
+    int n;
+    __device__ __forceinline__ float comp(int c, int t) const { return p[c * n + t]; }
+};
+
+template <int SX, int SY, typename Nodes>
+__device__ __forceinline__ void gather_tile(const Nodes& tile, int idx0, const AxisWeights& A,
+                                            float inv_dx, float v[3], float C[9], float G[9]) {
     const float c4 = 4.0f * inv_dx;
+    // one velocity component at a time keeps the live set small
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         float sv = 0.f, cx = 0.f, gx = 0.f, cy = 0.f, gy = 0.f, cz = 0.f, gz = 0.f;
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
-            sv = fmaf(A.w[0][i], yww[i][c], sv);
-            cx = fmaf(A.e[0][i], yww[i][c], cx);
-            gx = fmaf(A.d[0][i], yww[i][c], gx);
-            cy = fmaf(A.w[0][i], yew[i][c], cy);
-            gy = fmaf(A.w[0][i], ydw[i][c], gy);
-            cz = fmaf(A.w[0][i], ywe[i][c], cz);
-            gz = fmaf(A.w[0][i], ywd[i][c], gz);
+            float yww = 0.f, yew = 0.f, ydw = 0.f, ywe = 0.f, ywd = 0.f;
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                float zw = 0.f, ze = 0.f, zd = 0.f;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const float g = tile.comp(c, idx0 + i * SX + j * SY + k);
+                    zw = fmaf(A.w[2][k], g, zw);
+                    ze = fmaf(A.e[2][k], g, ze);
+                    zd = fmaf(A.d[2][k], g, zd);
+                }
+                yww = fmaf(A.w[1][j], zw, yww);
+                yew = fmaf(A.e[1][j], zw, yew);
+                ydw = fmaf(A.d[1][j], zw, ydw);
+                ywe = fmaf(A.w[1][j], ze, ywe);
+                ywd = fmaf(A.w[1][j], zd, ywd);
+            }
+            sv = fmaf(A.w[0][i], yww, sv);
+            cx = fmaf(A.e[0][i], yww, cx);
+            gx = fmaf(A.d[0][i], yww, gx);
+            cy = fmaf(A.w[0][i], yew, cy);
+            gy = fmaf(A.w[0][i], ydw, gy);
+            cz = fmaf(A.w[0][i], ywe, cz);
+            gz = fmaf(A.w[0][i], ywd, gz);
         }
         v[c] = sv;
         C[c * 3 + 0] = cx * c4;
@@ -113,26 +111,14 @@ __device__ __forceinline__ void gather_tile(const TileT* tile, int idx0, const A
 //   mv_a += w m (v_a + sum_b C_ab dpos_b) + dt sum_b (-vol tau_ab) grad_b w
 //   m    += w m
 // factored as mv_a(i,j,k) = P_a[i] w_y w_z + Q_a[j] w_x w_z + R_a[k] w_x w_y.
-// `emit(i, j, k, mvx, mvy, mvz, m)` receives each node's contribution.
+// `emit(c, i, j, k, val)` receives component c (0..2 momentum, 3 mass) of
+// node (i,j,k)'s contribution.
 template <typename Emit>
 __device__ __forceinline__ void p2g_contrib(const AxisWeights& A, float m, float vol, const float v[3],
                                             const float C[9], const float tau[9], float dx, float inv_dx,
                                             float dt, Emit&& emit) {
     const float ca = m * dx;                 // C dpos in physical units
     const float cb = -dt * vol * inv_dx;     // dt (-vol tau) grad w
-    float P[3][3], Q[3][3], R[3][3];         // [component][node]
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        const float ax = ca * C[a * 3 + 0], ay = ca * C[a * 3 + 1], az = ca * C[a * 3 + 2];
-        const float bx = cb * tau[a * 3 + 0], by = cb * tau[a * 3 + 1], bz = cb * tau[a * 3 + 2];
-        const float mv = m * v[a];
-#pragma unroll
-        for (int n = 0; n < 3; ++n) {
-            P[a][n] = fmaf(mv, A.w[0][n], fmaf(ax, A.e[0][n], bx * A.d[0][n]));
-            Q[a][n] = fmaf(ay, A.e[1][n], by * A.d[1][n]);
-            R[a][n] = fmaf(az, A.e[2][n], bz * A.d[2][n]);
-        }
-    }
     float wyz[3][3], wxz[3][3], wxy[3][3];
 #pragma unroll
     for (int p = 0; p < 3; ++p)
@@ -142,19 +128,34 @@ __device__ __forceinline__ void p2g_contrib(const AxisWeights& A, float m, float
             wxz[p][q] = A.w[0][p] * A.w[2][q];
             wxy[p][q] = A.w[0][p] * A.w[1][q];
         }
+    // mass, then one momentum component at a time (component index 3 = mass)
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
         const float mwx = m * A.w[0][i];
 #pragma unroll
         for (int j = 0; j < 3; ++j)
 #pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                float o[3];
+            for (int k = 0; k < 3; ++k) emit(3, i, j, k, mwx * wyz[j][k]);
+    }
 #pragma unroll
-                for (int a = 0; a < 3; ++a)
-                    o[a] = fmaf(P[a][i], wyz[j][k], fmaf(Q[a][j], wxz[i][k], R[a][k] * wxy[i][j]));
-                emit(i, j, k, o[0], o[1], o[2], mwx * wyz[j][k]);
-            }
+    for (int a = 0; a < 3; ++a) {
+        const float ax = ca * C[a * 3 + 0], ay = ca * C[a * 3 + 1], az = ca * C[a * 3 + 2];
+        const float bx = cb * tau[a * 3 + 0], by = cb * tau[a * 3 + 1], bz = cb * tau[a * 3 + 2];
+        const float mv = m * v[a];
+        float P[3], Q[3], R[3];
+#pragma unroll
+        for (int n = 0; n < 3; ++n) {
+            P[n] = fmaf(mv, A.w[0][n], fmaf(ax, A.e[0][n], bx * A.d[0][n]));
+            Q[n] = fmaf(ay, A.e[1][n], by * A.d[1][n]);
+            R[n] = fmaf(az, A.e[2][n], bz * A.d[2][n]);
+        }
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+#pragma unroll
+                for (int k = 0; k < 3; ++k)
+                    emit(a, i, j, k, fmaf(P[i], wyz[j][k], fmaf(Q[j], wxz[i][k], R[k] * wxy[i][j])));
     }
 }
 

@@ -305,12 +305,12 @@ class MPM_Simulator_HIP:
         idx = (b[0] * nb + b[1]) * nb + b[2]
         fullest = int(torch.bincount(idx, minlength=nb ** 3).max()) if n > 0 else 0
         cap = max(256, ((2 * fullest + 63) // 64) * 64)
-        ovf = max(4096, ((n // 16 + 255) // 256) * 256)
-        stride = nb ** 3 * cap + ovf
+        ovf = cap * max(1, -(-max(4096, n // 16) // cap))      # whole bins
+        stride = (nb ** 3 * cap + ovf) * L.NPLANES              # bin-major plane sets
         old = self._bins
-        if old is not None and old["nb"] == nb and old["cap"] >= cap and old["ovf"] >= ovf:
+        if old is not None and old["nb"] == nb and old["cap"] == cap and old["ovf"] == ovf:
             return old
-        planes = torch.zeros((2, L.NPLANES, stride), dtype=torch.float32, device=self.device)
+        planes = torch.zeros((2, stride), dtype=torch.float32, device=self.device)
         count = torch.zeros((2, nb ** 3 + 1), dtype=torch.int32, device=self.device)
         st = L.MpmBins()
         st.nb, st.bin_cap, st.overflow_cap, st.cur, st.epoch, st.stride = nb, cap, ovf, 0, 0, stride
