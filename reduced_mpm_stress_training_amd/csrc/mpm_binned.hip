@@ -43,6 +43,7 @@ constexpr int kTG = kBinEdge + 2;     // gather tile edge (nodes b0 .. b0+9)
 constexpr int kTS = kBinEdge + 4;     // scatter tile edge (nodes b0-1 .. b0+10)
 constexpr int kTG3 = kTG * kTG * kTG;
 constexpr int kTS3 = kTS * kTS * kTS;
+constexpr int kSortMax = 6144;        // bins up to this size are processed in bank order
 
 // Plane sets are bin-major: bin b (b >= n_bins: overflow run) is a block of
 // MPM_NPLANES planes of `cap` floats, so slot i of bin b, plane p, is at
@@ -244,8 +245,9 @@ __device__ __forceinline__ void g2p2g_particle(const SubstepArgs& a, bool act_sl
         const int lx = A.base[0] - b0[0], ly = A.base[1] - b0[1], lz = A.base[2] - b0[2];
         if (tile_vel != nullptr && lx >= 0 && ly >= 0 && lz >= 0 && lx < kBinEdge && ly < kBinEdge &&
             lz < kBinEdge) {
-            gather_tile<kTG * kTG, kTG>(SoANodes{tile_vel, kTG3}, (lx * kTG + ly) * kTG + lz, A, prm.inv_dx, v, Cm,
-                                        G);
+            // the velocity tile has the scatter tile's layout (nodes b0-1 .. b0+10)
+            gather_tile<kTS * kTS, kTS>(SoANodes{tile_vel, kTS3}, ((lx + 1) * kTS + ly + 1) * kTS + lz + 1, A,
+                                        prm.inv_dx, v, Cm, G);
         } else {
             // global gather: same contraction, nodes read from the grid
             const int ng = a.g.ng;
@@ -389,8 +391,16 @@ __global__ void __launch_bounds__(kWG)
 // Fused G2P (this substep) + P2G (next substep), one workgroup per bin plus
 // workgroups over the overflow run.
 // ---------------------------------------------------------------------------
+// LDS bank class of a stencil base node in the 12^3 tiles: t mod 32, t the
+// tile index of the base (origin b0-1).  Lanes of a wave are given particles
+// of distinct classes (see k_g2p2g), so that the 27-node gathers (4-byte
+// reads) and scatters (8-byte atomics) of a 32-lane half hit distinct banks.
+__device__ __forceinline__ int bank_class(const int base[3], const int b0[3]) {
+    return ((((base[0] - b0[0] + 1) * kTS + (base[1] - b0[1] + 1)) * kTS + (base[2] - b0[2] + 1)) & 31);
+}
+
 __global__ void __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(2, 2))) k_g2p2g(SubstepArgs a) {
-    __shared__ float s_vel[3 * kTG3];
+    __shared__ float s_vel[3 * kTS3];
     __shared__ double s_acc[4 * kTS3];
     __shared__ int s_stay;
     __shared__ int s_base;
@@ -414,24 +424,28 @@ __global__ void __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(2, 2))
     if (n_in == 0) return;
     const int nb = g.nb, ng = g.ng;
     const int b0[3] = {(bin / (nb * nb)) * kBinEdge, ((bin / nb) % nb) * kBinEdge, (bin % nb) * kBinEdge};
+    const int64_t off0 = (int64_t)bin * MPM_NPLANES * g.cap;
     if (tid == 0) {
         s_base = atomicAdd(&a.cnt_nxt[bin], n_in);   // the run stayers will fill
         s_stay = 0;
     }
-    for (int t = tid; t < kTG3; t += kWG) {
-        const int x = b0[0] + t / (kTG * kTG), y = b0[1] + (t / kTG) % kTG, z = b0[2] + t % kTG;
+    for (int t = tid; t < kTS3; t += kWG) {
+        const int x = b0[0] - 1 + t / (kTS * kTS), y = b0[1] - 1 + (t / kTS) % kTS, z = b0[2] - 1 + t % kTS;
         float4 val = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (x < ng && y < ng && z < ng) val = a.vel[((int64_t)x * ng + y) * ng + z];
+        if (x >= 0 && y >= 0 && z >= 0 && x < ng && y < ng && z < ng) val = a.vel[((int64_t)x * ng + y) * ng + z];
         s_vel[t] = val.x;
-        s_vel[kTG3 + t] = val.y;
-        s_vel[2 * kTG3 + t] = val.z;
+        s_vel[kTS3 + t] = val.y;
+        s_vel[2 * kTS3 + t] = val.z;
     }
     for (int t = tid; t < 4 * kTS3; t += kWG) s_acc[t] = 0.0;
-    const int64_t off0 = (int64_t)bin * MPM_NPLANES * g.cap;
+    __syncthreads();
+    // Particles are processed in storage order.  Storage is kept in bank order
+    // (k_bins_interleave after insertion; stayers are written back in
+    // processing order), so each 32-lane half mostly sees distinct bank
+    // classes of its base node.
+    const int stay_base = s_base;
     ParticleIn q;
     load_particle(a.src, off0 + tid, g.cap, tid < n_in, q);
-    __syncthreads();
-    const int stay_base = s_base;
     for (int it = 0; it < n_in; it += kWG) {
         const int i = it + tid;
         const ParticleIn cur = q;
@@ -508,6 +522,97 @@ __global__ void __launch_bounds__(kWG)
     if (!ovf) {
         __syncthreads();
         write_tile(s_acc, tiles, stamps, bin, epoch_out);
+    }
+}
+
+// Copies every bin of plane set `src` into plane set `dst` in bank order:
+// a counting sort by bank_class of the base node, then storage position q
+// holds sorted entry (q mod 32) * S + q / 32 (S = ceil(n / 32)), so that
+// consecutive runs of 32 particles span the 32 classes.  Holes are dropped.
+// Bins larger than kSortMax keep their order.
+__global__ void __launch_bounds__(kWG)
+    k_bins_interleave(const float* __restrict__ src, const int* __restrict__ cnt_src, float* __restrict__ dst,
+                      int* __restrict__ cnt_dst, BinGeom g, float inv_dx) {
+    __shared__ unsigned short s_order[kSortMax];
+    __shared__ unsigned short s_rank[kSortMax];
+    __shared__ unsigned char s_cls[kSortMax];
+    __shared__ short s_live[kSortMax];
+    __shared__ int s_ccnt[32];
+    __shared__ int s_coff[32];
+    __shared__ int s_n;
+    const int tid = threadIdx.x;
+    const int wg = blockIdx.x;
+    int64_t off0;
+    int n_in, bin;
+    if (wg >= g.n_bins) {  // overflow blocks are copied as they are
+        const int k = wg - g.n_bins;
+        const int n_ovf = min(cnt_src[g.n_bins], g.ovf_cap);
+        off0 = (int64_t)wg * MPM_NPLANES * g.cap;
+        n_in = min(max(n_ovf - k * g.cap, 0), g.cap);
+        for (int i = tid; i < n_in; i += kWG)
+            for (int pl = 0; pl < MPM_NPLANES; ++pl) dst[off0 + (int64_t)pl * g.cap + i] = src[off0 + (int64_t)pl * g.cap + i];
+        if (k == 0 && tid == 0) cnt_dst[g.n_bins] = n_ovf;
+        return;
+    }
+    bin = wg;
+    n_in = min(cnt_src[bin], g.cap);
+    off0 = (int64_t)bin * MPM_NPLANES * g.cap;
+    const int nb = g.nb;
+    const int b0[3] = {(bin / (nb * nb)) * kBinEdge, ((bin / nb) % nb) * kBinEdge, (bin % nb) * kBinEdge};
+    if (n_in > kSortMax) {
+        for (int i = tid; i < n_in; i += kWG)
+            for (int pl = 0; pl < MPM_NPLANES; ++pl) dst[off0 + (int64_t)pl * g.cap + i] = src[off0 + (int64_t)pl * g.cap + i];
+        if (tid == 0) cnt_dst[bin] = n_in;
+        return;
+    }
+    if (tid < 32) s_ccnt[tid] = 0;
+    if (tid == 0) s_n = 0;
+    __syncthreads();
+    for (int i = tid; i < n_in; i += kWG) {
+        const float* sp = src + off0 + i;
+        if (__float_as_int(sp[(int64_t)MPM_P_PID * g.cap]) < 0) { s_cls[i] = 255; continue; }  // hole
+        int base[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const float x = sp[(int64_t)(MPM_P_X0 + d) * g.cap] + sp[(int64_t)(MPM_P_U + d) * g.cap];
+            base[d] = min(max((int)(x * inv_dx - 0.5f), b0[d]), b0[d] + kBinEdge - 1);
+        }
+        const int c = bank_class(base, b0);
+        s_cls[i] = (unsigned char)c;
+        s_rank[i] = (unsigned short)atomicAdd(&s_ccnt[c], 1);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int acc = 0;
+        for (int c = 0; c < 32; ++c) { s_coff[c] = acc; acc += s_ccnt[c]; }
+        s_n = acc;
+    }
+    __syncthreads();
+    const int n = s_n;
+    for (int i = tid; i < n_in; i += kWG)
+        if (s_cls[i] != 255) s_order[s_coff[s_cls[i]] + s_rank[i]] = (unsigned short)i;
+    __syncthreads();
+    // storage position q <- sorted entry (q mod 32) * S + q / 32, skipping
+    // entries past n; s_live numbers the kept positions
+    const int S = (n + 31) >> 5;
+    for (int q = tid; q < 32 * S; q += kWG) {
+        const int p = (q & 31) * S + (q >> 5);
+        s_live[q < kSortMax ? q : 0] = (short)(p < n ? 1 : 0);
+    }
+    __syncthreads();
+    if (tid == 0) {  // exclusive scan of kept positions (n <= kSortMax)
+        int acc = 0;
+        for (int q = 0; q < 32 * S && q < kSortMax; ++q) { const int k = s_live[q]; s_live[q] = (short)(k ? acc : -1); acc += k; }
+        cnt_dst[bin] = acc;
+    }
+    __syncthreads();
+    for (int q = tid; q < 32 * S && q < kSortMax; q += kWG) {
+        const int dpos = s_live[q];
+        if (dpos < 0) continue;
+        const int p = (q & 31) * S + (q >> 5);
+        const int i = s_order[p];
+        for (int pl = 0; pl < MPM_NPLANES; ++pl)
+            dst[off0 + (int64_t)pl * g.cap + dpos] = src[off0 + (int64_t)pl * g.cap + i];
     }
 }
 
@@ -658,13 +763,21 @@ int mpm_bins_insert(const MpmParams* prm, const MpmState* flat, const MpmBins* b
     if (!workspace) return set_err(-10, "workspace is NULL");
     WsLayout w = ws_layout(prm, workspace);
     hipStream_t s = (hipStream_t)stream;
+    // insert into the spare plane set, then copy into `cur` in bank order
+    const int spare = 1 - bins->cur;
+    int* cnt_spare = bins->count + (int64_t)spare * (g.n_bins + 1);
     int* cnt = bins->count + (int64_t)bins->cur * (g.n_bins + 1);
+    hipLaunchKernelGGL(k_fill_int, dim3((g.n_bins + 1 + kWG - 1) / kWG), dim3(kWG), 0, s, cnt_spare, g.n_bins + 1, 0);
+    if ((rc = check_launch("k_fill_int"))) return rc;
     hipLaunchKernelGGL(k_fill_int, dim3((g.n_bins + 1 + kWG - 1) / kWG), dim3(kWG), 0, s, cnt, g.n_bins + 1, 0);
     if ((rc = check_launch("k_fill_int"))) return rc;
     if (flat->n == 0) return 0;
     hipLaunchKernelGGL(k_bins_insert, dim3((unsigned)((flat->n + kWG - 1) / kWG)), dim3(kWG), 0, s, flat->planes,
-                       flat->stride, (int)flat->n, prm->inv_dx, g, bins->planes[bins->cur], cnt, w.status);
-    return check_launch("k_bins_insert");
+                       flat->stride, (int)flat->n, prm->inv_dx, g, bins->planes[spare], cnt_spare, w.status);
+    if ((rc = check_launch("k_bins_insert"))) return rc;
+    hipLaunchKernelGGL(k_bins_interleave, dim3(g.n_bins + ovf_wgs(g)), dim3(kWG), 0, s, bins->planes[spare],
+                       cnt_spare, bins->planes[bins->cur], cnt, g, prm->inv_dx);
+    return check_launch("k_bins_interleave");
 }
 
 int mpm_bins_extract(const MpmParams* prm, const MpmBins* bins, const MpmState* flat, void* stream) {
