@@ -40,7 +40,7 @@
 extern "C" {
 #endif
 
-#define MPM_ABI_VERSION 2
+#define MPM_ABI_VERSION 3
 
 /* plane indices */
 #define MPM_P_X0 0
@@ -152,7 +152,8 @@ typedef struct MpmBins {
     int32_t reserved;
     int64_t stride;       /* floats per plane set >= (nb^3*bin_cap + overflow_cap) * MPM_NPLANES */
     float* planes[2];     /* two sets of MPM_NPLANES planes */
-    int32_t* count;       /* [2][nb^3 + 1] particles per bin; last = overflow run */
+    int32_t* count;       /* [2][2*nb^3 + 1]: per plane set, slots used per bin (nb^3),
+                             slots used in the overflow run (1), live particles per bin (nb^3) */
 } MpmBins;
 
 /* Workspace of the binned path (zeroed once before first use, 256-byte
@@ -195,6 +196,13 @@ int mpm_bins_g2p2g(const MpmParams* prm, MpmBins* bins, void* workspace, float d
  * binned state: mpm_bins_grid_update then mpm_bins_g2p2g. */
 int mpm_bins_substep(const MpmParams* prm, MpmBins* bins, const MpmCollider* colliders, int n_colliders,
                      void* workspace, float time, float dt, void* stream);
+
+/* n_substeps substeps like mpm_bins_run, each kernel bracketed by HIP events
+ * on `stream`; waits for each substep and returns the average duration of
+ * the grid update and of the fused kernel in milliseconds (benchmarking). */
+int mpm_bins_profile(const MpmParams* prm, MpmBins* bins, const MpmCollider* colliders, int n_colliders,
+                     void* workspace, double time, float dt, int n_substeps, void* stream, float* grid_ms,
+                     float* g2p2g_ms);
 
 /* n_substeps consecutive mpm_bins_substep calls; the solver time of substep
  * k is time + k*dt accumulated in double, as the Python solver accumulates

@@ -10,7 +10,7 @@ import os
 
 from ._build import LIB_PATH
 
-MPM_ABI_VERSION = 2
+MPM_ABI_VERSION = 3
 MAX_COLLIDERS = 16
 
 # plane indices of the particle state buffer
@@ -66,6 +66,9 @@ _SIGNATURES = {
     "mpm_bins_grid_update": (C.c_int, [C.POINTER(MpmParams), C.POINTER(MpmBins), C.POINTER(MpmCollider), C.c_int,
                                        C.c_void_p, C.c_float, C.c_float, C.c_void_p]),
     "mpm_bins_g2p2g": (C.c_int, [C.POINTER(MpmParams), C.POINTER(MpmBins), C.c_void_p, C.c_float, C.c_void_p]),
+    "mpm_bins_profile": (C.c_int, [C.POINTER(MpmParams), C.POINTER(MpmBins), C.POINTER(MpmCollider), C.c_int,
+                                   C.c_void_p, C.c_double, C.c_float, C.c_int, C.c_void_p,
+                                   C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "mpm_bins_run": (C.c_int, [C.POINTER(MpmParams), C.POINTER(MpmBins), C.POINTER(MpmCollider), C.c_int,
                                C.c_void_p, C.c_double, C.c_float, C.c_int, C.c_void_p]),
     "mpm_import": (C.c_int, [C.POINTER(MpmState), C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p]),

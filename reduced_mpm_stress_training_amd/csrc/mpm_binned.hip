@@ -154,7 +154,10 @@ __device__ __forceinline__ int64_t reserve_slot(bool act, int new_bin, int own_b
         base = __shfl(base, leader);
         pos = stay_base + base + __popcll(mask & ((1ull << lane) - 1ull));
     }
-    if (act && !stay) pos = atomicAdd(&a.cnt_nxt[new_bin], 1);
+    if (act && !stay) {
+        pos = atomicAdd(&a.cnt_nxt[new_bin], 1);
+        if (pos < a.g.cap) atomicAdd(&a.cnt_nxt[a.g.n_bins + 1 + new_bin], 1);  // live count
+    }
     if (!act) return -1;
     if (pos < a.g.cap) return (int64_t)new_bin * MPM_NPLANES * a.g.cap + pos;
     const int r = atomicAdd(&a.cnt_nxt[a.g.n_bins], 1);
@@ -333,7 +336,8 @@ __global__ void __launch_bounds__(kWG)
     const int nb = g.nb, ng = g.ng;
     const int bz = blk % nb, by = (blk / nb) % nb, bx = blk / (nb * nb);
     if (tid == 0 && cnt_nxt) {
-        cnt_nxt[blk] = 0;
+        cnt_nxt[blk] = 0;                 // slots
+        cnt_nxt[g.n_bins + 1 + blk] = 0;  // live particles
         if (blk == 0) cnt_nxt[g.n_bins] = 0;
     }
     if (tid == 0) s_any = 0;
@@ -425,8 +429,10 @@ __global__ void __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(2, 2))
     const int nb = g.nb, ng = g.ng;
     const int b0[3] = {(bin / (nb * nb)) * kBinEdge, ((bin / nb) % nb) * kBinEdge, (bin % nb) * kBinEdge};
     const int64_t off0 = (int64_t)bin * MPM_NPLANES * g.cap;
+    // the run stayers will fill: as many slots as the bin has live particles
+    const int n_live = min(a.cnt_cur[g.n_bins + 1 + bin], n_in);
     if (tid == 0) {
-        s_base = atomicAdd(&a.cnt_nxt[bin], n_in);   // the run stayers will fill
+        s_base = atomicAdd(&a.cnt_nxt[bin], n_live);
         s_stay = 0;
     }
     for (int t = tid; t < kTS3; t += kWG) {
@@ -454,9 +460,10 @@ __global__ void __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(2, 2))
     }
     __syncthreads();
     // reserved slots nobody stayed in become holes (pid -1)
-    const int hole_lo = stay_base + s_stay, hole_hi = min(stay_base + n_in, g.cap);
+    const int hole_lo = stay_base + s_stay, hole_hi = min(stay_base + n_live, g.cap);
     for (int h = hole_lo + tid; h < hole_hi; h += kWG)
         a.dst[off0 + (int64_t)MPM_P_PID * g.cap + h] = __int_as_float(-1);
+    if (tid == 0) atomicAdd(&a.cnt_nxt[g.n_bins + 1 + bin], max(0, min(s_stay, g.cap - stay_base)));
     write_tile(s_acc, a.tiles, a.stamps, bin, a.epoch_out);
 }
 
@@ -562,7 +569,7 @@ __global__ void __launch_bounds__(kWG)
     if (n_in > kSortMax) {
         for (int i = tid; i < n_in; i += kWG)
             for (int pl = 0; pl < MPM_NPLANES; ++pl) dst[off0 + (int64_t)pl * g.cap + i] = src[off0 + (int64_t)pl * g.cap + i];
-        if (tid == 0) cnt_dst[bin] = n_in;
+        if (tid == 0) { cnt_dst[bin] = n_in; cnt_dst[g.n_bins + 1 + bin] = n_in; }
         return;
     }
     if (tid < 32) s_ccnt[tid] = 0;
@@ -604,6 +611,7 @@ __global__ void __launch_bounds__(kWG)
         int acc = 0;
         for (int q = 0; q < 32 * S && q < kSortMax; ++q) { const int k = s_live[q]; s_live[q] = (short)(k ? acc : -1); acc += k; }
         cnt_dst[bin] = acc;
+        cnt_dst[g.n_bins + 1 + bin] = acc;
     }
     __syncthreads();
     for (int q = tid; q < 32 * S && q < kSortMax; q += kWG) {
@@ -765,11 +773,12 @@ int mpm_bins_insert(const MpmParams* prm, const MpmState* flat, const MpmBins* b
     hipStream_t s = (hipStream_t)stream;
     // insert into the spare plane set, then copy into `cur` in bank order
     const int spare = 1 - bins->cur;
-    int* cnt_spare = bins->count + (int64_t)spare * (g.n_bins + 1);
-    int* cnt = bins->count + (int64_t)bins->cur * (g.n_bins + 1);
-    hipLaunchKernelGGL(k_fill_int, dim3((g.n_bins + 1 + kWG - 1) / kWG), dim3(kWG), 0, s, cnt_spare, g.n_bins + 1, 0);
+    int* cnt_spare = bins->count + (int64_t)spare * (2 * g.n_bins + 1);
+    int* cnt = bins->count + (int64_t)bins->cur * (2 * g.n_bins + 1);
+    const int cb = 2 * g.n_bins + 1;
+    hipLaunchKernelGGL(k_fill_int, dim3((cb + kWG - 1) / kWG), dim3(kWG), 0, s, cnt_spare, cb, 0);
     if ((rc = check_launch("k_fill_int"))) return rc;
-    hipLaunchKernelGGL(k_fill_int, dim3((g.n_bins + 1 + kWG - 1) / kWG), dim3(kWG), 0, s, cnt, g.n_bins + 1, 0);
+    hipLaunchKernelGGL(k_fill_int, dim3((cb + kWG - 1) / kWG), dim3(kWG), 0, s, cnt, cb, 0);
     if ((rc = check_launch("k_fill_int"))) return rc;
     if (flat->n == 0) return 0;
     hipLaunchKernelGGL(k_bins_insert, dim3((unsigned)((flat->n + kWG - 1) / kWG)), dim3(kWG), 0, s, flat->planes,
@@ -785,7 +794,7 @@ int mpm_bins_extract(const MpmParams* prm, const MpmBins* bins, const MpmState* 
     int rc = check_bins(prm, bins, &g);
     if (rc) return rc;
     if (!flat || !flat->planes || flat->n < 0 || flat->stride < flat->n) return set_err(-25, "flat state invalid");
-    const int* cnt = bins->count + (int64_t)bins->cur * (g.n_bins + 1);
+    const int* cnt = bins->count + (int64_t)bins->cur * (2 * g.n_bins + 1);
     hipLaunchKernelGGL(k_bins_extract, dim3(g.n_bins + ovf_wgs(g)), dim3(kWG), 0, (hipStream_t)stream,
                        bins->planes[bins->cur], cnt, g, flat->planes, flat->stride, (int)flat->n);
     return check_launch("k_bins_extract");
@@ -814,7 +823,7 @@ int mpm_bins_p2g(const MpmParams* prm, MpmBins* bins, void* workspace, float dt,
     if ((rc = check_params(prm))) return rc;
     if (!workspace) return set_err(-10, "workspace is NULL");
     WsLayout w = ws_layout(prm, workspace);
-    const int* cnt = bins->count + (int64_t)bins->cur * (g.n_bins + 1);
+    const int* cnt = bins->count + (int64_t)bins->cur * (2 * g.n_bins + 1);
     const int epoch_out = bins->epoch + 1;
     hipLaunchKernelGGL(k_p2g_bins, dim3(g.n_bins + ovf_wgs(g)), dim3(kWG), 0, (hipStream_t)stream, *prm, g, dt,
                        bins->planes[bins->cur], cnt, (float*)w.acc, w.gflag, w.tiles, w.stamps, epoch_out);
@@ -833,7 +842,7 @@ int mpm_bins_grid_update(const MpmParams* prm, const MpmBins* bins, const MpmCol
     ColliderSet cs;
     if ((rc = make_collider_set(colliders, n_colliders, &cs))) return rc;
     WsLayout w = ws_layout(prm, workspace);
-    int* cnt_nxt = bins->count + (int64_t)(1 - bins->cur) * (g.n_bins + 1);
+    int* cnt_nxt = bins->count + (int64_t)(1 - bins->cur) * (2 * g.n_bins + 1);
     hipLaunchKernelGGL(k_grid_bins, dim3(g.n_bins), dim3(kWG), 0, (hipStream_t)stream, w.acc, w.vel, w.gflag,
                        (const int*)w.stamps, (const float4*)w.tiles, cnt_nxt, g, *prm, cs, time, dt, bins->epoch, 0);
     return check_launch("k_grid_bins");
@@ -854,8 +863,8 @@ int mpm_bins_g2p2g(const MpmParams* prm, MpmBins* bins, void* workspace, float d
     a.epoch_out = bins->epoch + 1;
     a.src = bins->planes[cur];
     a.dst = bins->planes[nxt];
-    a.cnt_cur = bins->count + (int64_t)cur * (g.n_bins + 1);
-    a.cnt_nxt = bins->count + (int64_t)nxt * (g.n_bins + 1);
+    a.cnt_cur = bins->count + (int64_t)cur * (2 * g.n_bins + 1);
+    a.cnt_nxt = bins->count + (int64_t)nxt * (2 * g.n_bins + 1);
     a.vel = w.vel;
     a.acc = (float*)w.acc;
     a.gflag = w.gflag;
@@ -874,6 +883,38 @@ int mpm_bins_substep(const MpmParams* prm, MpmBins* bins, const MpmCollider* col
     int rc = mpm_bins_grid_update(prm, bins, colliders, n_colliders, workspace, time, dt, stream);
     if (rc) return rc;
     return mpm_bins_g2p2g(prm, bins, workspace, dt, stream);
+}
+
+int mpm_bins_profile(const MpmParams* prm, MpmBins* bins, const MpmCollider* colliders, int n_colliders,
+                     void* workspace, double time, float dt, int n_substeps, void* stream, float* grid_ms,
+                     float* g2p2g_ms) {
+    if (!bins || !grid_ms || !g2p2g_ms) return set_err(-20, "profile: NULL pointer");
+    if (n_substeps < 1) return set_err(-26, "n_substeps < 1");
+    hipStream_t s = (hipStream_t)stream;
+    hipEvent_t ev[3];
+    for (int k = 0; k < 3; ++k)
+        if (hipEventCreate(&ev[k]) != hipSuccess) return set_err(-27, "hipEventCreate failed");
+    double t = time;
+    float acc_grid = 0.f, acc_g2p2g = 0.f;
+    int rc = 0;
+    for (int k = 0; k < n_substeps && rc == 0; ++k) {
+        (void)hipEventRecord(ev[0], s);
+        rc = mpm_bins_grid_update(prm, bins, colliders, n_colliders, workspace, (float)t, dt, stream);
+        (void)hipEventRecord(ev[1], s);
+        if (rc == 0) rc = mpm_bins_g2p2g(prm, bins, workspace, dt, stream);
+        (void)hipEventRecord(ev[2], s);
+        (void)hipEventSynchronize(ev[2]);
+        float a = 0.f, b = 0.f;
+        (void)hipEventElapsedTime(&a, ev[0], ev[1]);
+        (void)hipEventElapsedTime(&b, ev[1], ev[2]);
+        acc_grid += a;
+        acc_g2p2g += b;
+        t = t + (double)dt;
+    }
+    for (int k = 0; k < 3; ++k) (void)hipEventDestroy(ev[k]);
+    *grid_ms = acc_grid / n_substeps;
+    *g2p2g_ms = acc_g2p2g / n_substeps;
+    return rc;
 }
 
 int mpm_bins_run(const MpmParams* prm, MpmBins* bins, const MpmCollider* colliders, int n_colliders,

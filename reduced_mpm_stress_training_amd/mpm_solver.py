@@ -28,6 +28,7 @@ from __future__ import annotations
 import math
 import os
 import types
+from ctypes import c_float as C_float
 
 import numpy as np
 import torch
@@ -311,7 +312,7 @@ class MPM_Simulator_HIP:
         if old is not None and old["nb"] == nb and old["cap"] == cap and old["ovf"] == ovf:
             return old
         planes = torch.zeros((2, stride), dtype=torch.float32, device=self.device)
-        count = torch.zeros((2, nb ** 3 + 1), dtype=torch.int32, device=self.device)
+        count = torch.zeros((2, 2 * nb ** 3 + 1), dtype=torch.int32, device=self.device)
         st = L.MpmBins()
         st.nb, st.bin_cap, st.overflow_cap, st.cur, st.epoch, st.stride = nb, cap, ovf, 0, 0, stride
         st.planes[0] = planes[0].data_ptr()
@@ -338,7 +339,7 @@ class MPM_Simulator_HIP:
         st = self._bins_status()
         if int(st[0]) > 0:
             raise RuntimeError(f"{int(st[0])} particles were dropped: bins and overflow run full")
-        cnt = self._bins["count"][self._bins["struct"].cur, -1].item()
+        cnt = self._bins["count"][self._bins["struct"].cur, self._bins["nb"] ** 3].item()
         if cnt > self._bins["ovf"] // 2:
             # re-bin with larger bins at the next substep
             self._ensure_flat()
@@ -384,6 +385,23 @@ class MPM_Simulator_HIP:
         self._since_check += int(n_substeps)
         if self._since_check >= 256:
             self._check_bins_health()
+
+    def profile_kernels(self, n_substeps, dt):
+        """Runs n_substeps on the binned path with HIP events around each
+        kernel (waits after every substep); returns the average
+        (grid_update_ms, g2p2g_ms)."""
+        lib = L.load()
+        self._bins_run(0, float(dt))       # bins the state and computes the pending P2G
+        B = self._bins
+        cols, ncol = self._colliders()
+        gm, km = C_float(), C_float()
+        L.check(lib.mpm_bins_profile(self._params(), B["struct"], cols, ncol, self._bins_ws_ptr(), float(self.time),
+                                     float(dt), int(n_substeps), _stream_handle(self.device), gm, km),
+                "mpm_bins_profile")
+        for _ in range(int(n_substeps)):
+            self.time = self.time + float(dt)
+        self._flat_valid = False
+        return gm.value, km.value
 
     def _sync_flat(self):
         """Make the flat planes mirror the binned state (no change of owner)."""
