@@ -9,7 +9,8 @@
 //   k_grid_bins   per 8^3 grid block: sums the P2G tiles of the (up to 27)
 //                 overlapping bins and the spill grid, then
 //                 grid_normalization_and_gravity (mpm_utils.py:229-237) and
-//                 the colliders (mpm_solver_warp.py:429-449, 505-523).
+//                 the colliders (mpm_solver_warp.py:429-449, 505-523).  Its
+//                 first workgroup schedules the bins of k_g2p2g.
 //   k_g2p2g       per bin: G2P (mpm_utils.py:241-306) from an LDS tile of
 //                 grid velocities, the sand return map and stress in
 //                 registers, update_u_from_x, and the P2G of the next
@@ -67,12 +68,53 @@ __device__ __forceinline__ bool stencil_ok(const int b[3], int ng) {
 }
 
 // workgroup -> bin.  Workgroups are dealt to the 8 XCDs round-robin, so the
-// identity mapping spreads every row of bins (z fastest) over all XCDs.
-// Mapping contiguous slabs of bins to one XCD would leave XCDs idle for
-// scenes that fill only part of the domain (a sand column fills the middle).
+// identity mapping (used by k_p2g_bins and when no schedule is current).
 __device__ __forceinline__ int xcd_bin(int wg, int n_bins) {
     (void)n_bins;
     return wg;
+}
+
+// Workgroup -> bin schedule of k_g2p2g.  The command processor hands
+// workgroup w to XCD w mod 8, and each XCD starts its workgroups in order on
+// whichever CU frees up first.  Listing the non-empty bins by decreasing
+// work (longest processing time first) therefore gives every XCD every 8th
+// bin of a sorted list -- an even share -- and fills each XCD's CUs greedily
+// from the largest bin down.  Mapping workgroups to bins by index instead
+// ties XCDs to rows of bins (bin mod 8 = z row mod 8), and a settled sand
+// pile a few bins high leaves several XCDs idle.
+// order[0 .. n_bins) = bins, -1 past the last non-empty one;
+// order[n_bins] = the epoch of the bin counts it was made from.
+constexpr int kSchedKeys = 64;
+
+__device__ void schedule_bins(const int* __restrict__ cnt_cur, int n_bins, int cap, int epoch,
+                              int* __restrict__ order) {
+    __shared__ int s_hist[kSchedKeys];
+    __shared__ int s_cursor[kSchedKeys];
+    __shared__ int s_total;
+    const int tid = threadIdx.x;
+    if (tid < kSchedKeys) s_hist[tid] = 0;
+    __syncthreads();
+    // key = loop iterations of the bin's workgroup (its cost), capped
+    for (int b = tid; b < n_bins; b += kWG) {
+        const int n = min(cnt_cur[b], cap);
+        if (n > 0) atomicAdd(&s_hist[min((n + kWG - 1) / kWG, kSchedKeys - 1)], 1);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int pos = 0;
+        for (int k = kSchedKeys - 1; k >= 1; --k) {
+            s_cursor[k] = pos;
+            pos += s_hist[k];
+        }
+        s_total = pos;
+    }
+    __syncthreads();
+    for (int b = tid; b < n_bins; b += kWG) {
+        const int n = min(cnt_cur[b], cap);
+        if (n > 0) order[atomicAdd(&s_cursor[min((n + kWG - 1) / kWG, kSchedKeys - 1)], 1)] = b;
+    }
+    for (int p = s_total + tid; p < n_bins; p += kWG) order[p] = -1;
+    if (tid == 0) order[n_bins] = epoch;
 }
 
 __device__ __forceinline__ void mark_block(int* __restrict__ gflag, int nb, int x, int y, int z) {
@@ -133,6 +175,8 @@ struct SubstepArgs {
     int* __restrict__ status;
     float4* __restrict__ tiles;
     int* __restrict__ stamps;
+    const int* __restrict__ order;  // schedule (schedule_bins)
+    int sched_epoch;                // epoch the schedule must carry to be used
 };
 
 // Destination slot of every active lane.  A bin's workgroup reserves
@@ -323,15 +367,22 @@ __device__ __forceinline__ void g2p2g_particle(const SubstepArgs& a, bool act_sl
 // bins (x+1)>>3 and, when x+1-8((x+1)>>3) <= 2, the bin before) and the
 // spill grid, sums in FP64, then normalization, gravity and colliders.
 // Also resets the next plane set's bin counters.  clear_only: only drains
-// the spill grid.
+// the spill grid.  Workgroup 0 builds the schedule of k_g2p2g from the
+// current bin counts (when `order` is given); workgroup 1 + b updates grid
+// block b.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kWG)
     k_grid_bins(float4* __restrict__ acc, float4* __restrict__ vel, int* __restrict__ gflag,
                 const int* __restrict__ stamps, const float4* __restrict__ tiles, int* __restrict__ cnt_nxt,
-                BinGeom g, MpmParams prm, ColliderSet cs, float time, float dt, int epoch, int clear_only) {
+                const int* __restrict__ cnt_cur, int* __restrict__ order, BinGeom g, MpmParams prm,
+                ColliderSet cs, float time, float dt, int epoch, int clear_only) {
     __shared__ int s_live[27];
     __shared__ int s_any;
-    const int blk = blockIdx.x;
+    if (blockIdx.x == 0) {
+        if (order) schedule_bins(cnt_cur, g.n_bins, g.cap, epoch, order);
+        return;
+    }
+    const int blk = blockIdx.x - 1;
     const int tid = threadIdx.x;
     const int nb = g.nb, ng = g.ng;
     const int bz = blk % nb, by = (blk / nb) % nb, bx = blk / (nb * nb);
@@ -423,7 +474,8 @@ __global__ void __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(2, 2))
         g2p2g_particle(a, i < n_ovf, off, g.n_bins, -1, nullptr, b0, nullptr, nullptr, q);
         return;
     }
-    const int bin = xcd_bin(wg, g.n_bins);
+    const int bin = (a.order && a.order[g.n_bins] == a.sched_epoch) ? a.order[wg] : xcd_bin(wg, g.n_bins);
+    if (bin < 0) return;
     const int n_in = min(a.cnt_cur[bin], g.cap);
     if (n_in == 0) return;
     const int nb = g.nb, ng = g.ng;
@@ -715,6 +767,7 @@ struct WsLayout {
     int* gflag;      // spill flags per grid block
     int* status;     // 64 counters
     int* stamps;     // epoch of each bin's tile
+    int* order;      // k_g2p2g schedule: nb^3 bins + its epoch
     float4* tiles;   // nb^3 tiles of kTS^3 nodes
     int64_t bytes;
 };
@@ -740,6 +793,8 @@ WsLayout ws_layout(const MpmParams* prm, void* base) {
     off += 256;
     w.stamps = (int*)(p + off);
     off += align256(nbins * 4);
+    w.order = (int*)(p + off);
+    off += align256((nbins + 1) * 4);
     w.tiles = (float4*)(p + off);
     off += align256(nbins * kTS3 * 16);
     w.bytes = off;
@@ -780,6 +835,9 @@ int mpm_bins_insert(const MpmParams* prm, const MpmState* flat, const MpmBins* b
     if ((rc = check_launch("k_fill_int"))) return rc;
     hipLaunchKernelGGL(k_fill_int, dim3((cb + kWG - 1) / kWG), dim3(kWG), 0, s, cnt, cb, 0);
     if ((rc = check_launch("k_fill_int"))) return rc;
+    // new counts: the current k_g2p2g schedule no longer applies
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(kWG), 0, s, w.order + g.n_bins, 1, (int)0x80000000);
+    if ((rc = check_launch("k_fill_int"))) return rc;
     if (flat->n == 0) return 0;
     hipLaunchKernelGGL(k_bins_insert, dim3((unsigned)((flat->n + kWG - 1) / kWG)), dim3(kWG), 0, s, flat->planes,
                        flat->stride, (int)flat->n, prm->inv_dx, g, bins->planes[spare], cnt_spare, w.status);
@@ -808,9 +866,9 @@ int mpm_bins_clear_grid(const MpmParams* prm, MpmBins* bins, void* workspace, vo
     WsLayout w = ws_layout(prm, workspace);
     ColliderSet cs;
     make_collider_set(nullptr, 0, &cs);
-    hipLaunchKernelGGL(k_grid_bins, dim3(g.n_bins), dim3(kWG), 0, (hipStream_t)stream, w.acc, w.vel, w.gflag,
-                       (const int*)w.stamps, (const float4*)w.tiles, (int*)nullptr, g, *prm, cs, 0.f, 0.f,
-                       bins->epoch, 1);
+    hipLaunchKernelGGL(k_grid_bins, dim3(g.n_bins + 1), dim3(kWG), 0, (hipStream_t)stream, w.acc, w.vel, w.gflag,
+                       (const int*)w.stamps, (const float4*)w.tiles, (int*)nullptr, (const int*)nullptr,
+                       (int*)nullptr, g, *prm, cs, 0.f, 0.f, bins->epoch, 1);
     if ((rc = check_launch("k_grid_bins(clear)"))) return rc;
     bins->epoch += 1;  // the pending tiles are no longer current
     return 0;
@@ -843,8 +901,10 @@ int mpm_bins_grid_update(const MpmParams* prm, const MpmBins* bins, const MpmCol
     if ((rc = make_collider_set(colliders, n_colliders, &cs))) return rc;
     WsLayout w = ws_layout(prm, workspace);
     int* cnt_nxt = bins->count + (int64_t)(1 - bins->cur) * (2 * g.n_bins + 1);
-    hipLaunchKernelGGL(k_grid_bins, dim3(g.n_bins), dim3(kWG), 0, (hipStream_t)stream, w.acc, w.vel, w.gflag,
-                       (const int*)w.stamps, (const float4*)w.tiles, cnt_nxt, g, *prm, cs, time, dt, bins->epoch, 0);
+    const int* cnt_cur = bins->count + (int64_t)bins->cur * (2 * g.n_bins + 1);
+    hipLaunchKernelGGL(k_grid_bins, dim3(g.n_bins + 1), dim3(kWG), 0, (hipStream_t)stream, w.acc, w.vel, w.gflag,
+                       (const int*)w.stamps, (const float4*)w.tiles, cnt_nxt, cnt_cur, w.order, g, *prm, cs,
+                       time, dt, bins->epoch, 0);
     return check_launch("k_grid_bins");
 }
 
@@ -871,6 +931,8 @@ int mpm_bins_g2p2g(const MpmParams* prm, MpmBins* bins, void* workspace, float d
     a.status = w.status;
     a.tiles = w.tiles;
     a.stamps = w.stamps;
+    a.order = w.order;
+    a.sched_epoch = bins->epoch;
     hipLaunchKernelGGL(k_g2p2g, dim3(g.n_bins + ovf_wgs(g)), dim3(kWG), 0, (hipStream_t)stream, a);
     if ((rc = check_launch("k_g2p2g"))) return rc;
     bins->epoch = a.epoch_out;
