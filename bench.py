@@ -128,9 +128,9 @@ def main():
         sim.stage_timer = None
     else:
         # events recorded by the library itself between its own launches
-        names = ["grid_update", "g2p2g"]
-        gm, km = sim.profile_kernels(args.stage_steps, dt)
-        dur = {"grid_update": gm, "g2p2g": km}
+        names = ["grid_update", "g2p2g", "sort"]
+        gm, km, sm = sim.profile_kernels(args.stage_steps, dt)
+        dur = {"grid_update": gm, "g2p2g": km, "sort": sm}
     dom = max((k for k in names if k in ALG_BYTES), key=lambda k: dur[k])
     achieved = ALG_BYTES[dom] * n / (dur[dom] / 1e3) / 1e9
     step_gbs = ALG_BYTES["step"] * n / (ms_step / 1e3) / 1e9

@@ -40,7 +40,7 @@
 extern "C" {
 #endif
 
-#define MPM_ABI_VERSION 3
+#define MPM_ABI_VERSION 4
 
 /* plane indices */
 #define MPM_P_X0 0
@@ -141,15 +141,17 @@ int mpm_step(const MpmParams* prm, const MpmState* st, const MpmCollider* collid
  * G2P of this substep and P2G of the next.  The P2G lands in per-bin tiles
  * of the workspace stamped with `epoch`, so between calls the workspace
  * carries the pending P2G; mpm_bins_p2g computes it for a freshly inserted
- * state.  Entry points that advance the state update `cur` and `epoch` in
- * place. */
+ * state.  Entry points that advance the state update `cur`, `epoch` and
+ * `since_sort` in place.  Every 32 substeps mpm_bins_substep re-sorts the
+ * bins (mpm_bins_sort): the order of particles inside a bin decides how
+ * many LDS bank conflicts the fused kernel's gathers and scatters see. */
 typedef struct MpmBins {
     int32_t nb;           /* bins per axis = ceil(n_grid / 8) */
     int32_t bin_cap;      /* slots per bin */
     int32_t overflow_cap; /* slots of the overflow run, a multiple of bin_cap */
     int32_t cur;          /* plane set holding the current state (0 or 1) */
     int32_t epoch;        /* stamp of the pending P2G tiles; start at 0 */
-    int32_t reserved;
+    int32_t since_sort;   /* substeps since the last bank-order sort; start at 0 */
     int64_t stride;       /* floats per plane set >= (nb^3*bin_cap + overflow_cap) * MPM_NPLANES */
     float* planes[2];     /* two sets of MPM_NPLANES planes */
     int32_t* count;       /* [2][2*nb^3 + 1]: per plane set, slots used per bin (nb^3),
@@ -192,17 +194,26 @@ int mpm_bins_grid_update(const MpmParams* prm, const MpmBins* bins, const MpmCol
                          int n_colliders, void* workspace, float time, float dt, void* stream);
 int mpm_bins_g2p2g(const MpmParams* prm, MpmBins* bins, void* workspace, float dt, void* stream);
 
+/* Copies plane set cur into 1-cur in bank order (each bin counting-sorted
+ * by the LDS bank class of its particles' stencil base, holes dropped),
+ * zeroes the counters of the old set, then sets cur = 1-cur and
+ * since_sort = 0.  No effect on the physics; may sit between
+ * mpm_bins_grid_update and mpm_bins_g2p2g. */
+int mpm_bins_sort(const MpmParams* prm, MpmBins* bins, void* workspace, void* stream);
+
 /* One substep, MPM_Simulator_WARP.p2g2p (mpm_solver_warp.py:156-203), on the
- * binned state: mpm_bins_grid_update then mpm_bins_g2p2g. */
+ * binned state: mpm_bins_grid_update, mpm_bins_sort when since_sort reaches
+ * 32, then mpm_bins_g2p2g. */
 int mpm_bins_substep(const MpmParams* prm, MpmBins* bins, const MpmCollider* colliders, int n_colliders,
                      void* workspace, float time, float dt, void* stream);
 
 /* n_substeps substeps like mpm_bins_run, each kernel bracketed by HIP events
- * on `stream`; waits for each substep and returns the average duration of
- * the grid update and of the fused kernel in milliseconds (benchmarking). */
+ * on `stream`; waits for each substep and returns the average duration per
+ * substep of the grid update, of the fused kernel and of the periodic sort
+ * in milliseconds (benchmarking). */
 int mpm_bins_profile(const MpmParams* prm, MpmBins* bins, const MpmCollider* colliders, int n_colliders,
                      void* workspace, double time, float dt, int n_substeps, void* stream, float* grid_ms,
-                     float* g2p2g_ms);
+                     float* g2p2g_ms, float* sort_ms);
 
 /* n_substeps consecutive mpm_bins_substep calls; the solver time of substep
  * k is time + k*dt accumulated in double, as the Python solver accumulates

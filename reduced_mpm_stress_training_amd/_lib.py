@@ -10,7 +10,7 @@ import os
 
 from ._build import LIB_PATH
 
-MPM_ABI_VERSION = 3
+MPM_ABI_VERSION = 4
 MAX_COLLIDERS = 16
 
 # plane indices of the particle state buffer
@@ -38,7 +38,7 @@ class MpmState(C.Structure):
 
 class MpmBins(C.Structure):
     _fields_ = [("nb", C.c_int32), ("bin_cap", C.c_int32), ("overflow_cap", C.c_int32), ("cur", C.c_int32),
-                ("epoch", C.c_int32), ("reserved", C.c_int32),
+                ("epoch", C.c_int32), ("since_sort", C.c_int32),
                 ("stride", C.c_int64), ("planes", C.c_void_p * 2), ("count", C.c_void_p)]
 
 
@@ -68,7 +68,8 @@ _SIGNATURES = {
     "mpm_bins_g2p2g": (C.c_int, [C.POINTER(MpmParams), C.POINTER(MpmBins), C.c_void_p, C.c_float, C.c_void_p]),
     "mpm_bins_profile": (C.c_int, [C.POINTER(MpmParams), C.POINTER(MpmBins), C.POINTER(MpmCollider), C.c_int,
                                    C.c_void_p, C.c_double, C.c_float, C.c_int, C.c_void_p,
-                                   C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+                                   C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "mpm_bins_sort": (C.c_int, [C.POINTER(MpmParams), C.POINTER(MpmBins), C.c_void_p, C.c_void_p]),
     "mpm_bins_run": (C.c_int, [C.POINTER(MpmParams), C.POINTER(MpmBins), C.POINTER(MpmCollider), C.c_int,
                                C.c_void_p, C.c_double, C.c_float, C.c_int, C.c_void_p]),
     "mpm_import": (C.c_int, [C.POINTER(MpmState), C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p]),

@@ -45,6 +45,8 @@ constexpr int kTS = kBinEdge + 4;     // scatter tile edge (nodes b0-1 .. b0+10)
 constexpr int kTG3 = kTG * kTG * kTG;
 constexpr int kTS3 = kTS * kTS * kTS;
 constexpr int kSortMax = 6144;        // bins up to this size are processed in bank order
+constexpr int kSortPeriod = 32;       // substeps between bank-order sorts of the bins
+constexpr int kStageFloats = 16384;   // LDS staging of the sort's copy (64 KB)
 
 // Plane sets are bin-major: bin b (b >= n_bins: overflow run) is a block of
 // MPM_NPLANES planes of `cap` floats, so slot i of bin b, plane p, is at
@@ -179,12 +181,37 @@ struct SubstepArgs {
     int sched_epoch;                // epoch the schedule must carry to be used
 };
 
+// Bank order.  Slots q of a run of n particles are filled with the
+// particles' counting sort by bank class (sorted index k) as q = k * mul mod n,
+// mul the inverse of s ~ n/32 modulo n: slots q and q+1 then hold sorted
+// entries s apart, about one class apart, so any 32 consecutive slots span
+// nearly all 32 classes.
+__device__ __forceinline__ int gcd_int(int a, int b) {
+    while (b) { const int t = a % b; a = b; b = t; }
+    return a;
+}
+
+// mul = s^-1 mod n, s the smallest integer >= ceil(n/32) coprime to n
+__device__ __forceinline__ int interleave_mul(int n) {
+    if (n <= 1) return 0;
+    int st = max(1, (n + 31) >> 5);
+    while (gcd_int(st, n) != 1) ++st;
+    int r0 = n, r1 = st, t0 = 0, t1 = 1;  // extended Euclid on (n, st)
+    while (r1) {
+        const int qq = r0 / r1;
+        int t = r0 - qq * r1; r0 = r1; r1 = t;
+        t = t0 - qq * t1; t0 = t1; t1 = t;
+    }
+    return t0 < 0 ? t0 + n : t0;
+}
+
 // Destination slot of every active lane.  A bin's workgroup reserves
-// n_in slots of its own bin in the next plane set with one atomic before its
+// n_live slots of its own bin in the next plane set with one atomic before its
 // loop (`stay_base`, -1 on the overflow path); lanes that stay in the bin
-// take consecutive slots of that run through an LDS counter.  Lanes that move
-// take one atomic each on their new bin.  Returns the float offset of the
-// slot (slot_off), or -1 when both the bin and the overflow run are full.
+// take consecutive slots of that run through an LDS counter, so stayers keep
+// their order.  Lanes that move take one atomic each on their new bin.
+// Returns the float offset of the slot (slot_off), or -1 when both the bin
+// and the overflow run are full.
 __device__ __forceinline__ int64_t reserve_slot(bool act, int new_bin, int own_bin, int stay_base,
                                                 int* s_stay, const SubstepArgs& a) {
     const bool stay = act && new_bin == own_bin && stay_base >= 0;
@@ -584,48 +611,49 @@ __global__ void __launch_bounds__(kWG)
     }
 }
 
-// Copies every bin of plane set `src` into plane set `dst` in bank order:
-// a counting sort by bank_class of the base node, then storage position q
-// holds sorted entry (q mod 32) * S + q / 32 (S = ceil(n / 32)), so that
-// consecutive runs of 32 particles span the 32 classes.  Holes are dropped.
-// Bins larger than kSortMax keep their order.
-__global__ void __launch_bounds__(kWG)
-    k_bins_interleave(const float* __restrict__ src, const int* __restrict__ cnt_src, float* __restrict__ dst,
-                      int* __restrict__ cnt_dst, BinGeom g, float inv_dx) {
-    __shared__ unsigned short s_order[kSortMax];
+// Copies every bin of plane set `src` into plane set `dst` in bank order and
+// without holes: a counting sort of the live particles by bank_class of
+// their base node (sorted index k), and entry k goes to slot
+// q = k * mul mod n, mul the inverse of s ~ n/32 modulo n.  Slots q and q+1
+// then hold sorted entries s apart, about one class apart, so any 32
+// consecutive slots span nearly all 32 classes.  Bins larger than kSortMax
+// and the overflow run are copied as they are.
+__device__ void interleave_bin(const float* __restrict__ src, const int* __restrict__ cnt_src,
+                               float* __restrict__ dst, int* __restrict__ cnt_dst, const BinGeom& g, float inv_dx,
+                               int item) {
+    __shared__ float s_stage[kStageFloats];      // planes of the bin being copied
+    __shared__ unsigned short s_from[kSortMax];  // source slot of each destination slot
     __shared__ unsigned short s_rank[kSortMax];
     __shared__ unsigned char s_cls[kSortMax];
-    __shared__ short s_live[kSortMax];
     __shared__ int s_ccnt[32];
     __shared__ int s_coff[32];
-    __shared__ int s_n;
+    __shared__ int s_n, s_mul;
     const int tid = threadIdx.x;
-    const int wg = blockIdx.x;
     int64_t off0;
-    int n_in, bin;
-    if (wg >= g.n_bins) {  // overflow blocks are copied as they are
-        const int k = wg - g.n_bins;
+    int n_in;
+    if (item >= g.n_bins) {  // overflow blocks are copied as they are
+        const int k = item - g.n_bins;
         const int n_ovf = min(cnt_src[g.n_bins], g.ovf_cap);
-        off0 = (int64_t)wg * MPM_NPLANES * g.cap;
+        off0 = (int64_t)item * MPM_NPLANES * g.cap;
         n_in = min(max(n_ovf - k * g.cap, 0), g.cap);
         for (int i = tid; i < n_in; i += kWG)
             for (int pl = 0; pl < MPM_NPLANES; ++pl) dst[off0 + (int64_t)pl * g.cap + i] = src[off0 + (int64_t)pl * g.cap + i];
         if (k == 0 && tid == 0) cnt_dst[g.n_bins] = n_ovf;
         return;
     }
-    bin = wg;
+    const int bin = item;
     n_in = min(cnt_src[bin], g.cap);
     off0 = (int64_t)bin * MPM_NPLANES * g.cap;
     const int nb = g.nb;
     const int b0[3] = {(bin / (nb * nb)) * kBinEdge, ((bin / nb) % nb) * kBinEdge, (bin % nb) * kBinEdge};
     if (n_in > kSortMax) {
+        // kept in place, holes included (the live count is then an upper bound)
         for (int i = tid; i < n_in; i += kWG)
             for (int pl = 0; pl < MPM_NPLANES; ++pl) dst[off0 + (int64_t)pl * g.cap + i] = src[off0 + (int64_t)pl * g.cap + i];
         if (tid == 0) { cnt_dst[bin] = n_in; cnt_dst[g.n_bins + 1 + bin] = n_in; }
         return;
     }
     if (tid < 32) s_ccnt[tid] = 0;
-    if (tid == 0) s_n = 0;
     __syncthreads();
     for (int i = tid; i < n_in; i += kWG) {
         const float* sp = src + off0 + i;
@@ -641,39 +669,63 @@ __global__ void __launch_bounds__(kWG)
         s_rank[i] = (unsigned short)atomicAdd(&s_ccnt[c], 1);
     }
     __syncthreads();
-    if (tid == 0) {
-        int acc = 0;
-        for (int c = 0; c < 32; ++c) { s_coff[c] = acc; acc += s_ccnt[c]; }
-        s_n = acc;
+    if (tid < 64) {  // first wave: exclusive scan of the 32 class counts
+        const int h = tid < 32 ? s_ccnt[tid] : 0;
+        int incl = h;
+#pragma unroll
+        for (int o = 1; o < 32; o <<= 1) {
+            const int t = __shfl_up(incl, o);
+            if (tid >= o) incl += t;
+        }
+        if (tid < 32) s_coff[tid] = incl - h;
+        if (tid == 31) {
+            s_n = incl;
+            s_mul = interleave_mul(incl);
+            cnt_dst[bin] = incl;
+            cnt_dst[g.n_bins + 1 + bin] = incl;
+        }
     }
     __syncthreads();
-    const int n = s_n;
-    for (int i = tid; i < n_in; i += kWG)
-        if (s_cls[i] != 255) s_order[s_coff[s_cls[i]] + s_rank[i]] = (unsigned short)i;
-    __syncthreads();
-    // storage position q <- sorted entry (q mod 32) * S + q / 32, skipping
-    // entries past n; s_live numbers the kept positions
-    const int S = (n + 31) >> 5;
-    for (int q = tid; q < 32 * S; q += kWG) {
-        const int p = (q & 31) * S + (q >> 5);
-        s_live[q < kSortMax ? q : 0] = (short)(p < n ? 1 : 0);
+    const unsigned n = (unsigned)s_n, mul = (unsigned)s_mul;
+    for (int i = tid; i < n_in; i += kWG) {
+        const int c = s_cls[i];
+        if (c == 255) continue;
+        s_from[((unsigned)(s_coff[c] + s_rank[i]) * mul) % n] = (unsigned short)i;
     }
-    __syncthreads();
-    if (tid == 0) {  // exclusive scan of kept positions (n <= kSortMax)
-        int acc = 0;
-        for (int q = 0; q < 32 * S && q < kSortMax; ++q) { const int k = s_live[q]; s_live[q] = (short)(k ? acc : -1); acc += k; }
-        cnt_dst[bin] = acc;
-        cnt_dst[g.n_bins + 1 + bin] = acc;
+    // Permuted copy staged through LDS, as many planes at a time as fit:
+    // coalesced reads of whole planes (many loads in flight per thread), then
+    // coalesced writes gathering from LDS.
+    const int per = max(1, kStageFloats / max(n_in, 1));
+    for (int pl0 = 0; pl0 < MPM_NPLANES; pl0 += per) {
+        const int np = min(per, MPM_NPLANES - pl0);
+        __syncthreads();  // s_from ready / the previous stage is written out
+        for (int j = 0; j < np; ++j) {
+            const float* sp = src + off0 + (int64_t)(pl0 + j) * g.cap;
+            for (int i = tid; i < n_in; i += kWG) s_stage[j * n_in + i] = sp[i];
+        }
+        __syncthreads();
+        for (int j = 0; j < np; ++j) {
+            float* dp = dst + off0 + (int64_t)(pl0 + j) * g.cap;
+            for (int q = tid; q < (int)n; q += kWG) dp[q] = s_stage[j * n_in + s_from[q]];
+        }
     }
-    __syncthreads();
-    for (int q = tid; q < 32 * S && q < kSortMax; q += kWG) {
-        const int dpos = s_live[q];
-        if (dpos < 0) continue;
-        const int p = (q & 31) * S + (q >> 5);
-        const int i = s_order[p];
-        for (int pl = 0; pl < MPM_NPLANES; ++pl)
-            dst[off0 + (int64_t)pl * g.cap + dpos] = src[off0 + (int64_t)pl * g.cap + i];
+}
+
+// One workgroup per bin, then one per block of `cap` slots of the overflow
+// run.  With a current schedule (schedule_bins, order[n_bins] == epoch) the
+// bins are taken in its order, which spreads them over the XCDs and skips
+// the empty ones (their counts in `cnt_dst` are then already zero);
+// otherwise workgroup b takes bin b.
+__global__ void __launch_bounds__(kWG)
+    k_bins_interleave(const float* __restrict__ src, const int* __restrict__ cnt_src, float* __restrict__ dst,
+                      int* __restrict__ cnt_dst, BinGeom g, float inv_dx, const int* __restrict__ order,
+                      int sched_epoch) {
+    int item = blockIdx.x;
+    if (item < g.n_bins && order != nullptr && order[g.n_bins] == sched_epoch) {
+        item = order[item];
+        if (item < 0) return;
     }
+    interleave_bin(src, cnt_src, dst, cnt_dst, g, inv_dx, item);
 }
 
 // fill n ints with v (bin counters; a kernel rather than a memset so that the
@@ -767,7 +819,7 @@ struct WsLayout {
     int* gflag;      // spill flags per grid block
     int* status;     // 64 counters
     int* stamps;     // epoch of each bin's tile
-    int* order;      // k_g2p2g schedule: nb^3 bins + its epoch
+    int* order;      // k_g2p2g schedule: nb^3 bins + its epoch; then the sort queue
     float4* tiles;   // nb^3 tiles of kTS^3 nodes
     int64_t bytes;
 };
@@ -794,7 +846,7 @@ WsLayout ws_layout(const MpmParams* prm, void* base) {
     w.stamps = (int*)(p + off);
     off += align256(nbins * 4);
     w.order = (int*)(p + off);
-    off += align256((nbins + 1) * 4);
+    off += align256((nbins + 2) * 4);
     w.tiles = (float4*)(p + off);
     off += align256(nbins * kTS3 * 16);
     w.bytes = off;
@@ -802,6 +854,16 @@ WsLayout ws_layout(const MpmParams* prm, void* base) {
 }
 
 unsigned ovf_wgs(const BinGeom& g) { return (unsigned)((g.ovf_cap + kWG - 1) / kWG); }
+
+// k_bins_interleave over every bin and overflow block (order: a schedule
+// or nullptr).
+int launch_sort(const BinGeom& g, const float* src, const int* cnt_src, float* dst, int* cnt_dst, float inv_dx,
+                const int* order, int sched_epoch, hipStream_t s) {
+    const unsigned n_items = (unsigned)(g.n_bins + g.ovf_cap / g.cap);
+    hipLaunchKernelGGL(k_bins_interleave, dim3(n_items), dim3(kWG), 0, s, src, cnt_src, dst, cnt_dst, g, inv_dx,
+                       order, sched_epoch);
+    return check_launch("k_bins_interleave");
+}
 
 }  // namespace
 
@@ -842,9 +904,7 @@ int mpm_bins_insert(const MpmParams* prm, const MpmState* flat, const MpmBins* b
     hipLaunchKernelGGL(k_bins_insert, dim3((unsigned)((flat->n + kWG - 1) / kWG)), dim3(kWG), 0, s, flat->planes,
                        flat->stride, (int)flat->n, prm->inv_dx, g, bins->planes[spare], cnt_spare, w.status);
     if ((rc = check_launch("k_bins_insert"))) return rc;
-    hipLaunchKernelGGL(k_bins_interleave, dim3(g.n_bins + ovf_wgs(g)), dim3(kWG), 0, s, bins->planes[spare],
-                       cnt_spare, bins->planes[bins->cur], cnt, g, prm->inv_dx);
-    return check_launch("k_bins_interleave");
+    return launch_sort(g, bins->planes[spare], cnt_spare, bins->planes[bins->cur], cnt, prm->inv_dx, nullptr, 0, s);
 }
 
 int mpm_bins_extract(const MpmParams* prm, const MpmBins* bins, const MpmState* flat, void* stream) {
@@ -940,42 +1000,75 @@ int mpm_bins_g2p2g(const MpmParams* prm, MpmBins* bins, void* workspace, float d
     return 0;
 }
 
+int mpm_bins_sort(const MpmParams* prm, MpmBins* bins, void* workspace, void* stream) {
+    BinGeom g;
+    int rc = check_bins(prm, bins, &g);
+    if (rc) return rc;
+    if (!workspace) return set_err(-10, "workspace is NULL");
+    WsLayout w = ws_layout(prm, workspace);
+    const int cur = bins->cur, nxt = 1 - cur;
+    int* cnt_cur = bins->count + (int64_t)cur * (2 * g.n_bins + 1);
+    int* cnt_nxt = bins->count + (int64_t)nxt * (2 * g.n_bins + 1);
+    // With the schedule of this epoch's grid update (checked on the device),
+    // only scheduled bins are copied; that grid update has zeroed cnt_nxt, so
+    // the others read as empty.  Without one, every bin is copied.
+    rc = launch_sort(g, bins->planes[cur], cnt_cur, bins->planes[nxt], cnt_nxt, prm->inv_dx, w.order, bins->epoch,
+                     (hipStream_t)stream);
+    if (rc) return rc;
+    // the old set becomes the fused kernel's output: its counters start at zero
+    const int cb = 2 * g.n_bins + 1;
+    hipLaunchKernelGGL(k_fill_int, dim3((cb + kWG - 1) / kWG), dim3(kWG), 0, (hipStream_t)stream, cnt_cur, cb, 0);
+    if ((rc = check_launch("k_fill_int"))) return rc;
+    bins->cur = nxt;
+    bins->since_sort = 0;
+    return 0;
+}
+
 int mpm_bins_substep(const MpmParams* prm, MpmBins* bins, const MpmCollider* colliders, int n_colliders,
                      void* workspace, float time, float dt, void* stream) {
     int rc = mpm_bins_grid_update(prm, bins, colliders, n_colliders, workspace, time, dt, stream);
     if (rc) return rc;
+    // particles drift across cells and out of bank order; restore it now and
+    // then, between the grid update (which schedules the bins) and the fused
+    // kernel, which reads the sorted set
+    if (++bins->since_sort >= kSortPeriod && (rc = mpm_bins_sort(prm, bins, workspace, stream))) return rc;
     return mpm_bins_g2p2g(prm, bins, workspace, dt, stream);
 }
 
 int mpm_bins_profile(const MpmParams* prm, MpmBins* bins, const MpmCollider* colliders, int n_colliders,
                      void* workspace, double time, float dt, int n_substeps, void* stream, float* grid_ms,
-                     float* g2p2g_ms) {
-    if (!bins || !grid_ms || !g2p2g_ms) return set_err(-20, "profile: NULL pointer");
+                     float* g2p2g_ms, float* sort_ms) {
+    if (!bins || !grid_ms || !g2p2g_ms || !sort_ms) return set_err(-20, "profile: NULL pointer");
     if (n_substeps < 1) return set_err(-26, "n_substeps < 1");
     hipStream_t s = (hipStream_t)stream;
-    hipEvent_t ev[3];
-    for (int k = 0; k < 3; ++k)
+    hipEvent_t ev[4];
+    for (int k = 0; k < 4; ++k)
         if (hipEventCreate(&ev[k]) != hipSuccess) return set_err(-27, "hipEventCreate failed");
     double t = time;
-    float acc_grid = 0.f, acc_g2p2g = 0.f;
+    float acc_grid = 0.f, acc_g2p2g = 0.f, acc_sort = 0.f;
     int rc = 0;
     for (int k = 0; k < n_substeps && rc == 0; ++k) {
         (void)hipEventRecord(ev[0], s);
         rc = mpm_bins_grid_update(prm, bins, colliders, n_colliders, workspace, (float)t, dt, stream);
         (void)hipEventRecord(ev[1], s);
-        if (rc == 0) rc = mpm_bins_g2p2g(prm, bins, workspace, dt, stream);
+        if (rc == 0 && ++bins->since_sort >= kSortPeriod) rc = mpm_bins_sort(prm, bins, workspace, stream);
         (void)hipEventRecord(ev[2], s);
-        (void)hipEventSynchronize(ev[2]);
-        float a = 0.f, b = 0.f;
+        if (rc == 0) rc = mpm_bins_g2p2g(prm, bins, workspace, dt, stream);
+        (void)hipEventRecord(ev[3], s);
+        (void)hipEventSynchronize(ev[3]);
+        float a = 0.f, b = 0.f, c = 0.f;
         (void)hipEventElapsedTime(&a, ev[0], ev[1]);
-        (void)hipEventElapsedTime(&b, ev[1], ev[2]);
+        (void)hipEventElapsedTime(&c, ev[1], ev[2]);
+        (void)hipEventElapsedTime(&b, ev[2], ev[3]);
         acc_grid += a;
         acc_g2p2g += b;
+        acc_sort += c;
         t = t + (double)dt;
     }
-    for (int k = 0; k < 3; ++k) (void)hipEventDestroy(ev[k]);
+    for (int k = 0; k < 4; ++k) (void)hipEventDestroy(ev[k]);
     *grid_ms = acc_grid / n_substeps;
     *g2p2g_ms = acc_g2p2g / n_substeps;
+    *sort_ms = acc_sort / n_substeps;
     return rc;
 }
 

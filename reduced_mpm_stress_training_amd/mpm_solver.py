@@ -388,20 +388,20 @@ class MPM_Simulator_HIP:
 
     def profile_kernels(self, n_substeps, dt):
         """Runs n_substeps on the binned path with HIP events around each
-        kernel (waits after every substep); returns the average
-        (grid_update_ms, g2p2g_ms)."""
+        kernel (waits after every substep); returns the average per substep
+        (grid_update_ms, g2p2g_ms, sort_ms)."""
         lib = L.load()
         self._bins_run(0, float(dt))       # bins the state and computes the pending P2G
         B = self._bins
         cols, ncol = self._colliders()
-        gm, km = C_float(), C_float()
+        gm, km, sm = C_float(), C_float(), C_float()
         L.check(lib.mpm_bins_profile(self._params(), B["struct"], cols, ncol, self._bins_ws_ptr(), float(self.time),
-                                     float(dt), int(n_substeps), _stream_handle(self.device), gm, km),
+                                     float(dt), int(n_substeps), _stream_handle(self.device), gm, km, sm),
                 "mpm_bins_profile")
         for _ in range(int(n_substeps)):
             self.time = self.time + float(dt)
         self._flat_valid = False
-        return gm.value, km.value
+        return gm.value, km.value, sm.value
 
     def _sync_flat(self):
         """Make the flat planes mirror the binned state (no change of owner)."""
