@@ -54,7 +54,42 @@ def parse():
                     help="substeps timed per kernel for the roofline")
     ap.add_argument("--unbinned", action="store_true",
                     help="time the unbinned stage kernels (atomic P2G, grid update, G2P) instead")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 1 GiB device copy (reference for the HBM byte counters)")
     return ap.parse_args()
+
+
+def calibrate_copy():
+    """A 1 GiB fill and three 1 GiB copies: under rocprofv3 --pmc FETCH_SIZE /
+    WRITE_SIZE their per-dispatch readings give the counters' scale against
+    known byte counts."""
+    import torch
+    n = 256 * 1024 * 1024
+    src = torch.empty(n, dtype=torch.float32, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration": "copy", "bytes_read_per_copy": 4 * n, "bytes_written_per_copy": 4 * n,
+                      "bytes_written_per_fill": 4 * n}), flush=True)
+
+
+TRAFFIC_PROFILE = os.path.join(HERE, "profiles", "g2p2g_1M_128_hbm_counters.md")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` on the headline workload, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the copy
+    calibration recorded there); None if not recorded."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == "k_" + kernel:
+                    return float(parts[2])
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(scene, budget_s):
@@ -91,6 +126,9 @@ def main():
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the substep is not sharded "
               "across GPUs in this build", file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     import torch
     from reduced_mpm_stress_training_amd import scenes as S
 
@@ -162,7 +200,9 @@ def main():
                    "nu": scene.nu, "friction_angle": scene.friction_angle, "density": scene.density,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(dom) if (not args.unbinned and args.particles == 1_000_000
+                                                           and args.n_grid == 128) else None,
                      "bytes_per_particle": ALG_BYTES[dom],
                      "stage_ms": dur, "step_achieved_gbs": step_gbs, "step_frac": step_gbs / HBM_PEAK_GBS},
         "cpu_baseline": cpu,
