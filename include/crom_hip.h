@@ -1,0 +1,68 @@
+/* crom_hip.h -- C-ABI of the CROM decoder MLP on MI355X matrix cores.
+ *
+ * The network is the reference's decoder NetDec (offline_training_u/
+ * CROMnet.py:218-301): Prepare (standardize the 3 reference-position columns,
+ * :497-534) -> dec0..dec4 (Linear + ELU(alpha=1), :226-237, :457-473) -> enc
+ * (Linear) -> invStandardizeQ (mean_q + y * std_q, :475-487).  The stress head
+ * of offline_training_xhat_and_stress_and_mu/CROMnet_secondary.py:203-307 is
+ * the same trunk with input [label, X, mu] and 6 outputs; it uses the same
+ * entry points with in_dim = lbl + 4, out_dim = 6.
+ *
+ * Hidden width is CROM_HIDDEN (the configuration's 128); in_dim and out_dim
+ * are at most 32.  Matrix products run on BF16 MFMA with FP32 accumulation;
+ * weights are kept by the caller in FP32 (torch nn.Linear layout [out][in])
+ * and packed to BF16 operand fragments by crom_pack.
+ *
+ * All pointers are device pointers; `stream` is a hipStream_t.  Status 0 on
+ * success, negative on error (message in mpm_last_error(), include/mpm_hip.h).
+ * The library allocates nothing.
+ */
+#ifndef CROM_HIP_H
+#define CROM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CROM_HIDDEN 128
+#define CROM_N_LINEAR 6   /* dec0..dec4, enc */
+#define CROM_MAX_IN 32
+#define CROM_MAX_OUT 32
+
+typedef struct CromNet {
+    int32_t in_dim;           /* lbl + 3 (+ mu columns), <= 32 */
+    int32_t out_dim;          /* 3 (displacement) or 6 (stress), <= 32 */
+    int32_t x_col;            /* first of the 3 position columns Prepare standardizes (= lbl) */
+    int32_t reserved;
+    const float* weight[CROM_N_LINEAR];  /* [out][in] FP32: dec0 [128][in_dim], dec1..4 [128][128], enc [out_dim][128] */
+    const float* bias[CROM_N_LINEAR];
+    const float* mean_x;      /* 3: Prepare.standardizeX, CROMnet.py:527-528 */
+    const float* std_x;       /* 3 */
+    const float* mean_q;      /* out_dim: invStandardizeQ, CROMnet.py:486-487 */
+    const float* std_q;       /* out_dim */
+} CromNet;
+
+/* Bytes of the packed forward operands (BF16 weight fragments, biases,
+ * standardization constants). */
+int64_t crom_packed_bytes(void);
+
+/* Packs `net` for crom_forward (and the transposed fragments crom_backward
+ * uses) into `packed` (crom_packed_bytes() bytes, 256-byte aligned). */
+int crom_pack(const CromNet* net, void* packed, void* stream);
+
+/* NetDec.forward (CROMnet.py:293-296) on n points: x [n][in_dim] FP32
+ * (decoder input [label, X, (mu)]), y [n][out_dim] FP32.  `acts` may be NULL;
+ * otherwise it receives the five hidden activations in BF16 for
+ * crom_backward (crom_acts_bytes(n) bytes). */
+int crom_forward(const void* packed, const float* x, int64_t n, float* y, void* acts, void* stream);
+
+/* Bytes of the activation / gradient buffers of n points. */
+int64_t crom_acts_bytes(int64_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CROM_HIP_H */

@@ -42,6 +42,12 @@ class MpmBins(C.Structure):
                 ("stride", C.c_int64), ("planes", C.c_void_p * 2), ("count", C.c_void_p)]
 
 
+class CromNet(C.Structure):
+    _fields_ = [("in_dim", C.c_int32), ("out_dim", C.c_int32), ("x_col", C.c_int32), ("reserved", C.c_int32),
+                ("weight", C.c_void_p * 6), ("bias", C.c_void_p * 6),
+                ("mean_x", C.c_void_p), ("std_x", C.c_void_p), ("mean_q", C.c_void_p), ("std_q", C.c_void_p)]
+
+
 # exported symbol -> (restype, argtypes)
 _SIGNATURES = {
     "mpm_abi_version": (C.c_int, []),
@@ -72,6 +78,10 @@ _SIGNATURES = {
     "mpm_bins_sort": (C.c_int, [C.POINTER(MpmParams), C.POINTER(MpmBins), C.c_void_p, C.c_void_p]),
     "mpm_bins_run": (C.c_int, [C.POINTER(MpmParams), C.POINTER(MpmBins), C.POINTER(MpmCollider), C.c_int,
                                C.c_void_p, C.c_double, C.c_float, C.c_int, C.c_void_p]),
+    "crom_packed_bytes": (C.c_int64, []),
+    "crom_acts_bytes": (C.c_int64, [C.c_int64]),
+    "crom_pack": (C.c_int, [C.POINTER(CromNet), C.c_void_p, C.c_void_p]),
+    "crom_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]),
     "mpm_import": (C.c_int, [C.POINTER(MpmState), C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p]),
     "mpm_export": (C.c_int, [C.POINTER(MpmState), C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p]),
 }
