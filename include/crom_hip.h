@@ -54,12 +54,42 @@ int crom_pack(const CromNet* net, void* packed, void* stream);
 
 /* NetDec.forward (CROMnet.py:293-296) on n points: x [n][in_dim] FP32
  * (decoder input [label, X, (mu)]), y [n][out_dim] FP32.  `acts` may be NULL;
- * otherwise it receives the five hidden activations in BF16 for
- * crom_backward (crom_acts_bytes(n) bytes). */
+ * otherwise it receives the prepared input and the five hidden activations
+ * in BF16 for crom_backward / crom_wgrad (crom_acts_bytes(n) bytes). */
 int crom_forward(const void* packed, const float* x, int64_t n, float* y, void* acts, void* stream);
 
 /* Bytes of the activation / gradient buffers of n points. */
 int64_t crom_acts_bytes(int64_t n);
+
+/* Backward of the training loss (CROMnet.py:70-83: 1000 * MSELoss(y, q))
+ * through the decoder, written with the loss as  scale/2 * sum (y - q)^2
+ * (scale = 2000 / (n * out_dim) for that loss).  acts: from crom_forward of
+ * the same x; y, q [n][out_dim].  Writes the layers' pre-activation
+ * gradients to `grads_buf` (crom_acts_bytes(n) bytes) and, when dx is not
+ * NULL, the input gradient dx [n][in_dim] (through Prepare on the position
+ * columns) -- the latent columns' gradient is what the encoder needs. */
+int crom_backward(const void* packed, const void* acts, const float* y, const float* q, int64_t n, float scale,
+                  void* grads_buf, float* dx, void* stream);
+
+/* Number of parameters (weights [out][in] then bias, layer by layer: dec0,
+ * dec1..dec4, enc), the layout of the flat parameter / gradient / Adam
+ * moment buffers. */
+int64_t crom_param_count(int32_t in_dim, int32_t out_dim);
+
+/* Workspace of crom_wgrad for n points. */
+int64_t crom_wgrad_workspace_bytes(int64_t n);
+
+/* Parameter gradients (flat, crom_param_count floats) from the activations
+ * of crom_forward and the gradients of crom_backward over the same n points:
+ * dW = sum_p dz (x) a_prev, db = sum_p dz, on matrix cores over the points. */
+int crom_wgrad(const void* acts, const void* grads_buf, int64_t n, int32_t in_dim, int32_t out_dim, void* workspace,
+               float* grads, void* stream);
+
+/* One Adam step (torch.optim.Adam as CROMnet.py:193-196 configures it: lr
+ * from the LambdaLR schedule, betas, eps, no weight decay) over `count`
+ * FP32 parameters in place; step counts from 1. */
+int crom_adam(float* params, const float* grads, float* m, float* v, int64_t count, float lr, float beta1,
+              float beta2, float eps, int32_t step, void* stream);
 
 #ifdef __cplusplus
 }

@@ -82,6 +82,14 @@ _SIGNATURES = {
     "crom_acts_bytes": (C.c_int64, [C.c_int64]),
     "crom_pack": (C.c_int, [C.POINTER(CromNet), C.c_void_p, C.c_void_p]),
     "crom_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "crom_backward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_float, C.c_void_p,
+                                C.c_void_p, C.c_void_p]),
+    "crom_param_count": (C.c_int64, [C.c_int32, C.c_int32]),
+    "crom_wgrad_workspace_bytes": (C.c_int64, [C.c_int64]),
+    "crom_wgrad": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
+                             C.c_void_p]),
+    "crom_adam": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_float, C.c_float,
+                            C.c_float, C.c_float, C.c_int32, C.c_void_p]),
     "mpm_import": (C.c_int, [C.POINTER(MpmState), C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p]),
     "mpm_export": (C.c_int, [C.POINTER(MpmState), C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p]),
 }
