@@ -40,7 +40,7 @@
 extern "C" {
 #endif
 
-#define MPM_ABI_VERSION 4
+#define MPM_ABI_VERSION 5
 
 /* plane indices */
 #define MPM_P_X0 0
@@ -220,6 +220,55 @@ int mpm_bins_profile(const MpmParams* prm, MpmBins* bins, const MpmCollider* col
  * self.time (mpm_solver_warp.py:199). */
 int mpm_bins_run(const MpmParams* prm, MpmBins* bins, const MpmCollider* colliders, int n_colliders,
                  void* workspace, double time, float dt, int n_substeps, void* stream);
+
+/* ---- batches of independent scenes -----------------------------------
+ *
+ * No counterpart in the reference: its driver steps one scene per process
+ * (generate_training_data/generate_sand.py:17-32).  A batch runs many
+ * scenes -- each with its own grid size, material and colliders -- through
+ * the binned kernels above in the same launches: the scenes' bins are
+ * numbered one after the other (one MpmBins with nb = 0 holds them all),
+ * their grids likewise, and the batch's flat state (MpmState) holds the
+ * scenes' particles one scene after the other, PID plane = flat index.  All
+ * scenes advance with the same dt and time. */
+typedef struct MpmScene {
+    MpmParams params;     /* the scene's grid and material (material 2) */
+    int64_t n_particles;  /* its particles, consecutive in the batch's flat state */
+    int32_t n_colliders;  /* its colliders: the next n_colliders entries of the batch's collider array */
+    int32_t bin_cap;      /* slots per bin of this scene */
+} MpmScene;
+
+typedef struct MpmBatch { /* filled by mpm_batch_setup */
+    int32_t n_scenes;
+    int32_t n_bins;       /* bins of all scenes: sum of ceil(n_grid/8)^3 */
+    int32_t n_colliders;
+    int32_t max_cap;      /* largest bin_cap of the scenes */
+    int64_t n_slots;      /* bin slots of all scenes: sum of bins * bin_cap */
+    int64_t n_particles;  /* particles of all scenes */
+    int64_t n_nodes;      /* grid nodes of all scenes */
+    void* workspace;      /* the batch's workspace; holds its scene table */
+} MpmBatch;
+
+/* Workspace of a batch (zeroed once before use, 256-byte aligned). */
+int64_t mpm_batch_workspace_bytes(const MpmScene* scenes, int n_scenes);
+
+/* Checks the scenes, lays the batch out and copies its scene table and
+ * colliders (host arrays) into the workspace (a blocking copy).  Bins of
+ * scene i have scenes[i].bin_cap slots; the batch's MpmBins has nb = 0,
+ * bin_cap = the overflow run's block size, count arrays of
+ * [2][2 * n_bins + 1] and stride >= (n_slots + overflow_cap) * MPM_NPLANES. */
+int mpm_batch_setup(const MpmScene* scenes, int n_scenes, const MpmCollider* colliders, void* workspace,
+                    MpmBatch* batch);
+
+/* Byte offset of the status counters in the batch's workspace. */
+int64_t mpm_batch_status_offset(const MpmBatch* batch);
+
+/* The mpm_bins_* operations on every scene of a batch at once. */
+int mpm_batch_insert(const MpmBatch* batch, const MpmState* flat, const MpmBins* bins, void* stream);
+int mpm_batch_extract(const MpmBatch* batch, const MpmBins* bins, const MpmState* flat, void* stream);
+int mpm_batch_clear_grid(const MpmBatch* batch, MpmBins* bins, void* stream);
+int mpm_batch_p2g(const MpmBatch* batch, MpmBins* bins, float dt, void* stream);
+int mpm_batch_run(const MpmBatch* batch, MpmBins* bins, double time, float dt, int n_substeps, void* stream);
 
 /* ---- state import / export (torch loaders and export_uvFstress_from_warp,
  *      mpm_solver_warp.py:229-266 and :305-310) --------------------------- */

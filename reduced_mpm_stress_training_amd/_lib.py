@@ -10,7 +10,7 @@ import os
 
 from ._build import LIB_PATH
 
-MPM_ABI_VERSION = 4
+MPM_ABI_VERSION = 5
 MAX_COLLIDERS = 16
 
 # plane indices of the particle state buffer
@@ -40,6 +40,17 @@ class MpmBins(C.Structure):
     _fields_ = [("nb", C.c_int32), ("bin_cap", C.c_int32), ("overflow_cap", C.c_int32), ("cur", C.c_int32),
                 ("epoch", C.c_int32), ("since_sort", C.c_int32),
                 ("stride", C.c_int64), ("planes", C.c_void_p * 2), ("count", C.c_void_p)]
+
+
+class MpmScene(C.Structure):
+    _fields_ = [("params", MpmParams), ("n_particles", C.c_int64), ("n_colliders", C.c_int32),
+                ("bin_cap", C.c_int32)]
+
+
+class MpmBatch(C.Structure):
+    _fields_ = [("n_scenes", C.c_int32), ("n_bins", C.c_int32), ("n_colliders", C.c_int32), ("max_cap", C.c_int32),
+                ("n_slots", C.c_int64), ("n_particles", C.c_int64), ("n_nodes", C.c_int64),
+                ("workspace", C.c_void_p)]
 
 
 class CromNet(C.Structure):
@@ -78,6 +89,16 @@ _SIGNATURES = {
     "mpm_bins_sort": (C.c_int, [C.POINTER(MpmParams), C.POINTER(MpmBins), C.c_void_p, C.c_void_p]),
     "mpm_bins_run": (C.c_int, [C.POINTER(MpmParams), C.POINTER(MpmBins), C.POINTER(MpmCollider), C.c_int,
                                C.c_void_p, C.c_double, C.c_float, C.c_int, C.c_void_p]),
+    "mpm_batch_workspace_bytes": (C.c_int64, [C.POINTER(MpmScene), C.c_int]),
+    "mpm_batch_setup": (C.c_int, [C.POINTER(MpmScene), C.c_int, C.POINTER(MpmCollider), C.c_void_p,
+                                  C.POINTER(MpmBatch)]),
+    "mpm_batch_status_offset": (C.c_int64, [C.POINTER(MpmBatch)]),
+    "mpm_batch_insert": (C.c_int, [C.POINTER(MpmBatch), C.POINTER(MpmState), C.POINTER(MpmBins), C.c_void_p]),
+    "mpm_batch_extract": (C.c_int, [C.POINTER(MpmBatch), C.POINTER(MpmBins), C.POINTER(MpmState), C.c_void_p]),
+    "mpm_batch_clear_grid": (C.c_int, [C.POINTER(MpmBatch), C.POINTER(MpmBins), C.c_void_p]),
+    "mpm_batch_p2g": (C.c_int, [C.POINTER(MpmBatch), C.POINTER(MpmBins), C.c_float, C.c_void_p]),
+    "mpm_batch_run": (C.c_int, [C.POINTER(MpmBatch), C.POINTER(MpmBins), C.c_double, C.c_float, C.c_int,
+                                C.c_void_p]),
     "crom_packed_bytes": (C.c_int64, []),
     "crom_acts_bytes": (C.c_int64, [C.c_int64]),
     "crom_pack": (C.c_int, [C.POINTER(CromNet), C.c_void_p, C.c_void_p]),

@@ -28,6 +28,8 @@
 // k_p2g_bins.  So between substeps the tiles hold the next substep's P2G,
 // computed with that substep's dt.
 #include <hip/hip_runtime.h>
+#include <stdlib.h>
+#include <string.h>
 
 #include "mpm_common.h"
 #include "mpm_device.h"
@@ -48,17 +50,80 @@ constexpr int kSortMax = 6144;        // bins up to this size are processed in b
 constexpr int kSortPeriod = 32;       // substeps between bank-order sorts of the bins
 constexpr int kStageFloats = 16384;   // LDS staging of the sort's copy (64 KB)
 
-// Plane sets are bin-major: bin b (b >= n_bins: overflow run) is a block of
-// MPM_NPLANES planes of `cap` floats, so slot i of bin b, plane p, is at
-// (b * MPM_NPLANES + p) * cap + i.  A slot index is b * cap + i.
+// Plane sets are bin-major: bin b of a scene with `cap` slots per bin is a
+// block of MPM_NPLANES planes of cap floats starting at float offset
+// bin_off(scene, b), so slot i, plane p is at bin_off + p * cap + i.  After
+// all bins comes the overflow run, in blocks of ovf_blk slots laid out the
+// same way.  (One scene: bin b at b * MPM_NPLANES * cap, ovf_blk = cap.)
 struct BinGeom {
-    int ng, nb, n_bins, cap, ovf_cap;
+    int n_bins;        // bins of all scenes
+    int ovf_cap;       // overflow slots
+    int ovf_blk;       // slots per overflow block
+    int max_cap;       // largest slots-per-bin of any scene
+    int64_t ovf_off;   // float offset of the overflow run
     int64_t stride;
 };
 
-__device__ __forceinline__ int64_t slot_off(const BinGeom& g, int64_t slot) {
-    const int64_t b = slot / g.cap, i = slot - b * g.cap;
-    return b * MPM_NPLANES * g.cap + i;
+// One scene: the mpm_bins_* entry points run one, the mpm_batch_* entry
+// points many, through the same kernels.  A batch concatenates the scenes'
+// bins (global bin = bin_base + local bin), their grids (node_base) and their
+// particles (pid_base); every scene has its own parameters and colliders.
+struct SceneDesc {
+    MpmParams prm;
+    int nb;          // bins per axis = ceil(n_grid / 8)
+    int bin_base;    // first global bin (grid blocks are numbered like bins)
+    int col_base;    // first collider in the batch's collider array
+    int n_cols;
+    int cap;         // slots per bin
+    int pad;
+    int64_t node_base;  // first node of the scene's grids
+    int64_t pid_base;   // first particle (flat index) of the scene
+    int64_t slot_base;  // float offset of the scene's first bin
+};
+
+struct SceneTab {
+    SceneDesc one;              // the scene when dev == nullptr
+    const SceneDesc* dev;       // device table of n scenes (batch)
+    int n;
+    const MpmCollider* cols;    // device colliders (batch)
+};
+
+__device__ __forceinline__ SceneDesc scene_of_bin(const SceneTab& t, int bin) {
+    if (t.dev == nullptr) return t.one;
+    int lo = 0, hi = t.n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (t.dev[mid].bin_base <= bin) lo = mid; else hi = mid - 1;
+    }
+    return t.dev[lo];
+}
+
+__device__ __forceinline__ SceneDesc scene_of_pid(const SceneTab& t, int64_t pid) {
+    if (t.dev == nullptr) return t.one;
+    int lo = 0, hi = t.n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (t.dev[mid].pid_base <= pid) lo = mid; else hi = mid - 1;
+    }
+    return t.dev[lo];
+}
+
+// origin (cells) of local bin lb of a scene with nb bins per axis
+__device__ __forceinline__ void bin_origin(int lb, int nb, int b0[3]) {
+    b0[0] = (lb / (nb * nb)) * kBinEdge;
+    b0[1] = ((lb / nb) % nb) * kBinEdge;
+    b0[2] = (lb % nb) * kBinEdge;
+}
+
+// float offset of slot 0, plane 0 of global bin b (plane stride sc.cap)
+__device__ __forceinline__ int64_t bin_off(const SceneDesc& sc, int b) {
+    return sc.slot_base + (int64_t)(b - sc.bin_base) * MPM_NPLANES * sc.cap;
+}
+
+// float offset of overflow slot r, plane 0 (plane stride g.ovf_blk)
+__device__ __forceinline__ int64_t ovf_off(const BinGeom& g, int r) {
+    const int k = r / g.ovf_blk;
+    return g.ovf_off + (int64_t)k * MPM_NPLANES * g.ovf_blk + (r - k * g.ovf_blk);
 }
 
 __device__ __forceinline__ int bin_of_base(const int b[3], int nb) {
@@ -125,13 +190,14 @@ __device__ __forceinline__ void mark_block(int* __restrict__ gflag, int nb, int 
 
 // Scatter of one particle straight into the spill grid with float atomics
 // (overflow particles and particles that left their bin's scatter tile).
-__device__ void scatter_spill(float* __restrict__ acc, int* __restrict__ gflag, const BinGeom& g,
+__device__ void scatter_spill(float* __restrict__ acc, int* __restrict__ gflag, const SceneDesc& sc,
                               const AxisWeights& A, float m, float vol, const float v[3], const float C[9],
-                              const float tau[9], float dx, float inv_dx, float dt) {
-    const int ng = g.ng;
-    p2g_contrib(A, m, vol, v, C, tau, dx, inv_dx, dt, [&](int c, int i, int j, int k, float val) {
+                              const float tau[9], float dt) {
+    const int ng = sc.prm.n_grid;
+    float* sacc = acc + 4 * sc.node_base;
+    p2g_contrib(A, m, vol, v, C, tau, sc.prm.dx, sc.prm.inv_dx, dt, [&](int c, int i, int j, int k, float val) {
         const int x = A.base[0] + i, y = A.base[1] + j, z = A.base[2] + k;
-        unsafeAtomicAdd(acc + 4 * (((int64_t)x * ng + y) * ng + z) + c, val);
+        unsafeAtomicAdd(sacc + 4 * (((int64_t)x * ng + y) * ng + z) + c, val);
     });
 #pragma unroll
     for (int i = 0; i < 2; ++i)
@@ -139,7 +205,7 @@ __device__ void scatter_spill(float* __restrict__ acc, int* __restrict__ gflag, 
         for (int j = 0; j < 2; ++j)
 #pragma unroll
             for (int k = 0; k < 2; ++k)
-                mark_block(gflag, g.nb, A.base[0] + 2 * i, A.base[1] + 2 * j, A.base[2] + 2 * k);
+                mark_block(gflag + sc.bin_base, sc.nb, A.base[0] + 2 * i, A.base[1] + 2 * j, A.base[2] + 2 * k);
 }
 
 // Scatter of one particle into a bin's FP64 LDS tile (SoA: mx, my, mz, m).
@@ -163,7 +229,7 @@ __device__ void write_tile(const double* s_acc, float4* __restrict__ tiles, int*
 }
 
 struct SubstepArgs {
-    MpmParams prm;
+    SceneTab tab;
     BinGeom g;
     float dt;
     int epoch_out;
@@ -210,10 +276,11 @@ __device__ __forceinline__ int interleave_mul(int n) {
 // loop (`stay_base`, -1 on the overflow path); lanes that stay in the bin
 // take consecutive slots of that run through an LDS counter, so stayers keep
 // their order.  Lanes that move take one atomic each on their new bin.
-// Returns the float offset of the slot (slot_off), or -1 when both the bin
-// and the overflow run are full.
+// Returns the float offset of the slot (its plane stride in *stride), or
+// -1 when both the bin and the overflow run are full.
 __device__ __forceinline__ int64_t reserve_slot(bool act, int new_bin, int own_bin, int stay_base,
-                                                int* s_stay, const SubstepArgs& a) {
+                                                int* s_stay, const SubstepArgs& a, const SceneDesc& sc,
+                                                int64_t* stride) {
     const bool stay = act && new_bin == own_bin && stay_base >= 0;
     const unsigned long long mask = __ballot(stay);
     const int lane = __lane_id();
@@ -227,12 +294,16 @@ __device__ __forceinline__ int64_t reserve_slot(bool act, int new_bin, int own_b
     }
     if (act && !stay) {
         pos = atomicAdd(&a.cnt_nxt[new_bin], 1);
-        if (pos < a.g.cap) atomicAdd(&a.cnt_nxt[a.g.n_bins + 1 + new_bin], 1);  // live count
+        if (pos < sc.cap) atomicAdd(&a.cnt_nxt[a.g.n_bins + 1 + new_bin], 1);  // live count
     }
     if (!act) return -1;
-    if (pos < a.g.cap) return (int64_t)new_bin * MPM_NPLANES * a.g.cap + pos;
+    if (pos < sc.cap) {   // new_bin is a bin of the particle's own scene
+        *stride = sc.cap;
+        return bin_off(sc, new_bin) + pos;
+    }
     const int r = atomicAdd(&a.cnt_nxt[a.g.n_bins], 1);
-    if (r < a.g.ovf_cap) return slot_off(a.g, (int64_t)a.g.n_bins * a.g.cap + r);
+    *stride = a.g.ovf_blk;
+    if (r < a.g.ovf_cap) return ovf_off(a.g, r);
     atomicAdd(&a.status[0], 1);
     return -1;
 }
@@ -290,12 +361,12 @@ __device__ __forceinline__ void load_particle(const float* __restrict__ src, int
     }
 }
 
-__device__ __forceinline__ void g2p2g_particle(const SubstepArgs& a, bool act_slot, int64_t off, int own_bin,
-                                               int stay_base, int* s_stay, const int b0[3], const float* tile_vel,
-                                               double* tile_acc, const ParticleIn& q) {
-    const int64_t S = a.g.cap;
+__device__ __forceinline__ void g2p2g_particle(const SubstepArgs& a, const SceneDesc& sc, bool act_slot, int64_t off,
+                                               int64_t S, int own_bin, int stay_base, int* s_stay, const int b0[3],
+                                               const float* tile_vel, double* tile_acc, const ParticleIn& q) {
     const float* sp = a.src + off;
-    const MpmParams& prm = a.prm;
+    const MpmParams& prm = sc.prm;
+    const int ng = prm.n_grid;
     const float dt = a.dt;
     // slots left empty by particles that moved to another bin carry pid -1
     const bool act = act_slot && __float_as_int(q.pid) >= 0;
@@ -310,7 +381,7 @@ __device__ __forceinline__ void g2p2g_particle(const SubstepArgs& a, bool act_sl
     for (int d = 0; d < 3; ++d) x[d] = x0[d] + u[d];
     AxisWeights A;
     axis_weights(x, prm.inv_dx, A);
-    const bool live = act && stencil_ok(A.base, a.g.ng);
+    const bool live = act && stencil_ok(A.base, ng);
     float v[3], Cm[9], G[9], tau[9];
     int new_bin = own_bin;
     int mode = 0;  // next-substep scatter: 0 none, 1 LDS tile, 2 spill grid
@@ -324,8 +395,7 @@ __device__ __forceinline__ void g2p2g_particle(const SubstepArgs& a, bool act_sl
                                         prm.inv_dx, v, Cm, G);
         } else {
             // global gather: same contraction, nodes read from the grid
-            const int ng = a.g.ng;
-            const float4* vb = a.vel + (((int64_t)A.base[0] * ng + A.base[1]) * ng + A.base[2]);
+            const float4* vb = a.vel + sc.node_base + (((int64_t)A.base[0] * ng + A.base[1]) * ng + A.base[2]);
             float4 nodes[27];
 #pragma unroll
             for (int i = 0; i < 3; ++i)
@@ -344,8 +414,8 @@ __device__ __forceinline__ void g2p2g_particle(const SubstepArgs& a, bool act_sl
             x[d] = x0[d] + u[d];
         }
         axis_weights(x, prm.inv_dx, A);     // weights of the next substep's P2G
-        if (stencil_ok(A.base, a.g.ng)) {
-            new_bin = bin_of_base(A.base, a.g.nb);
+        if (stencil_ok(A.base, ng)) {
+            new_bin = sc.bin_base + bin_of_base(A.base, sc.nb);
             const int sx = A.base[0] - (b0[0] - 1), sy = A.base[1] - (b0[1] - 1), sz = A.base[2] - (b0[2] - 1);
             if (tile_acc != nullptr && sx >= 0 && sy >= 0 && sz >= 0 && sx <= kTS - 3 && sy <= kTS - 3 &&
                 sz <= kTS - 3) {
@@ -359,8 +429,8 @@ __device__ __forceinline__ void g2p2g_particle(const SubstepArgs& a, bool act_sl
             atomicAdd(&a.status[2], 1);  // left the grid: frozen from here on
             int bc[3];
 #pragma unroll
-            for (int d = 0; d < 3; ++d) bc[d] = min(max(A.base[d], 0), a.g.ng - 3);
-            new_bin = bin_of_base(bc, a.g.nb);
+            for (int d = 0; d < 3; ++d) bc[d] = min(max(A.base[d], 0), ng - 3);
+            new_bin = sc.bin_base + bin_of_base(bc, sc.nb);
         }
     } else if (act) {
         // frozen particle (stencil outside the grid): copied unchanged
@@ -373,16 +443,17 @@ __device__ __forceinline__ void g2p2g_particle(const SubstepArgs& a, bool act_sl
         }
         int bc[3];
 #pragma unroll
-        for (int d = 0; d < 3; ++d) bc[d] = min(max(A.base[d], 0), a.g.ng - 3);
-        new_bin = bin_of_base(bc, a.g.nb);
+        for (int d = 0; d < 3; ++d) bc[d] = min(max(A.base[d], 0), ng - 3);
+        new_bin = sc.bin_base + bin_of_base(bc, sc.nb);
     }
-    const int64_t dslot = reserve_slot(act, new_bin, own_bin, stay_base, s_stay, a);
-    if (dslot >= 0) store_particle(a.dst, dslot, a.g.cap, x0, u, v, Cm, F, tau, m, vol, pid);
+    int64_t dstride = 0;
+    const int64_t dslot = reserve_slot(act, new_bin, own_bin, stay_base, s_stay, a, sc, &dstride);
+    if (dslot >= 0) store_particle(a.dst, dslot, dstride, x0, u, v, Cm, F, tau, m, vol, pid);
     // P2G of the next substep, after the store so that x0, u, F are dead
     if (mode == 1) {
         scatter_tile(tile_acc, i0, A, m, vol, v, Cm, tau, prm.dx, prm.inv_dx, dt);
     } else if (mode == 2) {
-        scatter_spill(a.acc, a.gflag, a.g, A, m, vol, v, Cm, tau, prm.dx, prm.inv_dx, dt);
+        scatter_spill(a.acc, a.gflag, sc, A, m, vol, v, Cm, tau, dt);
     }
 }
 
@@ -401,18 +472,26 @@ __device__ __forceinline__ void g2p2g_particle(const SubstepArgs& a, bool act_sl
 __global__ void __launch_bounds__(kWG)
     k_grid_bins(float4* __restrict__ acc, float4* __restrict__ vel, int* __restrict__ gflag,
                 const int* __restrict__ stamps, const float4* __restrict__ tiles, int* __restrict__ cnt_nxt,
-                const int* __restrict__ cnt_cur, int* __restrict__ order, BinGeom g, MpmParams prm,
+                const int* __restrict__ cnt_cur, int* __restrict__ order, BinGeom g, SceneTab tab,
                 ColliderSet cs, float time, float dt, int epoch, int clear_only) {
     __shared__ int s_live[27];
     __shared__ int s_any;
     if (blockIdx.x == 0) {
-        if (order) schedule_bins(cnt_cur, g.n_bins, g.cap, epoch, order);
+        if (order) schedule_bins(cnt_cur, g.n_bins, g.max_cap, epoch, order);
         return;
     }
-    const int blk = blockIdx.x - 1;
+    const int blk = blockIdx.x - 1;   // global block (= global bin index)
     const int tid = threadIdx.x;
-    const int nb = g.nb, ng = g.ng;
-    const int bz = blk % nb, by = (blk / nb) % nb, bx = blk / (nb * nb);
+    const SceneDesc sc = scene_of_bin(tab, blk);
+    const int nb = sc.nb, ng = sc.prm.n_grid;
+    const int lblk = blk - sc.bin_base;
+    const int bz = lblk % nb, by = (lblk / nb) % nb, bx = lblk / (nb * nb);
+    acc += sc.node_base;
+    vel += sc.node_base;
+    stamps += sc.bin_base;
+    tiles += (int64_t)sc.bin_base * kTS3;
+    const MpmCollider* cols = tab.dev ? tab.cols + sc.col_base : cs.c;
+    const int n_cols = tab.dev ? sc.n_cols : cs.n;
     if (tid == 0 && cnt_nxt) {
         cnt_nxt[blk] = 0;                 // slots
         cnt_nxt[g.n_bins + 1 + blk] = 0;  // live particles
@@ -463,7 +542,7 @@ __global__ void __launch_bounds__(kWG)
                     sx += tv.x; sy += tv.y; sz += tv.z; sm += tv.w;
                 }
         const float4 a4 = make_float4((float)sx, (float)sy, (float)sz, (float)sm);
-        vel[node] = grid_node_update(a4, x, y, z, ng, prm.dx, dt, prm.gravity, cs, time);
+        vel[node] = grid_node_update(a4, x, y, z, ng, sc.prm.dx, dt, sc.prm.gravity, cols, n_cols, time);
     }
     __syncthreads();
     if (tid == 0 && spill) gflag[blk] = 0;
@@ -495,19 +574,24 @@ __global__ void __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(2, 2))
         const int i = (wg - g.n_bins) * kWG + tid;
         const int b0[3] = {0, 0, 0};
         if ((wg - g.n_bins) * kWG >= n_ovf) return;
-        const int64_t off = slot_off(g, (int64_t)g.n_bins * g.cap + i);
+        const int64_t off = ovf_off(g, i);
         ParticleIn q;
-        load_particle(a.src, off, g.cap, i < n_ovf, q);
-        g2p2g_particle(a, i < n_ovf, off, g.n_bins, -1, nullptr, b0, nullptr, nullptr, q);
+        load_particle(a.src, off, g.ovf_blk, i < n_ovf, q);
+        const SceneDesc sc = scene_of_pid(a.tab, __float_as_int(q.pid));   // the particle's own scene
+        g2p2g_particle(a, sc, i < n_ovf, off, g.ovf_blk, g.n_bins, -1, nullptr, b0, nullptr, nullptr, q);
         return;
     }
     const int bin = (a.order && a.order[g.n_bins] == a.sched_epoch) ? a.order[wg] : xcd_bin(wg, g.n_bins);
     if (bin < 0) return;
-    const int n_in = min(a.cnt_cur[bin], g.cap);
+    const SceneDesc sc = scene_of_bin(a.tab, bin);
+    const int cap = sc.cap;
+    const int n_in = min(a.cnt_cur[bin], cap);
     if (n_in == 0) return;
-    const int nb = g.nb, ng = g.ng;
-    const int b0[3] = {(bin / (nb * nb)) * kBinEdge, ((bin / nb) % nb) * kBinEdge, (bin % nb) * kBinEdge};
-    const int64_t off0 = (int64_t)bin * MPM_NPLANES * g.cap;
+    const int ng = sc.prm.n_grid;
+    int b0[3];
+    bin_origin(bin - sc.bin_base, sc.nb, b0);
+    const float4* svel = a.vel + sc.node_base;
+    const int64_t off0 = bin_off(sc, bin);
     // the run stayers will fill: as many slots as the bin has live particles
     const int n_live = min(a.cnt_cur[g.n_bins + 1 + bin], n_in);
     if (tid == 0) {
@@ -517,7 +601,7 @@ __global__ void __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(2, 2))
     for (int t = tid; t < kTS3; t += kWG) {
         const int x = b0[0] - 1 + t / (kTS * kTS), y = b0[1] - 1 + (t / kTS) % kTS, z = b0[2] - 1 + t % kTS;
         float4 val = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (x >= 0 && y >= 0 && z >= 0 && x < ng && y < ng && z < ng) val = a.vel[((int64_t)x * ng + y) * ng + z];
+        if (x >= 0 && y >= 0 && z >= 0 && x < ng && y < ng && z < ng) val = svel[((int64_t)x * ng + y) * ng + z];
         s_vel[t] = val.x;
         s_vel[kTS3 + t] = val.y;
         s_vel[2 * kTS3 + t] = val.z;
@@ -525,24 +609,24 @@ __global__ void __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(2, 2))
     for (int t = tid; t < 4 * kTS3; t += kWG) s_acc[t] = 0.0;
     __syncthreads();
     // Particles are processed in storage order.  Storage is kept in bank order
-    // (k_bins_interleave after insertion; stayers are written back in
-    // processing order), so each 32-lane half mostly sees distinct bank
-    // classes of its base node.
+    // (k_bins_interleave after insertion and every kSortPeriod substeps;
+    // stayers are written back in processing order), so each 32-lane half
+    // mostly sees distinct bank classes of its base node.
     const int stay_base = s_base;
     ParticleIn q;
-    load_particle(a.src, off0 + tid, g.cap, tid < n_in, q);
+    load_particle(a.src, off0 + tid, cap, tid < n_in, q);
     for (int it = 0; it < n_in; it += kWG) {
         const int i = it + tid;
         const ParticleIn cur = q;
-        if (it + kWG < n_in) load_particle(a.src, off0 + i + kWG, g.cap, i + kWG < n_in, q);  // prefetch
-        g2p2g_particle(a, i < n_in, off0 + i, bin, stay_base, &s_stay, b0, s_vel, s_acc, cur);
+        if (it + kWG < n_in) load_particle(a.src, off0 + i + kWG, cap, i + kWG < n_in, q);  // prefetch
+        g2p2g_particle(a, sc, i < n_in, off0 + i, cap, bin, stay_base, &s_stay, b0, s_vel, s_acc, cur);
     }
     __syncthreads();
     // reserved slots nobody stayed in become holes (pid -1)
-    const int hole_lo = stay_base + s_stay, hole_hi = min(stay_base + n_live, g.cap);
+    const int hole_lo = stay_base + s_stay, hole_hi = min(stay_base + n_live, cap);
     for (int h = hole_lo + tid; h < hole_hi; h += kWG)
-        a.dst[off0 + (int64_t)MPM_P_PID * g.cap + h] = __int_as_float(-1);
-    if (tid == 0) atomicAdd(&a.cnt_nxt[g.n_bins + 1 + bin], max(0, min(s_stay, g.cap - stay_base)));
+        a.dst[off0 + (int64_t)MPM_P_PID * cap + h] = __int_as_float(-1);
+    if (tid == 0) atomicAdd(&a.cnt_nxt[g.n_bins + 1 + bin], max(0, min(s_stay, cap - stay_base)));
     write_tile(s_acc, a.tiles, a.stamps, bin, a.epoch_out);
 }
 
@@ -551,37 +635,36 @@ __global__ void __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(2, 2))
 // the binned state: same tiles as k_g2p2g.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kWG)
-    k_p2g_bins(MpmParams prm, BinGeom g, float dt, const float* __restrict__ planes, const int* __restrict__ cnt,
+    k_p2g_bins(SceneTab tab, BinGeom g, float dt, const float* __restrict__ planes, const int* __restrict__ cnt,
                float* __restrict__ acc, int* __restrict__ gflag, float4* __restrict__ tiles,
                int* __restrict__ stamps, int epoch_out) {
     __shared__ double s_acc[4 * kTS3];
     const int tid = threadIdx.x;
     const int wg = blockIdx.x;
-    const int64_t S = g.cap;
     const bool ovf = wg >= g.n_bins;
     int n_in, bin;
-    int64_t slot0;
+    SceneDesc sc_bin = tab.one;
     if (ovf) {
         n_in = min(cnt[g.n_bins], g.ovf_cap) - (wg - g.n_bins) * kWG;
         n_in = min(max(n_in, 0), kWG);
         bin = g.n_bins;
-        slot0 = (int64_t)g.n_bins * g.cap + (int64_t)(wg - g.n_bins) * kWG;
     } else {
         bin = xcd_bin(wg, g.n_bins);
-        n_in = min(cnt[bin], g.cap);
-        slot0 = (int64_t)bin * g.cap;
+        sc_bin = scene_of_bin(tab, bin);
+        n_in = min(cnt[bin], sc_bin.cap);
     }
     if (n_in == 0) return;
-    const int nb = g.nb;
-    const int b0[3] = {(bin / (nb * nb)) * kBinEdge, ((bin / nb) % nb) * kBinEdge, (bin % nb) * kBinEdge};
+    const int64_t S = ovf ? g.ovf_blk : sc_bin.cap;
+    int b0[3] = {0, 0, 0};
     if (!ovf) {
+        bin_origin(bin - sc_bin.bin_base, sc_bin.nb, b0);
         for (int t = tid; t < 4 * kTS3; t += kWG) s_acc[t] = 0.0;
         __syncthreads();
     }
     for (int it = 0; it < n_in; it += kWG) {
         const int i = it + tid;
         if (i >= n_in) continue;
-        const float* sp = planes + slot_off(g, slot0 + i);
+        const float* sp = planes + (ovf ? ovf_off(g, (wg - g.n_bins) * kWG + i) : bin_off(sc_bin, bin) + i);
         float x[3], v[3], Cm[9], tau[9];
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
@@ -594,15 +677,17 @@ __global__ void __launch_bounds__(kWG)
             tau[k] = sp[(MPM_P_TAU + k) * S];
         }
         const float m = sp[MPM_P_MASS * S], vol = sp[MPM_P_VOL * S];
-        if (__float_as_int(sp[MPM_P_PID * S]) < 0) continue;  // hole
+        const int pid = __float_as_int(sp[MPM_P_PID * S]);
+        if (pid < 0) continue;  // hole
+        const SceneDesc sc = ovf ? scene_of_pid(tab, pid) : sc_bin;
         AxisWeights A;
-        axis_weights(x, prm.inv_dx, A);
-        if (!stencil_ok(A.base, g.ng)) continue;
+        axis_weights(x, sc.prm.inv_dx, A);
+        if (!stencil_ok(A.base, sc.prm.n_grid)) continue;
         const int sx = A.base[0] - (b0[0] - 1), sy = A.base[1] - (b0[1] - 1), sz = A.base[2] - (b0[2] - 1);
         if (!ovf && sx >= 0 && sy >= 0 && sz >= 0 && sx <= kTS - 3 && sy <= kTS - 3 && sz <= kTS - 3) {
-            scatter_tile(s_acc, (sx * kTS + sy) * kTS + sz, A, m, vol, v, Cm, tau, prm.dx, prm.inv_dx, dt);
+            scatter_tile(s_acc, (sx * kTS + sy) * kTS + sz, A, m, vol, v, Cm, tau, sc.prm.dx, sc.prm.inv_dx, dt);
         } else {
-            scatter_spill(acc, gflag, g, A, m, vol, v, Cm, tau, prm.dx, prm.inv_dx, dt);
+            scatter_spill(acc, gflag, sc, A, m, vol, v, Cm, tau, dt);
         }
     }
     if (!ovf) {
@@ -619,8 +704,8 @@ __global__ void __launch_bounds__(kWG)
 // consecutive slots span nearly all 32 classes.  Bins larger than kSortMax
 // and the overflow run are copied as they are.
 __device__ void interleave_bin(const float* __restrict__ src, const int* __restrict__ cnt_src,
-                               float* __restrict__ dst, int* __restrict__ cnt_dst, const BinGeom& g, float inv_dx,
-                               int item) {
+                               float* __restrict__ dst, int* __restrict__ cnt_dst, const BinGeom& g,
+                               const SceneTab& tab, int item) {
     __shared__ float s_stage[kStageFloats];      // planes of the bin being copied
     __shared__ unsigned short s_from[kSortMax];  // source slot of each destination slot
     __shared__ unsigned short s_rank[kSortMax];
@@ -634,22 +719,26 @@ __device__ void interleave_bin(const float* __restrict__ src, const int* __restr
     if (item >= g.n_bins) {  // overflow blocks are copied as they are
         const int k = item - g.n_bins;
         const int n_ovf = min(cnt_src[g.n_bins], g.ovf_cap);
-        off0 = (int64_t)item * MPM_NPLANES * g.cap;
-        n_in = min(max(n_ovf - k * g.cap, 0), g.cap);
+        const int B = g.ovf_blk;
+        off0 = ovf_off(g, k * B);
+        n_in = min(max(n_ovf - k * B, 0), B);
         for (int i = tid; i < n_in; i += kWG)
-            for (int pl = 0; pl < MPM_NPLANES; ++pl) dst[off0 + (int64_t)pl * g.cap + i] = src[off0 + (int64_t)pl * g.cap + i];
+            for (int pl = 0; pl < MPM_NPLANES; ++pl) dst[off0 + (int64_t)pl * B + i] = src[off0 + (int64_t)pl * B + i];
         if (k == 0 && tid == 0) cnt_dst[g.n_bins] = n_ovf;
         return;
     }
     const int bin = item;
-    n_in = min(cnt_src[bin], g.cap);
-    off0 = (int64_t)bin * MPM_NPLANES * g.cap;
-    const int nb = g.nb;
-    const int b0[3] = {(bin / (nb * nb)) * kBinEdge, ((bin / nb) % nb) * kBinEdge, (bin % nb) * kBinEdge};
+    const SceneDesc sc = scene_of_bin(tab, bin);
+    const int cap = sc.cap;
+    n_in = min(cnt_src[bin], cap);
+    off0 = bin_off(sc, bin);
+    const float inv_dx = sc.prm.inv_dx;
+    int b0[3];
+    bin_origin(bin - sc.bin_base, sc.nb, b0);
     if (n_in > kSortMax) {
         // kept in place, holes included (the live count is then an upper bound)
         for (int i = tid; i < n_in; i += kWG)
-            for (int pl = 0; pl < MPM_NPLANES; ++pl) dst[off0 + (int64_t)pl * g.cap + i] = src[off0 + (int64_t)pl * g.cap + i];
+            for (int pl = 0; pl < MPM_NPLANES; ++pl) dst[off0 + (int64_t)pl * cap + i] = src[off0 + (int64_t)pl * cap + i];
         if (tid == 0) { cnt_dst[bin] = n_in; cnt_dst[g.n_bins + 1 + bin] = n_in; }
         return;
     }
@@ -657,11 +746,11 @@ __device__ void interleave_bin(const float* __restrict__ src, const int* __restr
     __syncthreads();
     for (int i = tid; i < n_in; i += kWG) {
         const float* sp = src + off0 + i;
-        if (__float_as_int(sp[(int64_t)MPM_P_PID * g.cap]) < 0) { s_cls[i] = 255; continue; }  // hole
+        if (__float_as_int(sp[(int64_t)MPM_P_PID * cap]) < 0) { s_cls[i] = 255; continue; }  // hole
         int base[3];
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            const float x = sp[(int64_t)(MPM_P_X0 + d) * g.cap] + sp[(int64_t)(MPM_P_U + d) * g.cap];
+            const float x = sp[(int64_t)(MPM_P_X0 + d) * cap] + sp[(int64_t)(MPM_P_U + d) * cap];
             base[d] = min(max((int)(x * inv_dx - 0.5f), b0[d]), b0[d] + kBinEdge - 1);
         }
         const int c = bank_class(base, b0);
@@ -700,32 +789,32 @@ __device__ void interleave_bin(const float* __restrict__ src, const int* __restr
         const int np = min(per, MPM_NPLANES - pl0);
         __syncthreads();  // s_from ready / the previous stage is written out
         for (int j = 0; j < np; ++j) {
-            const float* sp = src + off0 + (int64_t)(pl0 + j) * g.cap;
+            const float* sp = src + off0 + (int64_t)(pl0 + j) * cap;
             for (int i = tid; i < n_in; i += kWG) s_stage[j * n_in + i] = sp[i];
         }
         __syncthreads();
         for (int j = 0; j < np; ++j) {
-            float* dp = dst + off0 + (int64_t)(pl0 + j) * g.cap;
+            float* dp = dst + off0 + (int64_t)(pl0 + j) * cap;
             for (int q = tid; q < (int)n; q += kWG) dp[q] = s_stage[j * n_in + s_from[q]];
         }
     }
 }
 
-// One workgroup per bin, then one per block of `cap` slots of the overflow
+// One workgroup per bin, then one per block of ovf_blk slots of the overflow
 // run.  With a current schedule (schedule_bins, order[n_bins] == epoch) the
 // bins are taken in its order, which spreads them over the XCDs and skips
 // the empty ones (their counts in `cnt_dst` are then already zero);
 // otherwise workgroup b takes bin b.
 __global__ void __launch_bounds__(kWG)
     k_bins_interleave(const float* __restrict__ src, const int* __restrict__ cnt_src, float* __restrict__ dst,
-                      int* __restrict__ cnt_dst, BinGeom g, float inv_dx, const int* __restrict__ order,
+                      int* __restrict__ cnt_dst, BinGeom g, SceneTab tab, const int* __restrict__ order,
                       int sched_epoch) {
     int item = blockIdx.x;
     if (item < g.n_bins && order != nullptr && order[g.n_bins] == sched_epoch) {
         item = order[item];
         if (item < 0) return;
     }
-    interleave_bin(src, cnt_src, dst, cnt_dst, g, inv_dx, item);
+    interleave_bin(src, cnt_src, dst, cnt_dst, g, tab, item);
 }
 
 // fill n ints with v (bin counters; a kernel rather than a memset so that the
@@ -735,109 +824,91 @@ __global__ void k_fill_int(int* __restrict__ p, int n, int v) {
     if (i < n) p[i] = v;
 }
 
-// Insert every particle of a flat state (slot = caller index) into the bins
-// of plane set `cur` by the base cell of x = x0 + u.
+// Insert every particle of a flat state (slot = caller index; in a batch
+// the scenes' particles follow each other) into the bins of plane set `cur`
+// by the base cell of x = x0 + u.
 __global__ void __launch_bounds__(kWG)
-    k_bins_insert(const float* __restrict__ flat, int64_t fstride, int n, float inv_dx, BinGeom g,
+    k_bins_insert(const float* __restrict__ flat, int64_t fstride, int n, SceneTab tab, BinGeom g,
                   float* __restrict__ planes, int* __restrict__ cnt, int* __restrict__ status) {
     const int p = blockIdx.x * kWG + threadIdx.x;
     if (p >= n) return;
+    const SceneDesc sc = scene_of_pid(tab, p);
     float x[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) x[d] = flat[(MPM_P_X0 + d) * fstride + p] + flat[(MPM_P_U + d) * fstride + p];
     int b[3];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) b[d] = min(max((int)(x[d] * inv_dx - 0.5f), 0), g.ng - 3);
-    const int bin = bin_of_base(b, g.nb);
+    for (int d = 0; d < 3; ++d) b[d] = min(max((int)(x[d] * sc.prm.inv_dx - 0.5f), 0), sc.prm.n_grid - 3);
+    const int bin = sc.bin_base + bin_of_base(b, sc.nb);
     const int pos = atomicAdd(&cnt[bin], 1);
-    int64_t s;
-    if (pos < g.cap) {
-        s = (int64_t)bin * g.cap + pos;
+    int64_t off, S;
+    if (pos < sc.cap) {
+        off = bin_off(sc, bin) + pos;
+        S = sc.cap;
     } else {
         const int r = atomicAdd(&cnt[g.n_bins], 1);
         if (r >= g.ovf_cap) {
             atomicAdd(&status[0], 1);
             return;
         }
-        s = (int64_t)g.n_bins * g.cap + r;
+        off = ovf_off(g, r);
+        S = g.ovf_blk;
     }
-    const int64_t off = slot_off(g, s);
-    for (int k = 0; k < MPM_NPLANES; ++k) planes[off + (int64_t)k * g.cap] = flat[k * fstride + p];
+    for (int k = 0; k < MPM_NPLANES; ++k) planes[off + (int64_t)k * S] = flat[k * fstride + p];
 }
 
 // flat[pid] <- every valid slot of plane set `cur`
 __global__ void __launch_bounds__(kWG)
-    k_bins_extract(const float* __restrict__ planes, const int* __restrict__ cnt, BinGeom g,
+    k_bins_extract(const float* __restrict__ planes, const int* __restrict__ cnt, BinGeom g, SceneTab tab,
                    float* __restrict__ flat, int64_t fstride, int n) {
     const int wg = blockIdx.x;
+    const bool ovf = wg >= g.n_bins;
     int n_in;
-    int64_t slot0;
-    if (wg >= g.n_bins) {
+    SceneDesc sc = tab.one;
+    if (ovf) {
         n_in = min(max(min(cnt[g.n_bins], g.ovf_cap) - (wg - g.n_bins) * kWG, 0), kWG);
-        slot0 = (int64_t)g.n_bins * g.cap + (int64_t)(wg - g.n_bins) * kWG;
     } else {
-        n_in = min(cnt[wg], g.cap);
-        slot0 = (int64_t)wg * g.cap;
+        sc = scene_of_bin(tab, wg);
+        n_in = min(cnt[wg], sc.cap);
     }
+    const int64_t S = ovf ? g.ovf_blk : sc.cap;
     for (int i = threadIdx.x; i < n_in; i += kWG) {
-        const int64_t off = slot_off(g, slot0 + i);
-        const int pid = __float_as_int(planes[off + (int64_t)MPM_P_PID * g.cap]);
+        const int64_t off = ovf ? ovf_off(g, (wg - g.n_bins) * kWG + i) : bin_off(sc, wg) + i;
+        const int pid = __float_as_int(planes[off + (int64_t)MPM_P_PID * S]);
         if (pid < 0 || pid >= n) continue;
-        for (int k = 0; k < MPM_NPLANES; ++k) flat[k * fstride + pid] = planes[off + (int64_t)k * g.cap];
+        for (int k = 0; k < MPM_NPLANES; ++k) flat[k * fstride + pid] = planes[off + (int64_t)k * S];
     }
 }
 
 // ===========================================================================
-// C-ABI of the binned path
+// C-ABI of the binned path (one scene) and of batches of scenes
 // ===========================================================================
 namespace {
 
-int check_bins(const MpmParams* prm, const MpmBins* b, BinGeom* g) {
-    int rc = check_grid(prm);
-    if (rc) return rc;
-    if (!b || !b->planes[0] || !b->planes[1] || !b->count) return set_err(-20, "bins: NULL pointer");
-    const int nb = (prm->n_grid + kBinEdge - 1) / kBinEdge;
-    if (b->nb != nb) return set_err(-21, "bins: nb must be ceil(n_grid / 8)");
-    if (b->bin_cap < 1 || b->overflow_cap < 0 || b->overflow_cap % b->bin_cap != 0)
-        return set_err(-22, "bins: bin_cap < 1 or overflow_cap not a multiple of bin_cap");
-    if (b->cur != 0 && b->cur != 1) return set_err(-23, "bins: cur must be 0 or 1");
-    const int64_t need = ((int64_t)nb * nb * nb * b->bin_cap + b->overflow_cap) * MPM_NPLANES;
-    if (b->stride < need) return set_err(-24, "bins: stride < (nb^3 * bin_cap + overflow_cap) * MPM_NPLANES");
-    g->ng = prm->n_grid;
-    g->nb = nb;
-    g->n_bins = nb * nb * nb;
-    g->cap = b->bin_cap;
-    g->ovf_cap = b->overflow_cap;
-    g->stride = b->stride;
-    return 0;
-}
-
 struct WsLayout {
     int64_t off_status;
-    float4* acc;     // spill grid
-    float4* vel;     // grid velocities (v, m)
-    int* gflag;      // spill flags per grid block
-    int* status;     // 64 counters
-    int* stamps;     // epoch of each bin's tile
-    int* order;      // k_g2p2g schedule: nb^3 bins + its epoch; then the sort queue
-    float4* tiles;   // nb^3 tiles of kTS^3 nodes
+    float4* acc;       // spill grid (all scenes' nodes)
+    float4* vel;       // grid velocities (v, m)
+    int* gflag;        // spill flags per grid block
+    int* status;       // 64 counters
+    int* stamps;       // epoch of each bin's tile
+    int* order;        // k_g2p2g schedule: one entry per bin + its epoch
+    float4* tiles;     // one tile of kTS^3 nodes per bin
+    SceneDesc* table;  // batch: scene table
+    MpmCollider* cols; // batch: colliders
     int64_t bytes;
 };
 
 int64_t align256(int64_t b) { return ((b + 255) / 256) * 256; }
 
-WsLayout ws_layout(const MpmParams* prm, void* base) {
-    const int64_t ng = prm->n_grid;
-    const int64_t nn = ng * ng * ng;
-    const int64_t nb = (ng + kBinEdge - 1) / kBinEdge;
-    const int64_t nbins = nb * nb * nb;
+WsLayout ws_layout(int64_t nodes, int64_t nbins, int n_scenes, int n_cols, void* base) {
     char* p = (char*)base;
     WsLayout w;
     int64_t off = 0;
     w.acc = (float4*)(p + off);
-    off += align256(nn * 16);
+    off += align256(nodes * 16);
     w.vel = (float4*)(p + off);
-    off += align256(nn * 16);
+    off += align256(nodes * 16);
     w.gflag = (int*)(p + off);
     off += align256(nbins * 4);
     w.status = (int*)(p + off);
@@ -849,20 +920,224 @@ WsLayout ws_layout(const MpmParams* prm, void* base) {
     off += align256((nbins + 2) * 4);
     w.tiles = (float4*)(p + off);
     off += align256(nbins * kTS3 * 16);
+    w.table = (SceneDesc*)(p + off);
+    off += align256((int64_t)n_scenes * (int64_t)sizeof(SceneDesc));
+    w.cols = (MpmCollider*)(p + off);
+    off += align256((int64_t)n_cols * (int64_t)sizeof(MpmCollider));
     w.bytes = off;
     return w;
 }
 
+int64_t bins_of_grid(int ng) {
+    const int64_t nb = (ng + kBinEdge - 1) / kBinEdge;
+    return nb * nb * nb;
+}
+
+// everything a launch needs
+struct Ctx {
+    BinGeom g;
+    SceneTab tab;
+    WsLayout w;
+    ColliderSet cs;   // the single scene's colliders
+};
+
+// n_slots: slots of all bins; max_cap: the largest slots per bin
+int check_bins(const MpmBins* b, int64_t n_bins, int64_t n_slots, int max_cap, BinGeom* g) {
+    if (!b || !b->planes[0] || !b->planes[1] || !b->count) return set_err(-20, "bins: NULL pointer");
+    if (b->bin_cap < 1 || b->overflow_cap < 0 || b->overflow_cap % b->bin_cap != 0)
+        return set_err(-22, "bins: bin_cap < 1 or overflow_cap not a multiple of bin_cap");
+    if (b->cur != 0 && b->cur != 1) return set_err(-23, "bins: cur must be 0 or 1");
+    if (n_slots + b->overflow_cap > 0x7fffffffLL) return set_err(-28, "bins: more than 2^31-1 slots");
+    const int64_t need = (n_slots + b->overflow_cap) * MPM_NPLANES;
+    if (b->stride < need) return set_err(-24, "bins: stride < (bin slots + overflow_cap) * MPM_NPLANES");
+    g->n_bins = (int)n_bins;
+    g->ovf_cap = b->overflow_cap;
+    g->ovf_blk = b->bin_cap;
+    g->max_cap = max_cap;
+    g->ovf_off = n_slots * MPM_NPLANES;
+    g->stride = b->stride;
+    return 0;
+}
+
+int ctx_single(const MpmParams* prm, const MpmBins* bins, void* workspace, const MpmCollider* colliders,
+               int n_colliders, bool need_material, Ctx* c) {
+    int rc = check_grid(prm);
+    if (rc) return rc;
+    if (need_material && (rc = check_params(prm))) return rc;
+    const int nb = (prm->n_grid + kBinEdge - 1) / kBinEdge;
+    if (bins && bins->nb != nb) return set_err(-21, "bins: nb must be ceil(n_grid / 8)");
+    const int64_t n_bins = bins_of_grid(prm->n_grid);
+    if (bins && (rc = check_bins(bins, n_bins, n_bins * bins->bin_cap, bins->bin_cap, &c->g))) return rc;
+    if ((rc = make_collider_set(colliders, n_colliders, &c->cs))) return rc;
+    c->tab.one.prm = *prm;
+    c->tab.one.nb = nb;
+    c->tab.one.bin_base = 0;
+    c->tab.one.col_base = 0;
+    c->tab.one.n_cols = 0;
+    c->tab.one.cap = bins ? bins->bin_cap : 0;
+    c->tab.one.pad = 0;
+    c->tab.one.node_base = 0;
+    c->tab.one.pid_base = 0;
+    c->tab.one.slot_base = 0;
+    c->tab.dev = nullptr;
+    c->tab.n = 1;
+    c->tab.cols = nullptr;
+    const int64_t ng = prm->n_grid;
+    c->w = ws_layout(ng * ng * ng, bins_of_grid(prm->n_grid), 0, 0, workspace);
+    return 0;
+}
+
+int ctx_batch(const MpmBatch* batch, const MpmBins* bins, Ctx* c) {
+    if (!batch || !batch->workspace || batch->n_scenes < 1) return set_err(-40, "batch: not set up");
+    int rc;
+    if (bins && (rc = check_bins(bins, batch->n_bins, batch->n_slots, batch->max_cap, &c->g))) return rc;
+    make_collider_set(nullptr, 0, &c->cs);
+    c->w = ws_layout(batch->n_nodes, batch->n_bins, batch->n_scenes, batch->n_colliders, batch->workspace);
+    c->tab.dev = c->w.table;
+    c->tab.n = batch->n_scenes;
+    c->tab.cols = c->w.cols;
+    memset(&c->tab.one, 0, sizeof(SceneDesc));
+    return 0;
+}
+
 unsigned ovf_wgs(const BinGeom& g) { return (unsigned)((g.ovf_cap + kWG - 1) / kWG); }
 
-// k_bins_interleave over every bin and overflow block (order: a schedule
-// or nullptr).
-int launch_sort(const BinGeom& g, const float* src, const int* cnt_src, float* dst, int* cnt_dst, float inv_dx,
-                const int* order, int sched_epoch, hipStream_t s) {
-    const unsigned n_items = (unsigned)(g.n_bins + g.ovf_cap / g.cap);
-    hipLaunchKernelGGL(k_bins_interleave, dim3(n_items), dim3(kWG), 0, s, src, cnt_src, dst, cnt_dst, g, inv_dx,
+int* counts(const MpmBins* b, const BinGeom& g, int set) { return b->count + (int64_t)set * (2 * g.n_bins + 1); }
+
+// k_bins_interleave over every bin and overflow block (order: a schedule or
+// nullptr)
+int launch_sort(const Ctx& c, const float* src, const int* cnt_src, float* dst, int* cnt_dst, const int* order,
+                int sched_epoch, hipStream_t s) {
+    const unsigned n_items = (unsigned)(c.g.n_bins + c.g.ovf_cap / c.g.ovf_blk);
+    hipLaunchKernelGGL(k_bins_interleave, dim3(n_items), dim3(kWG), 0, s, src, cnt_src, dst, cnt_dst, c.g, c.tab,
                        order, sched_epoch);
     return check_launch("k_bins_interleave");
+}
+
+int op_insert(const Ctx& c, const MpmState* flat, const MpmBins* bins, hipStream_t s) {
+    if (!flat || !flat->planes || flat->n < 0 || flat->stride < flat->n) return set_err(-25, "flat state invalid");
+    if (flat->n > 0x7fffffffLL) return set_err(-6, "more than 2^31-1 particles");
+    int rc;
+    // insert into the spare plane set, then copy into `cur` in bank order
+    const int spare = 1 - bins->cur;
+    int* cnt_spare = counts(bins, c.g, spare);
+    int* cnt = counts(bins, c.g, bins->cur);
+    const int cb = 2 * c.g.n_bins + 1;
+    hipLaunchKernelGGL(k_fill_int, dim3((cb + kWG - 1) / kWG), dim3(kWG), 0, s, cnt_spare, cb, 0);
+    if ((rc = check_launch("k_fill_int"))) return rc;
+    hipLaunchKernelGGL(k_fill_int, dim3((cb + kWG - 1) / kWG), dim3(kWG), 0, s, cnt, cb, 0);
+    if ((rc = check_launch("k_fill_int"))) return rc;
+    // new counts: the current k_g2p2g schedule no longer applies
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(kWG), 0, s, c.w.order + c.g.n_bins, 1, (int)0x80000000);
+    if ((rc = check_launch("k_fill_int"))) return rc;
+    if (flat->n == 0) return 0;
+    hipLaunchKernelGGL(k_bins_insert, dim3((unsigned)((flat->n + kWG - 1) / kWG)), dim3(kWG), 0, s, flat->planes,
+                       flat->stride, (int)flat->n, c.tab, c.g, bins->planes[spare], cnt_spare, c.w.status);
+    if ((rc = check_launch("k_bins_insert"))) return rc;
+    return launch_sort(c, bins->planes[spare], cnt_spare, bins->planes[bins->cur], cnt, nullptr, 0, s);
+}
+
+int op_extract(const Ctx& c, const MpmBins* bins, const MpmState* flat, hipStream_t s) {
+    if (!flat || !flat->planes || flat->n < 0 || flat->stride < flat->n) return set_err(-25, "flat state invalid");
+    hipLaunchKernelGGL(k_bins_extract, dim3(c.g.n_bins + ovf_wgs(c.g)), dim3(kWG), 0, s, bins->planes[bins->cur],
+                       (const int*)counts(bins, c.g, bins->cur), c.g, c.tab, flat->planes, flat->stride,
+                       (int)flat->n);
+    return check_launch("k_bins_extract");
+}
+
+int op_clear(const Ctx& c, MpmBins* bins, hipStream_t s) {
+    hipLaunchKernelGGL(k_grid_bins, dim3(c.g.n_bins + 1), dim3(kWG), 0, s, c.w.acc, c.w.vel, c.w.gflag,
+                       (const int*)c.w.stamps, (const float4*)c.w.tiles, (int*)nullptr, (const int*)nullptr,
+                       (int*)nullptr, c.g, c.tab, c.cs, 0.f, 0.f, bins->epoch, 1);
+    int rc = check_launch("k_grid_bins(clear)");
+    if (rc) return rc;
+    bins->epoch += 1;  // the pending tiles are no longer current
+    return 0;
+}
+
+int op_p2g(const Ctx& c, MpmBins* bins, float dt, hipStream_t s) {
+    const int epoch_out = bins->epoch + 1;
+    hipLaunchKernelGGL(k_p2g_bins, dim3(c.g.n_bins + ovf_wgs(c.g)), dim3(kWG), 0, s, c.tab, c.g, dt,
+                       bins->planes[bins->cur], (const int*)counts(bins, c.g, bins->cur), (float*)c.w.acc, c.w.gflag,
+                       c.w.tiles, c.w.stamps, epoch_out);
+    int rc = check_launch("k_p2g_bins");
+    if (rc) return rc;
+    bins->epoch = epoch_out;
+    return 0;
+}
+
+int op_grid(const Ctx& c, const MpmBins* bins, float time, float dt, hipStream_t s) {
+    hipLaunchKernelGGL(k_grid_bins, dim3(c.g.n_bins + 1), dim3(kWG), 0, s, c.w.acc, c.w.vel, c.w.gflag,
+                       (const int*)c.w.stamps, (const float4*)c.w.tiles, counts(bins, c.g, 1 - bins->cur),
+                       (const int*)counts(bins, c.g, bins->cur), c.w.order, c.g, c.tab, c.cs, time, dt, bins->epoch,
+                       0);
+    return check_launch("k_grid_bins");
+}
+
+int op_g2p2g(const Ctx& c, MpmBins* bins, float dt, hipStream_t s) {
+    const int cur = bins->cur, nxt = 1 - cur;
+    SubstepArgs a;
+    a.tab = c.tab;
+    a.g = c.g;
+    a.dt = dt;
+    a.epoch_out = bins->epoch + 1;
+    a.src = bins->planes[cur];
+    a.dst = bins->planes[nxt];
+    a.cnt_cur = counts(bins, c.g, cur);
+    a.cnt_nxt = counts(bins, c.g, nxt);
+    a.vel = c.w.vel;
+    a.acc = (float*)c.w.acc;
+    a.gflag = c.w.gflag;
+    a.status = c.w.status;
+    a.tiles = c.w.tiles;
+    a.stamps = c.w.stamps;
+    a.order = c.w.order;
+    a.sched_epoch = bins->epoch;
+    hipLaunchKernelGGL(k_g2p2g, dim3(c.g.n_bins + ovf_wgs(c.g)), dim3(kWG), 0, s, a);
+    int rc = check_launch("k_g2p2g");
+    if (rc) return rc;
+    bins->epoch = a.epoch_out;
+    bins->cur = nxt;
+    return 0;
+}
+
+int op_sort(const Ctx& c, MpmBins* bins, hipStream_t s) {
+    const int cur = bins->cur, nxt = 1 - cur;
+    int* cnt_cur = counts(bins, c.g, cur);
+    // With the schedule of this epoch's grid update (checked on the device),
+    // only scheduled bins are copied; that grid update has zeroed cnt_nxt, so
+    // the others read as empty.  Without one, every bin is copied.
+    int rc = launch_sort(c, bins->planes[cur], cnt_cur, bins->planes[nxt], counts(bins, c.g, nxt), c.w.order,
+                         bins->epoch, s);
+    if (rc) return rc;
+    // the old set becomes the fused kernel's output: its counters start at zero
+    const int cb = 2 * c.g.n_bins + 1;
+    hipLaunchKernelGGL(k_fill_int, dim3((cb + kWG - 1) / kWG), dim3(kWG), 0, s, cnt_cur, cb, 0);
+    if ((rc = check_launch("k_fill_int"))) return rc;
+    bins->cur = nxt;
+    bins->since_sort = 0;
+    return 0;
+}
+
+int op_substep(const Ctx& c, MpmBins* bins, float time, float dt, hipStream_t s) {
+    int rc = op_grid(c, bins, time, dt, s);
+    if (rc) return rc;
+    // particles drift across cells and out of bank order; restore it now and
+    // then, between the grid update (which schedules the bins) and the fused
+    // kernel, which reads the sorted set
+    if (++bins->since_sort >= kSortPeriod && (rc = op_sort(c, bins, s))) return rc;
+    return op_g2p2g(c, bins, dt, s);
+}
+
+int op_run(const Ctx& c, MpmBins* bins, double time, float dt, int n_substeps, hipStream_t s) {
+    if (n_substeps < 0) return set_err(-26, "n_substeps < 0");
+    double t = time;
+    for (int k = 0; k < n_substeps; ++k) {
+        int rc = op_substep(c, bins, (float)t, dt, s);
+        if (rc) return rc;
+        t = t + (double)dt;
+    }
+    return 0;
 }
 
 }  // namespace
@@ -871,168 +1146,80 @@ extern "C" {
 
 int64_t mpm_bins_workspace_bytes(const MpmParams* prm) {
     if (check_grid(prm)) return -1;
-    return ws_layout(prm, nullptr).bytes;
+    const int64_t ng = prm->n_grid;
+    return ws_layout(ng * ng * ng, bins_of_grid(prm->n_grid), 0, 0, nullptr).bytes;
 }
 
 int64_t mpm_bins_status_offset(const MpmParams* prm) {
     if (check_grid(prm)) return -1;
-    return ws_layout(prm, nullptr).off_status;
+    const int64_t ng = prm->n_grid;
+    return ws_layout(ng * ng * ng, bins_of_grid(prm->n_grid), 0, 0, nullptr).off_status;
 }
 
 int mpm_bins_insert(const MpmParams* prm, const MpmState* flat, const MpmBins* bins, void* workspace,
                     void* stream) {
-    BinGeom g;
-    int rc = check_bins(prm, bins, &g);
+    Ctx c;
+    int rc = ctx_single(prm, bins, workspace, nullptr, 0, false, &c);
     if (rc) return rc;
-    if (!flat || !flat->planes || flat->n < 0 || flat->stride < flat->n) return set_err(-25, "flat state invalid");
     if (!workspace) return set_err(-10, "workspace is NULL");
-    WsLayout w = ws_layout(prm, workspace);
-    hipStream_t s = (hipStream_t)stream;
-    // insert into the spare plane set, then copy into `cur` in bank order
-    const int spare = 1 - bins->cur;
-    int* cnt_spare = bins->count + (int64_t)spare * (2 * g.n_bins + 1);
-    int* cnt = bins->count + (int64_t)bins->cur * (2 * g.n_bins + 1);
-    const int cb = 2 * g.n_bins + 1;
-    hipLaunchKernelGGL(k_fill_int, dim3((cb + kWG - 1) / kWG), dim3(kWG), 0, s, cnt_spare, cb, 0);
-    if ((rc = check_launch("k_fill_int"))) return rc;
-    hipLaunchKernelGGL(k_fill_int, dim3((cb + kWG - 1) / kWG), dim3(kWG), 0, s, cnt, cb, 0);
-    if ((rc = check_launch("k_fill_int"))) return rc;
-    // new counts: the current k_g2p2g schedule no longer applies
-    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(kWG), 0, s, w.order + g.n_bins, 1, (int)0x80000000);
-    if ((rc = check_launch("k_fill_int"))) return rc;
-    if (flat->n == 0) return 0;
-    hipLaunchKernelGGL(k_bins_insert, dim3((unsigned)((flat->n + kWG - 1) / kWG)), dim3(kWG), 0, s, flat->planes,
-                       flat->stride, (int)flat->n, prm->inv_dx, g, bins->planes[spare], cnt_spare, w.status);
-    if ((rc = check_launch("k_bins_insert"))) return rc;
-    return launch_sort(g, bins->planes[spare], cnt_spare, bins->planes[bins->cur], cnt, prm->inv_dx, nullptr, 0, s);
+    return op_insert(c, flat, bins, (hipStream_t)stream);
 }
 
 int mpm_bins_extract(const MpmParams* prm, const MpmBins* bins, const MpmState* flat, void* stream) {
-    BinGeom g;
-    int rc = check_bins(prm, bins, &g);
+    Ctx c;
+    int rc = ctx_single(prm, bins, nullptr, nullptr, 0, false, &c);
     if (rc) return rc;
-    if (!flat || !flat->planes || flat->n < 0 || flat->stride < flat->n) return set_err(-25, "flat state invalid");
-    const int* cnt = bins->count + (int64_t)bins->cur * (2 * g.n_bins + 1);
-    hipLaunchKernelGGL(k_bins_extract, dim3(g.n_bins + ovf_wgs(g)), dim3(kWG), 0, (hipStream_t)stream,
-                       bins->planes[bins->cur], cnt, g, flat->planes, flat->stride, (int)flat->n);
-    return check_launch("k_bins_extract");
+    return op_extract(c, bins, flat, (hipStream_t)stream);
 }
 
 int mpm_bins_clear_grid(const MpmParams* prm, MpmBins* bins, void* workspace, void* stream) {
-    BinGeom g;
-    int rc = check_bins(prm, bins, &g);
+    Ctx c;
+    int rc = ctx_single(prm, bins, workspace, nullptr, 0, false, &c);
     if (rc) return rc;
     if (!workspace) return set_err(-10, "workspace is NULL");
-    WsLayout w = ws_layout(prm, workspace);
-    ColliderSet cs;
-    make_collider_set(nullptr, 0, &cs);
-    hipLaunchKernelGGL(k_grid_bins, dim3(g.n_bins + 1), dim3(kWG), 0, (hipStream_t)stream, w.acc, w.vel, w.gflag,
-                       (const int*)w.stamps, (const float4*)w.tiles, (int*)nullptr, (const int*)nullptr,
-                       (int*)nullptr, g, *prm, cs, 0.f, 0.f, bins->epoch, 1);
-    if ((rc = check_launch("k_grid_bins(clear)"))) return rc;
-    bins->epoch += 1;  // the pending tiles are no longer current
-    return 0;
+    return op_clear(c, bins, (hipStream_t)stream);
 }
 
 int mpm_bins_p2g(const MpmParams* prm, MpmBins* bins, void* workspace, float dt, void* stream) {
-    BinGeom g;
-    int rc = check_bins(prm, bins, &g);
+    Ctx c;
+    int rc = ctx_single(prm, bins, workspace, nullptr, 0, true, &c);
     if (rc) return rc;
-    if ((rc = check_params(prm))) return rc;
     if (!workspace) return set_err(-10, "workspace is NULL");
-    WsLayout w = ws_layout(prm, workspace);
-    const int* cnt = bins->count + (int64_t)bins->cur * (2 * g.n_bins + 1);
-    const int epoch_out = bins->epoch + 1;
-    hipLaunchKernelGGL(k_p2g_bins, dim3(g.n_bins + ovf_wgs(g)), dim3(kWG), 0, (hipStream_t)stream, *prm, g, dt,
-                       bins->planes[bins->cur], cnt, (float*)w.acc, w.gflag, w.tiles, w.stamps, epoch_out);
-    if ((rc = check_launch("k_p2g_bins"))) return rc;
-    bins->epoch = epoch_out;
-    return 0;
+    return op_p2g(c, bins, dt, (hipStream_t)stream);
 }
 
 int mpm_bins_grid_update(const MpmParams* prm, const MpmBins* bins, const MpmCollider* colliders,
                          int n_colliders, void* workspace, float time, float dt, void* stream) {
-    BinGeom g;
-    int rc = check_bins(prm, bins, &g);
+    Ctx c;
+    int rc = ctx_single(prm, bins, workspace, colliders, n_colliders, true, &c);
     if (rc) return rc;
-    if ((rc = check_params(prm))) return rc;
     if (!workspace) return set_err(-10, "workspace is NULL");
-    ColliderSet cs;
-    if ((rc = make_collider_set(colliders, n_colliders, &cs))) return rc;
-    WsLayout w = ws_layout(prm, workspace);
-    int* cnt_nxt = bins->count + (int64_t)(1 - bins->cur) * (2 * g.n_bins + 1);
-    const int* cnt_cur = bins->count + (int64_t)bins->cur * (2 * g.n_bins + 1);
-    hipLaunchKernelGGL(k_grid_bins, dim3(g.n_bins + 1), dim3(kWG), 0, (hipStream_t)stream, w.acc, w.vel, w.gflag,
-                       (const int*)w.stamps, (const float4*)w.tiles, cnt_nxt, cnt_cur, w.order, g, *prm, cs,
-                       time, dt, bins->epoch, 0);
-    return check_launch("k_grid_bins");
+    return op_grid(c, bins, time, dt, (hipStream_t)stream);
 }
 
 int mpm_bins_g2p2g(const MpmParams* prm, MpmBins* bins, void* workspace, float dt, void* stream) {
-    BinGeom g;
-    int rc = check_bins(prm, bins, &g);
+    Ctx c;
+    int rc = ctx_single(prm, bins, workspace, nullptr, 0, true, &c);
     if (rc) return rc;
-    if ((rc = check_params(prm))) return rc;
     if (!workspace) return set_err(-10, "workspace is NULL");
-    WsLayout w = ws_layout(prm, workspace);
-    const int cur = bins->cur, nxt = 1 - cur;
-    SubstepArgs a;
-    a.prm = *prm;
-    a.g = g;
-    a.dt = dt;
-    a.epoch_out = bins->epoch + 1;
-    a.src = bins->planes[cur];
-    a.dst = bins->planes[nxt];
-    a.cnt_cur = bins->count + (int64_t)cur * (2 * g.n_bins + 1);
-    a.cnt_nxt = bins->count + (int64_t)nxt * (2 * g.n_bins + 1);
-    a.vel = w.vel;
-    a.acc = (float*)w.acc;
-    a.gflag = w.gflag;
-    a.status = w.status;
-    a.tiles = w.tiles;
-    a.stamps = w.stamps;
-    a.order = w.order;
-    a.sched_epoch = bins->epoch;
-    hipLaunchKernelGGL(k_g2p2g, dim3(g.n_bins + ovf_wgs(g)), dim3(kWG), 0, (hipStream_t)stream, a);
-    if ((rc = check_launch("k_g2p2g"))) return rc;
-    bins->epoch = a.epoch_out;
-    bins->cur = nxt;
-    return 0;
+    return op_g2p2g(c, bins, dt, (hipStream_t)stream);
 }
 
 int mpm_bins_sort(const MpmParams* prm, MpmBins* bins, void* workspace, void* stream) {
-    BinGeom g;
-    int rc = check_bins(prm, bins, &g);
+    Ctx c;
+    int rc = ctx_single(prm, bins, workspace, nullptr, 0, false, &c);
     if (rc) return rc;
     if (!workspace) return set_err(-10, "workspace is NULL");
-    WsLayout w = ws_layout(prm, workspace);
-    const int cur = bins->cur, nxt = 1 - cur;
-    int* cnt_cur = bins->count + (int64_t)cur * (2 * g.n_bins + 1);
-    int* cnt_nxt = bins->count + (int64_t)nxt * (2 * g.n_bins + 1);
-    // With the schedule of this epoch's grid update (checked on the device),
-    // only scheduled bins are copied; that grid update has zeroed cnt_nxt, so
-    // the others read as empty.  Without one, every bin is copied.
-    rc = launch_sort(g, bins->planes[cur], cnt_cur, bins->planes[nxt], cnt_nxt, prm->inv_dx, w.order, bins->epoch,
-                     (hipStream_t)stream);
-    if (rc) return rc;
-    // the old set becomes the fused kernel's output: its counters start at zero
-    const int cb = 2 * g.n_bins + 1;
-    hipLaunchKernelGGL(k_fill_int, dim3((cb + kWG - 1) / kWG), dim3(kWG), 0, (hipStream_t)stream, cnt_cur, cb, 0);
-    if ((rc = check_launch("k_fill_int"))) return rc;
-    bins->cur = nxt;
-    bins->since_sort = 0;
-    return 0;
+    return op_sort(c, bins, (hipStream_t)stream);
 }
 
 int mpm_bins_substep(const MpmParams* prm, MpmBins* bins, const MpmCollider* colliders, int n_colliders,
                      void* workspace, float time, float dt, void* stream) {
-    int rc = mpm_bins_grid_update(prm, bins, colliders, n_colliders, workspace, time, dt, stream);
+    Ctx c;
+    int rc = ctx_single(prm, bins, workspace, colliders, n_colliders, true, &c);
     if (rc) return rc;
-    // particles drift across cells and out of bank order; restore it now and
-    // then, between the grid update (which schedules the bins) and the fused
-    // kernel, which reads the sorted set
-    if (++bins->since_sort >= kSortPeriod && (rc = mpm_bins_sort(prm, bins, workspace, stream))) return rc;
-    return mpm_bins_g2p2g(prm, bins, workspace, dt, stream);
+    if (!workspace) return set_err(-10, "workspace is NULL");
+    return op_substep(c, bins, time, dt, (hipStream_t)stream);
 }
 
 int mpm_bins_profile(const MpmParams* prm, MpmBins* bins, const MpmCollider* colliders, int n_colliders,
@@ -1040,29 +1227,32 @@ int mpm_bins_profile(const MpmParams* prm, MpmBins* bins, const MpmCollider* col
                      float* g2p2g_ms, float* sort_ms) {
     if (!bins || !grid_ms || !g2p2g_ms || !sort_ms) return set_err(-20, "profile: NULL pointer");
     if (n_substeps < 1) return set_err(-26, "n_substeps < 1");
+    Ctx c;
+    int rc = ctx_single(prm, bins, workspace, colliders, n_colliders, true, &c);
+    if (rc) return rc;
+    if (!workspace) return set_err(-10, "workspace is NULL");
     hipStream_t s = (hipStream_t)stream;
     hipEvent_t ev[4];
     for (int k = 0; k < 4; ++k)
         if (hipEventCreate(&ev[k]) != hipSuccess) return set_err(-27, "hipEventCreate failed");
     double t = time;
     float acc_grid = 0.f, acc_g2p2g = 0.f, acc_sort = 0.f;
-    int rc = 0;
     for (int k = 0; k < n_substeps && rc == 0; ++k) {
         (void)hipEventRecord(ev[0], s);
-        rc = mpm_bins_grid_update(prm, bins, colliders, n_colliders, workspace, (float)t, dt, stream);
+        rc = op_grid(c, bins, (float)t, dt, s);
         (void)hipEventRecord(ev[1], s);
-        if (rc == 0 && ++bins->since_sort >= kSortPeriod) rc = mpm_bins_sort(prm, bins, workspace, stream);
+        if (rc == 0 && ++bins->since_sort >= kSortPeriod) rc = op_sort(c, bins, s);
         (void)hipEventRecord(ev[2], s);
-        if (rc == 0) rc = mpm_bins_g2p2g(prm, bins, workspace, dt, stream);
+        if (rc == 0) rc = op_g2p2g(c, bins, dt, s);
         (void)hipEventRecord(ev[3], s);
         (void)hipEventSynchronize(ev[3]);
-        float a = 0.f, b = 0.f, c = 0.f;
+        float a = 0.f, b = 0.f, cc = 0.f;
         (void)hipEventElapsedTime(&a, ev[0], ev[1]);
-        (void)hipEventElapsedTime(&c, ev[1], ev[2]);
+        (void)hipEventElapsedTime(&cc, ev[1], ev[2]);
         (void)hipEventElapsedTime(&b, ev[2], ev[3]);
         acc_grid += a;
         acc_g2p2g += b;
-        acc_sort += c;
+        acc_sort += cc;
         t = t + (double)dt;
     }
     for (int k = 0; k < 4; ++k) (void)hipEventDestroy(ev[k]);
@@ -1074,15 +1264,128 @@ int mpm_bins_profile(const MpmParams* prm, MpmBins* bins, const MpmCollider* col
 
 int mpm_bins_run(const MpmParams* prm, MpmBins* bins, const MpmCollider* colliders, int n_colliders,
                  void* workspace, double time, float dt, int n_substeps, void* stream) {
-    if (!bins) return set_err(-20, "bins: NULL pointer");
-    if (n_substeps < 0) return set_err(-26, "n_substeps < 0");
-    double t = time;
-    for (int k = 0; k < n_substeps; ++k) {
-        int rc = mpm_bins_substep(prm, bins, colliders, n_colliders, workspace, (float)t, dt, stream);
-        if (rc) return rc;
-        t = t + (double)dt;
+    Ctx c;
+    int rc = ctx_single(prm, bins, workspace, colliders, n_colliders, true, &c);
+    if (rc) return rc;
+    if (!workspace) return set_err(-10, "workspace is NULL");
+    return op_run(c, bins, time, dt, n_substeps, (hipStream_t)stream);
+}
+
+// ---- batches ----------------------------------------------------------------
+
+int64_t mpm_batch_workspace_bytes(const MpmScene* scenes, int n_scenes) {
+    if (!scenes || n_scenes < 1) return -1;
+    int64_t nodes = 0, nbins = 0, ncols = 0;
+    for (int i = 0; i < n_scenes; ++i) {
+        if (check_grid(&scenes[i].params)) return -1;
+        const int64_t ng = scenes[i].params.n_grid;
+        nodes += ng * ng * ng;
+        nbins += bins_of_grid(scenes[i].params.n_grid);
+        ncols += scenes[i].n_colliders;
     }
+    return ws_layout(nodes, nbins, n_scenes, (int)ncols, nullptr).bytes;
+}
+
+int mpm_batch_setup(const MpmScene* scenes, int n_scenes, const MpmCollider* colliders, void* workspace,
+                    MpmBatch* batch) {
+    if (!scenes || n_scenes < 1) return set_err(-41, "batch: no scenes");
+    if (!workspace || !batch) return set_err(-42, "batch: NULL workspace or batch");
+    SceneDesc* tab = (SceneDesc*)malloc(sizeof(SceneDesc) * (size_t)n_scenes);
+    if (!tab) return set_err(-43, "batch: host allocation failed");
+    int64_t nodes = 0, nbins = 0, pids = 0, slots = 0;
+    int ncols = 0, max_cap = 0;
+    for (int i = 0; i < n_scenes; ++i) {
+        const MpmScene& sc = scenes[i];
+        int rc = check_params(&sc.params);
+        if (!rc && (sc.n_particles < 0 || sc.n_colliders < 0 || sc.n_colliders > MPM_MAX_COLLIDERS ||
+                    sc.bin_cap < 1))
+            rc = set_err(-44, "batch: scene with n_particles < 0, bin_cap < 1 or n_colliders out of [0, 16]");
+        if (rc) {
+            free(tab);
+            return rc;
+        }
+        const int64_t ng = sc.params.n_grid;
+        tab[i].prm = sc.params;
+        tab[i].nb = (int)((ng + kBinEdge - 1) / kBinEdge);
+        tab[i].bin_base = (int)nbins;
+        tab[i].col_base = ncols;
+        tab[i].n_cols = sc.n_colliders;
+        tab[i].cap = sc.bin_cap;
+        tab[i].pad = 0;
+        tab[i].node_base = nodes;
+        tab[i].pid_base = pids;
+        tab[i].slot_base = slots * MPM_NPLANES;
+        nodes += ng * ng * ng;
+        nbins += bins_of_grid(sc.params.n_grid);
+        pids += sc.n_particles;
+        slots += bins_of_grid(sc.params.n_grid) * sc.bin_cap;
+        ncols += sc.n_colliders;
+        max_cap = sc.bin_cap > max_cap ? sc.bin_cap : max_cap;
+    }
+    if (ncols > 0 && !colliders) {
+        free(tab);
+        return set_err(-45, "batch: colliders is NULL");
+    }
+    if (nbins > 0x3fffffff || pids > 0x7fffffffLL || slots > 0x7fffffffLL) {
+        free(tab);
+        return set_err(-46, "batch: too many bins or particles");
+    }
+    WsLayout w = ws_layout(nodes, nbins, n_scenes, ncols, workspace);
+    hipError_t e = hipMemcpy(w.table, tab, sizeof(SceneDesc) * (size_t)n_scenes, hipMemcpyHostToDevice);
+    free(tab);
+    if (e == hipSuccess && ncols > 0)
+        e = hipMemcpy(w.cols, colliders, sizeof(MpmCollider) * (size_t)ncols, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return set_err(-47, "batch: copying the scene table failed");
+    batch->n_scenes = n_scenes;
+    batch->n_bins = (int32_t)nbins;
+    batch->n_colliders = ncols;
+    batch->n_particles = pids;
+    batch->n_nodes = nodes;
+    batch->n_slots = slots;
+    batch->max_cap = max_cap;
+    batch->workspace = workspace;
     return 0;
+}
+
+int64_t mpm_batch_status_offset(const MpmBatch* batch) {
+    if (!batch || batch->n_scenes < 1) return -1;
+    return ws_layout(batch->n_nodes, batch->n_bins, batch->n_scenes, batch->n_colliders, nullptr).off_status;
+}
+
+int mpm_batch_insert(const MpmBatch* batch, const MpmState* flat, const MpmBins* bins, void* stream) {
+    Ctx c;
+    int rc = ctx_batch(batch, bins, &c);
+    if (rc) return rc;
+    if (flat && flat->n != batch->n_particles) return set_err(-48, "batch: flat state size != batch particles");
+    return op_insert(c, flat, bins, (hipStream_t)stream);
+}
+
+int mpm_batch_extract(const MpmBatch* batch, const MpmBins* bins, const MpmState* flat, void* stream) {
+    Ctx c;
+    int rc = ctx_batch(batch, bins, &c);
+    if (rc) return rc;
+    return op_extract(c, bins, flat, (hipStream_t)stream);
+}
+
+int mpm_batch_clear_grid(const MpmBatch* batch, MpmBins* bins, void* stream) {
+    Ctx c;
+    int rc = ctx_batch(batch, bins, &c);
+    if (rc) return rc;
+    return op_clear(c, bins, (hipStream_t)stream);
+}
+
+int mpm_batch_p2g(const MpmBatch* batch, MpmBins* bins, float dt, void* stream) {
+    Ctx c;
+    int rc = ctx_batch(batch, bins, &c);
+    if (rc) return rc;
+    return op_p2g(c, bins, dt, (hipStream_t)stream);
+}
+
+int mpm_batch_run(const MpmBatch* batch, MpmBins* bins, double time, float dt, int n_substeps, void* stream) {
+    Ctx c;
+    int rc = ctx_batch(batch, bins, &c);
+    if (rc) return rc;
+    return op_run(c, bins, time, dt, n_substeps, (hipStream_t)stream);
 }
 
 }  // extern "C"

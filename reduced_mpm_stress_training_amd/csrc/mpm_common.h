@@ -29,8 +29,8 @@ int make_collider_set(const MpmCollider* cols, int n, ColliderSet* cs);
 //     zeroed axis by axis.
 // Returns (v, m); v = 0 where m <= 1e-15.
 __device__ __forceinline__ float4 grid_node_update(float4 acc, int gx, int gy, int gz, int ng, float dx,
-                                                   float dt, const float grav[3], const ColliderSet& cs,
-                                                   float time) {
+                                                   float dt, const float grav[3], const MpmCollider* cols,
+                                                   int n_cols, float time) {
     float4 o = make_float4(0.f, 0.f, 0.f, acc.w);
     if (acc.w > 1e-15f) {
         float im = 1.0f / acc.w;
@@ -38,8 +38,8 @@ __device__ __forceinline__ float4 grid_node_update(float4 acc, int gx, int gy, i
         o.y = acc.y * im + dt * grav[1];
         o.z = acc.z * im + dt * grav[2];
     }
-    for (int c = 0; c < cs.n; ++c) {
-        const MpmCollider& cl = cs.c[c];
+    for (int c = 0; c < n_cols; ++c) {
+        const MpmCollider& cl = cols[c];
         if (!(time >= cl.start_time && time < cl.end_time)) continue;
         if (cl.kind == 0) {
             float o0 = (float)gx * dx - cl.point[0];
@@ -58,6 +58,12 @@ __device__ __forceinline__ float4 grid_node_update(float4 acc, int gx, int gy, i
         }
     }
     return o;
+}
+
+__device__ __forceinline__ float4 grid_node_update(float4 acc, int gx, int gy, int gz, int ng, float dx,
+                                                   float dt, const float grav[3], const ColliderSet& cs,
+                                                   float time) {
+    return grid_node_update(acc, gx, gy, gz, ng, dx, dt, grav, cs.c, cs.n, time);
 }
 
 }  // namespace mpm
