@@ -221,6 +221,34 @@ int mpm_bins_profile(const MpmParams* prm, MpmBins* bins, const MpmCollider* col
 int mpm_bins_run(const MpmParams* prm, MpmBins* bins, const MpmCollider* colliders, int n_colliders,
                  void* workspace, double time, float dt, int n_substeps, void* stream);
 
+/* ---- adjoint of a substep -------------------------------------------------
+ *
+ * No counterpart in the reference (it never records a wp.Tape).  The
+ * reverse-mode derivative of one substep as mpm_step computes it (flat
+ * state): given the adjoint of the state after the substep, the adjoint of
+ * the state before it, and the material parameters' adjoints accumulated.
+ * Adjoint planes (adj_stride floats each, particle order as MpmState):
+ * x (3), v (3), C (9), F (9), tau (9) -- the 33 MPM_ADJ_* planes.  The
+ * weights' base cells and the return-map branch are held fixed. */
+#define MPM_ADJ_X 0
+#define MPM_ADJ_V 3
+#define MPM_ADJ_C 6
+#define MPM_ADJ_F 15
+#define MPM_ADJ_TAU 24
+#define MPM_ADJ_NPLANES 33
+
+/* Workspace of mpm_adjoint_substep (256-byte aligned): 4 grids of n_grid^3
+ * float4. */
+int64_t mpm_adjoint_workspace_bytes(const MpmParams* prm);
+
+/* st: the state S_n before the substep (as mpm_step reads it); time, dt:
+ * that substep's.  adj_next: adjoint of S_{n+1}; adj_cur: receives the
+ * adjoint of S_n (may not alias adj_next).  param_adj[0..2] += adjoints of
+ * mu, lam, alpha (double). */
+int mpm_adjoint_substep(const MpmParams* prm, const MpmState* st, const MpmCollider* colliders, int n_colliders,
+                        void* workspace, float time, float dt, const float* adj_next, float* adj_cur,
+                        int64_t adj_stride, double* param_adj, void* stream);
+
 /* ---- batches of independent scenes -----------------------------------
  *
  * No counterpart in the reference: its driver steps one scene per process

@@ -16,6 +16,9 @@ MAX_COLLIDERS = 16
 # plane indices of the particle state buffer
 P_X0, P_U, P_V, P_C, P_F, P_TAU, P_MASS, P_VOL, P_PID = 0, 3, 6, 9, 18, 27, 36, 37, 38
 NPLANES = 39
+# adjoint planes
+ADJ_X, ADJ_V, ADJ_C, ADJ_F, ADJ_TAU = 0, 3, 6, 15, 24
+ADJ_NPLANES = 33
 
 
 class MpmParams(C.Structure):
@@ -99,6 +102,10 @@ _SIGNATURES = {
     "mpm_batch_p2g": (C.c_int, [C.POINTER(MpmBatch), C.POINTER(MpmBins), C.c_float, C.c_void_p]),
     "mpm_batch_run": (C.c_int, [C.POINTER(MpmBatch), C.POINTER(MpmBins), C.c_double, C.c_float, C.c_int,
                                 C.c_void_p]),
+    "mpm_adjoint_workspace_bytes": (C.c_int64, [C.POINTER(MpmParams)]),
+    "mpm_adjoint_substep": (C.c_int, [C.POINTER(MpmParams), C.POINTER(MpmState), C.POINTER(MpmCollider), C.c_int,
+                                      C.c_void_p, C.c_float, C.c_float, C.c_void_p, C.c_void_p, C.c_int64,
+                                      C.c_void_p, C.c_void_p]),
     "crom_packed_bytes": (C.c_int64, []),
     "crom_acts_bytes": (C.c_int64, [C.c_int64]),
     "crom_pack": (C.c_int, [C.POINTER(CromNet), C.c_void_p, C.c_void_p]),

@@ -1,0 +1,168 @@
+"""Differentiable sand rollouts on MI355X: the forward substeps (mpm_step)
+and their hand-written adjoints (mpm_adjoint_substep, include/mpm_hip.h)
+behind a torch.autograd.Function, with the state checkpointed every K
+substeps and the substeps between checkpoints recomputed on the way back.
+
+The reference never differentiates its solver (it records no wp.Tape,
+engine_stress/mpm_solver_warp.py); the parameters follow its conventions:
+E, nu -> mu, lam (mpm_solver_warp.py:41-42) and the friction angle in degrees
+-> alpha with pi = 3.14159265 (mpm_solver_warp.py:113-116).
+
+    x_final = sand_rollout(v0, E, nu, phi, scene=..., n_substeps=..., dt=...)
+
+is differentiable in v0 (the initial particle velocities, (N, 3)) and in the
+three material scalars.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from . import scenes as S
+
+_PI = 3.14159265
+
+
+def material(E: float, nu: float, phi: float):
+    mu = E / (2.0 * (1.0 + nu))
+    lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu))
+    s = math.sin(phi / 180.0 * _PI)
+    alpha = math.sqrt(2.0 / 3.0) * 2.0 * s / (3.0 - s)
+    return mu, lam, alpha
+
+
+def material_jacobian(E: float, nu: float, phi: float):
+    """d(mu, lam, alpha) / d(E, nu, phi) as a 3x3 list of rows."""
+    dmu_dE = 1.0 / (2.0 * (1.0 + nu))
+    dmu_dnu = -E / (2.0 * (1.0 + nu) ** 2)
+    dlam_dE = nu / ((1.0 + nu) * (1.0 - 2.0 * nu))
+    dlam_dnu = E * (1.0 + 2.0 * nu * nu) / ((1.0 + nu) ** 2 * (1.0 - 2.0 * nu) ** 2)
+    s = math.sin(phi / 180.0 * _PI)
+    dalpha_ds = math.sqrt(2.0 / 3.0) * 2.0 * 3.0 / (3.0 - s) ** 2
+    dalpha_dphi = dalpha_ds * math.cos(phi / 180.0 * _PI) * _PI / 180.0
+    return [[dmu_dE, dmu_dnu, 0.0], [dlam_dE, dlam_dnu, 0.0], [0.0, 0.0, dalpha_dphi]]
+
+
+class Rollout:
+    """n_substeps of a scene on the flat state with checkpoints every K."""
+
+    def __init__(self, scene: S.SandScene, n_substeps: int, dt: float, checkpoint_every: int = 20,
+                 device="cuda:0"):
+        self.scene, self.n_substeps, self.dt, self.K = scene, int(n_substeps), float(dt), int(checkpoint_every)
+        self.device = torch.device(device)
+        self.n = scene.n_particles
+        lib = L.load()
+        cols = S.scene_colliders(scene)
+        self._cols = (L.MpmCollider * max(1, len(cols)))()
+        for i, c in enumerate(cols):
+            self._cols[i].kind, self._cols[i].surface_type = c["kind"], c.get("surface_type", 0)
+            for d in range(3):
+                self._cols[i].point[d] = float(c.get("point", (0.0, 0.0, 0.0))[d])
+                self._cols[i].normal[d] = float(c.get("normal", (0.0, 0.0, 0.0))[d])
+            self._cols[i].start_time, self._cols[i].end_time = float(c["start_time"]), float(c["end_time"])
+        self._ncol = len(cols)
+        P = S.scene_params(scene)
+        self._prm = L.MpmParams()
+        self._prm.n_grid, self._prm.material = int(P["n_grid"]), 2
+        self._prm.dx, self._prm.inv_dx = float(P["dx"]), float(P["inv_dx"])
+        for d in range(3):
+            self._prm.gravity[d] = float(P["gravity"][d])
+        self._ws_step_bytes = int(lib.mpm_workspace_bytes(self._prm, self.n))
+        self._ws_step = torch.zeros(self._ws_step_bytes + 256, dtype=torch.uint8, device=self.device)
+        self._ws_adj = torch.zeros(int(lib.mpm_adjoint_workspace_bytes(self._prm)) + 256, dtype=torch.uint8,
+                                   device=self.device)
+        # initial state (v filled per call)
+        st = S.scene_state(scene, np.float32)
+        planes = torch.zeros((L.NPLANES, self.n), dtype=torch.float32, device=self.device)
+        t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(self.device)
+        planes[L.P_X0:L.P_X0 + 3] = t(st["x0"].T)
+        planes[L.P_F:L.P_F + 9] = t(st["F"].reshape(self.n, 9).T)
+        planes[L.P_MASS] = t(st["mass"])
+        planes[L.P_VOL] = t(st["vol"])
+        planes[L.P_PID] = torch.arange(self.n, dtype=torch.int32, device=self.device).view(torch.float32)
+        self._init = planes
+        self._checkpoints = {}
+
+    @staticmethod
+    def _aligned(buf: torch.Tensor) -> int:
+        return (buf.data_ptr() + 255) & ~255
+
+    def _state(self, planes: torch.Tensor) -> L.MpmState:
+        st = L.MpmState()
+        st.n, st.stride, st.planes = self.n, planes.shape[1], planes.data_ptr()
+        return st
+
+    def _set_material(self, E, nu, phi):
+        self.mu, self.lam, self.alpha = material(E, nu, phi)
+        self._prm.mu, self._prm.lam, self._prm.alpha = self.mu, self.lam, self.alpha
+
+    def _step(self, planes, k):
+        lib = L.load()
+        L.check(lib.mpm_step(self._prm, self._state(planes), self._cols, self._ncol, self._aligned(self._ws_step),
+                             self._ws_step_bytes, float(k * self.dt), self.dt,
+                             torch.cuda.current_stream(self.device).cuda_stream), "mpm_step")
+
+    def forward(self, v0: torch.Tensor, E: float, nu: float, phi: float) -> torch.Tensor:
+        """Positions x = x0 + u after n_substeps; keeps checkpoints."""
+        self._set_material(E, nu, phi)
+        self._material_args = (float(E), float(nu), float(phi))
+        planes = self._init.clone()
+        planes[L.P_V:L.P_V + 3] = v0.to(self.device, torch.float32).t()
+        self._checkpoints = {}
+        for k in range(self.n_substeps):
+            if k % self.K == 0:
+                self._checkpoints[k] = planes.clone()
+            self._step(planes, k)
+        return (planes[L.P_X0:L.P_X0 + 3] + planes[L.P_U:L.P_U + 3]).t().contiguous()
+
+    def backward(self, x_bar: torch.Tensor):
+        """Adjoint of forward() for the position adjoint x_bar (N, 3):
+        returns (v0_bar (N, 3), [E_bar, nu_bar, phi_bar])."""
+        lib = L.load()
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        adj = torch.zeros((L.ADJ_NPLANES, self.n), dtype=torch.float32, device=self.device)
+        adj[L.ADJ_X:L.ADJ_X + 3] = x_bar.to(self.device, torch.float32).t()
+        nxt = torch.empty_like(adj)
+        pb = torch.zeros(3, dtype=torch.float64, device=self.device)
+        for c0 in sorted(self._checkpoints, reverse=True):
+            c1 = min(c0 + self.K, self.n_substeps)
+            states = [self._checkpoints[c0].clone()]
+            for k in range(c0, c1 - 1):
+                s = states[-1].clone()
+                self._step(s, k)
+                states.append(s)
+            for k in range(c1 - 1, c0 - 1, -1):
+                nxt.zero_()
+                L.check(lib.mpm_adjoint_substep(self._prm, self._state(states[k - c0]), self._cols, self._ncol,
+                                                self._aligned(self._ws_adj), float(k * self.dt), self.dt,
+                                                adj.data_ptr(), nxt.data_ptr(), self.n, pb.data_ptr(), stream),
+                        "mpm_adjoint_substep")
+                adj, nxt = nxt, adj
+        v0_bar = adj[L.ADJ_V:L.ADJ_V + 3].t().contiguous()
+        mu_b, lam_b, al_b = (float(v) for v in pb.cpu())
+        J = material_jacobian(*self._material_args)
+        E_b = mu_b * J[0][0] + lam_b * J[1][0]
+        nu_b = mu_b * J[0][1] + lam_b * J[1][1]
+        phi_b = al_b * J[2][2]
+        return v0_bar, [E_b, nu_b, phi_b]
+
+
+class _RolloutFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, v0, params, rollout):
+        E, nu, phi = (float(p) for p in params.detach().cpu())
+        ctx.rollout = rollout
+        return rollout.forward(v0.detach(), E, nu, phi)
+
+    @staticmethod
+    def backward(ctx, x_bar):
+        v0_bar, pb = ctx.rollout.backward(x_bar)
+        return v0_bar, torch.tensor(pb, dtype=torch.float64), None
+
+
+def sand_rollout(v0: torch.Tensor, params: torch.Tensor, rollout: Rollout) -> torch.Tensor:
+    """Differentiable x_final(v0, params = [E, nu, friction angle in deg])."""
+    return _RolloutFn.apply(v0, params, rollout)

@@ -1,0 +1,583 @@
+// mpm_adjoint.hip -- reverse-mode derivative (adjoint) of one explicit
+// MLS/APIC-MPM sand substep (MPM_Simulator_WARP.p2g2p,
+// mpm_solver_warp.py:156-203) on the flat particle state, and its C-ABI
+// (mpm_adjoint_* in include/mpm_hip.h).
+//
+// The reference never differentiates its solver (no wp.Tape is recorded);
+// this is the derivative of the same forward arithmetic.  With the state
+// S_n = (x, v, C, F, tau) of every particle, one substep is
+//   P2G     m_i = sum_p w m_p,  mv_i = sum_p w m_p (v_p + C_p (x_i - x_p))
+//                                      - dt vol_p tau_p grad w   (mpm_utils.py:190-227)
+//   grid    v_i = BC(mv_i / m_i + dt g)                         (:229-237, colliders)
+//   G2P     v' = sum w v_i,  C' = 4 inv_dx^2 sum w v_i (x) (x_i - x_p),
+//           G = sum v_i (x) grad w,  x' = x + dt v',
+//           F' = RM((I + dt G) F),  tau' = stress(F')           (:241-306)
+// and its adjoint runs the three stages backwards:
+//   k_g2p_adj   per particle: stress and return-map adjoints through the
+//               SVD (closed forms of the derivative of isotropic matrix
+//               functions, divided differences where singular values meet),
+//               then scatters the grid-velocity adjoint (float atomics) and
+//               the position / F adjoints of the weights and F_tr;
+//   k_grid_adj  per node: through the collider masks and the normalization;
+//   k_p2g_adj   per particle: gathers v, C, tau and position adjoints.
+// The weights' base cell is piecewise constant and the return-map branch is
+// held fixed (derivative of the branch taken).  Material parameters collect
+// their adjoints (mu, lam, alpha) in double atomics.
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "mpm_common.h"
+#include "mpm_device.h"
+#include "mpm_hip.h"
+
+using namespace mpm;
+
+namespace {
+
+constexpr int kBlk = 256;
+
+// adjoint planes: x (3), v (3), C (9), F (9), tau (9)
+constexpr int kAX = 0, kAV = 3, kAC = 6, kAF = 15, kAT = 24;
+
+struct W3 {             // per-axis quadratic B-spline factors at fx
+    int base[3];
+    float fx[3];
+    float w[3][3], d[3][3], dd[3][3];   // weight, d/dfx, d2/dfx2
+};
+
+__device__ __forceinline__ void weights3(const float x[3], float inv_dx, W3& W) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float gp = x[a] * inv_dx;
+        const int b = (int)(gp - 0.5f);
+        W.base[a] = b;
+        const float fx = gp - (float)b;
+        W.fx[a] = fx;
+        const float wa = 1.5f - fx, wb = fx - 1.0f, wc = fx - 0.5f;
+        W.w[a][0] = 0.5f * wa * wa;
+        W.w[a][1] = 0.75f - wb * wb;
+        W.w[a][2] = 0.5f * wc * wc;
+        W.d[a][0] = fx - 1.5f;
+        W.d[a][1] = -2.0f * (fx - 1.0f);
+        W.d[a][2] = fx - 0.5f;
+        W.dd[a][0] = 1.0f;
+        W.dd[a][1] = -2.0f;
+        W.dd[a][2] = 1.0f;
+    }
+}
+
+// weight, gradient (d/dx_p) and Hessian of node (i, j, k)
+__device__ __forceinline__ void node_w(const W3& W, int i, int j, int k, float inv_dx, float& w, float g[3],
+                                       float H[9]) {
+    const float wx = W.w[0][i], wy = W.w[1][j], wz = W.w[2][k];
+    const float dx = W.d[0][i], dy = W.d[1][j], dz = W.d[2][k];
+    const float ddx = W.dd[0][i], ddy = W.dd[1][j], ddz = W.dd[2][k];
+    w = wx * wy * wz;
+    g[0] = inv_dx * dx * wy * wz;
+    g[1] = inv_dx * wx * dy * wz;
+    g[2] = inv_dx * wx * wy * dz;
+    const float i2 = inv_dx * inv_dx;
+    H[0] = i2 * ddx * wy * wz;
+    H[4] = i2 * wx * ddy * wz;
+    H[8] = i2 * wx * wy * ddz;
+    H[1] = H[3] = i2 * dx * dy * wz;
+    H[2] = H[6] = i2 * dx * wy * dz;
+    H[5] = H[7] = i2 * wx * dy * dz;
+}
+
+// (b - a) / (exp(b) - exp(a)) style divided differences, robust when the
+// arguments meet: (ln s_l - ln s_k) / (s_l - s_k)
+__device__ __forceinline__ float dlog_div(float sk, float sl) {
+    const float r = sl / sk - 1.0f;
+    if (fabsf(r) < 1e-4f) return (1.0f - 0.5f * r + r * r / 3.0f) / sk;
+    return logf(sl / sk) / (sl - sk);
+}
+
+// (s_k^B ... ) helper: (sigma_l^B - sigma_k^B) / (sigma_l - sigma_k)
+__device__ __forceinline__ float dpow_div(float sk, float sl, float B) {
+    const float d = logf(sl / sk);
+    const float den = expm1f(d);
+    if (fabsf(d) < 1e-5f) return B * powf(sk, B - 1.0f) * (1.0f + 0.5f * (B - 1.0f) * d);
+    return powf(sk, B - 1.0f) * expm1f(B * d) / den;
+}
+
+__device__ __forceinline__ void matmul(const float A[9], const float B[9], float C[9]) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) C[i * 3 + j] = A[i * 3] * B[j] + A[i * 3 + 1] * B[3 + j] + A[i * 3 + 2] * B[6 + j];
+}
+__device__ __forceinline__ void matmul_tn(const float A[9], const float B[9], float C[9]) {  // A^T B
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) C[i * 3 + j] = A[i] * B[j] + A[3 + i] * B[3 + j] + A[6 + i] * B[6 + j];
+}
+__device__ __forceinline__ void matmul_nt(const float A[9], const float B[9], float C[9]) {  // A B^T
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) C[i * 3 + j] = A[i * 3] * B[j * 3] + A[i * 3 + 1] * B[j * 3 + 1] + A[i * 3 + 2] * B[j * 3 + 2];
+}
+
+// Adjoint of  F_tr = (I + dt G) F,  F' = RM(F_tr),  tau' = stress(F')
+// (sand_update).  In: G, F, dt, parameters, the adjoints Fb' of F' and Tb of
+// tau'.  Out: Gb, Fb (adjoint of the input F), and the parameters' adjoints.
+__device__ void sand_update_adj(const float G[9], const float F[9], float dt, float mu, float lam, float alpha,
+                                const float Fbn[9], const float Tb[9], float Gb[9], float Fb[9], float& mub,
+                                float& lamb, float& alb) {
+    float A[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) A[k] = dt * G[k];
+    A[0] += 1.f; A[4] += 1.f; A[8] += 1.f;
+    float Ftr[9];
+    matmul(A, F, Ftr);
+    float U[9], s[3], V[9];
+    svd3_rot(Ftr, U, s, V);
+    // forward return map quantities (sand_return_map)
+    float eps[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) eps[d] = logf(fmaxf(fabsf(s[d]), 1e-14f));
+    const float tr = eps[0] + eps[1] + eps[2];
+    float eh[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) eh[d] = eps[d] - tr / 3.0f;
+    const float ehn = sqrtf(eh[0] * eh[0] + eh[1] * eh[1] + eh[2] * eh[2]);
+    const float kap = (3.0f * lam + 2.0f * mu) / (2.0f * mu);
+    const float dg = ehn + kap * tr * alpha;
+    int br = 1;
+    float se[3], ls[3];
+    float beta = 0.f;
+    if (dg <= 0.0f) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) { se[d] = s[d]; ls[d] = logf(s[d]); }
+    } else if (tr > 0.0f) {
+        br = 2;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) { se[d] = 1.f; ls[d] = 0.f; }
+    } else {
+        br = 3;
+        beta = kap * alpha * tr / ehn;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            ls[d] = eps[d] - eh[d] * (dg / ehn);
+            se[d] = expf(ls[d]);
+        }
+    }
+    // stress adjoint in the principal frame of F' = U diag(se) V^T
+    float T1[9], Tp[9];
+    matmul_tn(U, Tb, T1);
+    matmul(T1, U, Tp);          // U^T Tb U
+    const float lsum = ls[0] + ls[1] + ls[2];
+    const float tsum = Tp[0] + Tp[4] + Tp[8];
+    mub += 2.0f * (Tp[0] * ls[0] + Tp[4] * ls[1] + Tp[8] * ls[2]);
+    lamb += tsum * lsum;
+    float Q[9];
+    {
+        float Q1[9];
+        matmul_tn(U, Fbn, Q1);
+        matmul(Q1, V, Q);       // U^T Fb' V
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) Q[k * 4] += (2.0f * mu * Tp[k * 4] + lam * tsum) / se[k];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int l = 0; l < 3; ++l) {
+            if (k == l) continue;
+            const float c = 2.0f * mu * dlog_div(se[k], se[l]) / (se[k] + se[l]);
+            Q[k * 3 + l] += c * se[l] * (Tp[k * 3 + l] + Tp[l * 3 + k]);
+        }
+    // return-map adjoint: P = principal-frame adjoint of F_tr
+    float P[9];
+    if (br == 1) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) P[k] = Q[k];
+    } else {
+        float dsd[3][3];   // d se_i / d sigma_k
+        if (br == 2) {
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) dsd[i][k] = 0.f;
+        } else {
+            // H_i = -beta eps_i + (1 + beta) tr/3, beta = kap alpha tr / |eh|
+            float dbeta[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) dbeta[k] = kap * alpha * (1.0f / ehn - tr * eh[k] / (ehn * ehn * ehn));
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const float dH = (i == k ? -beta : 0.f) + (1.0f + beta) / 3.0f - eh[i] * dbeta[k];
+                    dsd[i][k] = se[i] * dH / s[k];
+                }
+            // parameters: dH_i/dalpha = -eh_i kap tr/|eh|, dH_i/dkap = -eh_i alpha tr/|eh|
+            float qa = 0.f, qk = 0.f;
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                qa += Q[i * 4] * se[i] * (-eh[i] * kap * tr / ehn);
+                qk += Q[i * 4] * se[i] * (-eh[i] * alpha * tr / ehn);
+            }
+            alb += qa;
+            // kap = 1 + 3 lam / (2 mu)
+            lamb += qk * 3.0f / (2.0f * mu);
+            mub += qk * (-3.0f * lam / (2.0f * mu * mu));
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            float v = 0.f;
+#pragma unroll
+            for (int i = 0; i < 3; ++i) v += Q[i * 4] * dsd[i][k];
+            P[k * 4] = v;
+        }
+        const float Bexp = -beta;
+        const float eA = br == 3 ? expf((1.0f + beta) * tr / 3.0f) : 0.f;
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+            for (int l = 0; l < 3; ++l) {
+                if (k == l) continue;
+                const float D = br == 3 ? eA * dpow_div(s[k], s[l], Bexp) : 0.f;
+                const float S = (se[l] + se[k]) / (s[l] + s[k]);
+                const float a = 0.5f * (D + S), b = 0.5f * (D - S);
+                P[k * 3 + l] = a * Q[k * 3 + l] + b * Q[l * 3 + k];
+            }
+    }
+    // Ftr adjoint = U P V^T
+    float Ftb[9];
+    {
+        float T2[9];
+        matmul(U, P, T2);
+        matmul_nt(T2, V, Ftb);
+    }
+    // F_tr = A F: Ab = Ftb F^T, Fb = A^T Ftb; A = I + dt G
+    float Ab[9];
+    matmul_nt(Ftb, F, Ab);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) Gb[k] = dt * Ab[k];
+    matmul_tn(A, Ftb, Fb);
+}
+
+__device__ __forceinline__ void load9(const float* p, int64_t S, int plane, float o[9]) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) o[k] = p[(int64_t)(plane + k) * S];
+}
+
+__device__ void block_add_params(float mub, float lamb, float alb, double* out) {
+    // wave reduction, then one double atomic per wave
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        mub += __shfl_xor(mub, o);
+        lamb += __shfl_xor(lamb, o);
+        alb += __shfl_xor(alb, o);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&out[0], (double)mub);
+        atomicAdd(&out[1], (double)lamb);
+        atomicAdd(&out[2], (double)alb);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// G2P adjoint.  st: S_n; gv: grid velocities of substep n; an: adjoint of
+// S_{n+1}; vb: grid-velocity adjoint (accumulated); ac: adjoint of S_n
+// (x and F written here, v, C, tau by k_p2g_adj).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlk)
+    k_g2p_adj(const float* __restrict__ st, int64_t S, int n, MpmParams prm, float dt, const float4* __restrict__ gv,
+              const float* __restrict__ an, int64_t SA, float* __restrict__ vb, float* __restrict__ ac,
+              double* __restrict__ pb) {
+    const int p = blockIdx.x * kBlk + threadIdx.x;
+    float mub = 0.f, lamb = 0.f, alb = 0.f;
+    if (p < n) {
+        const float* sp = st + p;
+        float x[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) x[d] = sp[(MPM_P_X0 + d) * S] + sp[(MPM_P_U + d) * S];
+        float F[9];
+        load9(sp, S, MPM_P_F, F);
+        const int ng = prm.n_grid;
+        const float inv_dx = prm.inv_dx;
+        W3 W;
+        weights3(x, inv_dx, W);
+        const bool ok = W.base[0] >= 0 && W.base[1] >= 0 && W.base[2] >= 0 && W.base[0] <= ng - 3 &&
+                        W.base[1] <= ng - 3 && W.base[2] <= ng - 3;
+        const float* ap = an + p;
+        float xb[3], vbp[3], Cb[9], Fbn[9], Tb[9];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) { xb[d] = ap[(kAX + d) * SA]; vbp[d] = ap[(kAV + d) * SA]; }
+        load9(ap, SA, kAC, Cb);
+        load9(ap, SA, kAF, Fbn);
+        load9(ap, SA, kAT, Tb);
+        float Fb[9];
+        if (!ok) {
+            // frozen particle: copied unchanged (x, F pass through)
+#pragma unroll
+            for (int k = 0; k < 9; ++k) Fb[k] = Fbn[k];
+        } else {
+            // forward gather: G = sum v_i (x) grad w
+            float G[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j)
+                    for (int k = 0; k < 3; ++k) {
+                        float w, g[3], H[9];
+                        node_w(W, i, j, k, inv_dx, w, g, H);
+                        const float4 vi = gv[((int64_t)(W.base[0] + i) * ng + (W.base[1] + j)) * ng + (W.base[2] + k)];
+                        const float va[3] = {vi.x, vi.y, vi.z};
+#pragma unroll
+                        for (int a = 0; a < 3; ++a)
+#pragma unroll
+                            for (int d = 0; d < 3; ++d) G[a * 3 + d] += va[a] * g[d];
+                    }
+            float Gb[9];
+            sand_update_adj(G, F, dt, prm.mu, prm.lam, prm.alpha, Fbn, Tb, Gb, Fb, mub, lamb, alb);
+            // x' = x + dt v'
+            float vt[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) vt[d] = vbp[d] + dt * xb[d];
+            const float c4 = 4.0f * inv_dx;
+            float xg[3] = {xb[0], xb[1], xb[2]};
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j)
+                    for (int k = 0; k < 3; ++k) {
+                        float w, g[3], H[9];
+                        node_w(W, i, j, k, inv_dx, w, g, H);
+                        const int64_t node = ((int64_t)(W.base[0] + i) * ng + (W.base[1] + j)) * ng + (W.base[2] + k);
+                        const float4 vi = gv[node];
+                        const float va[3] = {vi.x, vi.y, vi.z};
+                        const float dp[3] = {(float)i - W.fx[0], (float)j - W.fx[1], (float)k - W.fx[2]};  // grid units
+                        // grid velocity adjoint
+                        float vbn[3];
+#pragma unroll
+                        for (int a = 0; a < 3; ++a) {
+                            float t = w * vt[a];
+#pragma unroll
+                            for (int d = 0; d < 3; ++d) t += c4 * w * Cb[a * 3 + d] * dp[d] + Gb[a * 3 + d] * g[d];
+                            vbn[a] = t;
+                        }
+                        unsafeAtomicAdd(vb + 4 * node + 0, vbn[0]);
+                        unsafeAtomicAdd(vb + 4 * node + 1, vbn[1]);
+                        unsafeAtomicAdd(vb + 4 * node + 2, vbn[2]);
+                        // position adjoint through w, (x_i - x_p) and grad w
+                        float vv = 0.f, cv[3] = {0, 0, 0};   // v_i . vt,  C_b^T v_i
+#pragma unroll
+                        for (int a = 0; a < 3; ++a) {
+                            vv += va[a] * vt[a];
+#pragma unroll
+                            for (int d = 0; d < 3; ++d) cv[d] += Cb[a * 3 + d] * va[a];
+                        }
+                        float cdp = cv[0] * dp[0] + cv[1] * dp[1] + cv[2] * dp[2];
+                        float gh[3] = {0, 0, 0};              // sum_ad Gb_ad v_a H_de
+#pragma unroll
+                        for (int a = 0; a < 3; ++a)
+#pragma unroll
+                            for (int d = 0; d < 3; ++d)
+#pragma unroll
+                                for (int e = 0; e < 3; ++e) gh[e] += Gb[a * 3 + d] * va[a] * H[d * 3 + e];
+#pragma unroll
+                        for (int e = 0; e < 3; ++e)
+                            xg[e] += g[e] * (vv + c4 * cdp) - c4 * w * inv_dx * cv[e] + gh[e];
+                    }
+#pragma unroll
+            for (int d = 0; d < 3; ++d) xb[d] = xg[d];
+        }
+        float* cp = ac + p;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) cp[(kAX + d) * SA] = xb[d];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) cp[(kAF + k) * SA] = Fb[k];
+        if (!ok) {
+            // a frozen particle's v, C, tau pass through as well
+#pragma unroll
+            for (int d = 0; d < 3; ++d) cp[(kAV + d) * SA] = vbp[d];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) {
+                cp[(kAC + k) * SA] = Cb[k];
+                cp[(kAT + k) * SA] = Tb[k];
+            }
+        }
+    }
+    block_add_params(mub, lamb, alb, pb);
+}
+
+// ---------------------------------------------------------------------------
+// Grid adjoint: gb (mv_b, m_b) from vb through the colliders and the
+// normalization, on the recomputed P2G grid acc (mv, m).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlk)
+    k_grid_adj(const float4* __restrict__ acc, const float* __restrict__ vb, float4* __restrict__ gb, MpmParams prm,
+               ColliderSet cs, float time, float dt) {
+    const int64_t nn = (int64_t)prm.n_grid * prm.n_grid * prm.n_grid;
+    const int64_t t = (int64_t)blockIdx.x * kBlk + threadIdx.x;
+    if (t >= nn) return;
+    const int ng = prm.n_grid;
+    const int gx = (int)(t / ((int64_t)ng * ng)), gy = (int)((t / ng) % ng), gz = (int)(t % ng);
+    const float4 a4 = acc[t];
+    float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (a4.w > 1e-15f) {
+        const float im = 1.0f / a4.w;
+        float o[3] = {a4.x * im + dt * prm.gravity[0], a4.y * im + dt * prm.gravity[1], a4.z * im + dt * prm.gravity[2]};
+        float mask[3] = {1.f, 1.f, 1.f};
+        for (int c = 0; c < cs.n; ++c) {   // the colliders of grid_node_update, recording zeroed components
+            const MpmCollider& cl = cs.c[c];
+            if (!(time >= cl.start_time && time < cl.end_time)) continue;
+            if (cl.kind == 0) {
+                const float o0 = (float)gx * prm.dx - cl.point[0], o1 = (float)gy * prm.dx - cl.point[1],
+                            o2 = (float)gz * prm.dx - cl.point[2];
+                if (o0 * cl.normal[0] + o1 * cl.normal[1] + o2 * cl.normal[2] < 0.0f) {
+                    o[0] = o[1] = o[2] = 0.f;
+                    mask[0] = mask[1] = mask[2] = 0.f;
+                }
+            } else {
+                const int pad = 3;
+                const int gi[3] = {gx, gy, gz};
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    if ((gi[d] < pad && o[d] < 0.f) || (gi[d] >= ng - pad && o[d] > 0.f)) {
+                        o[d] = 0.f;
+                        mask[d] = 0.f;
+                    }
+                }
+            }
+        }
+        const float b0 = vb[4 * t] * mask[0], b1 = vb[4 * t + 1] * mask[1], b2 = vb[4 * t + 2] * mask[2];
+        out.x = b0 * im;
+        out.y = b1 * im;
+        out.z = b2 * im;
+        out.w = -(b0 * a4.x + b1 * a4.y + b2 * a4.z) * im * im;
+    }
+    gb[t] = out;
+}
+
+// ---------------------------------------------------------------------------
+// P2G adjoint: v, C, tau adjoints of S_n and the position adjoint's P2G part
+// (added to what k_g2p_adj wrote).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlk)
+    k_p2g_adj(const float* __restrict__ st, int64_t S, int n, MpmParams prm, float dt, const float4* __restrict__ gb,
+              float* __restrict__ ac, int64_t SA) {
+    const int p = blockIdx.x * kBlk + threadIdx.x;
+    if (p >= n) return;
+    const float* sp = st + p;
+    float x[3], v[3], C[9], tau[9];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        x[d] = sp[(MPM_P_X0 + d) * S] + sp[(MPM_P_U + d) * S];
+        v[d] = sp[(MPM_P_V + d) * S];
+    }
+    load9(sp, S, MPM_P_C, C);
+    load9(sp, S, MPM_P_TAU, tau);
+    const float m = sp[MPM_P_MASS * S], vol = sp[MPM_P_VOL * S];
+    const int ng = prm.n_grid;
+    const float inv_dx = prm.inv_dx, dx = prm.dx;
+    W3 W;
+    weights3(x, inv_dx, W);
+    const bool ok = W.base[0] >= 0 && W.base[1] >= 0 && W.base[2] >= 0 && W.base[0] <= ng - 3 &&
+                    W.base[1] <= ng - 3 && W.base[2] <= ng - 3;
+    if (!ok) return;   // frozen: k_g2p_adj passed v, C, tau through
+    float vb[3] = {0, 0, 0}, Cb[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, Tb[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    float xg[3] = {0, 0, 0};
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            for (int k = 0; k < 3; ++k) {
+                float w, g[3], H[9];
+                node_w(W, i, j, k, inv_dx, w, g, H);
+                const float4 b4 = gb[((int64_t)(W.base[0] + i) * ng + (W.base[1] + j)) * ng + (W.base[2] + k)];
+                const float mb[3] = {b4.x, b4.y, b4.z};
+                const float dpp[3] = {((float)i - W.fx[0]) * dx, ((float)j - W.fx[1]) * dx, ((float)k - W.fx[2]) * dx};
+                float q[3];   // v + C dpos
+#pragma unroll
+                for (int a = 0; a < 3; ++a) q[a] = v[a] + C[a * 3] * dpp[0] + C[a * 3 + 1] * dpp[1] + C[a * 3 + 2] * dpp[2];
+                float mq = 0.f, ct[3] = {0, 0, 0}, th[3] = {0, 0, 0};
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    vb[a] += w * m * mb[a];
+                    mq += mb[a] * q[a];
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) {
+                        Cb[a * 3 + d] += w * m * mb[a] * dpp[d];
+                        Tb[a * 3 + d] += -dt * vol * mb[a] * g[d];
+                        ct[d] += C[a * 3 + d] * mb[a];               // C^T mb
+                    }
+                }
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+#pragma unroll
+                    for (int d = 0; d < 3; ++d)
+#pragma unroll
+                        for (int e = 0; e < 3; ++e) th[e] += mb[a] * tau[a * 3 + d] * H[d * 3 + e];
+#pragma unroll
+                for (int e = 0; e < 3; ++e) xg[e] += g[e] * (m * mq + b4.w * m) - w * m * ct[e] - dt * vol * th[e];
+            }
+    float* cp = ac + p;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        cp[(kAV + d) * SA] = vb[d];
+        cp[(kAX + d) * SA] += xg[d];
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        cp[(kAC + k) * SA] = Cb[k];
+        cp[(kAT + k) * SA] = Tb[k];
+    }
+}
+
+__global__ void k_zero4(float4* __restrict__ g, int64_t nn) {
+    const int64_t t = (int64_t)blockIdx.x * kBlk + threadIdx.x;
+    if (t < nn) g[t] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+unsigned blocks(int64_t n) { return (unsigned)((n + kBlk - 1) / kBlk); }
+
+}  // namespace
+
+extern "C" {
+
+int64_t mpm_adjoint_workspace_bytes(const MpmParams* prm) {
+    if (check_grid(prm)) return -1;
+    const int64_t nn = (int64_t)prm->n_grid * prm->n_grid * prm->n_grid;
+    return 4 * nn * 16;   // acc (mv, m), grid v, v adjoint, (mv, m) adjoint
+}
+
+int mpm_adjoint_substep(const MpmParams* prm, const MpmState* st, const MpmCollider* colliders, int n_colliders,
+                        void* workspace, float time, float dt, const float* adj_next, float* adj_cur,
+                        int64_t adj_stride, double* param_adj, void* stream) {
+    int rc = check_params(prm);
+    if (rc) return rc;
+    if (!st || !st->planes || st->n < 0 || st->stride < st->n) return set_err(-4, "state invalid");
+    if (st->n > 0x7fffffffLL) return set_err(-6, "more than 2^31-1 particles");
+    if (!workspace || ((uintptr_t)workspace & 255)) return set_err(-11, "workspace NULL or not 256-byte aligned");
+    if (!adj_next || !adj_cur || !param_adj || adj_stride < st->n) return set_err(-30, "adjoint buffers invalid");
+    ColliderSet cs;
+    if ((rc = make_collider_set(colliders, n_colliders, &cs))) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t nn = (int64_t)prm->n_grid * prm->n_grid * prm->n_grid;
+    float4* acc = (float4*)workspace;
+    float4* gv = acc + nn;
+    float4* vbg = gv + nn;
+    float4* gbg = vbg + nn;
+    // forward grid of this substep (P2G, normalization, colliders)
+    if ((rc = mpm_zero_grid(prm, (float*)acc, stream))) return rc;
+    if ((rc = mpm_p2g(prm, st, (float*)acc, dt, stream))) return rc;
+    if ((rc = mpm_grid_update(prm, (const float*)acc, (float*)gv, colliders, n_colliders, time, dt, stream))) return rc;
+    hipLaunchKernelGGL(k_zero4, dim3(blocks(nn)), dim3(kBlk), 0, s, vbg, nn);
+    if ((rc = check_launch("k_zero4"))) return rc;
+    const int n = (int)st->n;
+    if (n > 0) {
+        hipLaunchKernelGGL(k_g2p_adj, dim3(blocks(n)), dim3(kBlk), 0, s, (const float*)st->planes, st->stride, n, *prm,
+                           dt, (const float4*)gv, adj_next, adj_stride, (float*)vbg, adj_cur, param_adj);
+        if ((rc = check_launch("k_g2p_adj"))) return rc;
+    }
+    hipLaunchKernelGGL(k_grid_adj, dim3(blocks(nn)), dim3(kBlk), 0, s, (const float4*)acc, (const float*)vbg, gbg,
+                       *prm, cs, time, dt);
+    if ((rc = check_launch("k_grid_adj"))) return rc;
+    if (n > 0) {
+        hipLaunchKernelGGL(k_p2g_adj, dim3(blocks(n)), dim3(kBlk), 0, s, (const float*)st->planes, st->stride, n,
+                           *prm, dt, (const float4*)gbg, adj_cur, adj_stride);
+        if ((rc = check_launch("k_p2g_adj"))) return rc;
+    }
+    return 0;
+}
+
+}  // extern "C"
