@@ -56,8 +56,9 @@ def parse():
                     help="time the unbinned stage kernels (atomic P2G, grid update, G2P) instead")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy (reference for the HBM byte counters)")
-    ap.add_argument("--workload", default="column", choices=["column", "batch"],
-                    help="column: the 1M-particle headline; batch: 64 uneven scenes advanced together")
+    ap.add_argument("--workload", default="column", choices=["column", "batch", "adjoint"],
+                    help="column: the 1M-particle headline; batch: 64 uneven scenes advanced together; "
+                         "adjoint: forward + adjoint of a differentiable rollout")
     ap.add_argument("--scenes", type=int, default=64)
     return ap.parse_args()
 
@@ -174,6 +175,49 @@ def cpu_baseline(scene, budget_s):
                       f"({el:.1f} s), FP32 C restatement with OpenMP atomics"}
 
 
+def bench_adjoint(args):
+    """Differentiable rollout: 250k particles i.i.d. in [0.3,0.7]^2 x
+    [0.05,0.6] (v ~ N(0, 0.1)) on a 64^3 grid, 200 substeps of dt = 1e-4,
+    checkpoints every 20; loss = mean |x_final - x_target|^2 against a target
+    rollout at E = 3e5 Pa, friction angle 35 deg; gradient with respect to
+    (E, nu, friction angle, initial velocities) by the adjoint kernels.
+    A step is one forward substep plus its adjoint (replay included)."""
+    import torch
+    from reduced_mpm_stress_training_amd import adjoint as A
+    from reduced_mpm_stress_training_amd import scenes as S
+    n = args.particles if args.particles != 1_000_000 else 250_000
+    ng = args.n_grid if args.n_grid != 128 else 64
+    steps = args.steps if args.steps != 500 else 200
+    sc = S.column_uniform(n_particles=n, n_grid=ng)
+    ro = A.Rollout(sc, steps, 1e-4, checkpoint_every=20)
+    v0 = torch.from_numpy(sc.v0).cuda()
+    with torch.no_grad():
+        x_t = ro.forward(v0, 3.0e5, 0.3, 35.0).clone()
+    torch.cuda.synchronize()
+    v = v0.clone().requires_grad_(True)
+    params = torch.tensor([2.0e5, 0.3, 30.0], dtype=torch.float64, requires_grad=True)
+    t0 = time.perf_counter()
+    x = A.sand_rollout(v, params, ro)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    loss = torch.mean(torch.sum((x - x_t) ** 2, 1))
+    loss.backward()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    g = params.grad.tolist()
+    finite = bool(torch.isfinite(v.grad).all()) and all(map(lambda z: z == z, g))
+    out = {
+        "metric": "particle-substeps/s (fwd+adjoint)", "value": n * steps / (t2 - t0), "unit": "particle-substeps/s",
+        "n_gpus": 1, "steps": steps, "warmup": 0, "ms_per_step": (t2 - t0) * 1e3 / steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "differentiable sand rollout, forward + adjoint", "n_particles": n, "n_grid": ng,
+                   "substeps": steps, "checkpoint_every": 20, "parallelism": "single GPU"},
+        "forward_ms_per_step": (t1 - t0) * 1e3 / steps, "backward_ms_per_step": (t2 - t1) * 1e3 / steps,
+        "grad": {"E": g[0], "nu": g[1], "friction_angle": g[2]}, "finite": finite,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -185,6 +229,9 @@ def main():
         return
     if args.workload == "batch":
         bench_batch(args)
+        return
+    if args.workload == "adjoint":
+        bench_adjoint(args)
         return
     import torch
     from reduced_mpm_stress_training_amd import scenes as S

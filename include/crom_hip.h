@@ -91,6 +91,31 @@ int crom_wgrad(const void* acts, const void* grads_buf, int64_t n, int32_t in_di
 int crom_adam(float* params, const float* grads, float* m, float* v, int64_t count, float lr, float beta1,
               float beta2, float eps, int32_t step, void* stream);
 
+/* ---- encoder NetEnc (CROMnet.py:361-438), FP32 ------------------------
+ * Conv1d(3 -> 3, kernel ks, stride) + ELU on [B][3][L_in] -> [B][3][L_out],
+ * L_out = (L_in - ks) / stride + 1, weights [3][3][ks] (torch Conv1d
+ * layout).  For the first layer pass the frame field q [B][L_in][3] with
+ * q_mean / q_std_inv (standardizeQ, CROMnet.py:451-452) non-NULL; they are
+ * NULL for later layers.  `out` holds post-ELU values, which the backward
+ * uses. */
+int crom_conv_forward(const float* in, int B, int L_in, int ks, int stride, const float* W, const float* bias,
+                      float* out, const float* q_mean, const float* q_std_inv, void* stream);
+/* dout: gradient of `out`; din (may be NULL): gradient of the layer input;
+ * gW [3][3][ks], gb [3] are overwritten. */
+int crom_conv_backward(const float* in, const float* out, const float* dout, int B, int L_in, int ks, int stride,
+                       const float* W, float* din, float* gW, float* gb, const float* q_mean, const float* q_std_inv,
+                       void* stream);
+/* Linear + ELU: y [B][n_out] = ELU(x W^T + b), W [n_out][n_in]. */
+int crom_dense_forward(const float* x, int B, int n_in, int n_out, const float* W, const float* bias, float* y,
+                       void* stream);
+int crom_dense_backward(const float* x, const float* y, const float* dy, int B, int n_in, int n_out, const float* W,
+                        float* dx, float* gW, float* gb, void* stream);
+/* Gradient of the frames' latents from the decoder's input gradient:
+ * g [n_frames][lbl] = sum over the frame's points of dx[p][0 .. lbl) (the
+ * latent is broadcast to every point, CROMnet.py:141-142). */
+int crom_latent_grad(const float* dx, int n_frames, int points_per_frame, int in_dim, int lbl, float* g,
+                     void* stream);
+
 #ifdef __cplusplus
 }
 #endif

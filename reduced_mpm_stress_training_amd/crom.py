@@ -16,6 +16,8 @@ from __future__ import annotations
 
 import math
 
+import numpy as np
+
 import torch
 
 from . import _lib as L
@@ -189,3 +191,196 @@ class CromDecoder:
         loss = loss_weight * torch.mean((y - q.to(y.device)) ** 2)
         self.adam_step_(lr)
         return loss
+
+
+# ---------------------------------------------------------------------------
+# Encoder and the full network (CROMnet)
+# ---------------------------------------------------------------------------
+def conv_lengths(npoints: int, ks: int = 6, strides: int = 4, o_dim: int = 3):
+    """Input lengths of NetEnc's Conv1d layers and the flattened length
+    (CROMnet.py:373-384, util.py:201-203: stack while o_dim * L_out >= 32)."""
+    lens = [int(npoints)]
+    while True:
+        l_out = (lens[-1] - (ks - 1) - 1) // strides + 1
+        if o_dim * l_out >= 32:
+            lens.append(l_out)
+        else:
+            break
+    return lens
+
+
+class CromEncoder:
+    """NetEnc (CROMnet.py:361-438): standardizeQ, Conv1d(3->3, ks, stride) +
+    ELU while 3 * L_out >= 32, flatten, Linear -> 32 + ELU, Linear -> lbl +
+    ELU.  FP32 kernels (crom_conv_*, crom_dense_*)."""
+
+    def __init__(self, npoints: int, lbl: int, ks: int = 6, strides: int = 4, device="cuda:0", seed: int = 0):
+        self.npoints, self.lbl, self.ks, self.strides = int(npoints), int(lbl), int(ks), int(strides)
+        self.device = torch.device(device)
+        self.lens = conv_lengths(self.npoints, ks, strides)
+        self.n_conv = len(self.lens) - 1
+        if self.n_conv < 1:
+            raise ValueError("CromEncoder: too few points for one Conv1d layer")
+        self.flat = 3 * self.lens[-1]
+        sizes = [3 * 3 * ks + 3] * self.n_conv + [32 * self.flat + 32, lbl * 32 + lbl]
+        self.n_params = sum(sizes)
+        self.params = torch.empty(self.n_params, dtype=torch.float32, device=self.device)
+        self.grads = torch.zeros_like(self.params)
+        self.adam_m = torch.zeros_like(self.params)
+        self.adam_v = torch.zeros_like(self.params)
+        self.adam_step = 0
+        g = torch.Generator().manual_seed(seed)
+        off = 0
+        self.W, self.b = [], []
+        shapes = [((3, 3, ks), 3 * ks, 3 * ks)] * self.n_conv + [((32, self.flat), self.flat, 32),
+                                                                  ((lbl, 32), 32, lbl)]
+        for shp, fan_in, fan_out in shapes:
+            nw = int(np.prod(shp))
+            w = self.params[off:off + nw].view(*shp)
+            off += nw
+            b = self.params[off:off + shp[0]]
+            off += shp[0]
+            # xavier_uniform_ (CROMnet.py:412-421); torch's default bias init
+            a = math.sqrt(6.0 / (fan_in + fan_out)) if len(shp) == 2 else math.sqrt(6.0 / (3 * ks + 3 * ks))
+            w.copy_(((torch.rand(shp, generator=g) * 2 - 1) * a).to(self.device))
+            b.copy_(((torch.rand((shp[0],), generator=g) * 2 - 1) / math.sqrt(fan_in)).to(self.device))
+            self.W.append(w)
+            self.b.append(b)
+        self.mean_q = torch.zeros(3, device=self.device)
+        self.std_q = torch.ones(3, device=self.device)
+        self._acts = None
+
+    # enc10 acts on the flattened [position][channel] order of the reference
+    # (CROMnet.py:427-429); the conv kernels produce [channel][position], so
+    # the kernel multiplies with enc10's columns taken in that order.
+    def _enc10_kernel_weight(self):
+        L = self.lens[-1]
+        return self.W[-2].view(32, L, 3).permute(0, 2, 1).contiguous().view(32, 3 * L)
+
+    def set_preprop(self, mean_q, std_q):
+        self.mean_q = torch.as_tensor(mean_q, dtype=torch.float32).reshape(3).to(self.device)
+        self.std_q = torch.as_tensor(std_q, dtype=torch.float32).reshape(3).to(self.device)
+
+    def forward(self, q: torch.Tensor) -> torch.Tensor:
+        """q (B, npoints, 3) -> latents (B, lbl)."""
+        lib = L.load()
+        q = q.to(self.device, torch.float32).contiguous()
+        B = q.shape[0]
+        if q.shape[1:] != (self.npoints, 3):
+            raise ValueError(f"q must be (B, {self.npoints}, 3)")
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        qis = (1.0 / self.std_q).contiguous()
+        acts = [q]
+        for k in range(self.n_conv):
+            out = torch.empty((B, 3, self.lens[k + 1]), dtype=torch.float32, device=self.device)
+            first = k == 0
+            L.check(lib.crom_conv_forward(acts[-1].data_ptr(), B, self.lens[k], self.ks, self.strides,
+                                          self.W[k].data_ptr(), self.b[k].data_ptr(), out.data_ptr(),
+                                          self.mean_q.data_ptr() if first else None,
+                                          qis.data_ptr() if first else None, stream), "crom_conv_forward")
+            acts.append(out)
+        w10 = self._enc10_kernel_weight()
+        h = torch.empty((B, 32), dtype=torch.float32, device=self.device)
+        L.check(lib.crom_dense_forward(acts[-1].data_ptr(), B, self.flat, 32, w10.data_ptr(), self.b[-2].data_ptr(),
+                                       h.data_ptr(), stream), "crom_dense_forward")
+        z = torch.empty((B, self.lbl), dtype=torch.float32, device=self.device)
+        L.check(lib.crom_dense_forward(h.data_ptr(), B, 32, self.lbl, self.W[-1].data_ptr(), self.b[-1].data_ptr(),
+                                       z.data_ptr(), stream), "crom_dense_forward")
+        self._acts = (acts, h, z, w10, qis)
+        return z
+
+    __call__ = forward
+
+    def backward(self, g_z: torch.Tensor):
+        """Gradients of the last forward's parameters for d(loss)/d(latents)."""
+        lib = L.load()
+        acts, h, z, w10, qis = self._acts
+        B = z.shape[0]
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        g_z = g_z.to(self.device, torch.float32).contiguous()
+        sizes = [3 * 3 * self.ks + 3] * self.n_conv + [32 * self.flat + 32, self.lbl * 32 + self.lbl]
+        offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+        gW = lambda k: self.grads[offs[k]:offs[k] + sizes[k] - self.W[k].shape[0]]
+        gB = lambda k: self.grads[offs[k] + sizes[k] - self.W[k].shape[0]:offs[k + 1]]
+        g_h = torch.empty_like(h)
+        L.check(lib.crom_dense_backward(h.data_ptr(), z.data_ptr(), g_z.data_ptr(), B, 32, self.lbl,
+                                        self.W[-1].data_ptr(), g_h.data_ptr(), gW(self.n_conv + 1).data_ptr(),
+                                        gB(self.n_conv + 1).data_ptr(), stream), "crom_dense_backward")
+        g_flat = torch.empty_like(acts[-1])
+        g10 = torch.empty((32, self.flat), dtype=torch.float32, device=self.device)
+        L.check(lib.crom_dense_backward(acts[-1].data_ptr(), h.data_ptr(), g_h.data_ptr(), B, self.flat, 32,
+                                        w10.data_ptr(), g_flat.data_ptr(), g10.data_ptr(),
+                                        gB(self.n_conv).data_ptr(), stream), "crom_dense_backward")
+        Lf = self.lens[-1]
+        gW(self.n_conv).view(32, Lf, 3).copy_(g10.view(32, 3, Lf).permute(0, 2, 1))
+        g_out = g_flat
+        for k in range(self.n_conv - 1, -1, -1):
+            first = k == 0
+            g_in = None if first else torch.empty_like(acts[k])
+            L.check(lib.crom_conv_backward(acts[k].data_ptr(), acts[k + 1].data_ptr(), g_out.data_ptr(), B,
+                                           self.lens[k], self.ks, self.strides, self.W[k].data_ptr(),
+                                           g_in.data_ptr() if g_in is not None else None, gW(k).data_ptr(),
+                                           gB(k).data_ptr(), self.mean_q.data_ptr() if first else None,
+                                           qis.data_ptr() if first else None, stream), "crom_conv_backward")
+            g_out = g_in
+
+    def adam_step_(self, lr: float, betas=(0.9, 0.999), eps: float = 1e-8):
+        lib = L.load()
+        self.adam_step += 1
+        L.check(lib.crom_adam(self.params.data_ptr(), self.grads.data_ptr(), self.adam_m.data_ptr(),
+                              self.adam_v.data_ptr(), self.n_params, float(lr), float(betas[0]), float(betas[1]),
+                              float(eps), self.adam_step, torch.cuda.current_stream(self.device).cuda_stream),
+                "crom_adam")
+
+
+def lr_factor(epoch: int, learning_rates=(10.0, 5.0, 2.0, 1.0), epochs=(400, 200, 100, 50)) -> float:
+    """LambdaLR factor of CROMnet.adaptiveLRfromRange (CROMnet.py:183-191,
+    util.py:187-194; run.py:33-34 defaults)."""
+    acc = np.concatenate([[0], np.cumsum(epochs)])
+    for i in range(len(acc) - 1):
+        if acc[i] <= epoch < acc[i + 1]:
+            return float(learning_rates[i])
+    if epoch == acc[-1]:
+        return float(learning_rates[-1])
+    raise ValueError("epoch outside the schedule")
+
+
+class Crom:
+    """CROMnet (CROMnet.py:20-153): encoder -> latent per frame -> decoder at
+    every point of the frame, trained with 1000 * MSE (CROMnet.py:70-83) by
+    Adam at initial_lr * LambdaLR factor."""
+
+    def __init__(self, npoints: int, lbl: int = 16, ks: int = 6, strides: int = 4, device="cuda:0", seed: int = 0):
+        self.encoder = CromEncoder(npoints, lbl, ks, strides, device=device, seed=seed)
+        self.decoder = CromDecoder(lbl, out_dim=3, device=device, seed=seed + 1)
+        self.lbl, self.npoints, self.device = lbl, npoints, torch.device(device)
+
+    def set_preprop(self, mean_x, std_x, mean_q, std_q):
+        self.encoder.set_preprop(mean_q, std_q)
+        self.decoder.set_preprop(mean_x, std_x, mean_q, std_q)
+
+    def _decoder_input(self, z: torch.Tensor, x: torch.Tensor) -> torch.Tensor:
+        B, P = x.shape[0], x.shape[1]
+        return torch.cat([z[:, None, :].expand(B, P, self.lbl), x], 2).reshape(B * P, self.lbl + 3)
+
+    def forward(self, q: torch.Tensor, x: torch.Tensor) -> torch.Tensor:
+        """CROMnet.forward (CROMnet.py:130-150): q, x (B, P, 3) -> q_hat."""
+        z = self.encoder(q)
+        return self.decoder(self._decoder_input(z, x.to(self.device))).view(x.shape[0], x.shape[1], 3)
+
+    def train_step(self, q: torch.Tensor, x: torch.Tensor, lr: float) -> torch.Tensor:
+        """One optimizer step on a batch of frames q, x (B, P, 3); returns the
+        loss 1000 * MSE (device scalar)."""
+        lib = L.load()
+        q = q.to(self.device, torch.float32).contiguous()
+        B, P = q.shape[0], q.shape[1]
+        z = self.encoder(q)
+        xin = self._decoder_input(z, x.to(self.device, torch.float32))
+        y, dx = self.decoder.backward(xin, q.reshape(B * P, 3), input_grad=True)
+        g_z = torch.empty((B, self.lbl), dtype=torch.float32, device=self.device)
+        L.check(lib.crom_latent_grad(dx.data_ptr(), B, P, self.lbl + 3, self.lbl, g_z.data_ptr(),
+                                     torch.cuda.current_stream(self.device).cuda_stream), "crom_latent_grad")
+        self.encoder.backward(g_z)
+        self.decoder.adam_step_(lr)
+        self.encoder.adam_step_(lr)
+        return 1000.0 * torch.mean((y - q.reshape(B * P, 3)) ** 2)

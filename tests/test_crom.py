@@ -169,3 +169,79 @@ def test_training_reduces_loss():
     last = float(last)
     print(first, last)
     assert last < first / 10
+
+
+def _torch_encoder(enc):
+    """torch.nn copy of NetEnc (CROMnet.py:361-438) with the encoder's weights."""
+    convs = []
+    for k in range(enc.n_conv):
+        c = torch.nn.Conv1d(3, 3, enc.ks, enc.strides)
+        with torch.no_grad():
+            c.weight.copy_(enc.W[k].cpu())
+            c.bias.copy_(enc.b[k].cpu())
+        convs.append(c)
+    l10 = torch.nn.Linear(enc.flat, 32)
+    l11 = torch.nn.Linear(32, enc.lbl)
+    with torch.no_grad():
+        l10.weight.copy_(enc.W[-2].cpu()); l10.bias.copy_(enc.b[-2].cpu())
+        l11.weight.copy_(enc.W[-1].cpu()); l11.bias.copy_(enc.b[-1].cpu())
+
+    def fwd(q):
+        s = (q - enc.mean_q.cpu()) / enc.std_q.cpu()
+        s = s.transpose(1, 2)
+        for c in convs:
+            s = torch.nn.functional.elu(c(s))
+        s = s.transpose(1, 2).reshape(-1, s.size(1) * s.size(2))
+        s = torch.nn.functional.elu(l10(s))
+        return torch.nn.functional.elu(l11(s))
+    params = [p for c in convs for p in (c.weight, c.bias)] + [l10.weight, l10.bias, l11.weight, l11.bias]
+    return fwd, params
+
+
+def test_encoder_matches_torch():
+    from reduced_mpm_stress_training_amd.crom import CromEncoder
+    torch.manual_seed(7)
+    P, B, lbl = 3000, 5, 16
+    enc = CromEncoder(P, lbl, seed=8)
+    enc.set_preprop([0.01, -0.02, 0.005], [0.02, 0.03, 0.01])
+    q = 0.02 * torch.randn(B, P, 3)
+    z = enc(q.cuda()).cpu()
+    g_z = torch.randn(B, lbl)
+    enc.backward(g_z.cuda())
+    torch.cuda.synchronize()
+    fwd, params = _torch_encoder(enc)
+    zr = fwd(q)
+    (zr * g_z).sum().backward()
+    g_ref = torch.cat([p.grad.reshape(-1) for p in params])
+    print(_rel(z, zr.detach()), _rel(enc.grads.cpu(), g_ref))
+    assert _rel(z, zr.detach()) <= 1e-5
+    assert _rel(enc.grads.cpu(), g_ref) <= 1e-4
+
+
+def test_crom_autoencoder_training_reduces_loss():
+    """CROMnet trained on a family of smooth displacement fields: the loss
+    falls by more than 10x and the encoder receives gradients through the
+    decoder's latent columns."""
+    from reduced_mpm_stress_training_amd.crom import Crom
+    torch.manual_seed(9)
+    P, F = 4096, 24
+    X = torch.rand(P, 3)
+    amps = torch.linspace(0.2, 1.0, F)
+    q = torch.stack([0.02 * a * torch.stack([torch.sin(3 * X[:, 0] + a), X[:, 1] * X[:, 2] * a,
+                                             torch.cos(2 * X[:, 2]) * a], 1) for a in amps])
+    x = X[None].expand(F, P, 3).contiguous()
+    net = Crom(P, lbl=4, seed=10)
+    net.set_preprop(X.mean(0), X.std(0), q.reshape(-1, 3).mean(0), q.reshape(-1, 3).std(0))
+    qd, xd = q.cuda(), x.cuda()
+    e0 = net.encoder.params.clone()
+    first = float(net.train_step(qd[:8], xd[:8], lr=1e-3))
+    for it in range(600):
+        b = (it * 8) % F
+        last = net.train_step(qd[b:b + 8], xd[b:b + 8], lr=1e-3)
+    with torch.no_grad():
+        rec = net.forward(qd, xd).cpu()
+    err = _rel(rec, q)
+    print(first, float(last), err)
+    assert float(last) < first / 10
+    assert err < 0.2
+    assert not torch.equal(e0, net.encoder.params)
