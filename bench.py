@@ -56,7 +56,7 @@ def parse():
                     help="time the unbinned stage kernels (atomic P2G, grid update, G2P) instead")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy (reference for the HBM byte counters)")
-    ap.add_argument("--workload", default="column", choices=["column", "batch", "adjoint"],
+    ap.add_argument("--workload", default="column", choices=["column", "batch", "adjoint", "crom"],
                     help="column: the 1M-particle headline; batch: 64 uneven scenes advanced together; "
                          "adjoint: forward + adjoint of a differentiable rollout")
     ap.add_argument("--scenes", type=int, default=64)
@@ -218,6 +218,81 @@ def bench_adjoint(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_crom(args):
+    """CROM on solver snapshots: sand columns of 1M particles (128^3, as the
+    headline) run by the MPM kernels at 4 random friction angles in
+    [20, 40] deg (seed 3), 40 frames of 10 substeps each; CROMnet (latent 16,
+    decoder 19 -> 128 x 5 -> 3, BF16 MFMA) trained on the first N/10 points of
+    each frame, batches of 10 frames; then a held-out run (another angle) is
+    encoded frame by frame and decoded at all 1M particles.  Reports the
+    decode throughput, its time against the MPM substeps it stands for, and
+    the relative L2 error of the reconstructed displacements."""
+    import torch
+    from reduced_mpm_stress_training_amd import crom as CR
+    from reduced_mpm_stress_training_amd import scenes as S
+    rng = np.random.default_rng(3)
+    n, dt, spf, frames = args.particles, 1e-4, 10, 40
+    sc = S.column_uniform(n_particles=n, n_grid=args.n_grid)
+    angles = [float(a) for a in rng.uniform(20.0, 40.0, size=5)]
+    t0 = time.perf_counter()
+    data = [CR.solver_snapshots(sc, a, frames, spf, dt) for a in angles]
+    torch.cuda.synchronize()
+    t_data = time.perf_counter() - t0
+    P = n // 10                                    # first N/10 points (SimulationDataset.py:36-43)
+    X = torch.from_numpy(sc.x0).cuda()
+    train = torch.cat([d[:, :P] for d in data[:4]])         # (4 * frames, P, 3)
+    q_all = train.reshape(-1, 3)
+    net = CR.Crom(P, lbl=16, seed=0)
+    net.set_preprop(X[:P].mean(0), X[:P].std(0), q_all.mean(0), q_all.std(0).clamp_min(1e-12))
+    xb = X[None, :P].expand(10, P, 3).contiguous()
+    steps = args.steps if args.steps != 500 else 1500
+    t0 = time.perf_counter()
+    for it in range(steps):
+        idx = torch.randint(0, train.shape[0], (10,), device="cuda")
+        lr = 1e-4 * CR.lr_factor(int(it * 750 / steps))
+        loss = net.train_step(train[idx], xb, lr)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    # held-out run: encode each frame from its first N/10 points, decode all N
+    held = data[4]
+    err_num = err_den = 0.0
+    torch.cuda.synchronize()
+    t_dec = 0.0
+    for f in range(1, frames):
+        z = net.encoder(held[f:f + 1, :P])
+        xin = torch.cat([z.expand(n, 16), X], 1).contiguous()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        qh = net.decoder(xin)
+        torch.cuda.synchronize()
+        t_dec += time.perf_counter() - t1
+        err_num += float(((qh - held[f]) ** 2).sum())
+        err_den += float((held[f] ** 2).sum())
+    rel = (err_num / err_den) ** 0.5
+    ms_dec = t_dec * 1e3 / (frames - 1)
+    # MPM time of one frame (10 substeps) on the same column
+    sim = S.make_simulator(sc)
+    sim.run(10, dt)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    sim.run(100, dt)
+    torch.cuda.synchronize()
+    ms_frame = (time.perf_counter() - t1) * 1e3 / 10
+    flop = 2 * (19 * 128 + 4 * 128 * 128 + 128 * 3)
+    out = {
+        "metric": "CROM decoder points/s (inference at 1M particles)", "value": n / (ms_dec / 1e3),
+        "unit": "points/s", "n_gpus": 1, "steps": steps, "warmup": 0, "ms_per_step": ms_dec,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16", "data": "synthetic",
+        "config": {"workload": "CROM decoder trained on solver snapshots", "n_particles": n, "latent": 16,
+                   "hidden": 128, "train_frames": 4 * frames, "points_per_frame": P, "train_steps": steps},
+        "decoder_tflops": flop * n / (ms_dec / 1e3) / 1e12,
+        "mpm_ms_per_frame_10_substeps": ms_frame, "decode_speedup_over_mpm_frame": ms_frame / ms_dec,
+        "held_out_relative_l2_error": rel, "final_train_loss": float(loss),
+        "train_seconds": t_train, "snapshot_seconds": t_data,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -232,6 +307,9 @@ def main():
         return
     if args.workload == "adjoint":
         bench_adjoint(args)
+        return
+    if args.workload == "crom":
+        bench_crom(args)
         return
     import torch
     from reduced_mpm_stress_training_amd import scenes as S

@@ -384,3 +384,22 @@ class Crom:
         self.decoder.adam_step_(lr)
         self.encoder.adam_step_(lr)
         return 1000.0 * torch.mean((y - q.reshape(B * P, 3)) ** 2)
+
+
+def solver_snapshots(scene, friction_angle: float, n_frames: int, substeps_per_frame: int, dt: float,
+                     device="cuda:0") -> torch.Tensor:
+    """Displacement snapshots q = u (frames, N, 3) of one sand run by the MPM
+    solver (the training data the reference writes with save_data_at_frame,
+    mpm_solver_warp.py:312-404, generate_sand.py:27-32), frame 0 = initial."""
+    from . import scenes as S
+    import copy
+    sc = copy.copy(scene)
+    sc.friction_angle = float(friction_angle)
+    sim = S.make_simulator(sc, device=device)
+    out = torch.empty((n_frames, sc.n_particles, 3), dtype=torch.float32, device=torch.device(device))
+    for f in range(n_frames):
+        u, _, _, _ = sim.export_uvFstress_from_warp()
+        out[f] = u
+        if f + 1 < n_frames:
+            sim.run(substeps_per_frame, dt)
+    return out
