@@ -40,7 +40,7 @@
 extern "C" {
 #endif
 
-#define MPM_ABI_VERSION 5
+#define MPM_ABI_VERSION 6
 
 /* plane indices */
 #define MPM_P_X0 0
@@ -241,13 +241,44 @@ int mpm_bins_run(const MpmParams* prm, MpmBins* bins, const MpmCollider* collide
  * float4. */
 int64_t mpm_adjoint_workspace_bytes(const MpmParams* prm);
 
-/* st: the state S_n before the substep (as mpm_step reads it); time, dt:
- * that substep's.  adj_next: adjoint of S_{n+1}; adj_cur: receives the
- * adjoint of S_n (may not alias adj_next).  param_adj[0..2] += adjoints of
- * mu, lam, alpha (double). */
-int mpm_adjoint_substep(const MpmParams* prm, const MpmState* st, const MpmCollider* colliders, int n_colliders,
-                        void* workspace, float time, float dt, const float* adj_next, float* adj_cur,
+/* st: the state S_n before the substep, in the order of `order` (see
+ * mpm_sort_by_bin; the substep's grid is recomputed as mpm_step_ordered
+ * computes it); time, dt: that substep's.  adj_next: adjoint of S_{n+1};
+ * adj_cur: receives the adjoint of S_n (may not alias adj_next).
+ * param_adj[0..2] += adjoints of mu, lam, alpha (double). */
+int mpm_adjoint_substep(const MpmParams* prm, const MpmState* st, const void* order, const MpmCollider* colliders,
+                        int n_colliders, void* workspace, float time, float dt, const float* adj_next, float* adj_cur,
                         int64_t adj_stride, double* param_adj, void* stream);
+
+/* ---- the flat state in bin order -------------------------------------
+ *
+ * The same substep as mpm_step (p2g2p, mpm_solver_warp.py:156-203) with
+ * the P2G scatter (p2g_apic_stress_given, mpm_utils.py:190-227) privatized
+ * per chunk of particles in LDS instead of 27 global float atomics per
+ * particle and component.  mpm_sort_by_bin counting-sorts a flat state by
+ * the 8^3-cell bin and 4^3-cell sub-block of each particle's stencil base
+ * (mpm_utils.py:197-199) into `dst` and fills `order` (mpm_order_bytes
+ * bytes, 256-byte aligned): the chunk table (runs of at most 512 particles
+ * inside one bin) and the permutation perm[sorted slot] = source slot (int32,
+ * at byte offset mpm_order_perm_offset).  The order stays valid for any state
+ * in the same slot order -- e.g. the substeps after the sort -- as particles
+ * drift; it only loses speed.  No counterpart in the reference (Warp's P2G
+ * runs one thread per particle in storage order). */
+int64_t mpm_order_bytes(const MpmParams* prm, int64_t n_particles);
+int64_t mpm_order_perm_offset(const MpmParams* prm, int64_t n_particles);
+int mpm_sort_by_bin(const MpmParams* prm, const MpmState* src, const MpmState* dst, void* order, void* stream);
+
+/* p2g2p on a state in the slot order of `order`; workspace as mpm_step. */
+int mpm_step_ordered(const MpmParams* prm, const MpmState* st, const void* order, const MpmCollider* colliders,
+                     int n_colliders, void* workspace, int64_t workspace_bytes, float time, float dt, void* stream);
+
+/* Plane-wise permutation of n columns: scatter = 0: dst[:, j] = src[:, idx[j]];
+ * scatter = 1: dst[:, idx[j]] = src[:, j].  idx: n int32 (a sort's perm, or a
+ * state's pid plane); entries outside [0, n) are skipped.  Moves states and
+ * adjoints between slot orders (the adjoint of a sort is its inverse
+ * permutation). */
+int mpm_permute(const float* src, int64_t src_stride, float* dst, int64_t dst_stride, int n_planes, const int* idx,
+                int64_t n, int scatter, void* stream);
 
 /* ---- batches of independent scenes -----------------------------------
  *

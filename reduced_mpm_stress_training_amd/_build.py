@@ -15,8 +15,8 @@ INCLUDE = os.path.join(REPO_DIR, "include")
 LIB_NAME = "libmpm_hip.so"
 LIB_PATH = os.path.join(PKG_DIR, LIB_NAME)
 
-SOURCES = ["mpm_kernels.hip", "mpm_binned.hip", "mpm_adjoint.hip", "crom_mlp.hip", "crom_encoder.hip"]
-HEADERS = ["mpm_common.h", "mpm_device.h", "mpm_stencil.h"]
+SOURCES = ["mpm_kernels.hip", "mpm_binned.hip", "mpm_ordered.hip", "mpm_adjoint.hip", "crom_mlp.hip", "crom_encoder.hip"]
+HEADERS = ["mpm_common.h", "mpm_device.h", "mpm_stencil.h", "mpm_ordered.h"]
 
 # -packed-fp32-ops: keep hipcc from emitting v_pk_{add,mul,fma}_f32.
 # -munsafe-fp-atomics: hardware global_atomic_add_f32 instead of CAS loops.

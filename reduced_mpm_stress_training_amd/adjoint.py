@@ -1,4 +1,4 @@
-"""Differentiable sand rollouts on MI355X: the forward substeps (mpm_step)
+"""Differentiable sand rollouts on MI355X: the forward substeps (mpm_step_ordered)
 and their hand-written adjoints (mpm_adjoint_substep, include/mpm_hip.h)
 behind a torch.autograd.Function, with the state checkpointed every K
 substeps and the substeps between checkpoints recomputed on the way back.
@@ -47,11 +47,20 @@ def material_jacobian(E: float, nu: float, phi: float):
 
 
 class Rollout:
-    """n_substeps of a scene on the flat state with checkpoints every K."""
+    """n_substeps of a scene on the flat state with checkpoints every K.
+
+    At every checkpoint the state is counting-sorted by bin
+    (mpm_sort_by_bin), and the substeps up to the next checkpoint run on
+    that slot order (mpm_step_ordered), so that their P2G and the adjoint's
+    grid scatter accumulate per chunk of particles in LDS.  The adjoint of a
+    sort is its inverse permutation (mpm_permute), so gradients come back in
+    the caller's particle order."""
 
     def __init__(self, scene: S.SandScene, n_substeps: int, dt: float, checkpoint_every: int = 20,
                  device="cuda:0"):
         self.scene, self.n_substeps, self.dt, self.K = scene, int(n_substeps), float(dt), int(checkpoint_every)
+        if self.K < 1:
+            raise ValueError("checkpoint_every must be >= 1")
         self.device = torch.device(device)
         self.n = scene.n_particles
         lib = L.load()
@@ -74,7 +83,9 @@ class Rollout:
         self._ws_step = torch.zeros(self._ws_step_bytes + 256, dtype=torch.uint8, device=self.device)
         self._ws_adj = torch.zeros(int(lib.mpm_adjoint_workspace_bytes(self._prm)) + 256, dtype=torch.uint8,
                                    device=self.device)
-        # initial state (v filled per call)
+        self._order_bytes = int(lib.mpm_order_bytes(self._prm, self.n))
+        self._perm_off = int(lib.mpm_order_perm_offset(self._prm, self.n))
+        # initial state in the caller's order (v filled per call)
         st = S.scene_state(scene, np.float32)
         planes = torch.zeros((L.NPLANES, self.n), dtype=torch.float32, device=self.device)
         t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(self.device)
@@ -85,10 +96,14 @@ class Rollout:
         planes[L.P_PID] = torch.arange(self.n, dtype=torch.int32, device=self.device).view(torch.float32)
         self._init = planes
         self._checkpoints = {}
+        self._final = None
 
     @staticmethod
     def _aligned(buf: torch.Tensor) -> int:
         return (buf.data_ptr() + 255) & ~255
+
+    def _stream(self):
+        return torch.cuda.current_stream(self.device).cuda_stream
 
     def _state(self, planes: torch.Tensor) -> L.MpmState:
         st = L.MpmState()
@@ -99,48 +114,74 @@ class Rollout:
         self.mu, self.lam, self.alpha = material(E, nu, phi)
         self._prm.mu, self._prm.lam, self._prm.alpha = self.mu, self.lam, self.alpha
 
-    def _step(self, planes, k):
-        lib = L.load()
-        L.check(lib.mpm_step(self._prm, self._state(planes), self._cols, self._ncol, self._aligned(self._ws_step),
-                             self._ws_step_bytes, float(k * self.dt), self.dt,
-                             torch.cuda.current_stream(self.device).cuda_stream), "mpm_step")
+    def _sort(self, planes):
+        """(planes sorted by bin, their order buffer)."""
+        dst = torch.empty_like(planes)
+        order = torch.empty(self._order_bytes + 256, dtype=torch.uint8, device=self.device)
+        L.check(L.load().mpm_sort_by_bin(self._prm, self._state(planes), self._state(dst), self._aligned(order),
+                                         self._stream()), "mpm_sort_by_bin")
+        return dst, order
+
+    def _step(self, planes, k, order):
+        L.check(L.load().mpm_step_ordered(self._prm, self._state(planes), self._aligned(order), self._cols,
+                                          self._ncol, self._aligned(self._ws_step), self._ws_step_bytes,
+                                          float(k * self.dt), self.dt, self._stream()), "mpm_step_ordered")
+
+    def _permute(self, src, n_planes, idx_ptr, scatter):
+        dst = torch.zeros((n_planes, self.n), dtype=torch.float32, device=self.device)
+        L.check(L.load().mpm_permute(src.data_ptr(), src.shape[1], dst.data_ptr(), self.n, n_planes, idx_ptr, self.n,
+                                     int(scatter), self._stream()), "mpm_permute")
+        return dst
 
     def forward(self, v0: torch.Tensor, E: float, nu: float, phi: float) -> torch.Tensor:
-        """Positions x = x0 + u after n_substeps; keeps checkpoints."""
+        """Positions x = x0 + u after n_substeps, in the caller's particle
+        order; keeps the checkpoints (state and slot order)."""
         self._set_material(E, nu, phi)
         self._material_args = (float(E), float(nu), float(phi))
         planes = self._init.clone()
         planes[L.P_V:L.P_V + 3] = v0.to(self.device, torch.float32).t()
         self._checkpoints = {}
+        order = None
         for k in range(self.n_substeps):
             if k % self.K == 0:
-                self._checkpoints[k] = planes.clone()
-            self._step(planes, k)
-        return (planes[L.P_X0:L.P_X0 + 3] + planes[L.P_U:L.P_U + 3]).t().contiguous()
+                planes, order = self._sort(planes)
+                self._checkpoints[k] = (planes.clone(), order)
+            self._step(planes, k, order)
+        self._final = planes
+        # x0 and u back to the caller's order (scatter by the pid plane)
+        xu = self._permute(planes[L.P_X0:L.P_U + 3], 6, planes[L.P_PID].data_ptr(), scatter=True)
+        return (xu[0:3] + xu[3:6]).t().contiguous()
 
     def backward(self, x_bar: torch.Tensor):
-        """Adjoint of forward() for the position adjoint x_bar (N, 3):
-        returns (v0_bar (N, 3), [E_bar, nu_bar, phi_bar])."""
+        """Adjoint of forward() for the position adjoint x_bar (N, 3) in the
+        caller's order: returns (v0_bar (N, 3), [E_bar, nu_bar, phi_bar])."""
         lib = L.load()
-        stream = torch.cuda.current_stream(self.device).cuda_stream
+        stream = self._stream()
+        # x_bar to the final slot order (gather by the pid plane)
+        xb = x_bar.to(self.device, torch.float32).t().contiguous()
         adj = torch.zeros((L.ADJ_NPLANES, self.n), dtype=torch.float32, device=self.device)
-        adj[L.ADJ_X:L.ADJ_X + 3] = x_bar.to(self.device, torch.float32).t()
+        if self.n_substeps == 0:
+            return adj[L.ADJ_V:L.ADJ_V + 3].t().contiguous(), [0.0, 0.0, 0.0]
+        adj[L.ADJ_X:L.ADJ_X + 3] = self._permute(xb, 3, self._final[L.P_PID].data_ptr(), scatter=False)
         nxt = torch.empty_like(adj)
         pb = torch.zeros(3, dtype=torch.float64, device=self.device)
         for c0 in sorted(self._checkpoints, reverse=True):
             c1 = min(c0 + self.K, self.n_substeps)
-            states = [self._checkpoints[c0].clone()]
+            cp, order = self._checkpoints[c0]
+            states = [cp.clone()]
             for k in range(c0, c1 - 1):
                 s = states[-1].clone()
-                self._step(s, k)
+                self._step(s, k, order)
                 states.append(s)
             for k in range(c1 - 1, c0 - 1, -1):
                 nxt.zero_()
-                L.check(lib.mpm_adjoint_substep(self._prm, self._state(states[k - c0]), self._cols, self._ncol,
-                                                self._aligned(self._ws_adj), float(k * self.dt), self.dt,
-                                                adj.data_ptr(), nxt.data_ptr(), self.n, pb.data_ptr(), stream),
-                        "mpm_adjoint_substep")
+                L.check(lib.mpm_adjoint_substep(self._prm, self._state(states[k - c0]), self._aligned(order),
+                                                self._cols, self._ncol, self._aligned(self._ws_adj),
+                                                float(k * self.dt), self.dt, adj.data_ptr(), nxt.data_ptr(), self.n,
+                                                pb.data_ptr(), stream), "mpm_adjoint_substep")
                 adj, nxt = nxt, adj
+            # adjoint of the sort at c0: its inverse permutation
+            adj = self._permute(adj, L.ADJ_NPLANES, self._aligned(order) + self._perm_off, scatter=True)
         v0_bar = adj[L.ADJ_V:L.ADJ_V + 3].t().contiguous()
         mu_b, lam_b, al_b = (float(v) for v in pb.cpu())
         J = material_jacobian(*self._material_args)

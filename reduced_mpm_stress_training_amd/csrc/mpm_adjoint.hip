@@ -13,11 +13,12 @@
 //           G = sum v_i (x) grad w,  x' = x + dt v',
 //           F' = RM((I + dt G) F),  tau' = stress(F')           (:241-306)
 // and its adjoint runs the three stages backwards:
-//   k_g2p_adj   per particle: stress and return-map adjoints through the
-//               SVD (closed forms of the derivative of isotropic matrix
+//   k_g2p_adj_chunks  per particle: stress and return-map adjoints through
+//               the SVD (closed forms of the derivative of isotropic matrix
 //               functions, divided differences where singular values meet),
-//               then scatters the grid-velocity adjoint (float atomics) and
-//               the position / F adjoints of the weights and F_tr;
+//               the position / F adjoints of the weights and F_tr, and the
+//               grid-velocity adjoint, scattered per chunk of the ordered
+//               state into an FP64 LDS tile (mpm_ordered.h);
 //   k_grid_adj  per node: through the collider masks and the normalization;
 //   k_p2g_adj   per particle: gathers v, C, tau and position adjoints.
 // The weights' base cell is piecewise constant and the return-map branch is
@@ -29,6 +30,7 @@
 #include "mpm_common.h"
 #include "mpm_device.h"
 #include "mpm_hip.h"
+#include "mpm_ordered.h"
 
 using namespace mpm;
 
@@ -280,17 +282,17 @@ __device__ void block_add_params(float mub, float lamb, float alb, double* out) 
 }
 
 // ---------------------------------------------------------------------------
-// G2P adjoint.  st: S_n; gv: grid velocities of substep n; an: adjoint of
-// S_{n+1}; vb: grid-velocity adjoint (accumulated); ac: adjoint of S_n
-// (x and F written here, v, C, tau by k_p2g_adj).
+// G2P adjoint of particle p.  st: S_n; gv: grid velocities of substep n; an:
+// adjoint of S_{n+1}; ac: adjoint of S_n (x and F written here, v, C, tau by
+// k_p2g_adj).  scatter(base, i, j, k, vbn[3]) receives the particle's
+// grid-velocity adjoint contribution to stencil node (i, j, k).
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlk)
-    k_g2p_adj(const float* __restrict__ st, int64_t S, int n, MpmParams prm, float dt, const float4* __restrict__ gv,
-              const float* __restrict__ an, int64_t SA, float* __restrict__ vb, float* __restrict__ ac,
-              double* __restrict__ pb) {
-    const int p = blockIdx.x * kBlk + threadIdx.x;
-    float mub = 0.f, lamb = 0.f, alb = 0.f;
-    if (p < n) {
+template <typename Scatter>
+__device__ __forceinline__ void g2p_adj_particle(const float* __restrict__ st, int64_t S, int p, const MpmParams& prm,
+                                                 float dt, const float4* __restrict__ gv, const float* __restrict__ an,
+                                                 int64_t SA, float* __restrict__ ac, float& mub, float& lamb,
+                                                 float& alb, Scatter&& scatter) {
+    {
         const float* sp = st + p;
         float x[3];
 #pragma unroll
@@ -356,9 +358,7 @@ __global__ void __launch_bounds__(kBlk)
                             for (int d = 0; d < 3; ++d) t += c4 * w * Cb[a * 3 + d] * dp[d] + Gb[a * 3 + d] * g[d];
                             vbn[a] = t;
                         }
-                        unsafeAtomicAdd(vb + 4 * node + 0, vbn[0]);
-                        unsafeAtomicAdd(vb + 4 * node + 1, vbn[1]);
-                        unsafeAtomicAdd(vb + 4 * node + 2, vbn[2]);
+                        scatter(W.base, i, j, k, vbn);
                         // position adjoint through w, (x_i - x_p) and grad w
                         float vv = 0.f, cv[3] = {0, 0, 0};   // v_i . vt,  C_b^T v_i
 #pragma unroll
@@ -398,7 +398,71 @@ __global__ void __launch_bounds__(kBlk)
             }
         }
     }
+}
+
+// G2P adjoint over the chunks of an ordered state (mpm_sort_by_bin): the
+// grid-velocity adjoint is accumulated in an FP64 LDS tile at the chunk's
+// smallest stencil base and flushed once per node (chunk_flush); particles
+// whose stencil leaves the tile add to vb directly.
+__global__ void __launch_bounds__(kChunkWG)
+    k_g2p_adj_chunks(const float* __restrict__ st, int64_t S, int n, const int4* __restrict__ chunks,
+                     const int* __restrict__ meta, MpmParams prm, float dt, const float4* __restrict__ gv,
+                     const float* __restrict__ an, int64_t SA, float* __restrict__ vb, float* __restrict__ ac,
+                     double* __restrict__ pb) {
+    __shared__ double s_t[3 * kCT3];
+    __shared__ int s_lo[3];
+    const int c = blockIdx.x;
+    if (c >= meta[0]) return;
+    const int4 ch = chunks[c];
+    const int tid = threadIdx.x;
+    const int ng = prm.n_grid;
+    constexpr int kPer = kChunk / kChunkWG;
+    if (tid < 3) s_lo[tid] = INT_MAX;
+    for (int t = tid; t < 3 * kCT3; t += kChunkWG) s_t[t] = 0.0;
+    __syncthreads();
+    int lo[3] = {INT_MAX, INT_MAX, INT_MAX};
+#pragma unroll
+    for (int r = 0; r < kPer; ++r) {
+        const int i = r * kChunkWG + tid;
+        const int p = ch.x + i;
+        if (i >= ch.y || p < 0 || p >= n) continue;
+        int b[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+            b[d] = (int)((st[(MPM_P_X0 + d) * S + p] + st[(MPM_P_U + d) * S + p]) * prm.inv_dx - 0.5f);
+        if (b[0] < 0 || b[1] < 0 || b[2] < 0 || b[0] > ng - 3 || b[1] > ng - 3 || b[2] > ng - 3) continue;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) lo[d] = min(lo[d], b[d]);
+    }
+    chunk_min_base(s_lo, lo[0], lo[1], lo[2]);
+    __syncthreads();
+    const int o[3] = {s_lo[0], s_lo[1], s_lo[2]};
+    const bool any = o[0] != INT_MAX;
+    float mub = 0.f, lamb = 0.f, alb = 0.f;
+    for (int r = 0; r < kPer; ++r) {
+        const int i = r * kChunkWG + tid;
+        const int p = ch.x + i;
+        if (i >= ch.y || p < 0 || p >= n) continue;
+        g2p_adj_particle(st, S, p, prm, dt, gv, an, SA, ac, mub, lamb, alb,
+                         [&](const int base[3], int i3, int j3, int k3, const float vbn[3]) {
+                             const int ti = any ? chunk_tile_index(base, o) : -1;
+                             if (ti >= 0) {
+                                 const int t = ti + (i3 * kCT + j3) * kCT + k3;
+                                 atomicAdd(&s_t[t], (double)vbn[0]);
+                                 atomicAdd(&s_t[kCT3 + t], (double)vbn[1]);
+                                 atomicAdd(&s_t[2 * kCT3 + t], (double)vbn[2]);
+                             } else {
+                                 const int64_t node =
+                                     ((int64_t)(base[0] + i3) * ng + (base[1] + j3)) * ng + (base[2] + k3);
+                                 unsafeAtomicAdd(vb + 4 * node + 0, vbn[0]);
+                                 unsafeAtomicAdd(vb + 4 * node + 1, vbn[1]);
+                                 unsafeAtomicAdd(vb + 4 * node + 2, vbn[2]);
+                             }
+                         });
+    }
     block_add_params(mub, lamb, alb, pb);
+    __syncthreads();
+    if (any) chunk_flush<3>(s_t, o, ng, vb);
 }
 
 // ---------------------------------------------------------------------------
@@ -540,14 +604,15 @@ int64_t mpm_adjoint_workspace_bytes(const MpmParams* prm) {
     return 4 * nn * 16;   // acc (mv, m), grid v, v adjoint, (mv, m) adjoint
 }
 
-int mpm_adjoint_substep(const MpmParams* prm, const MpmState* st, const MpmCollider* colliders, int n_colliders,
-                        void* workspace, float time, float dt, const float* adj_next, float* adj_cur,
+int mpm_adjoint_substep(const MpmParams* prm, const MpmState* st, const void* order, const MpmCollider* colliders,
+                        int n_colliders, void* workspace, float time, float dt, const float* adj_next, float* adj_cur,
                         int64_t adj_stride, double* param_adj, void* stream) {
     int rc = check_params(prm);
     if (rc) return rc;
     if (!st || !st->planes || st->n < 0 || st->stride < st->n) return set_err(-4, "state invalid");
     if (st->n > 0x7fffffffLL) return set_err(-6, "more than 2^31-1 particles");
     if (!workspace || ((uintptr_t)workspace & 255)) return set_err(-11, "workspace NULL or not 256-byte aligned");
+    if (!order || ((uintptr_t)order & 255)) return set_err(-40, "order buffer NULL or not 256-byte aligned");
     if (!adj_next || !adj_cur || !param_adj || adj_stride < st->n) return set_err(-30, "adjoint buffers invalid");
     ColliderSet cs;
     if ((rc = make_collider_set(colliders, n_colliders, &cs))) return rc;
@@ -557,17 +622,21 @@ int mpm_adjoint_substep(const MpmParams* prm, const MpmState* st, const MpmColli
     float4* gv = acc + nn;
     float4* vbg = gv + nn;
     float4* gbg = vbg + nn;
-    // forward grid of this substep (P2G, normalization, colliders)
+    // forward grid of this substep (P2G over the chunks, normalization, colliders)
     if ((rc = mpm_zero_grid(prm, (float*)acc, stream))) return rc;
-    if ((rc = mpm_p2g(prm, st, (float*)acc, dt, stream))) return rc;
+    if ((rc = p2g_ordered(prm, st, order, (float*)acc, dt, stream))) return rc;
     if ((rc = mpm_grid_update(prm, (const float*)acc, (float*)gv, colliders, n_colliders, time, dt, stream))) return rc;
     hipLaunchKernelGGL(k_zero4, dim3(blocks(nn)), dim3(kBlk), 0, s, vbg, nn);
     if ((rc = check_launch("k_zero4"))) return rc;
     const int n = (int)st->n;
     if (n > 0) {
-        hipLaunchKernelGGL(k_g2p_adj, dim3(blocks(n)), dim3(kBlk), 0, s, (const float*)st->planes, st->stride, n, *prm,
-                           dt, (const float4*)gv, adj_next, adj_stride, (float*)vbg, adj_cur, param_adj);
-        if ((rc = check_launch("k_g2p_adj"))) return rc;
+        const OrderLayout L = order_layout(*prm, n);
+        const char* ob = (const char*)order;
+        hipLaunchKernelGGL(k_g2p_adj_chunks, dim3((unsigned)L.max_chunks), dim3(kChunkWG), 0, s,
+                           (const float*)st->planes, st->stride, n, (const int4*)(ob + L.off_chunks),
+                           (const int*)(ob + L.off_meta), *prm, dt, (const float4*)gv, adj_next, adj_stride,
+                           (float*)vbg, adj_cur, param_adj);
+        if ((rc = check_launch("k_g2p_adj_chunks"))) return rc;
     }
     hipLaunchKernelGGL(k_grid_adj, dim3(blocks(nn)), dim3(kBlk), 0, s, (const float4*)acc, (const float*)vbg, gbg,
                        *prm, cs, time, dt);

@@ -1,0 +1,113 @@
+// mpm_ordered.h -- the flat particle state in bin order and the chunked,
+// LDS-privatized scatter shared by the ordered P2G (mpm_ordered.hip) and the
+// ordered G2P adjoint (mpm_adjoint.hip).
+//
+// mpm_sort_by_bin counting-sorts a flat state by the 8^3-cell bin of each
+// particle's stencil base (mpm_utils.py:197-199) and, inside a bin, by its
+// 4^3-cell sub-block, and writes a chunk table: runs of at most kChunk
+// consecutive particles that never straddle two bins.  A chunk's particles
+// then span at most ~8 cells per axis (plus what they drift in the substeps
+// after the sort), so one workgroup accumulates a whole chunk's scatter into
+// a 12^3-node FP64 LDS tile placed at the chunk's smallest stencil base, and
+// flushes each touched node to the global grid with one float atomic per
+// component -- instead of 27 atomics per particle per component.  Particles
+// whose stencil leaves the tile go straight to the global grid.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <limits.h>
+
+#include "mpm_hip.h"
+
+namespace mpm {
+
+constexpr int kChunk = 512;     // particles per chunk
+constexpr int kChunkWG = 256;   // threads per chunk workgroup (2 particles each)
+constexpr int kCT = 12;         // LDS tile edge in nodes: bases o .. o+9 fit
+constexpr int kCT3 = kCT * kCT * kCT;
+
+// Byte layout of the order buffer (256-byte aligned regions).
+struct OrderLayout {
+    int64_t nb, nbins, nkeys, max_chunks;
+    int64_t off_hist;     // int[nkeys + 1]: key counts, then exclusive offsets
+    int64_t off_cursor;   // int[nkeys]: scatter cursors
+    int64_t off_perm;     // int[n]: sorted slot -> source slot
+    int64_t off_chunks;   // int4[max_chunks]: (first slot, count, bin, 0)
+    int64_t off_meta;     // int[64]: [0] = number of chunks
+    int64_t bytes;
+};
+
+inline OrderLayout order_layout(const MpmParams& prm, int64_t n) {
+    auto up = [](int64_t b) { return (b + 255) & ~int64_t(255); };
+    OrderLayout L;
+    L.nb = (prm.n_grid + 7) / 8;
+    L.nbins = L.nb * L.nb * L.nb;
+    L.nkeys = 8 * L.nbins;
+    // every non-empty bin ends in at most one partial chunk
+    L.max_chunks = (n + kChunk - 1) / kChunk + (L.nbins < n ? L.nbins : n);
+    int64_t o = 0;
+    L.off_hist = o;   o += up(4 * (L.nkeys + 1));
+    L.off_cursor = o; o += up(4 * L.nkeys);
+    L.off_perm = o;   o += up(4 * (n > 0 ? n : 1));
+    L.off_chunks = o; o += up(16 * (L.max_chunks > 0 ? L.max_chunks : 1));
+    L.off_meta = o;   o += 256;
+    L.bytes = o;
+    return L;
+}
+
+// sort key of a stencil base: bin (8^3 cells, C order) then sub-block (4^3)
+__device__ __forceinline__ int order_key(const int base[3], int ng, int nb) {
+    int b[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) b[d] = min(max(base[d], 0), ng - 3);
+    const int bin = ((b[0] >> 3) * nb + (b[1] >> 3)) * nb + (b[2] >> 3);
+    const int sub = ((((b[0] >> 2) & 1) * 2 + ((b[1] >> 2) & 1)) * 2) + ((b[2] >> 2) & 1);
+    return bin * 8 + sub;
+}
+
+// Componentwise minimum over the workgroup of per-thread stencil bases
+// (INT_MAX where a thread has none) into s_lo[3], which the caller set to
+// INT_MAX before a barrier; a barrier must follow before s_lo is read.
+__device__ __forceinline__ void chunk_min_base(int* s_lo, int bx, int by, int bz) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        bx = min(bx, __shfl_xor(bx, o));
+        by = min(by, __shfl_xor(by, o));
+        bz = min(bz, __shfl_xor(bz, o));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMin(&s_lo[0], bx);
+        atomicMin(&s_lo[1], by);
+        atomicMin(&s_lo[2], bz);
+    }
+}
+
+// tile index of stencil node (0,0,0) of a particle with base b in the tile at
+// origin o, or -1 when its 3^3 stencil leaves the tile
+__device__ __forceinline__ int chunk_tile_index(const int b[3], const int o[3]) {
+    const int x = b[0] - o[0], y = b[1] - o[1], z = b[2] - o[2];
+    if (x < 0 || y < 0 || z < 0 || x > kCT - 3 || y > kCT - 3 || z > kCT - 3) return -1;
+    return (x * kCT + y) * kCT + z;
+}
+
+// Adds NC components of the LDS tile (SoA, kCT3 doubles each) at origin o to
+// the float4 grid (components 0 .. NC-1 of each node), skipping zeros.
+template <int NC>
+__device__ __forceinline__ void chunk_flush(const double* s_t, const int o[3], int ng, float* __restrict__ grid) {
+    for (int t = threadIdx.x; t < kCT3; t += blockDim.x) {
+        const int x = o[0] + t / (kCT * kCT), y = o[1] + (t / kCT) % kCT, z = o[2] + t % kCT;
+        if (x >= ng || y >= ng || z >= ng) continue;
+        float* g = grid + 4 * (((int64_t)x * ng + y) * ng + z);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const double v = s_t[c * kCT3 + t];
+            if (v != 0.0) unsafeAtomicAdd(g + c, (float)v);
+        }
+    }
+}
+
+// host: the chunked P2G (p2g_apic_stress_given) of an ordered state into a
+// zeroed float4 accumulator grid (mpm_ordered.hip)
+int p2g_ordered(const MpmParams* prm, const MpmState* st, const void* order, float* grid_acc, float dt,
+                void* stream);
+
+}  // namespace mpm

@@ -1,0 +1,309 @@
+// mpm_ordered.hip -- the flat state in bin order (C-ABI: mpm_order_bytes,
+// mpm_sort_by_bin, mpm_step_ordered, mpm_permute in include/mpm_hip.h).
+//
+// The stage path's P2G (k_p2g_atomic) issues 27 x 4 float atomics per
+// particle into the global grid; with every atomic resolved far from the
+// CU, it is the cost of a flat substep.  Here the flat state is first
+// counting-sorted by bin and sub-block of its stencil base (k_order_hist,
+// k_order_scan, k_order_scatter), which also yields a table of chunks --
+// runs of at most kChunk particles inside one bin.  k_p2g_chunks then gives
+// each chunk one workgroup that scatters into an FP64 LDS tile placed at the
+// chunk's smallest stencil base (ds_add_f64) and flushes the tile with one
+// float atomic per touched node and component.  The order stays valid while
+// particles drift (a particle whose stencil leaves its chunk's tile scatters
+// straight to the grid), so a caller sorts once and steps many substeps.
+#include <hip/hip_runtime.h>
+#include <limits.h>
+
+#include "mpm_common.h"
+#include "mpm_device.h"
+#include "mpm_hip.h"
+#include "mpm_ordered.h"
+#include "mpm_stencil.h"
+
+using namespace mpm;
+
+namespace {
+
+constexpr int kB = 256;       // threads of the per-particle kernels
+constexpr int kScan = 1024;   // threads of the single-workgroup scan
+
+__device__ __forceinline__ void stencil_base(const float* __restrict__ sp, int64_t S, float inv_dx, int base[3]) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const float x = sp[(MPM_P_X0 + d) * S] + sp[(MPM_P_U + d) * S];
+        base[d] = (int)(x * inv_dx - 0.5f);   // mpm_utils.py:197-199
+    }
+}
+
+__device__ __forceinline__ bool base_ok(const int b[3], int ng) {
+    return b[0] >= 0 && b[1] >= 0 && b[2] >= 0 && b[0] <= ng - 3 && b[1] <= ng - 3 && b[2] <= ng - 3;
+}
+
+__global__ void k_order_fill(int* __restrict__ p, int64_t n, int v) {
+    const int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void __launch_bounds__(kB)
+    k_order_hist(const float* __restrict__ planes, int64_t S, int n, float inv_dx, int ng, int nb,
+                 int* __restrict__ hist) {
+    const int p = blockIdx.x * kB + threadIdx.x;
+    if (p >= n) return;
+    int b[3];
+    stencil_base(planes + p, S, inv_dx, b);
+    atomicAdd(&hist[order_key(b, ng, nb)], 1);
+}
+
+// One workgroup: exclusive offsets of the key counts (in place, and copied
+// to the cursors) and the chunk table.  Thread t owns a contiguous range of
+// bins, so every bin's keys and chunks are handled by one thread.
+__global__ void __launch_bounds__(kScan)
+    k_order_scan(int* __restrict__ hist, int* __restrict__ cursor, int4* __restrict__ chunks, int* __restrict__ meta,
+                 int nbins, int max_chunks) {
+    __shared__ int s_p[kScan], s_c[kScan];
+    const int tid = threadIdx.x;
+    const int per = (nbins + kScan - 1) / kScan;
+    const int b_lo = min(tid * per, nbins), b_hi = min(b_lo + per, nbins);
+    int np = 0, nc = 0;
+    for (int b = b_lo; b < b_hi; ++b) {
+        int c = 0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) c += hist[8 * b + k];
+        np += c;
+        nc += (c + kChunk - 1) / kChunk;
+    }
+    s_p[tid] = np;
+    s_c[tid] = nc;
+    __syncthreads();
+    for (int o = 1; o < kScan; o <<= 1) {   // inclusive scan
+        const int a = tid >= o ? s_p[tid - o] : 0, ac = tid >= o ? s_c[tid - o] : 0;
+        __syncthreads();
+        s_p[tid] += a;
+        s_c[tid] += ac;
+        __syncthreads();
+    }
+    int run = s_p[tid] - np, crun = s_c[tid] - nc;
+    for (int b = b_lo; b < b_hi; ++b) {
+        const int first = run;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int c = hist[8 * b + k];
+            hist[8 * b + k] = run;
+            cursor[8 * b + k] = run;
+            run += c;
+        }
+        const int cnt = run - first;
+        for (int s = 0; s < cnt; s += kChunk, ++crun)
+            if (crun < max_chunks) chunks[crun] = make_int4(first + s, min(kChunk, cnt - s), b, 0);
+    }
+    if (tid == kScan - 1) {
+        hist[8 * nbins] = s_p[kScan - 1];
+        meta[0] = min(s_c[kScan - 1], max_chunks);
+    }
+}
+
+__global__ void __launch_bounds__(kB)
+    k_order_scatter(const float* __restrict__ src, int64_t SS, float* __restrict__ dst, int64_t DS, int n,
+                    float inv_dx, int ng, int nb, int* __restrict__ cursor, int* __restrict__ perm) {
+    const int p = blockIdx.x * kB + threadIdx.x;
+    if (p >= n) return;
+    int b[3];
+    stencil_base(src + p, SS, inv_dx, b);
+    const int pos = atomicAdd(&cursor[order_key(b, ng, nb)], 1);
+    if (pos < 0 || pos >= n) return;   // cannot happen: same keys as the histogram
+#pragma unroll 13
+    for (int k = 0; k < MPM_NPLANES; ++k) dst[(int64_t)k * DS + pos] = src[(int64_t)k * SS + p];
+    perm[pos] = p;
+}
+
+// ---------------------------------------------------------------------------
+// p2g_apic_stress_given (mpm_utils.py:190-227) over chunks: one workgroup per
+// chunk, kChunk / kChunkWG particles per thread.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kChunkWG)
+    k_p2g_chunks(const float* __restrict__ planes, int64_t S, int n, const int4* __restrict__ chunks,
+                 const int* __restrict__ meta, MpmParams prm, float dt, float* __restrict__ grid) {
+    __shared__ double s_t[4 * kCT3];
+    __shared__ int s_lo[3];
+    const int c = blockIdx.x;
+    if (c >= meta[0]) return;
+    const int4 ch = chunks[c];
+    const int tid = threadIdx.x;
+    const int ng = prm.n_grid;
+    constexpr int kPer = kChunk / kChunkWG;
+    if (tid < 3) s_lo[tid] = INT_MAX;
+    for (int t = tid; t < 4 * kCT3; t += kChunkWG) s_t[t] = 0.0;
+    __syncthreads();
+    int lo[3] = {INT_MAX, INT_MAX, INT_MAX};
+#pragma unroll
+    for (int r = 0; r < kPer; ++r) {
+        const int i = r * kChunkWG + tid;
+        const int p = ch.x + i;
+        if (i >= ch.y || p < 0 || p >= n) continue;
+        int b[3];
+        stencil_base(planes + p, S, prm.inv_dx, b);
+        if (!base_ok(b, ng)) continue;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) lo[d] = min(lo[d], b[d]);
+    }
+    chunk_min_base(s_lo, lo[0], lo[1], lo[2]);
+    __syncthreads();
+    if (s_lo[0] == INT_MAX) return;   // no particle inside the grid (uniform)
+    const int o[3] = {s_lo[0], s_lo[1], s_lo[2]};
+    for (int r = 0; r < kPer; ++r) {
+        const int i = r * kChunkWG + tid;
+        const int p = ch.x + i;
+        if (i >= ch.y || p < 0 || p >= n) continue;
+        const float* sp = planes + p;
+        float x[3], v[3], Cm[9], tau[9];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            x[d] = sp[(MPM_P_X0 + d) * S] + sp[(MPM_P_U + d) * S];
+            v[d] = sp[(MPM_P_V + d) * S];
+        }
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            Cm[k] = sp[(MPM_P_C + k) * S];
+            tau[k] = sp[(MPM_P_TAU + k) * S];
+        }
+        const float m = sp[MPM_P_MASS * S], vol = sp[MPM_P_VOL * S];
+        AxisWeights A;
+        axis_weights(x, prm.inv_dx, A);
+        if (!base_ok(A.base, ng)) continue;
+        const int ti = chunk_tile_index(A.base, o);
+        if (ti >= 0) {
+            p2g_contrib(A, m, vol, v, Cm, tau, prm.dx, prm.inv_dx, dt, [&](int cc, int i3, int j3, int k3, float val) {
+                atomicAdd(&s_t[cc * kCT3 + ti + (i3 * kCT + j3) * kCT + k3], (double)val);
+            });
+        } else {
+            p2g_contrib(A, m, vol, v, Cm, tau, prm.dx, prm.inv_dx, dt, [&](int cc, int i3, int j3, int k3, float val) {
+                const int64_t node = ((int64_t)(A.base[0] + i3) * ng + (A.base[1] + j3)) * ng + (A.base[2] + k3);
+                unsafeAtomicAdd(grid + 4 * node + cc, val);
+            });
+        }
+    }
+    __syncthreads();
+    chunk_flush<4>(s_t, o, ng, grid);
+}
+
+__global__ void __launch_bounds__(kB)
+    k_permute(const float* __restrict__ src, int64_t ss, float* __restrict__ dst, int64_t ds, int nplanes,
+              const int* __restrict__ idx, int n, int scatter) {
+    const int j = blockIdx.x * kB + threadIdx.x;
+    if (j >= n) return;
+    const int q = idx[j];
+    if (q < 0 || q >= n) return;
+    const int from = scatter ? j : q, to = scatter ? q : j;
+    for (int p = 0; p < nplanes; ++p) dst[(int64_t)p * ds + to] = src[(int64_t)p * ss + from];
+}
+
+unsigned nblk(int64_t n, int b = kB) { return (unsigned)((n + b - 1) / b); }
+
+int check_flat(const MpmState* st) {
+    if (!st || !st->planes) return set_err(-4, "state or planes is NULL");
+    if (st->n < 0 || st->stride < st->n) return set_err(-5, "state n/stride invalid");
+    if (st->n > 0x7fffffffLL) return set_err(-6, "more than 2^31-1 particles");
+    return 0;
+}
+
+}  // namespace
+
+namespace mpm {
+
+int p2g_ordered(const MpmParams* prm, const MpmState* st, const void* order, float* grid_acc, float dt,
+                void* stream) {
+    int rc = check_params(prm);
+    if (rc) return rc;
+    if ((rc = check_flat(st))) return rc;
+    if (!order || ((uintptr_t)order & 255)) return set_err(-40, "order buffer NULL or not 256-byte aligned");
+    if (!grid_acc) return set_err(-7, "grid is NULL");
+    if (st->n == 0) return 0;
+    const OrderLayout L = order_layout(*prm, st->n);
+    const char* ob = (const char*)order;
+    hipLaunchKernelGGL(k_p2g_chunks, dim3((unsigned)L.max_chunks), dim3(kChunkWG), 0, (hipStream_t)stream,
+                       (const float*)st->planes, st->stride, (int)st->n, (const int4*)(ob + L.off_chunks),
+                       (const int*)(ob + L.off_meta), *prm, dt, grid_acc);
+    return check_launch("k_p2g_chunks");
+}
+
+}  // namespace mpm
+
+extern "C" {
+
+int64_t mpm_order_bytes(const MpmParams* prm, int64_t n_particles) {
+    if (check_grid(prm) || n_particles < 0) return -1;
+    return order_layout(*prm, n_particles).bytes;
+}
+
+int64_t mpm_order_perm_offset(const MpmParams* prm, int64_t n_particles) {
+    if (check_grid(prm) || n_particles < 0) return -1;
+    return order_layout(*prm, n_particles).off_perm;
+}
+
+int mpm_sort_by_bin(const MpmParams* prm, const MpmState* src, const MpmState* dst, void* order, void* stream) {
+    int rc = check_grid(prm);
+    if (rc) return rc;
+    if ((rc = check_flat(src)) || (rc = check_flat(dst))) return rc;
+    if (src->n != dst->n) return set_err(-41, "sort: src and dst hold different particle counts");
+    if (src->planes == dst->planes) return set_err(-42, "sort: src and dst must not alias");
+    if (!order || ((uintptr_t)order & 255)) return set_err(-40, "order buffer NULL or not 256-byte aligned");
+    const int64_t n = src->n;
+    const OrderLayout L = order_layout(*prm, n);
+    if (L.nbins > INT_MAX / 8) return set_err(-43, "sort: too many bins");
+    char* ob = (char*)order;
+    int* hist = (int*)(ob + L.off_hist);
+    int* cursor = (int*)(ob + L.off_cursor);
+    int* perm = (int*)(ob + L.off_perm);
+    int4* chunks = (int4*)(ob + L.off_chunks);
+    int* meta = (int*)(ob + L.off_meta);
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_order_fill, dim3(nblk(L.nkeys + 1)), dim3(kB), 0, s, hist, L.nkeys + 1, 0);
+    if ((rc = check_launch("k_order_fill"))) return rc;
+    if (n > 0) {
+        hipLaunchKernelGGL(k_order_hist, dim3(nblk(n)), dim3(kB), 0, s, (const float*)src->planes, src->stride,
+                           (int)n, prm->inv_dx, prm->n_grid, (int)L.nb, hist);
+        if ((rc = check_launch("k_order_hist"))) return rc;
+    }
+    hipLaunchKernelGGL(k_order_scan, dim3(1), dim3(kScan), 0, s, hist, cursor, chunks, meta, (int)L.nbins,
+                       (int)L.max_chunks);
+    if ((rc = check_launch("k_order_scan"))) return rc;
+    if (n > 0) {
+        hipLaunchKernelGGL(k_order_scatter, dim3(nblk(n)), dim3(kB), 0, s, (const float*)src->planes, src->stride,
+                           (float*)dst->planes, dst->stride, (int)n, prm->inv_dx, prm->n_grid, (int)L.nb, cursor,
+                           perm);
+        if ((rc = check_launch("k_order_scatter"))) return rc;
+    }
+    return 0;
+}
+
+int mpm_step_ordered(const MpmParams* prm, const MpmState* st, const void* order, const MpmCollider* colliders,
+                     int n_colliders, void* workspace, int64_t workspace_bytes, float time, float dt, void* stream) {
+    int rc = check_params(prm);
+    if (rc) return rc;
+    if ((rc = check_flat(st))) return rc;
+    const int64_t need = mpm_workspace_bytes(prm, st->n);
+    if (!workspace || workspace_bytes < need) return set_err(-10, "workspace too small");
+    if (((uintptr_t)workspace) & 255) return set_err(-11, "workspace not 256-byte aligned");
+    const int64_t nn = (int64_t)prm->n_grid * prm->n_grid * prm->n_grid;
+    float* acc = (float*)workspace;
+    float* gv = acc + 4 * nn;
+    if ((rc = mpm_zero_grid(prm, acc, stream))) return rc;
+    if ((rc = p2g_ordered(prm, st, order, acc, dt, stream))) return rc;
+    if ((rc = mpm_grid_update(prm, acc, gv, colliders, n_colliders, time, dt, stream))) return rc;
+    return mpm_g2p(prm, st, gv, dt, stream);
+}
+
+int mpm_permute(const float* src, int64_t src_stride, float* dst, int64_t dst_stride, int n_planes, const int* idx,
+                int64_t n, int scatter, void* stream) {
+    if (!src || !dst || !idx) return set_err(-44, "permute: NULL pointer");
+    if (n < 0 || n > 0x7fffffffLL || src_stride < n || dst_stride < n || n_planes < 1)
+        return set_err(-45, "permute: bad sizes");
+    if (src == dst) return set_err(-46, "permute: src and dst must not alias");
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_permute, dim3(nblk(n)), dim3(kB), 0, (hipStream_t)stream, src, src_stride, dst, dst_stride,
+                       n_planes, idx, (int)n, scatter ? 1 : 0);
+    return check_launch("k_permute");
+}
+
+}  // extern "C"
