@@ -242,13 +242,28 @@ int mpm_bins_run(const MpmParams* prm, MpmBins* bins, const MpmCollider* collide
 int64_t mpm_adjoint_workspace_bytes(const MpmParams* prm);
 
 /* st: the state S_n before the substep, in the order of `order` (see
- * mpm_sort_by_bin; the substep's grid is recomputed as mpm_step_ordered
- * computes it); time, dt: that substep's.  adj_next: adjoint of S_{n+1};
- * adj_cur: receives the adjoint of S_n (may not alias adj_next).
- * param_adj[0..2] += adjoints of mu, lam, alpha (double). */
-int mpm_adjoint_substep(const MpmParams* prm, const MpmState* st, const void* order, const MpmCollider* colliders,
-                        int n_colliders, void* workspace, float time, float dt, const float* adj_next, float* adj_cur,
-                        int64_t adj_stride, double* param_adj, void* stream);
+ * mpm_sort_by_bin); time, dt: that substep's.  step_grids: the workspace
+ * mpm_step_ordered used for this substep (its P2G accumulator and grid
+ * velocities, untouched since), or NULL to recompute them here the same way.
+ * adj_next: adjoint of S_{n+1}; adj_cur: receives the adjoint of S_n (may
+ * not alias adj_next).  param_adj[0..2] += adjoints of mu, lam, alpha
+ * (double). */
+int mpm_adjoint_substep(const MpmParams* prm, const MpmState* st, const void* order, const void* step_grids,
+                        const MpmCollider* colliders, int n_colliders, void* workspace, float time, float dt,
+                        const float* adj_next, float* adj_cur, int64_t adj_stride, double* param_adj, void* stream);
+
+/* The adjoint of the n_sub substeps k0 .. k0+n_sub-1 of one ordered segment
+ * in one call: replays the segment from its first state (mpm_step_ordered,
+ * keeping each substep's grids), then applies the adjoint substeps in
+ * reverse.  states: n_sub states sharing n and stride; states[0] holds S_k0
+ * on entry, states[1..] receive S_{k0+1} .. S_{k0+n_sub-1}.  step_grids:
+ * n_sub - 1 workspaces of mpm_workspace_bytes (256-byte aligned).  Substep
+ * k runs at time (float)(k * dt) with step (float)dt.  adj_a holds the
+ * adjoint of S_{k0+n_sub} on entry and the adjoint of S_k0 on return; adj_b is
+ * scratch of the same size.  workspace: mpm_adjoint_workspace_bytes. */
+int mpm_adjoint_segment(const MpmParams* prm, const MpmState* states, const void* order, void* const* step_grids,
+                        const MpmCollider* colliders, int n_colliders, void* workspace, int k0, double dt, int n_sub,
+                        float* adj_a, float* adj_b, int64_t adj_stride, double* param_adj, void* stream);
 
 /* ---- the flat state in bin order -------------------------------------
  *
@@ -271,6 +286,12 @@ int mpm_sort_by_bin(const MpmParams* prm, const MpmState* src, const MpmState* d
 /* p2g2p on a state in the slot order of `order`; workspace as mpm_step. */
 int mpm_step_ordered(const MpmParams* prm, const MpmState* st, const void* order, const MpmCollider* colliders,
                      int n_colliders, void* workspace, int64_t workspace_bytes, float time, float dt, void* stream);
+
+/* n_sub substeps k0 .. k0+n_sub-1 of mpm_step_ordered, substep k at time
+ * (float)(k * dt) with step (float)dt. */
+int mpm_run_ordered(const MpmParams* prm, const MpmState* st, const void* order, const MpmCollider* colliders,
+                    int n_colliders, void* workspace, int64_t workspace_bytes, int k0, double dt, int n_sub,
+                    void* stream);
 
 /* Plane-wise permutation of n columns: scatter = 0: dst[:, j] = src[:, idx[j]];
  * scatter = 1: dst[:, idx[j]] = src[:, j].  idx: n int32 (a sort's perm, or a

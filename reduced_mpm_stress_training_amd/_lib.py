@@ -103,9 +103,14 @@ _SIGNATURES = {
     "mpm_batch_run": (C.c_int, [C.POINTER(MpmBatch), C.POINTER(MpmBins), C.c_double, C.c_float, C.c_int,
                                 C.c_void_p]),
     "mpm_adjoint_workspace_bytes": (C.c_int64, [C.POINTER(MpmParams)]),
-    "mpm_adjoint_substep": (C.c_int, [C.POINTER(MpmParams), C.POINTER(MpmState), C.c_void_p,
+    "mpm_adjoint_substep": (C.c_int, [C.POINTER(MpmParams), C.POINTER(MpmState), C.c_void_p, C.c_void_p,
                                       C.POINTER(MpmCollider), C.c_int, C.c_void_p, C.c_float, C.c_float, C.c_void_p,
                                       C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
+    "mpm_adjoint_segment": (C.c_int, [C.POINTER(MpmParams), C.POINTER(MpmState), C.c_void_p, C.c_void_p,
+                                      C.POINTER(MpmCollider), C.c_int, C.c_void_p, C.c_int, C.c_double, C.c_int,
+                                      C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
+    "mpm_run_ordered": (C.c_int, [C.POINTER(MpmParams), C.POINTER(MpmState), C.c_void_p, C.POINTER(MpmCollider),
+                                  C.c_int, C.c_void_p, C.c_int64, C.c_int, C.c_double, C.c_int, C.c_void_p]),
     "mpm_order_bytes": (C.c_int64, [C.POINTER(MpmParams), C.c_int64]),
     "mpm_order_perm_offset": (C.c_int64, [C.POINTER(MpmParams), C.c_int64]),
     "mpm_sort_by_bin": (C.c_int, [C.POINTER(MpmParams), C.POINTER(MpmState), C.POINTER(MpmState), C.c_void_p,

@@ -15,6 +15,7 @@ three material scalars.
 """
 from __future__ import annotations
 
+import ctypes
 import math
 
 import numpy as np
@@ -97,6 +98,8 @@ class Rollout:
         self._init = planes
         self._checkpoints = {}
         self._final = None
+        self._ws_seg = []
+        self._seg_states = []
 
     @staticmethod
     def _aligned(buf: torch.Tensor) -> int:
@@ -122,9 +125,10 @@ class Rollout:
                                          self._stream()), "mpm_sort_by_bin")
         return dst, order
 
-    def _step(self, planes, k, order):
+    def _step(self, planes, k, order, ws=None):
+        ws = self._ws_step if ws is None else ws
         L.check(L.load().mpm_step_ordered(self._prm, self._state(planes), self._aligned(order), self._cols,
-                                          self._ncol, self._aligned(self._ws_step), self._ws_step_bytes,
+                                          self._ncol, self._aligned(ws), self._ws_step_bytes,
                                           float(k * self.dt), self.dt, self._stream()), "mpm_step_ordered")
 
     def _permute(self, src, n_planes, idx_ptr, scatter):
@@ -141,12 +145,13 @@ class Rollout:
         planes = self._init.clone()
         planes[L.P_V:L.P_V + 3] = v0.to(self.device, torch.float32).t()
         self._checkpoints = {}
-        order = None
-        for k in range(self.n_substeps):
-            if k % self.K == 0:
-                planes, order = self._sort(planes)
-                self._checkpoints[k] = (planes.clone(), order)
-            self._step(planes, k, order)
+        lib = L.load()
+        for c0 in range(0, self.n_substeps, self.K):
+            planes, order = self._sort(planes)
+            self._checkpoints[c0] = (planes.clone(), order)
+            L.check(lib.mpm_run_ordered(self._prm, self._state(planes), self._aligned(order), self._cols,
+                                        self._ncol, self._aligned(self._ws_step), self._ws_step_bytes, c0, self.dt,
+                                        min(self.K, self.n_substeps - c0), self._stream()), "mpm_run_ordered")
         self._final = planes
         # x0 and u back to the caller's order (scatter by the pid plane)
         xu = self._permute(planes[L.P_X0:L.P_U + 3], 6, planes[L.P_PID].data_ptr(), scatter=True)
@@ -165,21 +170,22 @@ class Rollout:
         adj[L.ADJ_X:L.ADJ_X + 3] = self._permute(xb, 3, self._final[L.P_PID].data_ptr(), scatter=False)
         nxt = torch.empty_like(adj)
         pb = torch.zeros(3, dtype=torch.float64, device=self.device)
+        # a segment's replayed states and, per replayed substep, the grids it
+        # leaves for its adjoint (one native call per segment)
+        while len(self._seg_states) < self.K - 1:
+            self._seg_states.append(torch.empty((L.NPLANES, self.n), dtype=torch.float32, device=self.device))
+            self._ws_seg.append(torch.empty(self._ws_step_bytes + 256, dtype=torch.uint8, device=self.device))
         for c0 in sorted(self._checkpoints, reverse=True):
-            c1 = min(c0 + self.K, self.n_substeps)
+            nk = min(self.K, self.n_substeps - c0)
             cp, order = self._checkpoints[c0]
-            states = [cp.clone()]
-            for k in range(c0, c1 - 1):
-                s = states[-1].clone()
-                self._step(s, k, order)
-                states.append(s)
-            for k in range(c1 - 1, c0 - 1, -1):
-                nxt.zero_()
-                L.check(lib.mpm_adjoint_substep(self._prm, self._state(states[k - c0]), self._aligned(order),
-                                                self._cols, self._ncol, self._aligned(self._ws_adj),
-                                                float(k * self.dt), self.dt, adj.data_ptr(), nxt.data_ptr(), self.n,
-                                                pb.data_ptr(), stream), "mpm_adjoint_substep")
-                adj, nxt = nxt, adj
+            states = (L.MpmState * nk)()
+            states[0] = self._state(cp)
+            for j in range(1, nk):
+                states[j] = self._state(self._seg_states[j - 1])
+            grids = (ctypes.c_void_p * max(1, nk - 1))(*[self._aligned(w) for w in self._ws_seg[:nk - 1]])
+            L.check(lib.mpm_adjoint_segment(self._prm, states, self._aligned(order), grids, self._cols, self._ncol,
+                                            self._aligned(self._ws_adj), c0, self.dt, nk, adj.data_ptr(),
+                                            nxt.data_ptr(), self.n, pb.data_ptr(), stream), "mpm_adjoint_segment")
             # adjoint of the sort at c0: its inverse permutation
             adj = self._permute(adj, L.ADJ_NPLANES, self._aligned(order) + self._perm_off, scatter=True)
         v0_bar = adj[L.ADJ_V:L.ADJ_V + 3].t().contiguous()

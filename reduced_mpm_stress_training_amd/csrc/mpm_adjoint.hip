@@ -604,9 +604,9 @@ int64_t mpm_adjoint_workspace_bytes(const MpmParams* prm) {
     return 4 * nn * 16;   // acc (mv, m), grid v, v adjoint, (mv, m) adjoint
 }
 
-int mpm_adjoint_substep(const MpmParams* prm, const MpmState* st, const void* order, const MpmCollider* colliders,
-                        int n_colliders, void* workspace, float time, float dt, const float* adj_next, float* adj_cur,
-                        int64_t adj_stride, double* param_adj, void* stream) {
+int mpm_adjoint_substep(const MpmParams* prm, const MpmState* st, const void* order, const void* step_grids,
+                        const MpmCollider* colliders, int n_colliders, void* workspace, float time, float dt,
+                        const float* adj_next, float* adj_cur, int64_t adj_stride, double* param_adj, void* stream) {
     int rc = check_params(prm);
     if (rc) return rc;
     if (!st || !st->planes || st->n < 0 || st->stride < st->n) return set_err(-4, "state invalid");
@@ -618,14 +618,22 @@ int mpm_adjoint_substep(const MpmParams* prm, const MpmState* st, const void* or
     if ((rc = make_collider_set(colliders, n_colliders, &cs))) return rc;
     hipStream_t s = (hipStream_t)stream;
     const int64_t nn = (int64_t)prm->n_grid * prm->n_grid * prm->n_grid;
+    if (step_grids && ((uintptr_t)step_grids & 255)) return set_err(-11, "step grids not 256-byte aligned");
     float4* acc = (float4*)workspace;
     float4* gv = acc + nn;
     float4* vbg = gv + nn;
     float4* gbg = vbg + nn;
-    // forward grid of this substep (P2G over the chunks, normalization, colliders)
-    if ((rc = mpm_zero_grid(prm, (float*)acc, stream))) return rc;
-    if ((rc = p2g_ordered(prm, st, order, (float*)acc, dt, stream))) return rc;
-    if ((rc = mpm_grid_update(prm, (const float*)acc, (float*)gv, colliders, n_colliders, time, dt, stream))) return rc;
+    if (step_grids) {
+        // the grids mpm_step_ordered left in its workspace for this substep
+        acc = (float4*)step_grids;
+        gv = acc + nn;
+    } else {
+        // forward grid of this substep (P2G over the chunks, normalization, colliders)
+        if ((rc = mpm_zero_grid(prm, (float*)acc, stream))) return rc;
+        if ((rc = p2g_ordered(prm, st, order, (float*)acc, dt, stream))) return rc;
+        if ((rc = mpm_grid_update(prm, (const float*)acc, (float*)gv, colliders, n_colliders, time, dt, stream)))
+            return rc;
+    }
     hipLaunchKernelGGL(k_zero4, dim3(blocks(nn)), dim3(kBlk), 0, s, vbg, nn);
     if ((rc = check_launch("k_zero4"))) return rc;
     const int n = (int)st->n;
@@ -648,5 +656,46 @@ int mpm_adjoint_substep(const MpmParams* prm, const MpmState* st, const void* or
     }
     return 0;
 }
+
+int mpm_adjoint_segment(const MpmParams* prm, const MpmState* states, const void* order, void* const* step_grids,
+                        const MpmCollider* colliders, int n_colliders, void* workspace, int k0, double dt, int n_sub,
+                        float* adj_a, float* adj_b, int64_t adj_stride, double* param_adj, void* stream) {
+    if (!states || !step_grids || n_sub < 1) return set_err(-31, "segment: states, grids or n_sub invalid");
+    if (!adj_a || !adj_b || adj_a == adj_b) return set_err(-30, "adjoint buffers invalid");
+    const int64_t stride = states[0].stride, n = states[0].n;
+    for (int j = 0; j < n_sub; ++j)
+        if (!states[j].planes || states[j].stride != stride || states[j].n != n)
+            return set_err(-32, "segment: states must share n and stride");
+    const int64_t ws_bytes = mpm_workspace_bytes(prm, n);
+    if (ws_bytes < 0) return set_err(-2, "n_grid out of range");
+    hipStream_t s = (hipStream_t)stream;
+    const float fdt = (float)dt;
+    int rc;
+    // replay: S_{k0+j+1} = step(S_{k0+j}), each substep's grids kept
+    for (int j = 0; j + 1 < n_sub; ++j) {
+        if (hipMemcpyAsync(states[j + 1].planes, states[j].planes, (size_t)MPM_NPLANES * stride * sizeof(float),
+                           hipMemcpyDeviceToDevice, s) != hipSuccess)
+            return set_err(-33, "segment: state copy failed");
+        if ((rc = mpm_step_ordered(prm, &states[j + 1], order, colliders, n_colliders, step_grids[j], ws_bytes,
+                                   (float)((double)(k0 + j) * dt), fdt, stream)))
+            return rc;
+    }
+    // adjoint substeps in reverse; the last one recomputes its grids
+    float* cur = adj_a;
+    float* nxt = adj_b;
+    for (int j = n_sub - 1; j >= 0; --j) {
+        const void* grids = j + 1 < n_sub ? step_grids[j] : nullptr;
+        if ((rc = mpm_adjoint_substep(prm, &states[j], order, grids, colliders, n_colliders, workspace,
+                                      (float)((double)(k0 + j) * dt), fdt, cur, nxt, adj_stride, param_adj, stream)))
+            return rc;
+        float* t = cur; cur = nxt; nxt = t;
+    }
+    if (cur != adj_a &&
+        hipMemcpyAsync(adj_a, cur, (size_t)MPM_ADJ_NPLANES * adj_stride * sizeof(float), hipMemcpyDeviceToDevice, s) !=
+            hipSuccess)
+        return set_err(-33, "segment: adjoint copy failed");
+    return 0;
+}
+
 
 }  // extern "C"

@@ -45,14 +45,34 @@ __global__ void k_order_fill(int* __restrict__ p, int64_t n, int v) {
     if (i < n) p[i] = v;
 }
 
+// Runs of equal keys in consecutive lanes of a wave: *first = the run's
+// first lane, returns the run's length.  Lane 0 always starts a run.  The
+// sort's counters take one atomic per run instead of one per particle --
+// a state that is already in order has one or two runs per wave.
+__device__ __forceinline__ int key_run(int key, int* first) {
+    const int lane = __lane_id();
+    const int left = __shfl_up(key, 1);
+    const unsigned long long heads = __ballot(lane == 0 || left != key);
+    const unsigned long long upto = (2ull << lane) - 1ull;   // lanes 0 .. lane
+    *first = 63 - __clzll(heads & upto);
+    const unsigned long long above = heads & ~upto;
+    const int next = above ? __ffsll((long long)above) - 1 : 64;
+    return next - *first;
+}
+
 __global__ void __launch_bounds__(kB)
     k_order_hist(const float* __restrict__ planes, int64_t S, int n, float inv_dx, int ng, int nb,
                  int* __restrict__ hist) {
     const int p = blockIdx.x * kB + threadIdx.x;
-    if (p >= n) return;
-    int b[3];
-    stencil_base(planes + p, S, inv_dx, b);
-    atomicAdd(&hist[order_key(b, ng, nb)], 1);
+    int key = -1;
+    if (p < n) {
+        int b[3];
+        stencil_base(planes + p, S, inv_dx, b);
+        key = order_key(b, ng, nb);
+    }
+    int first;
+    const int len = key_run(key, &first);
+    if (key >= 0 && __lane_id() == first) atomicAdd(&hist[key], len);
 }
 
 // One workgroup: exclusive offsets of the key counts (in place, and copied
@@ -107,10 +127,19 @@ __global__ void __launch_bounds__(kB)
     k_order_scatter(const float* __restrict__ src, int64_t SS, float* __restrict__ dst, int64_t DS, int n,
                     float inv_dx, int ng, int nb, int* __restrict__ cursor, int* __restrict__ perm) {
     const int p = blockIdx.x * kB + threadIdx.x;
-    if (p >= n) return;
-    int b[3];
-    stencil_base(src + p, SS, inv_dx, b);
-    const int pos = atomicAdd(&cursor[order_key(b, ng, nb)], 1);
+    int key = -1;
+    if (p < n) {
+        int b[3];
+        stencil_base(src + p, SS, inv_dx, b);
+        key = order_key(b, ng, nb);
+    }
+    int first;
+    const int len = key_run(key, &first);
+    int base = 0;
+    if (key >= 0 && __lane_id() == first) base = atomicAdd(&cursor[key], len);
+    base = __shfl(base, first);
+    if (key < 0) return;
+    const int pos = base + (__lane_id() - first);   // a run keeps its order
     if (pos < 0 || pos >= n) return;   // cannot happen: same keys as the histogram
 #pragma unroll 13
     for (int k = 0; k < MPM_NPLANES; ++k) dst[(int64_t)k * DS + pos] = src[(int64_t)k * SS + p];
@@ -292,6 +321,18 @@ int mpm_step_ordered(const MpmParams* prm, const MpmState* st, const void* order
     if ((rc = p2g_ordered(prm, st, order, acc, dt, stream))) return rc;
     if ((rc = mpm_grid_update(prm, acc, gv, colliders, n_colliders, time, dt, stream))) return rc;
     return mpm_g2p(prm, st, gv, dt, stream);
+}
+
+int mpm_run_ordered(const MpmParams* prm, const MpmState* st, const void* order, const MpmCollider* colliders,
+                    int n_colliders, void* workspace, int64_t workspace_bytes, int k0, double dt, int n_sub,
+                    void* stream) {
+    if (n_sub < 0) return set_err(-31, "n_sub < 0");
+    for (int j = 0; j < n_sub; ++j) {
+        const int rc = mpm_step_ordered(prm, st, order, colliders, n_colliders, workspace, workspace_bytes,
+                                        (float)((double)(k0 + j) * dt), (float)dt, stream);
+        if (rc) return rc;
+    }
+    return 0;
 }
 
 int mpm_permute(const float* src, int64_t src_stride, float* dst, int64_t dst_stride, int n_planes, const int* idx,
