@@ -61,6 +61,11 @@ def parse():
                     help="column: the 1M-particle headline; batch: 64 uneven scenes advanced together; "
                          "adjoint: forward + adjoint of a differentiable rollout")
     ap.add_argument("--scenes", type=int, default=64)
+    ap.add_argument("--wg-threads", type=int, default=0, choices=[0, 256, 512],
+                    help="threads per bin workgroup of the fused kernel (0 = library default)")
+    ap.add_argument("--sort-period", type=int, default=0, help="substeps between bank-order sorts (0 = default)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="batch: use --wg-threads/--sort-period instead of timing the candidates first")
     args = ap.parse_args()
     adj = args.workload == "adjoint"
     if args.particles is None:
@@ -87,7 +92,14 @@ def bench_batch(args):
     total = sum(sc.n_particles for sc in scenes)
     print(f"[bench] batch: {len(scenes)} scenes, {total} particles, grids "
           f"{min(sc.n_grid for sc in scenes)}..{max(sc.n_grid for sc in scenes)}", file=sys.stderr, flush=True)
+    tuned = None
+    if not args.no_autotune:
+        from reduced_mpm_stress_training_amd.tune import autotune
+        tuned = autotune(lambda: B.SceneBatch(scenes), dt)
+        print(f"[bench] autotune: {tuned}", file=sys.stderr, flush=True)
     bt = B.SceneBatch(scenes)
+    launch = tuned["launch"] if tuned else {"wg_threads": args.wg_threads, "sort_period": args.sort_period}
+    bt.set_launch(**launch)
     bt.run(args.warmup, dt)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -118,7 +130,7 @@ def bench_batch(args):
                    "grids": "32^3-96^3", "dt": dt, "parallelism": "single GPU"},
         "sequential": {"value": total * steps / t_seq, "ms_per_step": t_seq * 1e3 / steps,
                        "speedup_of_batch": t_seq / t_batch},
-        "no_particles_dropped": ok,
+        "no_particles_dropped": ok, "launch": launch, "autotune_ms_per_substep": tuned["trials"] if tuned else None,
     }
     print(json.dumps(out), flush=True)
 
@@ -345,6 +357,7 @@ def main():
 
     scene = S.column_uniform(n_particles=args.particles, n_grid=args.n_grid)
     sim = S.make_simulator(scene)
+    sim.set_launch(args.wg_threads, args.sort_period)
     sim.use_bins = not args.unbinned
     n = scene.n_particles
     dt = scene.dt
@@ -407,6 +420,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "sand column collapse, forward substep", "n_particles": n,
                    "path": "unbinned stage kernels" if args.unbinned else "binned G2P2G",
+                   "launch": {"wg_threads": args.wg_threads or 512, "sort_period": args.sort_period or 32},
                    "n_grid": scene.n_grid, "box": "[0.3,0.7]^2x[0.05,0.6]", "dt": dt, "E": scene.E,
                    "nu": scene.nu, "friction_angle": scene.friction_angle, "density": scene.density,
                    "parallelism": "single GPU"},

@@ -40,7 +40,7 @@
 extern "C" {
 #endif
 
-#define MPM_ABI_VERSION 6
+#define MPM_ABI_VERSION 7
 
 /* plane indices */
 #define MPM_P_X0 0
@@ -142,9 +142,12 @@ int mpm_step(const MpmParams* prm, const MpmState* st, const MpmCollider* collid
  * of the workspace stamped with `epoch`, so between calls the workspace
  * carries the pending P2G; mpm_bins_p2g computes it for a freshly inserted
  * state.  Entry points that advance the state update `cur`, `epoch` and
- * `since_sort` in place.  Every 32 substeps mpm_bins_substep re-sorts the
- * bins (mpm_bins_sort): the order of particles inside a bin decides how
- * many LDS bank conflicts the fused kernel's gathers and scatters see. */
+ * `since_sort` in place.  Every sort_period substeps mpm_bins_substep
+ * re-sorts the bins (mpm_bins_sort): the order of particles inside a bin
+ * decides how many LDS bank conflicts the fused kernel's gathers and
+ * scatters see.  wg_threads and sort_period change speed, not results
+ * beyond the order of FP64 tile sums (reduced_mpm_stress_training_amd/tune.py
+ * picks them by timing). */
 typedef struct MpmBins {
     int32_t nb;           /* bins per axis = ceil(n_grid / 8) */
     int32_t bin_cap;      /* slots per bin */
@@ -156,6 +159,8 @@ typedef struct MpmBins {
     float* planes[2];     /* two sets of MPM_NPLANES planes */
     int32_t* count;       /* [2][2*nb^3 + 1]: per plane set, slots used per bin (nb^3),
                              slots used in the overflow run (1), live particles per bin (nb^3) */
+    int32_t wg_threads;   /* threads per bin workgroup of the fused kernel: 256 or 512 (0 = 512) */
+    int32_t sort_period;  /* substeps between bank-order sorts (0 = 32, < 0 = never) */
 } MpmBins;
 
 /* Workspace of the binned path (zeroed once before first use, 256-byte

@@ -10,7 +10,7 @@ import os
 
 from ._build import LIB_PATH
 
-MPM_ABI_VERSION = 6
+MPM_ABI_VERSION = 7
 MAX_COLLIDERS = 16
 
 # plane indices of the particle state buffer
@@ -42,7 +42,8 @@ class MpmState(C.Structure):
 class MpmBins(C.Structure):
     _fields_ = [("nb", C.c_int32), ("bin_cap", C.c_int32), ("overflow_cap", C.c_int32), ("cur", C.c_int32),
                 ("epoch", C.c_int32), ("since_sort", C.c_int32),
-                ("stride", C.c_int64), ("planes", C.c_void_p * 2), ("count", C.c_void_p)]
+                ("stride", C.c_int64), ("planes", C.c_void_p * 2), ("count", C.c_void_p),
+                ("wg_threads", C.c_int32), ("sort_period", C.c_int32)]
 
 
 class MpmScene(C.Structure):

@@ -102,6 +102,7 @@ class SceneBatch:
         self._count = torch.zeros((2, 2 * int(self.batch.n_bins) + 1), dtype=torch.int32, device=self.device)
         b = L.MpmBins()
         b.nb, b.bin_cap, b.overflow_cap, b.cur, b.epoch, b.since_sort, b.stride = 0, OVF_BLOCK, ovf, 0, 0, 0, stride
+        b.wg_threads, b.sort_period = 0, 0
         b.planes[0] = self._planes2[0].data_ptr()
         b.planes[1] = self._planes2[1].data_ptr()
         b.count = self._count.data_ptr()
@@ -114,6 +115,13 @@ class SceneBatch:
 
     def _stream(self):
         return torch.cuda.current_stream(self.device).cuda_stream
+
+    def set_launch(self, wg_threads: int = 0, sort_period: int = 0):
+        """Threads per bin workgroup (256 or 512; 0 = 512) and substeps
+        between bank-order sorts (0 = 32, < 0 = never); speed only."""
+        if wg_threads not in (0, 256, 512):
+            raise ValueError("wg_threads must be 0, 256 or 512")
+        self.bins.wg_threads, self.bins.sort_period = int(wg_threads), int(sort_period)
 
     def _flat(self) -> L.MpmState:
         st = L.MpmState()

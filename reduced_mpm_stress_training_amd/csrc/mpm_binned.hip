@@ -222,7 +222,7 @@ __device__ __forceinline__ void scatter_tile(double* tile, int i0, const AxisWei
 __device__ void write_tile(const double* s_acc, float4* __restrict__ tiles, int* __restrict__ stamps, int bin,
                            int epoch) {
     float4* out = tiles + (int64_t)bin * kTS3;
-    for (int t = threadIdx.x; t < kTS3; t += kWG)
+    for (int t = threadIdx.x; t < kTS3; t += blockDim.x)
         out[t] = make_float4((float)s_acc[t], (float)s_acc[kTS3 + t], (float)s_acc[2 * kTS3 + t],
                              (float)s_acc[3 * kTS3 + t]);
     if (threadIdx.x == 0) stamps[bin] = epoch;
@@ -560,7 +560,11 @@ __device__ __forceinline__ int bank_class(const int base[3], const int b0[3]) {
     return ((((base[0] - b0[0] + 1) * kTS + (base[1] - b0[1] + 1)) * kTS + (base[2] - b0[2] + 1)) & 31);
 }
 
-__global__ void __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(2, 2))) k_g2p2g(SubstepArgs a) {
+// WG: threads per bin workgroup (256 or 512).  At the kernel's ~226 VGPRs a
+// SIMD holds two waves either way; 256-thread workgroups put two bins on a
+// CU at once, so one bin's tile load and write-out overlap the other's loop.
+template <int WG>
+__global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2, 2))) k_g2p2g(SubstepArgs a) {
     __shared__ float s_vel[3 * kTS3];
     __shared__ double s_acc[4 * kTS3];
     __shared__ int s_stay;
@@ -571,9 +575,9 @@ __global__ void __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(2, 2))
     if (wg >= g.n_bins) {
         // overflow run of the current plane set: global gather and spill scatter
         const int n_ovf = min(a.cnt_cur[g.n_bins], g.ovf_cap);
-        const int i = (wg - g.n_bins) * kWG + tid;
+        const int i = (wg - g.n_bins) * WG + tid;
         const int b0[3] = {0, 0, 0};
-        if ((wg - g.n_bins) * kWG >= n_ovf) return;
+        if ((wg - g.n_bins) * WG >= n_ovf) return;
         const int64_t off = ovf_off(g, i);
         ParticleIn q;
         load_particle(a.src, off, g.ovf_blk, i < n_ovf, q);
@@ -598,7 +602,7 @@ __global__ void __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(2, 2))
         s_base = atomicAdd(&a.cnt_nxt[bin], n_live);
         s_stay = 0;
     }
-    for (int t = tid; t < kTS3; t += kWG) {
+    for (int t = tid; t < kTS3; t += WG) {
         const int x = b0[0] - 1 + t / (kTS * kTS), y = b0[1] - 1 + (t / kTS) % kTS, z = b0[2] - 1 + t % kTS;
         float4 val = make_float4(0.f, 0.f, 0.f, 0.f);
         if (x >= 0 && y >= 0 && z >= 0 && x < ng && y < ng && z < ng) val = svel[((int64_t)x * ng + y) * ng + z];
@@ -606,7 +610,7 @@ __global__ void __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(2, 2))
         s_vel[kTS3 + t] = val.y;
         s_vel[2 * kTS3 + t] = val.z;
     }
-    for (int t = tid; t < 4 * kTS3; t += kWG) s_acc[t] = 0.0;
+    for (int t = tid; t < 4 * kTS3; t += WG) s_acc[t] = 0.0;
     __syncthreads();
     // Particles are processed in storage order.  Storage is kept in bank order
     // (k_bins_interleave after insertion and every kSortPeriod substeps;
@@ -615,16 +619,16 @@ __global__ void __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(2, 2))
     const int stay_base = s_base;
     ParticleIn q;
     load_particle(a.src, off0 + tid, cap, tid < n_in, q);
-    for (int it = 0; it < n_in; it += kWG) {
+    for (int it = 0; it < n_in; it += WG) {
         const int i = it + tid;
         const ParticleIn cur = q;
-        if (it + kWG < n_in) load_particle(a.src, off0 + i + kWG, cap, i + kWG < n_in, q);  // prefetch
+        if (it + WG < n_in) load_particle(a.src, off0 + i + WG, cap, i + WG < n_in, q);  // prefetch
         g2p2g_particle(a, sc, i < n_in, off0 + i, cap, bin, stay_base, &s_stay, b0, s_vel, s_acc, cur);
     }
     __syncthreads();
     // reserved slots nobody stayed in become holes (pid -1)
     const int hole_lo = stay_base + s_stay, hole_hi = min(stay_base + n_live, cap);
-    for (int h = hole_lo + tid; h < hole_hi; h += kWG)
+    for (int h = hole_lo + tid; h < hole_hi; h += WG)
         a.dst[off0 + (int64_t)MPM_P_PID * cap + h] = __int_as_float(-1);
     if (tid == 0) atomicAdd(&a.cnt_nxt[g.n_bins + 1 + bin], max(0, min(s_stay, cap - stay_base)));
     write_tile(s_acc, a.tiles, a.stamps, bin, a.epoch_out);
@@ -947,6 +951,8 @@ int check_bins(const MpmBins* b, int64_t n_bins, int64_t n_slots, int max_cap, B
     if (b->bin_cap < 1 || b->overflow_cap < 0 || b->overflow_cap % b->bin_cap != 0)
         return set_err(-22, "bins: bin_cap < 1 or overflow_cap not a multiple of bin_cap");
     if (b->cur != 0 && b->cur != 1) return set_err(-23, "bins: cur must be 0 or 1");
+    if (b->wg_threads != 0 && b->wg_threads != 256 && b->wg_threads != 512)
+        return set_err(-29, "bins: wg_threads must be 0, 256 or 512");
     if (n_slots + b->overflow_cap > 0x7fffffffLL) return set_err(-28, "bins: more than 2^31-1 slots");
     const int64_t need = (n_slots + b->overflow_cap) * MPM_NPLANES;
     if (b->stride < need) return set_err(-24, "bins: stride < (bin slots + overflow_cap) * MPM_NPLANES");
@@ -1000,7 +1006,7 @@ int ctx_batch(const MpmBatch* batch, const MpmBins* bins, Ctx* c) {
     return 0;
 }
 
-unsigned ovf_wgs(const BinGeom& g) { return (unsigned)((g.ovf_cap + kWG - 1) / kWG); }
+unsigned ovf_wgs(const BinGeom& g, int wg = kWG) { return (unsigned)((g.ovf_cap + wg - 1) / wg); }
 
 int* counts(const MpmBins* b, const BinGeom& g, int set) { return b->count + (int64_t)set * (2 * g.n_bins + 1); }
 
@@ -1093,7 +1099,10 @@ int op_g2p2g(const Ctx& c, MpmBins* bins, float dt, hipStream_t s) {
     a.stamps = c.w.stamps;
     a.order = c.w.order;
     a.sched_epoch = bins->epoch;
-    hipLaunchKernelGGL(k_g2p2g, dim3(c.g.n_bins + ovf_wgs(c.g)), dim3(kWG), 0, s, a);
+    if (bins->wg_threads == 256)
+        hipLaunchKernelGGL(k_g2p2g<256>, dim3(c.g.n_bins + ovf_wgs(c.g, 256)), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_g2p2g<512>, dim3(c.g.n_bins + ovf_wgs(c.g, 512)), dim3(512), 0, s, a);
     int rc = check_launch("k_g2p2g");
     if (rc) return rc;
     bins->epoch = a.epoch_out;
@@ -1125,7 +1134,8 @@ int op_substep(const Ctx& c, MpmBins* bins, float time, float dt, hipStream_t s)
     // particles drift across cells and out of bank order; restore it now and
     // then, between the grid update (which schedules the bins) and the fused
     // kernel, which reads the sorted set
-    if (++bins->since_sort >= kSortPeriod && (rc = op_sort(c, bins, s))) return rc;
+    const int period = bins->sort_period == 0 ? kSortPeriod : bins->sort_period;
+    if (++bins->since_sort >= period && period > 0 && (rc = op_sort(c, bins, s))) return rc;
     return op_g2p2g(c, bins, dt, s);
 }
 

@@ -101,6 +101,7 @@ class MPM_Simulator_HIP:
         # substep and back to the flat caller-order planes when a loader runs
         self.use_bins = True
         self._bins = None            # dict of tensors + ctypes struct
+        self._launch = (0, 0)        # (wg_threads, sort_period) of the binned kernels; 0 = default
         self._loc = "flat"           # where the authoritative state lives
         self._flat_valid = True      # flat planes mirror the binned state
         self._pending_dt = None      # dt of the P2G held in the accumulator
@@ -315,6 +316,7 @@ class MPM_Simulator_HIP:
         count = torch.zeros((2, 2 * nb ** 3 + 1), dtype=torch.int32, device=self.device)
         st = L.MpmBins()
         st.nb, st.bin_cap, st.overflow_cap, st.cur, st.epoch, st.stride = nb, cap, ovf, 0, 0, stride
+        st.wg_threads, st.sort_period = self._launch
         st.planes[0] = planes[0].data_ptr()
         st.planes[1] = planes[1].data_ptr()
         st.count = count.data_ptr()
@@ -324,6 +326,16 @@ class MPM_Simulator_HIP:
         self._bins = dict(nb=nb, cap=cap, ovf=ovf, stride=stride, planes=planes, count=count, struct=st, ws=ws,
                           ng=ng)
         return self._bins
+
+    def set_launch(self, wg_threads: int = 0, sort_period: int = 0):
+        """Launch parameters of the binned kernels: threads per bin workgroup
+        (256 or 512; 0 = 512) and substeps between bank-order sorts (0 = 32,
+        < 0 = never).  They change speed only (see tune.py)."""
+        if wg_threads not in (0, 256, 512):
+            raise ValueError("wg_threads must be 0, 256 or 512")
+        self._launch = (int(wg_threads), int(sort_period))
+        if self._bins is not None:
+            self._bins["struct"].wg_threads, self._bins["struct"].sort_period = self._launch
 
     def _bins_ws_ptr(self):
         p = self._bins["ws"].data_ptr()

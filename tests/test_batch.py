@@ -26,9 +26,11 @@ def _scenes():
     return [a, b, c]
 
 
-def test_batch_matches_single_scenes():
+@pytest.mark.parametrize("launch", [(0, 0), (256, 16)])
+def test_batch_matches_single_scenes(launch):
     scenes = _scenes()
     batch = B.SceneBatch(scenes)
+    batch.set_launch(*launch)
     dt, n_steps = 1e-4, 70          # crosses two bank-order sorts
     batch.run(n_steps, dt)
     torch.cuda.synchronize()
@@ -64,3 +66,13 @@ def test_uneven_batch_runs_and_conserves_particles():
     for i in range(len(scenes)):
         u, v, Fd, tau, C = batch.export(i)
         assert bool(torch.isfinite(u).all() and torch.isfinite(tau).all()), i
+
+
+def test_autotune_returns_a_timed_candidate():
+    from reduced_mpm_stress_training_amd.tune import autotune
+    scenes = B.uneven_scenes(n_scenes=6, seed=3, n_min=2_000, n_max=20_000, grid_min=32, grid_max=48)
+    cands = ((512, 32), (256, 16))
+    best = autotune(lambda: B.SceneBatch(scenes), 1e-4, trial_substeps=4, candidates=cands)
+    assert (best["launch"]["wg_threads"], best["launch"]["sort_period"]) in cands
+    assert set(best["trials"]) == {"512x32", "256x16"}
+    assert best["ms_per_substep"] == min(best["trials"].values()) > 0.0

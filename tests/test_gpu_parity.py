@@ -113,14 +113,17 @@ def _run_oracle(oracle, sc, n_steps, dtype, order=None, nthreads=1):
     return out
 
 
-@pytest.mark.parametrize("n_steps,binned", [(10, True), (100, True), (10, False), (100, False)])
-def test_column_32_parity(oracle, n_steps, binned):
+@pytest.mark.parametrize("n_steps,binned,launch", [(10, True, (0, 0)), (100, True, (0, 0)), (100, True, (256, 16)),
+                                                   (10, False, (0, 0)), (100, False, (0, 0))])
+def test_column_32_parity(oracle, n_steps, binned, launch):
     """The 32^3 sand column (8 particles/cell, E=2e5, nu=0.3, 30 deg,
-    g=-9.8 z, dt=1e-4, sticky floor + box), on the binned G2P2G path and on
+    g=-9.8 z, dt=1e-4, sticky floor + box), on the binned G2P2G path (default
+    launch, and 256-thread bin workgroups sorting every 16 substeps) and on
     the unbinned stage kernels."""
     sc = S.column_lattice()
     sim = S.make_simulator(sc)
     sim.use_bins = binned
+    sim.set_launch(*launch)
     sim.run(n_steps, sc.dt)
     g = {k: v.reshape(sc.n_particles, -1) for k, v in _gpu_state(sim).items()}
     o32 = _run_oracle(oracle, sc, n_steps, np.float32)
