@@ -109,6 +109,22 @@ def bench_batch(args):
     ok = int(bt.status()[0]) == 0
     del bt
     torch.cuda.empty_cache()
+    # the same run replayed as a HIP graph (mpm_batch_graph_*), runs of 2 x sort period substeps
+    bg = B.SceneBatch(scenes)
+    bg.set_launch(**launch)
+    bg.run(args.warmup, dt)
+    k = bg.capture(dt)
+    n_rep = max(1, steps // k)
+    bg.run_captured(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    bg.run_captured(n_rep)
+    torch.cuda.synchronize()
+    ms_graph = (time.perf_counter() - t0) * 1e3 / (n_rep * k)
+    ok = ok and int(bg.status()[0]) == 0
+    bg.release_graph()
+    del bg
+    torch.cuda.empty_cache()
     print(f"[bench] batch {t_batch * 1e3 / steps:.3f} ms/substep; sequential ...", file=sys.stderr, flush=True)
     t_seq = 0.0
     for sc in scenes:
@@ -130,7 +146,7 @@ def bench_batch(args):
                    "grids": "32^3-96^3", "dt": dt, "parallelism": "single GPU"},
         "sequential": {"value": total * steps / t_seq, "ms_per_step": t_seq * 1e3 / steps,
                        "speedup_of_batch": t_seq / t_batch},
-        "no_particles_dropped": ok, "launch": launch, "autotune_ms_per_substep": tuned["trials"] if tuned else None,
+        "captured_graph_ms_per_step": ms_graph, "no_particles_dropped": ok, "launch": launch, "autotune_ms_per_substep": tuned["trials"] if tuned else None,
     }
     print(json.dumps(out), flush=True)
 

@@ -167,7 +167,8 @@ typedef struct MpmBins {
  * aligned): spill and velocity grids (n_grid^3 float4 each), spill flags,
  * 64 int32 status counters (0: particles dropped because bins and overflow
  * were full, 1: particles scattered outside their bin's tile, 2: particles
- * that left the grid), tile stamps and nb^3 P2G tiles of 12^3 float4. */
+ * that left the grid; 48-49: the start time, as a double, of a captured run,
+ * mpm_batch_graph_launch), tile stamps and nb^3 P2G tiles of 12^3 float4. */
 int64_t mpm_bins_workspace_bytes(const MpmParams* prm);
 
 /* Byte offset of the status counters in the workspace. */
@@ -354,6 +355,24 @@ int mpm_batch_extract(const MpmBatch* batch, const MpmBins* bins, const MpmState
 int mpm_batch_clear_grid(const MpmBatch* batch, MpmBins* bins, void* stream);
 int mpm_batch_p2g(const MpmBatch* batch, MpmBins* bins, float dt, void* stream);
 int mpm_batch_run(const MpmBatch* batch, MpmBins* bins, double time, float dt, int n_substeps, void* stream);
+
+/* A batched run of n_substeps captured once as a HIP graph and replayed:
+ * one graph launch per n_substeps instead of two kernel launches per substep
+ * (plus the sorts).  Capture does not advance the state.  A replay starts on
+ * the plane set current at capture (n_substeps plus the sorts it contains
+ * must be even -- twice the sort period is), voids every tile stamp except
+ * the pending tiles', and reads its start time from the device, so replays
+ * continue one another and the colliders' time windows see the true time.
+ * Launch: n_replays replays from time `time` (the caller advances its clock
+ * by n_replays * n_substeps * dt); the pending P2G must have been computed
+ * with the graph's dt.  Eager substeps may run between launches as long as the
+ * state is back on the capture's plane set.  The graph holds the batch's
+ * pointers: destroy it before the batch's buffers. */
+typedef struct MpmGraph MpmGraph;
+int mpm_batch_graph_create(const MpmBatch* batch, const MpmBins* bins, float dt, int n_substeps, MpmGraph** graph);
+int mpm_batch_graph_launch(MpmGraph* graph, const MpmBatch* batch, MpmBins* bins, double time, int n_replays,
+                           void* stream);
+int mpm_graph_destroy(MpmGraph* graph);
 
 /* ---- state import / export (torch loaders and export_uvFstress_from_warp,
  *      mpm_solver_warp.py:229-266 and :305-310) --------------------------- */

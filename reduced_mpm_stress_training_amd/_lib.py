@@ -103,6 +103,11 @@ _SIGNATURES = {
     "mpm_batch_p2g": (C.c_int, [C.POINTER(MpmBatch), C.POINTER(MpmBins), C.c_float, C.c_void_p]),
     "mpm_batch_run": (C.c_int, [C.POINTER(MpmBatch), C.POINTER(MpmBins), C.c_double, C.c_float, C.c_int,
                                 C.c_void_p]),
+    "mpm_batch_graph_create": (C.c_int, [C.POINTER(MpmBatch), C.POINTER(MpmBins), C.c_float, C.c_int,
+                                         C.POINTER(C.c_void_p)]),
+    "mpm_batch_graph_launch": (C.c_int, [C.c_void_p, C.POINTER(MpmBatch), C.POINTER(MpmBins), C.c_double, C.c_int,
+                                         C.c_void_p]),
+    "mpm_graph_destroy": (C.c_int, [C.c_void_p]),
     "mpm_adjoint_workspace_bytes": (C.c_int64, [C.POINTER(MpmParams)]),
     "mpm_adjoint_substep": (C.c_int, [C.POINTER(MpmParams), C.POINTER(MpmState), C.c_void_p, C.c_void_p,
                                       C.POINTER(MpmCollider), C.c_int, C.c_void_p, C.c_float, C.c_float, C.c_void_p,

@@ -13,6 +13,8 @@ advance with the same dt.
 """
 from __future__ import annotations
 
+import ctypes
+
 import numpy as np
 import torch
 
@@ -109,6 +111,7 @@ class SceneBatch:
         self.bins = b
         self.time = 0.0
         self._pending_dt = None
+        self._graph = None
         stream = self._stream()
         L.check(lib.mpm_batch_insert(self.batch, self._flat(), self.bins, stream), "mpm_batch_insert")
         L.check(lib.mpm_batch_clear_grid(self.batch, self.bins, stream), "mpm_batch_clear_grid")
@@ -145,6 +148,48 @@ class SceneBatch:
                 "mpm_batch_run")
         for _ in range(int(n_substeps)):
             self.time = self.time + dt
+
+    def capture(self, dt: float, n_substeps: int | None = None):
+        """Captures a run of n_substeps (default: twice the sort period) as
+        a HIP graph (mpm_batch_graph_create); run_captured replays it."""
+        lib = L.load()
+        if n_substeps is None:
+            period = self.bins.sort_period if self.bins.sort_period != 0 else 32
+            n_substeps = 2 * period if period > 0 else 64
+        self.release_graph()
+        stream = self._stream()
+        if self._pending_dt != dt:
+            if self._pending_dt is not None:
+                L.check(lib.mpm_batch_clear_grid(self.batch, self.bins, stream), "mpm_batch_clear_grid")
+            L.check(lib.mpm_batch_p2g(self.batch, self.bins, float(dt), stream), "mpm_batch_p2g")
+            self._pending_dt = dt
+        g = ctypes.c_void_p()
+        L.check(lib.mpm_batch_graph_create(self.batch, self.bins, float(dt), int(n_substeps), ctypes.byref(g)),
+                "mpm_batch_graph_create")
+        self._graph, self._graph_len, self._graph_dt = g, int(n_substeps), dt
+        return self._graph_len
+
+    def run_captured(self, n_replays: int):
+        """n_replays replays of the captured run (n_replays * its length
+        substeps); eager run() calls may come between."""
+        if self._graph is None:
+            raise RuntimeError("capture() first")
+        lib = L.load()
+        L.check(lib.mpm_batch_graph_launch(self._graph, self.batch, self.bins, float(self.time), int(n_replays),
+                                           self._stream()), "mpm_batch_graph_launch")
+        for _ in range(int(n_replays) * self._graph_len):
+            self.time = self.time + self._graph_dt
+
+    def release_graph(self):
+        if getattr(self, "_graph", None) is not None:
+            L.load().mpm_graph_destroy(self._graph)
+        self._graph = None
+
+    def __del__(self):
+        try:
+            self.release_graph()
+        except Exception:
+            pass
 
     def export(self, i: int):
         """(u, v, F_disp, stress, C) of scene i, in its particle order."""

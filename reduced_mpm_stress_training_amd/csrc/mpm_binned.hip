@@ -473,7 +473,7 @@ __global__ void __launch_bounds__(kWG)
     k_grid_bins(float4* __restrict__ acc, float4* __restrict__ vel, int* __restrict__ gflag,
                 const int* __restrict__ stamps, const float4* __restrict__ tiles, int* __restrict__ cnt_nxt,
                 const int* __restrict__ cnt_cur, int* __restrict__ order, BinGeom g, SceneTab tab,
-                ColliderSet cs, float time, float dt, int epoch, int clear_only) {
+                ColliderSet cs, float time, float dt, int epoch, int clear_only, const double* __restrict__ t_base) {
     __shared__ int s_live[27];
     __shared__ int s_any;
     if (blockIdx.x == 0) {
@@ -482,6 +482,8 @@ __global__ void __launch_bounds__(kWG)
     }
     const int blk = blockIdx.x - 1;   // global block (= global bin index)
     const int tid = threadIdx.x;
+    // a captured run reads its start time from the device: `time` is then the offset
+    const float tnow = t_base ? (float)(t_base[0] + (double)time) : time;
     const SceneDesc sc = scene_of_bin(tab, blk);
     const int nb = sc.nb, ng = sc.prm.n_grid;
     const int lblk = blk - sc.bin_base;
@@ -542,7 +544,7 @@ __global__ void __launch_bounds__(kWG)
                     sx += tv.x; sy += tv.y; sz += tv.z; sm += tv.w;
                 }
         const float4 a4 = make_float4((float)sx, (float)sy, (float)sz, (float)sm);
-        vel[node] = grid_node_update(a4, x, y, z, ng, sc.prm.dx, dt, sc.prm.gravity, cols, n_cols, time);
+        vel[node] = grid_node_update(a4, x, y, z, ng, sc.prm.dx, dt, sc.prm.gravity, cols, n_cols, tnow);
     }
     __syncthreads();
     if (tid == 0 && spill) gflag[blk] = 0;
@@ -823,6 +825,18 @@ __global__ void __launch_bounds__(kWG)
 
 // fill n ints with v (bin counters; a kernel rather than a memset so that the
 // sequence can be captured in a graph)
+// Tile stamps at the start of a captured run: the pending tiles (stamp
+// `from`) become the run's first epoch `to`, every other stamp is voided, so
+// that no tile of an earlier replay matches an epoch the run reuses.
+__global__ void k_stamp_rebase(int* __restrict__ stamps, int n, int from, int to) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) stamps[i] = stamps[i] == from ? to : (int)0x80000000;
+}
+
+__global__ void k_time_set(double* __restrict__ t, double v, int add) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) t[0] = add ? t[0] + v : v;
+}
+
 __global__ void k_fill_int(int* __restrict__ p, int n, int v) {
     const int i = blockIdx.x * kWG + threadIdx.x;
     if (i < n) p[i] = v;
@@ -1054,7 +1068,7 @@ int op_extract(const Ctx& c, const MpmBins* bins, const MpmState* flat, hipStrea
 int op_clear(const Ctx& c, MpmBins* bins, hipStream_t s) {
     hipLaunchKernelGGL(k_grid_bins, dim3(c.g.n_bins + 1), dim3(kWG), 0, s, c.w.acc, c.w.vel, c.w.gflag,
                        (const int*)c.w.stamps, (const float4*)c.w.tiles, (int*)nullptr, (const int*)nullptr,
-                       (int*)nullptr, c.g, c.tab, c.cs, 0.f, 0.f, bins->epoch, 1);
+                       (int*)nullptr, c.g, c.tab, c.cs, 0.f, 0.f, bins->epoch, 1, (const double*)nullptr);
     int rc = check_launch("k_grid_bins(clear)");
     if (rc) return rc;
     bins->epoch += 1;  // the pending tiles are no longer current
@@ -1072,11 +1086,12 @@ int op_p2g(const Ctx& c, MpmBins* bins, float dt, hipStream_t s) {
     return 0;
 }
 
-int op_grid(const Ctx& c, const MpmBins* bins, float time, float dt, hipStream_t s) {
+int op_grid(const Ctx& c, const MpmBins* bins, float time, float dt, hipStream_t s,
+            const double* t_base = nullptr) {
     hipLaunchKernelGGL(k_grid_bins, dim3(c.g.n_bins + 1), dim3(kWG), 0, s, c.w.acc, c.w.vel, c.w.gflag,
                        (const int*)c.w.stamps, (const float4*)c.w.tiles, counts(bins, c.g, 1 - bins->cur),
                        (const int*)counts(bins, c.g, bins->cur), c.w.order, c.g, c.tab, c.cs, time, dt, bins->epoch,
-                       0);
+                       0, t_base);
     return check_launch("k_grid_bins");
 }
 
@@ -1128,8 +1143,8 @@ int op_sort(const Ctx& c, MpmBins* bins, hipStream_t s) {
     return 0;
 }
 
-int op_substep(const Ctx& c, MpmBins* bins, float time, float dt, hipStream_t s) {
-    int rc = op_grid(c, bins, time, dt, s);
+int op_substep(const Ctx& c, MpmBins* bins, float time, float dt, hipStream_t s, const double* t_base = nullptr) {
+    int rc = op_grid(c, bins, time, dt, s, t_base);
     if (rc) return rc;
     // particles drift across cells and out of bank order; restore it now and
     // then, between the grid update (which schedules the bins) and the fused
@@ -1147,6 +1162,105 @@ int op_run(const Ctx& c, MpmBins* bins, double time, float dt, int n_substeps, h
         if (rc) return rc;
         t = t + (double)dt;
     }
+    return 0;
+}
+
+// the device time base of captured runs: two int32 status slots (48, 49)
+double* time_base(const Ctx& c) { return (double*)(c.w.status + 48); }
+
+}  // namespace
+
+// A captured run of n_substeps (opaque to callers, include/mpm_hip.h).
+struct MpmGraph {
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+    const void* workspace;   // the batch it was captured for
+    int n_bins, n_substeps;
+    int cur0;                // plane set a replay starts (and ends) on
+    int epoch0, epoch_end;   // epochs of its first grid update and of its pending tiles
+    int since_end;           // since_sort after a replay
+    float dt;
+};
+
+namespace {
+
+int op_graph_create(const Ctx& c, const void* ws_id, const MpmBins* bins, float dt, int n_substeps, MpmGraph** out) {
+    if (!out) return set_err(-50, "graph: output pointer is NULL");
+    *out = nullptr;
+    if (n_substeps < 2) return set_err(-51, "graph: n_substeps < 2");
+    MpmBins b = *bins;           // the capture's own bookkeeping; *bins is not advanced
+    const int e0 = b.epoch, e_end = b.epoch + n_substeps;
+    hipStream_t cs;
+    if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) return set_err(-52, "graph: stream");
+    int rc = 0;
+    hipGraph_t graph = nullptr;
+    if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        hipStreamDestroy(cs);
+        return set_err(-53, "graph: begin capture failed");
+    }
+    hipLaunchKernelGGL(k_stamp_rebase, dim3((c.g.n_bins + 255) / 256), dim3(256), 0, cs, c.w.stamps, c.g.n_bins,
+                       e_end, e0);
+    rc = check_launch("k_stamp_rebase");
+    double off = 0.0;
+    for (int k = 0; k < n_substeps && !rc; ++k) {
+        rc = op_substep(c, &b, (float)off, dt, cs, time_base(c));
+        off = off + (double)dt;
+    }
+    if (!rc) {
+        hipLaunchKernelGGL(k_time_set, dim3(1), dim3(64), 0, cs, time_base(c), off, 1);
+        rc = check_launch("k_time_set");
+    }
+    const hipError_t ec = hipStreamEndCapture(cs, &graph);
+    hipStreamDestroy(cs);
+    if (!rc && ec != hipSuccess) rc = set_err(-54, "graph: end capture failed");
+    if (!rc && b.cur != bins->cur)
+        rc = set_err(-55, "graph: a replay would end on the other plane set; n_substeps plus the sorts it "
+                          "contains must be even (e.g. twice the sort period)");
+    hipGraphExec_t exec = nullptr;
+    if (!rc && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess)
+        rc = set_err(-56, "graph: instantiate failed");
+    if (rc) {
+        if (graph) hipGraphDestroy(graph);
+        return rc;
+    }
+    MpmGraph* g = (MpmGraph*)malloc(sizeof(MpmGraph));
+    if (!g) {
+        hipGraphExecDestroy(exec);
+        hipGraphDestroy(graph);
+        return set_err(-57, "graph: out of host memory");
+    }
+    g->graph = graph;
+    g->exec = exec;
+    g->workspace = ws_id;
+    g->n_bins = c.g.n_bins;
+    g->n_substeps = n_substeps;
+    g->cur0 = bins->cur;
+    g->epoch0 = e0;
+    g->epoch_end = e_end;
+    g->since_end = b.since_sort;
+    g->dt = dt;
+    *out = g;
+    return 0;
+}
+
+int op_graph_launch(const Ctx& c, const void* ws_id, MpmGraph* g, MpmBins* bins, double time, int n_replays,
+                    hipStream_t s) {
+    if (!g) return set_err(-50, "graph is NULL");
+    if (g->workspace != ws_id || g->n_bins != c.g.n_bins) return set_err(-58, "graph: captured for another batch");
+    if (bins->cur != g->cur0) return set_err(-59, "graph: the state is on the other plane set than at capture");
+    if (n_replays < 0) return set_err(-26, "n_replays < 0");
+    if (n_replays == 0) return 0;
+    // the pending tiles take the stamp a replay's rebase expects; the time base
+    hipLaunchKernelGGL(k_stamp_rebase, dim3((c.g.n_bins + 255) / 256), dim3(256), 0, s, c.w.stamps, c.g.n_bins,
+                       bins->epoch, g->epoch_end);
+    int rc = check_launch("k_stamp_rebase");
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_time_set, dim3(1), dim3(64), 0, s, time_base(c), time, 0);
+    if ((rc = check_launch("k_time_set"))) return rc;
+    for (int r = 0; r < n_replays; ++r)
+        if (hipGraphLaunch(g->exec, s) != hipSuccess) return set_err(-60, "graph: launch failed");
+    bins->epoch = g->epoch_end;
+    bins->since_sort = g->since_end;
     return 0;
 }
 
@@ -1396,6 +1510,29 @@ int mpm_batch_run(const MpmBatch* batch, MpmBins* bins, double time, float dt, i
     int rc = ctx_batch(batch, bins, &c);
     if (rc) return rc;
     return op_run(c, bins, time, dt, n_substeps, (hipStream_t)stream);
+}
+
+int mpm_batch_graph_create(const MpmBatch* batch, const MpmBins* bins, float dt, int n_substeps, MpmGraph** graph) {
+    Ctx c;
+    int rc = ctx_batch(batch, bins, &c);
+    if (rc) return rc;
+    return op_graph_create(c, batch->workspace, bins, dt, n_substeps, graph);
+}
+
+int mpm_batch_graph_launch(MpmGraph* graph, const MpmBatch* batch, MpmBins* bins, double time, int n_replays,
+                           void* stream) {
+    Ctx c;
+    int rc = ctx_batch(batch, bins, &c);
+    if (rc) return rc;
+    return op_graph_launch(c, batch->workspace, graph, bins, time, n_replays, (hipStream_t)stream);
+}
+
+int mpm_graph_destroy(MpmGraph* graph) {
+    if (!graph) return 0;
+    hipGraphExecDestroy(graph->exec);
+    hipGraphDestroy(graph->graph);
+    free(graph);
+    return 0;
 }
 
 }  // extern "C"

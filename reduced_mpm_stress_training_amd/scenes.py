@@ -48,6 +48,7 @@ class SandScene:
     dt: float = 1.0e-4
     substeps: int = 100
     floor_z: float = 0.05
+    floor_start: float = 0.0      # the sticky floor acts from this time on
     grid_lim: float = 1.0
     extra: dict = field(default_factory=dict)
 
@@ -106,7 +107,7 @@ def scene_colliders(scene: SandScene) -> list:
     """Sticky floor at z = floor_z (normal +z) then the bounding box, the
     order generate_sand.py:24-26 registers them in."""
     return [dict(kind=0, surface_type=0, point=(0.0, 0.0, scene.floor_z), normal=(0.0, 0.0, 1.0),
-                 start_time=0.0, end_time=999.0),
+                 start_time=float(scene.floor_start), end_time=999.0),
             dict(kind=1, start_time=0.0, end_time=999.0)]
 
 

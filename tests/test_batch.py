@@ -76,3 +76,46 @@ def test_autotune_returns_a_timed_candidate():
     assert (best["launch"]["wg_threads"], best["launch"]["sort_period"]) in cands
     assert set(best["trials"]) == {"512x32", "256x16"}
     assert best["ms_per_substep"] == min(best["trials"].values()) > 0.0
+
+
+def _compare_batches(a, b, n_scenes):
+    for i in range(n_scenes):
+        ga, gb = a.export(i), b.export(i)
+        torch.cuda.synchronize()
+        for k, x, y in zip(("u", "v", "F", "tau", "C"), ga, gb):
+            if k not in SPREAD:
+                continue
+            e = rel_l2(y.reshape(len(y), -1).cpu().numpy(), x.reshape(len(x), -1).cpu().numpy())
+            assert np.isfinite(y.cpu().numpy()).all(), (i, k)
+            assert e < SPREAD[k], (i, k, e)
+
+
+def test_captured_run_matches_eager_run():
+    """The batched run captured as a HIP graph (mpm_batch_graph_*) replayed
+    several times, with eager work and synchronizations between replays and
+    eager substeps after them, against the same batch run eagerly; every
+    replay is checked.  The colliders start at t = 0.012 so that the device
+    time base matters."""
+    scenes = _scenes()
+    for sc in scenes:
+        sc.floor_start = 0.012
+    dt = 1e-4
+    eager, graph = B.SceneBatch(scenes), B.SceneBatch(scenes)
+    k = graph.capture(dt)
+    assert k == 64
+    for _ in range(3):
+        eager.run(k, dt)
+        graph.run_captured(1)
+        torch.cuda.synchronize()
+        assert abs(graph.time - eager.time) < 1e-12
+        float(torch.randn(4096, device="cuda").sum())     # unrelated eager work between replays
+        _compare_batches(eager, graph, len(scenes))
+    eager.run(k, dt)              # eager substeps continue from a replay's state
+    graph.run(k, dt)
+    _compare_batches(eager, graph, len(scenes))
+    eager.run(2 * k, dt)
+    graph.run_captured(2)
+    torch.cuda.synchronize()
+    _compare_batches(eager, graph, len(scenes))
+    assert int(graph.status()[0]) == 0
+    graph.release_graph()
