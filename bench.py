@@ -283,7 +283,10 @@ def bench_crom(args):
     each frame, batches of 10 frames; then a held-out run (another angle) is
     encoded frame by frame and decoded at all 1M particles.  Reports the
     decode throughput, its time against the MPM substeps it stands for, and
-    the relative L2 error of the reconstructed displacements."""
+    the relative L2 error of the reconstructed displacements.  Then the
+    CROM-reduced run of the held-out angle: MPM on the N/10 sample only
+    (half-resolution grid) + encoder + decoder per frame, judged frame by
+    frame by the full MPM run (ms per frame of both, error)."""
     import torch
     from reduced_mpm_stress_training_amd import crom as CR
     from reduced_mpm_stress_training_amd import scenes as S
@@ -335,6 +338,12 @@ def bench_crom(args):
     sim.run(100, dt)
     torch.cuda.synchronize()
     ms_frame = (time.perf_counter() - t1) * 1e3 / 10
+    # the CROM-reduced run of the held-out angle: MPM on the encoder's N/10
+    # sample (half-resolution grid), decoder for all N, the full run as judge
+    rr = CR.ReducedRun(net, sc, angles[4], n_sample=P, coarse_grid=max(16, args.n_grid // 2))
+    rr.run(2, spf, dt)                       # warm-up of the sampled run's buffers
+    red = rr.run(frames, spf, dt, judge=True)
+    errs = red["errors"][1:]
     flop = 2 * (19 * 128 + 4 * 128 * 128 + 128 * 3)
     out = {
         "metric": "CROM decoder points/s (inference at 1M particles)", "value": n / (ms_dec / 1e3),
@@ -345,6 +354,11 @@ def bench_crom(args):
         "decoder_tflops": flop * n / (ms_dec / 1e3) / 1e12,
         "mpm_ms_per_frame_10_substeps": ms_frame, "decode_speedup_over_mpm_frame": ms_frame / ms_dec,
         "held_out_relative_l2_error": rel, "final_train_loss": float(loss),
+        "reduced_run": {"sample": P, "coarse_grid": max(16, args.n_grid // 2),
+                        "ms_per_frame": red["reduced_seconds"] * 1e3 / (frames - 1),
+                        "full_mpm_ms_per_frame": red["full_seconds"] * 1e3 / (frames - 1),
+                        "speedup": red["full_seconds"] / red["reduced_seconds"],
+                        "mean_relative_l2_error": float(np.mean(errs)), "max_relative_l2_error": float(np.max(errs))},
         "train_seconds": t_train, "snapshot_seconds": t_data,
     }
     print(json.dumps(out), flush=True)

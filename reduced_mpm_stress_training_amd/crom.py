@@ -15,6 +15,7 @@ library's BF16-MFMA kernels with FP32 accumulation.  There is no CPU path.
 from __future__ import annotations
 
 import math
+import time
 
 import numpy as np
 
@@ -403,3 +404,105 @@ def solver_snapshots(scene, friction_angle: float, n_frames: int, substeps_per_f
         if f + 1 < n_frames:
             sim.run(substeps_per_frame, dt)
     return out
+
+
+# ---------------------------------------------------------------------------
+# CROM-reduced runs: a sampled MPM run + the decoder, judged by the full run
+# ---------------------------------------------------------------------------
+def sampled_scene(scene, n_sample: int, n_grid: int):
+    """The first n_sample particles of a (pre-shuffled) scene, each standing
+    for N / n_sample of them (volume scaled, so mass and volume are
+    conserved), on an n_grid^3 grid -- the points the encoder reads
+    (SimulationDataset.py:36-43 trains on the first N/10 points of a frame)."""
+    import copy
+    sc = copy.copy(scene)
+    scale = scene.n_particles / float(n_sample)
+    sc.x0 = np.ascontiguousarray(scene.x0[:n_sample])
+    sc.v0 = np.ascontiguousarray(scene.v0[:n_sample])
+    sc.vol = np.ascontiguousarray(scene.vol[:n_sample] * scale).astype(np.float32)
+    sc.n_grid = int(n_grid)
+    return sc
+
+
+class ReducedRun:
+    """A sand run on the CROM-reduced path: the MPM solver advances only the
+    encoder's sample of particles (on a coarser grid); every frame the
+    encoder turns the sample's displacements into a latent and the decoder
+    reconstructs the displacement of all N particles.  The full-order MPM
+    run is the judge of its error and the fallback: with `judge=True` it
+    advances in lockstep, each frame's relative L2 error is recorded, and
+    from the first frame whose error exceeds `tol` the frames are the full
+    run's own.
+
+        rr = ReducedRun(crom, scene, friction_angle, n_sample=N // 10, coarse_grid=64)
+        out = rr.run(n_frames, substeps_per_frame, dt, judge=True, tol=0.15)
+        out["frames"]  (n_frames, N, 3) displacements; out["source"] "crom" | "mpm" per frame
+    """
+
+    def __init__(self, crom: Crom, scene, friction_angle: float, n_sample: int | None = None,
+                 coarse_grid: int | None = None, device="cuda:0"):
+        self.crom, self.scene, self.phi = crom, scene, float(friction_angle)
+        self.n_sample = int(n_sample or crom.npoints)
+        if self.n_sample != crom.npoints:
+            raise ValueError("n_sample must equal the encoder's number of points")
+        self.coarse_grid = int(coarse_grid or max(16, scene.n_grid // 2))
+        self.device = torch.device(device)
+        self.X = torch.from_numpy(np.ascontiguousarray(scene.x0, dtype=np.float32)).to(self.device)
+
+    def _sim(self, scene):
+        import copy
+        from . import scenes as S
+        sc = copy.copy(scene)
+        sc.friction_angle = self.phi
+        return S.make_simulator(sc, device=self.device)
+
+    def decode(self, u_sample: torch.Tensor) -> torch.Tensor:
+        """Displacements of all N particles from the sample's (n_sample, 3)."""
+        z = self.crom.encoder(u_sample[None].contiguous())
+        n = self.X.shape[0]
+        xin = torch.cat([z.expand(n, self.crom.lbl), self.X], 1).contiguous()
+        return self.crom.decoder(xin)
+
+    def run(self, n_frames: int, substeps_per_frame: int, dt: float, judge: bool = False,
+            tol: float | None = None) -> dict:
+        coarse = self._sim(sampled_scene(self.scene, self.n_sample, self.coarse_grid))
+        full = self._sim(self.scene) if judge else None
+        frames = torch.empty((n_frames, self.X.shape[0], 3), dtype=torch.float32, device=self.device)
+        source, errors = [], []
+        fallen = False
+        t_reduced = t_full = 0.0
+        for f in range(n_frames):
+            if f > 0:
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                if not fallen:
+                    coarse.run(substeps_per_frame, dt)
+                    torch.cuda.synchronize()
+                t1 = time.perf_counter()
+                if full is not None:
+                    full.run(substeps_per_frame, dt)
+                    torch.cuda.synchronize()
+                t2 = time.perf_counter()
+                t_reduced += t1 - t0
+                t_full += t2 - t1
+            u_true = full.export_uvFstress_from_warp()[0] if full is not None else None
+            if not fallen:
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                q = self.decode(coarse.export_uvFstress_from_warp()[0])
+                torch.cuda.synchronize()
+                t_reduced += time.perf_counter() - t0
+            if u_true is not None:
+                den = float(u_true.norm())
+                err = float((q - u_true).norm()) / den if (den > 0 and not fallen) else 0.0
+                errors.append(err)
+                if tol is not None and not fallen and err > tol:
+                    fallen = True
+            if fallen:
+                frames[f] = u_true
+                source.append("mpm")
+            else:
+                frames[f] = q
+                source.append("crom")
+        return {"frames": frames, "source": source, "errors": errors if judge else None,
+                "reduced_seconds": t_reduced, "full_seconds": t_full if judge else None}
