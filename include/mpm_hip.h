@@ -279,9 +279,10 @@ int mpm_adjoint_segment(const MpmParams* prm, const MpmState* states, const void
  * particle and component.  mpm_sort_by_bin counting-sorts a flat state by
  * the 8^3-cell bin and 4^3-cell sub-block of each particle's stencil base
  * (mpm_utils.py:197-199) into `dst` and fills `order` (mpm_order_bytes
- * bytes, 256-byte aligned): the chunk table (runs of at most 512 particles
- * inside one bin) and the permutation perm[sorted slot] = source slot (int32,
- * at byte offset mpm_order_perm_offset).  The order stays valid for any state
+ * bytes, 256-byte aligned): the chunk table (runs of at most 512 particles,
+ * 1024 from 500k particles up, inside one bin) and the permutation
+ * perm[sorted slot] = source slot (int32, at byte offset
+ * mpm_order_perm_offset).  The order stays valid for any state
  * in the same slot order -- e.g. the substeps after the sort -- as particles
  * drift; it only loses speed.  No counterpart in the reference (Warp's P2G
  * runs one thread per particle in storage order). */

@@ -31,6 +31,7 @@
 #include "mpm_device.h"
 #include "mpm_hip.h"
 #include "mpm_ordered.h"
+#include "mpm_stencil.h"
 
 using namespace mpm;
 
@@ -282,150 +283,208 @@ __device__ void block_add_params(float mub, float lamb, float alb, double* out) 
 }
 
 // ---------------------------------------------------------------------------
-// G2P adjoint of particle p.  st: S_n; gv: grid velocities of substep n; an:
-// adjoint of S_{n+1}; ac: adjoint of S_n (x and F written here, v, C, tau by
-// k_p2g_adj).  scatter(base, i, j, k, vbn[3]) receives the particle's
-// grid-velocity adjoint contribution to stencil node (i, j, k).
+// G2P adjoint of one particle.
+//
+// Per axis a the quadratic B-spline factors at fx_a: w, e = w (n - fx),
+// d = dw/dfx, and their x-derivatives w' = inv_dx dw, e' = inv_dx (dw (n - fx)
+// - w), d' = inv_dx ddw (ddw = 1, -2, 1).  The grid-velocity adjoint the
+// particle scatters is
+//   vbn_a(ijk) = w vt_a + 4 inv_dx w sum_d Cb_ad (n_d - fx_d)
+//                + sum_d Gb_ad grad_d w
+//              = A_a[i] w_y w_z + w_x B_a[j] w_z + w_x w_y C_a[k]
+// with A_a = vt_a w_x + c4 Cb_a0 e_x + inv_dx Gb_a0 d_x, B_a = c4 Cb_a1 e_y +
+// inv_dx Gb_a1 d_y, C_a = c4 Cb_a2 e_z + inv_dx Gb_a2 d_z, and the position
+// adjoint it adds is sum_nodes sum_a v_a d(vbn_a)/dx_e: the same form with
+// the axis-e factor differentiated.  Both are evaluated as tensor-product
+// contractions (z, then y, then x) instead of per node.
 // ---------------------------------------------------------------------------
-template <typename Scatter>
-__device__ __forceinline__ void g2p_adj_particle(const float* __restrict__ st, int64_t S, int p, const MpmParams& prm,
-                                                 float dt, const float4* __restrict__ gv, const float* __restrict__ an,
-                                                 int64_t SA, float* __restrict__ ac, float& mub, float& lamb,
-                                                 float& alb, Scatter&& scatter) {
-    {
-        const float* sp = st + p;
-        float x[3];
+// per-axis B-spline factors at fx: w, e = w (n - fx), d = dw/dfx and their
+// x-derivatives (n = 0, 1, 2)
+struct AxisFactors {
+    float w[3], e[3], d[3], wp[3], ep[3], dp[3];
+};
+
+__device__ __forceinline__ AxisFactors axis_factors(float fx, float inv_dx) {
+    AxisFactors f;
+    const float wa = 1.5f - fx, wb = fx - 1.0f, wc = fx - 0.5f;
+    f.w[0] = 0.5f * wa * wa;
+    f.w[1] = 0.75f - wb * wb;
+    f.w[2] = 0.5f * wc * wc;
+    f.d[0] = fx - 1.5f;
+    f.d[1] = -2.0f * (fx - 1.0f);
+    f.d[2] = fx - 0.5f;
 #pragma unroll
-        for (int d = 0; d < 3; ++d) x[d] = sp[(MPM_P_X0 + d) * S] + sp[(MPM_P_U + d) * S];
-        float F[9];
-        load9(sp, S, MPM_P_F, F);
-        const int ng = prm.n_grid;
-        const float inv_dx = prm.inv_dx;
-        W3 W;
-        weights3(x, inv_dx, W);
-        const bool ok = W.base[0] >= 0 && W.base[1] >= 0 && W.base[2] >= 0 && W.base[0] <= ng - 3 &&
-                        W.base[1] <= ng - 3 && W.base[2] <= ng - 3;
-        const float* ap = an + p;
-        float xb[3], vbp[3], Cb[9], Fbn[9], Tb[9];
+    for (int n = 0; n < 3; ++n) {
+        const float off = (float)n - fx;
+        f.e[n] = f.w[n] * off;
+        f.wp[n] = inv_dx * f.d[n];
+        f.ep[n] = inv_dx * (f.d[n] * off - f.w[n]);
+        f.dp[n] = inv_dx * (n == 1 ? -2.0f : 1.0f);
+    }
+    return f;
+}
+
+// The 27 stencil nodes (t = i*9 + j*3 + k, gather_tile's strides 9, 3) of
+// a particle: in the chunk's LDS tile (SoA, kCT^3 nodes; ti = tile index of
+// node 0), or in the global grid (node 0 at index gb).
+struct TileNodes {
+    const float* s_v;
+    int ti;
+    __device__ __forceinline__ float comp(int c, int t) const {
+        const int i = t / 9, j = (t / 3) % 3, k = t % 3;
+        return s_v[c * kCT3 + ti + (i * kCT + j) * kCT + k];
+    }
+};
+struct GridNodes {
+    const float4* gv;
+    int64_t gb;
+    int ng;
+    __device__ __forceinline__ float comp(int c, int t) const {
+        const int i = t / 9, j = (t / 3) % 3, k = t % 3;
+        const float4 g = gv[gb + ((int64_t)i * ng + j) * ng + k];
+        return c == 0 ? g.x : (c == 1 ? g.y : g.z);
+    }
+};
+
+// vbn scatter (emit(a, i, j, k, value)) and position adjoint (added to xg).
+template <typename Nodes, typename Emit>
+__device__ __forceinline__ void g2p_adj_nodes(const Nodes& V, const float fx[3], const float vt[3], const float Cb[9],
+                                              const float Gb[9], float inv_dx, float xg[3], Emit&& emit) {
+    const float c4 = 4.0f * inv_dx;
+#pragma unroll 1
+    for (int a = 0; a < 3; ++a) {
+        const AxisFactors X0 = axis_factors(fx[0], inv_dx);
+        const AxisFactors X1 = axis_factors(fx[1], inv_dx);
+        const AxisFactors X2 = axis_factors(fx[2], inv_dx);
+        const float ca0 = c4 * Cb[a * 3], ca1 = c4 * Cb[a * 3 + 1], ca2 = c4 * Cb[a * 3 + 2];
+        const float ga0 = inv_dx * Gb[a * 3], ga1 = inv_dx * Gb[a * 3 + 1], ga2 = inv_dx * Gb[a * 3 + 2];
+        float A[3], Ap[3], B[3], Bp[3], Cc[3], Ccp[3];
 #pragma unroll
-        for (int d = 0; d < 3; ++d) { xb[d] = ap[(kAX + d) * SA]; vbp[d] = ap[(kAV + d) * SA]; }
-        load9(ap, SA, kAC, Cb);
-        load9(ap, SA, kAF, Fbn);
-        load9(ap, SA, kAT, Tb);
-        float Fb[9];
-        if (!ok) {
-            // frozen particle: copied unchanged (x, F pass through)
-#pragma unroll
-            for (int k = 0; k < 9; ++k) Fb[k] = Fbn[k];
-        } else {
-            // forward gather: G = sum v_i (x) grad w
-            float G[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-            for (int i = 0; i < 3; ++i)
-                for (int j = 0; j < 3; ++j)
-                    for (int k = 0; k < 3; ++k) {
-                        float w, g[3], H[9];
-                        node_w(W, i, j, k, inv_dx, w, g, H);
-                        const float4 vi = gv[((int64_t)(W.base[0] + i) * ng + (W.base[1] + j)) * ng + (W.base[2] + k)];
-                        const float va[3] = {vi.x, vi.y, vi.z};
-#pragma unroll
-                        for (int a = 0; a < 3; ++a)
-#pragma unroll
-                            for (int d = 0; d < 3; ++d) G[a * 3 + d] += va[a] * g[d];
-                    }
-            float Gb[9];
-            sand_update_adj(G, F, dt, prm.mu, prm.lam, prm.alpha, Fbn, Tb, Gb, Fb, mub, lamb, alb);
-            // x' = x + dt v'
-            float vt[3];
-#pragma unroll
-            for (int d = 0; d < 3; ++d) vt[d] = vbp[d] + dt * xb[d];
-            const float c4 = 4.0f * inv_dx;
-            float xg[3] = {xb[0], xb[1], xb[2]};
-            for (int i = 0; i < 3; ++i)
-                for (int j = 0; j < 3; ++j)
-                    for (int k = 0; k < 3; ++k) {
-                        float w, g[3], H[9];
-                        node_w(W, i, j, k, inv_dx, w, g, H);
-                        const int64_t node = ((int64_t)(W.base[0] + i) * ng + (W.base[1] + j)) * ng + (W.base[2] + k);
-                        const float4 vi = gv[node];
-                        const float va[3] = {vi.x, vi.y, vi.z};
-                        const float dp[3] = {(float)i - W.fx[0], (float)j - W.fx[1], (float)k - W.fx[2]};  // grid units
-                        // grid velocity adjoint
-                        float vbn[3];
-#pragma unroll
-                        for (int a = 0; a < 3; ++a) {
-                            float t = w * vt[a];
-#pragma unroll
-                            for (int d = 0; d < 3; ++d) t += c4 * w * Cb[a * 3 + d] * dp[d] + Gb[a * 3 + d] * g[d];
-                            vbn[a] = t;
-                        }
-                        scatter(W.base, i, j, k, vbn);
-                        // position adjoint through w, (x_i - x_p) and grad w
-                        float vv = 0.f, cv[3] = {0, 0, 0};   // v_i . vt,  C_b^T v_i
-#pragma unroll
-                        for (int a = 0; a < 3; ++a) {
-                            vv += va[a] * vt[a];
-#pragma unroll
-                            for (int d = 0; d < 3; ++d) cv[d] += Cb[a * 3 + d] * va[a];
-                        }
-                        float cdp = cv[0] * dp[0] + cv[1] * dp[1] + cv[2] * dp[2];
-                        float gh[3] = {0, 0, 0};              // sum_ad Gb_ad v_a H_de
-#pragma unroll
-                        for (int a = 0; a < 3; ++a)
-#pragma unroll
-                            for (int d = 0; d < 3; ++d)
-#pragma unroll
-                                for (int e = 0; e < 3; ++e) gh[e] += Gb[a * 3 + d] * va[a] * H[d * 3 + e];
-#pragma unroll
-                        for (int e = 0; e < 3; ++e)
-                            xg[e] += g[e] * (vv + c4 * cdp) - c4 * w * inv_dx * cv[e] + gh[e];
-                    }
-#pragma unroll
-            for (int d = 0; d < 3; ++d) xb[d] = xg[d];
+        for (int n = 0; n < 3; ++n) {
+            A[n] = fmaf(vt[a], X0.w[n], fmaf(ca0, X0.e[n], ga0 * X0.d[n]));
+            Ap[n] = fmaf(vt[a], X0.wp[n], fmaf(ca0, X0.ep[n], ga0 * X0.dp[n]));
+            B[n] = fmaf(ca1, X1.e[n], ga1 * X1.d[n]);
+            Bp[n] = fmaf(ca1, X1.ep[n], ga1 * X1.dp[n]);
+            Cc[n] = fmaf(ca2, X2.e[n], ga2 * X2.d[n]);
+            Ccp[n] = fmaf(ca2, X2.ep[n], ga2 * X2.dp[n]);
         }
-        float* cp = ac + p;
+        float t0 = 0.f, t1 = 0.f, t2 = 0.f;
 #pragma unroll
-        for (int d = 0; d < 3; ++d) cp[(kAX + d) * SA] = xb[d];
+        for (int i = 0; i < 3; ++i) {
+            float y0 = 0.f, y1 = 0.f, y2 = 0.f, y3 = 0.f, y4 = 0.f, y5 = 0.f, y6 = 0.f, y7 = 0.f, y8 = 0.f;
 #pragma unroll
-        for (int k = 0; k < 9; ++k) cp[(kAF + k) * SA] = Fb[k];
-        if (!ok) {
-            // a frozen particle's v, C, tau pass through as well
+            for (int j = 0; j < 3; ++j) {
+                const float ab = fmaf(A[i], X1.w[j], X0.w[i] * B[j]);
+                const float w01 = X0.w[i] * X1.w[j];
+                float zw = 0.f, zc = 0.f, zwp = 0.f, zcp = 0.f;
 #pragma unroll
-            for (int d = 0; d < 3; ++d) cp[(kAV + d) * SA] = vbp[d];
-#pragma unroll
-            for (int k = 0; k < 9; ++k) {
-                cp[(kAC + k) * SA] = Cb[k];
-                cp[(kAT + k) * SA] = Tb[k];
+                for (int k = 0; k < 3; ++k) {
+                    const float v = V.comp(a, i * 9 + j * 3 + k);
+                    zw = fmaf(X2.w[k], v, zw);
+                    zc = fmaf(Cc[k], v, zc);
+                    zwp = fmaf(X2.wp[k], v, zwp);
+                    zcp = fmaf(Ccp[k], v, zcp);
+                    emit(a, i, j, k, fmaf(ab, X2.w[k], w01 * Cc[k]));
+                }
+                y0 = fmaf(X1.w[j], zw, y0);
+                y1 = fmaf(B[j], zw, y1);
+                y2 = fmaf(X1.w[j], zc, y2);
+                y3 = fmaf(X1.wp[j], zw, y3);
+                y4 = fmaf(Bp[j], zw, y4);
+                y5 = fmaf(X1.wp[j], zc, y5);
+                y6 = fmaf(X1.w[j], zwp, y6);
+                y7 = fmaf(B[j], zwp, y7);
+                y8 = fmaf(X1.w[j], zcp, y8);
             }
+            t0 = fmaf(Ap[i], y0, fmaf(X0.wp[i], y1 + y2, t0));
+            t1 = fmaf(A[i], y3, fmaf(X0.w[i], y4 + y5, t1));
+            t2 = fmaf(A[i], y6, fmaf(X0.w[i], y7 + y8, t2));
         }
+        xg[0] += t0;
+        xg[1] += t1;
+        xg[2] += t2;
     }
 }
 
+// G2P adjoint of particle p (stencil inside the grid).  st: S_n; an: adjoint
+// of S_{n+1}; ac: adjoint of S_n (x and F written here, v, C, tau by
+// k_p2g_adj).  V: the particle's 27 grid velocities; emit(a, i, j, k, value)
+// receives its grid-velocity adjoint.
+template <typename Nodes, typename Emit>
+__device__ __forceinline__ void g2p_adj_particle(const float* __restrict__ st, int64_t S, int p, const MpmParams& prm,
+                                                 float dt, const float x[3], const AxisWeights& Aw, const Nodes& V,
+                                                 const float* __restrict__ an, int64_t SA, float* __restrict__ ac,
+                                                 float& mub, float& lamb, float& alb, Emit&& emit) {
+    const float inv_dx = prm.inv_dx;
+    float G[9];
+    {
+        float v[3], Cm[9];
+        gather_tile<9, 3>(V, 0, Aw, inv_dx, v, Cm, G);
+    }
+    const float* ap = an + p;
+    float xb[3], vt[3], Cb[9], Gb[9], Fb[9];
+    {
+        float F[9], Fbn[9], Tb[9];
+        load9(st + p, S, MPM_P_F, F);
+        load9(ap, SA, kAF, Fbn);
+        load9(ap, SA, kAT, Tb);
+        sand_update_adj(G, F, dt, prm.mu, prm.lam, prm.alpha, Fbn, Tb, Gb, Fb, mub, lamb, alb);
+    }
+    load9(ap, SA, kAC, Cb);
+    // x' = x + dt v'
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        xb[d] = ap[(kAX + d) * SA];
+        vt[d] = ap[(kAV + d) * SA] + dt * xb[d];
+    }
+    float fx[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) fx[d] = x[d] * inv_dx - (float)Aw.base[d];
+    g2p_adj_nodes(V, fx, vt, Cb, Gb, inv_dx, xb, emit);
+    float* cp = ac + p;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) cp[(kAX + d) * SA] = xb[d];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) cp[(kAF + k) * SA] = Fb[k];
+}
+
+__device__ __forceinline__ bool base_inside(const int b[3], int ng) {
+    return b[0] >= 0 && b[1] >= 0 && b[2] >= 0 && b[0] <= ng - 3 && b[1] <= ng - 3 && b[2] <= ng - 3;
+}
+
+// A frozen particle (stencil outside the grid) is copied unchanged by the
+// forward substep: its whole adjoint passes through.
+__device__ __forceinline__ void frozen_adj(const float* __restrict__ an, int64_t SA, int p, float* __restrict__ ac) {
+#pragma unroll 3
+    for (int k = 0; k < MPM_ADJ_NPLANES; ++k) ac[k * SA + p] = an[k * SA + p];
+}
+
 // G2P adjoint over the chunks of an ordered state (mpm_sort_by_bin): the
-// grid-velocity adjoint is accumulated in an FP64 LDS tile at the chunk's
-// smallest stencil base and flushed once per node (chunk_flush); particles
-// whose stencil leaves the tile add to vb directly.
+// chunk's grid velocities are staged in an LDS tile at its smallest stencil
+// base; the grid-velocity adjoint is accumulated in an FP64 LDS tile there and
+// flushed once per node (chunk_flush); particles whose stencil leaves the
+// tile gather from and add to the global grids directly.
 __global__ void __launch_bounds__(kChunkWG)
     k_g2p_adj_chunks(const float* __restrict__ st, int64_t S, int n, const int4* __restrict__ chunks,
                      const int* __restrict__ meta, MpmParams prm, float dt, const float4* __restrict__ gv,
                      const float* __restrict__ an, int64_t SA, float* __restrict__ vb, float* __restrict__ ac,
                      double* __restrict__ pb) {
     __shared__ double s_t[3 * kCT3];
+    __shared__ float s_v[3 * kCT3];
     __shared__ int s_lo[3];
     const int c = blockIdx.x;
     if (c >= meta[0]) return;
     const int4 ch = chunks[c];
     const int tid = threadIdx.x;
     const int ng = prm.n_grid;
-    constexpr int kPer = kChunk / kChunkWG;
+    const int n_per = min(ch.y, kChunkMax);
     if (tid < 3) s_lo[tid] = INT_MAX;
     for (int t = tid; t < 3 * kCT3; t += kChunkWG) s_t[t] = 0.0;
     __syncthreads();
     int lo[3] = {INT_MAX, INT_MAX, INT_MAX};
-#pragma unroll
-    for (int r = 0; r < kPer; ++r) {
-        const int i = r * kChunkWG + tid;
+    for (int i = tid; i < n_per; i += kChunkWG) {
         const int p = ch.x + i;
-        if (i >= ch.y || p < 0 || p >= n) continue;
+        if (p < 0 || p >= n) continue;
         int b[3];
 #pragma unroll
         for (int d = 0; d < 3; ++d)
@@ -438,26 +497,56 @@ __global__ void __launch_bounds__(kChunkWG)
     __syncthreads();
     const int o[3] = {s_lo[0], s_lo[1], s_lo[2]};
     const bool any = o[0] != INT_MAX;
+    if (any) {
+        // the chunk's grid velocities (nodes outside the grid read as 0)
+        for (int t = tid; t < kCT3; t += kChunkWG) {
+            const int x = o[0] + t / (kCT * kCT), y = o[1] + (t / kCT) % kCT, z = o[2] + t % kCT;
+            float4 g = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (x < ng && y < ng && z < ng) g = gv[((int64_t)x * ng + y) * ng + z];
+            s_v[t] = g.x;
+            s_v[kCT3 + t] = g.y;
+            s_v[2 * kCT3 + t] = g.z;
+        }
+    }
+    __syncthreads();
     float mub = 0.f, lamb = 0.f, alb = 0.f;
-    for (int r = 0; r < kPer; ++r) {
-        const int i = r * kChunkWG + tid;
+    // particles whose stencil lies in the chunk's tile; the others are
+    // marked (bit r for the thread's r-th particle) for the second loop
+    unsigned outside = 0;
+    for (int i = tid, r = 0; i < n_per; i += kChunkWG, ++r) {
         const int p = ch.x + i;
-        if (i >= ch.y || p < 0 || p >= n) continue;
-        g2p_adj_particle(st, S, p, prm, dt, gv, an, SA, ac, mub, lamb, alb,
-                         [&](const int base[3], int i3, int j3, int k3, const float vbn[3]) {
-                             const int ti = any ? chunk_tile_index(base, o) : -1;
-                             if (ti >= 0) {
-                                 const int t = ti + (i3 * kCT + j3) * kCT + k3;
-                                 atomicAdd(&s_t[t], (double)vbn[0]);
-                                 atomicAdd(&s_t[kCT3 + t], (double)vbn[1]);
-                                 atomicAdd(&s_t[2 * kCT3 + t], (double)vbn[2]);
-                             } else {
-                                 const int64_t node =
-                                     ((int64_t)(base[0] + i3) * ng + (base[1] + j3)) * ng + (base[2] + k3);
-                                 unsafeAtomicAdd(vb + 4 * node + 0, vbn[0]);
-                                 unsafeAtomicAdd(vb + 4 * node + 1, vbn[1]);
-                                 unsafeAtomicAdd(vb + 4 * node + 2, vbn[2]);
-                             }
+        if (p < 0 || p >= n) continue;
+        float x[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) x[d] = st[(MPM_P_X0 + d) * S + p] + st[(MPM_P_U + d) * S + p];
+        AxisWeights Aw;
+        axis_weights(x, prm.inv_dx, Aw);
+        if (!base_inside(Aw.base, ng)) {
+            frozen_adj(an, SA, p, ac);
+            continue;
+        }
+        const int ti = any ? chunk_tile_index(Aw.base, o) : -1;
+        if (ti < 0) {
+            outside |= 1u << r;
+            continue;
+        }
+        g2p_adj_particle(st, S, p, prm, dt, x, Aw, TileNodes{s_v, ti}, an, SA, ac, mub, lamb, alb,
+                         [&](int a, int i3, int j3, int k3, float val) {
+                             atomicAdd(&s_t[a * kCT3 + ti + (i3 * kCT + j3) * kCT + k3], (double)val);
+                         });
+    }
+    for (int i = tid, r = 0; outside && i < n_per; i += kChunkWG, ++r) {
+        if (!(outside & (1u << r))) continue;
+        const int p = ch.x + i;
+        float x[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) x[d] = st[(MPM_P_X0 + d) * S + p] + st[(MPM_P_U + d) * S + p];
+        AxisWeights Aw;
+        axis_weights(x, prm.inv_dx, Aw);
+        const int64_t gb = ((int64_t)Aw.base[0] * ng + Aw.base[1]) * ng + Aw.base[2];
+        g2p_adj_particle(st, S, p, prm, dt, x, Aw, GridNodes{gv, gb, ng}, an, SA, ac, mub, lamb, alb,
+                         [&](int a, int i3, int j3, int k3, float val) {
+                             unsafeAtomicAdd(vb + 4 * (gb + ((int64_t)i3 * ng + j3) * ng + k3) + a, val);
                          });
     }
     block_add_params(mub, lamb, alb, pb);

@@ -6,7 +6,7 @@
 // CU, it is the cost of a flat substep.  Here the flat state is first
 // counting-sorted by bin and sub-block of its stencil base (k_order_hist,
 // k_order_scan, k_order_scatter), which also yields a table of chunks --
-// runs of at most kChunk particles inside one bin.  k_p2g_chunks then gives
+// runs of at most 512 or 1024 particles inside one bin.  k_p2g_chunks then gives
 // each chunk one workgroup that scatters into an FP64 LDS tile placed at the
 // chunk's smallest stencil base (ds_add_f64) and flushes the tile with one
 // float atomic per touched node and component.  The order stays valid while
@@ -80,7 +80,7 @@ __global__ void __launch_bounds__(kB)
 // bins, so every bin's keys and chunks are handled by one thread.
 __global__ void __launch_bounds__(kScan)
     k_order_scan(int* __restrict__ hist, int* __restrict__ cursor, int4* __restrict__ chunks, int* __restrict__ meta,
-                 int nbins, int max_chunks) {
+                 int nbins, int max_chunks, int chunk) {
     __shared__ int s_p[kScan], s_c[kScan];
     const int tid = threadIdx.x;
     const int per = (nbins + kScan - 1) / kScan;
@@ -91,7 +91,7 @@ __global__ void __launch_bounds__(kScan)
 #pragma unroll
         for (int k = 0; k < 8; ++k) c += hist[8 * b + k];
         np += c;
-        nc += (c + kChunk - 1) / kChunk;
+        nc += (c + chunk - 1) / chunk;
     }
     s_p[tid] = np;
     s_c[tid] = nc;
@@ -114,8 +114,8 @@ __global__ void __launch_bounds__(kScan)
             run += c;
         }
         const int cnt = run - first;
-        for (int s = 0; s < cnt; s += kChunk, ++crun)
-            if (crun < max_chunks) chunks[crun] = make_int4(first + s, min(kChunk, cnt - s), b, 0);
+        for (int s = 0; s < cnt; s += chunk, ++crun)
+            if (crun < max_chunks) chunks[crun] = make_int4(first + s, min(chunk, cnt - s), b, 0);
     }
     if (tid == kScan - 1) {
         hist[8 * nbins] = s_p[kScan - 1];
@@ -148,7 +148,7 @@ __global__ void __launch_bounds__(kB)
 
 // ---------------------------------------------------------------------------
 // p2g_apic_stress_given (mpm_utils.py:190-227) over chunks: one workgroup per
-// chunk, kChunk / kChunkWG particles per thread.
+// chunk, its particles kChunkWG at a time.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kChunkWG)
     k_p2g_chunks(const float* __restrict__ planes, int64_t S, int n, const int4* __restrict__ chunks,
@@ -160,16 +160,14 @@ __global__ void __launch_bounds__(kChunkWG)
     const int4 ch = chunks[c];
     const int tid = threadIdx.x;
     const int ng = prm.n_grid;
-    constexpr int kPer = kChunk / kChunkWG;
+    const int n_per = min(ch.y, kChunkMax);   // a chunk's particles, kChunkWG at a time
     if (tid < 3) s_lo[tid] = INT_MAX;
     for (int t = tid; t < 4 * kCT3; t += kChunkWG) s_t[t] = 0.0;
     __syncthreads();
     int lo[3] = {INT_MAX, INT_MAX, INT_MAX};
-#pragma unroll
-    for (int r = 0; r < kPer; ++r) {
-        const int i = r * kChunkWG + tid;
+    for (int i = tid; i < n_per; i += kChunkWG) {
         const int p = ch.x + i;
-        if (i >= ch.y || p < 0 || p >= n) continue;
+        if (p < 0 || p >= n) continue;
         int b[3];
         stencil_base(planes + p, S, prm.inv_dx, b);
         if (!base_ok(b, ng)) continue;
@@ -180,10 +178,9 @@ __global__ void __launch_bounds__(kChunkWG)
     __syncthreads();
     if (s_lo[0] == INT_MAX) return;   // no particle inside the grid (uniform)
     const int o[3] = {s_lo[0], s_lo[1], s_lo[2]};
-    for (int r = 0; r < kPer; ++r) {
-        const int i = r * kChunkWG + tid;
+    for (int i = tid; i < n_per; i += kChunkWG) {
         const int p = ch.x + i;
-        if (i >= ch.y || p < 0 || p >= n) continue;
+        if (p < 0 || p >= n) continue;
         const float* sp = planes + p;
         float x[3], v[3], Cm[9], tau[9];
 #pragma unroll
@@ -295,7 +292,7 @@ int mpm_sort_by_bin(const MpmParams* prm, const MpmState* src, const MpmState* d
         if ((rc = check_launch("k_order_hist"))) return rc;
     }
     hipLaunchKernelGGL(k_order_scan, dim3(1), dim3(kScan), 0, s, hist, cursor, chunks, meta, (int)L.nbins,
-                       (int)L.max_chunks);
+                       (int)L.max_chunks, (int)L.chunk);
     if ((rc = check_launch("k_order_scan"))) return rc;
     if (n > 0) {
         hipLaunchKernelGGL(k_order_scatter, dim3(nblk(n)), dim3(kB), 0, s, (const float*)src->planes, src->stride,
