@@ -259,14 +259,15 @@ int mpm_adjoint_substep(const MpmParams* prm, const MpmState* st, const void* or
                         const float* adj_next, float* adj_cur, int64_t adj_stride, double* param_adj, void* stream);
 
 /* The adjoint of the n_sub substeps k0 .. k0+n_sub-1 of one ordered segment
- * in one call: replays the segment from its first state (mpm_step_ordered,
- * keeping each substep's grids), then applies the adjoint substeps in
- * reverse.  states: n_sub states sharing n and stride; states[0] holds S_k0
- * on entry, states[1..] receive S_{k0+1} .. S_{k0+n_sub-1}.  step_grids:
- * n_sub - 1 workspaces of mpm_workspace_bytes (256-byte aligned).  Substep
- * k runs at time (float)(k * dt) with step (float)dt.  adj_a holds the
- * adjoint of S_{k0+n_sub} on entry and the adjoint of S_k0 on return; adj_b is
- * scratch of the same size.  workspace: mpm_adjoint_workspace_bytes. */
+ * in one call: replays the segment from its first state (the substep of
+ * mpm_run_ordered, each substep's P2G accumulator and grid velocities kept in
+ * its own workspace), then applies the adjoint substeps in reverse on those
+ * grids.  states: n_sub states sharing n and stride; states[0] holds S_k0 on
+ * entry, states[1..] receive S_{k0+1} .. S_{k0+n_sub-1}.  step_grids: n_sub
+ * workspaces of mpm_workspace_bytes (256-byte aligned).  Substep k runs at
+ * time (float)(k * dt) with step (float)dt.  adj_a holds the adjoint of
+ * S_{k0+n_sub} on entry and the adjoint of S_k0 on return; adj_b is scratch of
+ * the same size.  workspace: mpm_adjoint_workspace_bytes. */
 int mpm_adjoint_segment(const MpmParams* prm, const MpmState* states, const void* order, void* const* step_grids,
                         const MpmCollider* colliders, int n_colliders, void* workspace, int k0, double dt, int n_sub,
                         float* adj_a, float* adj_b, int64_t adj_stride, double* param_adj, void* stream);
@@ -294,8 +295,12 @@ int mpm_sort_by_bin(const MpmParams* prm, const MpmState* src, const MpmState* d
 int mpm_step_ordered(const MpmParams* prm, const MpmState* st, const void* order, const MpmCollider* colliders,
                      int n_colliders, void* workspace, int64_t workspace_bytes, float time, float dt, void* stream);
 
-/* n_sub substeps k0 .. k0+n_sub-1 of mpm_step_ordered, substep k at time
- * (float)(k * dt) with step (float)dt. */
+/* n_sub substeps k0 .. k0+n_sub-1 on an ordered state, substep k at time
+ * (float)(k * dt) with step (float)dt: the P2G of the first, then per
+ * substep the grid update and one fused kernel doing this substep's G2P and
+ * the next substep's P2G over the chunks (LDS tiles of grid velocities and
+ * of the next P2G).  workspace: mpm_run_ordered_workspace_bytes. */
+int64_t mpm_run_ordered_workspace_bytes(const MpmParams* prm);
 int mpm_run_ordered(const MpmParams* prm, const MpmState* st, const void* order, const MpmCollider* colliders,
                     int n_colliders, void* workspace, int64_t workspace_bytes, int k0, double dt, int n_sub,
                     void* stream);

@@ -115,6 +115,7 @@ _SIGNATURES = {
     "mpm_adjoint_segment": (C.c_int, [C.POINTER(MpmParams), C.POINTER(MpmState), C.c_void_p, C.c_void_p,
                                       C.POINTER(MpmCollider), C.c_int, C.c_void_p, C.c_int, C.c_double, C.c_int,
                                       C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
+    "mpm_run_ordered_workspace_bytes": (C.c_int64, [C.POINTER(MpmParams)]),
     "mpm_run_ordered": (C.c_int, [C.POINTER(MpmParams), C.POINTER(MpmState), C.c_void_p, C.POINTER(MpmCollider),
                                   C.c_int, C.c_void_p, C.c_int64, C.c_int, C.c_double, C.c_int, C.c_void_p]),
     "mpm_order_bytes": (C.c_int64, [C.POINTER(MpmParams), C.c_int64]),

@@ -82,6 +82,8 @@ class Rollout:
             self._prm.gravity[d] = float(P["gravity"][d])
         self._ws_step_bytes = int(lib.mpm_workspace_bytes(self._prm, self.n))
         self._ws_step = torch.zeros(self._ws_step_bytes + 256, dtype=torch.uint8, device=self.device)
+        self._ws_run_bytes = int(lib.mpm_run_ordered_workspace_bytes(self._prm))
+        self._ws_run = torch.zeros(self._ws_run_bytes + 256, dtype=torch.uint8, device=self.device)
         self._ws_adj = torch.zeros(int(lib.mpm_adjoint_workspace_bytes(self._prm)) + 256, dtype=torch.uint8,
                                    device=self.device)
         self._order_bytes = int(lib.mpm_order_bytes(self._prm, self.n))
@@ -150,7 +152,7 @@ class Rollout:
             planes, order = self._sort(planes)
             self._checkpoints[c0] = (planes.clone(), order)
             L.check(lib.mpm_run_ordered(self._prm, self._state(planes), self._aligned(order), self._cols,
-                                        self._ncol, self._aligned(self._ws_step), self._ws_step_bytes, c0, self.dt,
+                                        self._ncol, self._aligned(self._ws_run), self._ws_run_bytes, c0, self.dt,
                                         min(self.K, self.n_substeps - c0), self._stream()), "mpm_run_ordered")
         self._final = planes
         # x0 and u back to the caller's order (scatter by the pid plane)
@@ -174,6 +176,7 @@ class Rollout:
         # leaves for its adjoint (one native call per segment)
         while len(self._seg_states) < self.K - 1:
             self._seg_states.append(torch.empty((L.NPLANES, self.n), dtype=torch.float32, device=self.device))
+        while len(self._ws_seg) < self.K:
             self._ws_seg.append(torch.empty(self._ws_step_bytes + 256, dtype=torch.uint8, device=self.device))
         for c0 in sorted(self._checkpoints, reverse=True):
             nk = min(self.K, self.n_substeps - c0)
@@ -182,7 +185,7 @@ class Rollout:
             states[0] = self._state(cp)
             for j in range(1, nk):
                 states[j] = self._state(self._seg_states[j - 1])
-            grids = (ctypes.c_void_p * max(1, nk - 1))(*[self._aligned(w) for w in self._ws_seg[:nk - 1]])
+            grids = (ctypes.c_void_p * nk)(*[self._aligned(w) for w in self._ws_seg[:nk]])
             L.check(lib.mpm_adjoint_segment(self._prm, states, self._aligned(order), grids, self._cols, self._ncol,
                                             self._aligned(self._ws_adj), c0, self.dt, nk, adj.data_ptr(),
                                             nxt.data_ptr(), self.n, pb.data_ptr(), stream), "mpm_adjoint_segment")

@@ -755,26 +755,35 @@ int mpm_adjoint_segment(const MpmParams* prm, const MpmState* states, const void
     for (int j = 0; j < n_sub; ++j)
         if (!states[j].planes || states[j].stride != stride || states[j].n != n)
             return set_err(-32, "segment: states must share n and stride");
-    const int64_t ws_bytes = mpm_workspace_bytes(prm, n);
-    if (ws_bytes < 0) return set_err(-2, "n_grid out of range");
+    if (check_grid(prm)) return set_err(-2, "n_grid out of range");
+    const int64_t nn = (int64_t)prm->n_grid * prm->n_grid * prm->n_grid;
+    for (int j = 0; j < n_sub; ++j)
+        if (!step_grids[j] || ((uintptr_t)step_grids[j] & 255))
+            return set_err(-11, "segment: step workspaces NULL or not 256-byte aligned");
     hipStream_t s = (hipStream_t)stream;
     const float fdt = (float)dt;
     int rc;
-    // replay: S_{k0+j+1} = step(S_{k0+j}), each substep's grids kept
-    for (int j = 0; j + 1 < n_sub; ++j) {
-        if (hipMemcpyAsync(states[j + 1].planes, states[j].planes, (size_t)MPM_NPLANES * stride * sizeof(float),
-                           hipMemcpyDeviceToDevice, s) != hipSuccess)
-            return set_err(-33, "segment: state copy failed");
-        if ((rc = mpm_step_ordered(prm, &states[j + 1], order, colliders, n_colliders, step_grids[j], ws_bytes,
-                                   (float)((double)(k0 + j) * dt), fdt, stream)))
+    // replay: the P2G of S_k0, then per substep its grid update and the fused
+    // G2P + next P2G (S_{k0+j} -> S_{k0+j+1}) into the next substep's workspace
+    float* acc0 = (float*)step_grids[0];
+    if ((rc = mpm_zero_grid(prm, acc0, stream))) return rc;
+    if ((rc = p2g_ordered(prm, &states[0], order, acc0, fdt, stream))) return rc;
+    for (int j = 0; j < n_sub; ++j) {
+        float* acc = (float*)step_grids[j];
+        float* gvj = acc + 4 * nn;
+        if ((rc = mpm_grid_update(prm, acc, gvj, colliders, n_colliders, (float)((double)(k0 + j) * dt), fdt, stream)))
             return rc;
+        if (j + 1 < n_sub) {
+            float* acc_next = (float*)step_grids[j + 1];
+            if ((rc = mpm_zero_grid(prm, acc_next, stream))) return rc;
+            if ((rc = g2p2g_ordered(prm, &states[j], &states[j + 1], order, gvj, acc_next, fdt, stream))) return rc;
+        }
     }
-    // adjoint substeps in reverse; the last one recomputes its grids
+    // adjoint substeps in reverse, each on the grids its replay left
     float* cur = adj_a;
     float* nxt = adj_b;
     for (int j = n_sub - 1; j >= 0; --j) {
-        const void* grids = j + 1 < n_sub ? step_grids[j] : nullptr;
-        if ((rc = mpm_adjoint_substep(prm, &states[j], order, grids, colliders, n_colliders, workspace,
+        if ((rc = mpm_adjoint_substep(prm, &states[j], order, step_grids[j], colliders, n_colliders, workspace,
                                       (float)((double)(k0 + j) * dt), fdt, cur, nxt, adj_stride, param_adj, stream)))
             return rc;
         float* t = cur; cur = nxt; nxt = t;

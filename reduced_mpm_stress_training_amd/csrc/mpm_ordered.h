@@ -38,7 +38,7 @@ struct OrderLayout {
     int64_t off_cursor;   // int[nkeys]: scatter cursors
     int64_t off_perm;     // int[n]: sorted slot -> source slot
     int64_t off_chunks;   // int4[max_chunks]: (first slot, count, bin, 0)
-    int64_t off_meta;     // int[64]: [0] = number of chunks
+    int64_t off_meta;     // int[64]: [0] = number of chunks; [32..47] status counters
     int64_t bytes;
 };
 
@@ -60,6 +60,10 @@ inline OrderLayout order_layout(const MpmParams& prm, int64_t n) {
     L.bytes = o;
     return L;
 }
+
+// status counters of the fused substep in the order buffer's meta block
+// (int index from off_meta; +2: particles that left the grid)
+constexpr int kOrderStatus = 32;
 
 // sort key of a stencil base: bin (8^3 cells, C order) then sub-block (4^3)
 __device__ __forceinline__ int order_key(const int base[3], int ng, int nb) {
@@ -116,5 +120,11 @@ __device__ __forceinline__ void chunk_flush(const double* s_t, const int o[3], i
 // zeroed float4 accumulator grid (mpm_ordered.hip)
 int p2g_ordered(const MpmParams* prm, const MpmState* st, const void* order, float* grid_acc, float dt,
                 void* stream);
+
+// host: the fused G2P (grid velocities grid_v) + next P2G (into the zeroed
+// acc_next) over the chunks; dst may be src (in place) or another state of
+// the same size, which then receives the whole new state (mpm_ordered.hip)
+int g2p2g_ordered(const MpmParams* prm, const MpmState* src, const MpmState* dst, const void* order,
+                  const float* grid_v, float* acc_next, float dt, void* stream);
 
 }  // namespace mpm

@@ -121,6 +121,7 @@ __global__ void __launch_bounds__(kScan)
         hist[8 * nbins] = s_p[kScan - 1];
         meta[0] = min(s_c[kScan - 1], max_chunks);
     }
+    if (tid < 63) meta[1 + tid] = 0;   // status counters of the substeps on this order
 }
 
 __global__ void __launch_bounds__(kB)
@@ -213,6 +214,184 @@ __global__ void __launch_bounds__(kChunkWG)
     chunk_flush<4>(s_t, o, ng, grid);
 }
 
+// ---------------------------------------------------------------------------
+// Fused G2P (this substep, g2p_apic_and_stress, mpm_utils.py:241-306) and
+// P2G (the next substep, p2g_apic_stress_given, :190-227) over chunks.  The
+// chunk's grid velocities are staged in an LDS tile at its smallest stencil
+// base o; each particle gathers, runs F update / return map / stress in
+// registers, writes its new state (u = x - x0, v, C, F, tau; in place, or
+// the whole state into `dst`), and
+// scatters its next-substep P2G into an FP64 LDS tile at o - 1 (particles
+// move less than a cell per substep), flushed once per node into acc_next.
+// Stencils outside the tiles use the global grids.  Frozen particles (stencil
+// outside the grid) are left unchanged and do not scatter, as in the stage
+// kernels.
+// ---------------------------------------------------------------------------
+struct OrdNodesTile {
+    const float* s_v;
+    int ti;
+    __device__ __forceinline__ float comp(int c, int t) const {
+        const int i = t / 9, j = (t / 3) % 3, k = t % 3;
+        return s_v[c * kCT3 + ti + (i * kCT + j) * kCT + k];
+    }
+};
+struct OrdNodesGrid {
+    const float4* gv;
+    int64_t gb;
+    int ng;
+    __device__ __forceinline__ float comp(int c, int t) const {
+        const int i = t / 9, j = (t / 3) % 3, k = t % 3;
+        const float4 g = gv[gb + ((int64_t)i * ng + j) * ng + k];
+        return c == 0 ? g.x : (c == 1 ? g.y : g.z);
+    }
+};
+
+template <typename Nodes>
+__device__ __forceinline__ void g2p2g_flat_particle(const float* src, float* dst, int64_t S, int p,
+                                                    const float x0[3], const float x[3], const AxisWeights& A,
+                                                    const Nodes& V, const MpmParams& prm, float dt, double* s_acc,
+                                                    const int o2[3], float* __restrict__ acc_next,
+                                                    int* __restrict__ status) {
+    const int ng = prm.n_grid;
+    const float* sp = src + p;
+    float* dp = dst + p;
+    float v[3], Cm[9], G[9], F[9], tau[9];
+    gather_tile<9, 3>(V, 0, A, prm.inv_dx, v, Cm, G);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) F[k] = sp[(MPM_P_F + k) * S];
+    const float m = sp[MPM_P_MASS * S], vol = sp[MPM_P_VOL * S];
+    if (dst != src) {   // the planes the substep does not change
+#pragma unroll
+        for (int d = 0; d < 3; ++d) dp[(MPM_P_X0 + d) * S] = x0[d];
+        dp[MPM_P_MASS * S] = m;
+        dp[MPM_P_VOL * S] = vol;
+        dp[MPM_P_PID * S] = sp[MPM_P_PID * S];
+    }
+    sand_update(G, dt, F, prm.mu, prm.lam, prm.alpha, tau);
+    float xn[3], u[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        u[d] = (x[d] + dt * v[d]) - x0[d];     // update_u_from_x (mpm_utils.py:12-15)
+        xn[d] = x0[d] + u[d];                  // the next substep's update_x_from_u
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        dp[(MPM_P_U + d) * S] = u[d];
+        dp[(MPM_P_V + d) * S] = v[d];
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        dp[(MPM_P_C + k) * S] = Cm[k];
+        dp[(MPM_P_F + k) * S] = F[k];
+        dp[(MPM_P_TAU + k) * S] = tau[k];
+    }
+    AxisWeights B;
+    axis_weights(xn, prm.inv_dx, B);
+    if (!base_ok(B.base, ng)) {
+        atomicAdd(&status[2], 1);   // left the grid: frozen from here on
+        return;
+    }
+    const int t2 = chunk_tile_index(B.base, o2);
+    const int64_t gb = ((int64_t)B.base[0] * ng + B.base[1]) * ng + B.base[2];
+    p2g_contrib(B, m, vol, v, Cm, tau, prm.dx, prm.inv_dx, dt, [&](int cc, int i3, int j3, int k3, float val) {
+        if (t2 >= 0) atomicAdd(&s_acc[cc * kCT3 + t2 + (i3 * kCT + j3) * kCT + k3], (double)val);
+        else unsafeAtomicAdd(acc_next + 4 * (gb + ((int64_t)i3 * ng + j3) * ng + k3) + cc, val);
+    });
+}
+
+__global__ void __launch_bounds__(kChunkWG) __attribute__((amdgpu_waves_per_eu(2, 2)))
+    k_g2p2g_chunks(const float* planes, float* dst, int64_t S, int n, const int4* __restrict__ chunks,
+                   const int* __restrict__ meta, MpmParams prm, float dt, const float4* __restrict__ gv,
+                   float* __restrict__ acc_next, int* __restrict__ status) {
+    __shared__ float s_v[3 * kCT3];
+    __shared__ double s_acc[4 * kCT3];
+    __shared__ int s_lo[3];
+    const int c = blockIdx.x;
+    if (c >= meta[0]) return;
+    const int4 ch = chunks[c];
+    const int tid = threadIdx.x;
+    const int ng = prm.n_grid;
+    const int n_per = min(ch.y, kChunkMax);
+    if (tid < 3) s_lo[tid] = INT_MAX;
+    for (int t = tid; t < 4 * kCT3; t += kChunkWG) s_acc[t] = 0.0;
+    __syncthreads();
+    int lo[3] = {INT_MAX, INT_MAX, INT_MAX};
+    for (int i = tid; i < n_per; i += kChunkWG) {
+        const int p = ch.x + i;
+        if (p < 0 || p >= n) continue;
+        int b[3];
+        stencil_base(planes + p, S, prm.inv_dx, b);
+        if (!base_ok(b, ng)) continue;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) lo[d] = min(lo[d], b[d]);
+    }
+    chunk_min_base(s_lo, lo[0], lo[1], lo[2]);
+    __syncthreads();
+    if (s_lo[0] == INT_MAX) {   // every particle frozen (uniform): unchanged
+        if (dst != planes)
+            for (int i = tid; i < n_per; i += kChunkWG) {
+                const int p = ch.x + i;
+                if (p < 0 || p >= n) continue;
+                for (int k = 0; k < MPM_NPLANES; ++k) dst[(int64_t)k * S + p] = planes[(int64_t)k * S + p];
+            }
+        return;
+    }
+    const int o[3] = {s_lo[0], s_lo[1], s_lo[2]};
+    const int o2[3] = {o[0] - 1, o[1] - 1, o[2] - 1};
+    for (int t = tid; t < kCT3; t += kChunkWG) {
+        const int x = o[0] + t / (kCT * kCT), y = o[1] + (t / kCT) % kCT, z = o[2] + t % kCT;
+        float4 g = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (x < ng && y < ng && z < ng) g = gv[((int64_t)x * ng + y) * ng + z];
+        s_v[t] = g.x;
+        s_v[kCT3 + t] = g.y;
+        s_v[2 * kCT3 + t] = g.z;
+    }
+    __syncthreads();
+    // stencils inside the velocity tile first; the others (bit r of the
+    // thread's r-th particle) gather from the global grid afterwards
+    unsigned outside = 0;
+    for (int i = tid, r = 0; i < n_per; i += kChunkWG, ++r) {
+        const int p = ch.x + i;
+        if (p < 0 || p >= n) continue;
+        float x0[3], x[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            x0[d] = planes[(MPM_P_X0 + d) * S + p];
+            x[d] = x0[d] + planes[(MPM_P_U + d) * S + p];
+        }
+        AxisWeights A;
+        axis_weights(x, prm.inv_dx, A);
+        if (!base_ok(A.base, ng)) {   // frozen: unchanged
+            if (dst != planes)
+                for (int k = 0; k < MPM_NPLANES; ++k) dst[(int64_t)k * S + p] = planes[(int64_t)k * S + p];
+            continue;
+        }
+        const int ti = chunk_tile_index(A.base, o);
+        if (ti < 0) {
+            outside |= 1u << r;
+            continue;
+        }
+        g2p2g_flat_particle(planes, dst, S, p, x0, x, A, OrdNodesTile{s_v, ti}, prm, dt, s_acc, o2, acc_next,
+                            status);
+    }
+    for (int i = tid, r = 0; outside && i < n_per; i += kChunkWG, ++r) {
+        if (!(outside & (1u << r))) continue;
+        const int p = ch.x + i;
+        float x0[3], x[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            x0[d] = planes[(MPM_P_X0 + d) * S + p];
+            x[d] = x0[d] + planes[(MPM_P_U + d) * S + p];
+        }
+        AxisWeights A;
+        axis_weights(x, prm.inv_dx, A);
+        const OrdNodesGrid V{gv, ((int64_t)A.base[0] * ng + A.base[1]) * ng + A.base[2], ng};
+        g2p2g_flat_particle(planes, dst, S, p, x0, x, A, V, prm, dt, s_acc, o2, acc_next, status);
+    }
+    __syncthreads();
+    chunk_flush<4>(s_acc, o2, ng, acc_next);
+}
+
 __global__ void __launch_bounds__(kB)
     k_permute(const float* __restrict__ src, int64_t ss, float* __restrict__ dst, int64_t ds, int nplanes,
               const int* __restrict__ idx, int n, int scatter) {
@@ -251,6 +430,25 @@ int p2g_ordered(const MpmParams* prm, const MpmState* st, const void* order, flo
                        (const float*)st->planes, st->stride, (int)st->n, (const int4*)(ob + L.off_chunks),
                        (const int*)(ob + L.off_meta), *prm, dt, grid_acc);
     return check_launch("k_p2g_chunks");
+}
+
+int g2p2g_ordered(const MpmParams* prm, const MpmState* src, const MpmState* dst, const void* order,
+                  const float* grid_v, float* acc_next, float dt, void* stream) {
+    int rc = check_params(prm);
+    if (rc) return rc;
+    if ((rc = check_flat(src)) || (rc = check_flat(dst))) return rc;
+    if (src->n != dst->n || src->stride != dst->stride) return set_err(-41, "src and dst differ in n or stride");
+    if (!order || ((uintptr_t)order & 255)) return set_err(-40, "order buffer NULL or not 256-byte aligned");
+    if (!grid_v || !acc_next) return set_err(-7, "grid is NULL");
+    if (src->n == 0) return 0;
+    const OrderLayout L = order_layout(*prm, src->n);
+    char* ob = (char*)order;
+    int* status = (int*)(ob + L.off_meta) + kOrderStatus;
+    hipLaunchKernelGGL(k_g2p2g_chunks, dim3((unsigned)L.max_chunks), dim3(kChunkWG), 0, (hipStream_t)stream,
+                       (const float*)src->planes, (float*)dst->planes, src->stride, (int)src->n,
+                       (const int4*)(ob + L.off_chunks), (const int*)(ob + L.off_meta), *prm, dt,
+                       (const float4*)grid_v, acc_next, status);
+    return check_launch("k_g2p2g_chunks");
 }
 
 }  // namespace mpm
@@ -320,14 +518,36 @@ int mpm_step_ordered(const MpmParams* prm, const MpmState* st, const void* order
     return mpm_g2p(prm, st, gv, dt, stream);
 }
 
+int64_t mpm_run_ordered_workspace_bytes(const MpmParams* prm) {
+    if (check_grid(prm)) return -1;
+    const int64_t nn = (int64_t)prm->n_grid * prm->n_grid * prm->n_grid;
+    return 3 * nn * 16;   // two P2G accumulators (this substep, the next) and the grid velocities
+}
+
 int mpm_run_ordered(const MpmParams* prm, const MpmState* st, const void* order, const MpmCollider* colliders,
                     int n_colliders, void* workspace, int64_t workspace_bytes, int k0, double dt, int n_sub,
                     void* stream) {
+    int rc = check_params(prm);
+    if (rc) return rc;
+    if ((rc = check_flat(st))) return rc;
     if (n_sub < 0) return set_err(-31, "n_sub < 0");
+    if (!workspace || workspace_bytes < mpm_run_ordered_workspace_bytes(prm))
+        return set_err(-10, "workspace too small (mpm_run_ordered_workspace_bytes)");
+    if (((uintptr_t)workspace) & 255) return set_err(-11, "workspace not 256-byte aligned");
+    if (n_sub == 0) return 0;
+    const int64_t nn = (int64_t)prm->n_grid * prm->n_grid * prm->n_grid;
+    float* acc = (float*)workspace;
+    float* gv = acc + 4 * nn;
+    float* acc2 = gv + 4 * nn;
+    if ((rc = mpm_zero_grid(prm, acc, stream))) return rc;
+    if ((rc = p2g_ordered(prm, st, order, acc, (float)dt, stream))) return rc;
     for (int j = 0; j < n_sub; ++j) {
-        const int rc = mpm_step_ordered(prm, st, order, colliders, n_colliders, workspace, workspace_bytes,
-                                        (float)((double)(k0 + j) * dt), (float)dt, stream);
-        if (rc) return rc;
+        if ((rc = mpm_grid_update(prm, acc, gv, colliders, n_colliders, (float)((double)(k0 + j) * dt), (float)dt,
+                                  stream)))
+            return rc;
+        if ((rc = mpm_zero_grid(prm, acc2, stream))) return rc;
+        if ((rc = g2p2g_ordered(prm, st, st, order, gv, acc2, (float)dt, stream))) return rc;
+        float* t = acc; acc = acc2; acc2 = t;
     }
     return 0;
 }

@@ -33,14 +33,21 @@ def _planes(ro, sc):
     return planes
 
 
-def _run_ordered(sc, n_steps, resort_every):
+def _run_ordered(sc, n_steps, resort_every, fused=False):
     ro = _rollout(sc)
     planes = _planes(ro, sc)
     order = None
-    for k in range(n_steps):
-        if k % resort_every == 0:
-            planes, order = ro._sort(planes)
-        ro._step(planes, k, order)
+    lib = L.load()
+    for c0 in range(0, n_steps, resort_every):
+        planes, order = ro._sort(planes)
+        nk = min(resort_every, n_steps - c0)
+        if fused:     # mpm_run_ordered: fused G2P + next P2G per substep
+            L.check(lib.mpm_run_ordered(ro._prm, ro._state(planes), ro._aligned(order), ro._cols, ro._ncol,
+                                        ro._aligned(ro._ws_run), ro._ws_run_bytes, c0, ro.dt, nk,
+                                        torch.cuda.current_stream().cuda_stream), "mpm_run_ordered")
+        else:         # mpm_step_ordered: the stage kernels with the chunked P2G
+            for k in range(c0, c0 + nk):
+                ro._step(planes, k, order)
     back = ro._permute(planes, L.NPLANES, planes[L.P_PID].data_ptr(), scatter=True)
     torch.cuda.synchronize()
     b = back.cpu().numpy()
@@ -106,12 +113,13 @@ def test_permute_gather_scatter_round_trip():
     assert torch.equal(out[:, 20:], src[:, idx[20:].long()])
 
 
-@pytest.mark.parametrize("n_steps", [10, 100])
-def test_ordered_column_32_parity(oracle, n_steps):
-    """The 32^3 sand column on the ordered substep, re-sorted every 20
-    substeps, against the FP64 oracle."""
+@pytest.mark.parametrize("n_steps,fused", [(10, False), (100, False), (10, True), (100, True)])
+def test_ordered_column_32_parity(oracle, n_steps, fused):
+    """The 32^3 sand column on the ordered substep (stage kernels with the
+    chunked P2G, and the fused chunk kernel of mpm_run_ordered), re-sorted
+    every 20 substeps, against the FP64 oracle."""
     sc = S.column_lattice()
-    g, n = _run_ordered(sc, n_steps, resort_every=20)
+    g, n = _run_ordered(sc, n_steps, resort_every=20, fused=fused)
     o32 = _run_oracle(oracle, sc, n_steps, np.float32)
     o32r = _run_oracle(oracle, sc, n_steps, np.float32, order=np.arange(n)[::-1].copy())
     o64 = _run_oracle(oracle, sc, n_steps, np.float64)
