@@ -279,7 +279,8 @@ int mpm_adjoint_segment(const MpmParams* prm, const MpmState* states, const void
  * per chunk of particles in LDS instead of 27 global float atomics per
  * particle and component.  mpm_sort_by_bin counting-sorts a flat state by
  * the 8^3-cell bin and 4^3-cell sub-block of each particle's stencil base
- * (mpm_utils.py:197-199) into `dst` and fills `order` (mpm_order_bytes
+ * (mpm_utils.py:197-199), then puts each chunk in LDS bank order, into `dst`
+ * and fills `order` (mpm_order_bytes
  * bytes, 256-byte aligned): the chunk table (runs of at most 512 particles,
  * 1024 from 500k particles up, inside one bin) and the permutation
  * perm[sorted slot] = source slot (int32, at byte offset

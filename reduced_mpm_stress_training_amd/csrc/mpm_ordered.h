@@ -18,6 +18,7 @@
 #include <limits.h>
 
 #include "mpm_hip.h"
+#include "mpm_stencil.h"
 
 namespace mpm {
 
@@ -98,6 +99,61 @@ __device__ __forceinline__ int chunk_tile_index(const int b[3], const int o[3]) 
     const int x = b[0] - o[0], y = b[1] - o[1], z = b[2] - o[2];
     if (x < 0 || y < 0 || z < 0 || x > kCT - 3 || y > kCT - 3 || z > kCT - 3) return -1;
     return (x * kCT + y) * kCT + z;
+}
+
+// Processing order of a chunk's particles (chunk slots first .. first+n_per
+// of `planes`): s_ord[q] = chunk index of the q-th particle to process, in
+// bank order (interleave_mul) of the LDS bank class of each stencil base in
+// the tile at origin o, so that the 32 lanes of a half-wave gather and
+// scatter through distinct banks.  Particles outside the tile or the grid
+// take class 0.  Scratch: s_cnt[64], s_mul; barriers inside (call from every
+// thread of the workgroup).
+__device__ __forceinline__ void chunk_bank_order(const float* __restrict__ planes, int64_t S, int first, int n_per,
+                                                 int n, float inv_dx, const int o[3], unsigned short* s_ord,
+                                                 int* s_cnt, int* s_mul) {
+    const int tid = threadIdx.x;
+    if (tid < 32) s_cnt[tid] = 0;
+    __syncthreads();
+    int cls[kChunkMax / kChunkWG], rank[kChunkMax / kChunkWG];
+#pragma unroll
+    for (int r = 0; r < kChunkMax / kChunkWG; ++r) {
+        const int i = tid + r * kChunkWG;
+        cls[r] = rank[r] = 0;
+        if (i >= n_per) continue;
+        const int p = first + i;
+        int c = 0;
+        if (p >= 0 && p < n) {
+            int b[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const float x = planes[(MPM_P_X0 + d) * S + p] + planes[(MPM_P_U + d) * S + p];
+                b[d] = (int)(x * inv_dx - 0.5f);
+            }
+            const int t = chunk_tile_index(b, o);
+            c = t >= 0 ? (t & 31) : 0;
+        }
+        cls[r] = c;
+        rank[r] = atomicAdd(&s_cnt[c], 1);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int acc = 0;
+        for (int c = 0; c < 32; ++c) {
+            s_cnt[32 + c] = acc;
+            acc += s_cnt[c];
+        }
+        *s_mul = interleave_mul(n_per);
+    }
+    __syncthreads();
+    const int mul = *s_mul;
+#pragma unroll
+    for (int r = 0; r < kChunkMax / kChunkWG; ++r) {
+        const int i = tid + r * kChunkWG;
+        if (i >= n_per) continue;
+        const int k = s_cnt[32 + cls[r]] + rank[r];
+        s_ord[n_per > 1 ? (int)(((long long)k * mul) % n_per) : 0] = (unsigned short)i;
+    }
+    __syncthreads();
 }
 
 // Adds NC components of the LDS tile (SoA, kCT3 doubles each) at origin o to

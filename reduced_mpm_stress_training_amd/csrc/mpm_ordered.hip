@@ -147,6 +147,51 @@ __global__ void __launch_bounds__(kB)
     perm[pos] = p;
 }
 
+// Bank order inside each chunk, in memory: one workgroup per chunk moves
+// the chunk's particles (all planes and the permutation) into bank order
+// (chunk_bank_order) of their stencil bases in the tile at the chunk's
+// smallest base, staging one plane at a time in LDS.  The chunk kernels then
+// read consecutive slots (coalesced) whose 32-lane halves gather and scatter
+// through distinct LDS banks; a shift of the chunk's tile origin in later
+// substeps shifts every class by the same amount, so the order keeps.
+__global__ void __launch_bounds__(kChunkWG)
+    k_order_interleave(float* __restrict__ planes, int64_t S, int n, const int4* __restrict__ chunks,
+                       const int* __restrict__ meta, float inv_dx, int ng, int* __restrict__ perm) {
+    __shared__ unsigned short s_ord[kChunkMax];
+    __shared__ float s_stage[kChunkMax];
+    __shared__ int s_cnt[64];
+    __shared__ int s_mul;
+    __shared__ int s_lo[3];
+    const int c = blockIdx.x;
+    if (c >= meta[0]) return;
+    const int4 ch = chunks[c];
+    const int tid = threadIdx.x;
+    const int n_per = min(ch.y, kChunkMax);
+    if (ch.x < 0 || ch.x + n_per > n || n_per < 2) return;
+    if (tid < 3) s_lo[tid] = INT_MAX;
+    __syncthreads();
+    int lo[3] = {INT_MAX, INT_MAX, INT_MAX};
+    for (int i = tid; i < n_per; i += kChunkWG) {
+        int b[3];
+        stencil_base(planes + ch.x + i, S, inv_dx, b);
+        if (!base_ok(b, ng)) continue;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) lo[d] = min(lo[d], b[d]);
+    }
+    chunk_min_base(s_lo, lo[0], lo[1], lo[2]);
+    __syncthreads();
+    const int o[3] = {s_lo[0] == INT_MAX ? 0 : s_lo[0], s_lo[1] == INT_MAX ? 0 : s_lo[1],
+                      s_lo[2] == INT_MAX ? 0 : s_lo[2]};
+    chunk_bank_order(planes, S, ch.x, n_per, n, inv_dx, o, s_ord, s_cnt, &s_mul);   // barriers inside
+    for (int k = 0; k <= MPM_NPLANES; ++k) {   // plane MPM_NPLANES: the permutation
+        float* pl = k < MPM_NPLANES ? planes + (int64_t)k * S + ch.x : (float*)(perm + ch.x);
+        for (int i = tid; i < n_per; i += kChunkWG) s_stage[i] = pl[i];
+        __syncthreads();
+        for (int q = tid; q < n_per; q += kChunkWG) pl[q] = s_stage[s_ord[q]];
+        __syncthreads();
+    }
+}
+
 // ---------------------------------------------------------------------------
 // p2g_apic_stress_given (mpm_utils.py:190-227) over chunks: one workgroup per
 // chunk, its particles kChunkWG at a time.
@@ -497,6 +542,10 @@ int mpm_sort_by_bin(const MpmParams* prm, const MpmState* src, const MpmState* d
                            (float*)dst->planes, dst->stride, (int)n, prm->inv_dx, prm->n_grid, (int)L.nb, cursor,
                            perm);
         if ((rc = check_launch("k_order_scatter"))) return rc;
+        hipLaunchKernelGGL(k_order_interleave, dim3((unsigned)L.max_chunks), dim3(kChunkWG), 0, s,
+                           (float*)dst->planes, dst->stride, (int)n, (const int4*)chunks, (const int*)meta,
+                           prm->inv_dx, prm->n_grid, perm);
+        if ((rc = check_launch("k_order_interleave"))) return rc;
     }
     return 0;
 }

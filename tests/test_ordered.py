@@ -79,13 +79,13 @@ def test_sort_is_a_bin_ordered_permutation():
     perm = order[off:off + 4 * sc.n_particles].view(torch.int32).long()
     assert torch.equal(torch.sort(perm).values, torch.arange(sc.n_particles, device=perm.device))
     assert torch.equal(dst.view(torch.int32), planes[:, perm].view(torch.int32))
-    # keys (bin of 8^3 cells, then sub-block of 4^3) never decrease
+    # bins of 8^3 cells never decrease (inside a bin: sub-blocks, then bank
+    # order inside each chunk)
     x = (dst[L.P_X0:L.P_X0 + 3] + dst[L.P_U:L.P_U + 3]).cpu().numpy()
     b = np.clip((x * np.float32(sc.n_grid) - np.float32(0.5)).astype(np.int64), 0, sc.n_grid - 3)
     nb = (sc.n_grid + 7) // 8
-    key = (((b[0] >> 3) * nb + (b[1] >> 3)) * nb + (b[2] >> 3)) * 8 + \
-        (((b[0] >> 2) & 1) * 2 + ((b[1] >> 2) & 1)) * 2 + ((b[2] >> 2) & 1)
-    assert np.all(np.diff(key) >= 0)
+    binid = ((b[0] >> 3) * nb + (b[1] >> 3)) * nb + (b[2] >> 3)
+    assert np.all(np.diff(binid) >= 0)
 
 
 def test_permute_gather_scatter_round_trip():
