@@ -63,8 +63,19 @@ __global__ void k_conv_bwd_in(const float* __restrict__ dout, const float* __res
     din[g] = s;
 }
 
-// one workgroup per weight (co, ci, j) and one per bias: reductions over
-// frames and output positions
+// Weight and bias gradients: reductions over frames and output positions.
+// Workgroup (w, s) sums slice s of the B * L_out products of weight w (or of
+// bias w - 9 ks), then adds its partial with one float atomic into the
+// zeroed gW / gb (k_zero_f32 first), so that the reduction spreads over
+// kConvSplit workgroups per weight instead of one.
+constexpr int kConvSplit = 64;
+
+__global__ void k_zero_f32(float* __restrict__ a, int na, float* __restrict__ b, int nb) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < na) a[i] = 0.f;
+    else if (i < na + nb) b[i - na] = 0.f;
+}
+
 __global__ void __launch_bounds__(kB)
     k_conv_bwd_w(const float* __restrict__ in, const float* __restrict__ dout, const float* __restrict__ out, int B,
                  int L_in, int ks, int st, int L_out, const float* __restrict__ qm, const float* __restrict__ qis,
@@ -73,17 +84,19 @@ __global__ void __launch_bounds__(kB)
     const int w = blockIdx.x;
     const int n_w = 9 * ks;
     const int64_t total = (int64_t)B * L_out;
+    const int64_t per = (total + kConvSplit - 1) / kConvSplit;
+    const int64_t e0 = (int64_t)blockIdx.y * per, e1 = min(total, e0 + per);
     float s = 0.f;
     if (w < n_w) {
         const int co = w / (3 * ks), ci = (w / ks) % 3, j = w % ks;
-        for (int64_t e = threadIdx.x; e < total; e += kB) {
+        for (int64_t e = e0 + threadIdx.x; e < e1; e += kB) {
             const int b = (int)(e / L_out), t = (int)(e % L_out);
             const int64_t o = ((int64_t)b * 3 + co) * L_out + t;
             s += dout[o] * elu_grad_of_act(out[o]) * conv_in(in, b, ci, t * st + j, L_in, qm, qis);
         }
     } else {
         const int co = w - n_w;
-        for (int64_t e = threadIdx.x; e < total; e += kB) {
+        for (int64_t e = e0 + threadIdx.x; e < e1; e += kB) {
             const int b = (int)(e / L_out), t = (int)(e % L_out);
             const int64_t o = ((int64_t)b * 3 + co) * L_out + t;
             s += dout[o] * elu_grad_of_act(out[o]);
@@ -95,9 +108,9 @@ __global__ void __launch_bounds__(kB)
         if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        if (w < n_w) gW[w] = red[0];
-        else gb[w - n_w] = red[0];
+    if (threadIdx.x == 0 && e0 < e1) {
+        if (w < n_w) atomicAdd(&gW[w], red[0]);
+        else atomicAdd(&gb[w - n_w], red[0]);
     }
 }
 
@@ -138,20 +151,44 @@ __global__ void k_dense_bwd_w(const float* __restrict__ dy, const float* __restr
     else gb[o] = s;
 }
 
-// g[f][c] = sum over the P points of frame f of dx[p][c], c < lbl
+// g[f][c] = sum over the P points of frame f of dx[p][c], c < lbl.
+// Workgroup (f, s) reads slice s of the frame's rows (each thread whole rows,
+// coalesced across the workgroup), keeps the lbl column sums in registers,
+// reduces them through LDS and adds them to the zeroed g with float atomics.
+constexpr int kLatSplit = 64;
+constexpr int kLatMax = 32;   // lbl <= in_dim <= 32
+
 __global__ void __launch_bounds__(kB) k_latent_grad(const float* __restrict__ dx, int P, int in_dim, int lbl,
                                                     float* __restrict__ g) {
-    __shared__ float red[kB];
-    const int f = blockIdx.x / lbl, c = blockIdx.x % lbl;
-    float s = 0.f;
-    for (int p = threadIdx.x; p < P; p += kB) s += dx[((int64_t)f * P + p) * in_dim + c];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int o = kB / 2; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-        __syncthreads();
+    __shared__ float red[kLatMax][kB / 64];
+    const int f = blockIdx.x;
+    const int per = (P + kLatSplit - 1) / kLatSplit;
+    const int p0 = blockIdx.y * per, p1 = min(P, p0 + per);
+    float s[kLatMax];
+#pragma unroll
+    for (int c = 0; c < kLatMax; ++c) s[c] = 0.f;
+    for (int p = p0 + threadIdx.x; p < p1; p += kB) {
+        const float* row = dx + ((int64_t)f * P + p) * in_dim;
+#pragma unroll
+        for (int c = 0; c < kLatMax; ++c)
+            if (c < lbl) s[c] += row[c];
     }
-    if (threadIdx.x == 0) g[blockIdx.x] = red[0];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int c = 0; c < kLatMax; ++c) {
+        if (c >= lbl) break;
+        float v = s[c];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+        if (lane == 0) red[c][wave] = v;
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < lbl && p0 < p1) {
+        float v = 0.f;
+#pragma unroll
+        for (int w = 0; w < kB / 64; ++w) v += red[threadIdx.x][w];
+        atomicAdd(&g[f * lbl + threadIdx.x], v);
+    }
 }
 
 unsigned nblk(int64_t n) { return (unsigned)((n + kB - 1) / kB); }
@@ -182,8 +219,10 @@ int crom_conv_backward(const float* in, const float* out, const float* dout, int
                            out, B, L_in, ks, stride, W, L_out, din);
         if ((rc = check_launch("k_conv_bwd_in"))) return rc;
     }
-    hipLaunchKernelGGL(k_conv_bwd_w, dim3(9 * ks + 3), dim3(kB), 0, (hipStream_t)stream, in, dout, out, B, L_in, ks,
-                       stride, L_out, q_mean, q_std_inv, gW, gb);
+    hipLaunchKernelGGL(k_zero_f32, dim3(nblk(9 * ks + 3)), dim3(kB), 0, (hipStream_t)stream, gW, 9 * ks, gb, 3);
+    if ((rc = check_launch("k_zero_f32"))) return rc;
+    hipLaunchKernelGGL(k_conv_bwd_w, dim3(9 * ks + 3, kConvSplit), dim3(kB), 0, (hipStream_t)stream, in, dout, out, B,
+                       L_in, ks, stride, L_out, q_mean, q_std_inv, gW, gb);
     return check_launch("k_conv_bwd_w");
 }
 
@@ -212,10 +251,14 @@ int crom_dense_backward(const float* x, const float* y, const float* dy, int B, 
 
 int crom_latent_grad(const float* dx, int n_frames, int points_per_frame, int in_dim, int lbl, float* g,
                      void* stream) {
-    if (!dx || !g || n_frames < 1 || points_per_frame < 1 || lbl < 1 || lbl > in_dim)
+    if (!dx || !g || n_frames < 1 || points_per_frame < 1 || lbl < 1 || lbl > in_dim || lbl > kLatMax)
         return set_err(-82, "crom latent grad: bad arguments");
-    hipLaunchKernelGGL(k_latent_grad, dim3(n_frames * lbl), dim3(kB), 0, (hipStream_t)stream, dx, points_per_frame,
-                       in_dim, lbl, g);
+    hipLaunchKernelGGL(k_zero_f32, dim3(nblk((int64_t)n_frames * lbl)), dim3(kB), 0, (hipStream_t)stream, g,
+                       n_frames * lbl, g, 0);
+    int rc = check_launch("k_zero_f32");
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_latent_grad, dim3(n_frames, kLatSplit), dim3(kB), 0, (hipStream_t)stream, dx,
+                       points_per_frame, in_dim, lbl, g);
     return check_launch("k_latent_grad");
 }
 
