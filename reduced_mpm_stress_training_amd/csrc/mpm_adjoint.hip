@@ -42,52 +42,6 @@ constexpr int kBlk = 256;
 // adjoint planes: x (3), v (3), C (9), F (9), tau (9)
 constexpr int kAX = 0, kAV = 3, kAC = 6, kAF = 15, kAT = 24;
 
-struct W3 {             // per-axis quadratic B-spline factors at fx
-    int base[3];
-    float fx[3];
-    float w[3][3], d[3][3], dd[3][3];   // weight, d/dfx, d2/dfx2
-};
-
-__device__ __forceinline__ void weights3(const float x[3], float inv_dx, W3& W) {
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        const float gp = x[a] * inv_dx;
-        const int b = (int)(gp - 0.5f);
-        W.base[a] = b;
-        const float fx = gp - (float)b;
-        W.fx[a] = fx;
-        const float wa = 1.5f - fx, wb = fx - 1.0f, wc = fx - 0.5f;
-        W.w[a][0] = 0.5f * wa * wa;
-        W.w[a][1] = 0.75f - wb * wb;
-        W.w[a][2] = 0.5f * wc * wc;
-        W.d[a][0] = fx - 1.5f;
-        W.d[a][1] = -2.0f * (fx - 1.0f);
-        W.d[a][2] = fx - 0.5f;
-        W.dd[a][0] = 1.0f;
-        W.dd[a][1] = -2.0f;
-        W.dd[a][2] = 1.0f;
-    }
-}
-
-// weight, gradient (d/dx_p) and Hessian of node (i, j, k)
-__device__ __forceinline__ void node_w(const W3& W, int i, int j, int k, float inv_dx, float& w, float g[3],
-                                       float H[9]) {
-    const float wx = W.w[0][i], wy = W.w[1][j], wz = W.w[2][k];
-    const float dx = W.d[0][i], dy = W.d[1][j], dz = W.d[2][k];
-    const float ddx = W.dd[0][i], ddy = W.dd[1][j], ddz = W.dd[2][k];
-    w = wx * wy * wz;
-    g[0] = inv_dx * dx * wy * wz;
-    g[1] = inv_dx * wx * dy * wz;
-    g[2] = inv_dx * wx * wy * dz;
-    const float i2 = inv_dx * inv_dx;
-    H[0] = i2 * ddx * wy * wz;
-    H[4] = i2 * wx * ddy * wz;
-    H[8] = i2 * wx * wy * ddz;
-    H[1] = H[3] = i2 * dx * dy * wz;
-    H[2] = H[6] = i2 * dx * wy * dz;
-    H[5] = H[7] = i2 * wx * dy * dz;
-}
-
 // (b - a) / (exp(b) - exp(a)) style divided differences, robust when the
 // arguments meet: (ln s_l - ln s_k) / (s_l - s_k)
 __device__ __forceinline__ float dlog_div(float sk, float sl) {
@@ -605,75 +559,190 @@ __global__ void __launch_bounds__(kBlk)
 
 // ---------------------------------------------------------------------------
 // P2G adjoint: v, C, tau adjoints of S_n and the position adjoint's P2G part
-// (added to what k_g2p_adj wrote).
+// (added to what k_g2p_adj_chunks wrote).  For each grid-adjoint field f
+// (mv_b x, y, z and m_b) the kernel needs 22 weighted sums over the 27
+// nodes, every weight a product of one factor per axis from w, e = w (n - fx),
+// d = dw, p = dw (n - fx), dd = d2w (grid units):
+//   sum w f (v, C via e; the C^T term of x), sum grad w f (tau, and the v
+//   and m_b terms of x), sum grad_e w (x_d - x_p) f (the C term of x), and
+//   sum Hessian_de w f (the tau term of x);
+// they are contracted z, then y, then x (tensor-product form) instead of
+// node by node.
 // ---------------------------------------------------------------------------
+struct AxisF5 {
+    float w[3], e[3], d[3], p[3], dd[3];
+};
+
+__device__ __forceinline__ AxisF5 axis_f5(float fx) {
+    AxisF5 f;
+    const float wa = 1.5f - fx, wb = fx - 1.0f, wc = fx - 0.5f;
+    f.w[0] = 0.5f * wa * wa;
+    f.w[1] = 0.75f - wb * wb;
+    f.w[2] = 0.5f * wc * wc;
+    f.d[0] = fx - 1.5f;
+    f.d[1] = -2.0f * (fx - 1.0f);
+    f.d[2] = fx - 0.5f;
+#pragma unroll
+    for (int n = 0; n < 3; ++n) {
+        const float off = (float)n - fx;
+        f.e[n] = f.w[n] * off;
+        f.p[n] = f.d[n] * off;
+        f.dd[n] = n == 1 ? -2.0f : 1.0f;
+    }
+    return f;
+}
+
+// indices of the 22 sums: (x, y, z) factors
+enum { kS0, kEx, kEy, kEz, kDx, kDy, kDz, kMxy, kMyx, kMxz, kMzx, kMyz, kMzy, kPx, kPy, kPz, kHxy, kHxz, kHyz,
+       kDDx, kDDy, kDDz, kNSum };
+
+// The 22 sums of the three momentum-adjoint fields and the three of the mass
+// adjoint in one pass over the 27 nodes (one float4 load per node).
+__device__ __forceinline__ void p2g_adj_sums(const float4* __restrict__ gb, int64_t b0, int ng, const AxisF5& X,
+                                             const AxisF5& Y, const AxisF5& Z, float r[3][kNSum], float rm[3]) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int q = 0; q < kNSum; ++q) r[a][q] = 0.f;
+    rm[0] = rm[1] = rm[2] = 0.f;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        float y[3][12], ym[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int q = 0; q < 12; ++q) y[a][q] = 0.f;
+        ym[0] = ym[1] = ym[2] = 0.f;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            float zw[4] = {0.f, 0.f, 0.f, 0.f}, zd[4] = {0.f, 0.f, 0.f, 0.f};
+            float ze[3] = {0.f, 0.f, 0.f}, zp[3] = {0.f, 0.f, 0.f}, zdd[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const float4 g = gb[b0 + ((int64_t)i * ng + j) * ng + k];
+                const float f[4] = {g.x, g.y, g.z, g.w};
+#pragma unroll
+                for (int a = 0; a < 4; ++a) {
+                    zw[a] = fmaf(Z.w[k], f[a], zw[a]);
+                    zd[a] = fmaf(Z.d[k], f[a], zd[a]);
+                }
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    ze[a] = fmaf(Z.e[k], f[a], ze[a]);
+                    zp[a] = fmaf(Z.p[k], f[a], zp[a]);
+                    zdd[a] = fmaf(Z.dd[k], f[a], zdd[a]);
+                }
+            }
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                y[a][0] = fmaf(Y.w[j], zw[a], y[a][0]);     // (., w, w)
+                y[a][1] = fmaf(Y.e[j], zw[a], y[a][1]);     // (., e, w)
+                y[a][2] = fmaf(Y.w[j], ze[a], y[a][2]);     // (., w, e)
+                y[a][3] = fmaf(Y.d[j], zw[a], y[a][3]);     // (., d, w)
+                y[a][4] = fmaf(Y.w[j], zd[a], y[a][4]);     // (., w, d)
+                y[a][5] = fmaf(Y.d[j], ze[a], y[a][5]);     // (., d, e)
+                y[a][6] = fmaf(Y.e[j], zd[a], y[a][6]);     // (., e, d)
+                y[a][7] = fmaf(Y.p[j], zw[a], y[a][7]);     // (., p, w)
+                y[a][8] = fmaf(Y.w[j], zp[a], y[a][8]);     // (., w, p)
+                y[a][9] = fmaf(Y.d[j], zd[a], y[a][9]);     // (., d, d)
+                y[a][10] = fmaf(Y.dd[j], zw[a], y[a][10]);  // (., dd, w)
+                y[a][11] = fmaf(Y.w[j], zdd[a], y[a][11]);  // (., w, dd)
+            }
+            ym[0] = fmaf(Y.w[j], zw[3], ym[0]);
+            ym[1] = fmaf(Y.d[j], zw[3], ym[1]);
+            ym[2] = fmaf(Y.w[j], zd[3], ym[2]);
+        }
+        const float wx = X.w[i], ex = X.e[i], dx = X.d[i], px = X.p[i], ddx = X.dd[i];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            float* ra = r[a];
+            const float* ya = y[a];
+            ra[kS0] = fmaf(wx, ya[0], ra[kS0]);
+            ra[kEx] = fmaf(ex, ya[0], ra[kEx]);
+            ra[kEy] = fmaf(wx, ya[1], ra[kEy]);
+            ra[kEz] = fmaf(wx, ya[2], ra[kEz]);
+            ra[kDx] = fmaf(dx, ya[0], ra[kDx]);
+            ra[kDy] = fmaf(wx, ya[3], ra[kDy]);
+            ra[kDz] = fmaf(wx, ya[4], ra[kDz]);
+            ra[kMxy] = fmaf(dx, ya[1], ra[kMxy]);
+            ra[kMyx] = fmaf(ex, ya[3], ra[kMyx]);
+            ra[kMxz] = fmaf(dx, ya[2], ra[kMxz]);
+            ra[kMzx] = fmaf(ex, ya[4], ra[kMzx]);
+            ra[kMyz] = fmaf(wx, ya[5], ra[kMyz]);
+            ra[kMzy] = fmaf(wx, ya[6], ra[kMzy]);
+            ra[kPx] = fmaf(px, ya[0], ra[kPx]);
+            ra[kPy] = fmaf(wx, ya[7], ra[kPy]);
+            ra[kPz] = fmaf(wx, ya[8], ra[kPz]);
+            ra[kHxy] = fmaf(dx, ya[3], ra[kHxy]);
+            ra[kHxz] = fmaf(dx, ya[4], ra[kHxz]);
+            ra[kHyz] = fmaf(wx, ya[9], ra[kHyz]);
+            ra[kDDx] = fmaf(ddx, ya[0], ra[kDDx]);
+            ra[kDDy] = fmaf(wx, ya[10], ra[kDDy]);
+            ra[kDDz] = fmaf(wx, ya[11], ra[kDDz]);
+        }
+        rm[0] = fmaf(dx, ym[0], rm[0]);   // mass adjoint: D_x
+        rm[1] = fmaf(wx, ym[1], rm[1]);   // D_y
+        rm[2] = fmaf(wx, ym[2], rm[2]);   // D_z
+    }
+}
+
 __global__ void __launch_bounds__(kBlk)
     k_p2g_adj(const float* __restrict__ st, int64_t S, int n, MpmParams prm, float dt, const float4* __restrict__ gb,
               float* __restrict__ ac, int64_t SA) {
     const int p = blockIdx.x * kBlk + threadIdx.x;
     if (p >= n) return;
     const float* sp = st + p;
-    float x[3], v[3], C[9], tau[9];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        x[d] = sp[(MPM_P_X0 + d) * S] + sp[(MPM_P_U + d) * S];
-        v[d] = sp[(MPM_P_V + d) * S];
-    }
-    load9(sp, S, MPM_P_C, C);
-    load9(sp, S, MPM_P_TAU, tau);
-    const float m = sp[MPM_P_MASS * S], vol = sp[MPM_P_VOL * S];
     const int ng = prm.n_grid;
-    const float inv_dx = prm.inv_dx, dx = prm.dx;
-    W3 W;
-    weights3(x, inv_dx, W);
-    const bool ok = W.base[0] >= 0 && W.base[1] >= 0 && W.base[2] >= 0 && W.base[0] <= ng - 3 &&
-                    W.base[1] <= ng - 3 && W.base[2] <= ng - 3;
-    if (!ok) return;   // frozen: k_g2p_adj passed v, C, tau through
-    float vb[3] = {0, 0, 0}, Cb[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, Tb[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    float xg[3] = {0, 0, 0};
-    for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j)
-            for (int k = 0; k < 3; ++k) {
-                float w, g[3], H[9];
-                node_w(W, i, j, k, inv_dx, w, g, H);
-                const float4 b4 = gb[((int64_t)(W.base[0] + i) * ng + (W.base[1] + j)) * ng + (W.base[2] + k)];
-                const float mb[3] = {b4.x, b4.y, b4.z};
-                const float dpp[3] = {((float)i - W.fx[0]) * dx, ((float)j - W.fx[1]) * dx, ((float)k - W.fx[2]) * dx};
-                float q[3];   // v + C dpos
+    const float inv_dx = prm.inv_dx, dxp = prm.dx;
+    float x[3];
 #pragma unroll
-                for (int a = 0; a < 3; ++a) q[a] = v[a] + C[a * 3] * dpp[0] + C[a * 3 + 1] * dpp[1] + C[a * 3 + 2] * dpp[2];
-                float mq = 0.f, ct[3] = {0, 0, 0}, th[3] = {0, 0, 0};
-#pragma unroll
-                for (int a = 0; a < 3; ++a) {
-                    vb[a] += w * m * mb[a];
-                    mq += mb[a] * q[a];
-#pragma unroll
-                    for (int d = 0; d < 3; ++d) {
-                        Cb[a * 3 + d] += w * m * mb[a] * dpp[d];
-                        Tb[a * 3 + d] += -dt * vol * mb[a] * g[d];
-                        ct[d] += C[a * 3 + d] * mb[a];               // C^T mb
-                    }
-                }
-#pragma unroll
-                for (int a = 0; a < 3; ++a)
-#pragma unroll
-                    for (int d = 0; d < 3; ++d)
-#pragma unroll
-                        for (int e = 0; e < 3; ++e) th[e] += mb[a] * tau[a * 3 + d] * H[d * 3 + e];
-#pragma unroll
-                for (int e = 0; e < 3; ++e) xg[e] += g[e] * (m * mq + b4.w * m) - w * m * ct[e] - dt * vol * th[e];
-            }
-    float* cp = ac + p;
+    for (int d = 0; d < 3; ++d) x[d] = sp[(MPM_P_X0 + d) * S] + sp[(MPM_P_U + d) * S];
+    int base[3];
+    float fx[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-        cp[(kAV + d) * SA] = vb[d];
-        cp[(kAX + d) * SA] += xg[d];
+        const float gp = x[d] * inv_dx;
+        base[d] = (int)(gp - 0.5f);
+        fx[d] = gp - (float)base[d];
+    }
+    if (!(base[0] >= 0 && base[1] >= 0 && base[2] >= 0 && base[0] <= ng - 3 && base[1] <= ng - 3 &&
+          base[2] <= ng - 3))
+        return;   // frozen: k_g2p_adj_chunks passed v, C, tau through
+    const AxisF5 X = axis_f5(fx[0]), Y = axis_f5(fx[1]), Z = axis_f5(fx[2]);
+    const int64_t b0 = ((int64_t)base[0] * ng + base[1]) * ng + base[2];
+    float r[3][kNSum], rm[3];
+    p2g_adj_sums(gb, b0, ng, X, Y, Z, r, rm);
+    const float m = sp[MPM_P_MASS * S], vol = sp[MPM_P_VOL * S];
+    const float ct = dt * vol * inv_dx * inv_dx;
+    float* cp = ac + p;
+    float xg[3] = {m * inv_dx * rm[0], m * inv_dx * rm[1], m * inv_dx * rm[2]};   // the mass adjoint m_b
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float* ra = r[a];
+        const float va = sp[(MPM_P_V + a) * S];
+        float C[3], tau[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            C[d] = sp[(MPM_P_C + a * 3 + d) * S];
+            tau[d] = sp[(MPM_P_TAU + a * 3 + d) * S];
+        }
+        cp[(kAV + a) * SA] = m * ra[kS0];
+        cp[(kAC + a * 3 + 0) * SA] = m * dxp * ra[kEx];
+        cp[(kAC + a * 3 + 1) * SA] = m * dxp * ra[kEy];
+        cp[(kAC + a * 3 + 2) * SA] = m * dxp * ra[kEz];
+        cp[(kAT + a * 3 + 0) * SA] = -dt * vol * inv_dx * ra[kDx];
+        cp[(kAT + a * 3 + 1) * SA] = -dt * vol * inv_dx * ra[kDy];
+        cp[(kAT + a * 3 + 2) * SA] = -dt * vol * inv_dx * ra[kDz];
+        const float mv = m * va * inv_dx;
+        xg[0] += mv * ra[kDx] + m * (C[0] * ra[kPx] + C[1] * ra[kMxy] + C[2] * ra[kMxz] - C[0] * ra[kS0]) -
+                 ct * (tau[0] * ra[kDDx] + tau[1] * ra[kHxy] + tau[2] * ra[kHxz]);
+        xg[1] += mv * ra[kDy] + m * (C[0] * ra[kMyx] + C[1] * ra[kPy] + C[2] * ra[kMyz] - C[1] * ra[kS0]) -
+                 ct * (tau[0] * ra[kHxy] + tau[1] * ra[kDDy] + tau[2] * ra[kHyz]);
+        xg[2] += mv * ra[kDz] + m * (C[0] * ra[kMzx] + C[1] * ra[kMzy] + C[2] * ra[kPz] - C[2] * ra[kS0]) -
+                 ct * (tau[0] * ra[kHxz] + tau[1] * ra[kHyz] + tau[2] * ra[kDDz]);
     }
 #pragma unroll
-    for (int k = 0; k < 9; ++k) {
-        cp[(kAC + k) * SA] = Cb[k];
-        cp[(kAT + k) * SA] = Tb[k];
-    }
+    for (int d = 0; d < 3; ++d) cp[(kAX + d) * SA] += xg[d];
 }
 
 __global__ void k_zero4(float4* __restrict__ g, int64_t nn) {
