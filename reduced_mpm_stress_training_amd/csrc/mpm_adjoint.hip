@@ -137,14 +137,16 @@ __device__ void sand_update_adj(const float G[9], const float F[9], float dt, fl
     }
 #pragma unroll
     for (int k = 0; k < 3; ++k) Q[k * 4] += (2.0f * mu * Tp[k * 4] + lam * tsum) / se[k];
+    // off-diagonal terms: the divided differences are symmetric in (k, l),
+    // so each is evaluated once per unordered pair
 #pragma unroll
-    for (int k = 0; k < 3; ++k)
-#pragma unroll
-        for (int l = 0; l < 3; ++l) {
-            if (k == l) continue;
-            const float c = 2.0f * mu * dlog_div(se[k], se[l]) / (se[k] + se[l]);
-            Q[k * 3 + l] += c * se[l] * (Tp[k * 3 + l] + Tp[l * 3 + k]);
-        }
+    for (int pr = 0; pr < 3; ++pr) {
+        const int k = pr == 2 ? 1 : 0, l = pr == 0 ? 1 : 2;
+        const float c = 2.0f * mu * dlog_div(se[k], se[l]) / (se[k] + se[l]);
+        const float tt = Tp[k * 3 + l] + Tp[l * 3 + k];
+        Q[k * 3 + l] += c * se[l] * tt;
+        Q[l * 3 + k] += c * se[k] * tt;
+    }
     // return-map adjoint: P = principal-frame adjoint of F_tr
     float P[9];
     if (br == 1) {
@@ -191,15 +193,15 @@ __device__ void sand_update_adj(const float G[9], const float F[9], float dt, fl
         const float Bexp = -beta;
         const float eA = br == 3 ? expf((1.0f + beta) * tr / 3.0f) : 0.f;
 #pragma unroll
-        for (int k = 0; k < 3; ++k)
-#pragma unroll
-            for (int l = 0; l < 3; ++l) {
-                if (k == l) continue;
-                const float D = br == 3 ? eA * dpow_div(s[k], s[l], Bexp) : 0.f;
-                const float S = (se[l] + se[k]) / (s[l] + s[k]);
-                const float a = 0.5f * (D + S), b = 0.5f * (D - S);
-                P[k * 3 + l] = a * Q[k * 3 + l] + b * Q[l * 3 + k];
-            }
+        for (int pr = 0; pr < 3; ++pr) {   // symmetric in (k, l): once per unordered pair
+            const int k = pr == 2 ? 1 : 0, l = pr == 0 ? 1 : 2;
+            const float D = br == 3 ? eA * dpow_div(s[k], s[l], Bexp) : 0.f;
+            const float S = (se[l] + se[k]) / (s[l] + s[k]);
+            const float a = 0.5f * (D + S), b = 0.5f * (D - S);
+            const float qkl = Q[k * 3 + l], qlk = Q[l * 3 + k];
+            P[k * 3 + l] = a * qkl + b * qlk;
+            P[l * 3 + k] = a * qlk + b * qkl;
+        }
     }
     // Ftr adjoint = U P V^T
     float Ftb[9];
