@@ -285,8 +285,8 @@ def bench_crom(args):
     decode throughput, its time against the MPM substeps it stands for, and
     the relative L2 error of the reconstructed displacements.  Then the
     CROM-reduced run of the held-out angle: MPM on the N/10 sample only
-    (half-resolution grid) + encoder + decoder per frame, judged frame by
-    frame by the full MPM run (ms per frame of both, error)."""
+    (half-resolution grid) + encoder + decoder per frame, checked frame by
+    frame against the full MPM run (ms per frame of both, error)."""
     import torch
     from reduced_mpm_stress_training_amd import crom as CR
     from reduced_mpm_stress_training_amd import scenes as S
@@ -339,10 +339,10 @@ def bench_crom(args):
     torch.cuda.synchronize()
     ms_frame = (time.perf_counter() - t1) * 1e3 / 10
     # the CROM-reduced run of the held-out angle: MPM on the encoder's N/10
-    # sample (half-resolution grid), decoder for all N, the full run as judge
+    # sample (half-resolution grid), decoder for all N, checked against the full run
     rr = CR.ReducedRun(net, sc, angles[4], n_sample=P, coarse_grid=max(16, args.n_grid // 2))
     rr.run(2, spf, dt)                       # warm-up of the sampled run's buffers
-    red = rr.run(frames, spf, dt, judge=True)
+    red = rr.run(frames, spf, dt, verify=True)
     errs = red["errors"][1:]
     flop = 2 * (19 * 128 + 4 * 128 * 128 + 128 * 3)
     out = {

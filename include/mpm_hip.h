@@ -316,7 +316,7 @@ int mpm_permute(const float* src, int64_t src_stride, float* dst, int64_t dst_st
 
 /* ---- batches of independent scenes -----------------------------------
  *
- * No counterpart in the reference: its driver steps one scene per process
+ * No counterpart in the reference: its generator script steps one scene per process
  * (generate_training_data/generate_sand.py:17-32).  A batch runs many
  * scenes -- each with its own grid size, material and colliders -- through
  * the binned kernels above in the same launches: the scenes' bins are

@@ -407,7 +407,7 @@ def solver_snapshots(scene, friction_angle: float, n_frames: int, substeps_per_f
 
 
 # ---------------------------------------------------------------------------
-# CROM-reduced runs: a sampled MPM run + the decoder, judged by the full run
+# CROM-reduced runs: a sampled MPM run + the decoder, checked against the full run
 # ---------------------------------------------------------------------------
 def sampled_scene(scene, n_sample: int, n_grid: int):
     """The first n_sample particles of a (pre-shuffled) scene, each standing
@@ -429,13 +429,13 @@ class ReducedRun:
     encoder's sample of particles (on a coarser grid); every frame the
     encoder turns the sample's displacements into a latent and the decoder
     reconstructs the displacement of all N particles.  The full-order MPM
-    run is the judge of its error and the fallback: with `judge=True` it
+    run is the check of its error and the fallback: with `verify=True` it
     advances in lockstep, each frame's relative L2 error is recorded, and
     from the first frame whose error exceeds `tol` the frames are the full
     run's own.
 
         rr = ReducedRun(crom, scene, friction_angle, n_sample=N // 10, coarse_grid=64)
-        out = rr.run(n_frames, substeps_per_frame, dt, judge=True, tol=0.15)
+        out = rr.run(n_frames, substeps_per_frame, dt, verify=True, tol=0.15)
         out["frames"]  (n_frames, N, 3) displacements; out["source"] "crom" | "mpm" per frame
     """
 
@@ -463,10 +463,10 @@ class ReducedRun:
         xin = torch.cat([z.expand(n, self.crom.lbl), self.X], 1).contiguous()
         return self.crom.decoder(xin)
 
-    def run(self, n_frames: int, substeps_per_frame: int, dt: float, judge: bool = False,
+    def run(self, n_frames: int, substeps_per_frame: int, dt: float, verify: bool = False,
             tol: float | None = None) -> dict:
         coarse = self._sim(sampled_scene(self.scene, self.n_sample, self.coarse_grid))
-        full = self._sim(self.scene) if judge else None
+        full = self._sim(self.scene) if verify else None
         frames = torch.empty((n_frames, self.X.shape[0], 3), dtype=torch.float32, device=self.device)
         source, errors = [], []
         fallen = False
@@ -504,5 +504,5 @@ class ReducedRun:
             else:
                 frames[f] = q
                 source.append("crom")
-        return {"frames": frames, "source": source, "errors": errors if judge else None,
-                "reduced_seconds": t_reduced, "full_seconds": t_full if judge else None}
+        return {"frames": frames, "source": source, "errors": errors if verify else None,
+                "reduced_seconds": t_reduced, "full_seconds": t_full if verify else None}

@@ -313,7 +313,7 @@ def test_reduced_run_gains_over_the_untrained_network_and_falls_back():
     """CROMnet trained (GPU kernels) on the solver's own runs of a 100k-particle
     column at four friction angles; the CROM-reduced run of a held-out angle
     (MPM on the encoder's 10% sample on a half-resolution grid, decoder for
-    every particle) judged frame by frame by the full MPM run.  What training
+    every particle) checked frame by frame against the full MPM run.  What training
     gains: the trained decoder's error is below 0.5 and below a tenth of the
     same network's untrained error (measured: 0.35 against 6.2; most of the
     0.35 is the half-resolution sampled run differing from the full run, not
@@ -339,7 +339,7 @@ def test_reduced_run_gains_over_the_untrained_network_and_falls_back():
         net.train_step(train[idx], xb, 1e-3 if it < 300 else 2e-4)
     res = {}
     for name, m in (("trained", net), ("untrained", untrained)):
-        out = CR.ReducedRun(m, sc, 31.0, n_sample=P, coarse_grid=24).run(frames, spf, dt, judge=True)
+        out = CR.ReducedRun(m, sc, 31.0, n_sample=P, coarse_grid=24).run(frames, spf, dt, verify=True)
         assert out["source"] == ["crom"] * frames
         res[name] = float(np.mean(out["errors"][1:]))
         res[name + "_ms"] = (out["reduced_seconds"] * 1e3 / (frames - 1), out["full_seconds"] * 1e3 / (frames - 1))
@@ -347,7 +347,7 @@ def test_reduced_run_gains_over_the_untrained_network_and_falls_back():
     assert res["trained"] < 0.5, res
     assert res["trained"] < 0.1 * res["untrained"], res
     rr = CR.ReducedRun(net, sc, 31.0, n_sample=P, coarse_grid=24)
-    first = rr.run(2, spf, dt, judge=True)["errors"][1]
-    out = rr.run(frames, spf, dt, judge=True, tol=0.5 * first)
+    first = rr.run(2, spf, dt, verify=True)["errors"][1]
+    out = rr.run(frames, spf, dt, verify=True, tol=0.5 * first)
     assert out["source"][:1] == ["crom"] and out["source"][1:] == ["mpm"] * (frames - 1)
     assert torch.isfinite(out["frames"]).all()

@@ -185,7 +185,7 @@ def test_headline_size_parity(oracle):
 
 
 def test_binned_step_by_step_matches_run_and_reloads(oracle):
-    """p2g2p one substep at a time (the driver's pattern, generate_sand.py:29-32)
+    """p2g2p one substep at a time (the reference script's pattern, generate_sand.py:29-32)
     equals one run() call, exports in between do not disturb the state, and a
     loader between substeps re-bins from the exported state."""
     sc = S.column_lattice(n_grid=32, per_axis=2)
