@@ -15,6 +15,7 @@ FP64 (the kernels accumulate in FP32).
 
 Reference citations (zeshunzong/reduced_mpm_stress_training):
   NetDec.forward             offline_training_u/CROMnet.py:293-296
+  NetDecFuncGrad (Jacobian)  offline_training_u/CROMnet.py:319-355
   five Linear+ELU layers     offline_training_u/CROMnet.py:226-237
   Prepare (standardize X)    offline_training_u/CROMnet.py:497-534
   Activation (ELU, alpha=1)  offline_training_u/CROMnet.py:457-473
@@ -84,6 +85,19 @@ def decoder_backward(cache, g_q):
     lbl = cache["lbl"]
     g_x[:, lbl:lbl + 3] /= cache["std_x"]
     return gW, gb, g_x
+
+
+def decoder_jacobian(W, b, x, lbl, mean_x, std_x, mean_q, std_q, bf16_ops=False):
+    """d q / d x per point, (n, out, in): one backward pass per output
+    component (NetDecFuncGrad, CROMnet.py:319-355)."""
+    q, cache = decoder_forward(W, b, x, lbl, mean_x, std_x, mean_q, std_q, bf16_ops)
+    n, out = q.shape
+    J = np.empty((n, out, np.asarray(x).shape[1]))
+    for c in range(out):
+        g = np.zeros_like(q)
+        g[:, c] = 1.0
+        J[:, c, :] = decoder_backward(cache, g)[2]
+    return J
 
 
 def decoder_loss_grads(W, b, x, q_target, lbl, mean_x, std_x, mean_q, std_q, loss_weight=1000.0,

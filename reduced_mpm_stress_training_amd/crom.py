@@ -175,6 +175,35 @@ class CromDecoder:
                 "crom_wgrad")
         return y, dx
 
+    def jacobian(self, x: torch.Tensor) -> torch.Tensor:
+        """d q / d x per point, (n, out_dim, in_dim): NetDecFuncGrad
+        (CROMnet.py:319-355), which the reference checks against autograd
+        (Exporter.py:104-106).  One pass of the backward chain per output
+        component c, with the output gradient e_c (crom_backward of the loss
+        1/2 sum (y - q)^2 at q = y - e_c)."""
+        lib = L.load()
+        x = x.to(self.device, torch.float32).contiguous()
+        n = x.shape[0]
+        if x.dim() != 2 or x.shape[1] != self.in_dim:
+            raise ValueError(f"x must be (n, {self.in_dim})")
+        if self._dirty:
+            self.pack()
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        nb = int(lib.crom_acts_bytes(n))
+        acts = self._buffer("acts", nb)
+        dlt = self._buffer("grads_buf", nb)
+        y = torch.empty((n, self.out_dim), dtype=torch.float32, device=self.device)
+        L.check(lib.crom_forward(self._packed, x.data_ptr(), n, y.data_ptr(), acts, stream), "crom_forward")
+        J = torch.empty((n, self.out_dim, self.in_dim), dtype=torch.float32, device=self.device)
+        dx = torch.empty((n, self.in_dim), dtype=torch.float32, device=self.device)
+        for c in range(self.out_dim):
+            q = y.clone()
+            q[:, c] -= 1.0
+            L.check(lib.crom_backward(self._packed, acts, y.data_ptr(), q.data_ptr(), n, 1.0, dlt, dx.data_ptr(),
+                                      stream), "crom_backward")
+            J[:, c, :] = dx
+        return J
+
     def adam_step_(self, lr: float, betas=(0.9, 0.999), eps: float = 1e-8):
         """torch.optim.Adam's update with the current self.grads
         (CROMnet.py:193-196), then re-pack."""

@@ -167,6 +167,32 @@ def test_decoder_gradients_match_the_cpu_oracle(lbl, mu_dim, out_dim, n):
     assert max(res[False]) <= 3e-2, res
 
 
+@pytest.mark.parametrize("lbl,mu_dim,out_dim", [(16, 0, 3), (16, 1, 6)])
+def test_decoder_jacobian_matches_the_cpu_oracle(lbl, mu_dim, out_dim):
+    """CromDecoder.jacobian (NetDecFuncGrad, CROMnet.py:319-355; the reference
+    checks it against autograd, Exporter.py:104-106) against the FP64 oracle:
+    <= 5e-3 relative L2 with the BF16 roundings emulated, <= 3e-2 exact."""
+    from oracle import crom_oracle_np as O
+    from reduced_mpm_stress_training_amd.crom import CromDecoder
+    torch.manual_seed(17)
+    dec = CromDecoder(lbl, out_dim=out_dim, mu_dim=mu_dim, seed=18)
+    mx, sx = [0.5, 0.4, 0.3], [0.1, 0.2, 0.15]
+    mq, sq = 0.01 * torch.randn(out_dim), 0.05 + 0.01 * torch.rand(out_dim)
+    dec.set_preprop(mean_x=mx, std_x=sx, mean_q=mq, std_q=sq)
+    n = 777
+    x = torch.randn(n, dec.in_dim)
+    x[:, lbl:lbl + 3] = torch.rand(n, 3)
+    J = dec.jacobian(x.cuda()).cpu().numpy()
+    W = [w.cpu().numpy() for w in dec.weights]
+    b = [c.cpu().numpy() for c in dec.biases]
+    res = {}
+    for emu in (True, False):
+        Jo = O.decoder_jacobian(W, b, x.numpy(), lbl, mx, sx, mq.numpy(), sq.numpy(), bf16_ops=emu)
+        res[emu] = float(np.linalg.norm(J - Jo) / np.linalg.norm(Jo))
+    print(res)
+    assert res[True] <= 5e-3 and res[False] <= 3e-2, res
+
+
 def test_encoder_gradients_match_the_cpu_oracle():
     """Encoder (FP32 kernels) latents and parameter gradients against the
     FP64 NumPy oracle, 1e-5 / 1e-4 relative L2."""

@@ -156,3 +156,28 @@ def test_adam_matches_torch():
         pt.grad = torch.tensor(g)
         opt.step()
     assert _rel(p, pt.detach().numpy()) <= 1e-13
+
+
+def test_decoder_jacobian_matches_autograd():
+    """The oracle's Jacobian d q / d x against torch.autograd.functional.jacobian
+    (FP64), the reference's own check of NetDecFuncGrad (Exporter.py:104-106)."""
+    rng = np.random.default_rng(5)
+    lbl, n = 16, 7
+    dims = [20, 128, 128, 128, 128, 128, 6]     # stress head: [label, X, mu] -> 6
+    W, b = _decoder(rng, dims)
+    x = rng.normal(size=(n, 20))
+    mx, sx, mq, sq = [0.5, 0.4, 0.3], [0.1, 0.2, 0.15], np.zeros(6), np.full(6, 0.05)
+    J = O.decoder_jacobian(W, b, x, lbl, mx, sx, mq, sq)
+    Wt = [torch.tensor(w) for w in W]
+    bt = [torch.tensor(c) for c in b]
+
+    def f(xi):
+        xs = torch.cat([xi[:lbl], (xi[lbl:lbl + 3] - torch.tensor(mx, dtype=torch.float64)) /
+                        torch.tensor(sx, dtype=torch.float64), xi[lbl + 3:]])
+        a = xs
+        for k in range(5):
+            a = torch.nn.functional.elu(a @ Wt[k].T + bt[k])
+        return torch.tensor(mq) + (a @ Wt[5].T + bt[5]) * torch.tensor(sq)
+    for i in range(n):
+        Jt = torch.autograd.functional.jacobian(f, torch.tensor(x[i])).numpy()
+        assert _rel(J[i], Jt) <= 1e-10
