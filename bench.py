@@ -57,6 +57,8 @@ def parse():
                     help="time the unbinned stage kernels (atomic P2G, grid update, G2P) instead")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy (reference for the HBM byte counters)")
+    ap.add_argument("--keep", default="auto", choices=["auto", "all", "checkpoints"],
+                    help="adjoint: keep every substep from the forward pass, or replay segments from checkpoints")
     ap.add_argument("--workload", default="column", choices=["column", "batch", "adjoint", "crom"],
                     help="column: the 1M-particle headline; batch: 64 uneven scenes advanced together; "
                          "adjoint: forward + adjoint of a differentiable rollout")
@@ -218,7 +220,10 @@ def bench_adjoint(args):
     checkpoints every 20; loss = mean |x_final - x_target|^2 against a target
     rollout at E = 3e5 Pa, friction angle 35 deg; gradient with respect to
     (E, nu, friction angle, initial velocities) by the adjoint kernels.
-    A step is one forward substep plus its adjoint (replay included).  One
+    A step is one forward substep plus its adjoint: with --keep all (the
+    default "auto" takes it where half the free HBM holds the record) the
+    forward keeps every substep's state and grids and the adjoint runs on
+    them; with --keep checkpoints each segment is replayed first.  One
     gradient evaluation runs untimed first (the rollout's buffers are then
     allocated, as in any fitting loop after its first iteration); the next one
     is timed."""
@@ -227,7 +232,7 @@ def bench_adjoint(args):
     from reduced_mpm_stress_training_amd import scenes as S
     n, ng, steps = args.particles, args.n_grid, args.steps
     sc = S.column_uniform(n_particles=n, n_grid=ng)
-    ro = A.Rollout(sc, steps, 1e-4, checkpoint_every=20)
+    ro = A.Rollout(sc, steps, 1e-4, checkpoint_every=20, keep=args.keep)
     v0 = torch.from_numpy(sc.v0).cuda()
     with torch.no_grad():
         x_t = ro.forward(v0, 3.0e5, 0.3, 35.0).clone()
@@ -267,7 +272,9 @@ def bench_adjoint(args):
         "n_gpus": 1, "steps": steps, "warmup": steps, "ms_per_step": (t2 - t0) * 1e3 / steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "differentiable sand rollout, forward + adjoint", "n_particles": n, "n_grid": ng,
-                   "substeps": steps, "checkpoint_every": 20, "parallelism": "single GPU"},
+                   "substeps": steps, "checkpoint_every": 20, "parallelism": "single GPU",
+                   "keep": "all" if ro._recorded else "checkpoints",
+                   "record_gb": ro.record_bytes() / 1e9 if ro._recorded else 0.0},
         "forward_ms_per_step": (t1 - t0) * 1e3 / steps, "backward_ms_per_step": (t2 - t1) * 1e3 / steps,
         "grad": {"E": g[0], "nu": g[1], "friction_angle": g[2]}, "finite": finite,
         "binned_forward_ms_per_step": ms_fwd, "fwd_adjoint_over_binned_forward": (t2 - t0) * 1e3 / steps / ms_fwd,

@@ -1,7 +1,9 @@
 """Differentiable sand rollouts on MI355X: the forward substeps (mpm_step_ordered)
 and their hand-written adjoints (mpm_adjoint_substep, include/mpm_hip.h)
 behind a torch.autograd.Function, with the state checkpointed every K
-substeps and the substeps between checkpoints recomputed on the way back.
+substeps and the substeps between checkpoints recomputed on the way back --
+or, where device memory holds them, every substep's state and grids kept by
+the forward pass so that the way back only runs the adjoint kernels.
 
 The reference never differentiates its solver (it records no wp.Tape,
 engine_stress/mpm_solver_warp.py); the parameters follow its conventions:
@@ -55,13 +57,25 @@ class Rollout:
     that slot order (mpm_step_ordered), so that their P2G and the adjoint's
     grid scatter accumulate per chunk of particles in LDS.  The adjoint of a
     sort is its inverse permutation (mpm_permute), so gradients come back in
-    the caller's particle order."""
+    the caller's particle order.
+
+    keep: "checkpoints" keeps the sorted state at each checkpoint and the
+    backward pass replays each segment (mpm_adjoint_segment); "all" keeps
+    every substep's state and grids from the forward pass itself
+    (mpm_record_segment, then mpm_adjoint_recorded on the way back: no
+    replay); "auto" takes "all" when that record needs at most half of the
+    free device memory (1M particles on 128^3 for 200 substeps: ~45 GB)."""
+
+    KEEP = ("auto", "all", "checkpoints")
 
     def __init__(self, scene: S.SandScene, n_substeps: int, dt: float, checkpoint_every: int = 20,
-                 device="cuda:0"):
+                 device="cuda:0", keep: str = "auto"):
         self.scene, self.n_substeps, self.dt, self.K = scene, int(n_substeps), float(dt), int(checkpoint_every)
         if self.K < 1:
             raise ValueError("checkpoint_every must be >= 1")
+        if keep not in self.KEEP:
+            raise ValueError(f"keep must be one of {self.KEEP}")
+        self.keep = keep
         self.device = torch.device(device)
         self.n = scene.n_particles
         lib = L.load()
@@ -102,6 +116,22 @@ class Rollout:
         self._final = None
         self._ws_seg = []
         self._seg_states = []
+        # the record of keep="all": per substep k a state and a grid
+        # workspace, per segment an order buffer (reused across calls)
+        self._tape_states, self._tape_grids, self._tape_orders = [], [], []
+        self._tape_last = None
+        self._recorded = False
+
+    def record_bytes(self) -> int:
+        """Device bytes of the keep="all" record (states and grids of every substep)."""
+        return self.n_substeps * (L.NPLANES * self.n * 4 + self._ws_step_bytes + 256)
+
+    def _records(self) -> bool:
+        if self.keep != "auto":
+            return self.keep == "all"
+        free, _ = torch.cuda.mem_get_info(self.device)
+        held = sum(t.numel() * t.element_size() for t in self._tape_states + self._tape_grids)
+        return self.record_bytes() <= 0.5 * (free + held)
 
     @staticmethod
     def _aligned(buf: torch.Tensor) -> int:
@@ -119,10 +149,10 @@ class Rollout:
         self.mu, self.lam, self.alpha = material(E, nu, phi)
         self._prm.mu, self._prm.lam, self._prm.alpha = self.mu, self.lam, self.alpha
 
-    def _sort(self, planes):
+    def _sort(self, planes, dst=None, order=None):
         """(planes sorted by bin, their order buffer)."""
-        dst = torch.empty_like(planes)
-        order = torch.empty(self._order_bytes + 256, dtype=torch.uint8, device=self.device)
+        dst = torch.empty_like(planes) if dst is None else dst
+        order = torch.empty(self._order_bytes + 256, dtype=torch.uint8, device=self.device) if order is None else order
         L.check(L.load().mpm_sort_by_bin(self._prm, self._state(planes), self._state(dst), self._aligned(order),
                                          self._stream()), "mpm_sort_by_bin")
         return dst, order
@@ -148,16 +178,56 @@ class Rollout:
         planes[L.P_V:L.P_V + 3] = v0.to(self.device, torch.float32).t()
         self._checkpoints = {}
         lib = L.load()
+        self._recorded = self._records()
+        if self._recorded:
+            return self._forward_recorded(planes)
+        self._release_record()
         for c0 in range(0, self.n_substeps, self.K):
             planes, order = self._sort(planes)
             self._checkpoints[c0] = (planes.clone(), order)
             L.check(lib.mpm_run_ordered(self._prm, self._state(planes), self._aligned(order), self._cols,
                                         self._ncol, self._aligned(self._ws_run), self._ws_run_bytes, c0, self.dt,
                                         min(self.K, self.n_substeps - c0), self._stream()), "mpm_run_ordered")
+        return self._finish(planes)
+
+    def _finish(self, planes):
         self._final = planes
         # x0 and u back to the caller's order (scatter by the pid plane)
         xu = self._permute(planes[L.P_X0:L.P_U + 3], 6, planes[L.P_PID].data_ptr(), scatter=True)
         return (xu[0:3] + xu[3:6]).t().contiguous()
+
+    def _release_record(self):
+        self._tape_states, self._tape_grids, self._tape_orders = [], [], []
+        self._tape_last = None
+
+    def _forward_recorded(self, planes):
+        """forward() keeping every substep: per segment the sort into the
+        record's state of its first substep, then mpm_record_segment."""
+        lib = L.load()
+        n_seg = (self.n_substeps + self.K - 1) // self.K
+        while len(self._tape_states) < self.n_substeps:
+            self._tape_states.append(torch.empty((L.NPLANES, self.n), dtype=torch.float32, device=self.device))
+        while len(self._tape_grids) < self.n_substeps:
+            self._tape_grids.append(torch.empty(self._ws_step_bytes + 256, dtype=torch.uint8, device=self.device))
+        while len(self._tape_orders) < n_seg:
+            self._tape_orders.append(torch.empty(self._order_bytes + 256, dtype=torch.uint8, device=self.device))
+        if self._tape_last is None:
+            self._tape_last = torch.empty((L.NPLANES, self.n), dtype=torch.float32, device=self.device)
+        last = self._tape_last
+        for s_i, c0 in enumerate(range(0, self.n_substeps, self.K)):
+            nk = min(self.K, self.n_substeps - c0)
+            self._sort(planes, dst=self._tape_states[c0], order=self._tape_orders[s_i])
+            states, grids = self._segment_args(c0, nk)
+            L.check(lib.mpm_record_segment(self._prm, states, self._aligned(self._tape_orders[s_i]), grids, self._cols,
+                                           self._ncol, c0, self.dt, nk, self._state(last), self._aligned(self._ws_run),
+                                           self._ws_run_bytes, self._stream()), "mpm_record_segment")
+            planes = last
+        return self._finish(planes)
+
+    def _segment_args(self, c0, nk):
+        states = (L.MpmState * nk)(*[self._state(self._tape_states[c0 + j]) for j in range(nk)])
+        grids = (ctypes.c_void_p * nk)(*[self._aligned(self._tape_grids[c0 + j]) for j in range(nk)])
+        return states, grids
 
     def backward(self, x_bar: torch.Tensor):
         """Adjoint of forward() for the position adjoint x_bar (N, 3) in the
@@ -172,6 +242,18 @@ class Rollout:
         adj[L.ADJ_X:L.ADJ_X + 3] = self._permute(xb, 3, self._final[L.P_PID].data_ptr(), scatter=False)
         nxt = torch.empty_like(adj)
         pb = torch.zeros(3, dtype=torch.float64, device=self.device)
+        if self._recorded:
+            for s_i in range((self.n_substeps + self.K - 1) // self.K - 1, -1, -1):
+                c0 = s_i * self.K
+                nk = min(self.K, self.n_substeps - c0)
+                states, grids = self._segment_args(c0, nk)
+                order = self._tape_orders[s_i]
+                L.check(lib.mpm_adjoint_recorded(self._prm, states, self._aligned(order), grids, self._cols,
+                                                 self._ncol, self._aligned(self._ws_adj), c0, self.dt, nk,
+                                                 adj.data_ptr(), nxt.data_ptr(), self.n, pb.data_ptr(), stream),
+                        "mpm_adjoint_recorded")
+                adj = self._permute(adj, L.ADJ_NPLANES, self._aligned(order) + self._perm_off, scatter=True)
+            return self._param_grads(adj, pb)
         # a segment's replayed states and, per replayed substep, the grids it
         # leaves for its adjoint (one native call per segment)
         while len(self._seg_states) < self.K - 1:
@@ -191,6 +273,9 @@ class Rollout:
                                             nxt.data_ptr(), self.n, pb.data_ptr(), stream), "mpm_adjoint_segment")
             # adjoint of the sort at c0: its inverse permutation
             adj = self._permute(adj, L.ADJ_NPLANES, self._aligned(order) + self._perm_off, scatter=True)
+        return self._param_grads(adj, pb)
+
+    def _param_grads(self, adj, pb):
         v0_bar = adj[L.ADJ_V:L.ADJ_V + 3].t().contiguous()
         mu_b, lam_b, al_b = (float(v) for v in pb.cpu())
         J = material_jacobian(*self._material_args)

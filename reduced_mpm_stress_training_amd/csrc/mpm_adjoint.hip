@@ -817,25 +817,31 @@ int mpm_adjoint_substep(const MpmParams* prm, const MpmState* st, const void* or
     return 0;
 }
 
-int mpm_adjoint_segment(const MpmParams* prm, const MpmState* states, const void* order, void* const* step_grids,
-                        const MpmCollider* colliders, int n_colliders, void* workspace, int k0, double dt, int n_sub,
-                        float* adj_a, float* adj_b, int64_t adj_stride, double* param_adj, void* stream) {
+// shared checks of a segment's states and per-substep workspaces
+static int check_segment(const MpmParams* prm, const MpmState* states, void* const* step_grids, int n_sub) {
     if (!states || !step_grids || n_sub < 1) return set_err(-31, "segment: states, grids or n_sub invalid");
-    if (!adj_a || !adj_b || adj_a == adj_b) return set_err(-30, "adjoint buffers invalid");
     const int64_t stride = states[0].stride, n = states[0].n;
     for (int j = 0; j < n_sub; ++j)
         if (!states[j].planes || states[j].stride != stride || states[j].n != n)
             return set_err(-32, "segment: states must share n and stride");
     if (check_grid(prm)) return set_err(-2, "n_grid out of range");
-    const int64_t nn = (int64_t)prm->n_grid * prm->n_grid * prm->n_grid;
     for (int j = 0; j < n_sub; ++j)
         if (!step_grids[j] || ((uintptr_t)step_grids[j] & 255))
             return set_err(-11, "segment: step workspaces NULL or not 256-byte aligned");
-    hipStream_t s = (hipStream_t)stream;
+    return 0;
+}
+
+// The forward half of a segment: the P2G of S_k0 (states[0]), then per
+// substep j its grid update into step_grids[j] and the fused G2P + next P2G
+// states[j] -> states[j+1] into step_grids[j+1].  When `last` is given, the
+// last substep writes S_{k0+n_sub} there, its (unused) next P2G going to
+// last_acc.
+static int record_segment(const MpmParams* prm, const MpmState* states, const void* order, void* const* step_grids,
+                          const MpmCollider* colliders, int n_colliders, int k0, double dt, int n_sub,
+                          const MpmState* last, float* last_acc, void* stream) {
+    const int64_t nn = (int64_t)prm->n_grid * prm->n_grid * prm->n_grid;
     const float fdt = (float)dt;
     int rc;
-    // replay: the P2G of S_k0, then per substep its grid update and the fused
-    // G2P + next P2G (S_{k0+j} -> S_{k0+j+1}) into the next substep's workspace
     float* acc0 = (float*)step_grids[0];
     if ((rc = mpm_zero_grid(prm, acc0, stream))) return rc;
     if ((rc = p2g_ordered(prm, &states[0], order, acc0, fdt, stream))) return rc;
@@ -844,26 +850,81 @@ int mpm_adjoint_segment(const MpmParams* prm, const MpmState* states, const void
         float* gvj = acc + 4 * nn;
         if ((rc = mpm_grid_update(prm, acc, gvj, colliders, n_colliders, (float)((double)(k0 + j) * dt), fdt, stream)))
             return rc;
-        if (j + 1 < n_sub) {
-            float* acc_next = (float*)step_grids[j + 1];
-            if ((rc = mpm_zero_grid(prm, acc_next, stream))) return rc;
-            if ((rc = g2p2g_ordered(prm, &states[j], &states[j + 1], order, gvj, acc_next, fdt, stream))) return rc;
-        }
+        const bool more = j + 1 < n_sub;
+        if (!more && !last) break;
+        float* acc_next = more ? (float*)step_grids[j + 1] : last_acc;
+        if ((rc = mpm_zero_grid(prm, acc_next, stream))) return rc;
+        if ((rc = g2p2g_ordered(prm, &states[j], more ? &states[j + 1] : last, order, gvj, acc_next, fdt, stream)))
+            return rc;
     }
-    // adjoint substeps in reverse, each on the grids its replay left
+    return 0;
+}
+
+// The reverse half: adjoint substeps k0+n_sub-1 .. k0 on the recorded states
+// and grids; adj_a holds the adjoint of S_{k0+n_sub} on entry and that of
+// S_k0 on return.
+static int adjoint_recorded(const MpmParams* prm, const MpmState* states, const void* order, void* const* step_grids,
+                            const MpmCollider* colliders, int n_colliders, void* workspace, int k0, double dt,
+                            int n_sub, float* adj_a, float* adj_b, int64_t adj_stride, double* param_adj,
+                            void* stream) {
+    int rc;
     float* cur = adj_a;
     float* nxt = adj_b;
     for (int j = n_sub - 1; j >= 0; --j) {
         if ((rc = mpm_adjoint_substep(prm, &states[j], order, step_grids[j], colliders, n_colliders, workspace,
-                                      (float)((double)(k0 + j) * dt), fdt, cur, nxt, adj_stride, param_adj, stream)))
+                                      (float)((double)(k0 + j) * dt), (float)dt, cur, nxt, adj_stride, param_adj,
+                                      stream)))
             return rc;
         float* t = cur; cur = nxt; nxt = t;
     }
-    if (cur != adj_a &&
-        hipMemcpyAsync(adj_a, cur, (size_t)MPM_ADJ_NPLANES * adj_stride * sizeof(float), hipMemcpyDeviceToDevice, s) !=
-            hipSuccess)
+    if (cur != adj_a && hipMemcpyAsync(adj_a, cur, (size_t)MPM_ADJ_NPLANES * adj_stride * sizeof(float),
+                                       hipMemcpyDeviceToDevice, (hipStream_t)stream) != hipSuccess)
         return set_err(-33, "segment: adjoint copy failed");
     return 0;
+}
+
+int mpm_adjoint_segment(const MpmParams* prm, const MpmState* states, const void* order, void* const* step_grids,
+                        const MpmCollider* colliders, int n_colliders, void* workspace, int k0, double dt, int n_sub,
+                        float* adj_a, float* adj_b, int64_t adj_stride, double* param_adj, void* stream) {
+    int rc;
+    if ((rc = check_segment(prm, states, step_grids, n_sub))) return rc;
+    if (!adj_a || !adj_b || adj_a == adj_b) return set_err(-30, "adjoint buffers invalid");
+    if ((rc = record_segment(prm, states, order, step_grids, colliders, n_colliders, k0, dt, n_sub, nullptr, nullptr,
+                             stream)))
+        return rc;
+    return adjoint_recorded(prm, states, order, step_grids, colliders, n_colliders, workspace, k0, dt, n_sub, adj_a,
+                            adj_b, adj_stride, param_adj, stream);
+}
+
+int mpm_record_segment(const MpmParams* prm, const MpmState* states, const void* order, void* const* step_grids,
+                       const MpmCollider* colliders, int n_colliders, int k0, double dt, int n_sub,
+                       const MpmState* last, void* workspace, int64_t workspace_bytes, void* stream) {
+    int rc;
+    if ((rc = check_segment(prm, states, step_grids, n_sub))) return rc;
+    if (states[0].n < 0 || states[0].stride < states[0].n || states[0].n > 0x7fffffffLL)
+        return set_err(-5, "segment: state n/stride invalid");
+    if (!order || ((uintptr_t)order & 255)) return set_err(-40, "order buffer NULL or not 256-byte aligned");
+    const int64_t nn = (int64_t)prm->n_grid * prm->n_grid * prm->n_grid;
+    if (last) {
+        if (!last->planes || last->n != states[0].n || last->stride != states[0].stride)
+            return set_err(-32, "segment: the last state must share n and stride");
+        for (int j = 0; j < n_sub; ++j)
+            if (states[j].planes == last->planes) return set_err(-32, "segment: the last state aliases a state");
+        if (!workspace || ((uintptr_t)workspace & 255) || workspace_bytes < 16 * nn)
+            return set_err(-10, "segment: workspace must be one 256-byte aligned float4 grid");
+    }
+    return record_segment(prm, states, order, step_grids, colliders, n_colliders, k0, dt, n_sub, last,
+                          (float*)workspace, stream);
+}
+
+int mpm_adjoint_recorded(const MpmParams* prm, const MpmState* states, const void* order, void* const* step_grids,
+                         const MpmCollider* colliders, int n_colliders, void* workspace, int k0, double dt, int n_sub,
+                         float* adj_a, float* adj_b, int64_t adj_stride, double* param_adj, void* stream) {
+    int rc;
+    if ((rc = check_segment(prm, states, step_grids, n_sub))) return rc;
+    if (!adj_a || !adj_b || adj_a == adj_b) return set_err(-30, "adjoint buffers invalid");
+    return adjoint_recorded(prm, states, order, step_grids, colliders, n_colliders, workspace, k0, dt, n_sub, adj_a,
+                            adj_b, adj_stride, param_adj, stream);
 }
 
 
