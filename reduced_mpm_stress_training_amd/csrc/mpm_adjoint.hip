@@ -42,20 +42,23 @@ constexpr int kBlk = 256;
 // adjoint planes: x (3), v (3), C (9), F (9), tau (9)
 constexpr int kAX = 0, kAV = 3, kAC = 6, kAF = 15, kAT = 24;
 
-// (b - a) / (exp(b) - exp(a)) style divided differences, robust when the
-// arguments meet: (ln s_l - ln s_k) / (s_l - s_k)
-__device__ __forceinline__ float dlog_div(float sk, float sl) {
+// Divided differences of the logarithm and of a power, robust when the
+// arguments meet, from the arguments s_k, s_l > 0 and their logarithms
+// lk = ln s_k, ll = ln s_l (which the return map already has: no further
+// logarithm or power is evaluated).
+// (ln s_l - ln s_k) / (s_l - s_k):
+__device__ __forceinline__ float dlog_div(float sk, float sl, float lk, float ll) {
     const float r = sl / sk - 1.0f;
     if (fabsf(r) < 1e-4f) return (1.0f - 0.5f * r + r * r / 3.0f) / sk;
-    return logf(sl / sk) / (sl - sk);
+    return (ll - lk) / (sl - sk);
 }
 
-// (s_k^B ... ) helper: (sigma_l^B - sigma_k^B) / (sigma_l - sigma_k)
-__device__ __forceinline__ float dpow_div(float sk, float sl, float B) {
-    const float d = logf(sl / sk);
-    const float den = expm1f(d);
-    if (fabsf(d) < 1e-5f) return B * powf(sk, B - 1.0f) * (1.0f + 0.5f * (B - 1.0f) * d);
-    return powf(sk, B - 1.0f) * expm1f(B * d) / den;
+// (s_l^B - s_k^B) / (s_l - s_k) = s_k^(B-1) expm1(B d) / expm1(d), d = ll - lk:
+__device__ __forceinline__ float dpow_div(float lk, float ll, float B) {
+    const float d = ll - lk;
+    const float pk = expf((B - 1.0f) * lk);
+    if (fabsf(d) < 1e-5f) return B * pk * (1.0f + 0.5f * (B - 1.0f) * d);
+    return pk * expm1f(B * d) / expm1f(d);
 }
 
 __device__ __forceinline__ void matmul(const float A[9], const float B[9], float C[9]) {
@@ -107,7 +110,7 @@ __device__ void sand_update_adj(const float G[9], const float F[9], float dt, fl
     float beta = 0.f;
     if (dg <= 0.0f) {
 #pragma unroll
-        for (int d = 0; d < 3; ++d) { se[d] = s[d]; ls[d] = logf(s[d]); }
+        for (int d = 0; d < 3; ++d) { se[d] = s[d]; ls[d] = (s[d] >= 1e-14f) ? eps[d] : logf(s[d]); }
     } else if (tr > 0.0f) {
         br = 2;
 #pragma unroll
@@ -142,7 +145,7 @@ __device__ void sand_update_adj(const float G[9], const float F[9], float dt, fl
 #pragma unroll
     for (int pr = 0; pr < 3; ++pr) {
         const int k = pr == 2 ? 1 : 0, l = pr == 0 ? 1 : 2;
-        const float c = 2.0f * mu * dlog_div(se[k], se[l]) / (se[k] + se[l]);
+        const float c = 2.0f * mu * dlog_div(se[k], se[l], ls[k], ls[l]) / (se[k] + se[l]);
         const float tt = Tp[k * 3 + l] + Tp[l * 3 + k];
         Q[k * 3 + l] += c * se[l] * tt;
         Q[l * 3 + k] += c * se[k] * tt;
@@ -195,7 +198,7 @@ __device__ void sand_update_adj(const float G[9], const float F[9], float dt, fl
 #pragma unroll
         for (int pr = 0; pr < 3; ++pr) {   // symmetric in (k, l): once per unordered pair
             const int k = pr == 2 ? 1 : 0, l = pr == 0 ? 1 : 2;
-            const float D = br == 3 ? eA * dpow_div(s[k], s[l], Bexp) : 0.f;
+            const float D = br == 3 ? eA * dpow_div(eps[k], eps[l], Bexp) : 0.f;
             const float S = (se[l] + se[k]) / (s[l] + s[k]);
             const float a = 0.5f * (D + S), b = 0.5f * (D - S);
             const float qkl = Q[k * 3 + l], qlk = Q[l * 3 + k];
@@ -366,12 +369,28 @@ __device__ __forceinline__ void g2p_adj_nodes(const Nodes& V, const float fx[3],
 // of S_{n+1}; ac: adjoint of S_n (x and F written here, v, C, tau by
 // k_p2g_adj).  V: the particle's 27 grid velocities; emit(a, i, j, k, value)
 // receives its grid-velocity adjoint.
+// A particle's first inputs to its G2P adjoint: position, F, and the
+// adjoint of the next F; the chunk loop loads the next particle's while it
+// processes the current one.
+struct AdjIn {
+    float x[3], F[9], Fbn[9];
+};
+
+__device__ __forceinline__ void load_adj_in(const float* __restrict__ st, int64_t S, const float* __restrict__ an,
+                                            int64_t SA, int p, AdjIn& q) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) q.x[d] = st[(MPM_P_X0 + d) * S + p] + st[(MPM_P_U + d) * S + p];
+    load9(st + p, S, MPM_P_F, q.F);
+    load9(an + p, SA, kAF, q.Fbn);
+}
+
 template <typename Nodes, typename Emit>
-__device__ __forceinline__ void g2p_adj_particle(const float* __restrict__ st, int64_t S, int p, const MpmParams& prm,
-                                                 float dt, const float x[3], const AxisWeights& Aw, const Nodes& V,
-                                                 const float* __restrict__ an, int64_t SA, float* __restrict__ ac,
-                                                 float& mub, float& lamb, float& alb, Emit&& emit) {
+__device__ __forceinline__ void g2p_adj_particle(const AdjIn& q, int p, const MpmParams& prm, float dt,
+                                                 const AxisWeights& Aw, const Nodes& V, const float* __restrict__ an,
+                                                 int64_t SA, float* __restrict__ ac, float& mub, float& lamb,
+                                                 float& alb, Emit&& emit) {
     const float inv_dx = prm.inv_dx;
+    const float* x = q.x;
     float G[9];
     {
         float v[3], Cm[9];
@@ -380,11 +399,9 @@ __device__ __forceinline__ void g2p_adj_particle(const float* __restrict__ st, i
     const float* ap = an + p;
     float xb[3], vt[3], Cb[9], Gb[9], Fb[9];
     {
-        float F[9], Fbn[9], Tb[9];
-        load9(st + p, S, MPM_P_F, F);
-        load9(ap, SA, kAF, Fbn);
+        float Tb[9];
         load9(ap, SA, kAT, Tb);
-        sand_update_adj(G, F, dt, prm.mu, prm.lam, prm.alpha, Fbn, Tb, Gb, Fb, mub, lamb, alb);
+        sand_update_adj(G, q.F, dt, prm.mu, prm.lam, prm.alpha, q.Fbn, Tb, Gb, Fb, mub, lamb, alb);
     }
     load9(ap, SA, kAC, Cb);
     // x' = x + dt v'
@@ -469,14 +486,15 @@ __global__ void __launch_bounds__(kChunkWG)
     // particles whose stencil lies in the chunk's tile; the others are
     // marked (bit r for the thread's r-th particle) for the second loop
     unsigned outside = 0;
+    AdjIn q;
+    if (tid < n_per && ch.x + tid < n) load_adj_in(st, S, an, SA, ch.x + tid, q);
     for (int i = tid, r = 0; i < n_per; i += kChunkWG, ++r) {
         const int p = ch.x + i;
+        const AdjIn cur = q;
+        if (i + kChunkWG < n_per && p + kChunkWG < n) load_adj_in(st, S, an, SA, p + kChunkWG, q);
         if (p < 0 || p >= n) continue;
-        float x[3];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) x[d] = st[(MPM_P_X0 + d) * S + p] + st[(MPM_P_U + d) * S + p];
         AxisWeights Aw;
-        axis_weights(x, prm.inv_dx, Aw);
+        axis_weights(cur.x, prm.inv_dx, Aw);
         if (!base_inside(Aw.base, ng)) {
             frozen_adj(an, SA, p, ac);
             continue;
@@ -486,7 +504,7 @@ __global__ void __launch_bounds__(kChunkWG)
             outside |= 1u << r;
             continue;
         }
-        g2p_adj_particle(st, S, p, prm, dt, x, Aw, TileNodes{s_v, ti}, an, SA, ac, mub, lamb, alb,
+        g2p_adj_particle(cur, p, prm, dt, Aw, TileNodes{s_v, ti}, an, SA, ac, mub, lamb, alb,
                          [&](int a, int i3, int j3, int k3, float val) {
                              atomicAdd(&s_t[a * kCT3 + ti + (i3 * kCT + j3) * kCT + k3], (double)val);
                          });
@@ -494,13 +512,11 @@ __global__ void __launch_bounds__(kChunkWG)
     for (int i = tid, r = 0; outside && i < n_per; i += kChunkWG, ++r) {
         if (!(outside & (1u << r))) continue;
         const int p = ch.x + i;
-        float x[3];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) x[d] = st[(MPM_P_X0 + d) * S + p] + st[(MPM_P_U + d) * S + p];
+        load_adj_in(st, S, an, SA, p, q);
         AxisWeights Aw;
-        axis_weights(x, prm.inv_dx, Aw);
+        axis_weights(q.x, prm.inv_dx, Aw);
         const int64_t gb = ((int64_t)Aw.base[0] * ng + Aw.base[1]) * ng + Aw.base[2];
-        g2p_adj_particle(st, S, p, prm, dt, x, Aw, GridNodes{gv, gb, ng}, an, SA, ac, mub, lamb, alb,
+        g2p_adj_particle(q, p, prm, dt, Aw, GridNodes{gv, gb, ng}, an, SA, ac, mub, lamb, alb,
                          [&](int a, int i3, int j3, int k3, float val) {
                              unsafeAtomicAdd(vb + 4 * (gb + ((int64_t)i3 * ng + j3) * ng + k3) + a, val);
                          });

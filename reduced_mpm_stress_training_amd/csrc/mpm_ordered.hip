@@ -291,26 +291,45 @@ struct OrdNodesGrid {
     }
 };
 
-template <typename Nodes>
-__device__ __forceinline__ void g2p2g_flat_particle(const float* src, float* dst, int64_t S, int p,
-                                                    const float x0[3], const float x[3], const AxisWeights& A,
-                                                    const Nodes& V, const MpmParams& prm, float dt, double* s_acc,
-                                                    const int o2[3], float* __restrict__ acc_next,
-                                                    int* __restrict__ status) {
-    const int ng = prm.n_grid;
+// A particle's inputs to the fused substep; the chunk loop loads the next
+// particle's while it processes the current one.
+struct FlatIn {
+    float x0[3], u[3], F[9], m, vol, pid;
+};
+
+__device__ __forceinline__ void load_flat(const float* src, int64_t S, int p, FlatIn& q) {
     const float* sp = src + p;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        q.x0[d] = sp[(MPM_P_X0 + d) * S];
+        q.u[d] = sp[(MPM_P_U + d) * S];
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) q.F[k] = sp[(MPM_P_F + k) * S];
+    q.m = sp[MPM_P_MASS * S];
+    q.vol = sp[MPM_P_VOL * S];
+    q.pid = sp[MPM_P_PID * S];
+}
+
+template <typename Nodes>
+__device__ __forceinline__ void g2p2g_flat_particle(const FlatIn& q, bool copy, float* dst, int64_t S, int p,
+                                                    const float x[3], const AxisWeights& A, const Nodes& V,
+                                                    const MpmParams& prm, float dt, double* s_acc, const int o2[3],
+                                                    float* __restrict__ acc_next, int* __restrict__ status) {
+    const int ng = prm.n_grid;
     float* dp = dst + p;
+    const float* x0 = q.x0;
     float v[3], Cm[9], G[9], F[9], tau[9];
     gather_tile<9, 3>(V, 0, A, prm.inv_dx, v, Cm, G);
 #pragma unroll
-    for (int k = 0; k < 9; ++k) F[k] = sp[(MPM_P_F + k) * S];
-    const float m = sp[MPM_P_MASS * S], vol = sp[MPM_P_VOL * S];
-    if (dst != src) {   // the planes the substep does not change
+    for (int k = 0; k < 9; ++k) F[k] = q.F[k];
+    const float m = q.m, vol = q.vol;
+    if (copy) {   // out of place: the planes the substep does not change
 #pragma unroll
         for (int d = 0; d < 3; ++d) dp[(MPM_P_X0 + d) * S] = x0[d];
         dp[MPM_P_MASS * S] = m;
         dp[MPM_P_VOL * S] = vol;
-        dp[MPM_P_PID * S] = sp[MPM_P_PID * S];
+        dp[MPM_P_PID * S] = q.pid;
     }
     sand_update(G, dt, F, prm.mu, prm.lam, prm.alpha, tau);
     float xn[3], u[3];
@@ -393,17 +412,20 @@ __global__ void __launch_bounds__(kChunkWG) __attribute__((amdgpu_waves_per_eu(2
     }
     __syncthreads();
     // stencils inside the velocity tile first; the others (bit r of the
-    // thread's r-th particle) gather from the global grid afterwards
+    // thread's r-th particle) gather from the global grid afterwards.  The
+    // next particle's inputs are loaded while the current one is processed.
+    const bool copy = dst != planes;
     unsigned outside = 0;
+    FlatIn q;
+    if (tid < n_per && ch.x + tid < n) load_flat(planes, S, ch.x + tid, q);
     for (int i = tid, r = 0; i < n_per; i += kChunkWG, ++r) {
         const int p = ch.x + i;
+        const FlatIn cur = q;
+        if (i + kChunkWG < n_per && p + kChunkWG < n) load_flat(planes, S, p + kChunkWG, q);
         if (p < 0 || p >= n) continue;
-        float x0[3], x[3];
+        float x[3];
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            x0[d] = planes[(MPM_P_X0 + d) * S + p];
-            x[d] = x0[d] + planes[(MPM_P_U + d) * S + p];
-        }
+        for (int d = 0; d < 3; ++d) x[d] = cur.x0[d] + cur.u[d];
         AxisWeights A;
         axis_weights(x, prm.inv_dx, A);
         if (!base_ok(A.base, ng)) {   // frozen: unchanged
@@ -416,22 +438,20 @@ __global__ void __launch_bounds__(kChunkWG) __attribute__((amdgpu_waves_per_eu(2
             outside |= 1u << r;
             continue;
         }
-        g2p2g_flat_particle(planes, dst, S, p, x0, x, A, OrdNodesTile{s_v, ti}, prm, dt, s_acc, o2, acc_next,
+        g2p2g_flat_particle(cur, copy, dst, S, p, x, A, OrdNodesTile{s_v, ti}, prm, dt, s_acc, o2, acc_next,
                             status);
     }
     for (int i = tid, r = 0; outside && i < n_per; i += kChunkWG, ++r) {
         if (!(outside & (1u << r))) continue;
         const int p = ch.x + i;
-        float x0[3], x[3];
+        load_flat(planes, S, p, q);
+        float x[3];
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            x0[d] = planes[(MPM_P_X0 + d) * S + p];
-            x[d] = x0[d] + planes[(MPM_P_U + d) * S + p];
-        }
+        for (int d = 0; d < 3; ++d) x[d] = q.x0[d] + q.u[d];
         AxisWeights A;
         axis_weights(x, prm.inv_dx, A);
         const OrdNodesGrid V{gv, ((int64_t)A.base[0] * ng + A.base[1]) * ng + A.base[2], ng};
-        g2p2g_flat_particle(planes, dst, S, p, x0, x, A, V, prm, dt, s_acc, o2, acc_next, status);
+        g2p2g_flat_particle(q, copy, dst, S, p, x, A, V, prm, dt, s_acc, o2, acc_next, status);
     }
     __syncthreads();
     chunk_flush<4>(s_acc, o2, ng, acc_next);
