@@ -226,8 +226,12 @@ __device__ __forceinline__ void load9(const float* p, int64_t S, int plane, floa
     for (int k = 0; k < 9; ++k) o[k] = p[(int64_t)(plane + k) * S];
 }
 
-__device__ void block_add_params(float mub, float lamb, float alb, double* out) {
-    // wave reduction, then one double atomic per wave
+// The material parameters' adjoints of a workgroup: a wave reduction, the
+// waves' sums in s_par (3 per wave), and after a barrier one double atomic per
+// parameter from thread 0 (block_flush_params).  Every wave of every
+// workgroup adds to the same three doubles, so per-wave atomics would
+// serialize thousands of same-address atomics at the end of the kernel.
+__device__ __forceinline__ void block_stage_params(float mub, float lamb, float alb, float* s_par) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
         mub += __shfl_xor(mub, o);
@@ -235,9 +239,18 @@ __device__ void block_add_params(float mub, float lamb, float alb, double* out) 
         alb += __shfl_xor(alb, o);
     }
     if ((threadIdx.x & 63) == 0) {
-        atomicAdd(&out[0], (double)mub);
-        atomicAdd(&out[1], (double)lamb);
-        atomicAdd(&out[2], (double)alb);
+        const int w = threadIdx.x >> 6;
+        s_par[3 * w] = mub;
+        s_par[3 * w + 1] = lamb;
+        s_par[3 * w + 2] = alb;
+    }
+}
+
+__device__ __forceinline__ void block_flush_params(const float* s_par, double* out) {
+    if (threadIdx.x < 3) {
+        double acc = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) acc += (double)s_par[3 * w + threadIdx.x];
+        atomicAdd(&out[threadIdx.x], acc);
     }
 }
 
@@ -444,6 +457,7 @@ __global__ void __launch_bounds__(kChunkWG)
                      double* __restrict__ pb) {
     __shared__ double s_t[3 * kCT3];
     __shared__ float s_v[3 * kCT3];
+    __shared__ float s_par[3 * (kChunkWG / 64)];
     __shared__ int s_lo[3];
     const int c = blockIdx.x;
     if (c >= meta[0]) return;
@@ -521,8 +535,9 @@ __global__ void __launch_bounds__(kChunkWG)
                              unsafeAtomicAdd(vb + 4 * (gb + ((int64_t)i3 * ng + j3) * ng + k3) + a, val);
                          });
     }
-    block_add_params(mub, lamb, alb, pb);
+    block_stage_params(mub, lamb, alb, s_par);
     __syncthreads();
+    block_flush_params(s_par, pb);
     if (any) chunk_flush<3>(s_t, o, ng, vb);
 }
 
