@@ -40,7 +40,7 @@
 extern "C" {
 #endif
 
-#define MPM_ABI_VERSION 8
+#define MPM_ABI_VERSION 9
 
 /* plane indices */
 #define MPM_P_X0 0
@@ -276,16 +276,19 @@ int mpm_adjoint_segment(const MpmParams* prm, const MpmState* states, const void
  * keep every substep's state and grids from the forward pass (no replay on
  * the way back; 288 GB of HBM holds all 200 substeps of a 1M-particle rollout).
  * mpm_record_segment: the forward substeps k0 .. k0+n_sub-1 of an ordered
- * segment, states and step_grids as in mpm_adjoint_segment; when `last` is
- * not NULL (same n and stride, no alias of a state) it receives
- * S_{k0+n_sub}, and workspace (256-byte aligned, >= 16 n_grid^3 bytes) takes
- * that substep's unused next P2G.  mpm_adjoint_recorded: the adjoint
- * substeps in reverse on what mpm_record_segment left, arguments as in
- * mpm_adjoint_segment.  No counterpart in the reference (it records no
- * tape); the two together equal mpm_adjoint_segment. */
+ * segment, states and step_grids as in mpm_adjoint_segment.  p2g_done != 0:
+ * step_grids[0] already holds the P2G accumulator of states[0] (the
+ * next_grid of the segment before, whose last state this segment sorted), so
+ * it is not recomputed.  When `last` is not NULL (same n and stride, no
+ * alias of a state) it receives S_{k0+n_sub}, and next_grid (256-byte
+ * aligned, >= 16 n_grid^3 bytes, not one of step_grids) that state's P2G
+ * accumulator.  mpm_adjoint_recorded: the adjoint substeps in reverse on what
+ * mpm_record_segment left, arguments as in mpm_adjoint_segment.  No
+ * counterpart in the reference (it records no tape); with p2g_done = 0 the
+ * two together equal mpm_adjoint_segment. */
 int mpm_record_segment(const MpmParams* prm, const MpmState* states, const void* order, void* const* step_grids,
-                       const MpmCollider* colliders, int n_colliders, int k0, double dt, int n_sub,
-                       const MpmState* last, void* workspace, int64_t workspace_bytes, void* stream);
+                       const MpmCollider* colliders, int n_colliders, int k0, double dt, int n_sub, int p2g_done,
+                       const MpmState* last, void* next_grid, int64_t next_grid_bytes, void* stream);
 int mpm_adjoint_recorded(const MpmParams* prm, const MpmState* states, const void* order, void* const* step_grids,
                          const MpmCollider* colliders, int n_colliders, void* workspace, int k0, double dt, int n_sub,
                          float* adj_a, float* adj_b, int64_t adj_stride, double* param_adj, void* stream);

@@ -10,7 +10,7 @@ import os
 
 from ._build import LIB_PATH
 
-MPM_ABI_VERSION = 8
+MPM_ABI_VERSION = 9
 MAX_COLLIDERS = 16
 
 # plane indices of the particle state buffer
@@ -116,7 +116,7 @@ _SIGNATURES = {
                                       C.POINTER(MpmCollider), C.c_int, C.c_void_p, C.c_int, C.c_double, C.c_int,
                                       C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
     "mpm_record_segment": (C.c_int, [C.POINTER(MpmParams), C.POINTER(MpmState), C.c_void_p, C.c_void_p,
-                                     C.POINTER(MpmCollider), C.c_int, C.c_int, C.c_double, C.c_int,
+                                     C.POINTER(MpmCollider), C.c_int, C.c_int, C.c_double, C.c_int, C.c_int,
                                      C.POINTER(MpmState), C.c_void_p, C.c_int64, C.c_void_p]),
     "mpm_adjoint_recorded": (C.c_int, [C.POINTER(MpmParams), C.POINTER(MpmState), C.c_void_p, C.c_void_p,
                                        C.POINTER(MpmCollider), C.c_int, C.c_void_p, C.c_int, C.c_double, C.c_int,

@@ -218,9 +218,13 @@ class Rollout:
             nk = min(self.K, self.n_substeps - c0)
             self._sort(planes, dst=self._tape_states[c0], order=self._tape_orders[s_i])
             states, grids = self._segment_args(c0, nk)
+            # the last substep's P2G of the next state goes straight into the
+            # next segment's first grid (the sort does not change the grid)
+            nxt = self._tape_grids[c0 + nk] if c0 + nk < self.n_substeps else self._ws_run
             L.check(lib.mpm_record_segment(self._prm, states, self._aligned(self._tape_orders[s_i]), grids, self._cols,
-                                           self._ncol, c0, self.dt, nk, self._state(last), self._aligned(self._ws_run),
-                                           self._ws_run_bytes, self._stream()), "mpm_record_segment")
+                                           self._ncol, c0, self.dt, nk, int(s_i > 0), self._state(last),
+                                           self._aligned(nxt), nxt.numel() - 256, self._stream()),
+                    "mpm_record_segment")
             planes = last
         return self._finish(planes)
 

@@ -862,20 +862,22 @@ static int check_segment(const MpmParams* prm, const MpmState* states, void* con
     return 0;
 }
 
-// The forward half of a segment: the P2G of S_k0 (states[0]), then per
-// substep j its grid update into step_grids[j] and the fused G2P + next P2G
-// states[j] -> states[j+1] into step_grids[j+1].  When `last` is given, the
-// last substep writes S_{k0+n_sub} there, its (unused) next P2G going to
-// last_acc.
+// The forward half of a segment: the P2G of S_k0 (states[0]; skipped when
+// p2g_done says step_grids[0] holds it already), then per substep j its grid
+// update into step_grids[j] and the fused G2P + next P2G states[j] ->
+// states[j+1] into step_grids[j+1].  When `last` is given, the last substep
+// writes S_{k0+n_sub} there and its P2G to last_acc.
 static int record_segment(const MpmParams* prm, const MpmState* states, const void* order, void* const* step_grids,
-                          const MpmCollider* colliders, int n_colliders, int k0, double dt, int n_sub,
+                          const MpmCollider* colliders, int n_colliders, int k0, double dt, int n_sub, int p2g_done,
                           const MpmState* last, float* last_acc, void* stream) {
     const int64_t nn = (int64_t)prm->n_grid * prm->n_grid * prm->n_grid;
     const float fdt = (float)dt;
     int rc;
     float* acc0 = (float*)step_grids[0];
-    if ((rc = mpm_zero_grid(prm, acc0, stream))) return rc;
-    if ((rc = p2g_ordered(prm, &states[0], order, acc0, fdt, stream))) return rc;
+    if (!p2g_done) {
+        if ((rc = mpm_zero_grid(prm, acc0, stream))) return rc;
+        if ((rc = p2g_ordered(prm, &states[0], order, acc0, fdt, stream))) return rc;
+    }
     for (int j = 0; j < n_sub; ++j) {
         float* acc = (float*)step_grids[j];
         float* gvj = acc + 4 * nn;
@@ -920,16 +922,16 @@ int mpm_adjoint_segment(const MpmParams* prm, const MpmState* states, const void
     int rc;
     if ((rc = check_segment(prm, states, step_grids, n_sub))) return rc;
     if (!adj_a || !adj_b || adj_a == adj_b) return set_err(-30, "adjoint buffers invalid");
-    if ((rc = record_segment(prm, states, order, step_grids, colliders, n_colliders, k0, dt, n_sub, nullptr, nullptr,
-                             stream)))
+    if ((rc = record_segment(prm, states, order, step_grids, colliders, n_colliders, k0, dt, n_sub, 0, nullptr,
+                             nullptr, stream)))
         return rc;
     return adjoint_recorded(prm, states, order, step_grids, colliders, n_colliders, workspace, k0, dt, n_sub, adj_a,
                             adj_b, adj_stride, param_adj, stream);
 }
 
 int mpm_record_segment(const MpmParams* prm, const MpmState* states, const void* order, void* const* step_grids,
-                       const MpmCollider* colliders, int n_colliders, int k0, double dt, int n_sub,
-                       const MpmState* last, void* workspace, int64_t workspace_bytes, void* stream) {
+                       const MpmCollider* colliders, int n_colliders, int k0, double dt, int n_sub, int p2g_done,
+                       const MpmState* last, void* next_grid, int64_t next_grid_bytes, void* stream) {
     int rc;
     if ((rc = check_segment(prm, states, step_grids, n_sub))) return rc;
     if (states[0].n < 0 || states[0].stride < states[0].n || states[0].n > 0x7fffffffLL)
@@ -941,11 +943,13 @@ int mpm_record_segment(const MpmParams* prm, const MpmState* states, const void*
             return set_err(-32, "segment: the last state must share n and stride");
         for (int j = 0; j < n_sub; ++j)
             if (states[j].planes == last->planes) return set_err(-32, "segment: the last state aliases a state");
-        if (!workspace || ((uintptr_t)workspace & 255) || workspace_bytes < 16 * nn)
-            return set_err(-10, "segment: workspace must be one 256-byte aligned float4 grid");
+        if (!next_grid || ((uintptr_t)next_grid & 255) || next_grid_bytes < 16 * nn)
+            return set_err(-10, "segment: next_grid must be one 256-byte aligned float4 grid");
+        for (int j = 0; j < n_sub; ++j)
+            if (step_grids[j] == next_grid) return set_err(-10, "segment: next_grid aliases a step grid");
     }
-    return record_segment(prm, states, order, step_grids, colliders, n_colliders, k0, dt, n_sub, last,
-                          (float*)workspace, stream);
+    return record_segment(prm, states, order, step_grids, colliders, n_colliders, k0, dt, n_sub, p2g_done, last,
+                          (float*)next_grid, stream);
 }
 
 int mpm_adjoint_recorded(const MpmParams* prm, const MpmState* states, const void* order, void* const* step_grids,

@@ -48,7 +48,8 @@ constexpr int kTG3 = kTG * kTG * kTG;
 constexpr int kTS3 = kTS * kTS * kTS;
 constexpr int kSortMax = 6144;        // bins up to this size are processed in bank order
 constexpr int kSortPeriod = 32;       // substeps between bank-order sorts of the bins
-constexpr int kStageFloats = 16384;   // LDS staging of the sort's copy (64 KB)
+constexpr int kStageFloats = 6144;    // LDS staging of the sort's copy (24 KB: two bins per CU)
+static_assert(kStageFloats >= kSortMax, "a bin's plane must fit the staging buffer");
 
 // Plane sets are bin-major: bin b of a scene with `cap` slots per bin is a
 // block of MPM_NPLANES planes of cap floats starting at float offset

@@ -60,19 +60,52 @@ __device__ __forceinline__ int key_run(int key, int* first) {
     return next - *first;
 }
 
+// Workgroup aggregation of the sort's counters.  The keys of a workgroup's
+// particles mostly lie in a narrow range (a state sorted an earlier segment
+// ago has moved little), so each workgroup counts its keys in LDS over a
+// window of kWin keys from its smallest key and takes one global atomic per
+// key present, instead of one per run of equal keys in a wave (bank order
+// inside chunks leaves runs of one or two particles, and thousands of waves
+// hitting the same few counters serialize).  Keys outside the window (an
+// unsorted state) take one atomic per run of equal keys in a wave.
+constexpr int kWin = 512;
+
+// smallest valid key of the workgroup (INT_MAX if none) in *s_kmin, which
+// thread 0 set to INT_MAX before a barrier; a barrier must follow
+__device__ __forceinline__ void wg_min_key(int key, int* s_kmin) {
+    int k = key >= 0 ? key : INT_MAX;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) k = min(k, __shfl_xor(k, o));
+    if (__lane_id() == 0) atomicMin(s_kmin, k);
+}
+
 __global__ void __launch_bounds__(kB)
     k_order_hist(const float* __restrict__ planes, int64_t S, int n, float inv_dx, int ng, int nb,
                  int* __restrict__ hist) {
-    const int p = blockIdx.x * kB + threadIdx.x;
+    __shared__ int s_cnt[kWin];
+    __shared__ int s_kmin;
+    const int tid = threadIdx.x;
+    const int p = blockIdx.x * kB + tid;
     int key = -1;
     if (p < n) {
         int b[3];
         stencil_base(planes + p, S, inv_dx, b);
         key = order_key(b, ng, nb);
     }
+    if (tid == 0) s_kmin = INT_MAX;
+    for (int t = tid; t < kWin; t += kB) s_cnt[t] = 0;
+    __syncthreads();
+    wg_min_key(key, &s_kmin);
+    __syncthreads();
+    const int kmin = s_kmin;
+    const bool in = key >= 0 && key - kmin < kWin;
+    if (in) atomicAdd(&s_cnt[key - kmin], 1);
     int first;
-    const int len = key_run(key, &first);
-    if (key >= 0 && __lane_id() == first) atomicAdd(&hist[key], len);
+    const int len = key_run(in ? -2 : key, &first);
+    if (!in && key >= 0 && __lane_id() == first) atomicAdd(&hist[key], len);
+    __syncthreads();
+    for (int t = tid; t < kWin; t += kB)
+        if (s_cnt[t] > 0) atomicAdd(&hist[kmin + t], s_cnt[t]);
 }
 
 // One workgroup: exclusive offsets of the key counts (in place, and copied
@@ -127,20 +160,35 @@ __global__ void __launch_bounds__(kScan)
 __global__ void __launch_bounds__(kB)
     k_order_scatter(const float* __restrict__ src, int64_t SS, float* __restrict__ dst, int64_t DS, int n,
                     float inv_dx, int ng, int nb, int* __restrict__ cursor, int* __restrict__ perm) {
-    const int p = blockIdx.x * kB + threadIdx.x;
+    __shared__ int s_cnt[kWin], s_base[kWin];
+    __shared__ int s_kmin;
+    const int tid = threadIdx.x;
+    const int p = blockIdx.x * kB + tid;
     int key = -1;
     if (p < n) {
         int b[3];
         stencil_base(src + p, SS, inv_dx, b);
         key = order_key(b, ng, nb);
     }
+    if (tid == 0) s_kmin = INT_MAX;
+    for (int t = tid; t < kWin; t += kB) s_cnt[t] = 0;
+    __syncthreads();
+    wg_min_key(key, &s_kmin);
+    __syncthreads();
+    const int kmin = s_kmin;
+    const bool in = key >= 0 && key - kmin < kWin;
+    const int local = in ? atomicAdd(&s_cnt[key - kmin], 1) : 0;   // rank among the workgroup's equal keys
     int first;
-    const int len = key_run(key, &first);
+    const int len = key_run(in ? -2 : key, &first);
     int base = 0;
-    if (key >= 0 && __lane_id() == first) base = atomicAdd(&cursor[key], len);
+    if (!in && key >= 0 && __lane_id() == first) base = atomicAdd(&cursor[key], len);
     base = __shfl(base, first);
+    __syncthreads();
+    for (int t = tid; t < kWin; t += kB)
+        if (s_cnt[t] > 0) s_base[t] = atomicAdd(&cursor[kmin + t], s_cnt[t]);
+    __syncthreads();
     if (key < 0) return;
-    const int pos = base + (__lane_id() - first);   // a run keeps its order
+    const int pos = in ? s_base[key - kmin] + local : base + (__lane_id() - first);
     if (pos < 0 || pos >= n) return;   // cannot happen: same keys as the histogram
 #pragma unroll 13
     for (int k = 0; k < MPM_NPLANES; ++k) dst[(int64_t)k * DS + pos] = src[(int64_t)k * SS + p];
