@@ -302,8 +302,9 @@ int mpm_adjoint_recorded(const MpmParams* prm, const MpmState* states, const voi
  * the 8^3-cell bin and 4^3-cell sub-block of each particle's stencil base
  * (mpm_utils.py:197-199), then puts each chunk in LDS bank order, into `dst`
  * and fills `order` (mpm_order_bytes
- * bytes, 256-byte aligned): the chunk table (runs of at most 512 particles,
- * 1024 from 500k particles up, inside one bin) and the permutation
+ * bytes, 256-byte aligned): the chunk table (runs of at most 256, 512 or
+ * 1024 particles -- the largest that still gives ~900 chunks -- inside one
+ * bin) and the permutation
  * perm[sorted slot] = source slot (int32, at byte offset
  * mpm_order_perm_offset).  The order stays valid for any state
  * in the same slot order -- e.g. the substeps after the sort -- as particles

@@ -25,10 +25,12 @@ namespace mpm {
 constexpr int kChunkWG = 256;   // threads per chunk workgroup
 constexpr int kChunkMax = 1024; // the largest chunk
 
-// Particles per chunk: 512 below half a million particles (more workgroups
-// than CUs on small scenes), 1024 above (measured at 1M: the G2P adjoint
-// 24% and the P2G 7% faster than with 512; 2048 is slower again).
-inline int chunk_size(int64_t n) { return n >= 500000 ? 1024 : 512; }
+// Particles per chunk: the largest of 1024, 512 and 256 that still gives
+// about 900 chunks (three or more workgroups per CU).  Measured on the sand
+// column: at 1M particles on 128^3, 1024 beats 512 by 5% in the forward
+// substep and 256 by 27%; at 250k on 64^3, 256 beats 512 and 1024 by 10% in
+// both the forward and the adjoint substep.
+inline int chunk_size(int64_t n) { return n >= 900 * 1024 ? 1024 : (n >= 900 * 512 ? 512 : 256); }
 constexpr int kCT = 12;         // LDS tile edge in nodes: bases o .. o+9 fit
 constexpr int kCT3 = kCT * kCT * kCT;
 

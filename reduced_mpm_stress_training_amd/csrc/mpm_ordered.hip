@@ -6,7 +6,7 @@
 // CU, it is the cost of a flat substep.  Here the flat state is first
 // counting-sorted by bin and sub-block of its stencil base (k_order_hist,
 // k_order_scan, k_order_scatter), which also yields a table of chunks --
-// runs of at most 512 or 1024 particles inside one bin.  k_p2g_chunks then gives
+// runs of at most 256, 512 or 1024 particles inside one bin.  k_p2g_chunks then gives
 // each chunk one workgroup that scatters into an FP64 LDS tile placed at the
 // chunk's smallest stencil base (ds_add_f64) and flushes the tile with one
 // float atomic per touched node and component.  The order stays valid while
