@@ -164,7 +164,9 @@ class Rollout:
                                           float(k * self.dt), self.dt, self._stream()), "mpm_step_ordered")
 
     def _permute(self, src, n_planes, idx_ptr, scatter):
-        dst = torch.zeros((n_planes, self.n), dtype=torch.float32, device=self.device)
+        # idx is a permutation of the n slots (a sort's or the pid plane), so
+        # every slot of dst is written
+        dst = torch.empty((n_planes, self.n), dtype=torch.float32, device=self.device)
         L.check(L.load().mpm_permute(src.data_ptr(), src.shape[1], dst.data_ptr(), self.n, n_planes, idx_ptr, self.n,
                                      int(scatter), self._stream()), "mpm_permute")
         return dst

@@ -546,8 +546,8 @@ __global__ void __launch_bounds__(kChunkWG)
 // normalization, on the recomputed P2G grid acc (mv, m).
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kBlk)
-    k_grid_adj(const float4* __restrict__ acc, const float* __restrict__ vb, float4* __restrict__ gb, MpmParams prm,
-               ColliderSet cs, float time, float dt) {
+    k_grid_adj(const float4* __restrict__ acc, float* __restrict__ vb, float4* __restrict__ gb, MpmParams prm,
+               ColliderSet cs, float time, float dt, int clear_vb) {
     const int64_t nn = (int64_t)prm.n_grid * prm.n_grid * prm.n_grid;
     const int64_t t = (int64_t)blockIdx.x * kBlk + threadIdx.x;
     if (t >= nn) return;
@@ -588,6 +588,9 @@ __global__ void __launch_bounds__(kBlk)
         out.w = -(b0 * a4.x + b1 * a4.y + b2 * a4.z) * im * im;
     }
     gb[t] = out;
+    // the velocity adjoint the next (earlier) substep scatters into, cleared
+    // here instead of by its own launch
+    if (clear_vb) reinterpret_cast<float4*>(vb)[t] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
 // ---------------------------------------------------------------------------
@@ -795,9 +798,13 @@ int64_t mpm_adjoint_workspace_bytes(const MpmParams* prm) {
     return 4 * nn * 16;   // acc (mv, m), grid v, v adjoint, (mv, m) adjoint
 }
 
-int mpm_adjoint_substep(const MpmParams* prm, const MpmState* st, const void* order, const void* step_grids,
-                        const MpmCollider* colliders, int n_colliders, void* workspace, float time, float dt,
-                        const float* adj_next, float* adj_cur, int64_t adj_stride, double* param_adj, void* stream) {
+// mpm_adjoint_substep; vb_clear: the workspace's velocity-adjoint grid is
+// already zero (cleared by the previous call's k_grid_adj); clear_after: leave
+// it zero for the next call.
+static int adjoint_substep(const MpmParams* prm, const MpmState* st, const void* order, const void* step_grids,
+                           const MpmCollider* colliders, int n_colliders, void* workspace, float time, float dt,
+                           const float* adj_next, float* adj_cur, int64_t adj_stride, double* param_adj,
+                           bool vb_clear, bool clear_after, void* stream) {
     int rc = check_params(prm);
     if (rc) return rc;
     if (!st || !st->planes || st->n < 0 || st->stride < st->n) return set_err(-4, "state invalid");
@@ -825,8 +832,10 @@ int mpm_adjoint_substep(const MpmParams* prm, const MpmState* st, const void* or
         if ((rc = mpm_grid_update(prm, (const float*)acc, (float*)gv, colliders, n_colliders, time, dt, stream)))
             return rc;
     }
-    hipLaunchKernelGGL(k_zero4, dim3(blocks(nn)), dim3(kBlk), 0, s, vbg, nn);
-    if ((rc = check_launch("k_zero4"))) return rc;
+    if (!vb_clear) {
+        hipLaunchKernelGGL(k_zero4, dim3(blocks(nn)), dim3(kBlk), 0, s, vbg, nn);
+        if ((rc = check_launch("k_zero4"))) return rc;
+    }
     const int n = (int)st->n;
     if (n > 0) {
         const OrderLayout L = order_layout(*prm, n);
@@ -837,8 +846,8 @@ int mpm_adjoint_substep(const MpmParams* prm, const MpmState* st, const void* or
                            (float*)vbg, adj_cur, param_adj);
         if ((rc = check_launch("k_g2p_adj_chunks"))) return rc;
     }
-    hipLaunchKernelGGL(k_grid_adj, dim3(blocks(nn)), dim3(kBlk), 0, s, (const float4*)acc, (const float*)vbg, gbg,
-                       *prm, cs, time, dt);
+    hipLaunchKernelGGL(k_grid_adj, dim3(blocks(nn)), dim3(kBlk), 0, s, (const float4*)acc, (float*)vbg, gbg,
+                       *prm, cs, time, dt, clear_after ? 1 : 0);
     if ((rc = check_launch("k_grid_adj"))) return rc;
     if (n > 0) {
         hipLaunchKernelGGL(k_p2g_adj, dim3(blocks(n)), dim3(kBlk), 0, s, (const float*)st->planes, st->stride, n,
@@ -846,6 +855,13 @@ int mpm_adjoint_substep(const MpmParams* prm, const MpmState* st, const void* or
         if ((rc = check_launch("k_p2g_adj"))) return rc;
     }
     return 0;
+}
+
+int mpm_adjoint_substep(const MpmParams* prm, const MpmState* st, const void* order, const void* step_grids,
+                        const MpmCollider* colliders, int n_colliders, void* workspace, float time, float dt,
+                        const float* adj_next, float* adj_cur, int64_t adj_stride, double* param_adj, void* stream) {
+    return adjoint_substep(prm, st, order, step_grids, colliders, n_colliders, workspace, time, dt, adj_next, adj_cur,
+                           adj_stride, param_adj, false, false, stream);
 }
 
 // shared checks of a segment's states and per-substep workspaces
@@ -881,12 +897,12 @@ static int record_segment(const MpmParams* prm, const MpmState* states, const vo
     for (int j = 0; j < n_sub; ++j) {
         float* acc = (float*)step_grids[j];
         float* gvj = acc + 4 * nn;
-        if ((rc = mpm_grid_update(prm, acc, gvj, colliders, n_colliders, (float)((double)(k0 + j) * dt), fdt, stream)))
-            return rc;
         const bool more = j + 1 < n_sub;
-        if (!more && !last) break;
-        float* acc_next = more ? (float*)step_grids[j + 1] : last_acc;
-        if ((rc = mpm_zero_grid(prm, acc_next, stream))) return rc;
+        float* acc_next = more ? (float*)step_grids[j + 1] : (last ? last_acc : nullptr);   // NULL: none after
+        if ((rc = grid_update_zero_next(prm, acc, gvj, colliders, n_colliders, (float)((double)(k0 + j) * dt), fdt,
+                                        acc_next, stream)))
+            return rc;
+        if (!acc_next) break;
         if ((rc = g2p2g_ordered(prm, &states[j], more ? &states[j + 1] : last, order, gvj, acc_next, fdt, stream)))
             return rc;
     }
@@ -904,9 +920,9 @@ static int adjoint_recorded(const MpmParams* prm, const MpmState* states, const 
     float* cur = adj_a;
     float* nxt = adj_b;
     for (int j = n_sub - 1; j >= 0; --j) {
-        if ((rc = mpm_adjoint_substep(prm, &states[j], order, step_grids[j], colliders, n_colliders, workspace,
-                                      (float)((double)(k0 + j) * dt), (float)dt, cur, nxt, adj_stride, param_adj,
-                                      stream)))
+        if ((rc = adjoint_substep(prm, &states[j], order, step_grids[j], colliders, n_colliders, workspace,
+                                  (float)((double)(k0 + j) * dt), (float)dt, cur, nxt, adj_stride, param_adj,
+                                  j < n_sub - 1, j > 0, stream)))
             return rc;
         float* t = cur; cur = nxt; nxt = t;
     }

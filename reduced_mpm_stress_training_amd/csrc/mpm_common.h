@@ -19,6 +19,10 @@ struct ColliderSet {
     int n;
 };
 int make_collider_set(const MpmCollider* cols, int n, ColliderSet* cs);
+// mpm_grid_update that also clears `zero_next` (a float4 grid of the same
+// size, or NULL): the accumulator the next substep's P2G scatters into
+int grid_update_zero_next(const MpmParams* prm, const float* grid_acc, float* grid_v, const MpmCollider* colliders,
+                          int n_colliders, float time, float dt, float* zero_next, void* stream);
 
 // grid_normalization_and_gravity (mpm_utils.py:229-237) then each active
 // collider in registration order (mpm_solver_warp.py:182-185):

@@ -155,7 +155,7 @@ __global__ void __launch_bounds__(kBlock)
 
 __global__ void __launch_bounds__(kBlock)
     k_grid_update(const float4* __restrict__ acc, float4* __restrict__ gv, MpmParams prm,
-                  ColliderSet cs, float time, float dt) {
+                  ColliderSet cs, float time, float dt, float4* __restrict__ zero_next) {
     const int ng = prm.n_grid;
     int64_t node = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     int64_t nn = (int64_t)ng * ng * ng;
@@ -164,6 +164,8 @@ __global__ void __launch_bounds__(kBlock)
     int gy = (int)((node / ng) % ng);
     int gx = (int)(node / ((int64_t)ng * ng));
     gv[node] = grid_node_update(acc[node], gx, gy, gz, ng, prm.dx, dt, prm.gravity, cs, time);
+    // the next substep's accumulator, cleared here instead of by its own launch
+    if (zero_next) zero_next[node] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
 // --------------------------------------------------------------------------
@@ -268,6 +270,24 @@ __global__ void __launch_bounds__(kBlock)
 // ==========================================================================
 // C-ABI
 // ==========================================================================
+namespace mpm {
+
+int grid_update_zero_next(const MpmParams* prm, const float* grid_acc, float* grid_v, const MpmCollider* colliders,
+                          int n_colliders, float time, float dt, float* zero_next, void* stream) {
+    int rc = check_params(prm);
+    if (rc) return rc;
+    if (!grid_acc || !grid_v) return set_err(-7, "grid is NULL");
+    if (zero_next && (zero_next == grid_acc || zero_next == grid_v)) return set_err(-7, "grid aliases the next one");
+    ColliderSet cs;
+    if ((rc = make_collider_set(colliders, n_colliders, &cs))) return rc;
+    int64_t nn = (int64_t)prm->n_grid * prm->n_grid * prm->n_grid;
+    hipLaunchKernelGGL(k_grid_update, dim3(grid_1d(nn)), dim3(kBlock), 0, (hipStream_t)stream,
+                       (const float4*)grid_acc, (float4*)grid_v, *prm, cs, time, dt, (float4*)zero_next);
+    return check_launch("k_grid_update");
+}
+
+}  // namespace mpm
+
 extern "C" {
 
 int mpm_abi_version(void) { return MPM_ABI_VERSION; }
@@ -299,15 +319,7 @@ int mpm_p2g(const MpmParams* prm, const MpmState* st, float* grid_acc, float dt,
 int mpm_grid_update(const MpmParams* prm, const float* grid_acc, float* grid_v,
                     const MpmCollider* colliders, int n_colliders, float time, float dt,
                     void* stream) {
-    int rc = check_params(prm);
-    if (rc) return rc;
-    if (!grid_acc || !grid_v) return set_err(-7, "grid is NULL");
-    ColliderSet cs;
-    if ((rc = make_collider_set(colliders, n_colliders, &cs))) return rc;
-    int64_t nn = (int64_t)prm->n_grid * prm->n_grid * prm->n_grid;
-    hipLaunchKernelGGL(k_grid_update, dim3(grid_1d(nn)), dim3(kBlock), 0, (hipStream_t)stream,
-                       (const float4*)grid_acc, (float4*)grid_v, *prm, cs, time, dt);
-    return check_launch("k_grid_update");
+    return mpm::grid_update_zero_next(prm, grid_acc, grid_v, colliders, n_colliders, time, dt, nullptr, stream);
 }
 
 int mpm_g2p(const MpmParams* prm, const MpmState* st, const float* grid_v, float dt,

@@ -659,10 +659,9 @@ int mpm_run_ordered(const MpmParams* prm, const MpmState* st, const void* order,
     if ((rc = mpm_zero_grid(prm, acc, stream))) return rc;
     if ((rc = p2g_ordered(prm, st, order, acc, (float)dt, stream))) return rc;
     for (int j = 0; j < n_sub; ++j) {
-        if ((rc = mpm_grid_update(prm, acc, gv, colliders, n_colliders, (float)((double)(k0 + j) * dt), (float)dt,
-                                  stream)))
+        if ((rc = grid_update_zero_next(prm, acc, gv, colliders, n_colliders, (float)((double)(k0 + j) * dt),
+                                        (float)dt, acc2, stream)))
             return rc;
-        if ((rc = mpm_zero_grid(prm, acc2, stream))) return rc;
         if ((rc = g2p2g_ordered(prm, st, st, order, gv, acc2, (float)dt, stream))) return rc;
         float* t = acc; acc = acc2; acc2 = t;
     }
