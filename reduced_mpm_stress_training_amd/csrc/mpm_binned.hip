@@ -246,6 +246,7 @@ struct SubstepArgs {
     int* __restrict__ stamps;
     const int* __restrict__ order;  // schedule (schedule_bins)
     int sched_epoch;                // epoch the schedule must carry to be used
+    int store_all;                  // 0: write only the planes the next substep reads (see g2p2g_particle)
 };
 
 // Destination slot of every active lane.  A bin's workgroup reserves
@@ -285,26 +286,32 @@ __device__ __forceinline__ int64_t reserve_slot(bool act, int new_bin, int own_b
     return -1;
 }
 
+// all = false: only the planes the next substep's G2P reads (x0, u, F, m,
+// vol, pid); v, C and tau are then stale in this plane set.
 __device__ __forceinline__ void store_particle(float* __restrict__ dst, int64_t off, int64_t stride,
                                                const float x0[3], const float u[3], const float v[3],
                                                const float C[9], const float F[9], const float tau[9],
-                                               float m, float vol, float pid) {
+                                               float m, float vol, float pid, bool all) {
     float* p = dst + off;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         p[(MPM_P_X0 + d) * stride] = x0[d];
         p[(MPM_P_U + d) * stride] = u[d];
-        p[(MPM_P_V + d) * stride] = v[d];
     }
 #pragma unroll
-    for (int k = 0; k < 9; ++k) {
-        p[(MPM_P_C + k) * stride] = C[k];
-        p[(MPM_P_F + k) * stride] = F[k];
-        p[(MPM_P_TAU + k) * stride] = tau[k];
-    }
+    for (int k = 0; k < 9; ++k) p[(MPM_P_F + k) * stride] = F[k];
     p[MPM_P_MASS * stride] = m;
     p[MPM_P_VOL * stride] = vol;
     p[MPM_P_PID * stride] = pid;
+    if (all) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) p[(MPM_P_V + d) * stride] = v[d];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            p[(MPM_P_C + k) * stride] = C[k];
+            p[(MPM_P_TAU + k) * stride] = tau[k];
+        }
+    }
 }
 
 // The per-particle body of k_g2p2g.  `tile_vel`/`tile_acc` are the bin's LDS
@@ -425,7 +432,14 @@ __device__ __forceinline__ void g2p2g_particle(const SubstepArgs& a, const Scene
     }
     int64_t dstride = 0;
     const int64_t dslot = reserve_slot(act, new_bin, own_bin, stay_base, s_stay, a, sc, &dstride);
-    if (dslot >= 0) store_particle(a.dst, dslot, dstride, x0, u, v, Cm, F, tau, m, vol, pid);
+    // v, C and tau are no input of the next substep's G2P, and this kernel
+    // scatters the next P2G from registers: between the substeps of one run
+    // only x0, u, F, m, vol, pid are written.  The last substep of a run
+    // (store_all) writes every plane, and so do frozen particles and particles
+    // that left the grid (mode 0: they are copied from the source planes in
+    // the substeps after, so those planes must hold their values).
+    if (dslot >= 0)
+        store_particle(a.dst, dslot, dstride, x0, u, v, Cm, F, tau, m, vol, pid, a.store_all || mode == 0);
     // P2G of the next substep, after the store so that x0, u, F are dead
     if (mode == 1) {
         scatter_tile(tile_acc, i0, A, m, vol, v, Cm, tau, prm.dx, prm.inv_dx, dt);
@@ -1072,7 +1086,9 @@ int op_grid(const Ctx& c, const MpmBins* bins, float time, float dt, hipStream_t
     return check_launch("k_grid_bins");
 }
 
-int op_g2p2g(const Ctx& c, MpmBins* bins, float dt, hipStream_t s) {
+// store_all = false only for substeps followed by another one of the same
+// run (no caller sees the plane set in between)
+int op_g2p2g(const Ctx& c, MpmBins* bins, float dt, hipStream_t s, bool store_all = true) {
     const int cur = bins->cur, nxt = 1 - cur;
     SubstepArgs a;
     a.tab = c.tab;
@@ -1091,6 +1107,7 @@ int op_g2p2g(const Ctx& c, MpmBins* bins, float dt, hipStream_t s) {
     a.stamps = c.w.stamps;
     a.order = c.w.order;
     a.sched_epoch = bins->epoch;
+    a.store_all = store_all ? 1 : 0;
     if (bins->wg_threads == 256)
         hipLaunchKernelGGL(k_g2p2g<256>, dim3(c.g.n_bins + ovf_wgs(c.g, 256)), dim3(256), 0, s, a);
     else
@@ -1120,7 +1137,8 @@ int op_sort(const Ctx& c, MpmBins* bins, hipStream_t s) {
     return 0;
 }
 
-int op_substep(const Ctx& c, MpmBins* bins, float time, float dt, hipStream_t s, const double* t_base = nullptr) {
+int op_substep(const Ctx& c, MpmBins* bins, float time, float dt, hipStream_t s, const double* t_base = nullptr,
+               bool store_all = true) {
     int rc = op_grid(c, bins, time, dt, s, t_base);
     if (rc) return rc;
     // particles drift across cells and out of bank order; restore it now and
@@ -1128,14 +1146,14 @@ int op_substep(const Ctx& c, MpmBins* bins, float time, float dt, hipStream_t s,
     // kernel, which reads the sorted set
     const int period = bins->sort_period == 0 ? kSortPeriod : bins->sort_period;
     if (++bins->since_sort >= period && period > 0 && (rc = op_sort(c, bins, s))) return rc;
-    return op_g2p2g(c, bins, dt, s);
+    return op_g2p2g(c, bins, dt, s, store_all);
 }
 
 int op_run(const Ctx& c, MpmBins* bins, double time, float dt, int n_substeps, hipStream_t s) {
     if (n_substeps < 0) return set_err(-26, "n_substeps < 0");
     double t = time;
     for (int k = 0; k < n_substeps; ++k) {
-        int rc = op_substep(c, bins, (float)t, dt, s);
+        int rc = op_substep(c, bins, (float)t, dt, s, nullptr, k == n_substeps - 1);
         if (rc) return rc;
         t = t + (double)dt;
     }
@@ -1180,7 +1198,7 @@ int op_graph_create(const Ctx& c, const void* ws_id, const MpmBins* bins, float 
     rc = check_launch("k_stamp_rebase");
     double off = 0.0;
     for (int k = 0; k < n_substeps && !rc; ++k) {
-        rc = op_substep(c, &b, (float)off, dt, cs, time_base(c));
+        rc = op_substep(c, &b, (float)off, dt, cs, time_base(c), k == n_substeps - 1);
         off = off + (double)dt;
     }
     if (!rc) {
@@ -1344,7 +1362,7 @@ int mpm_bins_profile(const MpmParams* prm, MpmBins* bins, const MpmCollider* col
         (void)hipEventRecord(ev[1], s);
         if (rc == 0 && ++bins->since_sort >= kSortPeriod) rc = op_sort(c, bins, s);
         (void)hipEventRecord(ev[2], s);
-        if (rc == 0) rc = op_g2p2g(c, bins, dt, s);
+        if (rc == 0) rc = op_g2p2g(c, bins, dt, s, k == n_substeps - 1);
         (void)hipEventRecord(ev[3], s);
         (void)hipEventSynchronize(ev[3]);
         float a = 0.f, b = 0.f, cc = 0.f;

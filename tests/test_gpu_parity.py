@@ -214,6 +214,43 @@ def test_binned_step_by_step_matches_run_and_reloads(oracle):
     assert int(st[0]) == 0
 
 
+def test_run_keeps_the_full_state_between_substeps():
+    """Inside one run() call the fused kernel writes v, C and tau only at the
+    run's last substep (and always for frozen particles): exports after
+    run(n) match n single substeps, and frozen particles (stencil outside the
+    grid) keep their loaded v, C and tau bit for bit across runs that include
+    a bank-order sort."""
+    base = S.column_lattice(n_grid=32, per_axis=2, seed=5)
+    rng = np.random.default_rng(11)
+    n_frozen = 100
+    xf = np.column_stack([np.full(n_frozen, 0.98), rng.uniform(0.3, 0.7, n_frozen),   # base 30 > n - 3: frozen
+                          rng.uniform(0.1, 0.5, n_frozen)]).astype(np.float32)
+    sc = S.SandScene(name="frozen", n_grid=32, x0=np.concatenate([base.x0, xf]),
+                     vol=np.concatenate([base.vol, np.full(n_frozen, base.vol[0], np.float32)]),
+                     v0=np.zeros((base.n_particles + n_frozen, 3), np.float32))
+    n = sc.n_particles
+    extra = dict(v=rng.normal(0, 0.2, (n, 3)).astype(np.float32),
+                 C=rng.normal(0, 2.0, (n, 3, 3)).astype(np.float32),
+                 tau=rng.normal(0, 500.0, (n, 3, 3)).astype(np.float32),
+                 F=(np.eye(3)[None] + 0.01 * rng.normal(size=(n, 3, 3))).astype(np.float32))
+    a, b = S.make_simulator(sc), S.make_simulator(sc)
+    for sim in (a, b):
+        _load_extra(sim, extra)
+    a.run(40, sc.dt)
+    for k in range(40):
+        b.p2g2p(k, sc.dt)
+    ga, gb = _gpu_state(a), _gpu_state(b)
+    spread = dict(u=1e-4, v=2e-5, C=2e-4, F=1e-6, tau=2e-2)
+    for k in FIELDS:
+        assert np.isfinite(ga[k]).all(), k
+        assert rel_l2(ga[k], gb[k]) < spread[k], (k, rel_l2(ga[k], gb[k]))
+    fz = slice(base.n_particles, n)
+    assert np.array_equal(ga["v"][fz], extra["v"][fz])
+    assert np.array_equal(ga["C"][fz].reshape(-1, 9), extra["C"][fz].reshape(-1, 9))
+    assert np.array_equal(ga["tau"][fz].reshape(-1, 9), extra["tau"][fz].reshape(-1, 9))
+    assert np.array_equal(ga["u"][fz], np.zeros((n_frozen, 3), np.float32))
+
+
 def test_loader_export_roundtrip():
     sc = S.column_lattice(n_grid=16, per_axis=1)
     sim = S.make_simulator(sc)
