@@ -458,42 +458,23 @@ __global__ void __launch_bounds__(kChunkWG)
     __shared__ double s_t[3 * kCT3];
     __shared__ float s_v[3 * kCT3];
     __shared__ float s_par[3 * (kChunkWG / 64)];
-    __shared__ int s_lo[3];
     const int c = blockIdx.x;
     if (c >= meta[0]) return;
     const int4 ch = chunks[c];
     const int tid = threadIdx.x;
     const int ng = prm.n_grid;
     const int n_per = min(ch.y, kChunkMax);
-    if (tid < 3) s_lo[tid] = INT_MAX;
     for (int t = tid; t < 3 * kCT3; t += kChunkWG) s_t[t] = 0.0;
-    __syncthreads();
-    int lo[3] = {INT_MAX, INT_MAX, INT_MAX};
-    for (int i = tid; i < n_per; i += kChunkWG) {
-        const int p = ch.x + i;
-        if (p < 0 || p >= n) continue;
-        int b[3];
-#pragma unroll
-        for (int d = 0; d < 3; ++d)
-            b[d] = (int)((st[(MPM_P_X0 + d) * S + p] + st[(MPM_P_U + d) * S + p]) * prm.inv_dx - 0.5f);
-        if (b[0] < 0 || b[1] < 0 || b[2] < 0 || b[0] > ng - 3 || b[1] > ng - 3 || b[2] > ng - 3) continue;
-#pragma unroll
-        for (int d = 0; d < 3; ++d) lo[d] = min(lo[d], b[d]);
-    }
-    chunk_min_base(s_lo, lo[0], lo[1], lo[2]);
-    __syncthreads();
-    const int o[3] = {s_lo[0], s_lo[1], s_lo[2]};
-    const bool any = o[0] != INT_MAX;
-    if (any) {
-        // the chunk's grid velocities (nodes outside the grid read as 0)
-        for (int t = tid; t < kCT3; t += kChunkWG) {
-            const int x = o[0] + t / (kCT * kCT), y = o[1] + (t / kCT) % kCT, z = o[2] + t % kCT;
-            float4 g = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (x < ng && y < ng && z < ng) g = gv[((int64_t)x * ng + y) * ng + z];
-            s_v[t] = g.x;
-            s_v[kCT3 + t] = g.y;
-            s_v[2 * kCT3 + t] = g.z;
-        }
+    int o[3];
+    chunk_origin(ch.z, (ng + 7) / 8, o);
+    // the chunk's grid velocities (nodes outside the grid read as 0)
+    for (int t = tid; t < kCT3; t += kChunkWG) {
+        const int x = o[0] + t / (kCT * kCT), y = o[1] + (t / kCT) % kCT, z = o[2] + t % kCT;
+        float4 g = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (x >= 0 && y >= 0 && z >= 0 && x < ng && y < ng && z < ng) g = gv[((int64_t)x * ng + y) * ng + z];
+        s_v[t] = g.x;
+        s_v[kCT3 + t] = g.y;
+        s_v[2 * kCT3 + t] = g.z;
     }
     __syncthreads();
     float mub = 0.f, lamb = 0.f, alb = 0.f;
@@ -513,7 +494,7 @@ __global__ void __launch_bounds__(kChunkWG)
             frozen_adj(an, SA, p, ac);
             continue;
         }
-        const int ti = any ? chunk_tile_index(Aw.base, o) : -1;
+        const int ti = chunk_tile_index(Aw.base, o);
         if (ti < 0) {
             outside |= 1u << r;
             continue;
@@ -538,7 +519,7 @@ __global__ void __launch_bounds__(kChunkWG)
     block_stage_params(mub, lamb, alb, s_par);
     __syncthreads();
     block_flush_params(s_par, pb);
-    if (any) chunk_flush<3>(s_t, o, ng, vb);
+    chunk_flush<3>(s_t, o, ng, vb);
 }
 
 // ---------------------------------------------------------------------------

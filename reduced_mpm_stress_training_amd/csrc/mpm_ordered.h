@@ -6,13 +6,14 @@
 // particle's stencil base (mpm_utils.py:197-199) and, inside a bin, by its
 // 4^3-cell sub-block, and writes a chunk table: runs of at most
 // chunk_size(n) consecutive particles that never straddle two bins.  A
-// chunk's particles then span at most ~8 cells per axis (plus what they drift
-// in the substeps after the sort), so one workgroup accumulates a whole
-// chunk's scatter into a 12^3-node FP64 LDS tile placed at the chunk's
-// smallest stencil base, and flushes each touched node to the global grid
-// with one float atomic per component -- instead of 27 atomics per particle
-// per component.  Particles
-// whose stencil leaves the tile go straight to the global grid.
+// chunk's particles then have stencil bases in their bin's 8 cells per axis
+// at sort time (plus what they drift in the substeps after), so one
+// workgroup accumulates a whole chunk's scatter into a 12^3-node FP64 LDS
+// tile placed one node below the bin (chunk_origin: bases b0 - 1 .. b0 + 8
+// fit, one cell of drift either way), and flushes each touched node to the
+// global grid with one float atomic per component -- instead of 27 atomics
+// per particle per component.  Particles whose stencil leaves the tile go
+// straight to the global grid.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <limits.h>
@@ -78,21 +79,15 @@ __device__ __forceinline__ int order_key(const int base[3], int ng, int nb) {
     return bin * 8 + sub;
 }
 
-// Componentwise minimum over the workgroup of per-thread stencil bases
-// (INT_MAX where a thread has none) into s_lo[3], which the caller set to
-// INT_MAX before a barrier; a barrier must follow before s_lo is read.
-__device__ __forceinline__ void chunk_min_base(int* s_lo, int bx, int by, int bz) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        bx = min(bx, __shfl_xor(bx, o));
-        by = min(by, __shfl_xor(by, o));
-        bz = min(bz, __shfl_xor(bz, o));
-    }
-    if ((threadIdx.x & 63) == 0) {
-        atomicMin(&s_lo[0], bx);
-        atomicMin(&s_lo[1], by);
-        atomicMin(&s_lo[2], bz);
-    }
+// Origin (node) of the LDS tiles of a chunk in bin `bin` (C order, nb bins
+// per axis): one node below the bin's first cell, so the tile's bases o ..
+// o + 9 cover the bin's bases and one cell of drift on each side.  The
+// origin may be -1; tile nodes outside the grid are neither read nor
+// flushed.
+__device__ __forceinline__ void chunk_origin(int bin, int nb, int o[3]) {
+    o[0] = 8 * (bin / (nb * nb)) - 1;
+    o[1] = 8 * ((bin / nb) % nb) - 1;
+    o[2] = 8 * (bin % nb) - 1;
 }
 
 // tile index of stencil node (0,0,0) of a particle with base b in the tile at
@@ -164,7 +159,7 @@ template <int NC>
 __device__ __forceinline__ void chunk_flush(const double* s_t, const int o[3], int ng, float* __restrict__ grid) {
     for (int t = threadIdx.x; t < kCT3; t += blockDim.x) {
         const int x = o[0] + t / (kCT * kCT), y = o[1] + (t / kCT) % kCT, z = o[2] + t % kCT;
-        if (x >= ng || y >= ng || z >= ng) continue;
+        if (x < 0 || y < 0 || z < 0 || x >= ng || y >= ng || z >= ng) continue;
         float* g = grid + 4 * (((int64_t)x * ng + y) * ng + z);
 #pragma unroll
         for (int c = 0; c < NC; ++c) {

@@ -7,8 +7,8 @@
 // counting-sorted by bin and sub-block of its stencil base (k_order_hist,
 // k_order_scan, k_order_scatter), which also yields a table of chunks --
 // runs of at most 256, 512 or 1024 particles inside one bin.  k_p2g_chunks then gives
-// each chunk one workgroup that scatters into an FP64 LDS tile placed at the
-// chunk's smallest stencil base (ds_add_f64) and flushes the tile with one
+// each chunk one workgroup that scatters into an FP64 LDS tile placed one
+// node below its bin (chunk_origin; ds_add_f64) and flushes the tile with one
 // float atomic per touched node and component.  The order stays valid while
 // particles drift (a particle whose stencil leaves its chunk's tile scatters
 // straight to the grid), so a caller sorts once and steps many substeps.
@@ -197,11 +197,10 @@ __global__ void __launch_bounds__(kB)
 
 // Bank order inside each chunk, in memory: one workgroup per chunk moves
 // the chunk's particles (all planes and the permutation) into bank order
-// (chunk_bank_order) of their stencil bases in the tile at the chunk's
-// smallest base, staging one plane at a time in LDS.  The chunk kernels then
+// (chunk_bank_order) of their stencil bases in the chunk's tile
+// (chunk_origin), staging one plane at a time in LDS.  The chunk kernels then
 // read consecutive slots (coalesced) whose 32-lane halves gather and scatter
-// through distinct LDS banks; a shift of the chunk's tile origin in later
-// substeps shifts every class by the same amount, so the order keeps.
+// through distinct LDS banks.
 __global__ void __launch_bounds__(kChunkWG)
     k_order_interleave(float* __restrict__ planes, int64_t S, int n, const int4* __restrict__ chunks,
                        const int* __restrict__ meta, float inv_dx, int ng, int* __restrict__ perm) {
@@ -209,27 +208,14 @@ __global__ void __launch_bounds__(kChunkWG)
     __shared__ float s_stage[kChunkMax];
     __shared__ int s_cnt[64];
     __shared__ int s_mul;
-    __shared__ int s_lo[3];
     const int c = blockIdx.x;
     if (c >= meta[0]) return;
     const int4 ch = chunks[c];
     const int tid = threadIdx.x;
     const int n_per = min(ch.y, kChunkMax);
     if (ch.x < 0 || ch.x + n_per > n || n_per < 2) return;
-    if (tid < 3) s_lo[tid] = INT_MAX;
-    __syncthreads();
-    int lo[3] = {INT_MAX, INT_MAX, INT_MAX};
-    for (int i = tid; i < n_per; i += kChunkWG) {
-        int b[3];
-        stencil_base(planes + ch.x + i, S, inv_dx, b);
-        if (!base_ok(b, ng)) continue;
-#pragma unroll
-        for (int d = 0; d < 3; ++d) lo[d] = min(lo[d], b[d]);
-    }
-    chunk_min_base(s_lo, lo[0], lo[1], lo[2]);
-    __syncthreads();
-    const int o[3] = {s_lo[0] == INT_MAX ? 0 : s_lo[0], s_lo[1] == INT_MAX ? 0 : s_lo[1],
-                      s_lo[2] == INT_MAX ? 0 : s_lo[2]};
+    int o[3];
+    chunk_origin(ch.z, (ng + 7) / 8, o);
     chunk_bank_order(planes, S, ch.x, n_per, n, inv_dx, o, s_ord, s_cnt, &s_mul);   // barriers inside
     for (int k = 0; k <= MPM_NPLANES; ++k) {   // plane MPM_NPLANES: the permutation
         float* pl = k < MPM_NPLANES ? planes + (int64_t)k * S + ch.x : (float*)(perm + ch.x);
@@ -248,30 +234,16 @@ __global__ void __launch_bounds__(kChunkWG)
     k_p2g_chunks(const float* __restrict__ planes, int64_t S, int n, const int4* __restrict__ chunks,
                  const int* __restrict__ meta, MpmParams prm, float dt, float* __restrict__ grid) {
     __shared__ double s_t[4 * kCT3];
-    __shared__ int s_lo[3];
     const int c = blockIdx.x;
     if (c >= meta[0]) return;
     const int4 ch = chunks[c];
     const int tid = threadIdx.x;
     const int ng = prm.n_grid;
     const int n_per = min(ch.y, kChunkMax);   // a chunk's particles, kChunkWG at a time
-    if (tid < 3) s_lo[tid] = INT_MAX;
     for (int t = tid; t < 4 * kCT3; t += kChunkWG) s_t[t] = 0.0;
+    int o[3];
+    chunk_origin(ch.z, (ng + 7) / 8, o);
     __syncthreads();
-    int lo[3] = {INT_MAX, INT_MAX, INT_MAX};
-    for (int i = tid; i < n_per; i += kChunkWG) {
-        const int p = ch.x + i;
-        if (p < 0 || p >= n) continue;
-        int b[3];
-        stencil_base(planes + p, S, prm.inv_dx, b);
-        if (!base_ok(b, ng)) continue;
-#pragma unroll
-        for (int d = 0; d < 3; ++d) lo[d] = min(lo[d], b[d]);
-    }
-    chunk_min_base(s_lo, lo[0], lo[1], lo[2]);
-    __syncthreads();
-    if (s_lo[0] == INT_MAX) return;   // no particle inside the grid (uniform)
-    const int o[3] = {s_lo[0], s_lo[1], s_lo[2]};
     for (int i = tid; i < n_per; i += kChunkWG) {
         const int p = ch.x + i;
         if (p < 0 || p >= n) continue;
@@ -417,43 +389,20 @@ __global__ void __launch_bounds__(kChunkWG) __attribute__((amdgpu_waves_per_eu(2
                    float* __restrict__ acc_next, int* __restrict__ status) {
     __shared__ float s_v[3 * kCT3];
     __shared__ double s_acc[4 * kCT3];
-    __shared__ int s_lo[3];
     const int c = blockIdx.x;
     if (c >= meta[0]) return;
     const int4 ch = chunks[c];
     const int tid = threadIdx.x;
     const int ng = prm.n_grid;
     const int n_per = min(ch.y, kChunkMax);
-    if (tid < 3) s_lo[tid] = INT_MAX;
     for (int t = tid; t < 4 * kCT3; t += kChunkWG) s_acc[t] = 0.0;
-    __syncthreads();
-    int lo[3] = {INT_MAX, INT_MAX, INT_MAX};
-    for (int i = tid; i < n_per; i += kChunkWG) {
-        const int p = ch.x + i;
-        if (p < 0 || p >= n) continue;
-        int b[3];
-        stencil_base(planes + p, S, prm.inv_dx, b);
-        if (!base_ok(b, ng)) continue;
-#pragma unroll
-        for (int d = 0; d < 3; ++d) lo[d] = min(lo[d], b[d]);
-    }
-    chunk_min_base(s_lo, lo[0], lo[1], lo[2]);
-    __syncthreads();
-    if (s_lo[0] == INT_MAX) {   // every particle frozen (uniform): unchanged
-        if (dst != planes)
-            for (int i = tid; i < n_per; i += kChunkWG) {
-                const int p = ch.x + i;
-                if (p < 0 || p >= n) continue;
-                for (int k = 0; k < MPM_NPLANES; ++k) dst[(int64_t)k * S + p] = planes[(int64_t)k * S + p];
-            }
-        return;
-    }
-    const int o[3] = {s_lo[0], s_lo[1], s_lo[2]};
-    const int o2[3] = {o[0] - 1, o[1] - 1, o[2] - 1};
+    int o[3];
+    chunk_origin(ch.z, (ng + 7) / 8, o);
+    const int* o2 = o;   // the next substep's P2G tile: the same origin
     for (int t = tid; t < kCT3; t += kChunkWG) {
         const int x = o[0] + t / (kCT * kCT), y = o[1] + (t / kCT) % kCT, z = o[2] + t % kCT;
         float4 g = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (x < ng && y < ng && z < ng) g = gv[((int64_t)x * ng + y) * ng + z];
+        if (x >= 0 && y >= 0 && z >= 0 && x < ng && y < ng && z < ng) g = gv[((int64_t)x * ng + y) * ng + z];
         s_v[t] = g.x;
         s_v[kCT3 + t] = g.y;
         s_v[2 * kCT3 + t] = g.z;
