@@ -74,3 +74,41 @@ def test_solver_host_logic_on_cpu_tensors():
     sim.add_bounding_box()
     with pytest.raises(FileNotFoundError):
         sim.load_from_sampling("/does/not/exist.h5")
+
+
+def test_segment_entry_points_reject_bad_arguments():
+    """mpm_record_segment / mpm_adjoint_recorded validate their arguments on
+    the host before anything is launched (the pointers here are never
+    dereferenced)."""
+    lib = L.load()
+    prm = L.MpmParams()
+    prm.n_grid, prm.material, prm.dx, prm.inv_dx = 32, 2, 1.0 / 32, 32.0
+    cols = (L.MpmCollider * 1)()
+    st = (L.MpmState * 2)()
+    for k in range(2):
+        st[k].n, st[k].stride, st[k].planes = 100, 128, 0x100000 * (k + 1)
+    grids = (Ct.c_void_p * 2)(0x400000, 0x800000)
+    order, grid_bytes = 0x1000000, 16 * 32 ** 3
+    last = L.MpmState()
+    last.n, last.stride, last.planes = 100, 128, st[1].planes
+    rec = lambda n_sub, last_p, nxt, nbytes: lib.mpm_record_segment(prm, st, order, grids, cols, 0, 0, 1e-4, n_sub, 0,
+                                                                    last_p, nxt, nbytes, None)
+    assert rec(0, None, None, 0) < 0                                      # n_sub < 1
+    assert rec(2, Ct.byref(last), 0x2000000, grid_bytes) < 0              # last aliases a state
+    assert b"aliases a state" in lib.mpm_last_error()
+    last.planes = 0x200000 * 3
+    assert rec(2, Ct.byref(last), grids[1], grid_bytes) < 0               # next grid is a step grid
+    assert b"aliases a step grid" in lib.mpm_last_error()
+    assert rec(2, Ct.byref(last), 0x2000000, grid_bytes - 16) < 0         # next grid too small
+    assert rec(2, Ct.byref(last), 0x2000010, grid_bytes) < 0              # next grid not 256-byte aligned
+    last.stride = 256
+    assert rec(2, Ct.byref(last), 0x2000000, grid_bytes) < 0              # last differs in stride
+    st[1].stride = 256
+    assert rec(2, None, None, 0) < 0                                      # states differ in stride
+    st[1].stride = 128
+    grids[1] = 0x800010
+    assert rec(2, None, None, 0) < 0                                      # step grid not aligned
+    grids[1] = 0x800000
+    adj = lib.mpm_adjoint_recorded
+    assert adj(prm, st, order, grids, cols, 0, 0x3000000, 0, 1e-4, 2, 0x4000000, 0x4000000, 128, 0x5000000, None) < 0
+    assert b"adjoint buffers" in lib.mpm_last_error()                     # adj_a == adj_b
