@@ -42,15 +42,21 @@ constexpr int kBlk = 256;
 // adjoint planes: x (3), v (3), C (9), F (9), tau (9)
 constexpr int kAX = 0, kAV = 3, kAC = 6, kAF = 15, kAT = 24;
 
+// The adjoint's own quotients (not those of the recomputed forward return
+// map, whose branch must match the forward's) multiply by the hardware
+// reciprocal (1 ulp) instead of the correctly rounded division sequence
+// (~11 instructions; ~45 of them per particle).
+__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
 // Divided differences of the logarithm and of a power, robust when the
 // arguments meet, from the arguments s_k, s_l > 0 and their logarithms
 // lk = ln s_k, ll = ln s_l (which the return map already has: no further
 // logarithm or power is evaluated).
-// (ln s_l - ln s_k) / (s_l - s_k):
-__device__ __forceinline__ float dlog_div(float sk, float sl, float lk, float ll) {
-    const float r = sl / sk - 1.0f;
-    if (fabsf(r) < 1e-4f) return (1.0f - 0.5f * r + r * r / 3.0f) / sk;
-    return (ll - lk) / (sl - sk);
+// (ln s_l - ln s_k) / (s_l - s_k), rsk = 1 / s_k:
+__device__ __forceinline__ float dlog_div(float sk, float sl, float lk, float ll, float rsk) {
+    const float r = sl * rsk - 1.0f;
+    if (fabsf(r) < 1e-4f) return (1.0f - 0.5f * r + r * r * (1.0f / 3.0f)) * rsk;
+    return (ll - lk) * rcp(sl - sk);
 }
 
 // (s_l^B - s_k^B) / (s_l - s_k) = s_k^(B-1) expm1(B d) / expm1(d), d = ll - lk:
@@ -58,7 +64,7 @@ __device__ __forceinline__ float dpow_div(float lk, float ll, float B) {
     const float d = ll - lk;
     const float pk = expf((B - 1.0f) * lk);
     if (fabsf(d) < 1e-5f) return B * pk * (1.0f + 0.5f * (B - 1.0f) * d);
-    return pk * expm1f(B * d) / expm1f(d);
+    return pk * expm1f(B * d) * rcp(expm1f(d));
 }
 
 __device__ __forceinline__ void matmul(const float A[9], const float B[9], float C[9]) {
@@ -117,7 +123,7 @@ __device__ void sand_update_adj(const float G[9], const float F[9], float dt, fl
         for (int d = 0; d < 3; ++d) { se[d] = 1.f; ls[d] = 0.f; }
     } else {
         br = 3;
-        beta = kap * alpha * tr / ehn;
+        beta = kap * alpha * tr * rcp(ehn);
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             ls[d] = eps[d] - eh[d] * (dg / ehn);
@@ -138,14 +144,17 @@ __device__ void sand_update_adj(const float G[9], const float F[9], float dt, fl
         matmul_tn(U, Fbn, Q1);
         matmul(Q1, V, Q);       // U^T Fb' V
     }
+    float rse[3];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) Q[k * 4] += (2.0f * mu * Tp[k * 4] + lam * tsum) / se[k];
+    for (int k = 0; k < 3; ++k) rse[k] = rcp(se[k]);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) Q[k * 4] += (2.0f * mu * Tp[k * 4] + lam * tsum) * rse[k];
     // off-diagonal terms: the divided differences are symmetric in (k, l),
     // so each is evaluated once per unordered pair
 #pragma unroll
     for (int pr = 0; pr < 3; ++pr) {
         const int k = pr == 2 ? 1 : 0, l = pr == 0 ? 1 : 2;
-        const float c = 2.0f * mu * dlog_div(se[k], se[l], ls[k], ls[l]) / (se[k] + se[l]);
+        const float c = 2.0f * mu * dlog_div(se[k], se[l], ls[k], ls[l], rse[k]) * rcp(se[k] + se[l]);
         const float tt = Tp[k * 3 + l] + Tp[l * 3 + k];
         Q[k * 3 + l] += c * se[l] * tt;
         Q[l * 3 + k] += c * se[k] * tt;
@@ -164,27 +173,33 @@ __device__ void sand_update_adj(const float G[9], const float F[9], float dt, fl
                 for (int k = 0; k < 3; ++k) dsd[i][k] = 0.f;
         } else {
             // H_i = -beta eps_i + (1 + beta) tr/3, beta = kap alpha tr / |eh|
-            float dbeta[3];
+            const float rehn = rcp(ehn);
+            float dbeta[3], rs[3];
 #pragma unroll
-            for (int k = 0; k < 3; ++k) dbeta[k] = kap * alpha * (1.0f / ehn - tr * eh[k] / (ehn * ehn * ehn));
+            for (int k = 0; k < 3; ++k) {
+                dbeta[k] = kap * alpha * (rehn - tr * eh[k] * (rehn * rehn * rehn));
+                rs[k] = rcp(s[k]);
+            }
+            const float b3 = (1.0f + beta) * (1.0f / 3.0f);
 #pragma unroll
             for (int i = 0; i < 3; ++i)
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
-                    const float dH = (i == k ? -beta : 0.f) + (1.0f + beta) / 3.0f - eh[i] * dbeta[k];
-                    dsd[i][k] = se[i] * dH / s[k];
+                    const float dH = (i == k ? -beta : 0.f) + b3 - eh[i] * dbeta[k];
+                    dsd[i][k] = se[i] * dH * rs[k];
                 }
             // parameters: dH_i/dalpha = -eh_i kap tr/|eh|, dH_i/dkap = -eh_i alpha tr/|eh|
             float qa = 0.f, qk = 0.f;
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
-                qa += Q[i * 4] * se[i] * (-eh[i] * kap * tr / ehn);
-                qk += Q[i * 4] * se[i] * (-eh[i] * alpha * tr / ehn);
+                qa += Q[i * 4] * se[i] * (-eh[i] * kap * tr * rehn);
+                qk += Q[i * 4] * se[i] * (-eh[i] * alpha * tr * rehn);
             }
             alb += qa;
             // kap = 1 + 3 lam / (2 mu)
-            lamb += qk * 3.0f / (2.0f * mu);
-            mub += qk * (-3.0f * lam / (2.0f * mu * mu));
+            const float r2mu = rcp(2.0f * mu);
+            lamb += qk * 3.0f * r2mu;
+            mub += qk * (-3.0f * lam * r2mu * rcp(mu));
         }
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
@@ -199,7 +214,7 @@ __device__ void sand_update_adj(const float G[9], const float F[9], float dt, fl
         for (int pr = 0; pr < 3; ++pr) {   // symmetric in (k, l): once per unordered pair
             const int k = pr == 2 ? 1 : 0, l = pr == 0 ? 1 : 2;
             const float D = br == 3 ? eA * dpow_div(eps[k], eps[l], Bexp) : 0.f;
-            const float S = (se[l] + se[k]) / (s[l] + s[k]);
+            const float S = (se[l] + se[k]) * rcp(s[l] + s[k]);
             const float a = 0.5f * (D + S), b = 0.5f * (D - S);
             const float qkl = Q[k * 3 + l], qlk = Q[l * 3 + k];
             P[k * 3 + l] = a * qkl + b * qlk;
