@@ -194,9 +194,9 @@ class Rollout:
 
     def _finish(self, planes):
         self._final = planes
-        # x0 and u back to the caller's order (scatter by the pid plane)
-        xu = self._permute(planes[L.P_X0:L.P_U + 3], 6, planes[L.P_PID].data_ptr(), scatter=True)
-        return (xu[0:3] + xu[3:6]).t().contiguous()
+        # x = x0 + u back to the caller's order (scatter by the pid plane)
+        x = planes[L.P_X0:L.P_X0 + 3] + planes[L.P_U:L.P_U + 3]
+        return self._permute(x, 3, planes[L.P_PID].data_ptr(), scatter=True).t().contiguous()
 
     def _release_record(self):
         self._tape_states, self._tape_grids, self._tape_orders = [], [], []
@@ -258,7 +258,7 @@ class Rollout:
                                                  self._ncol, self._aligned(self._ws_adj), c0, self.dt, nk,
                                                  adj.data_ptr(), nxt.data_ptr(), self.n, pb.data_ptr(), stream),
                         "mpm_adjoint_recorded")
-                adj = self._permute(adj, L.ADJ_NPLANES, self._aligned(order) + self._perm_off, scatter=True)
+                adj = self._unsort(adj, order, c0)
             return self._param_grads(adj, pb)
         # a segment's replayed states and, per replayed substep, the grids it
         # leaves for its adjoint (one native call per segment)
@@ -278,11 +278,19 @@ class Rollout:
                                             self._aligned(self._ws_adj), c0, self.dt, nk, adj.data_ptr(),
                                             nxt.data_ptr(), self.n, pb.data_ptr(), stream), "mpm_adjoint_segment")
             # adjoint of the sort at c0: its inverse permutation
-            adj = self._permute(adj, L.ADJ_NPLANES, self._aligned(order) + self._perm_off, scatter=True)
+            adj = self._unsort(adj, order, c0)
         return self._param_grads(adj, pb)
 
-    def _param_grads(self, adj, pb):
-        v0_bar = adj[L.ADJ_V:L.ADJ_V + 3].t().contiguous()
+    def _unsort(self, adj, order, c0):
+        """Adjoint of the sort at substep c0: its inverse permutation.  At
+        c0 = 0 only the initial-velocity planes are needed (in the caller's
+        order); the others are dropped."""
+        if c0 > 0:
+            return self._permute(adj, L.ADJ_NPLANES, self._aligned(order) + self._perm_off, scatter=True)
+        return self._permute(adj[L.ADJ_V:L.ADJ_V + 3], 3, self._aligned(order) + self._perm_off, scatter=True)
+
+    def _param_grads(self, v_planes, pb):
+        v0_bar = v_planes.t().contiguous()
         mu_b, lam_b, al_b = (float(v) for v in pb.cpu())
         J = material_jacobian(*self._material_args)
         E_b = mu_b * J[0][0] + lam_b * J[1][0]

@@ -98,16 +98,17 @@ __device__ __forceinline__ int chunk_tile_index(const int b[3], const int o[3]) 
     return (x * kCT + y) * kCT + z;
 }
 
-// Processing order of a chunk's particles (chunk slots first .. first+n_per
-// of `planes`): s_ord[q] = chunk index of the q-th particle to process, in
-// bank order (interleave_mul) of the LDS bank class of each stencil base in
-// the tile at origin o, so that the 32 lanes of a half-wave gather and
-// scatter through distinct banks.  Particles outside the tile or the grid
-// take class 0.  Scratch: s_cnt[64], s_mul; barriers inside (call from every
-// thread of the workgroup).
-__device__ __forceinline__ void chunk_bank_order(const float* __restrict__ planes, int64_t S, int first, int n_per,
-                                                 int n, float inv_dx, const int o[3], unsigned short* s_ord,
-                                                 int* s_cnt, int* s_mul) {
+// Processing order of a chunk's particles (sorted slots first .. first+n_per,
+// whose particles sit at src[idx[slot]]): s_ord[q] = chunk index of the q-th
+// particle to process, in bank order (interleave_mul) of the LDS bank class
+// of each stencil base in the tile at origin o, so that the 32 lanes of a
+// half-wave gather and scatter through distinct banks.  Particles outside
+// the tile or the grid take class 0.  Scratch: s_cnt[64], s_mul; barriers
+// inside (call from every thread of the workgroup).
+__device__ __forceinline__ void chunk_bank_order(const float* __restrict__ planes, int64_t S,
+                                                 const int* __restrict__ idx, int first, int n_per, int n,
+                                                 float inv_dx, const int o[3], unsigned short* s_ord, int* s_cnt,
+                                                 int* s_mul) {
     const int tid = threadIdx.x;
     if (tid < 32) s_cnt[tid] = 0;
     __syncthreads();
@@ -117,7 +118,7 @@ __device__ __forceinline__ void chunk_bank_order(const float* __restrict__ plane
         const int i = tid + r * kChunkWG;
         cls[r] = rank[r] = 0;
         if (i >= n_per) continue;
-        const int p = first + i;
+        const int p = idx[first + i];
         int c = 0;
         if (p >= 0 && p < n) {
             int b[3];

@@ -157,9 +157,11 @@ __global__ void __launch_bounds__(kScan)
     if (tid < 63) meta[1 + tid] = 0;   // status counters of the substeps on this order
 }
 
+// Each particle's sorted slot: perm[slot] = source index (the planes move in
+// k_order_interleave, which gathers them straight into bank order).
 __global__ void __launch_bounds__(kB)
-    k_order_scatter(const float* __restrict__ src, int64_t SS, float* __restrict__ dst, int64_t DS, int n,
-                    float inv_dx, int ng, int nb, int* __restrict__ cursor, int* __restrict__ perm) {
+    k_order_scatter(const float* __restrict__ src, int64_t SS, int n, float inv_dx, int ng, int nb,
+                    int* __restrict__ cursor, int* __restrict__ perm) {
     __shared__ int s_cnt[kWin], s_base[kWin];
     __shared__ int s_kmin;
     const int tid = threadIdx.x;
@@ -190,22 +192,20 @@ __global__ void __launch_bounds__(kB)
     if (key < 0) return;
     const int pos = in ? s_base[key - kmin] + local : base + (__lane_id() - first);
     if (pos < 0 || pos >= n) return;   // cannot happen: same keys as the histogram
-#pragma unroll 13
-    for (int k = 0; k < MPM_NPLANES; ++k) dst[(int64_t)k * DS + pos] = src[(int64_t)k * SS + p];
     perm[pos] = p;
 }
 
-// Bank order inside each chunk, in memory: one workgroup per chunk moves
-// the chunk's particles (all planes and the permutation) into bank order
-// (chunk_bank_order) of their stencil bases in the chunk's tile
-// (chunk_origin), staging one plane at a time in LDS.  The chunk kernels then
-// read consecutive slots (coalesced) whose 32-lane halves gather and scatter
-// through distinct LDS banks.
+// Bank order inside each chunk, in memory: one workgroup per chunk gathers
+// the chunk's particles (all planes) from the unsorted source straight into
+// bank order (chunk_bank_order) of their stencil bases in the chunk's tile
+// (chunk_origin), and reorders the chunk's permutation entries to match.
+// The chunk kernels then read consecutive slots (coalesced) whose 32-lane
+// halves gather and scatter through distinct LDS banks.
 __global__ void __launch_bounds__(kChunkWG)
-    k_order_interleave(float* __restrict__ planes, int64_t S, int n, const int4* __restrict__ chunks,
-                       const int* __restrict__ meta, float inv_dx, int ng, int* __restrict__ perm) {
+    k_order_interleave(const float* __restrict__ src, int64_t SS, float* __restrict__ dst, int64_t DS, int n,
+                       const int4* __restrict__ chunks, const int* __restrict__ meta, float inv_dx, int ng,
+                       int* __restrict__ perm) {
     __shared__ unsigned short s_ord[kChunkMax];
-    __shared__ float s_stage[kChunkMax];
     __shared__ int s_cnt[64];
     __shared__ int s_mul;
     const int c = blockIdx.x;
@@ -213,16 +213,29 @@ __global__ void __launch_bounds__(kChunkWG)
     const int4 ch = chunks[c];
     const int tid = threadIdx.x;
     const int n_per = min(ch.y, kChunkMax);
-    if (ch.x < 0 || ch.x + n_per > n || n_per < 2) return;
+    if (ch.x < 0 || ch.x + n_per > n || n_per < 1) return;
     int o[3];
     chunk_origin(ch.z, (ng + 7) / 8, o);
-    chunk_bank_order(planes, S, ch.x, n_per, n, inv_dx, o, s_ord, s_cnt, &s_mul);   // barriers inside
-    for (int k = 0; k <= MPM_NPLANES; ++k) {   // plane MPM_NPLANES: the permutation
-        float* pl = k < MPM_NPLANES ? planes + (int64_t)k * S + ch.x : (float*)(perm + ch.x);
-        for (int i = tid; i < n_per; i += kChunkWG) s_stage[i] = pl[i];
-        __syncthreads();
-        for (int q = tid; q < n_per; q += kChunkWG) pl[q] = s_stage[s_ord[q]];
-        __syncthreads();
+    chunk_bank_order(src, SS, perm, ch.x, n_per, n, inv_dx, o, s_ord, s_cnt, &s_mul);   // barriers inside
+    constexpr int R = kChunkMax / kChunkWG;
+    int from[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int q = tid + r * kChunkWG;
+        from[r] = q < n_per ? perm[ch.x + s_ord[q]] : -1;
+    }
+#pragma unroll 3
+    for (int k = 0; k < MPM_NPLANES; ++k)
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int q = tid + r * kChunkWG;
+            if (from[r] >= 0 && from[r] < n) dst[(int64_t)k * DS + ch.x + q] = src[(int64_t)k * SS + from[r]];
+        }
+    __syncthreads();   // every thread has read the chunk's old permutation entries
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int q = tid + r * kChunkWG;
+        if (q < n_per) perm[ch.x + q] = from[r];
     }
 }
 
@@ -556,12 +569,11 @@ int mpm_sort_by_bin(const MpmParams* prm, const MpmState* src, const MpmState* d
     if ((rc = check_launch("k_order_scan"))) return rc;
     if (n > 0) {
         hipLaunchKernelGGL(k_order_scatter, dim3(nblk(n)), dim3(kB), 0, s, (const float*)src->planes, src->stride,
-                           (float*)dst->planes, dst->stride, (int)n, prm->inv_dx, prm->n_grid, (int)L.nb, cursor,
-                           perm);
+                           (int)n, prm->inv_dx, prm->n_grid, (int)L.nb, cursor, perm);
         if ((rc = check_launch("k_order_scatter"))) return rc;
         hipLaunchKernelGGL(k_order_interleave, dim3((unsigned)L.max_chunks), dim3(kChunkWG), 0, s,
-                           (float*)dst->planes, dst->stride, (int)n, (const int4*)chunks, (const int*)meta,
-                           prm->inv_dx, prm->n_grid, perm);
+                           (const float*)src->planes, src->stride, (float*)dst->planes, dst->stride, (int)n,
+                           (const int4*)chunks, (const int*)meta, prm->inv_dx, prm->n_grid, perm);
         if ((rc = check_launch("k_order_interleave"))) return rc;
     }
     return 0;
