@@ -461,8 +461,8 @@ __device__ __forceinline__ void frozen_adj(const float* __restrict__ an, int64_t
 }
 
 // G2P adjoint over the chunks of an ordered state (mpm_sort_by_bin): the
-// chunk's grid velocities are staged in an LDS tile at its smallest stencil
-// base; the grid-velocity adjoint is accumulated in an FP64 LDS tile there and
+// chunk's grid velocities are staged in an LDS tile at its origin
+// (chunk_origin); the grid-velocity adjoint is accumulated in an FP64 LDS tile there and
 // flushed once per node (chunk_flush); particles whose stencil leaves the
 // tile gather from and add to the global grids directly.
 __global__ void __launch_bounds__(kChunkWG)

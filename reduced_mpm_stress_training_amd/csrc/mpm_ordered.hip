@@ -295,12 +295,12 @@ __global__ void __launch_bounds__(kChunkWG)
 // ---------------------------------------------------------------------------
 // Fused G2P (this substep, g2p_apic_and_stress, mpm_utils.py:241-306) and
 // P2G (the next substep, p2g_apic_stress_given, :190-227) over chunks.  The
-// chunk's grid velocities are staged in an LDS tile at its smallest stencil
-// base o; each particle gathers, runs F update / return map / stress in
-// registers, writes its new state (u = x - x0, v, C, F, tau; in place, or
-// the whole state into `dst`), and
-// scatters its next-substep P2G into an FP64 LDS tile at o - 1 (particles
-// move less than a cell per substep), flushed once per node into acc_next.
+// chunk's grid velocities are staged in an LDS tile at its origin o
+// (chunk_origin); each particle gathers, runs F update / return map / stress
+// in registers, writes its new state (u = x - x0, v, C, F, tau; in place, or
+// the whole state into `dst`), and scatters its next-substep P2G into an
+// FP64 LDS tile at the same origin (the tile holds a cell of drift either
+// way), flushed once per node into acc_next.
 // Stencils outside the tiles use the global grids.  Frozen particles (stencil
 // outside the grid) are left unchanged and do not scatter, as in the stage
 // kernels.
