@@ -263,7 +263,9 @@ int mpm_adjoint_substep(const MpmParams* prm, const MpmState* st, const void* or
  * mpm_run_ordered, each substep's P2G accumulator and grid velocities kept in
  * its own workspace), then applies the adjoint substeps in reverse on those
  * grids.  states: n_sub states sharing n and stride; states[0] holds S_k0 on
- * entry, states[1..] receive S_{k0+1} .. S_{k0+n_sub-1}.  step_grids: n_sub
+ * entry, states[1..] receive S_{k0+1} .. S_{k0+n_sub-1} -- the planes a
+ * substep changes (u, v, C, F, tau); their constant planes (x0, m, vol, id)
+ * are left as they are, those of states[0] being used.  step_grids: n_sub
  * workspaces of mpm_workspace_bytes (256-byte aligned).  Substep k runs at
  * time (float)(k * dt) with step (float)dt.  adj_a holds the adjoint of
  * S_{k0+n_sub} on entry and the adjoint of S_k0 on return; adj_b is scratch of
@@ -280,7 +282,7 @@ int mpm_adjoint_segment(const MpmParams* prm, const MpmState* states, const void
  * step_grids[0] already holds the P2G accumulator of states[0] (the
  * next_grid of the segment before, whose last state this segment sorted), so
  * it is not recomputed.  When `last` is not NULL (same n and stride, no
- * alias of a state) it receives S_{k0+n_sub}, and next_grid (256-byte
+ * alias of a state) it receives S_{k0+n_sub} with every plane, and next_grid (256-byte
  * aligned, >= 16 n_grid^3 bytes, not one of step_grids) that state's P2G
  * accumulator.  mpm_adjoint_recorded: the adjoint substeps in reverse on what
  * mpm_record_segment left, arguments as in mpm_adjoint_segment.  No

@@ -404,10 +404,10 @@ struct AdjIn {
     float x[3], F[9], Fbn[9];
 };
 
-__device__ __forceinline__ void load_adj_in(const float* __restrict__ st, int64_t S, const float* __restrict__ an,
-                                            int64_t SA, int p, AdjIn& q) {
+__device__ __forceinline__ void load_adj_in(const float* __restrict__ st, const float* __restrict__ cst, int64_t S,
+                                            const float* __restrict__ an, int64_t SA, int p, AdjIn& q) {
 #pragma unroll
-    for (int d = 0; d < 3; ++d) q.x[d] = st[(MPM_P_X0 + d) * S + p] + st[(MPM_P_U + d) * S + p];
+    for (int d = 0; d < 3; ++d) q.x[d] = cst[(MPM_P_X0 + d) * S + p] + st[(MPM_P_U + d) * S + p];
     load9(st + p, S, MPM_P_F, q.F);
     load9(an + p, SA, kAF, q.Fbn);
 }
@@ -466,7 +466,8 @@ __device__ __forceinline__ void frozen_adj(const float* __restrict__ an, int64_t
 // flushed once per node (chunk_flush); particles whose stencil leaves the
 // tile gather from and add to the global grids directly.
 __global__ void __launch_bounds__(kChunkWG)
-    k_g2p_adj_chunks(const float* __restrict__ st, int64_t S, int n, const int4* __restrict__ chunks,
+    k_g2p_adj_chunks(const float* __restrict__ st, const float* __restrict__ cst, int64_t S, int n,
+                     const int4* __restrict__ chunks,
                      const int* __restrict__ meta, MpmParams prm, float dt, const float4* __restrict__ gv,
                      const float* __restrict__ an, int64_t SA, float* __restrict__ vb, float* __restrict__ ac,
                      double* __restrict__ pb) {
@@ -497,11 +498,11 @@ __global__ void __launch_bounds__(kChunkWG)
     // marked (bit r for the thread's r-th particle) for the second loop
     unsigned outside = 0;
     AdjIn q;
-    if (tid < n_per && ch.x + tid < n) load_adj_in(st, S, an, SA, ch.x + tid, q);
+    if (tid < n_per && ch.x + tid < n) load_adj_in(st, cst, S, an, SA, ch.x + tid, q);
     for (int i = tid, r = 0; i < n_per; i += kChunkWG, ++r) {
         const int p = ch.x + i;
         const AdjIn cur = q;
-        if (i + kChunkWG < n_per && p + kChunkWG < n) load_adj_in(st, S, an, SA, p + kChunkWG, q);
+        if (i + kChunkWG < n_per && p + kChunkWG < n) load_adj_in(st, cst, S, an, SA, p + kChunkWG, q);
         if (p < 0 || p >= n) continue;
         AxisWeights Aw;
         axis_weights(cur.x, prm.inv_dx, Aw);
@@ -522,7 +523,7 @@ __global__ void __launch_bounds__(kChunkWG)
     for (int i = tid, r = 0; outside && i < n_per; i += kChunkWG, ++r) {
         if (!(outside & (1u << r))) continue;
         const int p = ch.x + i;
-        load_adj_in(st, S, an, SA, p, q);
+        load_adj_in(st, cst, S, an, SA, p, q);
         AxisWeights Aw;
         axis_weights(q.x, prm.inv_dx, Aw);
         const int64_t gb = ((int64_t)Aw.base[0] * ng + Aw.base[1]) * ng + Aw.base[2];
@@ -719,16 +720,17 @@ __device__ __forceinline__ void p2g_adj_sums(const float4* __restrict__ gb, int6
 }
 
 __global__ void __launch_bounds__(kBlk)
-    k_p2g_adj(const float* __restrict__ st, int64_t S, int n, MpmParams prm, float dt, const float4* __restrict__ gb,
-              float* __restrict__ ac, int64_t SA) {
+    k_p2g_adj(const float* __restrict__ st, const float* __restrict__ cst, int64_t S, int n, MpmParams prm, float dt,
+              const float4* __restrict__ gb, float* __restrict__ ac, int64_t SA) {
     const int p = blockIdx.x * kBlk + threadIdx.x;
     if (p >= n) return;
     const float* sp = st + p;
+    const float* kp = cst + p;   // constant planes
     const int ng = prm.n_grid;
     const float inv_dx = prm.inv_dx, dxp = prm.dx;
     float x[3];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) x[d] = sp[(MPM_P_X0 + d) * S] + sp[(MPM_P_U + d) * S];
+    for (int d = 0; d < 3; ++d) x[d] = kp[(MPM_P_X0 + d) * S] + sp[(MPM_P_U + d) * S];
     int base[3];
     float fx[3];
 #pragma unroll
@@ -744,7 +746,7 @@ __global__ void __launch_bounds__(kBlk)
     const int64_t b0 = ((int64_t)base[0] * ng + base[1]) * ng + base[2];
     float r[3][kNSum], rm[3];
     p2g_adj_sums(gb, b0, ng, X, Y, Z, r, rm);
-    const float m = sp[MPM_P_MASS * S], vol = sp[MPM_P_VOL * S];
+    const float m = kp[MPM_P_MASS * S], vol = kp[MPM_P_VOL * S];
     const float ct = dt * vol * inv_dx * inv_dx;
     float* cp = ac + p;
     float xg[3] = {m * inv_dx * rm[0], m * inv_dx * rm[1], m * inv_dx * rm[2]};   // the mass adjoint m_b
@@ -800,7 +802,7 @@ int64_t mpm_adjoint_workspace_bytes(const MpmParams* prm) {
 static int adjoint_substep(const MpmParams* prm, const MpmState* st, const void* order, const void* step_grids,
                            const MpmCollider* colliders, int n_colliders, void* workspace, float time, float dt,
                            const float* adj_next, float* adj_cur, int64_t adj_stride, double* param_adj,
-                           bool vb_clear, bool clear_after, void* stream) {
+                           bool vb_clear, bool clear_after, const float* cst, void* stream) {
     int rc = check_params(prm);
     if (rc) return rc;
     if (!st || !st->planes || st->n < 0 || st->stride < st->n) return set_err(-4, "state invalid");
@@ -837,7 +839,8 @@ static int adjoint_substep(const MpmParams* prm, const MpmState* st, const void*
         const OrderLayout L = order_layout(*prm, n);
         const char* ob = (const char*)order;
         hipLaunchKernelGGL(k_g2p_adj_chunks, dim3((unsigned)L.max_chunks), dim3(kChunkWG), 0, s,
-                           (const float*)st->planes, st->stride, n, (const int4*)(ob + L.off_chunks),
+                           (const float*)st->planes, cst ? cst : (const float*)st->planes, st->stride, n,
+                           (const int4*)(ob + L.off_chunks),
                            (const int*)(ob + L.off_meta), *prm, dt, (const float4*)gv, adj_next, adj_stride,
                            (float*)vbg, adj_cur, param_adj);
         if ((rc = check_launch("k_g2p_adj_chunks"))) return rc;
@@ -846,8 +849,9 @@ static int adjoint_substep(const MpmParams* prm, const MpmState* st, const void*
                        *prm, cs, time, dt, clear_after ? 1 : 0);
     if ((rc = check_launch("k_grid_adj"))) return rc;
     if (n > 0) {
-        hipLaunchKernelGGL(k_p2g_adj, dim3(blocks(n)), dim3(kBlk), 0, s, (const float*)st->planes, st->stride, n,
-                           *prm, dt, (const float4*)gbg, adj_cur, adj_stride);
+        hipLaunchKernelGGL(k_p2g_adj, dim3(blocks(n)), dim3(kBlk), 0, s, (const float*)st->planes,
+                           cst ? cst : (const float*)st->planes, st->stride, n, *prm, dt, (const float4*)gbg, adj_cur,
+                           adj_stride);
         if ((rc = check_launch("k_p2g_adj"))) return rc;
     }
     return 0;
@@ -857,7 +861,7 @@ int mpm_adjoint_substep(const MpmParams* prm, const MpmState* st, const void* or
                         const MpmCollider* colliders, int n_colliders, void* workspace, float time, float dt,
                         const float* adj_next, float* adj_cur, int64_t adj_stride, double* param_adj, void* stream) {
     return adjoint_substep(prm, st, order, step_grids, colliders, n_colliders, workspace, time, dt, adj_next, adj_cur,
-                           adj_stride, param_adj, false, false, stream);
+                           adj_stride, param_adj, false, false, nullptr, stream);
 }
 
 // shared checks of a segment's states and per-substep workspaces
@@ -899,7 +903,10 @@ static int record_segment(const MpmParams* prm, const MpmState* states, const vo
                                         acc_next, stream)))
             return rc;
         if (!acc_next) break;
-        if ((rc = g2p2g_ordered(prm, &states[j], more ? &states[j + 1] : last, order, gvj, acc_next, fdt, stream)))
+        // the segment's later states hold only the planes a substep changes;
+        // x0, m, vol, id stay in states[0] (and are written to `last`)
+        if ((rc = g2p2g_ordered(prm, &states[j], more ? &states[j + 1] : last, order, gvj, acc_next, fdt,
+                                (const float*)states[0].planes, !more, stream)))
             return rc;
     }
     return 0;
@@ -918,7 +925,7 @@ static int adjoint_recorded(const MpmParams* prm, const MpmState* states, const 
     for (int j = n_sub - 1; j >= 0; --j) {
         if ((rc = adjoint_substep(prm, &states[j], order, step_grids[j], colliders, n_colliders, workspace,
                                   (float)((double)(k0 + j) * dt), (float)dt, cur, nxt, adj_stride, param_adj,
-                                  j < n_sub - 1, j > 0, stream)))
+                                  j < n_sub - 1, j > 0, (const float*)states[0].planes, stream)))
             return rc;
         float* t = cur; cur = nxt; nxt = t;
     }

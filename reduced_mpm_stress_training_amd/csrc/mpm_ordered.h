@@ -177,8 +177,12 @@ int p2g_ordered(const MpmParams* prm, const MpmState* st, const void* order, flo
 
 // host: the fused G2P (grid velocities grid_v) + next P2G (into the zeroed
 // acc_next) over the chunks; dst may be src (in place) or another state of
-// the same size, which then receives the whole new state (mpm_ordered.hip)
+// the same size, which receives the planes the substep changes (u, v, C, F,
+// tau) and, when copy_const, the constant ones (x0, m, vol, id).  cst: the
+// planes base the constant planes are read from (NULL: src) -- the first
+// state of a recorded segment (mpm_ordered.hip)
 int g2p2g_ordered(const MpmParams* prm, const MpmState* src, const MpmState* dst, const void* order,
-                  const float* grid_v, float* acc_next, float dt, void* stream);
+                  const float* grid_v, float* acc_next, float dt, const float* cst, bool copy_const,
+                  void* stream);
 
 }  // namespace mpm

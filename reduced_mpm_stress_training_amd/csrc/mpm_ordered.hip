@@ -330,18 +330,26 @@ struct FlatIn {
     float x0[3], u[3], F[9], m, vol, pid;
 };
 
-__device__ __forceinline__ void load_flat(const float* src, int64_t S, int p, FlatIn& q) {
+// cst: where the constant planes (x0, m, vol, id) live -- src itself, or the
+// first state of a recorded segment, whose later states hold only the planes
+// a substep changes.
+__device__ __forceinline__ bool const_plane(int k) {
+    return k < MPM_P_U || k == MPM_P_MASS || k == MPM_P_VOL || k == MPM_P_PID;
+}
+
+__device__ __forceinline__ void load_flat(const float* src, const float* cst, int64_t S, int p, FlatIn& q) {
     const float* sp = src + p;
+    const float* cp = cst + p;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-        q.x0[d] = sp[(MPM_P_X0 + d) * S];
+        q.x0[d] = cp[(MPM_P_X0 + d) * S];
         q.u[d] = sp[(MPM_P_U + d) * S];
     }
 #pragma unroll
     for (int k = 0; k < 9; ++k) q.F[k] = sp[(MPM_P_F + k) * S];
-    q.m = sp[MPM_P_MASS * S];
-    q.vol = sp[MPM_P_VOL * S];
-    q.pid = sp[MPM_P_PID * S];
+    q.m = cp[MPM_P_MASS * S];
+    q.vol = cp[MPM_P_VOL * S];
+    q.pid = cp[MPM_P_PID * S];
 }
 
 template <typename Nodes>
@@ -357,7 +365,7 @@ __device__ __forceinline__ void g2p2g_flat_particle(const FlatIn& q, bool copy, 
 #pragma unroll
     for (int k = 0; k < 9; ++k) F[k] = q.F[k];
     const float m = q.m, vol = q.vol;
-    if (copy) {   // out of place: the planes the substep does not change
+    if (copy) {   // the constant planes, into a state that must hold them
 #pragma unroll
         for (int d = 0; d < 3; ++d) dp[(MPM_P_X0 + d) * S] = x0[d];
         dp[MPM_P_MASS * S] = m;
@@ -397,9 +405,9 @@ __device__ __forceinline__ void g2p2g_flat_particle(const FlatIn& q, bool copy, 
 }
 
 __global__ void __launch_bounds__(kChunkWG) __attribute__((amdgpu_waves_per_eu(2, 2)))
-    k_g2p2g_chunks(const float* planes, float* dst, int64_t S, int n, const int4* __restrict__ chunks,
-                   const int* __restrict__ meta, MpmParams prm, float dt, const float4* __restrict__ gv,
-                   float* __restrict__ acc_next, int* __restrict__ status) {
+    k_g2p2g_chunks(const float* planes, const float* cst, float* dst, int copy_const, int64_t S, int n,
+                   const int4* __restrict__ chunks, const int* __restrict__ meta, MpmParams prm, float dt,
+                   const float4* __restrict__ gv, float* __restrict__ acc_next, int* __restrict__ status) {
     __shared__ float s_v[3 * kCT3];
     __shared__ double s_acc[4 * kCT3];
     const int c = blockIdx.x;
@@ -424,14 +432,14 @@ __global__ void __launch_bounds__(kChunkWG) __attribute__((amdgpu_waves_per_eu(2
     // stencils inside the velocity tile first; the others (bit r of the
     // thread's r-th particle) gather from the global grid afterwards.  The
     // next particle's inputs are loaded while the current one is processed.
-    const bool copy = dst != planes;
+    const bool copy = copy_const != 0;
     unsigned outside = 0;
     FlatIn q;
-    if (tid < n_per && ch.x + tid < n) load_flat(planes, S, ch.x + tid, q);
+    if (tid < n_per && ch.x + tid < n) load_flat(planes, cst, S, ch.x + tid, q);
     for (int i = tid, r = 0; i < n_per; i += kChunkWG, ++r) {
         const int p = ch.x + i;
         const FlatIn cur = q;
-        if (i + kChunkWG < n_per && p + kChunkWG < n) load_flat(planes, S, p + kChunkWG, q);
+        if (i + kChunkWG < n_per && p + kChunkWG < n) load_flat(planes, cst, S, p + kChunkWG, q);
         if (p < 0 || p >= n) continue;
         float x[3];
 #pragma unroll
@@ -439,8 +447,13 @@ __global__ void __launch_bounds__(kChunkWG) __attribute__((amdgpu_waves_per_eu(2
         AxisWeights A;
         axis_weights(x, prm.inv_dx, A);
         if (!base_ok(A.base, ng)) {   // frozen: unchanged
-            if (dst != planes)
-                for (int k = 0; k < MPM_NPLANES; ++k) dst[(int64_t)k * S + p] = planes[(int64_t)k * S + p];
+            for (int k = 0; k < MPM_NPLANES; ++k) {
+                if (const_plane(k)) {
+                    if (copy) dst[(int64_t)k * S + p] = cst[(int64_t)k * S + p];
+                } else if (dst != planes) {
+                    dst[(int64_t)k * S + p] = planes[(int64_t)k * S + p];
+                }
+            }
             continue;
         }
         const int ti = chunk_tile_index(A.base, o);
@@ -454,7 +467,7 @@ __global__ void __launch_bounds__(kChunkWG) __attribute__((amdgpu_waves_per_eu(2
     for (int i = tid, r = 0; outside && i < n_per; i += kChunkWG, ++r) {
         if (!(outside & (1u << r))) continue;
         const int p = ch.x + i;
-        load_flat(planes, S, p, q);
+        load_flat(planes, cst, S, p, q);
         float x[3];
 #pragma unroll
         for (int d = 0; d < 3; ++d) x[d] = q.x0[d] + q.u[d];
@@ -508,7 +521,8 @@ int p2g_ordered(const MpmParams* prm, const MpmState* st, const void* order, flo
 }
 
 int g2p2g_ordered(const MpmParams* prm, const MpmState* src, const MpmState* dst, const void* order,
-                  const float* grid_v, float* acc_next, float dt, void* stream) {
+                  const float* grid_v, float* acc_next, float dt, const float* cst, bool copy_const,
+                  void* stream) {
     int rc = check_params(prm);
     if (rc) return rc;
     if ((rc = check_flat(src)) || (rc = check_flat(dst))) return rc;
@@ -520,9 +534,9 @@ int g2p2g_ordered(const MpmParams* prm, const MpmState* src, const MpmState* dst
     char* ob = (char*)order;
     int* status = (int*)(ob + L.off_meta) + kOrderStatus;
     hipLaunchKernelGGL(k_g2p2g_chunks, dim3((unsigned)L.max_chunks), dim3(kChunkWG), 0, (hipStream_t)stream,
-                       (const float*)src->planes, (float*)dst->planes, src->stride, (int)src->n,
-                       (const int4*)(ob + L.off_chunks), (const int*)(ob + L.off_meta), *prm, dt,
-                       (const float4*)grid_v, acc_next, status);
+                       (const float*)src->planes, cst ? cst : (const float*)src->planes, (float*)dst->planes,
+                       copy_const ? 1 : 0, src->stride, (int)src->n, (const int4*)(ob + L.off_chunks),
+                       (const int*)(ob + L.off_meta), *prm, dt, (const float4*)grid_v, acc_next, status);
     return check_launch("k_g2p2g_chunks");
 }
 
@@ -623,7 +637,7 @@ int mpm_run_ordered(const MpmParams* prm, const MpmState* st, const void* order,
         if ((rc = grid_update_zero_next(prm, acc, gv, colliders, n_colliders, (float)((double)(k0 + j) * dt),
                                         (float)dt, acc2, stream)))
             return rc;
-        if ((rc = g2p2g_ordered(prm, st, st, order, gv, acc2, (float)dt, stream))) return rc;
+        if ((rc = g2p2g_ordered(prm, st, st, order, gv, acc2, (float)dt, nullptr, false, stream))) return rc;
         float* t = acc; acc = acc2; acc2 = t;
     }
     return 0;
