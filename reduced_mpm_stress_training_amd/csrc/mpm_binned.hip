@@ -522,18 +522,28 @@ __global__ void __launch_bounds__(kWG)
             nlo[d] = (p[d] + 1 - 8 * hi[d] <= 2 && hi[d] >= 1) ? 2 : 1;
         }
         const int bb[3] = {bx, by, bz};
-        for (int ix = 0; ix < nlo[0]; ++ix)
-            for (int iy = 0; iy < nlo[1]; ++iy)
-                for (int iz = 0; iz < nlo[2]; ++iz) {
-                    const int b[3] = {hi[0] - ix, hi[1] - iy, hi[2] - iz};
-                    if (b[0] >= nb || b[1] >= nb || b[2] >= nb) continue;
-                    const int li = ((b[0] - bb[0] + 1) * 3 + (b[1] - bb[1] + 1)) * 3 + (b[2] - bb[2] + 1);
-                    if (!s_live[li]) continue;
-                    const int lx = p[0] - 8 * b[0] + 1, ly = p[1] - 8 * b[1] + 1, lz = p[2] - 8 * b[2] + 1;
-                    const float4 tv = tiles[(int64_t)((b[0] * nb + b[1]) * nb + b[2]) * kTS3 +
-                                            (lx * kTS + ly) * kTS + lz];
-                    sx += tv.x; sy += tv.y; sz += tv.z; sm += tv.w;
-                }
+        // the (up to 8) tiles covering the node, loaded all at once (absent
+        // ones read tile slot 0 and are dropped), summed in (ix, iy, iz) order
+        float4 tv[8];
+        bool use[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const int ix = c >> 2, iy = (c >> 1) & 1, iz = c & 1;
+            const int b[3] = {hi[0] - ix, hi[1] - iy, hi[2] - iz};
+            bool ok = ix < nlo[0] && iy < nlo[1] && iz < nlo[2] && b[0] < nb && b[1] < nb && b[2] < nb;
+            int64_t ti = 0;
+            if (ok) {
+                const int li = ((b[0] - bb[0] + 1) * 3 + (b[1] - bb[1] + 1)) * 3 + (b[2] - bb[2] + 1);
+                ok = s_live[li] != 0;
+                const int lx = p[0] - 8 * b[0] + 1, ly = p[1] - 8 * b[1] + 1, lz = p[2] - 8 * b[2] + 1;
+                ti = (int64_t)((b[0] * nb + b[1]) * nb + b[2]) * kTS3 + (lx * kTS + ly) * kTS + lz;
+            }
+            use[c] = ok;
+            tv[c] = tiles[ok ? ti : 0];
+        }
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+            if (use[c]) { sx += tv[c].x; sy += tv[c].y; sz += tv[c].z; sm += tv[c].w; }
         const float4 a4 = make_float4((float)sx, (float)sy, (float)sz, (float)sm);
         vel[node] = grid_node_update(a4, x, y, z, ng, sc.prm.dx, dt, sc.prm.gravity, cols, n_cols, tnow);
     }
