@@ -113,3 +113,40 @@ def test_recorded_and_replayed_adjoints_agree():
         assert np.linalg.norm(ga - gb) <= 1e-3 * np.linalg.norm(gb), call
         ls_a, ls_b = np.asarray(PARAMS[call]) * pa, np.asarray(PARAMS[call]) * pb
         assert np.all(np.abs(ls_a - ls_b) <= 1e-2 * np.linalg.norm(ls_b)), (call, pa, pb)
+
+
+def test_adjoint_matches_gpu_finite_differences_at_250k():
+    """The gradient-check configuration's size (250k particles, 64^3 grid;
+    20 substeps in one segment of the rollout's checkpoints every 20):
+    the adjoint's directional derivatives (initial velocities along a random
+    field, and E) against central differences of the GPU forward itself
+    (relative steps 2e-2 and 1e-2, where FP32 and float-atomic noise of the
+    loss stays far below the differences), within 5%."""
+    sc = S.column_uniform(n_particles=250_000, n_grid=64, seed=4)
+    n = sc.n_particles
+    rng = np.random.default_rng(12)
+    ro = A.Rollout(sc, 20, 1e-4, checkpoint_every=20)
+    v0 = torch.from_numpy(sc.v0).cuda()
+    E, nu, phi = 2.0e5, 0.3, 30.0
+    with torch.no_grad():
+        x_t = ro.forward(v0, 3.0e5, nu, 35.0).clone().double()
+
+    def loss(v, e):
+        with torch.no_grad():
+            x = ro.forward(v, e, nu, phi)
+        return float(torch.mean(torch.sum((x.double() - x_t) ** 2, 1)))
+
+    v = v0.clone().requires_grad_(True)
+    params = torch.tensor([E, nu, phi], dtype=torch.float64, requires_grad=True)
+    x = A.sand_rollout(v, params, ro)
+    torch.mean(torch.sum((x.double() - x_t) ** 2, 1)).backward()
+    D = torch.from_numpy(rng.normal(size=(n, 3)).astype(np.float32)).cuda()
+    eps = 2e-2 * float(v0.abs().mean())
+    fd_v = (loss(v0 + eps * D, E) - loss(v0 - eps * D, E)) / (2 * eps)
+    ad_v = float(torch.sum(v.grad.double() * D.double()))
+    h = 1e-2 * E
+    fd_E = (loss(v0, E + h) - loss(v0, E - h)) / (2 * h)
+    ad_E = float(params.grad[0])
+    print({"v0.D": (ad_v, fd_v), "E": (ad_E, fd_E)})
+    assert abs(ad_v - fd_v) <= 5e-2 * abs(fd_v), (ad_v, fd_v)
+    assert abs(ad_E - fd_E) <= 5e-2 * abs(fd_E), (ad_E, fd_E)
