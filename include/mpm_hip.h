@@ -145,9 +145,12 @@ int mpm_step(const MpmParams* prm, const MpmState* st, const MpmCollider* collid
  * `since_sort` in place.  Every sort_period substeps mpm_bins_substep
  * re-sorts the bins (mpm_bins_sort): the order of particles inside a bin
  * decides how many LDS bank conflicts the fused kernel's gathers and
- * scatters see.  wg_threads and sort_period change speed, not results
- * beyond the order of FP64 tile sums (reduced_mpm_stress_training_amd/tune.py
- * picks them by timing). */
+ * scatters see.  When one bin holds more than a CU's share of all
+ * particles, the fused kernel splits it into up to four slot ranges, each
+ * scattering into its own FP64 tile that the grid update adds before the one
+ * rounding to FP32.  wg_threads, sort_period and the splitting change speed,
+ * not results beyond the order of FP64 tile sums
+ * (reduced_mpm_stress_training_amd/tune.py picks the first two by timing). */
 typedef struct MpmBins {
     int32_t nb;           /* bins per axis = ceil(n_grid / 8) */
     int32_t bin_cap;      /* slots per bin */
@@ -168,7 +171,9 @@ typedef struct MpmBins {
  * 64 int32 status counters (0: particles dropped because bins and overflow
  * were full, 1: particles scattered outside their bin's tile, 2: particles
  * that left the grid; 48-49: the start time, as a double, of a captured run,
- * mpm_batch_graph_launch), tile stamps and nb^3 P2G tiles of 12^3 float4. */
+ * mpm_batch_graph_launch), tile stamps, the fused kernel's schedule, nb^3
+ * P2G tiles of 12^3 float4 and nb^3 FP64 tiles of 12^3 x 4 doubles for the
+ * parts of split bins. */
 int64_t mpm_bins_workspace_bytes(const MpmParams* prm);
 
 /* Byte offset of the status counters in the workspace. */

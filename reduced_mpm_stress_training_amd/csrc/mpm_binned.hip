@@ -10,15 +10,17 @@
 //                 overlapping bins and the spill grid, then
 //                 grid_normalization_and_gravity (mpm_utils.py:229-237) and
 //                 the colliders (mpm_solver_warp.py:429-449, 505-523).  Its
-//                 first workgroup schedules the bins of k_g2p2g.
-//   k_g2p2g       per bin: G2P (mpm_utils.py:241-306) from an LDS tile of
-//                 grid velocities, the sand return map and stress in
-//                 registers, update_u_from_x, and the P2G of the next
-//                 substep (mpm_utils.py:190-227) into an FP64 LDS tile
-//                 (ds_add_f64; FP32 LDS atomics run ~16x slower on gfx950),
-//                 written out as one plain, coalesced tile per bin; every
-//                 particle is written to the slot of its new bin in the other
-//                 plane set.
+//                 first workgroup schedules k_g2p2g's work items.
+//   k_g2p2g       per work item (a bin, or a run of a large bin's slots):
+//                 G2P (mpm_utils.py:241-306) from an LDS tile of grid
+//                 velocities, the sand return map and stress in registers,
+//                 update_u_from_x, and the P2G of the next substep
+//                 (mpm_utils.py:190-227) into an FP64 LDS tile (ds_add_f64;
+//                 FP32 LDS atomics run ~16x slower on gfx950), written out as
+//                 one plain, coalesced tile per item; every particle is
+//                 written to the slot of its new bin in the other plane set.
+//                 One persistent workgroup per CU slot takes the items from a
+//                 queue, largest first.
 // Bins' tiles carry an epoch stamp; the grid update uses the tiles of the
 // current epoch, so nothing needs clearing between substeps (zero_grid,
 // mpm_utils.py:115-123, becomes the epoch bump).  Particles whose next
@@ -50,6 +52,7 @@ constexpr int kSortMax = 6144;        // bins up to this size are processed in b
 constexpr int kSortPeriod = 32;       // substeps between bank-order sorts of the bins
 constexpr int kStageFloats = 6144;    // LDS staging of the sort's copy (24 KB: two bins per CU)
 static_assert(kStageFloats >= kSortMax, "a bin's plane must fit the staging buffer");
+constexpr int kParts = 4;             // work items (each with its own P2G tile) a bin may be split into
 
 // Plane sets are bin-major: bin b of a scene with `cap` slots per bin is a
 // block of MPM_NPLANES planes of cap floats starting at float offset
@@ -63,6 +66,8 @@ struct BinGeom {
     int max_cap;       // largest slots-per-bin of any scene
     int64_t ovf_off;   // float offset of the overflow run
     int64_t stride;
+    int n_cu;          // compute units of the device (sizes the fused kernel's work items)
+    int pad;
 };
 
 // One scene: the mpm_bins_* entry points run one, the mpm_batch_* entry
@@ -142,47 +147,211 @@ __device__ __forceinline__ int xcd_bin(int wg, int n_bins) {
     return wg;
 }
 
-// Workgroup -> bin schedule of k_g2p2g.  The command processor hands
-// workgroup w to XCD w mod 8, and each XCD starts its workgroups in order on
-// whichever CU frees up first.  Listing the non-empty bins by decreasing
-// work (longest processing time first) therefore gives every XCD every 8th
-// bin of a sorted list -- an even share -- and fills each XCD's CUs greedily
-// from the largest bin down.  Mapping workgroups to bins by index instead
-// ties XCDs to rows of bins (bin mod 8 = z row mod 8), and a settled sand
-// pile a few bins high leaves several XCDs idle.
-// order[0 .. n_bins) = bins, -1 past the last non-empty one;
-// order[n_bins] = the epoch of the bin counts it was made from.
+// Schedules of k_g2p2g and of the bank-order sort, built from the bin
+// counts by the grid update.  Work is counted in passes: a bin of n
+// particles takes ceil(n / 512) passes of a 512-thread workgroup.
+//
+// Both kernels take one workgroup per list entry, and the command processor
+// hands workgroup w to XCD w mod 8, each XCD starting its workgroups in
+// order on whichever CU frees up first.  Listing the work by decreasing
+// size (longest processing time first) gives every XCD every 8th entry of a
+// sorted list -- an even share -- and fills each XCD's CUs greedily from the
+// largest down.
+//
+// For k_g2p2g a whole bin can be several times a CU's share of the work
+// (250k particles on a 64^3 grid: 80 bins of up to 6 passes for 256 CUs),
+// so bins above q passes, q about half a CU's share of all passes, are split
+// into up to kParts work items of consecutive slot ranges; each item
+// scatters into its own tile (part p of bin b: tile p * n_bins + b) and the
+// grid update sums the parts.
+//
+// order[0 .. n_bins): bins for the sort, -1 past the last non-empty one;
+// order[n_bins + kQEpoch]: the epoch of the bin counts it was made from;
+// order[n_bins + kQFirst + w], w < n_bins: work item w of k_g2p2g,
+// (bin << 4) | (part << 2) | (parts - 1), -1 past the last.  Bins are split
+// only while the items fit in n_bins slots, so the fused kernel's grid stays
+// one workgroup per bin (plus the overflow run's).
 constexpr int kSchedKeys = 64;
+constexpr int kQEpoch = 0, kQFirst = 1;
 
-__device__ void schedule_bins(const int* __restrict__ cnt_cur, int n_bins, int cap, int epoch,
-                              int* __restrict__ order) {
-    __shared__ int s_hist[kSchedKeys];
-    __shared__ int s_cursor[kSchedKeys];
-    __shared__ int s_total;
-    const int tid = threadIdx.x;
-    if (tid < kSchedKeys) s_hist[tid] = 0;
-    __syncthreads();
-    // key = loop iterations of the bin's workgroup (its cost), capped
-    for (int b = tid; b < n_bins; b += kWG) {
-        const int n = min(cnt_cur[b], cap);
-        if (n > 0) atomicAdd(&s_hist[min((n + kWG - 1) / kWG, kSchedKeys - 1)], 1);
+__device__ __forceinline__ int bin_passes(int n) { return (n + kWG - 1) / kWG; }
+
+static_assert(kParts == 4, "split_bin and ceil_div_parts count up to four parts");
+
+// ceil(x / k) for k = 1 .. 4 and 0 <= x < 32766, without a division
+__device__ __forceinline__ int ceil_div_parts(int x, int k) {
+    return k == 1 ? x : k == 2 ? (x + 1) >> 1 : k == 3 ? ((x + 2) * 21846) >> 16 : (x + 3) >> 2;
+}
+
+// parts of a bin of p passes when items should not exceed q passes, and the
+// passes of each (the last one may have fewer): k = min(4, ceil(p / q)),
+// per = ceil(p / k), parts = ceil(p / per) <= k
+__device__ __forceinline__ void split_bin(int p, int q, int* parts, int* per) {
+    const int k = 1 + (p > q) + (p > 2 * q) + (p > 3 * q);
+    const int pp = ceil_div_parts(p, k);
+    *per = pp;
+    *parts = 1 + (p > pp) + (p > 2 * pp) + (p > 3 * pp);
+}
+
+// The schedule is built by one workgroup and sits on the grid update's
+// critical path: each thread reads the counts of its bins (tid + j * kWG)
+// kSchedRegs at a time, all loads of a chunk issued together, and keeps
+// their passes in registers; grids of up to kWG * kSchedRegs bins (a 128^3
+// grid's 4096) read them once for all passes.
+constexpr int kSchedRegs = 8;
+
+__device__ __forceinline__ void load_passes(const int* __restrict__ cnt_cur, int n_bins, int cap, int chunk,
+                                            int pv[kSchedRegs]) {
+    const int b0 = chunk * kWG * kSchedRegs + (int)threadIdx.x;
+#pragma unroll
+    for (int j = 0; j < kSchedRegs; ++j) {
+        const int b = b0 + j * kWG;
+        pv[j] = b < n_bins ? bin_passes(min(cnt_cur[b], cap)) : 0;
     }
-    __syncthreads();
-    if (tid == 0) {
-        int pos = 0;
+}
+
+// prefix sums of s_hist in decreasing key order into s_cursor; s_total = the
+// count, s_max = the largest key present (0 if none)
+__device__ __forceinline__ void hist_cursors(const int* s_hist, int* s_cursor, int* s_total, int* s_max) {
+    if (threadIdx.x == 0) {
+        int pos = 0, top = 0;
         for (int k = kSchedKeys - 1; k >= 1; --k) {
             s_cursor[k] = pos;
             pos += s_hist[k];
+            if (top == 0 && s_hist[k] > 0) top = k;
         }
-        s_total = pos;
+        *s_total = pos;
+        *s_max = top;
     }
+}
+
+// counter[key] += 1 for every active lane, one LDS atomic per distinct key
+// in the wave (most bins share a key, and lanes adding to one LDS address
+// are serialized); returns what a per-lane atomicAdd would have returned.
+// Called by every lane of the wave.
+__device__ __forceinline__ int wave_add(int* counter, int key, bool active) {
+    const int lane = __lane_id();
+    unsigned long long todo = __ballot(active);
+    int old = 0;
+    while (todo) {
+        const int leader = __ffsll((long long)todo) - 1;
+        const int k0 = __shfl(key, leader);
+        const unsigned long long same = todo & __ballot(key == k0);
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&counter[k0], (int)__popcll(same));
+        base = __shfl(base, leader);
+        if ((same >> lane) & 1ull) old = base + (int)__popcll(same & ((1ull << lane) - 1ull));
+        todo &= ~same;
+    }
+    return old;
+}
+
+__device__ void schedule_bins(const int* __restrict__ cnt_cur, const BinGeom& g, int epoch,
+                              int* __restrict__ order) {
+    __shared__ int s_hist[kSchedKeys];
+    __shared__ int s_cursor[kSchedKeys];
+    __shared__ int s_total, s_max, s_passes, s_items;
+    const int tid = threadIdx.x;
+    const int n_bins = g.n_bins, cap = g.max_cap;
+    const int n_chunks = (n_bins + kWG * kSchedRegs - 1) / (kWG * kSchedRegs);
+    const bool reload = n_chunks > 1;
+    int* queue = order + n_bins;
+    int pv[kSchedRegs];
+    if (tid < kSchedKeys) s_hist[tid] = 0;
+    if (tid == 0) s_passes = s_items = 0;
     __syncthreads();
-    for (int b = tid; b < n_bins; b += kWG) {
-        const int n = min(cnt_cur[b], cap);
-        if (n > 0) order[atomicAdd(&s_cursor[min((n + kWG - 1) / kWG, kSchedKeys - 1)], 1)] = b;
+    // bins keyed by their passes (their cost), capped
+    int passes = 0;
+    for (int c = 0; c < n_chunks; ++c) {
+        load_passes(cnt_cur, n_bins, cap, c, pv);
+#pragma unroll
+        for (int j = 0; j < kSchedRegs; ++j) {
+            wave_add(s_hist, min(pv[j], kSchedKeys - 1), pv[j] > 0);
+            passes += pv[j];
+        }
     }
-    for (int p = s_total + tid; p < n_bins; p += kWG) order[p] = -1;
-    if (tid == 0) order[n_bins] = epoch;
+    if (passes > 0) atomicAdd(&s_passes, passes);
+    __syncthreads();
+    hist_cursors(s_hist, s_cursor, &s_total, &s_max);
+    __syncthreads();
+    // Split only where a bin exceeds a CU's share of all passes (whole bins
+    // then leave CUs idle); items then take at most half a share each, and
+    // all items must fit in n_bins slots.
+    const int share = s_passes / max(1, g.n_cu);
+    bool split = s_max > max(1, share) && s_max < kSchedKeys - 1;
+    const int q = max(1, share / 2);
+    // the sort's list (and k_g2p2g's when no bin is split)
+    for (int c = 0; c < n_chunks; ++c) {
+        if (reload) load_passes(cnt_cur, n_bins, cap, c, pv);
+#pragma unroll
+        for (int j = 0; j < kSchedRegs; ++j) {
+            const int pos = wave_add(s_cursor, min(pv[j], kSchedKeys - 1), pv[j] > 0);
+            const int b = c * kWG * kSchedRegs + j * kWG + tid;
+            if (pv[j] > 0) {
+                order[pos] = b;
+                if (!split) queue[kQFirst + pos] = b << 4;
+            }
+        }
+    }
+    for (int p = s_total + tid; p < n_bins; p += kWG) {
+        order[p] = -1;
+        if (!split) queue[kQFirst + p] = -1;
+    }
+    if (split) {
+        int items = 0;
+        for (int c = 0; c < n_chunks; ++c) {
+            if (reload) load_passes(cnt_cur, n_bins, cap, c, pv);
+#pragma unroll
+            for (int j = 0; j < kSchedRegs; ++j)
+                if (pv[j] > 0) {
+                    int parts, per;
+                    split_bin(pv[j], q, &parts, &per);
+                    items += parts;
+                }
+        }
+        if (items > 0) atomicAdd(&s_items, items);
+        __syncthreads();   // s_total of the sort's list has been read
+        split = s_items <= n_bins;
+        if (tid < kSchedKeys) s_hist[tid] = 0;
+        __syncthreads();
+        for (int c = 0; c < n_chunks; ++c) {
+            if (reload) load_passes(cnt_cur, n_bins, cap, c, pv);
+#pragma unroll
+            for (int j = 0; j < kSchedRegs; ++j) {
+                const int p = pv[j];
+                int parts = 0, per = 1;
+                if (p > 0) {
+                    if (split) split_bin(p, q, &parts, &per); else { parts = 1; per = p; }
+                }
+#pragma unroll
+                for (int i = 0; i < kParts; ++i)
+                    wave_add(s_hist, min(max(min(per, p - i * per), 0), kSchedKeys - 1), i < parts);
+            }
+        }
+        __syncthreads();
+        hist_cursors(s_hist, s_cursor, &s_total, &s_max);
+        __syncthreads();
+        for (int c = 0; c < n_chunks; ++c) {
+            if (reload) load_passes(cnt_cur, n_bins, cap, c, pv);
+#pragma unroll
+            for (int j = 0; j < kSchedRegs; ++j) {
+                const int p = pv[j];
+                const int b = c * kWG * kSchedRegs + j * kWG + tid;
+                int parts = 0, per = 1;
+                if (p > 0) {
+                    if (split) split_bin(p, q, &parts, &per); else { parts = 1; per = p; }
+                }
+#pragma unroll
+                for (int i = 0; i < kParts; ++i) {
+                    const int pos =
+                        wave_add(s_cursor, min(max(min(per, p - i * per), 0), kSchedKeys - 1), i < parts);
+                    if (i < parts) queue[kQFirst + pos] = (b << 4) | (i << 2) | (parts - 1);
+                }
+            }
+        }
+        for (int p = s_total + tid; p < n_bins; p += kWG) queue[kQFirst + p] = -1;
+    }
+    if (tid == 0) queue[kQEpoch] = epoch;
 }
 
 __device__ __forceinline__ void mark_block(int* __restrict__ gflag, int nb, int x, int y, int z) {
@@ -219,15 +388,17 @@ __device__ __forceinline__ void scatter_tile(double* tile, int i0, const AxisWei
 }
 
 // Writes a bin's LDS tile (nodes b0-1 .. b0+10) to its slot of the global
-// tile buffer as float4 (mv, m) and stamps it with the epoch.
-__device__ void write_tile(const double* s_acc, float4* __restrict__ tiles, int* __restrict__ stamps, int bin,
-                           int epoch) {
+// tile buffer as float4 (mv, m) and stamps it with the epoch.  tid: the
+// thread's index in the workgroup.
+__device__ __forceinline__ void write_tile(const double* s_acc, float4* __restrict__ tiles,
+                                           int* __restrict__ stamps, int bin, int epoch, int tid) {
     float4* out = tiles + (int64_t)bin * kTS3;
-    for (int t = threadIdx.x; t < kTS3; t += blockDim.x)
+    for (int t = tid; t < kTS3; t += blockDim.x)
         out[t] = make_float4((float)s_acc[t], (float)s_acc[kTS3 + t], (float)s_acc[2 * kTS3 + t],
                              (float)s_acc[3 * kTS3 + t]);
-    if (threadIdx.x == 0) stamps[bin] = epoch;
+    if (tid == 0) stamps[bin] = epoch;
 }
+
 
 struct SubstepArgs {
     SceneTab tab;
@@ -243,8 +414,11 @@ struct SubstepArgs {
     int* __restrict__ gflag;
     int* __restrict__ status;
     float4* __restrict__ tiles;
-    int* __restrict__ stamps;
-    const int* __restrict__ order;  // schedule (schedule_bins)
+    int* __restrict__ stamps;       // epoch of each bin's tile; -epoch: split, tiles in ptiles
+    double* __restrict__ ptiles;    // FP64 tiles of split bins' parts, one slot per work item
+    int* __restrict__ pcount;       // parts of each split bin
+    int* __restrict__ pslot;        // ptiles slot of part p of bin b: pslot[p * n_bins + b]
+    int* __restrict__ queue;        // work items (schedule_bins): order + n_bins
     int sched_epoch;                // epoch the schedule must carry to be used
     int store_all;                  // 0: write only the planes the next substep reads (see g2p2g_particle)
 };
@@ -462,13 +636,15 @@ __device__ __forceinline__ void g2p2g_particle(const SubstepArgs& a, const Scene
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kWG)
     k_grid_bins(float4* __restrict__ acc, float4* __restrict__ vel, int* __restrict__ gflag,
-                const int* __restrict__ stamps, const float4* __restrict__ tiles, int* __restrict__ cnt_nxt,
+                const int* __restrict__ stamps, const float4* __restrict__ tiles, const int* __restrict__ pcount,
+                const int* __restrict__ pslot, const double* __restrict__ ptiles, int* __restrict__ cnt_nxt,
                 const int* __restrict__ cnt_cur, int* __restrict__ order, BinGeom g, SceneTab tab,
                 ColliderSet cs, float time, float dt, int epoch, int clear_only, const double* __restrict__ t_base) {
-    __shared__ int s_live[27];
-    __shared__ int s_any;
+    __shared__ int s_live[27];             // neighbour bin c has a whole-bin tile of this epoch
+    __shared__ int s_pslot[kParts * 27];   // FP64 tile slot of part p of neighbour bin c this epoch, or -1
+    __shared__ int s_any, s_anyp;
     if (blockIdx.x == 0) {
-        if (order) schedule_bins(cnt_cur, g.n_bins, g.max_cap, epoch, order);
+        if (order) schedule_bins(cnt_cur, g, epoch, order);
         return;
     }
     const int blk = blockIdx.x - 1;   // global block (= global bin index)
@@ -482,6 +658,8 @@ __global__ void __launch_bounds__(kWG)
     acc += sc.node_base;
     vel += sc.node_base;
     stamps += sc.bin_base;
+    pcount += sc.bin_base;
+    pslot += sc.bin_base;
     tiles += (int64_t)sc.bin_base * kTS3;
     const MpmCollider* cols = tab.dev ? tab.cols + sc.col_base : cs.c;
     const int n_cols = tab.dev ? sc.n_cols : cs.n;
@@ -490,19 +668,28 @@ __global__ void __launch_bounds__(kWG)
         cnt_nxt[g.n_bins + 1 + blk] = 0;  // live particles
         if (blk == 0) cnt_nxt[g.n_bins] = 0;
     }
-    if (tid == 0) s_any = 0;
+    if (tid == 0) s_any = s_anyp = 0;
     __syncthreads();
     if (tid < 27) {
-        const int x = bx + tid / 9 - 1, y = by + (tid / 3) % 3 - 1, z = bz + tid % 3 - 1;
-        int live = 0;
-        if (!clear_only && x >= 0 && y >= 0 && z >= 0 && x < nb && y < nb && z < nb)
-            live = stamps[(x * nb + y) * nb + z] == epoch;
-        s_live[tid] = live;
+        const int c = tid;
+        const int x = bx + c / 9 - 1, y = by + (c / 3) % 3 - 1, z = bz + c % 3 - 1;
+        const bool in = !clear_only && x >= 0 && y >= 0 && z >= 0 && x < nb && y < nb && z < nb;
+        const int lb = in ? (x * nb + y) * nb + z : 0;
+        const int st = in ? stamps[lb] : 0;
+        const int live = in && st == epoch;
+        s_live[c] = live;
         if (live) atomicOr(&s_any, 1);
+        // a split bin (stamp -epoch): its parts' FP64 tile slots
+        const bool split = in && st == -epoch && epoch > 0;
+        const int n_parts = split ? min(pcount[lb], kParts) : 0;
+#pragma unroll
+        for (int part = 0; part < kParts; ++part)
+            s_pslot[part * 27 + c] = part < n_parts ? pslot[(int64_t)part * g.n_bins + lb] : -1;
+        if (split) atomicOr(&s_anyp, 1);
     }
     const int spill = gflag[blk];
     __syncthreads();
-    if (!spill && !s_any) return;
+    if (!spill && !s_any && !s_anyp) return;
     for (int t = tid; t < 512; t += kWG) {
         const int x = bx * 8 + (t >> 6), y = by * 8 + ((t >> 3) & 7), z = bz * 8 + (t & 7);
         if (x >= ng || y >= ng || z >= ng) continue;
@@ -522,28 +709,67 @@ __global__ void __launch_bounds__(kWG)
             nlo[d] = (p[d] + 1 - 8 * hi[d] <= 2 && hi[d] >= 1) ? 2 : 1;
         }
         const int bb[3] = {bx, by, bz};
-        // the (up to 8) tiles covering the node, loaded all at once (absent
-        // ones read tile slot 0 and are dropped), summed in (ix, iy, iz) order
-        float4 tv[8];
-        bool use[8];
+        // the (up to 8) bins covering the node: their neighbour index and the
+        // node's index in their tiles
+        int li[8], lt[8];
+        bool cov[8];
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
             const int ix = c >> 2, iy = (c >> 1) & 1, iz = c & 1;
             const int b[3] = {hi[0] - ix, hi[1] - iy, hi[2] - iz};
-            bool ok = ix < nlo[0] && iy < nlo[1] && iz < nlo[2] && b[0] < nb && b[1] < nb && b[2] < nb;
-            int64_t ti = 0;
-            if (ok) {
-                const int li = ((b[0] - bb[0] + 1) * 3 + (b[1] - bb[1] + 1)) * 3 + (b[2] - bb[2] + 1);
-                ok = s_live[li] != 0;
-                const int lx = p[0] - 8 * b[0] + 1, ly = p[1] - 8 * b[1] + 1, lz = p[2] - 8 * b[2] + 1;
-                ti = (int64_t)((b[0] * nb + b[1]) * nb + b[2]) * kTS3 + (lx * kTS + ly) * kTS + lz;
-            }
-            use[c] = ok;
-            tv[c] = tiles[ok ? ti : 0];
+            cov[c] = ix < nlo[0] && iy < nlo[1] && iz < nlo[2] && b[0] < nb && b[1] < nb && b[2] < nb;
+            li[c] = ((b[0] - bb[0] + 1) * 3 + (b[1] - bb[1] + 1)) * 3 + (b[2] - bb[2] + 1);
+            const int lx = p[0] - 8 * b[0] + 1, ly = p[1] - 8 * b[1] + 1, lz = p[2] - 8 * b[2] + 1;
+            lt[c] = (lx * kTS + ly) * kTS + lz;
         }
+        // whole-bin tiles, loaded all at once (absent ones read tile slot 0
+        // and are dropped), summed in (ix, iy, iz) order
+        if (s_any) {
 #pragma unroll
-        for (int c = 0; c < 8; ++c)
-            if (use[c]) { sx += tv[c].x; sy += tv[c].y; sz += tv[c].z; sm += tv[c].w; }
+            for (int h = 0; h < 8; h += 4) {
+                float4 tv[4];
+                bool use[4];
+#pragma unroll
+                for (int cc = 0; cc < 4; ++cc) {
+                    const int c = h + cc;
+                    const int ix = c >> 2, iy = (c >> 1) & 1, iz = c & 1;
+                    use[cc] = cov[c] && s_live[li[c]] != 0;
+                    const int64_t ti =
+                        (int64_t)(((hi[0] - ix) * nb + (hi[1] - iy)) * nb + (hi[2] - iz)) * kTS3 + lt[c];
+                    tv[cc] = tiles[use[cc] ? ti : 0];
+                }
+#pragma unroll
+                for (int cc = 0; cc < 4; ++cc)
+                    if (use[cc]) { sx += tv[cc].x; sy += tv[cc].y; sz += tv[cc].z; sm += tv[cc].w; }
+            }
+        }
+        // the FP64 tiles of split bins' parts
+        if (s_anyp) {
+            // (one tile at a time: this path is taken only next to split
+            // bins, and the registers of a wider batch would lower the
+            // occupancy of every grid block)
+#pragma unroll 1
+            for (int k = 0; k < 8 * kParts; ++k) {
+                const int part = k >> 3, c = k & 7;
+                const int q[3] = {x, y, z}, d[3] = {c >> 2, (c >> 1) & 1, c & 1};
+                int b[3];
+                bool cv = true;
+#pragma unroll
+                for (int e = 0; e < 3; ++e) {
+                    const int h = (q[e] + 1) >> 3;
+                    b[e] = h - d[e];
+                    cv = cv && b[e] < nb && (d[e] == 0 || (q[e] + 1 - 8 * h <= 2 && h >= 1));
+                }
+                if (!cv) continue;
+                const int slot =
+                    s_pslot[part * 27 + ((b[0] - bx + 1) * 3 + (b[1] - by + 1)) * 3 + (b[2] - bz + 1)];
+                if (slot < 0) continue;
+                const int lt = ((q[0] - 8 * b[0] + 1) * kTS + (q[1] - 8 * b[1] + 1)) * kTS + (q[2] - 8 * b[2] + 1);
+                const double* src = ptiles + ((int64_t)slot * kTS3 + lt) * 4;
+                const double2 t0 = *(const double2*)src, t1 = *(const double2*)(src + 2);
+                sx += t0.x; sy += t0.y; sz += t1.x; sm += t1.y;
+            }
+        }
         const float4 a4 = make_float4((float)sx, (float)sy, (float)sz, (float)sm);
         vel[node] = grid_node_update(a4, x, y, z, ng, sc.prm.dx, dt, sc.prm.gravity, cols, n_cols, tnow);
     }
@@ -563,47 +789,38 @@ __device__ __forceinline__ int bank_class(const int base[3], const int b0[3]) {
     return ((((base[0] - b0[0] + 1) * kTS + (base[1] - b0[1] + 1)) * kTS + (base[2] - b0[2] + 1)) & 31);
 }
 
-// WG: threads per bin workgroup (256 or 512).  At the kernel's ~226 VGPRs a
-// SIMD holds two waves either way; 256-thread workgroups put two bins on a
-// CU at once, so one bin's tile load and write-out overlap the other's loop.
+// One work item of k_g2p2g: slots [lo, hi) of `bin`, part `part` of
+// `parts` (the bin's passes dealt out in runs of ceil(passes / parts)), with
+// its own velocity and P2G tiles.  Called by every thread of the workgroup.
 template <int WG>
-__global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2, 2))) k_g2p2g(SubstepArgs a) {
-    __shared__ float s_vel[3 * kTS3];
-    __shared__ double s_acc[4 * kTS3];
-    __shared__ int s_stay;
-    __shared__ int s_base;
+__device__ __forceinline__ void g2p2g_item(const SubstepArgs& a, int bin, int part, int parts, float* s_vel,
+                                           double* s_acc, int* s_stay, int* s_base, int* s_live) {
     const int tid = threadIdx.x;
-    const int wg = blockIdx.x;
     const BinGeom& g = a.g;
-    if (wg >= g.n_bins) {
-        // overflow run of the current plane set: global gather and spill scatter
-        const int n_ovf = min(a.cnt_cur[g.n_bins], g.ovf_cap);
-        const int i = (wg - g.n_bins) * WG + tid;
-        const int b0[3] = {0, 0, 0};
-        if ((wg - g.n_bins) * WG >= n_ovf) return;
-        const int64_t off = ovf_off(g, i);
-        ParticleIn q;
-        load_particle(a.src, off, g.ovf_blk, i < n_ovf, q);
-        const SceneDesc sc = scene_of_pid(a.tab, __float_as_int(q.pid));   // the particle's own scene
-        g2p2g_particle(a, sc, i < n_ovf, off, g.ovf_blk, g.n_bins, -1, nullptr, b0, nullptr, nullptr, q);
-        return;
-    }
-    const int bin = (a.order && a.order[g.n_bins] == a.sched_epoch) ? a.order[wg] : xcd_bin(wg, g.n_bins);
-    if (bin < 0) return;
     const SceneDesc sc = scene_of_bin(a.tab, bin);
     const int cap = sc.cap;
-    const int n_in = min(a.cnt_cur[bin], cap);
-    if (n_in == 0) return;
+    const int n_all = min(a.cnt_cur[bin], cap);
+    const int per = ceil_div_parts(bin_passes(n_all), parts) * kWG;
+    const int lo = part * per, hi = min(n_all, lo + per);
+    if (lo >= hi) return;
     const int ng = sc.prm.n_grid;
     int b0[3];
     bin_origin(bin - sc.bin_base, sc.nb, b0);
     const float4* svel = a.vel + sc.node_base;
     const int64_t off0 = bin_off(sc, bin);
-    // the run stayers will fill: as many slots as the bin has live particles
-    const int n_live = min(a.cnt_cur[g.n_bins + 1 + bin], n_in);
+    // the run stayers will fill: as many slots as the item has live
+    // particles (a whole bin: the live count kept with the bin counts; a
+    // part: counted here)
     if (tid == 0) {
-        s_base = atomicAdd(&a.cnt_nxt[bin], n_live);
-        s_stay = 0;
+        *s_stay = 0;
+        *s_live = 0;
+        if (parts == 1) *s_base = atomicAdd(&a.cnt_nxt[bin], min(a.cnt_cur[g.n_bins + 1 + bin], n_all));
+    }
+    if (parts > 1) {
+        __syncthreads();
+        int live = 0;
+        for (int i = lo + tid; i < hi; i += WG) live += __float_as_int(a.src[off0 + (int64_t)MPM_P_PID * cap + i]) >= 0;
+        if (live) atomicAdd(s_live, live);
     }
     for (int t = tid; t < kTS3; t += WG) {
         const int x = b0[0] - 1 + t / (kTS * kTS), y = b0[1] - 1 + (t / kTS) % kTS, z = b0[2] - 1 + t % kTS;
@@ -615,26 +832,93 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2, 2)))
     }
     for (int t = tid; t < 4 * kTS3; t += WG) s_acc[t] = 0.0;
     __syncthreads();
+    if (parts > 1) {
+        if (tid == 0) *s_base = atomicAdd(&a.cnt_nxt[bin], *s_live);
+        __syncthreads();
+    }
+    const int n_live =
+        parts > 1 ? __builtin_amdgcn_readfirstlane(*s_live) : min(a.cnt_cur[g.n_bins + 1 + bin], n_all);
     // Particles are processed in storage order.  Storage is kept in bank order
     // (k_bins_interleave after insertion and every kSortPeriod substeps;
     // stayers are written back in processing order), so each 32-lane half
     // mostly sees distinct bank classes of its base node.
-    const int stay_base = s_base;
-    ParticleIn q;
-    load_particle(a.src, off0 + tid, cap, tid < n_in, q);
-    for (int it = 0; it < n_in; it += WG) {
-        const int i = it + tid;
-        const ParticleIn cur = q;
-        if (it + WG < n_in) load_particle(a.src, off0 + i + WG, cap, i + WG < n_in, q);  // prefetch
-        g2p2g_particle(a, sc, i < n_in, off0 + i, cap, bin, stay_base, &s_stay, b0, s_vel, s_acc, cur);
+    const int stay_base = __builtin_amdgcn_readfirstlane(*s_base);
+    {
+        const int64_t offp = off0 + lo;   // the item's first slot
+        const int n_in = hi - lo;
+        ParticleIn q;
+        load_particle(a.src, offp + tid, cap, tid < n_in, q);
+        for (int it = 0; it < n_in; it += WG) {
+            const int i = it + tid;
+            const ParticleIn cur = q;
+            if (it + WG < n_in) load_particle(a.src, offp + i + WG, cap, i + WG < n_in, q);  // prefetch
+            g2p2g_particle(a, sc, i < n_in, offp + i, cap, bin, stay_base, s_stay, b0, s_vel, s_acc, cur);
+        }
     }
     __syncthreads();
     // reserved slots nobody stayed in become holes (pid -1)
-    const int hole_lo = stay_base + s_stay, hole_hi = min(stay_base + n_live, cap);
+    const int hole_lo = stay_base + *s_stay, hole_hi = min(stay_base + n_live, cap);
     for (int h = hole_lo + tid; h < hole_hi; h += WG)
         a.dst[off0 + (int64_t)MPM_P_PID * cap + h] = __int_as_float(-1);
-    if (tid == 0) atomicAdd(&a.cnt_nxt[g.n_bins + 1 + bin], max(0, min(s_stay, cap - stay_base)));
-    write_tile(s_acc, a.tiles, a.stamps, bin, a.epoch_out);
+    if (tid == 0) atomicAdd(&a.cnt_nxt[g.n_bins + 1 + bin], max(0, min(*s_stay, cap - stay_base)));
+    if (parts == 1) {
+        write_tile(s_acc, a.tiles, a.stamps, bin, a.epoch_out, tid);
+    } else {
+        // a part's sums stay FP64 until the grid update has added the parts:
+        // rounded once, as a whole bin's tile, whichever particles fell
+        // into which part
+        double* out = a.ptiles + (int64_t)blockIdx.x * kTS3 * 4;
+        for (int t = tid; t < kTS3; t += WG) {
+            *(double2*)(out + 4 * t) = make_double2(s_acc[t], s_acc[kTS3 + t]);
+            *(double2*)(out + 4 * t + 2) = make_double2(s_acc[2 * kTS3 + t], s_acc[3 * kTS3 + t]);
+        }
+        if (tid == 0) {
+            a.pslot[part * g.n_bins + bin] = blockIdx.x;
+            if (part == 0) {
+                // the parts with slots: those whose range starts below n_all
+                const int pb = bin_passes(n_all), pp = per / kWG;
+                a.pcount[bin] = 1 + (pb > pp) + (pb > 2 * pp) + (pb > 3 * pp);
+                a.stamps[bin] = -a.epoch_out;
+            }
+        }
+    }
+}
+
+// Block `blk` (WG particles) of the overflow run: global gather and spill
+// scatter, no tiles.
+template <int WG>
+__device__ __forceinline__ void g2p2g_overflow(const SubstepArgs& a, int blk) {
+    const BinGeom& g = a.g;
+    const int n_ovf = min(a.cnt_cur[g.n_bins], g.ovf_cap);
+    const int i = blk * WG + (int)threadIdx.x;
+    if (blk * WG >= n_ovf) return;
+    const int b0[3] = {0, 0, 0};
+    const int64_t off = ovf_off(g, i);
+    ParticleIn q;
+    load_particle(a.src, off, g.ovf_blk, i < n_ovf, q);
+    const SceneDesc sc = scene_of_pid(a.tab, __float_as_int(q.pid));   // the particle's own scene
+    g2p2g_particle(a, sc, i < n_ovf, off, g.ovf_blk, g.n_bins, -1, nullptr, b0, nullptr, nullptr, q);
+}
+
+// WG: threads per workgroup (256 or 512).  At the kernel's ~226 VGPRs a SIMD
+// holds two waves either way.  Workgroup w < n_bins runs work item w of the
+// schedule of this epoch (schedule_bins: at most n_bins items, largest
+// first; -1 past the last) or, without one, bin w whole; the workgroups
+// after them run the overflow run in blocks of WG particles.
+template <int WG>
+__global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2, 2))) k_g2p2g(SubstepArgs a) {
+    __shared__ float s_vel[3 * kTS3];
+    __shared__ double s_acc[4 * kTS3];
+    __shared__ int s_stay, s_base, s_live;
+    const BinGeom& g = a.g;
+    const int wg = blockIdx.x;
+    if (wg >= g.n_bins) {
+        g2p2g_overflow<WG>(a, wg - g.n_bins);
+        return;
+    }
+    const int it = a.queue[kQEpoch] == a.sched_epoch ? a.queue[kQFirst + wg] : (wg << 4);
+    if (it < 0) return;
+    g2p2g_item<WG>(a, it >> 4, (it >> 2) & 3, (it & 3) + 1, s_vel, s_acc, &s_stay, &s_base, &s_live);
 }
 
 // ---------------------------------------------------------------------------
@@ -699,7 +983,7 @@ __global__ void __launch_bounds__(kWG)
     }
     if (!ovf) {
         __syncthreads();
-        write_tile(s_acc, tiles, stamps, bin, epoch_out);
+        write_tile(s_acc, tiles, stamps, bin, epoch_out, threadIdx.x);
     }
 }
 
@@ -827,11 +1111,15 @@ __global__ void __launch_bounds__(kWG)
 // fill n ints with v (bin counters; a kernel rather than a memset so that the
 // sequence can be captured in a graph)
 // Tile stamps at the start of a captured run: the pending tiles (stamp
-// `from`) become the run's first epoch `to`, every other stamp is voided, so
-// that no tile of an earlier replay matches an epoch the run reuses.
+// `from`, or -from for a split bin) become the run's first epoch `to` (-to),
+// every other stamp is voided, so that no tile of an earlier replay matches
+// an epoch the run reuses.
 __global__ void k_stamp_rebase(int* __restrict__ stamps, int n, int from, int to) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) stamps[i] = stamps[i] == from ? to : (int)0x80000000;
+    if (i < n) {
+        const int st = stamps[i];
+        stamps[i] = st == from ? to : st == -from ? -to : (int)0x80000000;
+    }
 }
 
 __global__ void k_time_set(double* __restrict__ t, double v, int add) {
@@ -910,15 +1198,33 @@ struct WsLayout {
     float4* vel;       // grid velocities (v, m)
     int* gflag;        // spill flags per grid block
     int* status;       // 64 counters
-    int* stamps;       // epoch of each bin's tile
-    int* order;        // k_g2p2g schedule: one entry per bin + its epoch
+    int* stamps;       // epoch of each bin's tile (-epoch: the bin was split; see write_tile)
+    int* pcount;       // parts of each split bin
+    int* pslot;        // FP64 tile slot of part p of split bin b (p * nbins + b)
+    int* order;        // schedules of the sort and of k_g2p2g (schedule_bins)
     float4* tiles;     // one tile of kTS^3 nodes per bin
+    double* ptiles;    // nbins FP64 tiles of kTS^3 nodes (4 doubles each) for the parts of split bins
     SceneDesc* table;  // batch: scene table
     MpmCollider* cols; // batch: colliders
     int64_t bytes;
 };
 
 int64_t align256(int64_t b) { return ((b + 255) / 256) * 256; }
+
+// Compute units of the current device (sizes the fused kernel's work items),
+// queried once per device.  A failed query is not cached and reads as 256:
+// the count only steers how finely large bins are split.
+int device_cus() {
+    static int cache[64];
+    int d = 0;
+    const bool dev_ok = hipGetDevice(&d) == hipSuccess && d >= 0;
+    if (dev_ok && d < 64 && cache[d] > 0) return cache[d];
+    int n = 0;
+    if (!dev_ok || hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, d) != hipSuccess || n < 1)
+        return 256;
+    if (d < 64) cache[d] = n;
+    return n;
+}
 
 WsLayout ws_layout(int64_t nodes, int64_t nbins, int n_scenes, int n_cols, void* base) {
     char* p = (char*)base;
@@ -935,10 +1241,16 @@ WsLayout ws_layout(int64_t nodes, int64_t nbins, int n_scenes, int n_cols, void*
     off += 256;
     w.stamps = (int*)(p + off);
     off += align256(nbins * 4);
+    w.pcount = (int*)(p + off);
+    off += align256(nbins * 4);
+    w.pslot = (int*)(p + off);
+    off += align256(kParts * nbins * 4);
     w.order = (int*)(p + off);
-    off += align256((nbins + 2) * 4);
+    off += align256((2 * nbins + kQFirst) * 4);
     w.tiles = (float4*)(p + off);
     off += align256(nbins * kTS3 * 16);
+    w.ptiles = (double*)(p + off);
+    off += align256(nbins * kTS3 * 32);
     w.table = (SceneDesc*)(p + off);
     off += align256((int64_t)n_scenes * (int64_t)sizeof(SceneDesc));
     w.cols = (MpmCollider*)(p + off);
@@ -969,6 +1281,8 @@ int check_bins(const MpmBins* b, int64_t n_bins, int64_t n_slots, int max_cap, B
     if (b->wg_threads != 0 && b->wg_threads != 256 && b->wg_threads != 512)
         return set_err(-29, "bins: wg_threads must be 0, 256 or 512");
     if (n_slots + b->overflow_cap > 0x7fffffffLL) return set_err(-28, "bins: more than 2^31-1 slots");
+    if (n_bins >= (1LL << 27)) return set_err(-28, "bins: 2^27 bins or more");
+    if (max_cap > 16 * 1000 * 1000) return set_err(-22, "bins: more than 16e6 slots per bin");
     const int64_t need = (n_slots + b->overflow_cap) * MPM_NPLANES;
     if (b->stride < need) return set_err(-24, "bins: stride < (bin slots + overflow_cap) * MPM_NPLANES");
     g->n_bins = (int)n_bins;
@@ -977,6 +1291,8 @@ int check_bins(const MpmBins* b, int64_t n_bins, int64_t n_slots, int max_cap, B
     g->max_cap = max_cap;
     g->ovf_off = n_slots * MPM_NPLANES;
     g->stride = b->stride;
+    g->n_cu = device_cus();
+    g->pad = 0;
     return 0;
 }
 
@@ -1068,8 +1384,9 @@ int op_extract(const Ctx& c, const MpmBins* bins, const MpmState* flat, hipStrea
 
 int op_clear(const Ctx& c, MpmBins* bins, hipStream_t s) {
     hipLaunchKernelGGL(k_grid_bins, dim3(c.g.n_bins + 1), dim3(kWG), 0, s, c.w.acc, c.w.vel, c.w.gflag,
-                       (const int*)c.w.stamps, (const float4*)c.w.tiles, (int*)nullptr, (const int*)nullptr,
-                       (int*)nullptr, c.g, c.tab, c.cs, 0.f, 0.f, bins->epoch, 1, (const double*)nullptr);
+                       (const int*)c.w.stamps, (const float4*)c.w.tiles, (const int*)c.w.pcount,
+                       (const int*)c.w.pslot, (const double*)c.w.ptiles, (int*)nullptr, (const int*)nullptr, (int*)nullptr, c.g, c.tab,
+                       c.cs, 0.f, 0.f, bins->epoch, 1, (const double*)nullptr);
     int rc = check_launch("k_grid_bins(clear)");
     if (rc) return rc;
     bins->epoch += 1;  // the pending tiles are no longer current
@@ -1090,7 +1407,8 @@ int op_p2g(const Ctx& c, MpmBins* bins, float dt, hipStream_t s) {
 int op_grid(const Ctx& c, const MpmBins* bins, float time, float dt, hipStream_t s,
             const double* t_base = nullptr) {
     hipLaunchKernelGGL(k_grid_bins, dim3(c.g.n_bins + 1), dim3(kWG), 0, s, c.w.acc, c.w.vel, c.w.gflag,
-                       (const int*)c.w.stamps, (const float4*)c.w.tiles, counts(bins, c.g, 1 - bins->cur),
+                       (const int*)c.w.stamps, (const float4*)c.w.tiles, (const int*)c.w.pcount,
+                       (const int*)c.w.pslot, (const double*)c.w.ptiles, counts(bins, c.g, 1 - bins->cur),
                        (const int*)counts(bins, c.g, bins->cur), c.w.order, c.g, c.tab, c.cs, time, dt, bins->epoch,
                        0, t_base);
     return check_launch("k_grid_bins");
@@ -1115,7 +1433,10 @@ int op_g2p2g(const Ctx& c, MpmBins* bins, float dt, hipStream_t s, bool store_al
     a.status = c.w.status;
     a.tiles = c.w.tiles;
     a.stamps = c.w.stamps;
-    a.order = c.w.order;
+    a.ptiles = c.w.ptiles;
+    a.pcount = c.w.pcount;
+    a.pslot = c.w.pslot;
+    a.queue = c.w.order + c.g.n_bins;
     a.sched_epoch = bins->epoch;
     a.store_all = store_all ? 1 : 0;
     if (bins->wg_threads == 256)
