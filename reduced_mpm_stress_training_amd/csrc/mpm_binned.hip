@@ -626,8 +626,9 @@ __device__ __forceinline__ void g2p2g_particle(const SubstepArgs& a, const Scene
 
 // ---------------------------------------------------------------------------
 // Grid update of one 8^3 grid block: gathers the P2G tiles of the current
-// epoch from the bins overlapping it (a node x is covered by the tiles of
-// bins (x+1)>>3 and, when x+1-8((x+1)>>3) <= 2, the bin before) and the
+// epoch from the bins overlapping it (the tile of bin B spans nodes 8B-1 ..
+// 8B+10, so a node x is covered by the tiles of bins (x+1)>>3 and, when
+// x+1-8((x+1)>>3) <= 3, the bin before) and the
 // spill grid, sums in FP64, then normalization, gravity and colliders.
 // Also resets the next plane set's bin counters.  clear_only: only drains
 // the spill grid.  Workgroup 0 builds the schedule of k_g2p2g from the
@@ -706,7 +707,7 @@ __global__ void __launch_bounds__(kWG)
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             hi[d] = (p[d] + 1) >> 3;
-            nlo[d] = (p[d] + 1 - 8 * hi[d] <= 2 && hi[d] >= 1) ? 2 : 1;
+            nlo[d] = (p[d] + 1 - 8 * hi[d] <= 3 && hi[d] >= 1) ? 2 : 1;
         }
         const int bb[3] = {bx, by, bz};
         // the (up to 8) bins covering the node: their neighbour index and the
@@ -758,7 +759,7 @@ __global__ void __launch_bounds__(kWG)
                 for (int e = 0; e < 3; ++e) {
                     const int h = (q[e] + 1) >> 3;
                     b[e] = h - d[e];
-                    cv = cv && b[e] < nb && (d[e] == 0 || (q[e] + 1 - 8 * h <= 2 && h >= 1));
+                    cv = cv && b[e] < nb && (d[e] == 0 || (q[e] + 1 - 8 * h <= 3 && h >= 1));
                 }
                 if (!cv) continue;
                 const int slot =
