@@ -746,29 +746,37 @@ __global__ void __launch_bounds__(kWG)
         }
         // the FP64 tiles of split bins' parts
         if (s_anyp) {
-            // (one tile at a time: this path is taken only next to split
+            // (two tiles at a time: this path is taken only next to split
             // bins, and the registers of a wider batch would lower the
             // occupancy of every grid block)
 #pragma unroll 1
-            for (int k = 0; k < 8 * kParts; ++k) {
-                const int part = k >> 3, c = k & 7;
-                const int q[3] = {x, y, z}, d[3] = {c >> 2, (c >> 1) & 1, c & 1};
-                int b[3];
-                bool cv = true;
+            for (int k = 0; k < 8 * kParts; k += 2) {
+                int64_t off[2];
+                bool use[2];
 #pragma unroll
-                for (int e = 0; e < 3; ++e) {
-                    const int h = (q[e] + 1) >> 3;
-                    b[e] = h - d[e];
-                    cv = cv && b[e] < nb && (d[e] == 0 || (q[e] + 1 - 8 * h <= 3 && h >= 1));
+                for (int u = 0; u < 2; ++u) {
+                    const int part = (k + u) >> 3, c = (k + u) & 7;
+                    const int q[3] = {x, y, z}, d[3] = {c >> 2, (c >> 1) & 1, c & 1};
+                    int b[3];
+                    bool cv = true;
+#pragma unroll
+                    for (int e = 0; e < 3; ++e) {
+                        const int h = (q[e] + 1) >> 3;
+                        b[e] = h - d[e];
+                        cv = cv && b[e] < nb && (d[e] == 0 || (q[e] + 1 - 8 * h <= 3 && h >= 1));
+                    }
+                    const int slot =
+                        cv ? s_pslot[part * 27 + ((b[0] - bx + 1) * 3 + (b[1] - by + 1)) * 3 + (b[2] - bz + 1)] : -1;
+                    use[u] = slot >= 0;
+                    const int lt =
+                        ((q[0] - 8 * b[0] + 1) * kTS + (q[1] - 8 * b[1] + 1)) * kTS + (q[2] - 8 * b[2] + 1);
+                    off[u] = use[u] ? ((int64_t)slot * kTS3 + lt) * 4 : 0;
                 }
-                if (!cv) continue;
-                const int slot =
-                    s_pslot[part * 27 + ((b[0] - bx + 1) * 3 + (b[1] - by + 1)) * 3 + (b[2] - bz + 1)];
-                if (slot < 0) continue;
-                const int lt = ((q[0] - 8 * b[0] + 1) * kTS + (q[1] - 8 * b[1] + 1)) * kTS + (q[2] - 8 * b[2] + 1);
-                const double* src = ptiles + ((int64_t)slot * kTS3 + lt) * 4;
-                const double2 t0 = *(const double2*)src, t1 = *(const double2*)(src + 2);
-                sx += t0.x; sy += t0.y; sz += t1.x; sm += t1.y;
+                if (!use[0] && !use[1]) continue;
+                const double2 a0 = *(const double2*)(ptiles + off[0]), a1 = *(const double2*)(ptiles + off[0] + 2);
+                const double2 b0 = *(const double2*)(ptiles + off[1]), b1 = *(const double2*)(ptiles + off[1] + 2);
+                if (use[0]) { sx += a0.x; sy += a0.y; sz += a1.x; sm += a1.y; }
+                if (use[1]) { sx += b0.x; sy += b0.y; sz += b1.x; sm += b1.y; }
             }
         }
         const float4 a4 = make_float4((float)sx, (float)sy, (float)sz, (float)sm);
