@@ -159,12 +159,15 @@ def test_free_fall_gpu():
     assert np.abs(out["tau"]).max() < 1e-2
 
 
-def test_headline_size_parity(oracle):
-    """1M particles on 128^3 (the headline workload) for 3 substeps against
-    the FP32 and FP64 oracles (OpenMP), plus mass conservation of the P2G
-    grid.  Positions are compared as x = x0 + u (u itself is ~3e-5 m after 3
-    substeps, where one ulp of x is 2e-3 of u)."""
-    sc = S.column_uniform()
+@pytest.mark.parametrize("n_particles, n_grid", [(1_000_000, 128), (250_000, 64)])
+def test_headline_size_parity(oracle, n_particles, n_grid):
+    """1M particles on 128^3 (the headline workload), and 250k on 64^3, where
+    the fused kernel splits every full bin into three parts with FP64 tiles
+    (80 bins for 256 CUs), for 3 substeps against the FP32 and FP64 oracles
+    (OpenMP), plus mass conservation of the P2G grid.  Positions are compared
+    as x = x0 + u (u itself is ~3e-5 m after 3 substeps, where one ulp of x is
+    2e-3 of u)."""
+    sc = S.column_uniform(n_particles=n_particles, n_grid=n_grid)
     sim = S.make_simulator(sc)
     n_steps = 3
     sim.run(n_steps, sc.dt)
