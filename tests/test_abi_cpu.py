@@ -112,3 +112,23 @@ def test_segment_entry_points_reject_bad_arguments():
     adj = lib.mpm_adjoint_recorded
     assert adj(prm, st, order, grids, cols, 0, 0x3000000, 0, 1e-4, 2, 0x4000000, 0x4000000, 128, 0x5000000, None) < 0
     assert b"adjoint buffers" in lib.mpm_last_error()                     # adj_a == adj_b
+
+
+def test_bins_reject_oversized_bins_and_size_the_part_tiles():
+    """The binned entry points check MpmBins on the host before any launch
+    (the pointers here are never dereferenced): more than 16e6 slots per bin
+    is refused (the fused kernel's split arithmetic counts passes of 512 in
+    15 bits).  The workspace holds, per bin, a float4 tile and an FP64 tile
+    of 12^3 nodes (the latter for the parts of split bins)."""
+    lib = L.load()
+    prm = L.MpmParams()
+    prm.n_grid, prm.material, prm.dx, prm.inv_dx = 32, 2, 1.0 / 32, 32.0
+    prm.mu, prm.lam, prm.alpha = 7.7e4, 1.15e5, 0.2
+    nb = 4
+    bins = L.MpmBins()
+    bins.nb, bins.bin_cap, bins.overflow_cap = nb, 16_100_000, 16_100_000
+    bins.stride = (nb ** 3 * bins.bin_cap + bins.overflow_cap) * L.NPLANES
+    bins.planes[0], bins.planes[1], bins.count = 0x1000000, 0x2000000, 0x3000000
+    assert lib.mpm_bins_g2p2g(prm, bins, 0x4000000, 1e-4, None) < 0
+    assert b"16e6 slots per bin" in lib.mpm_last_error()
+    assert lib.mpm_bins_workspace_bytes(prm) >= nb ** 3 * 12 ** 3 * (16 + 32) + 2 * 32 ** 3 * 16
