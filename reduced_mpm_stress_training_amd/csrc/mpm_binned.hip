@@ -19,8 +19,7 @@
 //                 FP32 LDS atomics run ~16x slower on gfx950), written out as
 //                 one plain, coalesced tile per item; every particle is
 //                 written to the slot of its new bin in the other plane set.
-//                 One persistent workgroup per CU slot takes the items from a
-//                 queue, largest first.
+//                 Workgroup w takes item w of a list sorted largest first.
 // Bins' tiles carry an epoch stamp; the grid update uses the tiles of the
 // current epoch, so nothing needs clearing between substeps (zero_grid,
 // mpm_utils.py:115-123, becomes the epoch bump).  Particles whose next
@@ -166,13 +165,13 @@ __device__ __forceinline__ int xcd_bin(int wg, int n_bins) {
 // grid update sums the parts.
 //
 // order[0 .. n_bins): bins for the sort, -1 past the last non-empty one;
-// order[n_bins + kQEpoch]: the epoch of the bin counts it was made from;
-// order[n_bins + kQFirst + w], w < n_bins: work item w of k_g2p2g,
+// order[n_bins + kItemsEpoch]: the epoch of the bin counts it was made from;
+// order[n_bins + kItemsFirst + w], w < n_bins: work item w of k_g2p2g,
 // (bin << 4) | (part << 2) | (parts - 1), -1 past the last.  Bins are split
 // only while the items fit in n_bins slots, so the fused kernel's grid stays
 // one workgroup per bin (plus the overflow run's).
 constexpr int kSchedKeys = 64;
-constexpr int kQEpoch = 0, kQFirst = 1;
+constexpr int kItemsEpoch = 0, kItemsFirst = 1;
 
 __device__ __forceinline__ int bin_passes(int n) { return (n + kWG - 1) / kWG; }
 
@@ -255,7 +254,7 @@ __device__ void schedule_bins(const int* __restrict__ cnt_cur, const BinGeom& g,
     const int n_bins = g.n_bins, cap = g.max_cap;
     const int n_chunks = (n_bins + kWG * kSchedRegs - 1) / (kWG * kSchedRegs);
     const bool reload = n_chunks > 1;
-    int* queue = order + n_bins;
+    int* items = order + n_bins;
     int pv[kSchedRegs];
     if (tid < kSchedKeys) s_hist[tid] = 0;
     if (tid == 0) s_passes = s_items = 0;
@@ -289,16 +288,16 @@ __device__ void schedule_bins(const int* __restrict__ cnt_cur, const BinGeom& g,
             const int b = c * kWG * kSchedRegs + j * kWG + tid;
             if (pv[j] > 0) {
                 order[pos] = b;
-                if (!split) queue[kQFirst + pos] = b << 4;
+                if (!split) items[kItemsFirst + pos] = b << 4;
             }
         }
     }
     for (int p = s_total + tid; p < n_bins; p += kWG) {
         order[p] = -1;
-        if (!split) queue[kQFirst + p] = -1;
+        if (!split) items[kItemsFirst + p] = -1;
     }
     if (split) {
-        int items = 0;
+        int my_items = 0;
         for (int c = 0; c < n_chunks; ++c) {
             if (reload) load_passes(cnt_cur, n_bins, cap, c, pv);
 #pragma unroll
@@ -306,10 +305,10 @@ __device__ void schedule_bins(const int* __restrict__ cnt_cur, const BinGeom& g,
                 if (pv[j] > 0) {
                     int parts, per;
                     split_bin(pv[j], q, &parts, &per);
-                    items += parts;
+                    my_items += parts;
                 }
         }
-        if (items > 0) atomicAdd(&s_items, items);
+        if (my_items > 0) atomicAdd(&s_items, my_items);
         __syncthreads();   // s_total of the sort's list has been read
         split = s_items <= n_bins;
         if (tid < kSchedKeys) s_hist[tid] = 0;
@@ -345,13 +344,13 @@ __device__ void schedule_bins(const int* __restrict__ cnt_cur, const BinGeom& g,
                 for (int i = 0; i < kParts; ++i) {
                     const int pos =
                         wave_add(s_cursor, min(max(min(per, p - i * per), 0), kSchedKeys - 1), i < parts);
-                    if (i < parts) queue[kQFirst + pos] = (b << 4) | (i << 2) | (parts - 1);
+                    if (i < parts) items[kItemsFirst + pos] = (b << 4) | (i << 2) | (parts - 1);
                 }
             }
         }
-        for (int p = s_total + tid; p < n_bins; p += kWG) queue[kQFirst + p] = -1;
+        for (int p = s_total + tid; p < n_bins; p += kWG) items[kItemsFirst + p] = -1;
     }
-    if (tid == 0) queue[kQEpoch] = epoch;
+    if (tid == 0) items[kItemsEpoch] = epoch;
 }
 
 __device__ __forceinline__ void mark_block(int* __restrict__ gflag, int nb, int x, int y, int z) {
@@ -418,7 +417,7 @@ struct SubstepArgs {
     double* __restrict__ ptiles;    // FP64 tiles of split bins' parts, one slot per work item
     int* __restrict__ pcount;       // parts of each split bin
     int* __restrict__ pslot;        // ptiles slot of part p of bin b: pslot[p * n_bins + b]
-    int* __restrict__ queue;        // work items (schedule_bins): order + n_bins
+    int* __restrict__ items;        // work items (schedule_bins): order + n_bins
     int sched_epoch;                // epoch the schedule must carry to be used
     int store_all;                  // 0: write only the planes the next substep reads (see g2p2g_particle)
 };
@@ -925,7 +924,7 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2, 2)))
         g2p2g_overflow<WG>(a, wg - g.n_bins);
         return;
     }
-    const int it = a.queue[kQEpoch] == a.sched_epoch ? a.queue[kQFirst + wg] : (wg << 4);
+    const int it = a.items[kItemsEpoch] == a.sched_epoch ? a.items[kItemsFirst + wg] : (wg << 4);
     if (it < 0) return;
     g2p2g_item<WG>(a, it >> 4, (it >> 2) & 3, (it & 3) + 1, s_vel, s_acc, &s_stay, &s_base, &s_live);
 }
@@ -1255,7 +1254,7 @@ WsLayout ws_layout(int64_t nodes, int64_t nbins, int n_scenes, int n_cols, void*
     w.pslot = (int*)(p + off);
     off += align256(kParts * nbins * 4);
     w.order = (int*)(p + off);
-    off += align256((2 * nbins + kQFirst) * 4);
+    off += align256((2 * nbins + kItemsFirst) * 4);
     w.tiles = (float4*)(p + off);
     off += align256(nbins * kTS3 * 16);
     w.ptiles = (double*)(p + off);
@@ -1445,7 +1444,7 @@ int op_g2p2g(const Ctx& c, MpmBins* bins, float dt, hipStream_t s, bool store_al
     a.ptiles = c.w.ptiles;
     a.pcount = c.w.pcount;
     a.pslot = c.w.pslot;
-    a.queue = c.w.order + c.g.n_bins;
+    a.items = c.w.order + c.g.n_bins;
     a.sched_epoch = bins->epoch;
     a.store_all = store_all ? 1 : 0;
     if (bins->wg_threads == 256)
