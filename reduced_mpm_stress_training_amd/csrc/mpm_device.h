@@ -57,7 +57,7 @@ __device__ __forceinline__ void jacobi_rotate(float& app, float& aqq, float& apq
     // t only needs to define a rotation (c, s below are normalised from it),
     // so the hardware sqrt and reciprocal are enough here
     float d = aqq - app;
-    float den = fabsf(d) + __builtin_sqrtf(d * d + 4.0f * apq * apq);
+    float den = fabsf(d) + __builtin_amdgcn_sqrtf(d * d + 4.0f * apq * apq);
     float t = (den > 0.0f) ? ((d >= 0.0f ? 2.0f : -2.0f) * apq) * __builtin_amdgcn_rcpf(den) : 0.0f;
     float c = rsqrt_nr(1.0f + t * t);
     float s = t * c;
@@ -179,7 +179,14 @@ __device__ __forceinline__ int sand_return_map(const float Ftr[9], const float U
 #pragma unroll
         for (int k = 0; k < 9; ++k) Fe[k] = Ftr[k];
 #pragma unroll
-        for (int d = 0; d < 3; ++d) ls[d] = (s[d] >= 1e-14f) ? eps[d] : logf(s[d]);
+        for (int d = 0; d < 3; ++d) ls[d] = eps[d];
+        // ln of a singular value below 1e-14 (NaN when negative), as the
+        // stress takes it; a branch so that the common case skips the logf
+        if (fminf(fminf(s[0], s[1]), s[2]) < 1e-14f) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+                if (s[d] < 1e-14f) ls[d] = logf(s[d]);
+        }
         return 1;
     }
     float se[3];
