@@ -20,10 +20,12 @@
  *
  * Particle state layout ("planes"): one float32 buffer of MPM_NPLANES planes,
  * plane k starting at planes + k*stride (stride >= n, in elements), indexed by
- * solver slot.  Plane order: x0(3) u(3) v(3) C(9) F(9) tau(9) mass vol pid,
- * with 3x3 matrices row-major (component a*3+b) like Warp mat33 and torch
- * (N,3,3).  The pid plane holds the caller's particle index of each slot as
- * int32 bits.
+ * solver slot.  Plane order: x0(3) u(3) v(3) C(9) F(9) tau(9) mass vol
+ * yield_stress pid, with 3x3 matrices row-major (component a*3+b) like Warp
+ * mat33 and torch (N,3,3).  The pid plane holds the caller's particle index of
+ * each slot as int32 bits.  yield_stress is the reference's per-particle
+ * model.yield_stress array (warp_utils.py:75, mpm_solver_warp.py:43); only
+ * metal (material 1) reads or writes it.
  *
  * Grid layout: n_grid^3 nodes in C order [ix][iy][iz] (as the reference's
  * wp.array(ndim=3)), node i at position i*dx, one float4 per node:
@@ -40,7 +42,7 @@
 extern "C" {
 #endif
 
-#define MPM_ABI_VERSION 9
+#define MPM_ABI_VERSION 10
 
 /* plane indices */
 #define MPM_P_X0 0
@@ -51,19 +53,30 @@ extern "C" {
 #define MPM_P_TAU 27
 #define MPM_P_MASS 36
 #define MPM_P_VOL 37
-#define MPM_P_PID 38
-#define MPM_NPLANES 39
+#define MPM_P_YIELD 38
+#define MPM_P_PID 39
+#define MPM_NPLANES 40
 
 #define MPM_MAX_COLLIDERS 16
 
 /* Model scalars: MPMModelStruct (engine_stress/warp_utils.py:61-81).
- * gravity generalises the reference's (0, -g, 0) (mpm_utils.py:236). */
+ * gravity generalises the reference's (0, -g, 0) (mpm_utils.py:236).
+ * material (the G2P switch of mpm_utils.py:284-304):
+ *   0 jelly: F = F_trial, FCR stress (mpm_utils.py:29-34, 298-299);
+ *   1 metal: von Mises return map on the yield_stress plane, hardening
+ *            when hardening == 1 (mpm_utils.py:59-84), StVK stress
+ *            (mpm_utils.py:36-43, 301-302);
+ *   2 sand:  Drucker-Prager return map (mpm_utils.py:87-112), stress
+ *            (mpm_utils.py:45-52).
+ * The adjoint and ordered-state entry points implement sand only. */
 typedef struct MpmParams {
     int32_t n_grid;
-    int32_t material; /* 2 = sand (Drucker-Prager); the only material on this path */
+    int32_t material;
     float dx, inv_dx;
     float mu, lam, alpha;
     float gravity[3];
+    int32_t hardening; /* metal: 1 = yield_stress += 2 mu xi dgamma (mpm_utils.py:80-81) */
+    float xi;
 } MpmParams;
 
 /* Dirichlet_collider (engine_stress/warp_utils.py:126-151).
@@ -354,7 +367,7 @@ int mpm_permute(const float* src, int64_t src_stride, float* dst, int64_t dst_st
  * scenes' particles one scene after the other, PID plane = flat index.  All
  * scenes advance with the same dt and time. */
 typedef struct MpmScene {
-    MpmParams params;     /* the scene's grid and material (material 2) */
+    MpmParams params;     /* the scene's grid and material */
     int64_t n_particles;  /* its particles, consecutive in the batch's flat state */
     int32_t n_colliders;  /* its colliders: the next n_colliders entries of the batch's collider array */
     int32_t bin_cap;      /* slots per bin of this scene */

@@ -54,10 +54,12 @@ typedef float real;
  * are rounded to `real` on entry, as Warp rounds the Python floats. */
 typedef struct {
     int32_t n_grid;
-    int32_t material;     /* 2 = sand; the only material on this path */
+    int32_t material;     /* 0 jelly, 1 metal, 2 sand (mpm_solver_warp.py:90-94) */
     double dx, inv_dx;
     double mu, lam, alpha;
     double gravity[3];    /* acceleration vector; reference: (0, -g, 0) */
+    int32_t hardening;    /* metal: MPMModelStruct.hardening (warp_utils.py:80) */
+    double xi;            /* metal: MPMModelStruct.xi (warp_utils.py:81) */
 } OracleParams;
 
 /* Grid collider (warp_utils.py:126-151, Dirichlet_collider). */
@@ -264,6 +266,69 @@ void SFX(oracle_dp_stress)(const real* F, real mu, real lam, real* tau) {
 }
 
 /* ------------------------------------------------------------------ */
+/* Metal: von_mises_return_mapping, mpm_utils.py:59-84.  Updates *ys      */
+/* (model.yield_stress[p]) when hardening == 1 and the particle yields.  */
+/* ------------------------------------------------------------------ */
+void SFX(oracle_von_mises)(const real* Ftr, real mu, real lam, real* ys,
+                           int hardening, real xi, real* Fe) {
+    real U[9], V[9], so[3];
+    SFX(oracle_svd3)(Ftr, U, so, V);
+    real eps[3];
+    for (int d = 0; d < 3; ++d) eps[d] = R_LOG(so[d] > (real)0.01 ? so[d] : (real)0.01);
+    real temp = (eps[0] + eps[1] + eps[2]) / (real)3;
+    real t[3];
+    for (int d = 0; d < 3; ++d) t[d] = (real)2 * mu * eps[d] + lam * (eps[0] + eps[1] + eps[2]);
+    real sum_tau = t[0] + t[1] + t[2];
+    real cond[3] = {t[0] - sum_tau / (real)3, t[1] - sum_tau / (real)3, t[2] - sum_tau / (real)3};
+    real cn = R_SQRT(cond[0] * cond[0] + cond[1] * cond[1] + cond[2] * cond[2]);
+    if (cn > *ys) {
+        real eh[3] = {eps[0] - temp, eps[1] - temp, eps[2] - temp};
+        real ehn = R_SQRT(eh[0] * eh[0] + eh[1] * eh[1] + eh[2] * eh[2]) + (real)1e-6;
+        real dg = ehn - *ys / ((real)2 * mu);
+        real sn[3];
+        for (int d = 0; d < 3; ++d) sn[d] = R_EXP(eps[d] - (dg / ehn) * eh[d]);
+        real US[9];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) US[i * 3 + j] = U[i * 3 + j] * sn[j];
+        mmT3(US, V, Fe);
+        if (hardening == 1) *ys = *ys + (real)2 * mu * xi * dg;
+    } else {
+        memcpy(Fe, Ftr, 9 * sizeof(real));
+    }
+}
+
+/* Metal stress: kirchoff_stress_StVK, mpm_utils.py:36-43, on a fresh SVD */
+/* of F (mpm_utils.py:297, 301-302): tau = U diag(2 mu ln s + lam sum) V^T */
+void SFX(oracle_stvk_stress)(const real* F, real mu, real lam, real* tau) {
+    real U[9], V[9], s[3];
+    SFX(oracle_svd3)(F, U, s, V);
+    for (int d = 0; d < 3; ++d) s[d] = s[d] > (real)0.01 ? s[d] : (real)0.01;
+    real ls = R_LOG(s[0]) + R_LOG(s[1]) + R_LOG(s[2]);
+    real c[3];
+    for (int d = 0; d < 3; ++d) c[d] = (real)2 * mu * R_LOG(s[d]) + lam * ls;
+    real UC[9];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) UC[i * 3 + j] = U[i * 3 + j] * c[j];
+    mmT3(UC, V, tau);
+}
+
+/* Jelly stress: kirchoff_stress_FCR, mpm_utils.py:29-34, with R = U V^T  */
+/* of a fresh SVD of F and J = det F (mpm_utils.py:292, 297-299).         */
+void SFX(oracle_fcr_stress)(const real* F, real mu, real lam, real* tau) {
+    real U[9], V[9], s[3];
+    SFX(oracle_svd3)(F, U, s, V);
+    real R[9];
+    mmT3(U, V, R);
+    real D[9];
+    for (int k = 0; k < 9; ++k) D[k] = F[k] - R[k];
+    real J = F[0] * (F[4] * F[8] - F[5] * F[7]) - F[1] * (F[3] * F[8] - F[5] * F[6]) +
+             F[2] * (F[3] * F[7] - F[4] * F[6]);
+    mmT3(D, F, tau);
+    for (int k = 0; k < 9; ++k) tau[k] = (real)2 * mu * tau[k];
+    for (int d = 0; d < 3; ++d) tau[d * 4] = tau[d * 4] + lam * J * (J - (real)1);
+}
+
+/* ------------------------------------------------------------------ */
 /* R6: p2g_apic_stress_given, mpm_utils.py:190-227.  Sequential in       */
 /* particle order (the reference's float atomics have no fixed order).   */
 /* grid_mv is v_in (n^3 x 3), grid_m is m (n^3).                          */
@@ -356,11 +421,13 @@ void SFX(oracle_grid_update)(const OracleParams* P, const real* grid_m,
 }
 
 /* ------------------------------------------------------------------ */
-/* R13: g2p_apic_and_stress, mpm_utils.py:241-306 (sand, material 2),    */
-/* with R10 and R12.  x is updated in place; F holds F_e on exit.        */
+/* R13: g2p_apic_and_stress, mpm_utils.py:241-306, with the material     */
+/* switch of :284-304 (sand R10/R12, metal von Mises + StVK, jelly FCR). */
+/* x is updated in place; F holds F_e on exit; ys (metal) is the         */
+/* particle's yield stress.                                              */
 /* ------------------------------------------------------------------ */
 static void g2p_one(const OracleParams* P, const real* grid_v, real* x, real* v,
-                    real* C, real* F, real* tau, real dt) {
+                    real* C, real* F, real* tau, real* ys, real dt) {
     const int ng = P->n_grid;
     const real inv_dx = (real)P->inv_dx, mu = (real)P->mu, lam = (real)P->lam,
                alpha = (real)P->alpha;
@@ -395,29 +462,37 @@ static void g2p_one(const OracleParams* P, const real* grid_v, real* x, real* v,
     A[0] = A[0] + (real)1; A[4] = A[4] + (real)1; A[8] = A[8] + (real)1;
     real Ftr[9];
     mm3(A, F, Ftr);
-    if (P->material == 2) {
+    if (P->material == 1) {
+        SFX(oracle_von_mises)(Ftr, mu, lam, ys, P->hardening, (real)P->xi, F);
+        SFX(oracle_stvk_stress)(F, mu, lam, tau);
+    } else if (P->material == 2) {
         SFX(oracle_sand_return_map)(Ftr, mu, lam, alpha, F);
+        SFX(oracle_dp_stress)(F, mu, lam, tau);
     } else {
         memcpy(F, Ftr, sizeof(Ftr));
+        SFX(oracle_fcr_stress)(F, mu, lam, tau);
     }
-    SFX(oracle_dp_stress)(F, mu, lam, tau);
 }
 
 void SFX(oracle_g2p)(const OracleParams* P, int64_t n, const real* grid_v,
-                     real* x, real* v, real* C, real* F, real* tau,
+                     real* x, real* v, real* C, real* F, real* tau, real* yield,
                      double dt_in, int nthreads) {
     const real dt = (real)dt_in;
 #ifdef _OPENMP
     if (nthreads < 1) nthreads = 1;
 #pragma omp parallel for num_threads(nthreads) schedule(static)
 #endif
-    for (int64_t p = 0; p < n; ++p)
-        g2p_one(P, grid_v, x + 3 * p, v + 3 * p, C + 9 * p, F + 9 * p, tau + 9 * p, dt);
+    for (int64_t p = 0; p < n; ++p) {
+        real ys0 = 0;
+        real* ys = yield ? yield + p : &ys0;
+        g2p_one(P, grid_v, x + 3 * p, v + 3 * p, C + 9 * p, F + 9 * p, tau + 9 * p, ys, dt);
+    }
     (void)nthreads;
 }
 
 /* ------------------------------------------------------------------ */
 /* R15: one substep p2g2p, mpm_solver_warp.py:156-203, in its order:     */
+/* (yield: the per-particle yield stress array, metal only; may be NULL) */
 /*   update_x_from_u (mpm_utils.py:4-7), zero_grid (:115-123),           */
 /*   P2G, grid update + colliders, G2P, update_u_from_x (:12-15).        */
 /* F_disp = F - I (:16-19) is a derived output and is not kept here.     */
@@ -427,7 +502,7 @@ void SFX(oracle_g2p)(const OracleParams* P, int64_t n, const real* grid_v,
 /* ------------------------------------------------------------------ */
 int SFX(oracle_step)(const OracleParams* P, int64_t n, const real* x0, real* u,
                      real* v, real* C, real* F, real* tau, const real* mass,
-                     const real* vol, real* x, real* grid_m, real* grid_mv,
+                     const real* vol, real* yield, real* x, real* grid_m, real* grid_mv,
                      real* grid_v, const OracleCollider* cols, int ncol,
                      double time, double dt, int nthreads) {
     const int ng = P->n_grid;
@@ -476,7 +551,7 @@ int SFX(oracle_step)(const OracleParams* P, int64_t n, const real* x0, real* u,
 #endif
     }
     SFX(oracle_grid_update)(P, grid_m, grid_mv, grid_v, cols, ncol, time, dt);
-    SFX(oracle_g2p)(P, n, grid_v, x, v, C, F, tau, dt, nthreads);
+    SFX(oracle_g2p)(P, n, grid_v, x, v, C, F, tau, yield, dt, nthreads);
     for (int64_t p = 0; p < 3 * n; ++p) u[p] = x[p] - x0[p];
     return 0;
 }

@@ -22,6 +22,10 @@ Reference citations (zeshunzong/reduced_mpm_stress_training):
   g2p_apic_and_stress        engine_stress/mpm_utils.py:241-306
   sand_return_mapping        engine_stress/mpm_utils.py:87-112
   kirchoff_stress_drucker_prager engine_stress/mpm_utils.py:45-52
+  von_mises_return_mapping   engine_stress/mpm_utils.py:59-84
+  kirchoff_stress_StVK       engine_stress/mpm_utils.py:36-43
+  kirchoff_stress_FCR        engine_stress/mpm_utils.py:29-34
+  G2P material switch        engine_stress/mpm_utils.py:284-304
   update_u_from_x            engine_stress/mpm_utils.py:12-15
   p2g2p (order)              engine_stress/mpm_solver_warp.py:156-203
 """
@@ -99,6 +103,46 @@ def dp_stress(F, mu, lam):
     return (U * c[:, None, :]) @ np.swapaxes(V, -1, -2) @ np.swapaxes(F, -1, -2)
 
 
+def von_mises(Ftr, mu, lam, ys, hardening=0, xi=0.0):
+    """von_mises_return_mapping (mpm_utils.py:59-84) on a stack of F_tr with
+    per-matrix yield stresses ys; returns (F_e, ys after hardening)."""
+    U, s, V = svd3_rot(Ftr)
+    eps = np.log(np.maximum(s, 0.01))
+    temp = eps.sum(-1) / 3.0
+    t = 2.0 * mu * eps + lam * eps.sum(-1)[:, None]
+    cond = t - t.sum(-1)[:, None] / 3.0
+    yielded = np.linalg.norm(cond, axis=-1) > ys
+    Fe = Ftr.copy()
+    ys = np.array(ys, dtype=np.float64, copy=True)
+    if yielded.any():
+        eh = eps[yielded] - temp[yielded, None]
+        ehn = np.linalg.norm(eh, axis=-1) + 1e-6
+        dg = ehn - ys[yielded] / (2.0 * mu)
+        e_new = eps[yielded] - (dg / ehn)[:, None] * eh
+        Fe[yielded] = (U[yielded] * np.exp(e_new)[:, None, :]) @ np.swapaxes(V[yielded], -1, -2)
+        if hardening == 1:
+            ys[yielded] = ys[yielded] + 2.0 * mu * xi * dg
+    return Fe, ys
+
+
+def stvk_stress(F, mu, lam):
+    """kirchoff_stress_StVK (mpm_utils.py:36-43) on a fresh SVD of F:
+    U diag(2 mu ln s + lam sum ln s) V^T, s clamped at 0.01."""
+    U, s, V = svd3_rot(F)
+    ls = np.log(np.maximum(s, 0.01))
+    c = 2.0 * mu * ls + lam * ls.sum(-1)[:, None]
+    return (U * c[:, None, :]) @ np.swapaxes(V, -1, -2)
+
+
+def fcr_stress(F, mu, lam):
+    """kirchoff_stress_FCR (mpm_utils.py:29-34): 2 mu (F - U V^T) F^T +
+    lam J (J - 1) I with J = det F (mpm_utils.py:292)."""
+    U, s, V = svd3_rot(F)
+    R = U @ np.swapaxes(V, -1, -2)
+    J = np.linalg.det(F)
+    return 2.0 * mu * (F - R) @ np.swapaxes(F, -1, -2) + (lam * J * (J - 1.0))[:, None, None] * np.eye(3)
+
+
 def p2g(params, x, v, C, tau, mass, vol, dt):
     """p2g_apic_stress_given (mpm_utils.py:190-227) -> grid (m, mv)."""
     ng = params["n_grid"]
@@ -145,8 +189,10 @@ def grid_update(params, grid_m, grid_mv, colliders, time, dt):
     return gv.reshape(-1, 3)
 
 
-def g2p(params, grid_v, x, F, dt):
-    """g2p_apic_and_stress (mpm_utils.py:241-306), sand branch."""
+def g2p(params, grid_v, x, F, dt, ys=None):
+    """g2p_apic_and_stress (mpm_utils.py:241-306) with the material switch
+    of :284-304 (params["material"]: 0 jelly, 1 metal, 2 sand).  Returns
+    x, v, C, F, tau and the yield stresses (metal)."""
     ng = params["n_grid"]
     inv_dx = params["inv_dx"]
     base, fx, w, dw = weights(x, inv_dx)
@@ -159,9 +205,20 @@ def g2p(params, grid_v, x, F, dt):
     gradv = np.einsum("pna,pnb->pab", g, grad)
     x_new = x + dt * v_new
     Ftr = (np.eye(3)[None] + gradv * dt) @ F
-    Fe = sand_return_map(Ftr, params["mu"], params["lam"], params["alpha"])
-    tau = dp_stress(Fe, params["mu"], params["lam"])
-    return x_new, v_new, C_new, Fe, tau
+    mat = params.get("material", 2)
+    mu, lam = params["mu"], params["lam"]
+    if ys is None:
+        ys = np.zeros(x.shape[0])
+    if mat == 2:
+        Fe = sand_return_map(Ftr, mu, lam, params["alpha"])
+        tau = dp_stress(Fe, mu, lam)
+    elif mat == 1:
+        Fe, ys = von_mises(Ftr, mu, lam, ys, params.get("hardening", 0), params.get("xi", 0.0))
+        tau = stvk_stress(Fe, mu, lam)
+    else:
+        Fe = Ftr
+        tau = fcr_stress(Fe, mu, lam)
+    return x_new, v_new, C_new, Fe, tau, ys
 
 
 def step(params, state, colliders, time, dt):
@@ -172,7 +229,7 @@ def step(params, state, colliders, time, dt):
     x = state["x0"] + state["u"]
     gm, gmv = p2g(params, x, state["v"], state["C"], state["tau"], state["mass"], state["vol"], dt)
     gv = grid_update(params, gm, gmv, colliders, time, dt)
-    x_new, v, C, F, tau = g2p(params, gv, x, state["F"], dt)
+    x_new, v, C, F, tau, ys = g2p(params, gv, x, state["F"], dt, state.get("yield_stress"))
     out = dict(state)
-    out.update(u=x_new - state["x0"], v=v, C=C, F=F, tau=tau)
+    out.update(u=x_new - state["x0"], v=v, C=C, F=F, tau=tau, yield_stress=ys)
     return out

@@ -24,7 +24,7 @@ class OracleParams(C.Structure):
     _fields_ = [("n_grid", C.c_int32), ("material", C.c_int32),
                 ("dx", C.c_double), ("inv_dx", C.c_double),
                 ("mu", C.c_double), ("lam", C.c_double), ("alpha", C.c_double),
-                ("gravity", C.c_double * 3)]
+                ("gravity", C.c_double * 3), ("hardening", C.c_int32), ("xi", C.c_double)]
 
 
 class OracleCollider(C.Structure):
@@ -61,10 +61,12 @@ def _sfx(dtype) -> str:
     return "_f64" if np.dtype(dtype) == np.float64 else "_f32"
 
 
-def make_params(n_grid, dx, inv_dx, mu, lam, alpha, gravity, material=2) -> OracleParams:
+def make_params(n_grid, dx, inv_dx, mu, lam, alpha, gravity, material=2, hardening=0, xi=0.0) -> OracleParams:
     p = OracleParams()
     p.n_grid = int(n_grid)
     p.material = int(material)
+    p.hardening = int(hardening)
+    p.xi = float(xi)
     p.dx, p.inv_dx, p.mu, p.lam, p.alpha = float(dx), float(inv_dx), float(mu), float(lam), float(alpha)
     for i in range(3):
         p.gravity[i] = float(gravity[i])
@@ -120,19 +122,48 @@ def dp_stress(F: np.ndarray, mu, lam):
     return tau
 
 
+def von_mises(Ftr: np.ndarray, mu, lam, yield_stress, hardening=0, xi=0.0):
+    """oracle_von_mises on one 3x3 matrix; returns (F_e, updated yield stress)."""
+    Ftr = np.ascontiguousarray(Ftr)
+    Fe = np.zeros((3, 3), Ftr.dtype)
+    ys = np.array([yield_stress], Ftr.dtype)
+    ct = C.c_double if Ftr.dtype == np.float64 else C.c_float
+    getattr(lib(), "oracle_von_mises" + _sfx(Ftr.dtype))(_p(Ftr), ct(mu), ct(lam), _p(ys), C.c_int(hardening),
+                                                           ct(xi), _p(Fe))
+    return Fe, float(ys[0])
+
+
+def stvk_stress(F: np.ndarray, mu, lam):
+    F = np.ascontiguousarray(F)
+    tau = np.zeros((3, 3), F.dtype)
+    ct = C.c_double if F.dtype == np.float64 else C.c_float
+    getattr(lib(), "oracle_stvk_stress" + _sfx(F.dtype))(_p(F), ct(mu), ct(lam), _p(tau))
+    return tau
+
+
+def fcr_stress(F: np.ndarray, mu, lam):
+    F = np.ascontiguousarray(F)
+    tau = np.zeros((3, 3), F.dtype)
+    ct = C.c_double if F.dtype == np.float64 else C.c_float
+    getattr(lib(), "oracle_fcr_stress" + _sfx(F.dtype))(_p(F), ct(mu), ct(lam), _p(tau))
+    return tau
+
+
 # ---- scene state ----------------------------------------------------------
 
 class OracleScene:
     """Particle state + grid scratch of one scene in float32 or float64."""
 
-    FIELDS = ("x0", "u", "v", "C", "F", "tau", "mass", "vol")
+    FIELDS = ("x0", "u", "v", "C", "F", "tau", "mass", "vol", "yield_stress")
 
     def __init__(self, params: dict, state: dict, colliders=(), dtype=np.float64):
         self.dtype = np.dtype(dtype)
         self.params = dict(params)
         self.n = int(state["x0"].shape[0])
         shapes = dict(x0=(self.n, 3), u=(self.n, 3), v=(self.n, 3), C=(self.n, 3, 3),
-                      F=(self.n, 3, 3), tau=(self.n, 3, 3), mass=(self.n,), vol=(self.n,))
+                      F=(self.n, 3, 3), tau=(self.n, 3, 3), mass=(self.n,), vol=(self.n,), yield_stress=(self.n,))
+        state = dict(state)
+        state.setdefault("yield_stress", np.zeros(self.n))
         self.s = {k: np.array(state[k], dtype=self.dtype, copy=True, order="C").reshape(shapes[k])
                   for k in self.FIELDS}
         ng = int(params["n_grid"])
@@ -142,7 +173,8 @@ class OracleScene:
         self.grid_v = np.zeros((ng ** 3, 3), self.dtype)
         self.colliders = list(colliders)
         self._cp = make_params(ng, params["dx"], params["inv_dx"], params["mu"], params["lam"],
-                               params["alpha"], params["gravity"], params.get("material", 2))
+                               params["alpha"], params["gravity"], params.get("material", 2),
+                               params.get("hardening", 0), params.get("xi", 0.0))
         self._cc = make_colliders(self.colliders)
         self.time = float(params.get("time", 0.0))
 
@@ -151,7 +183,7 @@ class OracleScene:
         fn = getattr(lib(), "oracle_step" + _sfx(self.dtype))
         fn.restype = C.c_int
         fn(C.byref(self._cp), C.c_int64(self.n), _p(s["x0"]), _p(s["u"]), _p(s["v"]), _p(s["C"]),
-           _p(s["F"]), _p(s["tau"]), _p(s["mass"]), _p(s["vol"]), _p(self.x), _p(self.grid_m),
+           _p(s["F"]), _p(s["tau"]), _p(s["mass"]), _p(s["vol"]), _p(s["yield_stress"]), _p(self.x), _p(self.grid_m),
            _p(self.grid_mv), _p(self.grid_v), self._cc, C.c_int(len(self.colliders)),
            C.c_double(self.time), C.c_double(dt), C.c_int(nthreads))
         self.time = self.time + dt

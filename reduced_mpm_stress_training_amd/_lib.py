@@ -10,12 +10,12 @@ import os
 
 from ._build import LIB_PATH
 
-MPM_ABI_VERSION = 9
+MPM_ABI_VERSION = 10
 MAX_COLLIDERS = 16
 
 # plane indices of the particle state buffer
-P_X0, P_U, P_V, P_C, P_F, P_TAU, P_MASS, P_VOL, P_PID = 0, 3, 6, 9, 18, 27, 36, 37, 38
-NPLANES = 39
+P_X0, P_U, P_V, P_C, P_F, P_TAU, P_MASS, P_VOL, P_YIELD, P_PID = 0, 3, 6, 9, 18, 27, 36, 37, 38, 39
+NPLANES = 40
 # adjoint planes
 ADJ_X, ADJ_V, ADJ_C, ADJ_F, ADJ_TAU = 0, 3, 6, 15, 24
 ADJ_NPLANES = 33
@@ -25,7 +25,7 @@ class MpmParams(C.Structure):
     _fields_ = [("n_grid", C.c_int32), ("material", C.c_int32),
                 ("dx", C.c_float), ("inv_dx", C.c_float),
                 ("mu", C.c_float), ("lam", C.c_float), ("alpha", C.c_float),
-                ("gravity", C.c_float * 3)]
+                ("gravity", C.c_float * 3), ("hardening", C.c_int32), ("xi", C.c_float)]
 
 
 class MpmCollider(C.Structure):

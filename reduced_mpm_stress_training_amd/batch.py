@@ -38,7 +38,7 @@ def _bin_capacity(scene: S.SandScene) -> int:
 def _params(scene: S.SandScene) -> L.MpmParams:
     P = S.scene_params(scene)
     prm = L.MpmParams()
-    prm.n_grid, prm.material = int(P["n_grid"]), 2
+    prm.n_grid, prm.material = int(P["n_grid"]), int(P.get("material", 2))
     prm.dx, prm.inv_dx = float(P["dx"]), float(P["inv_dx"])
     prm.mu, prm.lam, prm.alpha = float(P["mu"]), float(P["lam"]), float(P["alpha"])
     for d in range(3):

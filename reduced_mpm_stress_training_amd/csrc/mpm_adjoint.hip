@@ -803,7 +803,7 @@ static int adjoint_substep(const MpmParams* prm, const MpmState* st, const void*
                            const MpmCollider* colliders, int n_colliders, void* workspace, float time, float dt,
                            const float* adj_next, float* adj_cur, int64_t adj_stride, double* param_adj,
                            bool vb_clear, bool clear_after, const float* cst, void* stream) {
-    int rc = check_params(prm);
+    int rc = check_sand(prm);
     if (rc) return rc;
     if (!st || !st->planes || st->n < 0 || st->stride < st->n) return set_err(-4, "state invalid");
     if (st->n > 0x7fffffffLL) return set_err(-6, "more than 2^31-1 particles");

@@ -460,12 +460,15 @@ __device__ __forceinline__ int64_t reserve_slot(bool act, int new_bin, int own_b
 }
 
 // all = false: only the planes the next substep's G2P reads (x0, u, F, m,
-// vol, pid); v, C and tau are then stale in this plane set.
+// vol, pid, and yield_stress for metal); v, C and tau are then stale in this
+// plane set.  Other materials do not carry yield_stress (k_bins_extract then
+// leaves the flat plane as it was).
 __device__ __forceinline__ void store_particle(float* __restrict__ dst, int64_t off, int64_t stride,
                                                const float x0[3], const float u[3], const float v[3],
                                                const float C[9], const float F[9], const float tau[9],
-                                               float m, float vol, float pid, bool all) {
+                                               float m, float vol, float pid, float ys, bool metal, bool all) {
     float* p = dst + off;
+    if (metal) p[MPM_P_YIELD * stride] = ys;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         p[(MPM_P_X0 + d) * stride] = x0[d];
@@ -491,12 +494,15 @@ __device__ __forceinline__ void store_particle(float* __restrict__ dst, int64_t 
 // tiles (nullptr on the overflow path); b0 the bin origin in cells.
 // The G2P inputs of one particle (x0, u, F, m, vol, pid), loaded ahead of use.
 struct ParticleIn {
-    float x0[3], u[3], F[9], m, vol, pid;
+    float x0[3], u[3], F[9], m, vol, pid, ys;
 };
 
+// metal: also the yield_stress plane
 __device__ __forceinline__ void load_particle(const float* __restrict__ src, int64_t off, int64_t S, bool act,
-                                              ParticleIn& q) {
+                                              bool metal, ParticleIn& q) {
+    q.ys = 0.f;
     if (act) {
+        if (metal) q.ys = src[off + MPM_P_YIELD * S];
         const float* sp = src + off;
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
@@ -533,6 +539,8 @@ __device__ __forceinline__ void g2p2g_particle(const SubstepArgs& a, const Scene
 #pragma unroll
     for (int k = 0; k < 9; ++k) F[k] = q.F[k];
     const float m = q.m, vol = q.vol, pid = q.pid;
+    const bool metal = prm.material == 1;
+    float ys = q.ys;
     float x[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) x[d] = x0[d] + u[d];
@@ -562,7 +570,7 @@ __device__ __forceinline__ void g2p2g_particle(const SubstepArgs& a, const Scene
                     for (int k = 0; k < 3; ++k) nodes[(i * 3 + j) * 3 + k] = vb[((int64_t)i * ng + j) * ng + k];
             gather_tile<9, 3>(Float4Nodes{nodes}, 0, A, prm.inv_dx, v, Cm, G);
         }
-        sand_update(G, dt, F, prm.mu, prm.lam, prm.alpha, tau);
+        constitutive_update(G, dt, F, prm.material, prm.mu, prm.lam, prm.alpha, prm.hardening, prm.xi, ys, tau);
         // x += dt v ; u = x - x0 ; the next substep recomputes x = x0 + u
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
@@ -612,7 +620,7 @@ __device__ __forceinline__ void g2p2g_particle(const SubstepArgs& a, const Scene
     // that left the grid (mode 0: they are copied from the source planes in
     // the substeps after, so those planes must hold their values).
     if (dslot >= 0)
-        store_particle(a.dst, dslot, dstride, x0, u, v, Cm, F, tau, m, vol, pid, a.store_all || mode == 0);
+        store_particle(a.dst, dslot, dstride, x0, u, v, Cm, F, tau, m, vol, pid, ys, metal, a.store_all || mode == 0);
     // P2G of the next substep, after the store so that x0, u, F are dead
     if (mode == 1) {
         scatter_tile(tile_acc, i0, A, m, vol, v, Cm, tau, prm.dx, prm.inv_dx, dt);
@@ -854,12 +862,13 @@ __device__ __forceinline__ void g2p2g_item(const SubstepArgs& a, int bin, int pa
     {
         const int64_t offp = off0 + lo;   // the item's first slot
         const int n_in = hi - lo;
+        const bool metal = sc.prm.material == 1;
         ParticleIn q;
-        load_particle(a.src, offp + tid, cap, tid < n_in, q);
+        load_particle(a.src, offp + tid, cap, tid < n_in, metal, q);
         for (int it = 0; it < n_in; it += WG) {
             const int i = it + tid;
             const ParticleIn cur = q;
-            if (it + WG < n_in) load_particle(a.src, offp + i + WG, cap, i + WG < n_in, q);  // prefetch
+            if (it + WG < n_in) load_particle(a.src, offp + i + WG, cap, i + WG < n_in, metal, q);  // prefetch
             g2p2g_particle(a, sc, i < n_in, offp + i, cap, bin, stay_base, s_stay, b0, s_vel, s_acc, cur);
         }
     }
@@ -903,7 +912,7 @@ __device__ __forceinline__ void g2p2g_overflow(const SubstepArgs& a, int blk) {
     const int b0[3] = {0, 0, 0};
     const int64_t off = ovf_off(g, i);
     ParticleIn q;
-    load_particle(a.src, off, g.ovf_blk, i < n_ovf, q);
+    load_particle(a.src, off, g.ovf_blk, i < n_ovf, true, q);
     const SceneDesc sc = scene_of_pid(a.tab, __float_as_int(q.pid));   // the particle's own scene
     g2p2g_particle(a, sc, i < n_ovf, off, g.ovf_blk, g.n_bins, -1, nullptr, b0, nullptr, nullptr, q);
 }
@@ -1191,7 +1200,10 @@ __global__ void __launch_bounds__(kWG)
         const int64_t off = ovf ? ovf_off(g, (wg - g.n_bins) * kWG + i) : bin_off(sc, wg) + i;
         const int pid = __float_as_int(planes[off + (int64_t)MPM_P_PID * S]);
         if (pid < 0 || pid >= n) continue;
-        for (int k = 0; k < MPM_NPLANES; ++k) flat[k * fstride + pid] = planes[off + (int64_t)k * S];
+        // yield_stress is carried by metal scenes only
+        const bool metal = (ovf ? scene_of_pid(tab, pid) : sc).prm.material == 1;
+        for (int k = 0; k < MPM_NPLANES; ++k)
+            if (k != MPM_P_YIELD || metal) flat[k * fstride + pid] = planes[off + (int64_t)k * S];
     }
 }
 

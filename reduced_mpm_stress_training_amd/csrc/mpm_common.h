@@ -13,6 +13,7 @@ int set_err(int code, const char* msg);
 int check_launch(const char* what);
 int check_grid(const MpmParams* prm);
 int check_params(const MpmParams* prm);
+int check_sand(const MpmParams* prm);  // check_params and material 2
 
 struct ColliderSet {
     MpmCollider c[MPM_MAX_COLLIDERS];

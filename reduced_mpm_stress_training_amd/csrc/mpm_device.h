@@ -4,6 +4,10 @@
 //   svd3_rot          3x3 SVD (rotation U, V)       wp.svd3 at mpm_utils.py:92,297
 //   sand_return_map   Drucker-Prager projection     mpm_utils.py:87-112
 //   dp_stress_sym     Kirchhoff stress of F_e       mpm_utils.py:45-52
+//   von_mises_return_map  metal projection          mpm_utils.py:59-84
+//   stvk_stress       StVK (Hencky) stress          mpm_utils.py:36-43
+//   fcr_stress        fixed-corotated stress        mpm_utils.py:29-34
+//   constitutive_update  the G2P material switch    mpm_utils.py:279-306
 //
 // All matrices are row-major float[9].  Loops have constant trip counts and
 // are fully unrolled so that every array lives in VGPRs.
@@ -257,6 +261,138 @@ __device__ __forceinline__ void sand_update(const float gradv[9], float dt, floa
     float ls[3];
     sand_return_map(Ftr, U, s, V, mu, lam, alpha, F, ls);
     dp_stress_sym(U, ls, mu, lam, tau);
+}
+
+// von_mises_return_mapping (mpm_utils.py:59-84) on F_tr given its SVD, for
+// metal.  sigma = max(s, 0.01); eps = ln sigma; the trial stress
+// t = 2 mu eps + lam (sum eps) and its deviator cond = t - sum(t)/3.  If
+// |cond| > ys: eps_hat = eps - mean(eps), n = |eps_hat| + 1e-6,
+// dg = n - ys/(2 mu), eps -= (dg/n) eps_hat, F_e = U diag(exp eps) V^T and,
+// when hardening == 1, ys += 2 mu xi dg.  Otherwise F_e = F_tr.  Writes ls,
+// the logs of the clamped singular values of F_e that the StVK stress takes
+// (the reference recomputes the SVD of F_e, mpm_utils.py:297: its singular
+// values are exp(eps) in the same order, and max(., 0.01) leaves them).
+// Returns true when the particle was projected.
+__device__ __forceinline__ bool von_mises_return_map(const float Ftr[9], const float U[9], const float s[3],
+                                                     const float V[9], float mu, float lam, float& ys,
+                                                     int hardening, float xi, float Fe[9], float ls[3]) {
+    float eps[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) eps[d] = logf(fmaxf(s[d], 0.01f));
+    const float esum = eps[0] + eps[1] + eps[2];
+    const float temp = esum / 3.0f;
+    float t[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) t[d] = 2.0f * mu * eps[d] + lam * esum;
+    const float st3 = (t[0] + t[1] + t[2]) / 3.0f;
+    const float c0 = t[0] - st3, c1 = t[1] - st3, c2 = t[2] - st3;
+    const float cn = sqrtf(c0 * c0 + c1 * c1 + c2 * c2);
+    if (!(cn > ys)) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) Fe[k] = Ftr[k];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) ls[d] = eps[d];
+        return false;
+    }
+    float eh[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) eh[d] = eps[d] - temp;
+    const float ehn = sqrtf(eh[0] * eh[0] + eh[1] * eh[1] + eh[2] * eh[2]) + 1e-6f;
+    const float dg = ehn - ys / (2.0f * mu);
+    const float r = dg / ehn;
+    float se[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        ls[d] = eps[d] - r * eh[d];
+        se[d] = expf(ls[d]);
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            Fe[i * 3 + j] = U[i * 3 + 0] * se[0] * V[j * 3 + 0] + U[i * 3 + 1] * se[1] * V[j * 3 + 1] +
+                            U[i * 3 + 2] * se[2] * V[j * 3 + 2];
+    if (hardening == 1) ys = ys + 2.0f * mu * xi * dg;
+    return true;
+}
+
+// kirchoff_stress_StVK (mpm_utils.py:36-43) from the SVD of F and
+// ls = ln max(s, 0.01): tau = U diag(2 mu ls_i + lam sum ls) V^T -- with V^T,
+// as the reference writes it (so tau is not symmetric unless U = V).
+__device__ __forceinline__ void stvk_stress(const float U[9], const float V[9], const float ls[3], float mu,
+                                            float lam, float tau[9]) {
+    const float lsum = ls[0] + ls[1] + ls[2];
+    const float c0 = 2.0f * mu * ls[0] + lam * lsum;
+    const float c1 = 2.0f * mu * ls[1] + lam * lsum;
+    const float c2 = 2.0f * mu * ls[2] + lam * lsum;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            tau[i * 3 + j] = U[i * 3 + 0] * c0 * V[j * 3 + 0] + U[i * 3 + 1] * c1 * V[j * 3 + 1] +
+                             U[i * 3 + 2] * c2 * V[j * 3 + 2];
+}
+
+// kirchoff_stress_FCR (mpm_utils.py:29-34) with R = U V^T from the SVD of F
+// and J = det F (wp.determinant, mpm_utils.py:292):
+//   tau = 2 mu (F - R) F^T + lam J (J - 1) I.
+__device__ __forceinline__ void fcr_stress(const float F[9], const float U[9], const float V[9], float mu, float lam,
+                                           float tau[9]) {
+    float D[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            D[i * 3 + j] = F[i * 3 + j] - (U[i * 3 + 0] * V[j * 3 + 0] + U[i * 3 + 1] * V[j * 3 + 1] +
+                                           U[i * 3 + 2] * V[j * 3 + 2]);
+    const float J = F[0] * (F[4] * F[8] - F[5] * F[7]) - F[1] * (F[3] * F[8] - F[5] * F[6]) +
+                    F[2] * (F[3] * F[7] - F[4] * F[6]);
+    const float lj = lam * J * (J - 1.0f);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            tau[i * 3 + j] = 2.0f * mu * (D[i * 3 + 0] * F[j * 3 + 0] + D[i * 3 + 1] * F[j * 3 + 1] +
+                                          D[i * 3 + 2] * F[j * 3 + 2]) +
+                             (i == j ? lj : 0.0f);
+}
+
+// The per-particle constitutive part of g2p_apic_and_stress
+// (mpm_utils.py:279-306) for every material: F_tr = (I + dt gradv) F, then
+//   material 1 (metal): F = von Mises(F_tr) on *ys, StVK stress;
+//   material 2 (sand):  F = Drucker-Prager(F_tr), stress of F_e;
+//   otherwise (jelly):  F = F_tr, FCR stress.
+// One SVD of F_tr serves the return map and the stress (the reference
+// recomputes it on F_e, mpm_utils.py:297).  `material` is uniform across a
+// workgroup, so the branches do not diverge.
+__device__ __forceinline__ void constitutive_update(const float gradv[9], float dt, float F[9], int material,
+                                                    float mu, float lam, float alpha, int hardening, float xi,
+                                                    float& ys, float tau[9]) {
+    float A[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) A[k] = gradv[k] * dt;
+    A[0] += 1.0f; A[4] += 1.0f; A[8] += 1.0f;
+    float Ftr[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            Ftr[i * 3 + j] = A[i * 3 + 0] * F[0 * 3 + j] + A[i * 3 + 1] * F[1 * 3 + j] +
+                             A[i * 3 + 2] * F[2 * 3 + j];
+    float U[9], s[3], V[9];
+    svd3_rot(Ftr, U, s, V);
+    float ls[3];
+    if (material == 2) {
+        sand_return_map(Ftr, U, s, V, mu, lam, alpha, F, ls);
+        dp_stress_sym(U, ls, mu, lam, tau);
+    } else if (material == 1) {
+        von_mises_return_map(Ftr, U, s, V, mu, lam, ys, hardening, xi, F, ls);
+        stvk_stress(U, V, ls, mu, lam, tau);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) F[k] = Ftr[k];
+        fcr_stress(F, U, V, mu, lam, tau);
+    }
 }
 
 }  // namespace mpm

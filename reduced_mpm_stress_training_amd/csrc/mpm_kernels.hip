@@ -60,7 +60,15 @@ int check_grid(const MpmParams* prm) {
 int check_params(const MpmParams* prm) {
     int rc = check_grid(prm);
     if (rc) return rc;
-    if (prm->material != 2) return set_err(-3, "only material 2 (sand) is implemented");
+    if (prm->material < 0 || prm->material > 2) return set_err(-3, "material must be 0 (jelly), 1 (metal) or 2 (sand)");
+    if (prm->material == 1 && prm->hardening != 0 && prm->hardening != 1) return set_err(-3, "hardening must be 0 or 1");
+    return 0;
+}
+
+int check_sand(const MpmParams* prm) {
+    int rc = check_params(prm);
+    if (rc) return rc;
+    if (prm->material != 2) return set_err(-3, "this entry point implements material 2 (sand) only");
     return 0;
 }
 
@@ -171,8 +179,8 @@ __global__ void __launch_bounds__(kBlock)
 // --------------------------------------------------------------------------
 // g2p_apic_and_stress (mpm_utils.py:241-306) + update_x_from_u / _u_from_x.
 // dpos = (i,j,k) - fx in grid units; C += v_i (x) dpos * (w 4 inv_dx);
-// grad v += v_i (x) grad w; x += dt v; F_tr = (I + dt grad v) F; return map;
-// stress.
+// grad v += v_i (x) grad w; x += dt v; F_tr = (I + dt grad v) F; the
+// material's return map and stress (constitutive_update).
 // --------------------------------------------------------------------------
 __global__ void __launch_bounds__(kBlock)
     k_g2p(float* __restrict__ planes, int64_t stride, int n, MpmParams prm, float dt,
@@ -222,7 +230,9 @@ __global__ void __launch_bounds__(kBlock)
     float F[9], tau[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) F[k] = pl[(MPM_P_F + k) * stride];
-    sand_update(gradv, dt, F, prm.mu, prm.lam, prm.alpha, tau);
+    float ys = prm.material == 1 ? pl[MPM_P_YIELD * stride] : 0.0f;
+    constitutive_update(gradv, dt, F, prm.material, prm.mu, prm.lam, prm.alpha, prm.hardening, prm.xi, ys, tau);
+    if (prm.material == 1) pl[MPM_P_YIELD * stride] = ys;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         float xn = x[d] + dt * nv[d];

@@ -506,7 +506,7 @@ namespace mpm {
 
 int p2g_ordered(const MpmParams* prm, const MpmState* st, const void* order, float* grid_acc, float dt,
                 void* stream) {
-    int rc = check_params(prm);
+    int rc = check_sand(prm);
     if (rc) return rc;
     if ((rc = check_flat(st))) return rc;
     if (!order || ((uintptr_t)order & 255)) return set_err(-40, "order buffer NULL or not 256-byte aligned");
@@ -523,7 +523,7 @@ int p2g_ordered(const MpmParams* prm, const MpmState* st, const void* order, flo
 int g2p2g_ordered(const MpmParams* prm, const MpmState* src, const MpmState* dst, const void* order,
                   const float* grid_v, float* acc_next, float dt, const float* cst, bool copy_const,
                   void* stream) {
-    int rc = check_params(prm);
+    int rc = check_sand(prm);
     if (rc) return rc;
     if ((rc = check_flat(src)) || (rc = check_flat(dst))) return rc;
     if (src->n != dst->n || src->stride != dst->stride) return set_err(-41, "src and dst differ in n or stride");
@@ -595,7 +595,7 @@ int mpm_sort_by_bin(const MpmParams* prm, const MpmState* src, const MpmState* d
 
 int mpm_step_ordered(const MpmParams* prm, const MpmState* st, const void* order, const MpmCollider* colliders,
                      int n_colliders, void* workspace, int64_t workspace_bytes, float time, float dt, void* stream) {
-    int rc = check_params(prm);
+    int rc = check_sand(prm);
     if (rc) return rc;
     if ((rc = check_flat(st))) return rc;
     const int64_t need = mpm_workspace_bytes(prm, st->n);
@@ -619,7 +619,7 @@ int64_t mpm_run_ordered_workspace_bytes(const MpmParams* prm) {
 int mpm_run_ordered(const MpmParams* prm, const MpmState* st, const void* order, const MpmCollider* colliders,
                     int n_colliders, void* workspace, int64_t workspace_bytes, int k0, double dt, int n_sub,
                     void* stream) {
-    int rc = check_params(prm);
+    int rc = check_sand(prm);
     if (rc) return rc;
     if ((rc = check_flat(st))) return rc;
     if (n_sub < 0) return set_err(-31, "n_sub < 0");

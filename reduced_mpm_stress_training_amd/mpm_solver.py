@@ -132,6 +132,8 @@ class MPM_Simulator_HIP:
         g = self._gravity_vec()
         for i in range(3):
             prm.gravity[i] = float(g[i])
+        prm.hardening = int(m.hardening)
+        prm.xi = float(m.xi)
         return prm
 
     def _gravity_vec(self):
@@ -190,7 +192,9 @@ class MPM_Simulator_HIP:
             m.nu = kwargs.get("nu", m.nu)
             m.mu, m.lam = lame_from_E_nu(m.E, m.nu)
         if "yield_stress" in kwargs:
+            # set_value_to_float_array over the per-particle array (mpm_solver_warp.py:108-110)
             m.yield_stress = kwargs["yield_stress"]
+            self.planes[L.P_YIELD, :self.n_particles].fill_(float(kwargs["yield_stress"]))
         if "hardening" in kwargs:
             m.hardening = kwargs["hardening"]
         if "xi" in kwargs:
@@ -229,8 +233,6 @@ class MPM_Simulator_HIP:
     # p2g2p (mpm_solver_warp.py:156-203)
     # ------------------------------------------------------------------
     def p2g2p(self, step, dt, print_norm=False, device="cuda:0"):
-        if self.mpm_model.material != 2:
-            raise NotImplementedError("only material='sand' is implemented by the HIP substep")
         lib = L.load()
         prm = self._params()
         cols, ncol = self._colliders()
@@ -282,8 +284,6 @@ class MPM_Simulator_HIP:
     def run(self, n_substeps, dt):
         """n_substeps consecutive substeps (p2g2p without prints) in one
         library call, without host synchronisation."""
-        if self.mpm_model.material != 2:
-            raise NotImplementedError("only material='sand' is implemented by the HIP substep")
         if self.use_bins and self.stage_timer is None and not self.debug:
             self._bins_run(int(n_substeps), float(dt))
         else:
@@ -525,6 +525,17 @@ class MPM_Simulator_HIP:
 
     def export_mass(self):
         return self._export(L.P_MASS, 1).view(-1)
+
+    def export_yield_stress(self):
+        """The per-particle yield stress (metal; mpm_model.yield_stress of
+        the reference, warp_utils.py:75)."""
+        return self._export(L.P_YIELD, 1).view(-1)
+
+    def load_yield_stress_from_tensor(self, tensor_ys, device="cuda:0"):
+        """Extension: per-particle yield stress (the reference fills one value
+        with set_parameters(yield_stress=))."""
+        _check_float_tensor(tensor_ys)
+        self._import(L.P_YIELD, 1, tensor_ys.reshape(-1).contiguous())
 
     def export_vol(self):
         return self._export(L.P_VOL, 1).view(-1)

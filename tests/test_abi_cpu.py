@@ -39,8 +39,16 @@ def test_abi_version_and_errors():
     assert lib.mpm_p2g(prm, st, None, 1e-4, None) < 0
     assert b"n_grid" in lib.mpm_last_error()
     prm.n_grid = 32
-    prm.material = 0
+    prm.material = 3
     assert lib.mpm_g2p(prm, st, None, 1e-4, None) < 0
+    assert b"material" in lib.mpm_last_error()
+    prm.material = 1
+    prm.hardening = 2
+    assert lib.mpm_g2p(prm, st, None, 1e-4, None) < 0
+    assert b"hardening" in lib.mpm_last_error()
+    prm.hardening = 0
+    # the ordered-state (adjoint) path is sand only
+    assert lib.mpm_step_ordered(prm, st, None, None, 0, None, 0, 1e-4, 0.0, None) < 0
     assert b"sand" in lib.mpm_last_error()
     prm.material = 2
     assert lib.mpm_step(prm, st, None, 0, None, 0, 0.0, 1e-4, None) < 0

@@ -229,7 +229,7 @@ def test_g2p_reproduces_affine_field(oracle):
     x = rng.uniform(0.3, 0.7, (50, 3))
     F = np.tile(np.eye(3), (50, 1, 1))
     dt = 1e-4
-    x_new, v, C, Fe, tau = NP.g2p(P, gv, x, F, dt)
+    x_new, v, C, Fe, tau, _ = NP.g2p(P, gv, x, F, dt)
     assert np.abs(v - (x @ A.T + b)).max() < 1e-12
     assert np.abs(C - A).max() < 1e-11
     Ftr = np.eye(3) + dt * A
@@ -241,8 +241,8 @@ def test_g2p_reproduces_affine_field(oracle):
     xc = np.ascontiguousarray(x.copy())
     gvc = np.ascontiguousarray(gv)
     oracle.lib().oracle_g2p_f64(Ct.byref(o._cp), Ct.c_int64(50), oracle._p(gvc), oracle._p(xc), oracle._p(o.s["v"]),
-                                oracle._p(o.s["C"]), oracle._p(o.s["F"]), oracle._p(o.s["tau"]), Ct.c_double(dt),
-                                Ct.c_int(1))
+                                oracle._p(o.s["C"]), oracle._p(o.s["F"]), oracle._p(o.s["tau"]), Ct.c_void_p(None),
+                                Ct.c_double(dt), Ct.c_int(1))
     assert np.abs(o.s["v"] - v).max() < 1e-12
     assert np.abs(o.s["C"] - C).max() < 1e-11
     assert np.abs(xc - x_new).max() < 1e-15
