@@ -42,7 +42,7 @@
 extern "C" {
 #endif
 
-#define MPM_ABI_VERSION 10
+#define MPM_ABI_VERSION 11
 
 /* plane indices */
 #define MPM_P_X0 0
@@ -79,10 +79,30 @@ typedef struct MpmParams {
     float xi;
 } MpmParams;
 
-/* Dirichlet_collider (engine_stress/warp_utils.py:126-151).
- * kind 0: add_surface_collider (mpm_solver_warp.py:407-451) -- every surface
- *         type writes v = 0 on the far side, as the reference does (:449);
- * kind 1: add_bounding_box (mpm_solver_warp.py:499-525), padding 3. */
+/* Dirichlet_collider (engine_stress/warp_utils.py:126-151).  Each kind is
+ * one of the reference's grid post-process kernels, applied to a node's v_out
+ * in registration order; "active" means start_time <= time < end_time with
+ * the time before the substep's increment (mpm_solver_warp.py:183, 199).
+ * kind 0: add_surface_collider / add_surface_collider2 (mpm_solver_warp.py:
+ *         407-497) -- every surface type writes v = 0 where
+ *         (x_node - point) . normal < 0, as the reference does (:449, :495);
+ * kind 1: add_bounding_box (:499-525), padding 3;
+ * kind 2: set_velocity_on_infinite_plane (:527-566): v = velocity where
+ *         |(x_node - point) . normal| <= threshold; when inactive and reset,
+ *         v = 0 there while time < end_time + 15 dt (:552-558);
+ * kind 3: set_column_moving_as_circle (:568-605): nodes with
+ *         dx^2 + dz^2 < radius^2 and |dy| < height get
+ *         v_scale (-sin t, 0, cos t), cos t = Z.x / |Z|, Z = point - (0.5, 0, 0.5);
+ * kind 4: set_rotation_on_disk (:607-664): nodes with |offset . normal| <
+ *         width and |r| < radius, r = (0, dy, dz), get the rotation
+ *         v_scale (-(r.y_unit) x_unit + (r.x_unit) y_unit); when inactive
+ *         and reset, every node gets 0 while time < end_time + 1.5 dt;
+ * kind 5: set_velocity_on_cube (:666-693): v = velocity where every
+ *         |offset_d| < width; inactive + reset: 0 everywhere while
+ *         time < end_time + 1.5 dt;
+ * kind 6: add_sphere_leader (:695-720): v = velocity where |offset| < radius.
+ * The moving kinds (2, 3, 5, 6) advance `point` on the host between substeps
+ * (the reference's `modify` callbacks); the library reads it as given. */
 typedef struct MpmCollider {
     int32_t kind;
     int32_t surface_type;
@@ -90,6 +110,10 @@ typedef struct MpmCollider {
     float normal[3];
     float start_time, end_time;
     float friction;
+    float velocity[3];
+    float threshold, radius, height, width, v_scale;
+    float x_unit[3], y_unit[3];
+    int32_t reset;
 } MpmCollider;
 
 /* Particle state: MPMStateStruct particle arrays (warp_utils.py:84-124). */

@@ -38,6 +38,9 @@ typedef double real;
 #define R_EXP exp
 #define R_SQRT sqrt
 #define R_FABS fabs
+#define R_ACOS acos
+#define R_SIN sin
+#define R_COS cos
 #define ORACLE_JACOBI_TOL 1e-30
 #else
 typedef float real;
@@ -46,6 +49,9 @@ typedef float real;
 #define R_EXP expf
 #define R_SQRT sqrtf
 #define R_FABS fabsf
+#define R_ACOS acosf
+#define R_SIN sinf
+#define R_COS cosf
 #define ORACLE_JACOBI_TOL 1e-14f
 #endif
 
@@ -62,13 +68,21 @@ typedef struct {
     double xi;            /* metal: MPMModelStruct.xi (warp_utils.py:81) */
 } OracleParams;
 
-/* Grid collider (warp_utils.py:126-151, Dirichlet_collider). */
+/* Grid collider (warp_utils.py:126-151, Dirichlet_collider).
+ * kind 0 surface half-space (mpm_solver_warp.py:407-497), 1 bounding box
+ * (:499-525), 2 velocity on an infinite plane (:527-566), 3 column moving on
+ * a circle (:568-605), 4 rotating disk (:607-664), 5 velocity on a cube
+ * (:666-693), 6 sphere leader (:695-720). */
 typedef struct {
-    int32_t kind;         /* 0 = surface half-space, 1 = bounding box */
+    int32_t kind;
     int32_t surface_type; /* 0 sticky, 1 slip, 2 frictional */
     double point[3];
     double normal[3];     /* already normalised (mpm_solver_warp.py:411-412) */
     double start_time, end_time, friction;
+    double velocity[3];
+    double threshold, radius, height, width, v_scale;
+    double x_unit[3], y_unit[3];
+    int32_t reset;
 } OracleCollider;
 
 /* ------------------------------------------------------------------ */
@@ -377,6 +391,54 @@ void SFX(oracle_p2g)(const OracleParams* P, int64_t n, const real* x,
 /*   bounding box      mpm_solver_warp.py:505-523.                       */
 /* grid_v (v_out) must be zero on entry (zero_grid, mpm_utils.py:120).   */
 /* ------------------------------------------------------------------ */
+/* The active branch of the velocity colliders (kinds 2-6) on one node:    */
+/* off = x_node - point, dot = off . normal.                               */
+static void moving_collider(const OracleCollider* cl, const real* off, real dot, real* vo) {
+    const real vs = (real)cl->v_scale;
+    if (cl->kind == 2) {                       /* mpm_solver_warp.py:550-551 */
+        if (R_FABS(dot) <= (real)cl->threshold) {
+            for (int d = 0; d < 3; ++d) vo[d] = (real)cl->velocity[d];
+        }
+    } else if (cl->kind == 3) {                /* :584-592 */
+        if (off[0] * off[0] + off[2] * off[2] < (real)cl->radius * (real)cl->radius &&
+            R_FABS(off[1]) < (real)cl->height) {
+            real Z[3] = {(real)cl->point[0] - (real)0.5, (real)cl->point[1], (real)cl->point[2] - (real)0.5};
+            real dp = Z[0] / R_SQRT(Z[0] * Z[0] + Z[1] * Z[1] + Z[2] * Z[2]);
+            real th = R_ACOS(dp);
+            if (Z[2] < (real)0) th = -th;
+            vo[0] = -vs * R_SIN(th);
+            vo[1] = 0;
+            vo[2] = vs * R_COS(th);
+        }
+    } else if (cl->kind == 4) {                /* :642-657 */
+        if (R_FABS(dot) < (real)cl->width) {
+            real r[3] = {0, off[1], off[2]};
+            real rl = R_SQRT(r[1] * r[1] + r[2] * r[2]);
+            if (rl < (real)cl->radius) {
+                if (rl > (real)0) {
+                    real dp = (r[1] * (real)cl->x_unit[1] + r[2] * (real)cl->x_unit[2]) / rl;
+                    real rdoty = r[1] * (real)cl->y_unit[1] + r[2] * (real)cl->y_unit[2];
+                    real th = R_ACOS(dp);
+                    if (rdoty < (real)0) th = -th;
+                    real sx = -rl * R_SIN(th) * vs, sy = rl * R_COS(th) * vs;
+                    for (int d = 0; d < 3; ++d) vo[d] = sx * (real)cl->x_unit[d] + sy * (real)cl->y_unit[d];
+                } else {
+                    /* on the axis: the reference divides 0 by 0; the limit of the rotation is 0 */
+                    vo[0] = vo[1] = vo[2] = 0;
+                }
+            }
+        }
+    } else if (cl->kind == 5) {                /* :683-684 */
+        if (R_FABS(off[0]) < (real)cl->width && R_FABS(off[1]) < (real)cl->width && R_FABS(off[2]) < (real)cl->width) {
+            for (int d = 0; d < 3; ++d) vo[d] = (real)cl->velocity[d];
+        }
+    } else if (cl->kind == 6) {                /* :712-713 */
+        if (R_SQRT(off[0] * off[0] + off[1] * off[1] + off[2] * off[2]) < (real)cl->radius) {
+            for (int d = 0; d < 3; ++d) vo[d] = (real)cl->velocity[d];
+        }
+    }
+}
+
 void SFX(oracle_grid_update)(const OracleParams* P, const real* grid_m,
                              const real* grid_mv, real* grid_v,
                              const OracleCollider* cols, int ncol,
@@ -393,28 +455,41 @@ void SFX(oracle_grid_update)(const OracleParams* P, const real* grid_m,
                 grid_v[node * 3 + a] = grid_mv[node * 3 + a] * im + dt * gv[a];
         }
     }
+    /* each collider's kernel over every node, in registration order */
     for (int c = 0; c < ncol; ++c) {
         const OracleCollider* cl = cols + c;
-        if (!(time >= (real)cl->start_time && time < (real)cl->end_time)) continue;
+        const int active = time >= (real)cl->start_time && time < (real)cl->end_time;
+        /* outside the window only the reset branches of kinds 2, 4, 5 act */
+        int reset = 0;
+        if (!active && cl->reset == 1) {
+            if (cl->kind == 2) reset = time < (real)cl->end_time + (real)1.5 * dt * (real)10.0;
+            if (cl->kind == 4 || cl->kind == 5) reset = time < (real)cl->end_time + (real)1.5 * dt;
+        }
+        if (!active && !reset) continue;
         for (int gx = 0; gx < ng; ++gx)
             for (int gy = 0; gy < ng; ++gy)
                 for (int gz = 0; gz < ng; ++gz) {
                     int64_t node = ((int64_t)gx * ng + gy) * ng + gz;
                     real* vo = grid_v + node * 3;
-                    if (cl->kind == 0) {
-                        real off[3] = {(real)gx * dx - (real)cl->point[0],
-                                       (real)gy * dx - (real)cl->point[1],
-                                       (real)gz * dx - (real)cl->point[2]};
-                        real dot = off[0] * (real)cl->normal[0] + off[1] * (real)cl->normal[1] +
-                                   off[2] * (real)cl->normal[2];
+                    real off[3] = {(real)gx * dx - (real)cl->point[0],
+                                   (real)gy * dx - (real)cl->point[1],
+                                   (real)gz * dx - (real)cl->point[2]};
+                    real dot = off[0] * (real)cl->normal[0] + off[1] * (real)cl->normal[1] +
+                               off[2] * (real)cl->normal[2];
+                    if (reset) {
+                        /* kind 2 zeroes its slab (:557-558), kinds 4, 5 every node (:661, :687) */
+                        if (cl->kind != 2 || R_FABS(dot) <= (real)cl->threshold) vo[0] = vo[1] = vo[2] = 0;
+                    } else if (cl->kind == 0) {
                         if (dot < (real)0) { vo[0] = vo[1] = vo[2] = 0; }
-                    } else {
+                    } else if (cl->kind == 1) {
                         const int pad = 3;
                         const int idx[3] = {gx, gy, gz};
                         for (int d = 0; d < 3; ++d) {
                             if (idx[d] < pad && vo[d] < (real)0) vo[d] = 0;
                             if (idx[d] >= ng - pad && vo[d] > (real)0) vo[d] = 0;
                         }
+                    } else {
+                        moving_collider(cl, off, dot, vo);
                     }
                 }
     }

@@ -31,7 +31,10 @@ class OracleCollider(C.Structure):
     _fields_ = [("kind", C.c_int32), ("surface_type", C.c_int32),
                 ("point", C.c_double * 3), ("normal", C.c_double * 3),
                 ("start_time", C.c_double), ("end_time", C.c_double),
-                ("friction", C.c_double)]
+                ("friction", C.c_double), ("velocity", C.c_double * 3),
+                ("threshold", C.c_double), ("radius", C.c_double), ("height", C.c_double), ("width", C.c_double),
+                ("v_scale", C.c_double), ("x_unit", C.c_double * 3), ("y_unit", C.c_double * 3),
+                ("reset", C.c_int32)]
 
 
 def build() -> str:
@@ -73,6 +76,20 @@ def make_params(n_grid, dx, inv_dx, mu, lam, alpha, gravity, material=2, hardeni
     return p
 
 
+def grid_update(params: dict, grid_m: np.ndarray, grid_mv: np.ndarray, colliders, time: float, dt: float):
+    """oracle_grid_update on (n^3,) masses and (n^3, 3) momenta; returns v_out (n^3, 3)."""
+    dtype = grid_m.dtype
+    P = make_params(params["n_grid"], params["dx"], params["inv_dx"], params["mu"], params["lam"],
+                    params["alpha"], params["gravity"], params.get("material", 2))
+    gm = np.ascontiguousarray(grid_m)
+    gmv = np.ascontiguousarray(grid_mv)
+    gv = np.zeros_like(gmv)
+    cc = make_colliders(colliders)
+    getattr(lib(), "oracle_grid_update" + _sfx(dtype))(C.byref(P), _p(gm), _p(gmv), _p(gv), cc,
+                                                         C.c_int(len(colliders)), C.c_double(time), C.c_double(dt))
+    return gv
+
+
 def make_colliders(colliders):
     arr = (OracleCollider * max(1, len(colliders)))()
     for i, c in enumerate(colliders):
@@ -84,7 +101,36 @@ def make_colliders(colliders):
         arr[i].start_time = float(c.get("start_time", 0.0))
         arr[i].end_time = float(c.get("end_time", 999.0))
         arr[i].friction = float(c.get("friction", 0.0))
+        for d in range(3):
+            arr[i].velocity[d] = float(c.get("velocity", (0, 0, 0))[d])
+            arr[i].x_unit[d] = float(c.get("x_unit", (0, 0, 0))[d])
+            arr[i].y_unit[d] = float(c.get("y_unit", (0, 0, 0))[d])
+        for k in ("threshold", "radius", "height", "width", "v_scale"):
+            setattr(arr[i], k, float(c.get(k, 0.0)))
+        arr[i].reset = int(c.get("reset", 0))
     return arr
+
+
+def move_collider(c: dict, time: float, dt: float):
+    """The host `modify` callbacks of the moving colliders, after the substep
+    at `time`: kinds 2, 5, 6 move the point by dt * velocity
+    (mpm_solver_warp.py:560-563, 688-691, 715-717), kind 3 along its circle
+    (:594-603); Python arithmetic stored back into a float32 vec3, as Warp's
+    struct fields are."""
+    k = c["kind"]
+    if k not in (2, 3, 5, 6) or not (time >= c["start_time"] and time < c["end_time"]):
+        return
+    p = c["point"]
+    if k == 3:
+        z = [np.float32(p[0] - 0.5), np.float32(p[1]), np.float32(p[2] - 0.5)]
+        ln = np.sqrt(np.float32(z[0] * z[0] + z[1] * z[1] + z[2] * z[2]))
+        th = float(np.arccos(np.float32(z[0] / ln)))
+        if z[2] < 0:
+            th = -th
+        vel = (-c["v_scale"] * np.sin(th), 0.0, c["v_scale"] * np.cos(th))
+    else:
+        vel = c["velocity"]
+    c["point"] = tuple(float(np.float32(p[d] + dt * vel[d])) for d in range(3))
 
 
 # ---- single-function entry points ---------------------------------------
@@ -171,7 +217,7 @@ class OracleScene:
         self.grid_m = np.zeros(ng ** 3, self.dtype)
         self.grid_mv = np.zeros((ng ** 3, 3), self.dtype)
         self.grid_v = np.zeros((ng ** 3, 3), self.dtype)
-        self.colliders = list(colliders)
+        self.colliders = [dict(c) for c in colliders]
         self._cp = make_params(ng, params["dx"], params["inv_dx"], params["mu"], params["lam"],
                                params["alpha"], params["gravity"], params.get("material", 2),
                                params.get("hardening", 0), params.get("xi", 0.0))
@@ -180,12 +226,16 @@ class OracleScene:
 
     def step(self, dt: float, nthreads: int = 1):
         s = self.s
+        if any(c["kind"] in (2, 3, 5, 6) for c in self.colliders):
+            self._cc = make_colliders(self.colliders)
         fn = getattr(lib(), "oracle_step" + _sfx(self.dtype))
         fn.restype = C.c_int
         fn(C.byref(self._cp), C.c_int64(self.n), _p(s["x0"]), _p(s["u"]), _p(s["v"]), _p(s["C"]),
            _p(s["F"]), _p(s["tau"]), _p(s["mass"]), _p(s["vol"]), _p(s["yield_stress"]), _p(self.x), _p(self.grid_m),
            _p(self.grid_mv), _p(self.grid_v), self._cc, C.c_int(len(self.colliders)),
            C.c_double(self.time), C.c_double(dt), C.c_int(nthreads))
+        for c in self.colliders:
+            move_collider(c, self.time, dt)
         self.time = self.time + dt
 
     def run(self, n_steps: int, dt: float, nthreads: int = 1):

@@ -10,7 +10,7 @@ import os
 
 from ._build import LIB_PATH
 
-MPM_ABI_VERSION = 10
+MPM_ABI_VERSION = 11
 MAX_COLLIDERS = 16
 
 # plane indices of the particle state buffer
@@ -32,7 +32,10 @@ class MpmCollider(C.Structure):
     _fields_ = [("kind", C.c_int32), ("surface_type", C.c_int32),
                 ("point", C.c_float * 3), ("normal", C.c_float * 3),
                 ("start_time", C.c_float), ("end_time", C.c_float),
-                ("friction", C.c_float)]
+                ("friction", C.c_float), ("velocity", C.c_float * 3),
+                ("threshold", C.c_float), ("radius", C.c_float), ("height", C.c_float), ("width", C.c_float),
+                ("v_scale", C.c_float), ("x_unit", C.c_float * 3), ("y_unit", C.c_float * 3),
+                ("reset", C.c_int32)]
 
 
 class MpmState(C.Structure):

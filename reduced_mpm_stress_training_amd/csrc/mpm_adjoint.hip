@@ -556,28 +556,8 @@ __global__ void __launch_bounds__(kBlk)
         const float im = 1.0f / a4.w;
         float o[3] = {a4.x * im + dt * prm.gravity[0], a4.y * im + dt * prm.gravity[1], a4.z * im + dt * prm.gravity[2]};
         float mask[3] = {1.f, 1.f, 1.f};
-        for (int c = 0; c < cs.n; ++c) {   // the colliders of grid_node_update, recording zeroed components
-            const MpmCollider& cl = cs.c[c];
-            if (!(time >= cl.start_time && time < cl.end_time)) continue;
-            if (cl.kind == 0) {
-                const float o0 = (float)gx * prm.dx - cl.point[0], o1 = (float)gy * prm.dx - cl.point[1],
-                            o2 = (float)gz * prm.dx - cl.point[2];
-                if (o0 * cl.normal[0] + o1 * cl.normal[1] + o2 * cl.normal[2] < 0.0f) {
-                    o[0] = o[1] = o[2] = 0.f;
-                    mask[0] = mask[1] = mask[2] = 0.f;
-                }
-            } else {
-                const int pad = 3;
-                const int gi[3] = {gx, gy, gz};
-#pragma unroll
-                for (int d = 0; d < 3; ++d) {
-                    if ((gi[d] < pad && o[d] < 0.f) || (gi[d] >= ng - pad && o[d] > 0.f)) {
-                        o[d] = 0.f;
-                        mask[d] = 0.f;
-                    }
-                }
-            }
-        }
+        for (int c = 0; c < cs.n; ++c)   // the colliders of grid_node_update, recording replaced components
+            apply_collider(cs.c[c], gx, gy, gz, ng, prm.dx, dt, time, o, mask);
         const float b0 = vb[4 * t] * mask[0], b1 = vb[4 * t + 1] * mask[1], b2 = vb[4 * t + 2] * mask[2];
         out.x = b0 * im;
         out.y = b1 * im;

@@ -52,6 +52,32 @@ def _check_float_tensor(t: torch.Tensor, ncols=None):
         assert t.shape[1] == ncols
 
 
+def _f32v(v):
+    """A Warp vec3 (float32) from three numbers."""
+    return tuple(float(np.float32(x)) for x in v)
+
+
+def _move_with_velocity(time, dt, c):
+    """modify of the plane, cube and sphere colliders (mpm_solver_warp.py:560-563,
+    688-691, 715-717): point += dt * velocity while active, evaluated as
+    Warp does it (Python floats, stored back into a float32 vec3)."""
+    if time >= c["start_time"] and time < c["end_time"]:
+        c["point"] = _f32v([c["point"][d] + dt * c["velocity"][d] for d in range(3)])
+
+
+def _move_on_circle(time, dt, c):
+    """modify of set_column_moving_as_circle (mpm_solver_warp.py:594-603)."""
+    if time >= c["start_time"] and time < c["end_time"]:
+        p = c["point"]
+        z = np.array([p[0] - 0.5, p[1], p[2] - 0.5], np.float32)
+        dp = np.float32(z[0] / np.sqrt(np.float32(z @ z)))
+        th = float(np.arccos(dp))
+        if z[2] < 0:
+            th = -th
+        vel = (-c["v_scale"] * math.sin(th), 0.0, c["v_scale"] * math.cos(th))
+        c["point"] = _f32v([p[d] + dt * vel[d] for d in range(3)])
+
+
 class MPM_Simulator_HIP:
     """Explicit MLS/APIC-MPM sand simulator on MI355X
     (MPM_Simulator_WARP, engine_stress/mpm_solver_warp.py:24-720; the
@@ -154,16 +180,33 @@ class MPM_Simulator_HIP:
         if n > L.MAX_COLLIDERS:
             raise ValueError(f"at most {L.MAX_COLLIDERS} colliders are supported")
         arr = (L.MpmCollider * max(n, 1))()
+        zero = (0.0, 0.0, 0.0)
         for i, c in enumerate(self.colliders):
             arr[i].kind = c["kind"]
             arr[i].surface_type = c.get("surface_type", 0)
             for d in range(3):
-                arr[i].point[d] = float(c.get("point", (0.0, 0.0, 0.0))[d])
-                arr[i].normal[d] = float(c.get("normal", (0.0, 0.0, 0.0))[d])
+                arr[i].point[d] = float(c.get("point", zero)[d])
+                arr[i].normal[d] = float(c.get("normal", zero)[d])
+                arr[i].velocity[d] = float(c.get("velocity", zero)[d])
+                arr[i].x_unit[d] = float(c.get("x_unit", zero)[d])
+                arr[i].y_unit[d] = float(c.get("y_unit", zero)[d])
             arr[i].start_time = float(c["start_time"])
             arr[i].end_time = float(c["end_time"])
             arr[i].friction = float(c.get("friction", 0.0))
+            for k in ("threshold", "radius", "height", "width", "v_scale"):
+                setattr(arr[i], k, float(c.get(k, 0.0)))
+            arr[i].reset = int(c.get("reset", 0))
         return arr, n
+
+    def _moving(self):
+        return any(c.get("modify") is not None for c in self.colliders)
+
+    def _apply_modify(self, time, dt):
+        """The colliders' host callbacks after a substep at `time` (the time
+        before the increment), in registration order (mpm_solver_warp.py:184-185)."""
+        for c in self.colliders:
+            if c.get("modify") is not None:
+                c["modify"](time, dt, c)
 
     # ------------------------------------------------------------------
     # set_parameters (mpm_solver_warp.py:89-124)
@@ -226,8 +269,62 @@ class MPM_Simulator_HIP:
         self.colliders.append(dict(kind=0, surface_type=stype, point=tuple(point), normal=tuple(normal),
                                    start_time=start_time, end_time=end_time, friction=friction))
 
+    def add_surface_collider2(self, point, normal, surface="sticky", friction=0.0, start_time=0.0,
+                              end_time=999.0):
+        """Same collider as add_surface_collider (mpm_solver_warp.py:453-497)."""
+        self.add_surface_collider(point, normal, surface, friction, start_time, end_time)
+
     def add_bounding_box(self, start_time=0.0, end_time=999.0):
         self.colliders.append(dict(kind=1, start_time=start_time, end_time=end_time))
+
+    def set_velocity_on_infinite_plane(self, point, normal, velocity, threshold, start_time=0.0, end_time=999.0,
+                                       reset=0):
+        """Nodes within `threshold` of the plane take `velocity`; the plane
+        moves with it (mpm_solver_warp.py:527-566)."""
+        s = 1.0 / math.sqrt(sum(x ** 2 for x in normal))
+        self.colliders.append(dict(kind=2, point=_f32v(point), normal=_f32v([s * x for x in normal]),
+                                   velocity=_f32v(velocity), threshold=float(threshold), start_time=start_time,
+                                   end_time=end_time, reset=int(reset), modify=_move_with_velocity))
+
+    def set_column_moving_as_circle(self, R, radius, height, v_scale, start_time=0.0, end_time=999.0):
+        """A column of nodes around `point` (initially (0.5, 0.069, 0.5 + R))
+        moving on a circle about x = z = 0.5 (mpm_solver_warp.py:568-605)."""
+        self.colliders.append(dict(kind=3, point=_f32v((0.5, 0.069, 0.5 + R)), radius=float(radius),
+                                   height=float(height), v_scale=float(v_scale), R=float(R), start_time=start_time,
+                                   end_time=end_time, modify=_move_on_circle))
+
+    def set_rotation_on_disk(self, point, normal, radius, width, v_scale, start_time=0.0, end_time=999.0, reset=0):
+        """Nodes in a disk take a rotation about its normal (mpm_solver_warp.py:607-664).
+        Local axes as the reference builds them: x = (-(n_y + n_z)/n_x, 1, 1),
+        y = n x x, both normalised.  The reference's axis search never ends
+        when |n_x| <= 0.1 (its loop index is not incremented, :611-614); that
+        case raises ValueError here."""
+        s = 1.0 / math.sqrt(sum(x ** 2 for x in normal))
+        n = [s * x for x in normal]
+        if not abs(n[0]) > 1e-1:
+            raise ValueError("set_rotation_on_disk: the reference's local axes need |normal[0]| > 0.1 "
+                             "(its axis search loops forever otherwise)")
+        ax = [1.0, 1.0, 1.0]
+        ax[0] = -(n[0] + n[1] + n[2] - n[0]) / n[0]
+        ay = list(np.cross(np.array(n), np.array(ax)))
+        ax = [x / math.sqrt(sum(y ** 2 for y in ax)) for x in ax]
+        ay = [x / math.sqrt(sum(y ** 2 for y in ay)) for x in ay]
+        self.colliders.append(dict(kind=4, point=_f32v(point), normal=_f32v(n), x_unit=_f32v(ax), y_unit=_f32v(ay),
+                                   radius=float(radius), width=float(width), v_scale=float(v_scale),
+                                   start_time=start_time, end_time=end_time, reset=int(reset)))
+
+    def set_velocity_on_cube(self, point, width, velocity, start_time=0.0, end_time=999.0, reset=0):
+        """Nodes within `width` of `point` in every axis take `velocity`; the
+        cube moves with it (mpm_solver_warp.py:666-693)."""
+        self.colliders.append(dict(kind=5, point=_f32v(point), width=float(width), velocity=_f32v(velocity),
+                                   start_time=start_time, end_time=end_time, reset=int(reset),
+                                   modify=_move_with_velocity))
+
+    def add_sphere_leader(self, center, radius, velocity, start_time=0, end_time=999):
+        """Nodes within `radius` of the center take `velocity`; the sphere
+        moves with it (mpm_solver_warp.py:695-720)."""
+        self.colliders.append(dict(kind=6, point=_f32v(center), radius=float(radius), velocity=_f32v(velocity),
+                                   start_time=start_time, end_time=end_time, modify=_move_with_velocity))
 
     # ------------------------------------------------------------------
     # p2g2p (mpm_solver_warp.py:156-203)
@@ -237,6 +334,7 @@ class MPM_Simulator_HIP:
         prm = self._params()
         cols, ncol = self._colliders()
         stream = _stream_handle(self.device)
+        t0 = self.time
         if self.use_bins:
             self._bins_run(1, float(dt))          # advances self.time
         else:
@@ -249,6 +347,7 @@ class MPM_Simulator_HIP:
             else:
                 self._timed_stages(lib, prm, st, cols, ncol, float(dt), stream)
             self.time = self.time + dt
+        self._apply_modify(t0, dt)
         if self.debug:
             v = self.export_uvFstress_from_warp()[1]
             vmax = float(torch.max(torch.abs(v)))
@@ -283,8 +382,9 @@ class MPM_Simulator_HIP:
 
     def run(self, n_substeps, dt):
         """n_substeps consecutive substeps (p2g2p without prints) in one
-        library call, without host synchronisation."""
-        if self.use_bins and self.stage_timer is None and not self.debug:
+        library call, without host synchronisation (one call per substep
+        while a collider moves: its host callback runs between substeps)."""
+        if self.use_bins and not self._moving() and self.stage_timer is None and not self.debug:
             self._bins_run(int(n_substeps), float(dt))
         else:
             for k in range(int(n_substeps)):
