@@ -164,6 +164,45 @@ int mpm_step(const MpmParams* prm, const MpmState* st, const MpmCollider* collid
              int n_colliders, void* workspace, int64_t workspace_bytes, float time,
              float dt, void* stream);
 
+/* ---- reduced-order sampled substep (CROM time stepper) ------------------
+ *
+ * MPM_Simulator_WARP.p2g2p_reduced (mpm_solver_warp.py:213-227) and the
+ * sample G2P its caller launches after it, on the flat state.  `samples`
+ * holds n_samples particle indices (int32, the reference's
+ * mpm_state.sample_particle_index_short); `density` is the per-particle
+ * density (n floats, mpm_state.particle_density).  workspace:
+ * mpm_reduced_workspace_bytes() bytes, 256-byte aligned, laid out as the
+ * accumulator grid (mv, m), the velocity grid (v, m) (float4 per node, C
+ * order as the stage grids), then the int32 grids neighboring_cells and
+ * neighboring_cells_grid_v (n_grid^3 each). */
+int64_t mpm_reduced_workspace_bytes(const MpmParams* prm);
+
+/* p2g2p_reduced (mpm_solver_warp.py:214-227) without the time increment:
+ * update_x_from_u, zero_grid (both grids and flag grids), update_neighboring_cells
+ * (mpm_utils.py:318-335: cells base-2 .. base+2 of every sample),
+ * p2g_sample_all (:356-410: particles in flagged cells set F = F_disp + I and
+ * scatter PIC momentum, mass and dt (-(m/rho) tau grad w), tau from F: FCR
+ * for jelly and metal, Drucker-Prager for sand), grid_normalization_and_
+ * gravity_sample_all (:413-421) and the colliders on every node. */
+int mpm_step_reduced(const MpmParams* prm, const MpmState* st, const int* samples, int64_t n_samples,
+                     const float* density, const MpmCollider* colliders, int n_colliders, void* workspace,
+                     int64_t workspace_bytes, float time, float dt, void* stream);
+
+/* g2p_sample (mpm_utils.py:423-471) on the velocity grid of the last
+ * mpm_step_reduced: each sample particle gets v, x += dt v, F from the
+ * material's return map of (I + dt grad v) F, u = x - x0.  C and tau are not
+ * written.  Metal reads and hardens yield_stress[s] of the sample's slot s
+ * in `samples`, as the reference does (:464). */
+int mpm_g2p_sample(const MpmParams* prm, const MpmState* st, const int* samples, int64_t n_samples,
+                   const void* workspace, float dt, void* stream);
+
+/* update_neighboring_cells_for_grid_v (mpm_utils.py:337-354), behind the
+ * reference's experiment_for_C (mpm_solver_warp.py:207-209): flags cells
+ * base-2 .. base+4 of every sample in the workspace's neighboring_cells_grid_v
+ * (cleared by mpm_step_reduced). */
+int mpm_mark_cells_grid_v(const MpmParams* prm, const MpmState* st, const int* samples, int64_t n_samples,
+                          void* workspace, void* stream);
+
 /* ---- binned path: the solver's own particle layout --------------------
  *
  * Particles are kept in bins of 8^3 grid cells keyed by the base cell of

@@ -439,22 +439,12 @@ static void moving_collider(const OracleCollider* cl, const real* off, real dot,
     }
 }
 
-void SFX(oracle_grid_update)(const OracleParams* P, const real* grid_m,
-                             const real* grid_mv, real* grid_v,
-                             const OracleCollider* cols, int ncol,
-                             double time_in, double dt_in) {
+/* The colliders' grid kernels over every node, in registration order */
+/* (mpm_solver_warp.py:182-185, :222-225).                             */
+static void apply_colliders(const OracleParams* P, real* grid_v, const OracleCollider* cols, int ncol,
+                            real time, real dt) {
     const int ng = P->n_grid;
-    const real dt = (real)dt_in, dx = (real)P->dx, time = (real)time_in;
-    const real gv[3] = {(real)P->gravity[0], (real)P->gravity[1], (real)P->gravity[2]};
-    const int64_t nn = (int64_t)ng * ng * ng;
-    for (int64_t node = 0; node < nn; ++node) {
-        real m = grid_m[node];
-        if (m > (real)1e-15) {
-            real im = (real)1.0 / m;
-            for (int a = 0; a < 3; ++a)
-                grid_v[node * 3 + a] = grid_mv[node * 3 + a] * im + dt * gv[a];
-        }
-    }
+    const real dx = (real)P->dx;
     /* each collider's kernel over every node, in registration order */
     for (int c = 0; c < ncol; ++c) {
         const OracleCollider* cl = cols + c;
@@ -493,6 +483,25 @@ void SFX(oracle_grid_update)(const OracleParams* P, const real* grid_m,
                     }
                 }
     }
+}
+
+void SFX(oracle_grid_update)(const OracleParams* P, const real* grid_m,
+                             const real* grid_mv, real* grid_v,
+                             const OracleCollider* cols, int ncol,
+                             double time_in, double dt_in) {
+    const int ng = P->n_grid;
+    const real dt = (real)dt_in, time = (real)time_in;
+    const real gv[3] = {(real)P->gravity[0], (real)P->gravity[1], (real)P->gravity[2]};
+    const int64_t nn = (int64_t)ng * ng * ng;
+    for (int64_t node = 0; node < nn; ++node) {
+        real m = grid_m[node];
+        if (m > (real)1e-15) {
+            real im = (real)1.0 / m;
+            for (int a = 0; a < 3; ++a)
+                grid_v[node * 3 + a] = grid_mv[node * 3 + a] * im + dt * gv[a];
+        }
+    }
+    apply_colliders(P, grid_v, cols, ncol, time, dt);
 }
 
 /* ------------------------------------------------------------------ */
@@ -629,4 +638,152 @@ int SFX(oracle_step)(const OracleParams* P, int64_t n, const real* x0, real* u,
     SFX(oracle_g2p)(P, n, grid_v, x, v, C, F, tau, yield, dt, nthreads);
     for (int64_t p = 0; p < 3 * n; ++p) u[p] = x[p] - x0[p];
     return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Reduced-order sampled substep (mpm_solver_warp.py:207-227).           */
+/* ------------------------------------------------------------------ */
+
+/* update_neighboring_cells (mpm_utils.py:318-335, lo = -2, hi = 2) and   */
+/* update_neighboring_cells_for_grid_v (:337-354, lo = -2, hi = 4).      */
+void SFX(oracle_mark_cells)(const OracleParams* P, const real* x, const int32_t* samples, int64_t ns, int lo,
+                            int hi, int32_t* flags) {
+    const int ng = P->n_grid;
+    const real inv_dx = (real)P->inv_dx;
+    for (int64_t s = 0; s < ns; ++s) {
+        const real* xp = x + 3 * (int64_t)samples[s];
+        int b[3];
+        for (int d = 0; d < 3; ++d) b[d] = (int)(xp[d] * inv_dx - (real)0.5);
+        for (int i = lo; i <= hi; ++i)
+            for (int j = lo; j <= hi; ++j)
+                for (int k = lo; k <= hi; ++k) {
+                    int gx = b[0] + i, gy = b[1] + j, gz = b[2] + k;
+                    if (0 <= gx && gx < ng && 0 <= gy && gy < ng && 0 <= gz && gz < ng)
+                        flags[((int64_t)gx * ng + gy) * ng + gz] = 1;
+                }
+    }
+}
+
+/* p2g_sample_all (mpm_utils.py:356-410).  F holds F_disp + I on entry    */
+/* (F_disp = F - I is formed and added back as the reference's arrays     */
+/* hold it) and is written for the particles in flagged cells.           */
+void SFX(oracle_p2g_sample_all)(const OracleParams* P, int64_t n, const real* x, const real* v, real* F,
+                                const real* mass, const real* density, const int32_t* flags, real* grid_m,
+                                real* grid_mv, double dt_in) {
+    const int ng = P->n_grid;
+    const real inv_dx = (real)P->inv_dx, dt = (real)dt_in, mu = (real)P->mu, lam = (real)P->lam;
+    for (int64_t p = 0; p < n; ++p) {
+        int base[3]; real fx[3], w[9], dw[9];
+        SFX(oracle_weights)(x + 3 * p, inv_dx, base, fx, w, dw);
+        if (base[0] < 0 || base[1] < 0 || base[2] < 0 || base[0] > ng - 3 || base[1] > ng - 3 || base[2] > ng - 3)
+            continue;
+        if (flags[((int64_t)base[0] * ng + base[1]) * ng + base[2]] <= 0) continue;
+        real* Fp = F + 9 * p;
+        for (int k = 0; k < 9; ++k) {
+            real id = (k % 4 == 0) ? (real)1 : (real)0;
+            Fp[k] = (Fp[k] - id) + id;
+        }
+        real U[9], V[9], sig[3], tau[9];
+        SFX(oracle_svd3)(Fp, U, sig, V);
+        real J = (real)1;
+        for (int d = 0; d < 3; ++d) J = J * sig[d];
+        if (P->material == 2) {
+            real ls = R_LOG(sig[0]) + R_LOG(sig[1]) + R_LOG(sig[2]);
+            real c[3], UC[9], UCV[9];
+            for (int d = 0; d < 3; ++d)
+                c[d] = (real)2 * mu * R_LOG(sig[d]) * ((real)1 / sig[d]) + lam * ls * ((real)1 / sig[d]);
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) UC[i * 3 + j] = U[i * 3 + j] * c[j];
+            mmT3(UC, V, UCV);
+            mmT3(UCV, Fp, tau);
+        } else {
+            real R[9], D[9];
+            mmT3(U, V, R);
+            for (int k = 0; k < 9; ++k) D[k] = Fp[k] - R[k];
+            mmT3(D, Fp, tau);
+            for (int k = 0; k < 9; ++k) tau[k] = (real)2 * mu * tau[k];
+            for (int d = 0; d < 3; ++d) tau[d * 4] = tau[d * 4] + lam * J * (J - (real)1);
+        }
+        const real mp = mass[p], nvol = -(mass[p] / density[p]);
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                for (int k = 0; k < 3; ++k) {
+                    int64_t node = ((int64_t)(base[0] + i) * ng + (base[1] + j)) * ng + (base[2] + k);
+                    real weight = w[0 * 3 + i] * w[1 * 3 + j] * w[2 * 3 + k];
+                    real g[3];
+                    dweight(w, dw, i, j, k, inv_dx, g);
+                    for (int a = 0; a < 3; ++a) {
+                        grid_mv[node * 3 + a] += weight * mp * v[3 * p + a];
+                        real f = (nvol * tau[a * 3 + 0]) * g[0] + (nvol * tau[a * 3 + 1]) * g[1] +
+                                 (nvol * tau[a * 3 + 2]) * g[2];
+                        grid_mv[node * 3 + a] += dt * f;
+                    }
+                    grid_m[node] += weight * mp;
+                }
+    }
+}
+
+/* grid_normalization_and_gravity_sample_all (mpm_utils.py:413-421) on the */
+/* flagged nodes, then the colliders; grid_v is zero on entry (zero_grid). */
+void SFX(oracle_grid_update_sample)(const OracleParams* P, const int32_t* flags, const real* grid_m,
+                                    const real* grid_mv, real* grid_v, const OracleCollider* cols, int ncol,
+                                    double time_in, double dt_in) {
+    const int ng = P->n_grid;
+    const real dt = (real)dt_in;
+    const int64_t nn = (int64_t)ng * ng * ng;
+    for (int64_t node = 0; node < nn; ++node) {
+        if (flags[node] <= 0) continue;
+        real m = grid_m[node];
+        if (m > (real)1e-10) {
+            for (int a = 0; a < 3; ++a) grid_v[node * 3 + a] = grid_mv[node * 3 + a] / m + dt * (real)P->gravity[a];
+        }
+    }
+    apply_colliders(P, grid_v, cols, ncol, (real)time_in, dt);
+}
+
+/* g2p_sample (mpm_utils.py:423-471): PIC gather, x += dt v, return map of */
+/* (I + dt grad v) F, u = x - x0.  Metal uses yield[s] of the sample slot  */
+/* s (the reference passes its thread index, :464).                        */
+void SFX(oracle_g2p_sample)(const OracleParams* P, const real* grid_v, const real* x0, real* u, real* v, real* F,
+                            real* yield, const int32_t* samples, int64_t ns, double dt_in) {
+    const int ng = P->n_grid;
+    const real inv_dx = (real)P->inv_dx, dt = (real)dt_in, mu = (real)P->mu, lam = (real)P->lam;
+    for (int64_t s = 0; s < ns; ++s) {
+        const int64_t q = samples[s];
+        real x[3];
+        for (int d = 0; d < 3; ++d) x[d] = x0[3 * q + d] + u[3 * q + d];
+        int base[3]; real fx[3], w[9], dw[9];
+        SFX(oracle_weights)(x, inv_dx, base, fx, w, dw);
+        if (base[0] < 0 || base[1] < 0 || base[2] < 0 || base[0] > ng - 3 || base[1] > ng - 3 || base[2] > ng - 3)
+            continue;
+        real nv[3] = {0, 0, 0}, nF[9] = {0};
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                for (int k = 0; k < 3; ++k) {
+                    int64_t node = ((int64_t)(base[0] + i) * ng + (base[1] + j)) * ng + (base[2] + k);
+                    real weight = w[0 * 3 + i] * w[1 * 3 + j] * w[2 * 3 + k];
+                    const real* g = grid_v + node * 3;
+                    real gw[3];
+                    dweight(w, dw, i, j, k, inv_dx, gw);
+                    for (int a = 0; a < 3; ++a) {
+                        nv[a] = nv[a] + g[a] * weight;
+                        for (int b = 0; b < 3; ++b) nF[a * 3 + b] = nF[a * 3 + b] + g[a] * gw[b];
+                    }
+                }
+        real A[9], Ftr[9];
+        for (int k = 0; k < 9; ++k) A[k] = dt * nF[k];
+        A[0] = A[0] + (real)1; A[4] = A[4] + (real)1; A[8] = A[8] + (real)1;
+        mm3(A, F + 9 * q, Ftr);
+        if (P->material == 1) {
+            SFX(oracle_von_mises)(Ftr, mu, lam, yield + s, P->hardening, (real)P->xi, F + 9 * q);
+        } else if (P->material == 2) {
+            SFX(oracle_sand_return_map)(Ftr, mu, lam, (real)P->alpha, F + 9 * q);
+        } else {
+            memcpy(F + 9 * q, Ftr, sizeof(Ftr));
+        }
+        for (int d = 0; d < 3; ++d) {
+            v[3 * q + d] = nv[d];
+            u[3 * q + d] = (x[d] + dt * nv[d]) - x0[3 * q + d];
+        }
+    }
 }

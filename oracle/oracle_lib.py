@@ -238,6 +238,49 @@ class OracleScene:
             move_collider(c, self.time, dt)
         self.time = self.time + dt
 
+    # ---- reduced-order sampled substep (mpm_solver_warp.py:207-227) --------
+    def _x(self):
+        return np.ascontiguousarray(self.s["x0"] + self.s["u"])
+
+    def mark_cells(self, samples, lo=-2, hi=2):
+        """update_neighboring_cells (lo, hi = -2, 2) / ..._for_grid_v (-2, 4)."""
+        ng = int(self.params["n_grid"])
+        flags = np.zeros(ng ** 3, np.int32)
+        smp = np.ascontiguousarray(samples, dtype=np.int32)
+        getattr(lib(), "oracle_mark_cells" + _sfx(self.dtype))(C.byref(self._cp), _p(self._x()), _p(smp),
+                                                                C.c_int64(smp.size), C.c_int(lo), C.c_int(hi),
+                                                                _p(flags))
+        return flags
+
+    def step_reduced(self, samples, density, dt: float):
+        """p2g2p_reduced: zero grids, flag cells, p2g_sample_all, grid update on
+        flagged nodes, colliders (moving ones advanced), time += dt."""
+        s = self.s
+        self.flags = self.mark_cells(samples)
+        self.grid_m[:] = 0
+        self.grid_mv[:] = 0
+        self.grid_v[:] = 0
+        dens = np.ascontiguousarray(density, dtype=self.dtype)
+        if any(c["kind"] in (2, 3, 5, 6) for c in self.colliders):
+            self._cc = make_colliders(self.colliders)
+        L = lib()
+        getattr(L, "oracle_p2g_sample_all" + _sfx(self.dtype))(
+            C.byref(self._cp), C.c_int64(self.n), _p(self._x()), _p(s["v"]), _p(s["F"]), _p(s["mass"]), _p(dens),
+            _p(self.flags), _p(self.grid_m), _p(self.grid_mv), C.c_double(dt))
+        getattr(L, "oracle_grid_update_sample" + _sfx(self.dtype))(
+            C.byref(self._cp), _p(self.flags), _p(self.grid_m), _p(self.grid_mv), _p(self.grid_v), self._cc,
+            C.c_int(len(self.colliders)), C.c_double(self.time), C.c_double(dt))
+        for c in self.colliders:
+            move_collider(c, self.time, dt)
+        self.time = self.time + dt
+
+    def g2p_sample(self, samples, dt: float):
+        s = self.s
+        smp = np.ascontiguousarray(samples, dtype=np.int32)
+        getattr(lib(), "oracle_g2p_sample" + _sfx(self.dtype))(
+            C.byref(self._cp), _p(self.grid_v), _p(s["x0"]), _p(s["u"]), _p(s["v"]), _p(s["F"]),
+            _p(s["yield_stress"]), _p(smp), C.c_int64(smp.size), C.c_double(dt))
+
     def run(self, n_steps: int, dt: float, nthreads: int = 1):
         for _ in range(n_steps):
             self.step(dt, nthreads)

@@ -78,6 +78,14 @@ _SIGNATURES = {
     "mpm_workspace_bytes": (C.c_int64, [C.POINTER(MpmParams), C.c_int64]),
     "mpm_step": (C.c_int, [C.POINTER(MpmParams), C.POINTER(MpmState), C.POINTER(MpmCollider), C.c_int,
                            C.c_void_p, C.c_int64, C.c_float, C.c_float, C.c_void_p]),
+    "mpm_reduced_workspace_bytes": (C.c_int64, [C.POINTER(MpmParams)]),
+    "mpm_step_reduced": (C.c_int, [C.POINTER(MpmParams), C.POINTER(MpmState), C.c_void_p, C.c_int64, C.c_void_p,
+                                   C.POINTER(MpmCollider), C.c_int, C.c_void_p, C.c_int64, C.c_float, C.c_float,
+                                   C.c_void_p]),
+    "mpm_g2p_sample": (C.c_int, [C.POINTER(MpmParams), C.POINTER(MpmState), C.c_void_p, C.c_int64, C.c_void_p,
+                                 C.c_float, C.c_void_p]),
+    "mpm_mark_cells_grid_v": (C.c_int, [C.POINTER(MpmParams), C.POINTER(MpmState), C.c_void_p, C.c_int64, C.c_void_p,
+                                        C.c_void_p]),
     "mpm_bins_workspace_bytes": (C.c_int64, [C.POINTER(MpmParams)]),
     "mpm_bins_insert": (C.c_int, [C.POINTER(MpmParams), C.POINTER(MpmState), C.POINTER(MpmBins), C.c_void_p,
                                   C.c_void_p]),

@@ -14,6 +14,7 @@ int check_launch(const char* what);
 int check_grid(const MpmParams* prm);
 int check_params(const MpmParams* prm);
 int check_sand(const MpmParams* prm);  // check_params and material 2
+int check_state(const MpmState* st);
 
 struct ColliderSet {
     MpmCollider c[MPM_MAX_COLLIDERS];

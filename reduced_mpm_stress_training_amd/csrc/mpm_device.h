@@ -333,11 +333,17 @@ __device__ __forceinline__ void stvk_stress(const float U[9], const float V[9], 
                              U[i * 3 + 2] * c2 * V[j * 3 + 2];
 }
 
-// kirchoff_stress_FCR (mpm_utils.py:29-34) with R = U V^T from the SVD of F
-// and J = det F (wp.determinant, mpm_utils.py:292):
+__device__ __forceinline__ float det3(const float F[9]) {
+    return F[0] * (F[4] * F[8] - F[5] * F[7]) - F[1] * (F[3] * F[8] - F[5] * F[6]) +
+           F[2] * (F[3] * F[7] - F[4] * F[6]);
+}
+
+// kirchoff_stress_FCR (mpm_utils.py:29-34) with R = U V^T from the SVD of F:
 //   tau = 2 mu (F - R) F^T + lam J (J - 1) I.
-__device__ __forceinline__ void fcr_stress(const float F[9], const float U[9], const float V[9], float mu, float lam,
-                                           float tau[9]) {
+// The full G2P passes J = det F (wp.determinant, mpm_utils.py:292), the
+// sampled P2G the product of the singular values (mpm_utils.py:372-374).
+__device__ __forceinline__ void fcr_stress(const float F[9], const float U[9], const float V[9], float J, float mu,
+                                           float lam, float tau[9]) {
     float D[9];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
@@ -345,8 +351,6 @@ __device__ __forceinline__ void fcr_stress(const float F[9], const float U[9], c
         for (int j = 0; j < 3; ++j)
             D[i * 3 + j] = F[i * 3 + j] - (U[i * 3 + 0] * V[j * 3 + 0] + U[i * 3 + 1] * V[j * 3 + 1] +
                                            U[i * 3 + 2] * V[j * 3 + 2]);
-    const float J = F[0] * (F[4] * F[8] - F[5] * F[7]) - F[1] * (F[3] * F[8] - F[5] * F[6]) +
-                    F[2] * (F[3] * F[7] - F[4] * F[6]);
     const float lj = lam * J * (J - 1.0f);
 #pragma unroll
     for (int i = 0; i < 3; ++i)
@@ -391,7 +395,7 @@ __device__ __forceinline__ void constitutive_update(const float gradv[9], float 
     } else {
 #pragma unroll
         for (int k = 0; k < 9; ++k) F[k] = Ftr[k];
-        fcr_stress(F, U, V, mu, lam, tau);
+        fcr_stress(F, U, V, det3(F), mu, lam, tau);
     }
 }
 

@@ -81,16 +81,13 @@ int make_collider_set(const MpmCollider* cols, int n, ColliderSet* cs) {
     return 0;
 }
 
-}  // namespace mpm
-
-namespace {
 int check_state(const MpmState* st) {
     if (!st || !st->planes) return set_err(-4, "state or planes is NULL");
     if (st->n < 0 || st->stride < st->n) return set_err(-5, "state n/stride invalid");
     if (st->n > 0x7fffffffLL) return set_err(-6, "more than 2^31-1 particles");
     return 0;
 }
-}  // namespace
+}  // namespace mpm
 
 // --------------------------------------------------------------------------
 // zero_grid (mpm_utils.py:115-123)

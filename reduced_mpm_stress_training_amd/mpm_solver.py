@@ -132,6 +132,10 @@ class MPM_Simulator_HIP:
         self._flat_valid = True      # flat planes mirror the binned state
         self._pending_dt = None      # dt of the P2G held in the accumulator
         self._since_check = 0
+        # reduced-order sampled step (p2g2p_reduced): sample indices and its
+        # own workspace (grids + neighboring_cells flag grids)
+        self.sample_particle_index_short = torch.zeros(1, dtype=torch.int32, device=self.device)
+        self._reduced_ws = None
         self._alloc_grid()
 
     def _alloc_grid(self):
@@ -361,6 +365,85 @@ class MPM_Simulator_HIP:
             print("norm of u, v, F_disp, stress, respectively: ",
                   "{:.4f}".format(float(torch.linalg.norm(u))), "{:.4f}".format(float(torch.linalg.norm(v))),
                   "{:.4f}".format(float(torch.linalg.norm(F_disp))), "{:.4f}".format(float(torch.linalg.norm(stress))))
+
+    # ------------------------------------------------------------------
+    # reduced-order sampled step (mpm_solver_warp.py:207-227, mpm_utils.py:318-471)
+    # ------------------------------------------------------------------
+    def load_sample_particle_index_from_tensor(self, index, device="cuda:0"):
+        """The sample particles of p2g2p_reduced / g2p_sample (the reference's
+        mpm_state.sample_particle_index_short, int32 particle indices)."""
+        idx = torch.as_tensor(index).reshape(-1).to(device=self.device, dtype=torch.int32).contiguous()
+        if idx.numel() and (int(idx.min()) < 0 or int(idx.max()) >= self.n_particles):
+            raise ValueError("sample indices must lie in [0, n_particles)")
+        self.sample_particle_index_short = idx
+
+    def _reduced_workspace(self):
+        lib = L.load()
+        nbytes = int(lib.mpm_reduced_workspace_bytes(self._params()))
+        if nbytes < 0:
+            L.check(-1, "mpm_reduced_workspace_bytes")
+        if self._reduced_ws is None or self._reduced_ws.numel() < nbytes + 256:
+            self._reduced_ws = torch.zeros(nbytes + 256, dtype=torch.uint8, device=self.device)
+        return (self._reduced_ws.data_ptr() + 255) & ~255, nbytes
+
+    def p2g2p_reduced(self, step, dt, device="cuda:0"):
+        """p2g2p_reduced (mpm_solver_warp.py:214-227): zero the grids, flag
+        the cells around the sample particles, P2G of the particles in flagged
+        cells (PIC, F = F_disp + I, stress from F, volume m / density), grid
+        normalization on flagged nodes, the colliders, time += dt.  The caller
+        then advances the samples with g2p_sample."""
+        self._ensure_flat()
+        lib = L.load()
+        ws, nbytes = self._reduced_workspace()
+        cols, ncol = self._colliders()
+        smp = self.sample_particle_index_short
+        t0 = self.time
+        L.check(lib.mpm_step_reduced(self._params(), self._state(), smp.data_ptr(), smp.numel(),
+                                     self.particle_density.data_ptr(), cols, ncol, ws, nbytes, float(self.time),
+                                     float(dt), _stream_handle(self.device)), "mpm_step_reduced")
+        self._apply_modify(t0, dt)
+        self.time = self.time + dt
+
+    def g2p_sample(self, dt, device="cuda:0"):
+        """g2p_sample (mpm_utils.py:423-471) on the sample particles, from the
+        grid of the last p2g2p_reduced: v, x, F (return map) and u; C and the
+        stress are left as they were."""
+        self._ensure_flat()
+        if self._reduced_ws is None:
+            raise RuntimeError("g2p_sample needs a p2g2p_reduced first")
+        lib = L.load()
+        ws, _ = self._reduced_workspace()
+        smp = self.sample_particle_index_short
+        L.check(lib.mpm_g2p_sample(self._params(), self._state(), smp.data_ptr(), smp.numel(), ws, float(dt),
+                                   _stream_handle(self.device)), "mpm_g2p_sample")
+
+    def experiment_for_C(self, device="cuda:0"):
+        """update_neighboring_cells_for_grid_v over the samples; returns the
+        flag grid neighboring_cells_grid_v (n, n, n) int32
+        (mpm_solver_warp.py:207-209)."""
+        self._ensure_flat()
+        lib = L.load()
+        ws, _ = self._reduced_workspace()
+        smp = self.sample_particle_index_short
+        L.check(lib.mpm_mark_cells_grid_v(self._params(), self._state(), smp.data_ptr(), smp.numel(), ws,
+                                          _stream_handle(self.device)), "mpm_mark_cells_grid_v")
+        return self._reduced_views()["cells_v"]
+
+    def _reduced_views(self):
+        ng = self.mpm_model.n_grid
+        nn = ng ** 3
+        ws, _ = self._reduced_workspace()
+        off = ws - self._reduced_ws.data_ptr()
+        b = self._reduced_ws
+        return dict(acc=b[off:off + 16 * nn].view(torch.float32).view(ng, ng, ng, 4),
+                    vel=b[off + 16 * nn:off + 32 * nn].view(torch.float32).view(ng, ng, ng, 4),
+                    cells=b[off + 32 * nn:off + 36 * nn].view(torch.int32).view(ng, ng, ng),
+                    cells_v=b[off + 36 * nn:off + 40 * nn].view(torch.int32).view(ng, ng, ng))
+
+    @property
+    def neighboring_cells(self):
+        """The flag grid of the last p2g2p_reduced (mpm_state.neighboring_cells)."""
+        return self._reduced_views()["cells"]
 
     def _timed_stages(self, lib, prm, st, cols, ncol, dt, stream):
         """The substep as its stage launches with a torch.cuda.Event between
