@@ -42,7 +42,7 @@
 extern "C" {
 #endif
 
-#define MPM_ABI_VERSION 11
+#define MPM_ABI_VERSION 12
 
 /* plane indices */
 #define MPM_P_X0 0
@@ -240,6 +240,9 @@ typedef struct MpmBins {
                              slots used in the overflow run (1), live particles per bin (nb^3) */
     int32_t wg_threads;   /* threads per bin workgroup of the fused kernel: 256 or 512 (0 = 512) */
     int32_t sort_period;  /* substeps between bank-order sorts (0 = 32, < 0 = never) */
+    float* record_acc;    /* NULL, or an n_grid^3 float4 grid that each grid update fills with the
+                             substep's summed P2G (mv, m) -- the reference's grid_v_in and grid_m after
+                             p2g2p (mpm_solver_warp.py:162); single scene only */
 } MpmBins;
 
 /* Workspace of the binned path (zeroed once before first use, 256-byte
@@ -262,6 +265,12 @@ int mpm_bins_insert(const MpmParams* prm, const MpmState* flat, const MpmBins* b
 
 /* flat[pid] <- every particle of plane set `cur` (all planes). */
 int mpm_bins_extract(const MpmParams* prm, const MpmBins* bins, const MpmState* flat, void* stream);
+
+/* mpm_bins_extract of the planes whose bits are set in plane_mask (bit k =
+ * plane k): what export_uvFstress_from_warp (mpm_solver_warp.py:305-310) or a
+ * snapshot needs, without moving the other planes. */
+int mpm_bins_extract_planes(const MpmParams* prm, const MpmBins* bins, const MpmState* flat, uint64_t plane_mask,
+                            void* stream);
 
 /* P2G (p2g_apic_stress_given, mpm_utils.py:190-227) of plane set `cur` into
  * the tiles of epoch+1, with the given dt; advances epoch. */

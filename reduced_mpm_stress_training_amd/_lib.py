@@ -10,7 +10,7 @@ import os
 
 from ._build import LIB_PATH
 
-MPM_ABI_VERSION = 11
+MPM_ABI_VERSION = 12
 MAX_COLLIDERS = 16
 
 # plane indices of the particle state buffer
@@ -46,7 +46,7 @@ class MpmBins(C.Structure):
     _fields_ = [("nb", C.c_int32), ("bin_cap", C.c_int32), ("overflow_cap", C.c_int32), ("cur", C.c_int32),
                 ("epoch", C.c_int32), ("since_sort", C.c_int32),
                 ("stride", C.c_int64), ("planes", C.c_void_p * 2), ("count", C.c_void_p),
-                ("wg_threads", C.c_int32), ("sort_period", C.c_int32)]
+                ("wg_threads", C.c_int32), ("sort_period", C.c_int32), ("record_acc", C.c_void_p)]
 
 
 class MpmScene(C.Structure):
@@ -90,6 +90,8 @@ _SIGNATURES = {
     "mpm_bins_insert": (C.c_int, [C.POINTER(MpmParams), C.POINTER(MpmState), C.POINTER(MpmBins), C.c_void_p,
                                   C.c_void_p]),
     "mpm_bins_extract": (C.c_int, [C.POINTER(MpmParams), C.POINTER(MpmBins), C.POINTER(MpmState), C.c_void_p]),
+    "mpm_bins_extract_planes": (C.c_int, [C.POINTER(MpmParams), C.POINTER(MpmBins), C.POINTER(MpmState), C.c_uint64,
+                                          C.c_void_p]),
     "mpm_bins_p2g": (C.c_int, [C.POINTER(MpmParams), C.POINTER(MpmBins), C.c_void_p, C.c_float, C.c_void_p]),
     "mpm_bins_status_offset": (C.c_int64, [C.POINTER(MpmParams)]),
     "mpm_bins_clear_grid": (C.c_int, [C.POINTER(MpmParams), C.POINTER(MpmBins), C.c_void_p, C.c_void_p]),

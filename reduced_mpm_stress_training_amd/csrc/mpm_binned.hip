@@ -647,7 +647,8 @@ __global__ void __launch_bounds__(kWG)
                 const int* __restrict__ stamps, const float4* __restrict__ tiles, const int* __restrict__ pcount,
                 const int* __restrict__ pslot, const double* __restrict__ ptiles, int* __restrict__ cnt_nxt,
                 const int* __restrict__ cnt_cur, int* __restrict__ order, BinGeom g, SceneTab tab,
-                ColliderSet cs, float time, float dt, int epoch, int clear_only, const double* __restrict__ t_base) {
+                ColliderSet cs, float time, float dt, int epoch, int clear_only, const double* __restrict__ t_base,
+                float4* __restrict__ rec) {
     __shared__ int s_live[27];             // neighbour bin c has a whole-bin tile of this epoch
     __shared__ int s_pslot[kParts * 27];   // FP64 tile slot of part p of neighbour bin c this epoch, or -1
     __shared__ int s_any, s_anyp;
@@ -695,9 +696,27 @@ __global__ void __launch_bounds__(kWG)
             s_pslot[part * 27 + c] = part < n_parts ? pslot[(int64_t)part * g.n_bins + lb] : -1;
         if (split) atomicOr(&s_anyp, 1);
     }
-    const int spill = gflag[blk];
+    // gflag bit 0: the spill grid holds contributions in this block; bit 1:
+    // the block's velocity nodes were written by an earlier grid update
+    const int gf = gflag[blk];
+    const int spill = gf & 1, written = gf & 2;
     __syncthreads();
-    if (!spill && !s_any && !s_anyp) return;
+    const bool live = spill || s_any || s_anyp;
+    if (!live && !written) return;
+    if (!live || clear_only) {
+        // drain the spill grid; a block that particles have left gets its
+        // velocity nodes zeroed once (zero_grid, mpm_utils.py:115-123)
+        for (int t = tid; t < 512; t += kWG) {
+            const int x = bx * 8 + (t >> 6), y = by * 8 + ((t >> 3) & 7), z = bz * 8 + (t & 7);
+            if (x >= ng || y >= ng || z >= ng) continue;
+            const int64_t node = ((int64_t)x * ng + y) * ng + z;
+            if (spill) acc[node] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (written) vel[node] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        __syncthreads();
+        if (tid == 0) gflag[blk] = 0;
+        return;
+    }
     for (int t = tid; t < 512; t += kWG) {
         const int x = bx * 8 + (t >> 6), y = by * 8 + ((t >> 3) & 7), z = bz * 8 + (t & 7);
         if (x >= ng || y >= ng || z >= ng) continue;
@@ -708,7 +727,6 @@ __global__ void __launch_bounds__(kWG)
             sx = a4.x; sy = a4.y; sz = a4.z; sm = a4.w;
             acc[node] = make_float4(0.f, 0.f, 0.f, 0.f);
         }
-        if (clear_only) continue;
         const int p[3] = {x, y, z};
         int hi[3], nlo[3];
 #pragma unroll
@@ -787,10 +805,11 @@ __global__ void __launch_bounds__(kWG)
             }
         }
         const float4 a4 = make_float4((float)sx, (float)sy, (float)sz, (float)sm);
+        if (rec) rec[sc.node_base + node] = a4;   // grid_v_in and grid_m of the reference
         vel[node] = grid_node_update(a4, x, y, z, ng, sc.prm.dx, dt, sc.prm.gravity, cols, n_cols, tnow);
     }
     __syncthreads();
-    if (tid == 0 && spill) gflag[blk] = 0;
+    if (tid == 0) gflag[blk] = 2;
 }
 
 // ---------------------------------------------------------------------------
@@ -1143,6 +1162,11 @@ __global__ void k_time_set(double* __restrict__ t, double v, int add) {
     if (threadIdx.x == 0 && blockIdx.x == 0) t[0] = add ? t[0] + v : v;
 }
 
+__global__ void __launch_bounds__(kWG) k_zero4(float4* __restrict__ g, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * kWG + threadIdx.x;
+    if (i < n) g[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
 __global__ void k_fill_int(int* __restrict__ p, int n, int v) {
     const int i = blockIdx.x * kWG + threadIdx.x;
     if (i < n) p[i] = v;
@@ -1184,7 +1208,7 @@ __global__ void __launch_bounds__(kWG)
 // flat[pid] <- every valid slot of plane set `cur`
 __global__ void __launch_bounds__(kWG)
     k_bins_extract(const float* __restrict__ planes, const int* __restrict__ cnt, BinGeom g, SceneTab tab,
-                   float* __restrict__ flat, int64_t fstride, int n) {
+                   float* __restrict__ flat, int64_t fstride, int n, uint64_t plane_mask) {
     const int wg = blockIdx.x;
     const bool ovf = wg >= g.n_bins;
     int n_in;
@@ -1203,7 +1227,8 @@ __global__ void __launch_bounds__(kWG)
         // yield_stress is carried by metal scenes only
         const bool metal = (ovf ? scene_of_pid(tab, pid) : sc).prm.material == 1;
         for (int k = 0; k < MPM_NPLANES; ++k)
-            if (k != MPM_P_YIELD || metal) flat[k * fstride + pid] = planes[off + (int64_t)k * S];
+            if (((plane_mask >> k) & 1) && (k != MPM_P_YIELD || metal))
+                flat[k * fstride + pid] = planes[off + (int64_t)k * S];
     }
 }
 
@@ -1394,11 +1419,12 @@ int op_insert(const Ctx& c, const MpmState* flat, const MpmBins* bins, hipStream
     return launch_sort(c, bins->planes[spare], cnt_spare, bins->planes[bins->cur], cnt, nullptr, 0, s);
 }
 
-int op_extract(const Ctx& c, const MpmBins* bins, const MpmState* flat, hipStream_t s) {
+int op_extract(const Ctx& c, const MpmBins* bins, const MpmState* flat, hipStream_t s,
+               uint64_t plane_mask = ~0ull) {
     if (!flat || !flat->planes || flat->n < 0 || flat->stride < flat->n) return set_err(-25, "flat state invalid");
     hipLaunchKernelGGL(k_bins_extract, dim3(c.g.n_bins + ovf_wgs(c.g)), dim3(kWG), 0, s, bins->planes[bins->cur],
                        (const int*)counts(bins, c.g, bins->cur), c.g, c.tab, flat->planes, flat->stride,
-                       (int)flat->n);
+                       (int)flat->n, plane_mask);
     return check_launch("k_bins_extract");
 }
 
@@ -1406,7 +1432,7 @@ int op_clear(const Ctx& c, MpmBins* bins, hipStream_t s) {
     hipLaunchKernelGGL(k_grid_bins, dim3(c.g.n_bins + 1), dim3(kWG), 0, s, c.w.acc, c.w.vel, c.w.gflag,
                        (const int*)c.w.stamps, (const float4*)c.w.tiles, (const int*)c.w.pcount,
                        (const int*)c.w.pslot, (const double*)c.w.ptiles, (int*)nullptr, (const int*)nullptr, (int*)nullptr, c.g, c.tab,
-                       c.cs, 0.f, 0.f, bins->epoch, 1, (const double*)nullptr);
+                       c.cs, 0.f, 0.f, bins->epoch, 1, (const double*)nullptr, (float4*)nullptr);
     int rc = check_launch("k_grid_bins(clear)");
     if (rc) return rc;
     bins->epoch += 1;  // the pending tiles are no longer current
@@ -1426,11 +1452,20 @@ int op_p2g(const Ctx& c, MpmBins* bins, float dt, hipStream_t s) {
 
 int op_grid(const Ctx& c, const MpmBins* bins, float time, float dt, hipStream_t s,
             const double* t_base = nullptr) {
+    float4* rec = (float4*)bins->record_acc;
+    if (rec && c.tab.dev) return set_err(-27, "bins: record_acc is for a single scene, not a batch");
+    if (rec) {
+        // the record holds this substep's P2G at every node: cleared first
+        const int64_t nn = c.tab.dev ? 0 : (int64_t)c.tab.one.prm.n_grid * c.tab.one.prm.n_grid * c.tab.one.prm.n_grid;
+        hipLaunchKernelGGL(k_zero4, dim3((unsigned)((nn + kWG - 1) / kWG)), dim3(kWG), 0, s, rec, nn);
+        int rc = check_launch("k_zero4");
+        if (rc) return rc;
+    }
     hipLaunchKernelGGL(k_grid_bins, dim3(c.g.n_bins + 1), dim3(kWG), 0, s, c.w.acc, c.w.vel, c.w.gflag,
                        (const int*)c.w.stamps, (const float4*)c.w.tiles, (const int*)c.w.pcount,
                        (const int*)c.w.pslot, (const double*)c.w.ptiles, counts(bins, c.g, 1 - bins->cur),
                        (const int*)counts(bins, c.g, bins->cur), c.w.order, c.g, c.tab, c.cs, time, dt, bins->epoch,
-                       0, t_base);
+                       0, t_base, rec);
     return check_launch("k_grid_bins");
 }
 
@@ -1640,6 +1675,15 @@ int mpm_bins_extract(const MpmParams* prm, const MpmBins* bins, const MpmState* 
     int rc = ctx_single(prm, bins, nullptr, nullptr, 0, false, &c);
     if (rc) return rc;
     return op_extract(c, bins, flat, (hipStream_t)stream);
+}
+
+int mpm_bins_extract_planes(const MpmParams* prm, const MpmBins* bins, const MpmState* flat, uint64_t plane_mask,
+                            void* stream) {
+    Ctx c;
+    int rc = ctx_single(prm, bins, nullptr, nullptr, 0, false, &c);
+    if (rc) return rc;
+    if (plane_mask >> MPM_NPLANES) return set_err(-26, "plane_mask names planes past MPM_NPLANES");
+    return op_extract(c, bins, flat, (hipStream_t)stream, plane_mask);
 }
 
 int mpm_bins_clear_grid(const MpmParams* prm, MpmBins* bins, void* workspace, void* stream) {

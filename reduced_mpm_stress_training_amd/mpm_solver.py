@@ -52,6 +52,14 @@ def _check_float_tensor(t: torch.Tensor, ncols=None):
         assert t.shape[1] == ncols
 
 
+_ALL_PLANES = (1 << L.NPLANES) - 1
+
+
+def _planes(first, n):
+    """Bitmask of planes first .. first + n - 1."""
+    return ((1 << n) - 1) << first
+
+
 def _f32v(v):
     """A Warp vec3 (float32) from three numbers."""
     return tuple(float(np.float32(x)) for x in v)
@@ -126,10 +134,15 @@ class MPM_Simulator_HIP:
         # binned path (default): state moves into per-block bins at the first
         # substep and back to the flat caller-order planes when a loader runs
         self.use_bins = True
+        # binned path: keep the summed P2G of every substep (the reference's
+        # grid_v_in), costing one extra grid write per substep
+        self.keep_grid_v_in = False
+        self._record = None
         self._bins = None            # dict of tensors + ctypes struct
         self._launch = (0, 0)        # (wg_threads, sort_period) of the binned kernels; 0 = default
         self._loc = "flat"           # where the authoritative state lives
-        self._flat_valid = True      # flat planes mirror the binned state
+        self._flat_valid = _ALL_PLANES   # bit k: flat plane k mirrors the binned state
+        self._fdisp_loaded = None        # F_disp of a sampling file, exported until the first substep
         self._pending_dt = None      # dt of the P2G held in the accumulator
         self._since_check = 0
         # reduced-order sampled step (p2g2p_reduced): sample indices and its
@@ -334,6 +347,7 @@ class MPM_Simulator_HIP:
     # p2g2p (mpm_solver_warp.py:156-203)
     # ------------------------------------------------------------------
     def p2g2p(self, step, dt, print_norm=False, device="cuda:0"):
+        self._fdisp_loaded = None
         lib = L.load()
         prm = self._params()
         cols, ncol = self._colliders()
@@ -393,6 +407,7 @@ class MPM_Simulator_HIP:
         normalization on flagged nodes, the colliders, time += dt.  The caller
         then advances the samples with g2p_sample."""
         self._ensure_flat()
+        self._fdisp_loaded = None
         lib = L.load()
         ws, nbytes = self._reduced_workspace()
         cols, ncol = self._colliders()
@@ -411,6 +426,7 @@ class MPM_Simulator_HIP:
         self._ensure_flat()
         if self._reduced_ws is None:
             raise RuntimeError("g2p_sample needs a p2g2p_reduced first")
+        self._fdisp_loaded = None
         lib = L.load()
         ws, _ = self._reduced_workspace()
         smp = self.sample_particle_index_short
@@ -559,6 +575,14 @@ class MPM_Simulator_HIP:
             L.check(lib.mpm_bins_p2g(prm, B["struct"], ws, dt, stream), "mpm_bins_p2g")
             self._pending_dt = dt
         cols, ncol = self._colliders()
+        if self.keep_grid_v_in:
+            ng = self.mpm_model.n_grid
+            if self._record is None or self._record.shape[0] != ng:
+                self._record = torch.zeros((ng, ng, ng, 4), dtype=torch.float32, device=self.device)
+            B["struct"].record_acc = self._record.data_ptr()
+        else:
+            B["struct"].record_acc = None
+            self._record = None
         if self.stage_timer is None:
             L.check(lib.mpm_bins_run(prm, B["struct"], cols, ncol, ws, float(self.time), dt, int(n_substeps),
                                      stream), "mpm_bins_run")
@@ -576,7 +600,8 @@ class MPM_Simulator_HIP:
                 ev[2].record()
                 self.stage_timer.append(ev)
                 self.time = self.time + dt
-        self._flat_valid = False
+        self._flat_valid = 0
+        self._fdisp_loaded = None
         self._since_check += int(n_substeps)
         if self._since_check >= 256:
             self._check_bins_health()
@@ -595,16 +620,19 @@ class MPM_Simulator_HIP:
                 "mpm_bins_profile")
         for _ in range(int(n_substeps)):
             self.time = self.time + float(dt)
-        self._flat_valid = False
+        self._flat_valid = 0
+        self._fdisp_loaded = None
         return gm.value, km.value, sm.value
 
-    def _sync_flat(self):
-        """Make the flat planes mirror the binned state (no change of owner)."""
-        if self._loc == "bins" and not self._flat_valid:
+    def _sync_flat(self, planes=_ALL_PLANES):
+        """Make the flat planes named by the bitmask `planes` mirror the binned
+        state (no change of owner); only the planes not yet copied move."""
+        miss = planes & ~self._flat_valid
+        if self._loc == "bins" and miss:
             lib = L.load()
-            L.check(lib.mpm_bins_extract(self._params(), self._bins["struct"], self._flat_state(),
-                                         _stream_handle(self.device)), "mpm_bins_extract")
-            self._flat_valid = True
+            L.check(lib.mpm_bins_extract_planes(self._params(), self._bins["struct"], self._flat_state(), miss,
+                                                _stream_handle(self.device)), "mpm_bins_extract_planes")
+            self._flat_valid |= miss
             self._check_bins_health()
 
     def _ensure_flat(self):
@@ -647,6 +675,7 @@ class MPM_Simulator_HIP:
         self._import(L.P_U, 3, tensor_u)
         self._import(L.P_V, 3, tensor_v)
         self._import(L.P_F, 9, tensor_F, mode=2)       # F = F_disp + I
+        self._fdisp_loaded = None
 
     def load_C_from_tensor(self, tensor_C, clone=True, device="cuda:0"):
         tensor_C = torch.reshape(tensor_C, (-1, 3, 3)).contiguous()
@@ -671,12 +700,14 @@ class MPM_Simulator_HIP:
         tensor_F = torch.reshape(tensor_F, (-1, 3, 3)).contiguous()
         _check_float_tensor(tensor_F, 3)
         self._import(L.P_F, 9, tensor_F)
+        self._fdisp_loaded = None
 
     # ------------------------------------------------------------------
     # export (mpm_solver_warp.py:305-310)
     # ------------------------------------------------------------------
     def _export(self, plane, ncomp, mode=0):
-        self._sync_flat()
+        need = _planes(L.P_X0, 3) | _planes(L.P_U, 3) if mode == 3 else _planes(plane, ncomp)
+        self._sync_flat(need | _planes(L.P_PID, 1))
         out = torch.empty((self.n_particles, ncomp), dtype=torch.float32, device=self.device)
         lib = L.load()
         L.check(lib.mpm_export(self._state(), plane, ncomp, out.data_ptr(), mode, _stream_handle(self.device)),
@@ -686,9 +717,14 @@ class MPM_Simulator_HIP:
     def export_uvFstress_from_warp(self, device="cuda:0"):
         """(u (N,3), v (N,3), F_disp (N,3,3), stress (N,3,3)) in the caller's
         particle order, as fresh tensors."""
+        self._sync_flat(_planes(L.P_U, 3) | _planes(L.P_V, 3) | _planes(L.P_F, 9) | _planes(L.P_TAU, 9) |
+                        _planes(L.P_PID, 1))
         u = self._export(L.P_U, 3)
         v = self._export(L.P_V, 3)
-        F_disp = self._export(L.P_F, 9, mode=1).view(-1, 3, 3)
+        if self._fdisp_loaded is not None:
+            F_disp = self._fdisp_loaded.clone()
+        else:
+            F_disp = self._export(L.P_F, 9, mode=1).view(-1, 3, 3)
         stress = self._export(L.P_TAU, 9).view(-1, 3, 3)
         return u, v, F_disp, stress
 
@@ -725,14 +761,18 @@ class MPM_Simulator_HIP:
 
     # grid views (reference: mpm_state.grid_m / grid_v_in / grid_v_out)
     def _grid_views(self):
-        """(accumulator, velocity) grids, (ng, ng, ng, 4) each.  On the binned
-        path the accumulator holds the pending P2G of the next substep."""
+        """(accumulator, velocity) grids, (ng, ng, ng, 4) each.  On the
+        binned path the P2G lives in per-bin tiles: the accumulator view is
+        the record of the last substep's summed P2G (keep_grid_v_in = True),
+        or None."""
         ng = self.mpm_model.n_grid
         nn = ng ** 3
         if self._loc == "bins" and self._bins is not None:
             ws = self._bins["ws"]
             off = self._bins_ws_ptr() - ws.data_ptr()
             vel_off = ((nn * 16 + 255) // 256) * 256      # workspace layout of the binned path
+            vel = ws[off + vel_off:off + vel_off + nn * 16].view(torch.float32).view(ng, ng, ng, 4)
+            return self._record, vel
         else:
             ws = self.workspace
             off = self._ws_ptr() - ws.data_ptr()
@@ -748,7 +788,13 @@ class MPM_Simulator_HIP:
 
     @property
     def grid_v_in(self):
-        return self._grid_views()[0][..., :3]
+        """The last substep's P2G momentum (grid_v_in).  On the binned path it
+        is recorded only with keep_grid_v_in = True set before stepping."""
+        acc = self._grid_views()[0]
+        if acc is None:
+            raise RuntimeError("grid_v_in on the binned path needs keep_grid_v_in = True before the substep "
+                               "(or use_bins = False)")
+        return acc[..., :3]
 
     @property
     def grid_v_out(self):
@@ -758,19 +804,44 @@ class MPM_Simulator_HIP:
     # samplings and snapshots (mpm_solver_warp.py:129-153, 312-404)
     # ------------------------------------------------------------------
     def load_from_sampling(self, sampling_h5, n_grid=100, grid_lim=1.0, device="cuda:0"):
+        """load_from_sampling (mpm_solver_warp.py:129-153): positions, masses,
+        volumes and densities of a sampling file (path = os.getcwd() + name,
+        as the reference builds it); the solver is re-initialised for its
+        particle count, v = 0, F = I.  The file's f_tensor is the reference's
+        F_disp until the first substep (:148): exports and a frame saved
+        before stepping show it, and the first substep starts from F = I as
+        the reference's does."""
         from . import snapshot
-        path = os.getcwd() + sampling_h5
-        if not os.path.exists(path):
-            raise FileNotFoundError(f"h5 file cannot be found at {path}")
-        d = snapshot.read_sampling(path)
+        d = snapshot.read_sampling(os.getcwd() + sampling_h5)   # FileNotFoundError if absent
         x, q = d["x"].T, d["q"].T
         self.dim, n = q.shape[1], q.shape[0]
         self.initialize(n, n_grid, grid_lim, device=device)
+        print("Sampling particles are loaded from h5 file. Simulator is re-initialized for the correct n_particles")
         dev = self.device
-        self.load_x_initial_from_tensor(torch.from_numpy(np.ascontiguousarray(x, dtype=np.float32)).to(dev))
-        self.load_mass_from_tensor(torch.from_numpy(np.ascontiguousarray(np.squeeze(d["masses"], 0), dtype=np.float32)).to(dev))
-        self.load_vol_from_tensor(torch.from_numpy(np.ascontiguousarray(np.squeeze(d["particle_volume"], 0), dtype=np.float32)).to(dev))
-        self.particle_density[:n] = torch.from_numpy(np.ascontiguousarray(np.squeeze(d["particle_density"], 0), dtype=np.float32)).to(dev)
+        f32 = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).to(dev)
+        self.load_x_initial_from_tensor(f32(x))
+        self.load_mass_from_tensor(f32(np.squeeze(d["masses"], 0)))
+        self.load_vol_from_tensor(f32(np.squeeze(d["particle_volume"], 0)))
+        self.particle_density[:n] = f32(np.squeeze(d["particle_density"], 0))
+        if "f_tensor" in d:
+            self._fdisp_loaded = f32(snapshot.f_tensor_rows(d["f_tensor"], n)).view(n, 3, 3)
+        print("Particle position, F, mass, and density are loaded from initial sampling.")
+
+    def load_uvFstress_from_h5(self, h5filename, sample_point=0, device="cuda:0"):
+        """Restart loader (mpm_solver_warp.py:268-303): u, v and the stress of a
+        saved frame (path = os.getcwd() + name).  The stress comes back from
+        its Voigt rows [xx, xy, xz, yy, yz, zz] as a symmetric 3x3; the
+        frame's f_tensor is read but not loaded, as in the reference (:301)."""
+        from . import snapshot
+        path = os.getcwd() + h5filename
+        d = snapshot.read_frame(snapshot.resolve_frame_path(path))
+        q = np.ascontiguousarray(d["q"].T, dtype=np.float32)
+        v = np.ascontiguousarray(d["v"].T, dtype=np.float32)
+        stress_in = snapshot.stress_from_voigt(d["stress"].T)
+        dev = self.device
+        f32 = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).to(dev)
+        self.load_uvstress_from_tensor(f32(q), f32(v), f32(stress_in.reshape(-1, 9)))
+        print("stress in:", np.linalg.norm(stress_in))
 
     def save_data_at_frame(self, dir_name, frame, include_parameter="friction_angle", save_to_h5=True,
                            save_to_obj=True, save_visual_F=False):

@@ -23,22 +23,84 @@ except Exception:  # pragma: no cover - depends on the image
 VOIGT = [0, 1, 2, 4, 5, 8]
 
 
-def frame_arrays(sim, include_parameter="friction_angle") -> dict:
-    n = sim.n_particles
-    u, v, F_disp, stress = sim.export_uvFstress_from_warp()
+def frame_dict(x0, u, v, F_disp, stress, C, mass, time, friction_angle=None) -> dict:
+    """The datasets of one frame (mpm_solver_warp.py:350-403) from (N, ...)
+    arrays: x, q, v (3, N); time (1, 1); masses (1, N); f_tensor, C (9, N)
+    row-major; stress (6, N) Voigt rows [xx, xy, xz, yy, yz, zz] (:383);
+    mu (1, N) = the friction angle when given."""
+    n = int(np.shape(x0)[0])
     out = {
-        "x": sim.export_x_initial().cpu().numpy().T,
-        "q": u.cpu().numpy().T,
-        "time": np.array([sim.time]).reshape(1, 1),
-        "masses": sim.export_mass().cpu().numpy().reshape(1, n),
-        "f_tensor": F_disp.reshape(-1, 9).cpu().numpy().T,
-        "stress": stress.reshape(-1, 9).cpu().numpy()[:, VOIGT].T,
-        "v": v.cpu().numpy().T,
-        "C": sim.export_C().reshape(-1, 9).cpu().numpy().T,
+        "x": np.asarray(x0).T,
+        "q": np.asarray(u).T,
+        "time": np.array([time]).reshape(1, 1),
+        "masses": np.asarray(mass).reshape(1, n),
+        "f_tensor": np.asarray(F_disp).reshape(-1, 9).T,
+        "stress": np.asarray(stress).reshape(-1, 9)[:, VOIGT].T,
+        "v": np.asarray(v).T,
+        "C": np.asarray(C).reshape(-1, 9).T,
     }
-    if include_parameter == "friction_angle":
-        out["mu"] = np.ones((1, n)) * sim.mpm_model.friction_angle
+    if friction_angle is not None:
+        out["mu"] = np.ones((1, n)) * friction_angle
     return out
+
+
+def frame_arrays(sim, include_parameter="friction_angle") -> dict:
+    u, v, F_disp, stress = sim.export_uvFstress_from_warp()
+    fa = sim.mpm_model.friction_angle if include_parameter == "friction_angle" else None
+    return frame_dict(sim.export_x_initial().cpu().numpy(), u.cpu().numpy(), v.cpu().numpy(),
+                      F_disp.cpu().numpy(), stress.cpu().numpy(), sim.export_C().cpu().numpy(),
+                      sim.export_mass().cpu().numpy(), sim.time, fa)
+
+
+def stress_from_voigt(st6: np.ndarray) -> np.ndarray:
+    """(N, 6) Voigt rows [xx, xy, xz, yy, yz, zz] -> symmetric (N, 3, 3), as
+    load_uvFstress_from_h5 rebuilds it (mpm_solver_warp.py:285-294)."""
+    st6 = np.asarray(st6)
+    out = np.zeros((st6.shape[0], 3, 3))
+    for k, (i, j) in enumerate([(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)]):
+        out[:, i, j] = st6[:, k]
+        out[:, j, i] = st6[:, k]
+    return out
+
+
+def f_tensor_rows(f, n: int) -> np.ndarray:
+    """A sampling file's f_tensor as (n, 9) row-major: the reference reads it
+    as (n, 3, 3) (mpm_solver_warp.py:136, 148); frames store (9, n)."""
+    f = np.asarray(f)
+    if f.shape == (9, n) and n != 9:
+        return f.T.reshape(n, 9)
+    return f.reshape(n, 9)
+
+
+def resolve_frame_path(path: str) -> str:
+    """The file to read for a frame or sampling path: the path itself, or its
+    .npz twin when the .h5 name is given and h5py is not installed."""
+    if os.path.exists(path) and (h5py is not None or not path.endswith(".h5")):
+        return path
+    if path.endswith(".h5") and os.path.exists(path[:-3] + ".npz"):
+        return path[:-3] + ".npz"
+    raise FileNotFoundError(f"h5 file cannot be found at {path}")
+
+
+def write_sampling(path, x, masses, particle_volume, particle_density, f_tensor=None):
+    """A sampling file in the layout load_from_sampling reads
+    (mpm_solver_warp.py:134-143): x, q (3, N), masses, particle_volume,
+    particle_density (1, N), f_tensor (N, 3, 3).  .npz (with these dataset
+    names) unless the path ends in .h5 and h5py is installed."""
+    x = np.asarray(x, dtype=np.float32)
+    n = x.shape[0]
+    arrs = {"x": x.T, "q": np.zeros((3, n), np.float32),
+            "masses": np.asarray(masses, np.float32).reshape(1, n),
+            "particle_volume": np.asarray(particle_volume, np.float32).reshape(1, n),
+            "particle_density": np.asarray(particle_density, np.float32).reshape(1, n),
+            "f_tensor": (np.zeros((n, 3, 3), np.float32) if f_tensor is None
+                         else np.asarray(f_tensor, np.float32).reshape(n, 3, 3))}
+    if path.endswith(".h5") and h5py is not None:
+        with h5py.File(path, "w") as f:
+            for k, a in arrs.items():
+                f.create_dataset(k, data=a)
+    else:
+        np.savez(path[:-3] + ".npz" if path.endswith(".h5") else path, **arrs)
 
 
 def write_obj(path, xyz: np.ndarray, color: np.ndarray):
@@ -67,6 +129,8 @@ def save_frame(sim, dir_name, frame, include_parameter="friction_angle", save_to
                 f.write("v %r %r %r %r 0 0\n" % (float(p[0]), float(p[1]), float(p[2]), float(ci)))
     if save_to_h5:
         arrs = frame_arrays(sim, include_parameter)
+        # the reference's messages (mpm_solver_warp.py:382, 404)
+        print("stress out:", np.linalg.norm(sim.export_uvFstress_from_warp()[3].reshape(-1, 9).cpu().numpy()))
         if h5py is not None:
             path = stem + ".h5"
             if os.path.exists(path):
@@ -75,7 +139,9 @@ def save_frame(sim, dir_name, frame, include_parameter="friction_angle", save_to
                 for k, a in arrs.items():
                     f.create_dataset(k, data=a)
         else:
-            np.savez(stem + ".npz", **arrs)
+            path = stem + ".npz"
+            np.savez(path, **arrs)
+        print("save siumlation data at frame ", frame, " to ", path)
 
 
 def read_frame(path) -> dict:
@@ -91,4 +157,4 @@ def read_frame(path) -> dict:
 def read_sampling(path) -> dict:
     """Initial sampling file (mpm_solver_warp.py:134-143): datasets x, q (3,N),
     masses, particle_volume, particle_density (1,N), f_tensor."""
-    return read_frame(path)
+    return read_frame(resolve_frame_path(path))
