@@ -9,8 +9,18 @@ resident in HBM before the timed region starts.
 
 Prints one JSON line with the metric, the roofline of the dominant kernel
 (algorithmic bytes per launch / its average duration, HIP events on the
-launch stream) and the CPU baseline (the FP32 CPU restatement of the same
-substep, OpenMP over the host cores, timed on a bounded sample).
+launch stream), the metric's second half -- forward + adjoint of a 200-substep
+differentiable rollout of the same 1M-particle column (`fwd_adjoint`, with its
+roofline at 700 B per particle-substep) -- and the CPU baseline (the FP32 CPU
+restatement of the same substep, OpenMP over the host cores allotted to the
+process, timed on a bounded sample; plus the 32^3 column on the same threads
+and the FP64 numpy oracle on one core).
+
+Exactly --steps substeps are timed.  The bank-order sort of the bins runs
+every sort-period substeps (32), so a short window holds none or one of
+them; the reported ms_per_step carries steps / period sorts (the window's
+own sorts replaced by that share, from the sort's measured duration), and
+`timing` gives the raw window.
 """
 from __future__ import annotations
 
@@ -38,6 +48,9 @@ ALG_BYTES = {
     # G2P reads x0,u,F and writes u,v,C,F,tau
     "p2g": 116,
     "g2p": 192,
+    # forward 284 + adjoint: read the forward state 152, read the adjoint of
+    # the next state 132, write the adjoint of this state 132
+    "fwd_adjoint": 700,
 }
 
 
@@ -57,6 +70,8 @@ def parse():
                     help="time the unbinned stage kernels (atomic P2G, grid update, G2P) instead")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy (reference for the HBM byte counters)")
+    ap.add_argument("--adjoint-steps", type=int, default=200,
+                    help="column: substeps of the forward + adjoint rollout measured beside the forward (0 = skip)")
     ap.add_argument("--keep", default="auto", choices=["auto", "all", "checkpoints"],
                     help="adjoint: keep every substep from the forward pass, or replay segments from checkpoints")
     ap.add_argument("--workload", default="column", choices=["column", "batch", "adjoint", "crom"],
@@ -186,57 +201,92 @@ def measured_traffic(kernel):
     return None
 
 
-def cpu_baseline(scene, budget_s):
-    """FP32 CPU restatement (oracle/, OpenMP) on the same workload for as many
-    substeps as fit in the budget (at least 1)."""
-    from oracle import oracle_lib
-    from reduced_mpm_stress_training_amd import scenes as S
+def _cpu_threads():
+    """Threads of the CPU baselines: the host cores allotted to this process
+    (OMP_NUM_THREADS, the box's share for one GPU), else every core in the
+    affinity mask."""
     try:
-        cores = len(os.sched_getaffinity(0))
+        visible = len(os.sched_getaffinity(0))
     except AttributeError:
-        cores = os.cpu_count() or 1
-    cores = max(1, min(cores, 16))
-    oracle_lib.build()
-    o = oracle_lib.OracleScene(S.scene_params(scene), S.scene_state(scene, np.float32),
-                               S.scene_colliders(scene), np.float32)
-    o.step(scene.dt, nthreads=cores)          # first touch / warm-up
+        visible = os.cpu_count() or 1
+    share = os.environ.get("OMP_NUM_THREADS")
+    use = int(share) if share and share.isdigit() and int(share) > 0 else visible
+    return max(1, min(use, visible)), visible
+
+
+def _time_oracle(o, dt, budget_s, nthreads):
+    o.step(dt, nthreads=nthreads)          # first touch / warm-up
     steps = 0
     t0 = time.perf_counter()
     while True:
-        o.step(scene.dt, nthreads=cores)
+        o.step(dt, nthreads=nthreads)
         steps += 1
         el = time.perf_counter() - t0
         if el >= budget_s or el / steps * (steps + 1) > 2 * budget_s:
-            break
-    return {"value": scene.n_particles * steps / el, "unit": "particle-substeps/s", "cores": cores,
-            "kind": "port",
-            "sample": f"{steps} substeps of the same {scene.n_particles}-particle, {scene.n_grid}^3 workload "
-                      f"({el:.1f} s), FP32 C restatement with OpenMP atomics"}
+            return steps, el
 
 
-def bench_adjoint(args):
-    """Differentiable rollout: 250k particles i.i.d. in [0.3,0.7]^2 x
-    [0.05,0.6] (v ~ N(0, 0.1)) on a 64^3 grid, 200 substeps of dt = 1e-4,
+def cpu_baseline(scene, budget_s):
+    """FP32 CPU restatement (oracle/, OpenMP) on the same workload for as many
+    substeps as fit in the budget (at least 1); beside it the same C
+    restatement on the 32^3 column, and the FP64 numpy oracle (one process)
+    on that column."""
+    from oracle import oracle_lib
+    from oracle import mpm_oracle_np as NP
+    from reduced_mpm_stress_training_amd import scenes as S
+    cores, visible = _cpu_threads()
+    oracle_lib.build()
+    o = oracle_lib.OracleScene(S.scene_params(scene), S.scene_state(scene, np.float32),
+                               S.scene_colliders(scene), np.float32)
+    steps, el = _time_oracle(o, scene.dt, budget_s, cores)
+    out = {"value": scene.n_particles * steps / el, "unit": "particle-substeps/s", "cores": cores,
+           "host_cores_visible": visible, "kind": "port",
+           "sample": f"{steps} substeps of the same {scene.n_particles}-particle, {scene.n_grid}^3 workload "
+                     f"({el:.1f} s), FP32 C restatement with OpenMP atomics, {cores} threads"}
+    c1 = S.column_lattice()
+    o1 = oracle_lib.OracleScene(S.scene_params(c1), S.scene_state(c1, np.float32), S.scene_colliders(c1),
+                                np.float32)
+    s1, e1 = _time_oracle(o1, c1.dt, min(3.0, budget_s), cores)
+    out["column_32"] = {"value": c1.n_particles * s1 / e1, "unit": "particle-substeps/s", "cores": cores,
+                        "sample": f"{s1} substeps of the 32^3 column ({c1.n_particles} particles, {e1:.1f} s)"}
+    P = S.scene_params(c1)
+    st = S.scene_state(c1, np.float64)
+    cols = S.scene_colliders(c1)
+    NP.step(P, st, cols, 0.0, c1.dt)
+    k, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < min(5.0, budget_s) or k == 0:
+        st = NP.step(P, st, cols, k * c1.dt, c1.dt)
+        k += 1
+    e2 = time.perf_counter() - t0
+    out["fp64_numpy_column_32"] = {"value": c1.n_particles * k / e2, "unit": "particle-substeps/s", "cores": 1,
+                                   "sample": f"{k} substeps of the 32^3 column, FP64 numpy oracle ({e2:.1f} s)"}
+    return out
+
+
+def measure_adjoint(n, ng, steps, keep="auto", fwd_ms=None):
+    """Differentiable rollout: n particles i.i.d. in [0.3,0.7]^2 x [0.05,0.6]
+    (v ~ N(0, 0.1)) on an ng^3 grid, `steps` substeps of dt = 1e-4,
     checkpoints every 20; loss = mean |x_final - x_target|^2 against a target
     rollout at E = 3e5 Pa, friction angle 35 deg; gradient with respect to
     (E, nu, friction angle, initial velocities) by the adjoint kernels.
-    A step is one forward substep plus its adjoint: with --keep all (the
+    A step is one forward substep plus its adjoint: with keep "all" (the
     default "auto" takes it where half the free HBM holds the record) the
     forward keeps every substep's state and grids and the adjoint runs on
-    them; with --keep checkpoints each segment is replayed first.  One
+    them; with keep "checkpoints" each segment is replayed first.  One
     gradient evaluation runs untimed first (the rollout's buffers are then
     allocated, as in any fitting loop after its first iteration); the next one
-    is timed."""
+    is timed.  fwd_ms: the binned forward's ms per substep on the same scene
+    (measured here when not given)."""
     import torch
     from reduced_mpm_stress_training_amd import adjoint as A
     from reduced_mpm_stress_training_amd import scenes as S
-    n, ng, steps = args.particles, args.n_grid, args.steps
     sc = S.column_uniform(n_particles=n, n_grid=ng)
-    ro = A.Rollout(sc, steps, 1e-4, checkpoint_every=20, keep=args.keep)
+    ro = A.Rollout(sc, steps, 1e-4, checkpoint_every=20, keep=keep)
     v0 = torch.from_numpy(sc.v0).cuda()
     with torch.no_grad():
         x_t = ro.forward(v0, 3.0e5, 0.3, 35.0).clone()
     torch.cuda.synchronize()
+
     def grad_eval():
         v = v0.clone().requires_grad_(True)
         params = torch.tensor([2.0e5, 0.3, 30.0], dtype=torch.float64, requires_grad=True)
@@ -259,26 +309,42 @@ def bench_adjoint(args):
     t2 = time.perf_counter()
     g = params.grad.tolist()
     finite = bool(torch.isfinite(v.grad).all()) and all(map(lambda z: z == z, g))
-    # the forward alone on the production (binned) path, same scene and substeps
-    sim = S.make_simulator(sc)
-    sim.run(10, 1e-4)
-    torch.cuda.synchronize()
-    t3 = time.perf_counter()
-    sim.run(steps, 1e-4)
-    torch.cuda.synchronize()
-    ms_fwd = (time.perf_counter() - t3) * 1e3 / steps
-    out = {
-        "metric": "particle-substeps/s (fwd+adjoint)", "value": n * steps / (t2 - t0), "unit": "particle-substeps/s",
-        "n_gpus": 1, "steps": steps, "warmup": steps, "ms_per_step": (t2 - t0) * 1e3 / steps, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "differentiable sand rollout, forward + adjoint", "n_particles": n, "n_grid": ng,
-                   "substeps": steps, "checkpoint_every": 20, "parallelism": "single GPU",
-                   "keep": "all" if ro._recorded else "checkpoints",
-                   "record_gb": ro.record_bytes() / 1e9 if ro._recorded else 0.0},
-        "forward_ms_per_step": (t1 - t0) * 1e3 / steps, "backward_ms_per_step": (t2 - t1) * 1e3 / steps,
-        "grad": {"E": g[0], "nu": g[1], "friction_angle": g[2]}, "finite": finite,
-        "binned_forward_ms_per_step": ms_fwd, "fwd_adjoint_over_binned_forward": (t2 - t0) * 1e3 / steps / ms_fwd,
-    }
+    if fwd_ms is None:
+        # the forward alone on the production (binned) path, same scene and substeps
+        sim = S.make_simulator(sc)
+        sim.run(10, 1e-4)
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        sim.run(steps, 1e-4)
+        torch.cuda.synchronize()
+        fwd_ms = (time.perf_counter() - t3) * 1e3 / steps
+        del sim
+    ms = (t2 - t0) * 1e3 / steps
+    out = {"value": n * steps / (t2 - t0), "unit": "particle-substeps/s", "ms_per_step": ms,
+           "forward_ms_per_step": (t1 - t0) * 1e3 / steps, "backward_ms_per_step": (t2 - t1) * 1e3 / steps,
+           "binned_forward_ms_per_step": fwd_ms, "fwd_adjoint_over_binned_forward": ms / fwd_ms,
+           "config": {"workload": "differentiable sand rollout, forward + adjoint", "n_particles": n, "n_grid": ng,
+                      "substeps": steps, "checkpoint_every": 20,
+                      "keep": "all" if ro._recorded else "checkpoints",
+                      "record_gb": ro.record_bytes() / 1e9 if ro._recorded else 0.0},
+           "roofline": {"bound": "hbm", "achieved": ALG_BYTES["fwd_adjoint"] * n / (ms / 1e3) / 1e9,
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": ALG_BYTES["fwd_adjoint"] * n / (ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+                        "bytes_per_particle": ALG_BYTES["fwd_adjoint"]},
+           "grad": {"E": g[0], "nu": g[1], "friction_angle": g[2]}, "finite": finite}
+    del ro
+    torch.cuda.empty_cache()
+    return out
+
+
+def bench_adjoint(args):
+    """The forward + adjoint rollout alone (measure_adjoint), as its own JSON line."""
+    r = measure_adjoint(args.particles, args.n_grid, args.steps, args.keep)
+    out = {"metric": "particle-substeps/s (fwd+adjoint)", "value": r["value"], "unit": r["unit"], "n_gpus": 1,
+           "steps": args.steps, "warmup": args.steps, "ms_per_step": r["ms_per_step"], "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": dict(r["config"], parallelism="single GPU")}
+    out.update({k: v for k, v in r.items() if k not in ("value", "unit", "ms_per_step", "config")})
     print(json.dumps(out), flush=True)
 
 
@@ -402,6 +468,8 @@ def main():
 
     sim.run(args.warmup, dt)
     torch.cuda.synchronize()
+    period = args.sort_period or 32
+    since0 = int(sim._bins["struct"].since_sort) if sim.use_bins and sim._bins is not None else 0
     t_start = torch.cuda.Event(enable_timing=True)
     t_end = torch.cuda.Event(enable_timing=True)
     t_start.record()
@@ -409,9 +477,8 @@ def main():
     t_end.record()
     torch.cuda.synchronize()
     ms_total = t_start.elapsed_time(t_end)
-    ms_step = ms_total / args.steps
-    value = n * args.steps / (ms_total / 1e3)
-    print(f"[bench] timed {args.steps} substeps: {ms_step:.4f} ms/substep", file=sys.stderr, flush=True)
+    ms_measured = ms_total / args.steps
+    print(f"[bench] timed {args.steps} substeps: {ms_measured:.4f} ms/substep", file=sys.stderr, flush=True)
 
     # per-kernel durations: HIP events on the launch stream around each kernel
     if args.unbinned:
@@ -425,17 +492,36 @@ def main():
                 dur[nm] += ev[i].elapsed_time(ev[i + 1])
         dur = {k: v / len(sim.stage_timer) for k, v in dur.items()}
         sim.stage_timer = None
+        sorts_in_window, sort_call_ms, ms_step = 0, 0.0, ms_measured
     else:
-        # events recorded by the library itself between its own launches
+        # events recorded by the library itself between its own launches, over
+        # a whole number of sort periods (so the sort's average is per call / period)
         names = ["grid_update", "g2p2g", "sort"]
-        gm, km, sm = sim.profile_kernels(args.stage_steps, dt)
+        n_prof = max(args.stage_steps, period)
+        n_prof = -(-n_prof // period) * period
+        gm, km, sm = sim.profile_kernels(n_prof, dt)
         dur = {"grid_update": gm, "g2p2g": km, "sort": sm}
+        # The bank-order sort runs every `period` substeps; a timed window of
+        # K substeps holds (since + K) // period of them.  The step time
+        # reported carries exactly K / period sorts: the measured window with
+        # its own sorts replaced by the sort's share.
+        sorts_in_window = (since0 + args.steps) // period
+        sort_call_ms = sm * n_prof / max(1, n_prof // period)
+        ms_step = (ms_total - sorts_in_window * sort_call_ms + args.steps / period * sort_call_ms) / args.steps
+    value = n / (ms_step / 1e3)
     dom = max((k for k in names if k in ALG_BYTES), key=lambda k: dur[k])
     achieved = ALG_BYTES[dom] * n / (dur[dom] / 1e3) / 1e9
     step_gbs = ALG_BYTES["step"] * n / (ms_step / 1e3) / 1e9
 
     u, v, Fd, tau = sim.export_uvFstress_from_warp()
     finite = bool(torch.isfinite(u).all() and torch.isfinite(tau).all())
+    del sim, u, v, Fd, tau
+    torch.cuda.empty_cache()
+
+    fwd_adj = None
+    if args.adjoint_steps > 0 and not args.unbinned:
+        print("[bench] forward + adjoint ...", file=sys.stderr, flush=True)
+        fwd_adj = measure_adjoint(n, scene.n_grid, args.adjoint_steps, "auto", fwd_ms=ms_step)
 
     cpu = None
     if args.cpu_seconds > 0:
@@ -467,6 +553,11 @@ def main():
                                                            and args.n_grid == 128) else None,
                      "bytes_per_particle": ALG_BYTES[dom],
                      "stage_ms": dur, "step_achieved_gbs": step_gbs, "step_frac": step_gbs / HBM_PEAK_GBS},
+        "timing": {"ms_per_step_measured": ms_measured, "sorts_in_window": sorts_in_window,
+                   "sort_ms_per_call": sort_call_ms, "sort_period": period,
+                   "note": "ms_per_step carries steps/sort_period bank-order sorts: the timed window with its own "
+                           "sorts replaced by that share"},
+        "fwd_adjoint": fwd_adj,
         "cpu_baseline": cpu,
         "finite": finite,
     }

@@ -1751,11 +1751,12 @@ int mpm_bins_profile(const MpmParams* prm, MpmBins* bins, const MpmCollider* col
         if (hipEventCreate(&ev[k]) != hipSuccess) return set_err(-27, "hipEventCreate failed");
     double t = time;
     float acc_grid = 0.f, acc_g2p2g = 0.f, acc_sort = 0.f;
+    const int period = bins->sort_period == 0 ? kSortPeriod : bins->sort_period;
     for (int k = 0; k < n_substeps && rc == 0; ++k) {
         (void)hipEventRecord(ev[0], s);
         rc = op_grid(c, bins, (float)t, dt, s);
         (void)hipEventRecord(ev[1], s);
-        if (rc == 0 && ++bins->since_sort >= kSortPeriod) rc = op_sort(c, bins, s);
+        if (rc == 0 && ++bins->since_sort >= period && period > 0) rc = op_sort(c, bins, s);
         (void)hipEventRecord(ev[2], s);
         if (rc == 0) rc = op_g2p2g(c, bins, dt, s, k == n_substeps - 1);
         (void)hipEventRecord(ev[3], s);
