@@ -164,6 +164,20 @@ int mpm_step(const MpmParams* prm, const MpmState* st, const MpmCollider* collid
              int n_colliders, void* workspace, int64_t workspace_bytes, float time,
              float dt, void* stream);
 
+/* ---- no-allocation substep ------------------------------------------------
+ *
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  mpm_step_noalloc is mpm_step with its workspace (the
+ * accumulator and velocity grids) in a static device array inside the
+ * library, sized for n_grid <= mpm_noalloc_max_grid() (64: 2 x 64^3 float4,
+ * 8 MiB per GPU, held until the process ends).  Larger grids are rejected
+ * with an error (use mpm_step).  It allocates nothing.  Every call on one GPU
+ * shares that GPU's array: calls that overlap on one GPU race. */
+int mpm_noalloc_max_grid(void);
+int mpm_step_noalloc(const MpmParams* prm, const MpmState* st, const MpmCollider* colliders, int n_colliders,
+                     float time, float dt, void* stream);
+
 /* ---- reduced-order sampled substep (CROM time stepper) ------------------
  *
  * MPM_Simulator_WARP.p2g2p_reduced (mpm_solver_warp.py:213-227) and the

@@ -15,7 +15,7 @@ INCLUDE = os.path.join(REPO_DIR, "include")
 LIB_NAME = "libmpm_hip.so"
 LIB_PATH = os.path.join(PKG_DIR, LIB_NAME)
 
-SOURCES = ["mpm_kernels.hip", "mpm_reduced.hip", "mpm_binned.hip", "mpm_ordered.hip", "mpm_adjoint.hip", "crom_mlp.hip", "crom_encoder.hip"]
+SOURCES = ["mpm_kernels.hip", "mpm_noalloc.hip", "mpm_reduced.hip", "mpm_binned.hip", "mpm_ordered.hip", "mpm_adjoint.hip", "crom_mlp.hip", "crom_encoder.hip"]
 HEADERS = ["mpm_common.h", "mpm_device.h", "mpm_stencil.h", "mpm_ordered.h"]
 
 # -packed-fp32-ops: keep hipcc from emitting v_pk_{add,mul,fma}_f32.

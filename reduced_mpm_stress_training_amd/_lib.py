@@ -78,6 +78,9 @@ _SIGNATURES = {
     "mpm_workspace_bytes": (C.c_int64, [C.POINTER(MpmParams), C.c_int64]),
     "mpm_step": (C.c_int, [C.POINTER(MpmParams), C.POINTER(MpmState), C.POINTER(MpmCollider), C.c_int,
                            C.c_void_p, C.c_int64, C.c_float, C.c_float, C.c_void_p]),
+    "mpm_noalloc_max_grid": (C.c_int, []),
+    "mpm_step_noalloc": (C.c_int, [C.POINTER(MpmParams), C.POINTER(MpmState), C.POINTER(MpmCollider), C.c_int,
+                                   C.c_float, C.c_float, C.c_void_p]),
     "mpm_reduced_workspace_bytes": (C.c_int64, [C.POINTER(MpmParams)]),
     "mpm_step_reduced": (C.c_int, [C.POINTER(MpmParams), C.POINTER(MpmState), C.c_void_p, C.c_int64, C.c_void_p,
                                    C.POINTER(MpmCollider), C.c_int, C.c_void_p, C.c_int64, C.c_float, C.c_float,
