@@ -1,8 +1,9 @@
 """Adjoint of the substep (mpm_adjoint_substep) against central finite
 differences of the FP64 CPU oracle's rollout.
 
-Loss L = mean_p |x_p(T) - x_target,p|^2 after 12 substeps of a 16^3 sand
-column with random initial velocities; gradients with respect to E, nu, the
+Loss L = mean_p |x_p(T) - x_target,p|^2 after 23 substeps of a 16^3 sand
+column with random initial velocities, checkpoints every 5 (five segments,
+the last one ragged); gradients with respect to E, nu, the
 friction angle and the initial velocity field (along a random direction).
 Central differences with a relative step of 1e-3 on the FP64 oracle; the
 GPU adjoint (FP32, differentiating the FP32 trajectory) must agree to 3e-2
@@ -19,7 +20,7 @@ from reduced_mpm_stress_training_amd import scenes as S
 
 pytestmark = pytest.mark.gpu
 
-N_STEPS, DT = 12, 1e-4
+N_STEPS, DT = 23, 1e-4
 
 
 def _scene():
@@ -150,3 +151,48 @@ def test_adjoint_matches_gpu_finite_differences_at_250k():
     print({"v0.D": (ad_v, fd_v), "E": (ad_E, fd_E)})
     assert abs(ad_v - fd_v) <= 5e-2 * abs(fd_v), (ad_v, fd_v)
     assert abs(ad_E - fd_E) <= 5e-2 * abs(fd_E), (ad_E, fd_E)
+
+
+@pytest.mark.parametrize("keep", ["all", "checkpoints"])
+def test_differentiable_rollout_configuration(keep):
+    """The differentiable-rollout configuration at its own length: 250k
+    particles on a 64^3 grid, 200 substeps with checkpoints every 20 (ten
+    segments, recorded or replayed), loss = mean |x - x_target|^2 against a
+    rollout at E = 3e5 Pa, friction angle 35 deg.  The adjoint's derivatives
+    with respect to E, nu, the friction angle and the initial velocities
+    (along a random field) against central differences of the GPU forward
+    (relative steps 1e-2, 2e-2 for the velocities), within 5%."""
+    sc = S.column_uniform(n_particles=250_000, n_grid=64, seed=4)
+    n = sc.n_particles
+    rng = np.random.default_rng(13)
+    ro = A.Rollout(sc, 200, 1e-4, checkpoint_every=20, keep=keep)
+    v0 = torch.from_numpy(sc.v0).cuda()
+    P0 = [2.0e5, 0.3, 30.0]
+    with torch.no_grad():
+        x_t = ro.forward(v0, 3.0e5, 0.3, 35.0).clone().double()
+
+    def loss(v, p):
+        with torch.no_grad():
+            x = ro.forward(v, *p)
+        return float(torch.mean(torch.sum((x.double() - x_t) ** 2, 1)))
+
+    v = v0.clone().requires_grad_(True)
+    params = torch.tensor(P0, dtype=torch.float64, requires_grad=True)
+    x = A.sand_rollout(v, params, ro)
+    torch.mean(torch.sum((x.double() - x_t) ** 2, 1)).backward()
+    assert ro._recorded == (keep == "all")
+    report = {}
+    for k, name in enumerate(("E", "nu", "phi")):
+        h = 1e-2 * P0[k]
+        pp, pm = list(P0), list(P0)
+        pp[k] += h
+        pm[k] -= h
+        report[name] = (float(params.grad[k]), (loss(v0, pp) - loss(v0, pm)) / (2 * h))
+    D = torch.from_numpy(rng.normal(size=(n, 3)).astype(np.float32)).cuda()
+    eps = 2e-2 * float(v0.abs().mean())
+    report["v0.D"] = (float(torch.sum(v.grad.double() * D.double())),
+                      (loss(v0 + eps * D, P0) - loss(v0 - eps * D, P0)) / (2 * eps))
+    print(keep, report)
+    for name, (ad, fd) in report.items():
+        assert np.isfinite(ad), name
+        assert abs(ad - fd) <= 5e-2 * abs(fd), (name, ad, fd)
