@@ -416,23 +416,33 @@ class Crom:
         return 1000.0 * torch.mean((y - q.reshape(B * P, 3)) ** 2)
 
 
+VOIGT = [0, 1, 2, 4, 5, 8]     # stress (N, 9) row-major -> [xx, xy, xz, yy, yz, zz] (mpm_solver_warp.py:383)
+
+
 def solver_snapshots(scene, friction_angle: float, n_frames: int, substeps_per_frame: int, dt: float,
-                     device="cuda:0") -> torch.Tensor:
-    """Displacement snapshots q = u (frames, N, 3) of one sand run by the MPM
-    solver (the training data the reference writes with save_data_at_frame,
-    mpm_solver_warp.py:312-404, generate_sand.py:27-32), frame 0 = initial."""
+                     device="cuda:0", n_points: int | None = None, with_stress: bool = False):
+    """Snapshots of one sand run by the MPM solver (the training data the
+    reference writes with save_data_at_frame, mpm_solver_warp.py:312-404,
+    generate_sand.py:27-32), frame 0 = initial: displacements q = u
+    (frames, P, 3) of the first P = n_points particles (default all), and with
+    with_stress also the Voigt stress (frames, P, 6)."""
     from . import scenes as S
     import copy
     sc = copy.copy(scene)
     sc.friction_angle = float(friction_angle)
     sim = S.make_simulator(sc, device=device)
-    out = torch.empty((n_frames, sc.n_particles, 3), dtype=torch.float32, device=torch.device(device))
+    P = int(n_points or sc.n_particles)
+    dev = torch.device(device)
+    out = torch.empty((n_frames, P, 3), dtype=torch.float32, device=dev)
+    st = torch.empty((n_frames, P, 6), dtype=torch.float32, device=dev) if with_stress else None
     for f in range(n_frames):
-        u, _, _, _ = sim.export_uvFstress_from_warp()
-        out[f] = u
+        u, _, _, tau = sim.export_uvFstress_from_warp()
+        out[f] = u[:P]
+        if with_stress:
+            st[f] = tau[:P].reshape(P, 9)[:, VOIGT]
         if f + 1 < n_frames:
             sim.run(substeps_per_frame, dt)
-    return out
+    return (out, st) if with_stress else out
 
 
 # ---------------------------------------------------------------------------
@@ -457,14 +467,22 @@ class ReducedRun:
     """A sand run on the CROM-reduced path: the MPM solver advances only the
     encoder's sample of particles (on a coarser grid); every frame the
     encoder turns the sample's displacements into a latent and the decoder
-    reconstructs the displacement of all N particles.  The full-order MPM
-    run is the check of its error and the fallback: with `verify=True` it
-    advances in lockstep, each frame's relative L2 error is recorded, and
-    from the first frame whose error exceeds `tol` the frames are the full
-    run's own.
+    reconstructs the displacement of all N particles.  The plain MPM step is
+    the fallback and the judge of the error, in one of two ways:
+
+    * verify=True: the full-order run advances in lockstep, each frame's
+      relative L2 error against it is recorded, and from the first frame
+      whose error exceeds `tol` the frames are the full run's own (the
+      measurement mode: it costs the full run);
+    * check_every=K (no lockstep run): every K frames the decoded
+      displacements of the sample points are compared with the sampled MPM
+      run's own; from the first check above `self_tol` the full-order run is
+      replayed from frame 0 to that frame once and continues in place of the
+      reduced one.  This catches the decoder leaving its training manifold;
+      the sampled run's own drift from the full run is what verify measures.
 
         rr = ReducedRun(crom, scene, friction_angle, n_sample=N // 10, coarse_grid=64)
-        out = rr.run(n_frames, substeps_per_frame, dt, verify=True, tol=0.15)
+        out = rr.run(n_frames, substeps_per_frame, dt, check_every=5, self_tol=0.1)
         out["frames"]  (n_frames, N, 3) displacements; out["source"] "crom" | "mpm" per frame
     """
 
@@ -493,13 +511,14 @@ class ReducedRun:
         return self.crom.decoder(xin)
 
     def run(self, n_frames: int, substeps_per_frame: int, dt: float, verify: bool = False,
-            tol: float | None = None) -> dict:
+            tol: float | None = None, check_every: int | None = None, self_tol: float | None = None) -> dict:
         coarse = self._sim(sampled_scene(self.scene, self.n_sample, self.coarse_grid))
         full = self._sim(self.scene) if verify else None
         frames = torch.empty((n_frames, self.X.shape[0], 3), dtype=torch.float32, device=self.device)
-        source, errors = [], []
+        source, errors, checks = [], [], []
         fallen = False
         t_reduced = t_full = 0.0
+        P = self.n_sample
         for f in range(n_frames):
             if f > 0:
                 torch.cuda.synchronize()
@@ -513,17 +532,31 @@ class ReducedRun:
                     torch.cuda.synchronize()
                 t2 = time.perf_counter()
                 t_reduced += t1 - t0
-                t_full += t2 - t1
-            u_true = full.export_uvFstress_from_warp()[0] if full is not None else None
+                if verify:
+                    t_full += t2 - t1
+                else:
+                    t_reduced += t2 - t1       # the fallen-back run's own steps
+            q = None
             if not fallen:
                 torch.cuda.synchronize()
                 t0 = time.perf_counter()
-                q = self.decode(coarse.export_uvFstress_from_warp()[0])
+                u_s = coarse.export_uvFstress_from_warp()[0]
+                q = self.decode(u_s)
+                if check_every and f > 0 and f % check_every == 0:
+                    den = float(u_s.norm())
+                    e = float((q[:P] - u_s).norm()) / den if den > 0 else 0.0
+                    checks.append((f, e))
+                    if self_tol is not None and e > self_tol and not verify:
+                        # the full-order run replayed to this frame, once
+                        fallen = True
+                        full = self._sim(self.scene)
+                        full.run(f * substeps_per_frame, dt)
                 torch.cuda.synchronize()
                 t_reduced += time.perf_counter() - t0
-            if u_true is not None:
+            u_true = full.export_uvFstress_from_warp()[0] if full is not None else None
+            if verify and u_true is not None:
                 den = float(u_true.norm())
-                err = float((q - u_true).norm()) / den if (den > 0 and not fallen) else 0.0
+                err = float((q - u_true).norm()) / den if (den > 0 and q is not None) else 0.0
                 errors.append(err)
                 if tol is not None and not fallen and err > tol:
                     fallen = True
@@ -533,5 +566,5 @@ class ReducedRun:
             else:
                 frames[f] = q
                 source.append("crom")
-        return {"frames": frames, "source": source, "errors": errors if verify else None,
+        return {"frames": frames, "source": source, "errors": errors if verify else None, "checks": checks,
                 "reduced_seconds": t_reduced, "full_seconds": t_full if verify else None}
