@@ -349,60 +349,89 @@ def bench_adjoint(args):
 
 
 def bench_crom(args):
-    """CROM on solver snapshots: sand columns of 1M particles (128^3, as the
-    headline) run by the MPM kernels at 4 random friction angles in
-    [20, 40] deg (seed 3), 40 frames of 10 substeps each; CROMnet (latent 16,
-    decoder 19 -> 128 x 5 -> 3, BF16 MFMA) trained on the first N/10 points of
-    each frame, batches of 10 frames; then a held-out run (another angle) is
-    encoded frame by frame and decoded at all 1M particles.  Reports the
-    decode throughput, its time against the MPM substeps it stands for, and
-    the relative L2 error of the reconstructed displacements.  Then the
-    CROM-reduced run of the held-out angle: MPM on the N/10 sample only
-    (half-resolution grid) + encoder + decoder per frame, checked frame by
-    frame against the full MPM run (ms per frame of both, error)."""
+    """CROM on solver snapshots: 2000 snapshots = 50 sand columns of 1M
+    particles (128^3, the headline column) run by the MPM kernels at friction
+    angles drawn from U[20, 40] deg (seed 3), 40 frames of 10 substeps each,
+    the first N/10 particles of every frame kept (SimulationDataset.py:36-43).
+    CROMnet (latent 16, decoder 19 -> 128 x 5 -> 3, BF16 MFMA) trains on the
+    displacements, batches of 10 frames; its encoder labels every frame; the
+    stress network [label, X, friction angle] -> 6 trains on the solver's
+    Voigt stress.  A held-out column (another angle) is encoded frame by
+    frame and both networks are evaluated at all 1M particles: the line's
+    value is the decoder's points/s, with the stress network's, and the
+    relative L2 errors of displacement and stress against the full MPM run.
+    Then the CROM-reduced run of the held-out angle: MPM on the N/10 sample
+    (half-resolution grid) + encoder + both networks per frame, (a) checked
+    frame by frame against the full run in lockstep, (b) on its own with the
+    self-check every 5 frames and the full-MPM fallback."""
     import torch
     from reduced_mpm_stress_training_amd import crom as CR
+    from reduced_mpm_stress_training_amd import crom_pipeline as CP
     from reduced_mpm_stress_training_amd import scenes as S
     rng = np.random.default_rng(3)
-    n, dt, spf, frames = args.particles, 1e-4, 10, 40
+    n, dt, spf, frames, n_runs = args.particles, 1e-4, 10, 40, 50
+    P = n // 10                                    # first N/10 points (SimulationDataset.py:36-43)
     sc = S.column_uniform(n_particles=n, n_grid=args.n_grid)
-    angles = [float(a) for a in rng.uniform(20.0, 40.0, size=5)]
+    angles = [float(a) for a in rng.uniform(20.0, 40.0, size=n_runs + 1)]
     t0 = time.perf_counter()
-    data = [CR.solver_snapshots(sc, a, frames, spf, dt) for a in angles]
+    q_tr = torch.empty((n_runs * frames, P, 3), device="cuda")
+    s_tr = torch.empty((n_runs * frames, P, 6), device="cuda")
+    for r in range(n_runs):
+        q, st = CR.solver_snapshots(sc, angles[r], frames, spf, dt, n_points=P, with_stress=True)
+        q_tr[r * frames:(r + 1) * frames], s_tr[r * frames:(r + 1) * frames] = q, st
+        del q, st
+        print(f"[bench] crom snapshots: run {r + 1}/{n_runs} (angle {angles[r]:.2f})", file=sys.stderr, flush=True)
+    held_q, held_s = CR.solver_snapshots(sc, angles[-1], frames, spf, dt, with_stress=True)
     torch.cuda.synchronize()
     t_data = time.perf_counter() - t0
-    P = n // 10                                    # first N/10 points (SimulationDataset.py:36-43)
+    mu_tr = torch.tensor([angles[r] for r in range(n_runs) for _ in range(frames)], device="cuda")
     X = torch.from_numpy(sc.x0).cuda()
-    train = torch.cat([d[:, :P] for d in data[:4]])         # (4 * frames, P, 3)
-    q_all = train.reshape(-1, 3)
+    qf = q_tr.reshape(-1, 3)
     net = CR.Crom(P, lbl=16, seed=0)
-    net.set_preprop(X[:P].mean(0), X[:P].std(0), q_all.mean(0), q_all.std(0).clamp_min(1e-12))
+    net.set_preprop(X[:P].mean(0), X[:P].std(0), qf.mean(0), qf.std(0).clamp_min(1e-12))
     xb = X[None, :P].expand(10, P, 3).contiguous()
     steps = args.steps
+    g = torch.Generator(device="cuda").manual_seed(0)
     t0 = time.perf_counter()
     for it in range(steps):
-        idx = torch.randint(0, train.shape[0], (10,), device="cuda")
-        lr = 1e-4 * CR.lr_factor(int(it * 750 / steps))
-        loss = net.train_step(train[idx], xb, lr)
+        idx = torch.randint(0, q_tr.shape[0], (10,), device="cuda", generator=g)
+        loss = net.train_step(q_tr[idx], xb, 1e-4 * CR.lr_factor(int(it * 750 / steps)))
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
-    # held-out run: encode each frame from its first N/10 points, decode all N
-    held = data[4]
-    err_num = err_den = 0.0
+    print(f"[bench] crom: displacement network trained ({steps} steps)", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    labels = CP.label_frames(net, q_tr)
+    head = CP.StressHead(16, seed=1)
+    head.set_preprop(X[:P], s_tr)
+    loss_s = CP.train_stress_head(head, labels, X[:P], mu_tr, s_tr, steps=steps)
     torch.cuda.synchronize()
-    t_dec = 0.0
+    t_train_s = time.perf_counter() - t0
+    print(f"[bench] crom: stress network trained ({steps} steps)", file=sys.stderr, flush=True)
+    # held-out column: encode each frame from its first N/10 points, evaluate
+    # both networks at all N particles
+    num = {"u": 0.0, "s": 0.0}
+    den = {"u": 0.0, "s": 0.0}
+    t_dec = t_str = 0.0
     for f in range(1, frames):
-        z = net.encoder(held[f:f + 1, :P])
+        z = CP.label_frames(net, held_q[f:f + 1, :P])
         xin = torch.cat([z.expand(n, 16), X], 1).contiguous()
+        sin = CP.StressHead.inputs(z, X, angles[-1])
         torch.cuda.synchronize()
         t1 = time.perf_counter()
         qh = net.decoder(xin)
         torch.cuda.synchronize()
-        t_dec += time.perf_counter() - t1
-        err_num += float(((qh - held[f]) ** 2).sum())
-        err_den += float((held[f] ** 2).sum())
-    rel = (err_num / err_den) ** 0.5
-    ms_dec = t_dec * 1e3 / (frames - 1)
+        t2 = time.perf_counter()
+        sh = head.net(sin)
+        torch.cuda.synchronize()
+        t_dec += t2 - t1
+        t_str += time.perf_counter() - t2
+        num["u"] += float(((qh - held_q[f]) ** 2).sum())
+        den["u"] += float((held_q[f] ** 2).sum())
+        num["s"] += float(((sh - held_s[f]) ** 2).sum())
+        den["s"] += float((held_s[f] ** 2).sum())
+    rel_u, rel_s = (num["u"] / den["u"]) ** 0.5, (num["s"] / den["s"]) ** 0.5
+    ms_dec, ms_str = t_dec * 1e3 / (frames - 1), t_str * 1e3 / (frames - 1)
+    del held_q, held_s, q_tr, s_tr
     # MPM time of one frame (10 substeps) on the same column
     sim = S.make_simulator(sc)
     sim.run(10, dt)
@@ -411,28 +440,43 @@ def bench_crom(args):
     sim.run(100, dt)
     torch.cuda.synchronize()
     ms_frame = (time.perf_counter() - t1) * 1e3 / 10
-    # the CROM-reduced run of the held-out angle: MPM on the encoder's N/10
-    # sample (half-resolution grid), decoder for all N, checked against the full run
-    rr = CR.ReducedRun(net, sc, angles[4], n_sample=P, coarse_grid=max(16, args.n_grid // 2))
+    del sim
+    # the CROM-reduced run of the held-out angle
+    coarse = max(16, args.n_grid // 2)
+    rr = CR.ReducedRun(net, sc, angles[-1], n_sample=P, coarse_grid=coarse, stress_head=head)
     rr.run(2, spf, dt)                       # warm-up of the sampled run's buffers
     red = rr.run(frames, spf, dt, verify=True)
-    errs = red["errors"][1:]
+    errs, serrs = red["errors"][1:], red["stress_errors"][1:]
+    solo = rr.run(frames, spf, dt, check_every=5, self_tol=0.1)
+    n_crom = solo["source"].count("crom")
     flop = 2 * (19 * 128 + 4 * 128 * 128 + 128 * 3)
+    flop_s = 2 * (20 * 128 + 4 * 128 * 128 + 128 * 6)
     out = {
         "metric": "CROM decoder points/s (inference at 1M particles)", "value": n / (ms_dec / 1e3),
         "unit": "points/s", "n_gpus": 1, "steps": steps, "warmup": 0, "ms_per_step": ms_dec,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16", "data": "synthetic",
-        "config": {"workload": "CROM decoder trained on solver snapshots", "n_particles": n, "latent": 16,
-                   "hidden": 128, "train_frames": 4 * frames, "points_per_frame": P, "train_steps": steps},
+        "config": {"workload": "CROM decoder + stress network trained on 2000 solver snapshots", "n_particles": n,
+                   "n_grid": args.n_grid, "latent": 16, "hidden": 128, "snapshot_runs": n_runs,
+                   "frames_per_run": frames, "substeps_per_frame": spf, "points_per_frame": P,
+                   "friction_angles": "U[20, 40] deg, seed 3", "train_steps": steps},
         "decoder_tflops": flop * n / (ms_dec / 1e3) / 1e12,
+        "stress_network": {"points_per_s": n / (ms_str / 1e3), "ms_at_1M": ms_str,
+                           "tflops": flop_s * n / (ms_str / 1e3) / 1e12},
         "mpm_ms_per_frame_10_substeps": ms_frame, "decode_speedup_over_mpm_frame": ms_frame / ms_dec,
-        "held_out_relative_l2_error": rel, "final_train_loss": float(loss),
-        "reduced_run": {"sample": P, "coarse_grid": max(16, args.n_grid // 2),
+        "held_out": {"angle": angles[-1], "displacement_relative_l2": rel_u, "stress_relative_l2": rel_s},
+        "final_train_loss": float(loss), "final_stress_loss": float(loss_s),
+        "reduced_run": {"sample": P, "coarse_grid": coarse,
                         "ms_per_frame": red["reduced_seconds"] * 1e3 / (frames - 1),
                         "full_mpm_ms_per_frame": red["full_seconds"] * 1e3 / (frames - 1),
                         "speedup": red["full_seconds"] / red["reduced_seconds"],
-                        "mean_relative_l2_error": float(np.mean(errs)), "max_relative_l2_error": float(np.max(errs))},
-        "train_seconds": t_train, "snapshot_seconds": t_data,
+                        "displacement_relative_l2_mean": float(np.mean(errs)),
+                        "displacement_relative_l2_max": float(np.max(errs)),
+                        "stress_relative_l2_mean": float(np.mean(serrs)),
+                        "stress_relative_l2_max": float(np.max(serrs))},
+        "reduced_run_self_checked": {"check_every": 5, "self_tol": 0.1, "frames_reduced": n_crom,
+                                     "frames_fallback": frames - n_crom, "checks": solo["checks"],
+                                     "ms_per_frame": solo["reduced_seconds"] * 1e3 / (frames - 1)},
+        "snapshot_seconds": t_data, "train_seconds": t_train, "stress_train_seconds": t_train_s,
     }
     print(json.dumps(out), flush=True)
 

@@ -484,11 +484,17 @@ class ReducedRun:
         rr = ReducedRun(crom, scene, friction_angle, n_sample=N // 10, coarse_grid=64)
         out = rr.run(n_frames, substeps_per_frame, dt, check_every=5, self_tol=0.1)
         out["frames"]  (n_frames, N, 3) displacements; out["source"] "crom" | "mpm" per frame
+
+    With a stress_head (crom_pipeline.StressHead) each reduced frame's latent
+    also gives the Voigt stress of all N particles (out["stress"]), and
+    verify records its relative L2 error against the full run's stress
+    (out["stress_errors"]).
     """
 
     def __init__(self, crom: Crom, scene, friction_angle: float, n_sample: int | None = None,
-                 coarse_grid: int | None = None, device="cuda:0"):
+                 coarse_grid: int | None = None, device="cuda:0", stress_head=None):
         self.crom, self.scene, self.phi = crom, scene, float(friction_angle)
+        self.stress_head = stress_head
         self.n_sample = int(n_sample or crom.npoints)
         if self.n_sample != crom.npoints:
             raise ValueError("n_sample must equal the encoder's number of points")
@@ -503,19 +509,24 @@ class ReducedRun:
         sc.friction_angle = self.phi
         return S.make_simulator(sc, device=self.device)
 
-    def decode(self, u_sample: torch.Tensor) -> torch.Tensor:
-        """Displacements of all N particles from the sample's (n_sample, 3)."""
+    def decode(self, u_sample: torch.Tensor, with_latent: bool = False):
+        """Displacements of all N particles from the sample's (n_sample, 3)
+        (and the latent (1, lbl) with with_latent)."""
         z = self.crom.encoder(u_sample[None].contiguous())
         n = self.X.shape[0]
         xin = torch.cat([z.expand(n, self.crom.lbl), self.X], 1).contiguous()
-        return self.crom.decoder(xin)
+        q = self.crom.decoder(xin)
+        return (q, z) if with_latent else q
 
     def run(self, n_frames: int, substeps_per_frame: int, dt: float, verify: bool = False,
             tol: float | None = None, check_every: int | None = None, self_tol: float | None = None) -> dict:
         coarse = self._sim(sampled_scene(self.scene, self.n_sample, self.coarse_grid))
         full = self._sim(self.scene) if verify else None
         frames = torch.empty((n_frames, self.X.shape[0], 3), dtype=torch.float32, device=self.device)
-        source, errors, checks = [], [], []
+        head = self.stress_head
+        stress = (torch.empty((n_frames, self.X.shape[0], 6), dtype=torch.float32, device=self.device)
+                  if head is not None else None)
+        source, errors, checks, stress_errors = [], [], [], []
         fallen = False
         t_reduced = t_full = 0.0
         P = self.n_sample
@@ -536,12 +547,14 @@ class ReducedRun:
                     t_full += t2 - t1
                 else:
                     t_reduced += t2 - t1       # the fallen-back run's own steps
-            q = None
+            q = s_hat = None
             if not fallen:
                 torch.cuda.synchronize()
                 t0 = time.perf_counter()
                 u_s = coarse.export_uvFstress_from_warp()[0]
-                q = self.decode(u_s)
+                q, z = self.decode(u_s, with_latent=True)
+                if head is not None:
+                    s_hat = head.predict(z, self.X, self.phi)[0]
                 if check_every and f > 0 and f % check_every == 0:
                     den = float(u_s.norm())
                     e = float((q[:P] - u_s).norm()) / den if den > 0 else 0.0
@@ -553,18 +566,29 @@ class ReducedRun:
                         full.run(f * substeps_per_frame, dt)
                 torch.cuda.synchronize()
                 t_reduced += time.perf_counter() - t0
-            u_true = full.export_uvFstress_from_warp()[0] if full is not None else None
+            u_true = tau_true = None
+            if full is not None:
+                u_true, _, _, tau_true = full.export_uvFstress_from_warp()
+                tau_true = tau_true.reshape(-1, 9)[:, VOIGT] if head is not None else None
             if verify and u_true is not None:
                 den = float(u_true.norm())
                 err = float((q - u_true).norm()) / den if (den > 0 and q is not None) else 0.0
                 errors.append(err)
+                if s_hat is not None:
+                    den = float(tau_true.norm())
+                    stress_errors.append(float((s_hat - tau_true).norm()) / den if den > 0 else 0.0)
                 if tol is not None and not fallen and err > tol:
                     fallen = True
             if fallen:
                 frames[f] = u_true
+                if stress is not None:
+                    stress[f] = tau_true
                 source.append("mpm")
             else:
                 frames[f] = q
+                if stress is not None:
+                    stress[f] = s_hat
                 source.append("crom")
-        return {"frames": frames, "source": source, "errors": errors if verify else None, "checks": checks,
+        return {"frames": frames, "stress": stress, "source": source, "errors": errors if verify else None,
+                "stress_errors": stress_errors if (verify and head is not None) else None, "checks": checks,
                 "reduced_seconds": t_reduced, "full_seconds": t_full if verify else None}
