@@ -42,7 +42,7 @@
 extern "C" {
 #endif
 
-#define MPM_ABI_VERSION 12
+#define MPM_ABI_VERSION 13
 
 /* plane indices */
 #define MPM_P_X0 0
@@ -260,10 +260,13 @@ typedef struct MpmBins {
 } MpmBins;
 
 /* Workspace of the binned path (zeroed once before first use, 256-byte
- * aligned): spill and velocity grids (n_grid^3 float4 each), spill flags,
+ * aligned): the spill grid (n_grid^3 x 4 doubles, FP64 atomics), the
+ * velocity grid (n_grid^3 float4: v, m after the grid update), spill flags,
  * 64 int32 status counters (0: particles dropped because bins and overflow
  * were full, 1: particles scattered outside their bin's tile, 2: particles
- * that left the grid; 48-49: the start time, as a double, of a captured run,
+ * that left the grid (held at their last position), 3: grid nodes whose
+ * P2G sum was NaN or Inf (a particle's state was not finite); all four
+ * accumulate until the caller clears them; 48-49: the start time, as a double, of a captured run,
  * mpm_batch_graph_launch), tile stamps, the fused kernel's schedule, nb^3
  * P2G tiles of 12^3 float4 and nb^3 FP64 tiles of 12^3 x 4 doubles for the
  * parts of split bins. */
@@ -271,6 +274,11 @@ int64_t mpm_bins_workspace_bytes(const MpmParams* prm);
 
 /* Byte offset of the status counters in the workspace. */
 int64_t mpm_bins_status_offset(const MpmParams* prm);
+
+/* Byte offset of the velocity grid (n_grid^3 float4: v after the grid
+ * update, m) in the workspace: the reference's grid_v_out and grid_m
+ * (mpm_solver_warp.py:65-67) of the last substep. */
+int64_t mpm_bins_velocity_offset(const MpmParams* prm);
 
 /* Clears count[cur] and inserts every particle of a flat state (slot =
  * caller index, as MpmState) into the bins of plane set `cur`. */

@@ -24,7 +24,7 @@
 // current epoch, so nothing needs clearing between substeps (zero_grid,
 // mpm_utils.py:115-123, becomes the epoch bump).  Particles whose next
 // stencil leaves their bin's tile (or that sit in the overflow run) scatter
-// with float atomics into a spill grid that the grid update adds and clears.
+// with FP64 atomics into a spill grid that the grid update adds and clears.
 // The P2G of the first substep (or after the state is replaced) is
 // k_p2g_bins.  So between substeps the tiles hold the next substep's P2G,
 // computed with that substep's dt.
@@ -357,16 +357,19 @@ __device__ __forceinline__ void mark_block(int* __restrict__ gflag, int nb, int 
     gflag[((x >> 3) * nb + (y >> 3)) * nb + (z >> 3)] = 1;
 }
 
-// Scatter of one particle straight into the spill grid with float atomics
-// (overflow particles and particles that left their bin's scatter tile).
-__device__ void scatter_spill(float* __restrict__ acc, int* __restrict__ gflag, const SceneDesc& sc,
+// Scatter of one particle straight into the spill grid with FP64 atomics
+// (overflow particles and particles that left their bin's scatter tile): the
+// FP32 contributions sum in FP64 as in the LDS tiles, so the node's value
+// does not depend on the order the atomics land in unless the sum needs more
+// than 53 bits.
+__device__ void scatter_spill(double* __restrict__ acc, int* __restrict__ gflag, const SceneDesc& sc,
                               const AxisWeights& A, float m, float vol, const float v[3], const float C[9],
                               const float tau[9], float dt) {
     const int ng = sc.prm.n_grid;
-    float* sacc = acc + 4 * sc.node_base;
+    double* sacc = acc + 4 * sc.node_base;
     p2g_contrib(A, m, vol, v, C, tau, sc.prm.dx, sc.prm.inv_dx, dt, [&](int c, int i, int j, int k, float val) {
         const int x = A.base[0] + i, y = A.base[1] + j, z = A.base[2] + k;
-        unsafeAtomicAdd(sacc + 4 * (((int64_t)x * ng + y) * ng + z) + c, val);
+        unsafeAtomicAdd(sacc + 4 * (((int64_t)x * ng + y) * ng + z) + c, (double)val);
     });
 #pragma unroll
     for (int i = 0; i < 2; ++i)
@@ -409,7 +412,7 @@ struct SubstepArgs {
     const int* __restrict__ cnt_cur;
     int* __restrict__ cnt_nxt;
     const float4* __restrict__ vel;
-    float* __restrict__ acc;
+    double* __restrict__ acc;
     int* __restrict__ gflag;
     int* __restrict__ status;
     float4* __restrict__ tiles;
@@ -643,15 +646,15 @@ __device__ __forceinline__ void g2p2g_particle(const SubstepArgs& a, const Scene
 // block b.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kWG)
-    k_grid_bins(float4* __restrict__ acc, float4* __restrict__ vel, int* __restrict__ gflag,
+    k_grid_bins(double4* __restrict__ acc, float4* __restrict__ vel, int* __restrict__ gflag,
                 const int* __restrict__ stamps, const float4* __restrict__ tiles, const int* __restrict__ pcount,
                 const int* __restrict__ pslot, const double* __restrict__ ptiles, int* __restrict__ cnt_nxt,
                 const int* __restrict__ cnt_cur, int* __restrict__ order, BinGeom g, SceneTab tab,
                 ColliderSet cs, float time, float dt, int epoch, int clear_only, const double* __restrict__ t_base,
-                float4* __restrict__ rec) {
+                float4* __restrict__ rec, int* __restrict__ status) {
     __shared__ int s_live[27];             // neighbour bin c has a whole-bin tile of this epoch
     __shared__ int s_pslot[kParts * 27];   // FP64 tile slot of part p of neighbour bin c this epoch, or -1
-    __shared__ int s_any, s_anyp;
+    __shared__ int s_any, s_anyp, s_bad;
     if (blockIdx.x == 0) {
         if (order) schedule_bins(cnt_cur, g, epoch, order);
         return;
@@ -677,7 +680,7 @@ __global__ void __launch_bounds__(kWG)
         cnt_nxt[g.n_bins + 1 + blk] = 0;  // live particles
         if (blk == 0) cnt_nxt[g.n_bins] = 0;
     }
-    if (tid == 0) s_any = s_anyp = 0;
+    if (tid == 0) s_any = s_anyp = s_bad = 0;
     __syncthreads();
     if (tid < 27) {
         const int c = tid;
@@ -710,7 +713,7 @@ __global__ void __launch_bounds__(kWG)
             const int x = bx * 8 + (t >> 6), y = by * 8 + ((t >> 3) & 7), z = bz * 8 + (t & 7);
             if (x >= ng || y >= ng || z >= ng) continue;
             const int64_t node = ((int64_t)x * ng + y) * ng + z;
-            if (spill) acc[node] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (spill) acc[node] = make_double4(0.0, 0.0, 0.0, 0.0);
             if (written) vel[node] = make_float4(0.f, 0.f, 0.f, 0.f);
         }
         __syncthreads();
@@ -723,9 +726,9 @@ __global__ void __launch_bounds__(kWG)
         const int64_t node = ((int64_t)x * ng + y) * ng + z;
         double sx = 0.0, sy = 0.0, sz = 0.0, sm = 0.0;
         if (spill) {
-            const float4 a4 = acc[node];
+            const double4 a4 = acc[node];
             sx = a4.x; sy = a4.y; sz = a4.z; sm = a4.w;
-            acc[node] = make_float4(0.f, 0.f, 0.f, 0.f);
+            acc[node] = make_double4(0.0, 0.0, 0.0, 0.0);
         }
         const int p[3] = {x, y, z};
         int hi[3], nlo[3];
@@ -806,10 +809,15 @@ __global__ void __launch_bounds__(kWG)
         }
         const float4 a4 = make_float4((float)sx, (float)sy, (float)sz, (float)sm);
         if (rec) rec[sc.node_base + node] = a4;   // grid_v_in and grid_m of the reference
+        // a NaN or Inf in any particle's state reaches its nodes' P2G sums
+        if (!isfinite(a4.x + a4.y + a4.z + a4.w)) atomicAdd(&s_bad, 1);
         vel[node] = grid_node_update(a4, x, y, z, ng, sc.prm.dx, dt, sc.prm.gravity, cols, n_cols, tnow);
     }
     __syncthreads();
-    if (tid == 0) gflag[blk] = 2;
+    if (tid == 0) {
+        gflag[blk] = 2;
+        if (s_bad) atomicAdd(&status[3], s_bad);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -963,7 +971,7 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2, 2)))
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kWG)
     k_p2g_bins(SceneTab tab, BinGeom g, float dt, const float* __restrict__ planes, const int* __restrict__ cnt,
-               float* __restrict__ acc, int* __restrict__ gflag, float4* __restrict__ tiles,
+               double* __restrict__ acc, int* __restrict__ gflag, float4* __restrict__ tiles,
                int* __restrict__ stamps, int epoch_out) {
     __shared__ double s_acc[4 * kTS3];
     const int tid = threadIdx.x;
@@ -1239,7 +1247,8 @@ namespace {
 
 struct WsLayout {
     int64_t off_status;
-    float4* acc;       // spill grid (all scenes' nodes)
+    int64_t off_vel;
+    double4* acc;      // spill grid (all scenes' nodes), FP64
     float4* vel;       // grid velocities (v, m)
     int* gflag;        // spill flags per grid block
     int* status;       // 64 counters
@@ -1275,8 +1284,9 @@ WsLayout ws_layout(int64_t nodes, int64_t nbins, int n_scenes, int n_cols, void*
     char* p = (char*)base;
     WsLayout w;
     int64_t off = 0;
-    w.acc = (float4*)(p + off);
-    off += align256(nodes * 16);
+    w.acc = (double4*)(p + off);
+    off += align256(nodes * 32);
+    w.off_vel = off;
     w.vel = (float4*)(p + off);
     off += align256(nodes * 16);
     w.gflag = (int*)(p + off);
@@ -1432,7 +1442,7 @@ int op_clear(const Ctx& c, MpmBins* bins, hipStream_t s) {
     hipLaunchKernelGGL(k_grid_bins, dim3(c.g.n_bins + 1), dim3(kWG), 0, s, c.w.acc, c.w.vel, c.w.gflag,
                        (const int*)c.w.stamps, (const float4*)c.w.tiles, (const int*)c.w.pcount,
                        (const int*)c.w.pslot, (const double*)c.w.ptiles, (int*)nullptr, (const int*)nullptr, (int*)nullptr, c.g, c.tab,
-                       c.cs, 0.f, 0.f, bins->epoch, 1, (const double*)nullptr, (float4*)nullptr);
+                       c.cs, 0.f, 0.f, bins->epoch, 1, (const double*)nullptr, (float4*)nullptr, c.w.status);
     int rc = check_launch("k_grid_bins(clear)");
     if (rc) return rc;
     bins->epoch += 1;  // the pending tiles are no longer current
@@ -1442,7 +1452,7 @@ int op_clear(const Ctx& c, MpmBins* bins, hipStream_t s) {
 int op_p2g(const Ctx& c, MpmBins* bins, float dt, hipStream_t s) {
     const int epoch_out = bins->epoch + 1;
     hipLaunchKernelGGL(k_p2g_bins, dim3(c.g.n_bins + ovf_wgs(c.g)), dim3(kWG), 0, s, c.tab, c.g, dt,
-                       bins->planes[bins->cur], (const int*)counts(bins, c.g, bins->cur), (float*)c.w.acc, c.w.gflag,
+                       bins->planes[bins->cur], (const int*)counts(bins, c.g, bins->cur), (double*)c.w.acc, c.w.gflag,
                        c.w.tiles, c.w.stamps, epoch_out);
     int rc = check_launch("k_p2g_bins");
     if (rc) return rc;
@@ -1465,7 +1475,7 @@ int op_grid(const Ctx& c, const MpmBins* bins, float time, float dt, hipStream_t
                        (const int*)c.w.stamps, (const float4*)c.w.tiles, (const int*)c.w.pcount,
                        (const int*)c.w.pslot, (const double*)c.w.ptiles, counts(bins, c.g, 1 - bins->cur),
                        (const int*)counts(bins, c.g, bins->cur), c.w.order, c.g, c.tab, c.cs, time, dt, bins->epoch,
-                       0, t_base, rec);
+                       0, t_base, rec, c.w.status);
     return check_launch("k_grid_bins");
 }
 
@@ -1483,7 +1493,7 @@ int op_g2p2g(const Ctx& c, MpmBins* bins, float dt, hipStream_t s, bool store_al
     a.cnt_cur = counts(bins, c.g, cur);
     a.cnt_nxt = counts(bins, c.g, nxt);
     a.vel = c.w.vel;
-    a.acc = (float*)c.w.acc;
+    a.acc = (double*)c.w.acc;
     a.gflag = c.w.gflag;
     a.status = c.w.status;
     a.tiles = c.w.tiles;
@@ -1659,6 +1669,12 @@ int64_t mpm_bins_status_offset(const MpmParams* prm) {
     if (check_grid(prm)) return -1;
     const int64_t ng = prm->n_grid;
     return ws_layout(ng * ng * ng, bins_of_grid(prm->n_grid), 0, 0, nullptr).off_status;
+}
+
+int64_t mpm_bins_velocity_offset(const MpmParams* prm) {
+    if (check_grid(prm)) return -1;
+    const int64_t ng = prm->n_grid;
+    return ws_layout(ng * ng * ng, bins_of_grid(prm->n_grid), 0, 0, nullptr).off_vel;
 }
 
 int mpm_bins_insert(const MpmParams* prm, const MpmState* flat, const MpmBins* bins, void* workspace,

@@ -145,6 +145,7 @@ class MPM_Simulator_HIP:
         self._fdisp_loaded = None        # F_disp of a sampling file, exported until the first substep
         self._pending_dt = None      # dt of the P2G held in the accumulator
         self._since_check = 0
+        self._left_warned = 0        # particles that had left the grid at the last health check
         # reduced-order sampled step (p2g2p_reduced): sample indices and its
         # own workspace (grids + neighboring_cells flag grids)
         self.sample_particle_index_short = torch.zeros(1, dtype=torch.int32, device=self.device)
@@ -546,10 +547,41 @@ class MPM_Simulator_HIP:
         start = off + int(L.load().mpm_bins_status_offset(self._params()))
         return self._bins["ws"][start:start + 256].view(torch.int32).cpu()
 
+    def health(self) -> dict:
+        """The binned path's device counters (a host sync): particles dropped
+        because bins and overflow were full, particles whose scatter left
+        their bin's tile (the spill path), particles that left the grid (held
+        at their last position: the reference indexes past the grid there),
+        and grid nodes whose P2G sum was NaN or Inf -- these accumulate over
+        the run -- and the particles in the overflow run now (bins past their
+        capacity: which particles land there depends on the order of the
+        atomics, so two runs agree bit for bit only while it stays empty).
+        Zeros before the first binned substep."""
+        keys = ("dropped", "outside_tile", "left_grid", "nonfinite_nodes")
+        if self._bins is None:
+            return dict.fromkeys(keys + ("overflow",), 0)
+        st = self._bins_status()
+        out = {k: int(st[i]) for i, k in enumerate(keys)}
+        B = self._bins
+        out["overflow"] = min(int(B["count"][B["struct"].cur, B["nb"] ** 3]), B["ovf"])
+        return out
+
     def _check_bins_health(self):
+        """Run every 256 substeps and at every export: raises on dropped
+        particles and on NaN/Inf in the grid (FloatingPointError), warns once
+        per increase of the particles that left the grid."""
         st = self._bins_status()
         if int(st[0]) > 0:
             raise RuntimeError(f"{int(st[0])} particles were dropped: bins and overflow run full")
+        if int(st[3]) > 0:
+            raise FloatingPointError(f"{int(st[3])} grid node sums were NaN or Inf: a particle's state is not "
+                                     "finite (check dt against dx / max|v|)")
+        left = int(st[2])
+        if left > self._left_warned:
+            import warnings
+            warnings.warn(f"{left} particles left the grid and are held at their last position", RuntimeWarning,
+                          stacklevel=3)
+        self._left_warned = left
         cnt = self._bins["count"][self._bins["struct"].cur, self._bins["nb"] ** 3].item()
         if cnt > self._bins["ovf"] // 2:
             # re-bin with larger bins at the next substep
@@ -770,7 +802,7 @@ class MPM_Simulator_HIP:
         if self._loc == "bins" and self._bins is not None:
             ws = self._bins["ws"]
             off = self._bins_ws_ptr() - ws.data_ptr()
-            vel_off = ((nn * 16 + 255) // 256) * 256      # workspace layout of the binned path
+            vel_off = int(L.load().mpm_bins_velocity_offset(self._params()))
             vel = ws[off + vel_off:off + vel_off + nn * 16].view(torch.float32).view(ng, ng, ng, 4)
             return self._record, vel
         else:

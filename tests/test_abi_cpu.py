@@ -127,7 +127,8 @@ def test_bins_reject_oversized_bins_and_size_the_part_tiles():
     (the pointers here are never dereferenced): more than 16e6 slots per bin
     is refused (the fused kernel's split arithmetic counts passes of 512 in
     15 bits).  The workspace holds, per bin, a float4 tile and an FP64 tile
-    of 12^3 nodes (the latter for the parts of split bins)."""
+    of 12^3 nodes (the latter for the parts of split bins), after the FP64
+    spill grid and the velocity grid."""
     lib = L.load()
     prm = L.MpmParams()
     prm.n_grid, prm.material, prm.dx, prm.inv_dx = 32, 2, 1.0 / 32, 32.0
@@ -139,4 +140,8 @@ def test_bins_reject_oversized_bins_and_size_the_part_tiles():
     bins.planes[0], bins.planes[1], bins.count = 0x1000000, 0x2000000, 0x3000000
     assert lib.mpm_bins_g2p2g(prm, bins, 0x4000000, 1e-4, None) < 0
     assert b"16e6 slots per bin" in lib.mpm_last_error()
-    assert lib.mpm_bins_workspace_bytes(prm) >= nb ** 3 * 12 ** 3 * (16 + 32) + 2 * 32 ** 3 * 16
+    # the spill grid (4 doubles per node) first, then the velocity grid (float4)
+    nodes = 32 ** 3
+    assert lib.mpm_bins_velocity_offset(prm) == nodes * 32
+    assert lib.mpm_bins_status_offset(prm) >= nodes * 32 + nodes * 16
+    assert lib.mpm_bins_workspace_bytes(prm) >= nb ** 3 * 12 ** 3 * (16 + 32) + nodes * (32 + 16)
