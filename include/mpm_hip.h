@@ -285,7 +285,10 @@ int64_t mpm_bins_velocity_offset(const MpmParams* prm);
 int mpm_bins_insert(const MpmParams* prm, const MpmState* flat, const MpmBins* bins, void* workspace,
                     void* stream);
 
-/* flat[pid] <- every particle of plane set `cur` (all planes). */
+/* flat[pid] <- every particle of plane set `cur` (all planes): an inverse
+ * slot map (8 bytes per particle, written into the other plane set, whose
+ * contents are dead between the bins' operations), then a gather per
+ * particle with coalesced stores into flat. */
 int mpm_bins_extract(const MpmParams* prm, const MpmBins* bins, const MpmState* flat, void* stream);
 
 /* mpm_bins_extract of the planes whose bits are set in plane_mask (bit k =

@@ -1180,9 +1180,22 @@ __global__ void k_fill_int(int* __restrict__ p, int n, int v) {
     if (i < n) p[i] = v;
 }
 
-// Insert every particle of a flat state (slot = caller index; in a batch
-// the scenes' particles follow each other) into the bins of plane set `cur`
-// by the base cell of x = x0 + u.
+__global__ void k_fill_i64(int64_t* __restrict__ p, int n, int64_t v) {
+    const int i = blockIdx.x * kWG + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// Moving a state between the caller's order (flat, slot = caller index) and
+// the bins is a permutation of 40 planes; bins hold their particles in no
+// pid order, so one side of it is random.
+//
+// Insert: every particle of a flat state (in a batch the scenes' particles
+// follow each other) takes a slot in the bin of its base cell (x = x0 + u)
+// in plane set `planes` and scatters its planes there.  (A gather instead --
+// slots first, then every slot reading its particle's planes -- measured
+// slower at 1M: 1.09 ms against 0.69 ms, its per-bin workgroups too few to
+// hide the random reads; the per-bin slot counters' atomics take ~0.24 ms
+// of either.)
 __global__ void __launch_bounds__(kWG)
     k_bins_insert(const float* __restrict__ flat, int64_t fstride, int n, SceneTab tab, BinGeom g,
                   float* __restrict__ planes, int* __restrict__ cnt, int* __restrict__ status) {
@@ -1210,33 +1223,85 @@ __global__ void __launch_bounds__(kWG)
         off = ovf_off(g, r);
         S = g.ovf_blk;
     }
-    for (int k = 0; k < MPM_NPLANES; ++k) planes[off + (int64_t)k * S] = flat[k * fstride + p];
+#pragma unroll
+    for (int k0 = 0; k0 < MPM_NPLANES; k0 += 8) {
+        float v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = flat[(int64_t)(k0 + j) * fstride + p];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) planes[off + (int64_t)(k0 + j) * S] = v[j];
+    }
 }
 
-// flat[pid] <- every valid slot of plane set `cur`
-__global__ void __launch_bounds__(kWG)
-    k_bins_extract(const float* __restrict__ planes, const int* __restrict__ cnt, BinGeom g, SceneTab tab,
-                   float* __restrict__ flat, int64_t fstride, int n, uint64_t plane_mask) {
-    const int wg = blockIdx.x;
-    const bool ovf = wg >= g.n_bins;
+// The filled slots of workgroup wg: slots [0, n_in) of bin wg, or, for
+// wg >= n_bins, overflow slots r0 + [0, n_in) with r0 = (wg - n_bins) * kWG
+// (they may straddle two overflow blocks).  slot_at gives slot i's float
+// offset (plane 0); S is the plane stride.
+struct BlockSlots {
     int n_in;
-    SceneDesc sc = tab.one;
-    if (ovf) {
-        n_in = min(max(min(cnt[g.n_bins], g.ovf_cap) - (wg - g.n_bins) * kWG, 0), kWG);
-    } else {
-        sc = scene_of_bin(tab, wg);
-        n_in = min(cnt[wg], sc.cap);
+    bool ovf;
+    int64_t base;   // bin: offset of slot 0; overflow: r0
+    int64_t S;
+    __device__ __forceinline__ int64_t slot_at(const BinGeom& g, int i) const {
+        return ovf ? ovf_off(g, (int)base + i) : base + i;
     }
-    const int64_t S = ovf ? g.ovf_blk : sc.cap;
-    for (int i = threadIdx.x; i < n_in; i += kWG) {
-        const int64_t off = ovf ? ovf_off(g, (wg - g.n_bins) * kWG + i) : bin_off(sc, wg) + i;
-        const int pid = __float_as_int(planes[off + (int64_t)MPM_P_PID * S]);
-        if (pid < 0 || pid >= n) continue;
-        // yield_stress is carried by metal scenes only
-        const bool metal = (ovf ? scene_of_pid(tab, pid) : sc).prm.material == 1;
-        for (int k = 0; k < MPM_NPLANES; ++k)
-            if (((plane_mask >> k) & 1) && (k != MPM_P_YIELD || metal))
-                flat[k * fstride + pid] = planes[off + (int64_t)k * S];
+};
+
+__device__ __forceinline__ BlockSlots block_slots(const int* __restrict__ cnt, const BinGeom& g,
+                                                  const SceneTab& tab, int wg) {
+    BlockSlots b;
+    b.ovf = wg >= g.n_bins;
+    if (b.ovf) {
+        const int r0 = (wg - g.n_bins) * kWG;
+        b.base = r0;
+        b.S = g.ovf_blk;
+        b.n_in = min(max(min(cnt[g.n_bins], g.ovf_cap) - r0, 0), kWG);
+    } else {
+        const SceneDesc sc = scene_of_bin(tab, wg);
+        b.base = bin_off(sc, wg);
+        b.S = sc.cap;
+        b.n_in = min(cnt[wg], sc.cap);
+    }
+    return b;
+}
+
+// Extract, pass 1: the inverse slot map of plane set `planes`: map[pid] =
+// float offset (plane 0) of the particle's slot, for every filled slot
+// (the caller fills map with -1: particles not in the bins keep it).
+__global__ void __launch_bounds__(kWG)
+    k_bins_slotmap(const float* __restrict__ planes, const int* __restrict__ cnt, BinGeom g, SceneTab tab,
+                   int64_t* __restrict__ map, int n) {
+    const BlockSlots bs = block_slots(cnt, g, tab, blockIdx.x);
+    for (int i = threadIdx.x; i < bs.n_in; i += kWG) {
+        const int64_t off = bs.slot_at(g, i);
+        const int pid = __float_as_int(planes[off + (int64_t)MPM_P_PID * bs.S]);
+        if (pid >= 0 && pid < n) map[pid] = off;
+    }
+}
+
+// Extract, pass 2: flat[k][pid] <- plane k of the particle's slot, for the
+// planes of plane_mask (yield_stress only in metal scenes: the other
+// materials do not carry it).  blockIdx.y takes planes 8y .. 8y + 7, so the
+// workgroups in flight read a few planes' lines, which the caches then hold
+// while the other particles of those lines come by.
+__global__ void __launch_bounds__(kWG)
+    k_bins_gather(const float* __restrict__ planes, const int64_t* __restrict__ map, BinGeom g, SceneTab tab,
+                  float* __restrict__ flat, int64_t fstride, int n, uint64_t plane_mask) {
+    const int pid = blockIdx.x * kWG + threadIdx.x;
+    const int k0 = 8 * blockIdx.y;
+    if (pid >= n || ((plane_mask >> k0) & 0xffull) == 0) return;
+    const int64_t off = map[pid];
+    if (off < 0) return;
+    const SceneDesc sc = scene_of_pid(tab, pid);
+    const int64_t S = off >= g.ovf_off ? g.ovf_blk : sc.cap;
+    const uint64_t want = plane_mask & (sc.prm.material == 1 ? ~0ull : ~(1ull << MPM_P_YIELD));
+    {
+        float v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = ((want >> (k0 + j)) & 1) ? planes[off + (int64_t)(k0 + j) * S] : 0.f;
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            if ((want >> (k0 + j)) & 1) flat[(int64_t)(k0 + j) * fstride + pid] = v[j];
     }
 }
 
@@ -1432,10 +1497,23 @@ int op_insert(const Ctx& c, const MpmState* flat, const MpmBins* bins, hipStream
 int op_extract(const Ctx& c, const MpmBins* bins, const MpmState* flat, hipStream_t s,
                uint64_t plane_mask = ~0ull) {
     if (!flat || !flat->planes || flat->n < 0 || flat->stride < flat->n) return set_err(-25, "flat state invalid");
-    hipLaunchKernelGGL(k_bins_extract, dim3(c.g.n_bins + ovf_wgs(c.g)), dim3(kWG), 0, s, bins->planes[bins->cur],
-                       (const int*)counts(bins, c.g, bins->cur), c.g, c.tab, flat->planes, flat->stride,
-                       (int)flat->n, plane_mask);
-    return check_launch("k_bins_extract");
+    if (flat->n > 0x7fffffffLL) return set_err(-6, "more than 2^31-1 particles");
+    if (flat->n == 0 || plane_mask == 0) return 0;
+    // the inverse slot map lives in the other plane set, whose contents are
+    // dead between operations (every operation that reads it writes it first)
+    if (2 * flat->n > bins->stride) return set_err(-28, "bins: plane set too small for the slot map");
+    int64_t* map = (int64_t*)bins->planes[1 - bins->cur];
+    const int n = (int)flat->n;
+    hipLaunchKernelGGL(k_fill_i64, dim3((unsigned)((n + kWG - 1) / kWG)), dim3(kWG), 0, s, map, n, (int64_t)-1);
+    int rc = check_launch("k_fill_i64");
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_bins_slotmap, dim3(c.g.n_bins + ovf_wgs(c.g)), dim3(kWG), 0, s, bins->planes[bins->cur],
+                       (const int*)counts(bins, c.g, bins->cur), c.g, c.tab, map, n);
+    if ((rc = check_launch("k_bins_slotmap"))) return rc;
+    hipLaunchKernelGGL(k_bins_gather, dim3((unsigned)((n + kWG - 1) / kWG), MPM_NPLANES / 8), dim3(kWG), 0, s,
+                       (const float*)bins->planes[bins->cur], (const int64_t*)map, c.g, c.tab, flat->planes,
+                       flat->stride, n, plane_mask);
+    return check_launch("k_bins_gather");
 }
 
 int op_clear(const Ctx& c, MpmBins* bins, hipStream_t s) {
@@ -1772,7 +1850,8 @@ int mpm_bins_profile(const MpmParams* prm, MpmBins* bins, const MpmCollider* col
         (void)hipEventRecord(ev[0], s);
         rc = op_grid(c, bins, (float)t, dt, s);
         (void)hipEventRecord(ev[1], s);
-        if (rc == 0 && ++bins->since_sort >= period && period > 0) rc = op_sort(c, bins, s);
+        const bool sorted = rc == 0 && ++bins->since_sort >= period && period > 0;
+        if (sorted) rc = op_sort(c, bins, s);
         (void)hipEventRecord(ev[2], s);
         if (rc == 0) rc = op_g2p2g(c, bins, dt, s, k == n_substeps - 1);
         (void)hipEventRecord(ev[3], s);
@@ -1783,7 +1862,9 @@ int mpm_bins_profile(const MpmParams* prm, MpmBins* bins, const MpmCollider* col
         (void)hipEventElapsedTime(&b, ev[2], ev[3]);
         acc_grid += a;
         acc_g2p2g += b;
-        acc_sort += cc;
+        // the window of a substep without a sort holds only the two events'
+        // own overhead (~5 us), which is not the sort's
+        if (sorted) acc_sort += cc;
         t = t + (double)dt;
     }
     for (int k = 0; k < 4; ++k) (void)hipEventDestroy(ev[k]);
