@@ -21,7 +21,17 @@ Deviations from the reference (each deliberate):
     raises RuntimeError; a missing sampling file raises FileNotFoundError
     instead of exit() (:130-132);
   * snapshots are written as .npz with the reference's h5 dataset names when
-    h5py is not importable.
+    h5py is not importable;
+  * load_from_sampling keeps the file's f_tensor as the exported F_disp until
+    the first substep (the reference stores it in particle_F_disp, :146,
+    while F starts as I);
+  * failures surface: particles that leave the grid are held at their last
+    position and counted (the reference indexes past its grid), NaN/Inf in
+    the grid sums raise FloatingPointError at the next export or health
+    check, and health() returns the device counters;
+  * set_rotation_on_disk raises ValueError when |normal_x| <= 0.1 (the
+    reference's axis search never ends there, :611-614), and a node on the
+    disk's own axis gets v = 0 (the reference divides 0 by 0).
 """
 from __future__ import annotations
 
