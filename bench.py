@@ -78,7 +78,7 @@ def parse():
                     help="column: the 1M-particle headline; batch: 64 uneven scenes advanced together; "
                          "adjoint: forward + adjoint of a differentiable rollout")
     ap.add_argument("--scenes", type=int, default=64)
-    ap.add_argument("--wg-threads", type=int, default=0, choices=[0, 256, 512],
+    ap.add_argument("--wg-threads", type=int, default=0, choices=[0, 256, 512, 768],
                     help="threads per bin workgroup of the fused kernel (0 = library default)")
     ap.add_argument("--sort-period", type=int, default=0, help="substeps between bank-order sorts (0 = default)")
     ap.add_argument("--no-autotune", action="store_true",
@@ -587,7 +587,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "sand column collapse, forward substep", "n_particles": n,
                    "path": "unbinned stage kernels" if args.unbinned else "binned G2P2G",
-                   "launch": {"wg_threads": args.wg_threads or 512, "sort_period": args.sort_period or 32},
+                   "launch": {"wg_threads": args.wg_threads or 768, "sort_period": args.sort_period or 32},
                    "n_grid": scene.n_grid, "box": "[0.3,0.7]^2x[0.05,0.6]", "dt": dt, "E": scene.E,
                    "nu": scene.nu, "friction_angle": scene.friction_angle, "density": scene.density,
                    "parallelism": "single GPU"},

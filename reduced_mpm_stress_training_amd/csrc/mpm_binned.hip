@@ -367,7 +367,7 @@ __device__ void scatter_spill(double* __restrict__ acc, int* __restrict__ gflag,
                               const float tau[9], float dt) {
     const int ng = sc.prm.n_grid;
     double* sacc = acc + 4 * sc.node_base;
-    p2g_contrib(A, m, vol, v, C, tau, sc.prm.dx, sc.prm.inv_dx, dt, [&](int c, int i, int j, int k, float val) {
+    p2g_contrib_rolled(A, m, vol, v, C, tau, sc.prm.dx, sc.prm.inv_dx, dt, [&](int c, int i, int j, int k, float val) {
         const int x = A.base[0] + i, y = A.base[1] + j, z = A.base[2] + k;
         unsafeAtomicAdd(sacc + 4 * (((int64_t)x * ng + y) * ng + z) + c, (double)val);
     });
@@ -392,13 +392,29 @@ __device__ __forceinline__ void scatter_tile(double* tile, int i0, const AxisWei
 // Writes a bin's LDS tile (nodes b0-1 .. b0+10) to its slot of the global
 // tile buffer as float4 (mv, m) and stamps it with the epoch.  tid: the
 // thread's index in the workgroup.
+// Marks the grid blocks that bin `bin`'s tile overlaps (nodes 8B-1 .. 8B+10
+// along each axis: blocks B-1 .. B+1, up to 27) with the tile's epoch, so
+// that the grid update passes over every other block after one load, without
+// the scene lookup and the 27 stamp loads.  tid: the thread's index in the
+// workgroup.
+__device__ __forceinline__ void mark_tile_blocks(int* __restrict__ blive, const SceneDesc& sc, int bin, int epoch,
+                                                 int tid) {
+    if (tid < 27) {
+        const int nb = sc.nb, lb = bin - sc.bin_base;
+        const int x = lb / (nb * nb) + tid / 9 - 1, y = (lb / nb) % nb + (tid / 3) % 3 - 1, z = lb % nb + tid % 3 - 1;
+        if (x >= 0 && y >= 0 && z >= 0 && x < nb && y < nb && z < nb) blive[sc.bin_base + (x * nb + y) * nb + z] = epoch;
+    }
+}
+
 __device__ __forceinline__ void write_tile(const double* s_acc, float4* __restrict__ tiles,
-                                           int* __restrict__ stamps, int bin, int epoch, int tid) {
+                                           int* __restrict__ stamps, int* __restrict__ blive, const SceneDesc& sc,
+                                           int bin, int epoch, int tid) {
     float4* out = tiles + (int64_t)bin * kTS3;
     for (int t = tid; t < kTS3; t += blockDim.x)
         out[t] = make_float4((float)s_acc[t], (float)s_acc[kTS3 + t], (float)s_acc[2 * kTS3 + t],
                              (float)s_acc[3 * kTS3 + t]);
     if (tid == 0) stamps[bin] = epoch;
+    mark_tile_blocks(blive, sc, bin, epoch, tid);
 }
 
 
@@ -417,6 +433,7 @@ struct SubstepArgs {
     int* __restrict__ status;
     float4* __restrict__ tiles;
     int* __restrict__ stamps;       // epoch of each bin's tile; -epoch: split, tiles in ptiles
+    int* __restrict__ blive;        // epoch of the newest tile overlapping each grid block
     double* __restrict__ ptiles;    // FP64 tiles of split bins' parts, one slot per work item
     int* __restrict__ pcount;       // parts of each split bin
     int* __restrict__ pslot;        // ptiles slot of part p of bin b: pslot[p * n_bins + b]
@@ -564,14 +581,7 @@ __device__ __forceinline__ void g2p2g_particle(const SubstepArgs& a, const Scene
         } else {
             // global gather: same contraction, nodes read from the grid
             const float4* vb = a.vel + sc.node_base + (((int64_t)A.base[0] * ng + A.base[1]) * ng + A.base[2]);
-            float4 nodes[27];
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-#pragma unroll
-                for (int j = 0; j < 3; ++j)
-#pragma unroll
-                    for (int k = 0; k < 3; ++k) nodes[(i * 3 + j) * 3 + k] = vb[((int64_t)i * ng + j) * ng + k];
-            gather_tile<9, 3>(Float4Nodes{nodes}, 0, A, prm.inv_dx, v, Cm, G);
+            gather_global_rolled(vb, ng, A, prm.inv_dx, v, Cm, G);
         }
         constitutive_update(G, dt, F, prm.material, prm.mu, prm.lam, prm.alpha, prm.hardening, prm.xi, ys, tau);
         // x += dt v ; u = x - x0 ; the next substep recomputes x = x0 + u
@@ -651,7 +661,7 @@ __global__ void __launch_bounds__(kWG)
                 const int* __restrict__ pslot, const double* __restrict__ ptiles, int* __restrict__ cnt_nxt,
                 const int* __restrict__ cnt_cur, int* __restrict__ order, BinGeom g, SceneTab tab,
                 ColliderSet cs, float time, float dt, int epoch, int clear_only, const double* __restrict__ t_base,
-                float4* __restrict__ rec, int* __restrict__ status) {
+                float4* __restrict__ rec, int* __restrict__ status, const int* __restrict__ blive) {
     __shared__ int s_live[27];             // neighbour bin c has a whole-bin tile of this epoch
     __shared__ int s_pslot[kParts * 27];   // FP64 tile slot of part p of neighbour bin c this epoch, or -1
     __shared__ int s_any, s_anyp, s_bad;
@@ -661,6 +671,16 @@ __global__ void __launch_bounds__(kWG)
     }
     const int blk = blockIdx.x - 1;   // global block (= global bin index)
     const int tid = threadIdx.x;
+    if (tid == 0 && cnt_nxt) {
+        cnt_nxt[blk] = 0;                 // slots
+        cnt_nxt[g.n_bins + 1 + blk] = 0;  // live particles
+        if (blk == 0) cnt_nxt[g.n_bins] = 0;
+    }
+    // most blocks hold nothing: no tile of this epoch overlaps them (the
+    // tiles' writers mark the blocks they overlap), no spill and no velocities
+    // of an earlier grid update to clear -- they leave here, before the scene
+    // lookup and the neighbours' stamps
+    if (gflag[blk] == 0 && (clear_only || blive[blk] != epoch)) return;
     // a captured run reads its start time from the device: `time` is then the offset
     const float tnow = t_base ? (float)(t_base[0] + (double)time) : time;
     const SceneDesc sc = scene_of_bin(tab, blk);
@@ -675,11 +695,6 @@ __global__ void __launch_bounds__(kWG)
     tiles += (int64_t)sc.bin_base * kTS3;
     const MpmCollider* cols = tab.dev ? tab.cols + sc.col_base : cs.c;
     const int n_cols = tab.dev ? sc.n_cols : cs.n;
-    if (tid == 0 && cnt_nxt) {
-        cnt_nxt[blk] = 0;                 // slots
-        cnt_nxt[g.n_bins + 1 + blk] = 0;  // live particles
-        if (blk == 0) cnt_nxt[g.n_bins] = 0;
-    }
     if (tid == 0) s_any = s_anyp = s_bad = 0;
     __syncthreads();
     if (tid < 27) {
@@ -890,13 +905,25 @@ __device__ __forceinline__ void g2p2g_item(const SubstepArgs& a, int bin, int pa
         const int64_t offp = off0 + lo;   // the item's first slot
         const int n_in = hi - lo;
         const bool metal = sc.prm.material == 1;
-        ParticleIn q;
-        load_particle(a.src, offp + tid, cap, tid < n_in, metal, q);
-        for (int it = 0; it < n_in; it += WG) {
-            const int i = it + tid;
-            const ParticleIn cur = q;
-            if (it + WG < n_in) load_particle(a.src, offp + i + WG, cap, i + WG < n_in, metal, q);  // prefetch
-            g2p2g_particle(a, sc, i < n_in, offp + i, cap, bin, stay_base, s_stay, b0, s_vel, s_acc, cur);
+        if constexpr (WG == 768) {
+            // three waves per SIMD leave no registers for loading the next
+            // particle ahead
+            for (int it = 0; it < n_in; it += WG) {
+                const int i = it + tid;
+                ParticleIn cur;
+                load_particle(a.src, offp + i, cap, i < n_in, metal, cur);
+                g2p2g_particle(a, sc, i < n_in, offp + i, cap, bin, stay_base, s_stay, b0, s_vel, s_acc, cur);
+            }
+        } else {
+            // two waves per SIMD: the next particle's inputs load during this one
+            ParticleIn q;
+            load_particle(a.src, offp + tid, cap, tid < n_in, metal, q);
+            for (int it = 0; it < n_in; it += WG) {
+                const int i = it + tid;
+                const ParticleIn cur = q;
+                if (it + WG < n_in) load_particle(a.src, offp + i + WG, cap, i + WG < n_in, metal, q);
+                g2p2g_particle(a, sc, i < n_in, offp + i, cap, bin, stay_base, s_stay, b0, s_vel, s_acc, cur);
+            }
         }
     }
     __syncthreads();
@@ -906,7 +933,7 @@ __device__ __forceinline__ void g2p2g_item(const SubstepArgs& a, int bin, int pa
         a.dst[off0 + (int64_t)MPM_P_PID * cap + h] = __int_as_float(-1);
     if (tid == 0) atomicAdd(&a.cnt_nxt[g.n_bins + 1 + bin], max(0, min(*s_stay, cap - stay_base)));
     if (parts == 1) {
-        write_tile(s_acc, a.tiles, a.stamps, bin, a.epoch_out, tid);
+        write_tile(s_acc, a.tiles, a.stamps, a.blive, sc, bin, a.epoch_out, tid);
     } else {
         // a part's sums stay FP64 until the grid update has added the parts:
         // rounded once, as a whole bin's tile, whichever particles fell
@@ -925,6 +952,7 @@ __device__ __forceinline__ void g2p2g_item(const SubstepArgs& a, int bin, int pa
                 a.stamps[bin] = -a.epoch_out;
             }
         }
+        if (part == 0) mark_tile_blocks(a.blive, sc, bin, a.epoch_out, tid);
     }
 }
 
@@ -944,13 +972,18 @@ __device__ __forceinline__ void g2p2g_overflow(const SubstepArgs& a, int blk) {
     g2p2g_particle(a, sc, i < n_ovf, off, g.ovf_blk, g.n_bins, -1, nullptr, b0, nullptr, nullptr, q);
 }
 
-// WG: threads per workgroup (256 or 512).  At the kernel's ~226 VGPRs a SIMD
-// holds two waves either way.  Workgroup w < n_bins runs work item w of the
-// schedule of this epoch (schedule_bins: at most n_bins items, largest
+// WG: threads per workgroup (256, 512 or 768).  The 76 KB of LDS tiles
+// allow one 768- or 512-thread workgroup per CU, or two of 256 threads.  The
+// kernel needs ~161 VGPRs (its rarely taken paths -- the spill scatter and
+// the overflow run's global gather -- are written with small live sets so
+// that they do not set the budget), so 768 threads give three waves per
+// SIMD; 256 and 512 threads two.  Workgroup w < n_bins runs work item w of
+// the schedule of this epoch (schedule_bins: at most n_bins items, largest
 // first; -1 past the last) or, without one, bin w whole; the workgroups
 // after them run the overflow run in blocks of WG particles.
 template <int WG>
-__global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2, 2))) k_g2p2g(SubstepArgs a) {
+__global__ void __launch_bounds__(WG)
+    __attribute__((amdgpu_waves_per_eu(WG == 768 ? 3 : 2, WG == 768 ? 3 : 2))) k_g2p2g(SubstepArgs a) {
     __shared__ float s_vel[3 * kTS3];
     __shared__ double s_acc[4 * kTS3];
     __shared__ int s_stay, s_base, s_live;
@@ -972,7 +1005,7 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2, 2)))
 __global__ void __launch_bounds__(kWG)
     k_p2g_bins(SceneTab tab, BinGeom g, float dt, const float* __restrict__ planes, const int* __restrict__ cnt,
                double* __restrict__ acc, int* __restrict__ gflag, float4* __restrict__ tiles,
-               int* __restrict__ stamps, int epoch_out) {
+               int* __restrict__ stamps, int* __restrict__ blive, int epoch_out) {
     __shared__ double s_acc[4 * kTS3];
     const int tid = threadIdx.x;
     const int wg = blockIdx.x;
@@ -1027,7 +1060,7 @@ __global__ void __launch_bounds__(kWG)
     }
     if (!ovf) {
         __syncthreads();
-        write_tile(s_acc, tiles, stamps, bin, epoch_out, threadIdx.x);
+        write_tile(s_acc, tiles, stamps, blive, sc_bin, bin, epoch_out, threadIdx.x);
     }
 }
 
@@ -1158,11 +1191,12 @@ __global__ void __launch_bounds__(kWG)
 // `from`, or -from for a split bin) become the run's first epoch `to` (-to),
 // every other stamp is voided, so that no tile of an earlier replay matches
 // an epoch the run reuses.
-__global__ void k_stamp_rebase(int* __restrict__ stamps, int n, int from, int to) {
+__global__ void k_stamp_rebase(int* __restrict__ stamps, int* __restrict__ blive, int n, int from, int to) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
         const int st = stamps[i];
         stamps[i] = st == from ? to : st == -from ? -to : (int)0x80000000;
+        blive[i] = blive[i] == from ? to : 0;
     }
 }
 
@@ -1320,6 +1354,7 @@ struct WsLayout {
     int* stamps;       // epoch of each bin's tile (-epoch: the bin was split; see write_tile)
     int* pcount;       // parts of each split bin
     int* pslot;        // FP64 tile slot of part p of split bin b (p * nbins + b)
+    int* blive;        // epoch of the newest tile overlapping each grid block (mark_tile_blocks)
     int* order;        // schedules of the sort and of k_g2p2g (schedule_bins)
     float4* tiles;     // one tile of kTS^3 nodes per bin
     double* ptiles;    // nbins FP64 tiles of kTS^3 nodes (4 doubles each) for the parts of split bins
@@ -1365,6 +1400,8 @@ WsLayout ws_layout(int64_t nodes, int64_t nbins, int n_scenes, int n_cols, void*
     off += align256(nbins * 4);
     w.pslot = (int*)(p + off);
     off += align256(kParts * nbins * 4);
+    w.blive = (int*)(p + off);
+    off += align256(nbins * 4);
     w.order = (int*)(p + off);
     off += align256((2 * nbins + kItemsFirst) * 4);
     w.tiles = (float4*)(p + off);
@@ -1398,8 +1435,8 @@ int check_bins(const MpmBins* b, int64_t n_bins, int64_t n_slots, int max_cap, B
     if (b->bin_cap < 1 || b->overflow_cap < 0 || b->overflow_cap % b->bin_cap != 0)
         return set_err(-22, "bins: bin_cap < 1 or overflow_cap not a multiple of bin_cap");
     if (b->cur != 0 && b->cur != 1) return set_err(-23, "bins: cur must be 0 or 1");
-    if (b->wg_threads != 0 && b->wg_threads != 256 && b->wg_threads != 512)
-        return set_err(-29, "bins: wg_threads must be 0, 256 or 512");
+    if (b->wg_threads != 0 && b->wg_threads != 256 && b->wg_threads != 512 && b->wg_threads != 768)
+        return set_err(-29, "bins: wg_threads must be 0, 256, 512 or 768");
     if (n_slots + b->overflow_cap > 0x7fffffffLL) return set_err(-28, "bins: more than 2^31-1 slots");
     if (n_bins >= (1LL << 27)) return set_err(-28, "bins: 2^27 bins or more");
     if (max_cap > 16 * 1000 * 1000) return set_err(-22, "bins: more than 16e6 slots per bin");
@@ -1520,7 +1557,8 @@ int op_clear(const Ctx& c, MpmBins* bins, hipStream_t s) {
     hipLaunchKernelGGL(k_grid_bins, dim3(c.g.n_bins + 1), dim3(kWG), 0, s, c.w.acc, c.w.vel, c.w.gflag,
                        (const int*)c.w.stamps, (const float4*)c.w.tiles, (const int*)c.w.pcount,
                        (const int*)c.w.pslot, (const double*)c.w.ptiles, (int*)nullptr, (const int*)nullptr, (int*)nullptr, c.g, c.tab,
-                       c.cs, 0.f, 0.f, bins->epoch, 1, (const double*)nullptr, (float4*)nullptr, c.w.status);
+                       c.cs, 0.f, 0.f, bins->epoch, 1, (const double*)nullptr, (float4*)nullptr, c.w.status,
+                       (const int*)c.w.blive);
     int rc = check_launch("k_grid_bins(clear)");
     if (rc) return rc;
     bins->epoch += 1;  // the pending tiles are no longer current
@@ -1531,7 +1569,7 @@ int op_p2g(const Ctx& c, MpmBins* bins, float dt, hipStream_t s) {
     const int epoch_out = bins->epoch + 1;
     hipLaunchKernelGGL(k_p2g_bins, dim3(c.g.n_bins + ovf_wgs(c.g)), dim3(kWG), 0, s, c.tab, c.g, dt,
                        bins->planes[bins->cur], (const int*)counts(bins, c.g, bins->cur), (double*)c.w.acc, c.w.gflag,
-                       c.w.tiles, c.w.stamps, epoch_out);
+                       c.w.tiles, c.w.stamps, c.w.blive, epoch_out);
     int rc = check_launch("k_p2g_bins");
     if (rc) return rc;
     bins->epoch = epoch_out;
@@ -1553,7 +1591,7 @@ int op_grid(const Ctx& c, const MpmBins* bins, float time, float dt, hipStream_t
                        (const int*)c.w.stamps, (const float4*)c.w.tiles, (const int*)c.w.pcount,
                        (const int*)c.w.pslot, (const double*)c.w.ptiles, counts(bins, c.g, 1 - bins->cur),
                        (const int*)counts(bins, c.g, bins->cur), c.w.order, c.g, c.tab, c.cs, time, dt, bins->epoch,
-                       0, t_base, rec, c.w.status);
+                       0, t_base, rec, c.w.status, (const int*)c.w.blive);
     return check_launch("k_grid_bins");
 }
 
@@ -1576,6 +1614,7 @@ int op_g2p2g(const Ctx& c, MpmBins* bins, float dt, hipStream_t s, bool store_al
     a.status = c.w.status;
     a.tiles = c.w.tiles;
     a.stamps = c.w.stamps;
+    a.blive = c.w.blive;
     a.ptiles = c.w.ptiles;
     a.pcount = c.w.pcount;
     a.pslot = c.w.pslot;
@@ -1584,8 +1623,10 @@ int op_g2p2g(const Ctx& c, MpmBins* bins, float dt, hipStream_t s, bool store_al
     a.store_all = store_all ? 1 : 0;
     if (bins->wg_threads == 256)
         hipLaunchKernelGGL(k_g2p2g<256>, dim3(c.g.n_bins + ovf_wgs(c.g, 256)), dim3(256), 0, s, a);
-    else
+    else if (bins->wg_threads == 512)
         hipLaunchKernelGGL(k_g2p2g<512>, dim3(c.g.n_bins + ovf_wgs(c.g, 512)), dim3(512), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_g2p2g<768>, dim3(c.g.n_bins + ovf_wgs(c.g, 768)), dim3(768), 0, s, a);
     int rc = check_launch("k_g2p2g");
     if (rc) return rc;
     bins->epoch = a.epoch_out;
@@ -1667,8 +1708,8 @@ int op_graph_create(const Ctx& c, const void* ws_id, const MpmBins* bins, float 
         hipStreamDestroy(cs);
         return set_err(-53, "graph: begin capture failed");
     }
-    hipLaunchKernelGGL(k_stamp_rebase, dim3((c.g.n_bins + 255) / 256), dim3(256), 0, cs, c.w.stamps, c.g.n_bins,
-                       e_end, e0);
+    hipLaunchKernelGGL(k_stamp_rebase, dim3((c.g.n_bins + 255) / 256), dim3(256), 0, cs, c.w.stamps, c.w.blive,
+                       c.g.n_bins, e_end, e0);
     rc = check_launch("k_stamp_rebase");
     double off = 0.0;
     for (int k = 0; k < n_substeps && !rc; ++k) {
@@ -1720,8 +1761,8 @@ int op_graph_launch(const Ctx& c, const void* ws_id, MpmGraph* g, MpmBins* bins,
     if (n_replays < 0) return set_err(-26, "n_replays < 0");
     if (n_replays == 0) return 0;
     // the pending tiles take the stamp a replay's rebase expects; the time base
-    hipLaunchKernelGGL(k_stamp_rebase, dim3((c.g.n_bins + 255) / 256), dim3(256), 0, s, c.w.stamps, c.g.n_bins,
-                       bins->epoch, g->epoch_end);
+    hipLaunchKernelGGL(k_stamp_rebase, dim3((c.g.n_bins + 255) / 256), dim3(256), 0, s, c.w.stamps, c.w.blive,
+                       c.g.n_bins, bins->epoch, g->epoch_end);
     int rc = check_launch("k_stamp_rebase");
     if (rc) return rc;
     hipLaunchKernelGGL(k_time_set, dim3(1), dim3(64), 0, s, time_base(c), time, 0);

@@ -159,6 +159,105 @@ __device__ __forceinline__ void p2g_contrib(const AxisWeights& A, float m, float
     }
 }
 
+// Factor n of a per-axis triple for a run-time n (selects, no dynamic
+// register indexing).
+__device__ __forceinline__ float pick3(const float a[3], int n) { return n == 0 ? a[0] : (n == 1 ? a[1] : a[2]); }
+
+// p2g_contrib with the x loop kept rolled: every emitted value is the same
+// expression with the same FP32 roundings, but the live set stays small.
+// For paths few particles take (the spill scatter), so that they do not set
+// the register budget of the kernels that contain them.
+template <typename Emit>
+__device__ __forceinline__ void p2g_contrib_rolled(const AxisWeights& A, float m, float vol, const float v[3],
+                                                   const float C[9], const float tau[9], float dx, float inv_dx,
+                                                   float dt, Emit&& emit) {
+    const float ca = m * dx;
+    const float cb = -dt * vol * inv_dx;
+    float P[3][3], Q[3][3], R[3][3];   // [component][node]
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float ax = ca * C[a * 3 + 0], ay = ca * C[a * 3 + 1], az = ca * C[a * 3 + 2];
+        const float bx = cb * tau[a * 3 + 0], by = cb * tau[a * 3 + 1], bz = cb * tau[a * 3 + 2];
+        const float mv = m * v[a];
+#pragma unroll
+        for (int n = 0; n < 3; ++n) {
+            P[a][n] = fmaf(mv, A.w[0][n], fmaf(ax, A.e[0][n], bx * A.d[0][n]));
+            Q[a][n] = fmaf(ay, A.e[1][n], by * A.d[1][n]);
+            R[a][n] = fmaf(az, A.e[2][n], bz * A.d[2][n]);
+        }
+    }
+#pragma unroll 1
+    for (int i = 0; i < 3; ++i) {
+        const float wx = pick3(A.w[0], i);
+        const float mwx = m * wx;
+        const float Pi[3] = {pick3(P[0], i), pick3(P[1], i), pick3(P[2], i)};
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const float wyz = A.w[1][j] * A.w[2][k];
+                const float wxz = wx * A.w[2][k];
+                const float wxy = wx * A.w[1][j];
+                emit(3, i, j, k, mwx * wyz);
+#pragma unroll
+                for (int a = 0; a < 3; ++a) emit(a, i, j, k, fmaf(Pi[a], wyz, fmaf(Q[a][j], wxz, R[a][k] * wxy)));
+            }
+    }
+}
+
+// G2P gather from the 27 global float4 nodes vb[(i ng + j) ng + k]: the
+// contraction of gather_tile (each accumulator receives the same FMAs in
+// the same order), one velocity component at a time with the x and y loops
+// rolled, reading only that component of three nodes at a time: a small live
+// set for the paths few particles take (the overflow run).
+__device__ __forceinline__ void gather_global_rolled(const float4* __restrict__ vb, int64_t ng,
+                                                     const AxisWeights& A, float inv_dx, float v[3], float C[9],
+                                                     float G[9]) {
+    const float c4 = 4.0f * inv_dx;
+    const float* vf = reinterpret_cast<const float*>(vb);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        float sv = 0.f, cx = 0.f, gx = 0.f, cy = 0.f, gy = 0.f, cz = 0.f, gz = 0.f;
+#pragma unroll 1
+        for (int i = 0; i < 3; ++i) {
+            float yww = 0.f, yew = 0.f, ydw = 0.f, ywe = 0.f, ywd = 0.f;
+#pragma unroll 1
+            for (int j = 0; j < 3; ++j) {
+                const float* row = vf + 4 * (((int64_t)i * ng + j) * ng) + c;
+                const float g[3] = {row[0], row[4], row[8]};
+                float zw = 0.f, ze = 0.f, zd = 0.f;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    zw = fmaf(A.w[2][k], g[k], zw);
+                    ze = fmaf(A.e[2][k], g[k], ze);
+                    zd = fmaf(A.d[2][k], g[k], zd);
+                }
+                const float wj = pick3(A.w[1], j), ej = pick3(A.e[1], j), dj = pick3(A.d[1], j);
+                yww = fmaf(wj, zw, yww);
+                yew = fmaf(ej, zw, yew);
+                ydw = fmaf(dj, zw, ydw);
+                ywe = fmaf(wj, ze, ywe);
+                ywd = fmaf(wj, zd, ywd);
+            }
+            const float wi = pick3(A.w[0], i), ei = pick3(A.e[0], i), di = pick3(A.d[0], i);
+            sv = fmaf(wi, yww, sv);
+            cx = fmaf(ei, yww, cx);
+            gx = fmaf(di, yww, gx);
+            cy = fmaf(wi, yew, cy);
+            gy = fmaf(wi, ydw, gy);
+            cz = fmaf(wi, ywe, cz);
+            gz = fmaf(wi, ywd, gz);
+        }
+        v[c] = sv;
+        C[c * 3 + 0] = cx * c4;
+        C[c * 3 + 1] = cy * c4;
+        C[c * 3 + 2] = cz * c4;
+        G[c * 3 + 0] = gx * inv_dx;
+        G[c * 3 + 1] = gy * inv_dx;
+        G[c * 3 + 2] = gz * inv_dx;
+    }
+}
+
 // Bank order.  Slots q of a run of n particles are filled with the
 // particles' counting sort by LDS bank class (sorted index k) as
 // q = k * mul mod n, mul the inverse of s ~ n/32 modulo n: slots q and q+1

@@ -2,8 +2,8 @@
 
 The fused G2P2G kernel has two free launch parameters that change speed but
 not results (beyond the order of the FP64 tile sums): the threads per bin
-workgroup (256 puts two bins on a CU at once, 512 covers a full bin in fewer
-loop trips) and the period of the bank-order sort (sorting costs a pass over
+workgroup (256 puts two bins on a CU at once, 768 runs one bin with three
+waves per SIMD, 512 with two) and the period of the bank-order sort (sorting costs a pass over
 the state, not sorting lets LDS bank conflicts grow as particles drift).
 Which is best depends on the scenes -- bin occupancy, particle counts, how
 fast particles move -- so `autotune` times each candidate on a throwaway
@@ -23,7 +23,7 @@ import time
 
 import torch
 
-CANDIDATES = ((512, 32), (256, 32), (512, 64), (256, 64), (512, 16), (256, 16))
+CANDIDATES = ((768, 32), (512, 32), (256, 32), (768, 64), (256, 64), (256, 16))
 
 
 def autotune(factory, dt: float, trial_substeps: int = 32, warmup_substeps: int = 4,

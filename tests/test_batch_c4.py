@@ -65,7 +65,7 @@ def test_64_uneven_scenes_300_captured_substeps(oracle):
     order = np.argsort(sizes)
     picks = [int(order[0]), int(order[len(order) // 2]), int(order[-1])]
     best = autotune(lambda: B.SceneBatch(scenes), DT, trial_substeps=8,
-                    candidates=((512, 32), (256, 32), (256, 16)), rounds=1)
+                    candidates=((768, 32), (512, 32), (256, 32), (256, 16)), rounds=1)
     launch = best["launch"]
 
     # the first 10 substeps of three scenes against the oracle

@@ -149,6 +149,7 @@ def _run_oracle(oracle, sc, n_steps, dtype, order=None, nthreads=1):
 
 
 @pytest.mark.parametrize("n_steps,binned,launch", [(10, True, (0, 0)), (100, True, (0, 0)), (100, True, (256, 16)),
+                                                   (100, True, (512, 32)),
                                                    (10, False, (0, 0)), (100, False, (0, 0))])
 def test_column_32_parity(oracle, n_steps, binned, launch):
     """The 32^3 sand column (8 particles/cell, E=2e5, nu=0.3, 30 deg,
