@@ -42,7 +42,7 @@
 extern "C" {
 #endif
 
-#define MPM_ABI_VERSION 13
+#define MPM_ABI_VERSION 14
 
 /* plane indices */
 #define MPM_P_X0 0
@@ -358,8 +358,16 @@ int mpm_bins_run(const MpmParams* prm, MpmBins* bins, const MpmCollider* collide
 #define MPM_ADJ_TAU 24
 #define MPM_ADJ_NPLANES 33
 
-/* Workspace of mpm_adjoint_substep (256-byte aligned): 4 grids of n_grid^3
- * float4. */
+/* A grid set of the ordered segment path (256-byte aligned): the P2G
+ * accumulator and the grid velocities (n_grid^3 float4 each), then one int32
+ * per 8^3 grid block, nonzero where a P2G into the accumulator wrote.  The
+ * grid update, the next substep's clearing and the grid adjoint visit only
+ * marked blocks.  A grid set's accumulator must be zero outside its marked
+ * blocks: allocate it zeroed, and afterwards leave it to these calls. */
+int64_t mpm_step_grids_bytes(const MpmParams* prm);
+
+/* Workspace of mpm_adjoint_substep (256-byte aligned): a grid set, then two
+ * grids of n_grid^3 float4 (the velocity adjoint and the (mv, m) adjoint). */
 int64_t mpm_adjoint_workspace_bytes(const MpmParams* prm);
 
 /* st: the state S_n before the substep, in the order of `order` (see
@@ -381,7 +389,8 @@ int mpm_adjoint_substep(const MpmParams* prm, const MpmState* st, const void* or
  * entry, states[1..] receive S_{k0+1} .. S_{k0+n_sub-1} -- the planes a
  * substep changes (u, v, C, F, tau); their constant planes (x0, m, vol, id)
  * are left as they are, those of states[0] being used.  step_grids: n_sub
- * workspaces of mpm_workspace_bytes (256-byte aligned).  Substep k runs at
+ * grid sets of mpm_step_grids_bytes (256-byte aligned, zeroed before their
+ * first use).  Substep k runs at
  * time (float)(k * dt) with step (float)dt.  adj_a holds the adjoint of
  * S_{k0+n_sub} on entry and the adjoint of S_k0 on return; adj_b is scratch of
  * the same size.  workspace: mpm_adjoint_workspace_bytes. */
@@ -397,9 +406,9 @@ int mpm_adjoint_segment(const MpmParams* prm, const MpmState* states, const void
  * step_grids[0] already holds the P2G accumulator of states[0] (the
  * next_grid of the segment before, whose last state this segment sorted), so
  * it is not recomputed.  When `last` is not NULL (same n and stride, no
- * alias of a state) it receives S_{k0+n_sub} with every plane, and next_grid (256-byte
- * aligned, >= 16 n_grid^3 bytes, not one of step_grids) that state's P2G
- * accumulator.  mpm_adjoint_recorded: the adjoint substeps in reverse on what
+ * alias of a state) it receives S_{k0+n_sub} with every plane, and next_grid (a
+ * grid set of mpm_step_grids_bytes, not one of step_grids) that state's P2G
+ * accumulator and marks.  mpm_adjoint_recorded: the adjoint substeps in reverse on what
  * mpm_record_segment left, arguments as in mpm_adjoint_segment.  No
  * counterpart in the reference (it records no tape); with p2g_done = 0 the
  * two together equal mpm_adjoint_segment. */
@@ -439,7 +448,8 @@ int mpm_step_ordered(const MpmParams* prm, const MpmState* st, const void* order
  * (float)(k * dt) with step (float)dt: the P2G of the first, then per
  * substep the grid update and one fused kernel doing this substep's G2P and
  * the next substep's P2G over the chunks (LDS tiles of grid velocities and
- * of the next P2G).  workspace: mpm_run_ordered_workspace_bytes. */
+ * of the next P2G; the grid work covers only the 8^3 blocks the P2G marked).
+ * workspace: mpm_run_ordered_workspace_bytes (two grid sets). */
 int64_t mpm_run_ordered_workspace_bytes(const MpmParams* prm);
 int mpm_run_ordered(const MpmParams* prm, const MpmState* st, const void* order, const MpmCollider* colliders,
                     int n_colliders, void* workspace, int64_t workspace_bytes, int k0, double dt, int n_sub,

@@ -10,7 +10,7 @@ import os
 
 from ._build import LIB_PATH
 
-MPM_ABI_VERSION = 13
+MPM_ABI_VERSION = 14
 MAX_COLLIDERS = 16
 
 # plane indices of the particle state buffer
@@ -126,6 +126,7 @@ _SIGNATURES = {
                                          C.c_void_p]),
     "mpm_graph_destroy": (C.c_int, [C.c_void_p]),
     "mpm_adjoint_workspace_bytes": (C.c_int64, [C.POINTER(MpmParams)]),
+    "mpm_step_grids_bytes": (C.c_int64, [C.POINTER(MpmParams)]),
     "mpm_adjoint_substep": (C.c_int, [C.POINTER(MpmParams), C.POINTER(MpmState), C.c_void_p, C.c_void_p,
                                       C.POINTER(MpmCollider), C.c_int, C.c_void_p, C.c_float, C.c_float, C.c_void_p,
                                       C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),

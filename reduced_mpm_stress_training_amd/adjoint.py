@@ -96,6 +96,9 @@ class Rollout:
             self._prm.gravity[d] = float(P["gravity"][d])
         self._ws_step_bytes = int(lib.mpm_workspace_bytes(self._prm, self.n))
         self._ws_step = torch.zeros(self._ws_step_bytes + 256, dtype=torch.uint8, device=self.device)
+        # grid sets (accumulator, velocities, block marks) of the recorded
+        # and replayed segments: zeroed once, kept consistent by the library
+        self._set_bytes = int(lib.mpm_step_grids_bytes(self._prm))
         self._ws_run_bytes = int(lib.mpm_run_ordered_workspace_bytes(self._prm))
         self._ws_run = torch.zeros(self._ws_run_bytes + 256, dtype=torch.uint8, device=self.device)
         self._ws_adj = torch.zeros(int(lib.mpm_adjoint_workspace_bytes(self._prm)) + 256, dtype=torch.uint8,
@@ -124,7 +127,7 @@ class Rollout:
 
     def record_bytes(self) -> int:
         """Device bytes of the keep="all" record (states and grids of every substep)."""
-        return self.n_substeps * (L.NPLANES * self.n * 4 + self._ws_step_bytes + 256)
+        return self.n_substeps * (L.NPLANES * self.n * 4 + self._set_bytes + 256)
 
     def _records(self) -> bool:
         if self.keep != "auto":
@@ -210,7 +213,7 @@ class Rollout:
         while len(self._tape_states) < self.n_substeps:
             self._tape_states.append(torch.empty((L.NPLANES, self.n), dtype=torch.float32, device=self.device))
         while len(self._tape_grids) < self.n_substeps:
-            self._tape_grids.append(torch.empty(self._ws_step_bytes + 256, dtype=torch.uint8, device=self.device))
+            self._tape_grids.append(torch.zeros(self._set_bytes + 256, dtype=torch.uint8, device=self.device))
         while len(self._tape_orders) < n_seg:
             self._tape_orders.append(torch.empty(self._order_bytes + 256, dtype=torch.uint8, device=self.device))
         if self._tape_last is None:
@@ -265,7 +268,7 @@ class Rollout:
         while len(self._seg_states) < self.K - 1:
             self._seg_states.append(torch.empty((L.NPLANES, self.n), dtype=torch.float32, device=self.device))
         while len(self._ws_seg) < self.K:
-            self._ws_seg.append(torch.empty(self._ws_step_bytes + 256, dtype=torch.uint8, device=self.device))
+            self._ws_seg.append(torch.zeros(self._set_bytes + 256, dtype=torch.uint8, device=self.device))
         for c0 in sorted(self._checkpoints, reverse=True):
             nk = min(self.K, self.n_substeps - c0)
             cp, order = self._checkpoints[c0]

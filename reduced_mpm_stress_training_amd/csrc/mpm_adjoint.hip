@@ -570,6 +570,40 @@ __global__ void __launch_bounds__(kBlk)
     if (clear_vb) reinterpret_cast<float4*>(vb)[t] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
+// k_grid_adj over the 8^3 grid blocks the substep's P2G marked (one
+// workgroup per block, one thread per node): the G2P adjoint scatters vb
+// only into the stencils of the substep's particles, which its P2G marked,
+// and the P2G adjoint reads gb only there; the other blocks leave after one
+// load (their vb is zero, their gb never read).
+__global__ void __launch_bounds__(512)
+    k_grid_adj_blocks(const float4* __restrict__ acc, const int* __restrict__ flags, float* __restrict__ vb,
+                      float4* __restrict__ gb, MpmParams prm, ColliderSet cs, float time, float dt, int clear_vb) {
+    const int ng = prm.n_grid, nb = (ng + 7) / 8;
+    const int blk = blockIdx.x;
+    if (!flags[blk]) return;
+    const int tt = threadIdx.x;
+    const int bz = blk % nb, by = (blk / nb) % nb, bx = blk / (nb * nb);
+    const int gx = bx * 8 + (tt >> 6), gy = by * 8 + ((tt >> 3) & 7), gz = bz * 8 + (tt & 7);
+    if (gx >= ng || gy >= ng || gz >= ng) return;
+    const int64_t t = ((int64_t)gx * ng + gy) * ng + gz;
+    const float4 a4 = acc[t];
+    float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (a4.w > 1e-15f) {
+        const float im = 1.0f / a4.w;
+        float o[3] = {a4.x * im + dt * prm.gravity[0], a4.y * im + dt * prm.gravity[1], a4.z * im + dt * prm.gravity[2]};
+        float mask[3] = {1.f, 1.f, 1.f};
+        for (int c = 0; c < cs.n; ++c)
+            apply_collider(cs.c[c], gx, gy, gz, ng, prm.dx, dt, time, o, mask);
+        const float b0 = vb[4 * t] * mask[0], b1 = vb[4 * t + 1] * mask[1], b2 = vb[4 * t + 2] * mask[2];
+        out.x = b0 * im;
+        out.y = b1 * im;
+        out.z = b2 * im;
+        out.w = -(b0 * a4.x + b1 * a4.y + b2 * a4.z) * im * im;
+    }
+    gb[t] = out;
+    if (clear_vb) reinterpret_cast<float4*>(vb)[t] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
 // ---------------------------------------------------------------------------
 // P2G adjoint: v, C, tau adjoints of S_n and the position adjoint's P2G part
 // (added to what k_g2p_adj_chunks wrote).  For each grid-adjoint field f
@@ -773,16 +807,20 @@ extern "C" {
 int64_t mpm_adjoint_workspace_bytes(const MpmParams* prm) {
     if (check_grid(prm)) return -1;
     const int64_t nn = (int64_t)prm->n_grid * prm->n_grid * prm->n_grid;
-    return 4 * nn * 16;   // acc (mv, m), grid v, v adjoint, (mv, m) adjoint
+    // a grid set (acc (mv, m), grid v, block marks), then the v adjoint and
+    // the (mv, m) adjoint
+    return mpm_step_grids_bytes(prm) + 2 * nn * 16;
 }
 
 // mpm_adjoint_substep; vb_clear: the workspace's velocity-adjoint grid is
 // already zero (cleared by the previous call's k_grid_adj); clear_after: leave
-// it zero for the next call.
+// it zero for the next call.  step_flags: the block marks of step_grids when
+// it is a grid set (a recorded segment), NULL for the two dense grids
+// mpm_step_ordered leaves.
 static int adjoint_substep(const MpmParams* prm, const MpmState* st, const void* order, const void* step_grids,
-                           const MpmCollider* colliders, int n_colliders, void* workspace, float time, float dt,
-                           const float* adj_next, float* adj_cur, int64_t adj_stride, double* param_adj,
-                           bool vb_clear, bool clear_after, const float* cst, void* stream) {
+                           const int* step_flags, const MpmCollider* colliders, int n_colliders, void* workspace,
+                           float time, float dt, const float* adj_next, float* adj_cur, int64_t adj_stride,
+                           double* param_adj, bool vb_clear, bool clear_after, const float* cst, void* stream) {
     int rc = check_sand(prm);
     if (rc) return rc;
     if (!st || !st->planes || st->n < 0 || st->stride < st->n) return set_err(-4, "state invalid");
@@ -795,20 +833,25 @@ static int adjoint_substep(const MpmParams* prm, const MpmState* st, const void*
     hipStream_t s = (hipStream_t)stream;
     const int64_t nn = (int64_t)prm->n_grid * prm->n_grid * prm->n_grid;
     if (step_grids && ((uintptr_t)step_grids & 255)) return set_err(-11, "step grids not 256-byte aligned");
-    float4* acc = (float4*)workspace;
+    // workspace: a grid set (accumulator, velocities, block marks), then the
+    // velocity adjoint and the (mv, m) adjoint
+    float* wset = (float*)workspace;
+    float4* acc = (float4*)wset;
     float4* gv = acc + nn;
-    float4* vbg = gv + nn;
+    float4* vbg = (float4*)(wset + mpm_step_grids_bytes(prm) / 4);
     float4* gbg = vbg + nn;
+    const int* flags = step_flags;
     if (step_grids) {
-        // the grids mpm_step_ordered left in its workspace for this substep
+        // the grids of this substep recorded by the forward
         acc = (float4*)step_grids;
         gv = acc + nn;
     } else {
         // forward grid of this substep (P2G over the chunks, normalization, colliders)
-        if ((rc = mpm_zero_grid(prm, (float*)acc, stream))) return rc;
-        if ((rc = p2g_ordered(prm, st, order, (float*)acc, dt, stream))) return rc;
-        if ((rc = mpm_grid_update(prm, (const float*)acc, (float*)gv, colliders, n_colliders, time, dt, stream)))
-            return rc;
+        if ((rc = clear_grid_set(prm, wset, stream))) return rc;
+        int* wflags = grid_set_flags(wset, prm->n_grid);
+        if ((rc = p2g_ordered(prm, st, order, wset, dt, stream, wflags))) return rc;
+        if ((rc = grid_update_blocks(prm, wset, colliders, n_colliders, time, dt, nullptr, stream))) return rc;
+        flags = wflags;
     }
     if (!vb_clear) {
         hipLaunchKernelGGL(k_zero4, dim3(blocks(nn)), dim3(kBlk), 0, s, vbg, nn);
@@ -825,9 +868,16 @@ static int adjoint_substep(const MpmParams* prm, const MpmState* st, const void*
                            (float*)vbg, adj_cur, param_adj);
         if ((rc = check_launch("k_g2p_adj_chunks"))) return rc;
     }
-    hipLaunchKernelGGL(k_grid_adj, dim3(blocks(nn)), dim3(kBlk), 0, s, (const float4*)acc, (float*)vbg, gbg,
-                       *prm, cs, time, dt, clear_after ? 1 : 0);
-    if ((rc = check_launch("k_grid_adj"))) return rc;
+    if (flags) {
+        // only the blocks this substep's P2G marked hold its particles' stencils
+        hipLaunchKernelGGL(k_grid_adj_blocks, dim3((unsigned)grid_blocks(prm->n_grid)), dim3(512), 0, s,
+                           (const float4*)acc, flags, (float*)vbg, gbg, *prm, cs, time, dt, clear_after ? 1 : 0);
+        if ((rc = check_launch("k_grid_adj_blocks"))) return rc;
+    } else {
+        hipLaunchKernelGGL(k_grid_adj, dim3(blocks(nn)), dim3(kBlk), 0, s, (const float4*)acc, (float*)vbg, gbg,
+                           *prm, cs, time, dt, clear_after ? 1 : 0);
+        if ((rc = check_launch("k_grid_adj"))) return rc;
+    }
     if (n > 0) {
         hipLaunchKernelGGL(k_p2g_adj, dim3(blocks(n)), dim3(kBlk), 0, s, (const float*)st->planes,
                            cst ? cst : (const float*)st->planes, st->stride, n, *prm, dt, (const float4*)gbg, adj_cur,
@@ -840,8 +890,8 @@ static int adjoint_substep(const MpmParams* prm, const MpmState* st, const void*
 int mpm_adjoint_substep(const MpmParams* prm, const MpmState* st, const void* order, const void* step_grids,
                         const MpmCollider* colliders, int n_colliders, void* workspace, float time, float dt,
                         const float* adj_next, float* adj_cur, int64_t adj_stride, double* param_adj, void* stream) {
-    return adjoint_substep(prm, st, order, step_grids, colliders, n_colliders, workspace, time, dt, adj_next, adj_cur,
-                           adj_stride, param_adj, false, false, nullptr, stream);
+    return adjoint_substep(prm, st, order, step_grids, nullptr, colliders, n_colliders, workspace, time, dt, adj_next,
+                           adj_cur, adj_stride, param_adj, false, false, nullptr, stream);
 }
 
 // shared checks of a segment's states and per-substep workspaces
@@ -862,31 +912,35 @@ static int check_segment(const MpmParams* prm, const MpmState* states, void* con
 // p2g_done says step_grids[0] holds it already), then per substep j its grid
 // update into step_grids[j] and the fused G2P + next P2G states[j] ->
 // states[j+1] into step_grids[j+1].  When `last` is given, the last substep
-// writes S_{k0+n_sub} there and its P2G to last_acc.
+// writes S_{k0+n_sub} there and its P2G to last_acc.  Every grid is a grid
+// set whose accumulator is zero outside its marked blocks (the caller
+// zeroes it before its first use); a grid update clears the marked blocks
+// of the set the next P2G writes, so no grid is swept whole.
 static int record_segment(const MpmParams* prm, const MpmState* states, const void* order, void* const* step_grids,
                           const MpmCollider* colliders, int n_colliders, int k0, double dt, int n_sub, int p2g_done,
                           const MpmState* last, float* last_acc, void* stream) {
-    const int64_t nn = (int64_t)prm->n_grid * prm->n_grid * prm->n_grid;
+    const int ng = prm->n_grid;
+    const int64_t nn = (int64_t)ng * ng * ng;
     const float fdt = (float)dt;
     int rc;
     float* acc0 = (float*)step_grids[0];
     if (!p2g_done) {
-        if ((rc = mpm_zero_grid(prm, acc0, stream))) return rc;
-        if ((rc = p2g_ordered(prm, &states[0], order, acc0, fdt, stream))) return rc;
+        if ((rc = clear_grid_set(prm, acc0, stream))) return rc;
+        if ((rc = p2g_ordered(prm, &states[0], order, acc0, fdt, stream, grid_set_flags(acc0, ng)))) return rc;
     }
     for (int j = 0; j < n_sub; ++j) {
         float* acc = (float*)step_grids[j];
         float* gvj = acc + 4 * nn;
         const bool more = j + 1 < n_sub;
         float* acc_next = more ? (float*)step_grids[j + 1] : (last ? last_acc : nullptr);   // NULL: none after
-        if ((rc = grid_update_zero_next(prm, acc, gvj, colliders, n_colliders, (float)((double)(k0 + j) * dt), fdt,
-                                        acc_next, stream)))
+        if ((rc = grid_update_blocks(prm, acc, colliders, n_colliders, (float)((double)(k0 + j) * dt), fdt, acc_next,
+                                     stream)))
             return rc;
         if (!acc_next) break;
         // the segment's later states hold only the planes a substep changes;
         // x0, m, vol, id stay in states[0] (and are written to `last`)
         if ((rc = g2p2g_ordered(prm, &states[j], more ? &states[j + 1] : last, order, gvj, acc_next, fdt,
-                                (const float*)states[0].planes, !more, stream)))
+                                (const float*)states[0].planes, !more, stream, grid_set_flags(acc_next, ng))))
             return rc;
     }
     return 0;
@@ -903,7 +957,8 @@ static int adjoint_recorded(const MpmParams* prm, const MpmState* states, const 
     float* cur = adj_a;
     float* nxt = adj_b;
     for (int j = n_sub - 1; j >= 0; --j) {
-        if ((rc = adjoint_substep(prm, &states[j], order, step_grids[j], colliders, n_colliders, workspace,
+        if ((rc = adjoint_substep(prm, &states[j], order, step_grids[j],
+                                  grid_set_flags((float*)step_grids[j], prm->n_grid), colliders, n_colliders, workspace,
                                   (float)((double)(k0 + j) * dt), (float)dt, cur, nxt, adj_stride, param_adj,
                                   j < n_sub - 1, j > 0, (const float*)states[0].planes, stream)))
             return rc;
@@ -936,14 +991,13 @@ int mpm_record_segment(const MpmParams* prm, const MpmState* states, const void*
     if (states[0].n < 0 || states[0].stride < states[0].n || states[0].n > 0x7fffffffLL)
         return set_err(-5, "segment: state n/stride invalid");
     if (!order || ((uintptr_t)order & 255)) return set_err(-40, "order buffer NULL or not 256-byte aligned");
-    const int64_t nn = (int64_t)prm->n_grid * prm->n_grid * prm->n_grid;
     if (last) {
         if (!last->planes || last->n != states[0].n || last->stride != states[0].stride)
             return set_err(-32, "segment: the last state must share n and stride");
         for (int j = 0; j < n_sub; ++j)
             if (states[j].planes == last->planes) return set_err(-32, "segment: the last state aliases a state");
-        if (!next_grid || ((uintptr_t)next_grid & 255) || next_grid_bytes < 16 * nn)
-            return set_err(-10, "segment: next_grid must be one 256-byte aligned float4 grid");
+        if (!next_grid || ((uintptr_t)next_grid & 255) || next_grid_bytes < mpm_step_grids_bytes(prm))
+            return set_err(-10, "segment: next_grid must be one 256-byte aligned grid set (mpm_step_grids_bytes)");
         for (int j = 0; j < n_sub; ++j)
             if (step_grids[j] == next_grid) return set_err(-10, "segment: next_grid aliases a step grid");
     }

@@ -17,6 +17,7 @@
 #include "mpm_common.h"
 #include "mpm_device.h"
 #include "mpm_hip.h"
+#include "mpm_ordered.h"
 
 using namespace mpm;
 
@@ -173,6 +174,38 @@ __global__ void __launch_bounds__(kBlock)
     if (zero_next) zero_next[node] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
+// The grid update on grid sets (mpm_ordered.h): one workgroup per 8^3 grid
+// block, one thread per node.  A block the P2G marked gets its velocities;
+// a block the next accumulator's earlier P2G marked is cleared and unmarked.
+// Blocks marked in neither leave after two loads.
+__global__ void __launch_bounds__(512)
+    k_grid_update_blocks(const float4* __restrict__ acc, const int* __restrict__ flags, float4* __restrict__ gv,
+                         MpmParams prm, ColliderSet cs, float time, float dt, float4* __restrict__ next,
+                         int* __restrict__ next_flags) {
+    const int ng = prm.n_grid, nb = (ng + 7) / 8;
+    const int blk = blockIdx.x;
+    const int mark = flags[blk];
+    const int dirty = next_flags ? next_flags[blk] : 0;
+    if (!mark && !dirty) return;
+    const int t = threadIdx.x;
+    const int bz = blk % nb, by = (blk / nb) % nb, bx = blk / (nb * nb);
+    const int gx = bx * 8 + (t >> 6), gy = by * 8 + ((t >> 3) & 7), gz = bz * 8 + (t & 7);
+    if (gx < ng && gy < ng && gz < ng) {
+        const int64_t node = ((int64_t)gx * ng + gy) * ng + gz;
+        if (mark) gv[node] = grid_node_update(acc[node], gx, gy, gz, ng, prm.dx, dt, prm.gravity, cs, time);
+        if (dirty) next[node] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    if (dirty) {
+        __syncthreads();   // every thread has read the mark
+        if (t == 0) next_flags[blk] = 0;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_zero_ints(int* __restrict__ p, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
 // --------------------------------------------------------------------------
 // g2p_apic_and_stress (mpm_utils.py:241-306) + update_x_from_u / _u_from_x.
 // dpos = (i,j,k) - fx in grid units; C += v_i (x) dpos * (w 4 inv_dx);
@@ -293,6 +326,31 @@ int grid_update_zero_next(const MpmParams* prm, const float* grid_acc, float* gr
     return check_launch("k_grid_update");
 }
 
+int grid_update_blocks(const MpmParams* prm, float* set, const MpmCollider* colliders, int n_colliders, float time,
+                       float dt, float* next, void* stream) {
+    int rc = check_params(prm);
+    if (rc) return rc;
+    if (!set) return set_err(-7, "grid set is NULL");
+    if (next == set) return set_err(-7, "grid set aliases the next one");
+    ColliderSet cs;
+    if ((rc = make_collider_set(colliders, n_colliders, &cs))) return rc;
+    const int ng = prm->n_grid;
+    const int64_t nn = (int64_t)ng * ng * ng;
+    hipLaunchKernelGGL(k_grid_update_blocks, dim3((unsigned)grid_blocks(ng)), dim3(512), 0, (hipStream_t)stream,
+                       (const float4*)set, (const int*)grid_set_flags(set, ng), (float4*)(set + 4 * nn), *prm, cs, time,
+                       dt, (float4*)next, next ? grid_set_flags(next, ng) : (int*)nullptr);
+    return check_launch("k_grid_update_blocks");
+}
+
+int clear_grid_set(const MpmParams* prm, float* set, void* stream) {
+    int rc = mpm_zero_grid(prm, set, stream);
+    if (rc) return rc;
+    const int64_t nbk = grid_blocks(prm->n_grid);
+    hipLaunchKernelGGL(k_zero_ints, dim3(grid_1d(nbk)), dim3(kBlock), 0, (hipStream_t)stream,
+                       grid_set_flags(set, prm->n_grid), nbk);
+    return check_launch("k_zero_ints");
+}
+
 }  // namespace mpm
 
 extern "C" {
@@ -346,6 +404,12 @@ int64_t mpm_workspace_bytes(const MpmParams* prm, int64_t n_particles) {
     (void)n_particles;
     int64_t nn = (int64_t)prm->n_grid * prm->n_grid * prm->n_grid;
     return 2 * nn * 16;  // accumulator + velocity grid
+}
+
+int64_t mpm_step_grids_bytes(const MpmParams* prm) {
+    if (check_grid(prm)) return -1;
+    const int64_t nn = (int64_t)prm->n_grid * prm->n_grid * prm->n_grid;
+    return 2 * nn * 16 + ((grid_blocks(prm->n_grid) * 4 + 255) / 256) * 256;   // accumulator, velocities, marks
 }
 
 int mpm_step(const MpmParams* prm, const MpmState* st, const MpmCollider* colliders,

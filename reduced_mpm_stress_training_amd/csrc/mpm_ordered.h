@@ -170,10 +170,43 @@ __device__ __forceinline__ void chunk_flush(const double* s_t, const int o[3], i
     }
 }
 
+// Grid sets of the ordered path (mpm_step_grids_bytes): the P2G accumulator
+// (n^3 float4), the grid velocities (n^3 float4), then one int per 8^3 grid
+// block, nonzero where a P2G into this accumulator may have written.  The
+// P2G kernels mark the blocks they touch; the grid update and the grid
+// adjoint visit only marked blocks, and an accumulator is cleared block by
+// block from its marks before it takes the next P2G.
+inline int64_t grid_blocks(int ng) {
+    const int64_t nb = (ng + 7) / 8;
+    return nb * nb * nb;
+}
+inline int* grid_set_flags(float* set, int ng) { return (int*)(set + 8 * (int64_t)ng * ng * ng); }
+
+// Marks the grid blocks a chunk's P2G tile (nodes 8B-1 .. 8B+10 of its bin B
+// along each axis: blocks B-1 .. B+1) may write.  tid: index in the workgroup.
+__device__ __forceinline__ void mark_chunk_blocks(int* __restrict__ flags, int bin, int nb, int tid) {
+    if (tid < 27) {
+        const int x = bin / (nb * nb) + tid / 9 - 1, y = (bin / nb) % nb + (tid / 3) % 3 - 1,
+                  z = bin % nb + tid % 3 - 1;
+        if (x >= 0 && y >= 0 && z >= 0 && x < nb && y < nb && z < nb) flags[(x * nb + y) * nb + z] = 1;
+    }
+}
+
+// Marks the blocks of one particle's 3^3 stencil at `base` (its P2G outside
+// the chunk's tile).
+__device__ __forceinline__ void mark_stencil_blocks(int* __restrict__ flags, const int base[3], int nb) {
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const int x = (base[0] + 2 * (c >> 2)) >> 3, y = (base[1] + 2 * ((c >> 1) & 1)) >> 3, z = (base[2] + 2 * (c & 1)) >> 3;
+        flags[(x * nb + y) * nb + z] = 1;
+    }
+}
+
 // host: the chunked P2G (p2g_apic_stress_given) of an ordered state into a
-// zeroed float4 accumulator grid (mpm_ordered.hip)
+// zeroed float4 accumulator grid (mpm_ordered.hip); flags: the grid set's
+// block marks, or NULL
 int p2g_ordered(const MpmParams* prm, const MpmState* st, const void* order, float* grid_acc, float dt,
-                void* stream);
+                void* stream, int* flags = nullptr);
 
 // host: the fused G2P (grid velocities grid_v) + next P2G (into the zeroed
 // acc_next) over the chunks; dst may be src (in place) or another state of
@@ -183,6 +216,16 @@ int p2g_ordered(const MpmParams* prm, const MpmState* st, const void* order, flo
 // state of a recorded segment (mpm_ordered.hip)
 int g2p2g_ordered(const MpmParams* prm, const MpmState* src, const MpmState* dst, const void* order,
                   const float* grid_v, float* acc_next, float dt, const float* cst, bool copy_const,
-                  void* stream);
+                  void* stream, int* flags_next = nullptr);
+
+// host: the grid update of the marked blocks of grid set `set` (its
+// velocities from its accumulator), and the clearing of the blocks of
+// `next` (a grid set, or NULL) that its earlier P2G marked, marks included
+// (mpm_kernels.hip)
+int grid_update_blocks(const MpmParams* prm, float* set, const MpmCollider* colliders, int n_colliders, float time,
+                       float dt, float* next, void* stream);
+
+// host: the whole accumulator and the marks of grid set `set` to zero
+int clear_grid_set(const MpmParams* prm, float* set, void* stream);
 
 }  // namespace mpm

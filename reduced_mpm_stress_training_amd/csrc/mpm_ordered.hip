@@ -13,6 +13,7 @@
 // particles drift (a particle whose stencil leaves its chunk's tile scatters
 // straight to the grid), so a caller sorts once and steps many substeps.
 #include <hip/hip_runtime.h>
+#include <type_traits>
 #include <limits.h>
 
 #include "mpm_common.h"
@@ -245,7 +246,8 @@ __global__ void __launch_bounds__(kChunkWG)
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kChunkWG)
     k_p2g_chunks(const float* __restrict__ planes, int64_t S, int n, const int4* __restrict__ chunks,
-                 const int* __restrict__ meta, MpmParams prm, float dt, float* __restrict__ grid) {
+                 const int* __restrict__ meta, MpmParams prm, float dt, float* __restrict__ grid,
+                 int* __restrict__ flags) {
     __shared__ double s_t[4 * kCT3];
     const int c = blockIdx.x;
     if (c >= meta[0]) return;
@@ -282,14 +284,18 @@ __global__ void __launch_bounds__(kChunkWG)
                 atomicAdd(&s_t[cc * kCT3 + ti + (i3 * kCT + j3) * kCT + k3], (double)val);
             });
         } else {
-            p2g_contrib(A, m, vol, v, Cm, tau, prm.dx, prm.inv_dx, dt, [&](int cc, int i3, int j3, int k3, float val) {
-                const int64_t node = ((int64_t)(A.base[0] + i3) * ng + (A.base[1] + j3)) * ng + (A.base[2] + k3);
-                unsafeAtomicAdd(grid + 4 * node + cc, val);
-            });
+            p2g_contrib_rolled(A, m, vol, v, Cm, tau, prm.dx, prm.inv_dx, dt,
+                               [&](int cc, int i3, int j3, int k3, float val) {
+                                   const int64_t node =
+                                       ((int64_t)(A.base[0] + i3) * ng + (A.base[1] + j3)) * ng + (A.base[2] + k3);
+                                   unsafeAtomicAdd(grid + 4 * node + cc, val);
+                               });
+            if (flags) mark_stencil_blocks(flags, A.base, (ng + 7) / 8);
         }
     }
     __syncthreads();
     chunk_flush<4>(s_t, o, ng, grid);
+    if (flags) mark_chunk_blocks(flags, ch.z, (ng + 7) / 8, tid);
 }
 
 // ---------------------------------------------------------------------------
@@ -356,12 +362,16 @@ template <typename Nodes>
 __device__ __forceinline__ void g2p2g_flat_particle(const FlatIn& q, bool copy, float* dst, int64_t S, int p,
                                                     const float x[3], const AxisWeights& A, const Nodes& V,
                                                     const MpmParams& prm, float dt, double* s_acc, const int o2[3],
-                                                    float* __restrict__ acc_next, int* __restrict__ status) {
+                                                    float* __restrict__ acc_next, int* __restrict__ flags_next,
+                                                    int* __restrict__ status) {
     const int ng = prm.n_grid;
     float* dp = dst + p;
     const float* x0 = q.x0;
     float v[3], Cm[9], G[9], F[9], tau[9];
-    gather_tile<9, 3>(V, 0, A, prm.inv_dx, v, Cm, G);
+    if constexpr (std::is_same<Nodes, OrdNodesGrid>::value)
+        gather_global_rolled(V.gv + V.gb, V.ng, A, prm.inv_dx, v, Cm, G);   // few particles: small live set
+    else
+        gather_tile<9, 3>(V, 0, A, prm.inv_dx, v, Cm, G);
 #pragma unroll
     for (int k = 0; k < 9; ++k) F[k] = q.F[k];
     const float m = q.m, vol = q.vol;
@@ -398,16 +408,25 @@ __device__ __forceinline__ void g2p2g_flat_particle(const FlatIn& q, bool copy, 
     }
     const int t2 = chunk_tile_index(B.base, o2);
     const int64_t gb = ((int64_t)B.base[0] * ng + B.base[1]) * ng + B.base[2];
-    p2g_contrib(B, m, vol, v, Cm, tau, prm.dx, prm.inv_dx, dt, [&](int cc, int i3, int j3, int k3, float val) {
-        if (t2 >= 0) atomicAdd(&s_acc[cc * kCT3 + t2 + (i3 * kCT + j3) * kCT + k3], (double)val);
-        else unsafeAtomicAdd(acc_next + 4 * (gb + ((int64_t)i3 * ng + j3) * ng + k3) + cc, val);
-    });
+    if (t2 >= 0) {
+        p2g_contrib(B, m, vol, v, Cm, tau, prm.dx, prm.inv_dx, dt, [&](int cc, int i3, int j3, int k3, float val) {
+            atomicAdd(&s_acc[cc * kCT3 + t2 + (i3 * kCT + j3) * kCT + k3], (double)val);
+        });
+    } else {
+        // outside the tile (few particles): the same values with a small live set
+        p2g_contrib_rolled(B, m, vol, v, Cm, tau, prm.dx, prm.inv_dx, dt,
+                           [&](int cc, int i3, int j3, int k3, float val) {
+                               unsafeAtomicAdd(acc_next + 4 * (gb + ((int64_t)i3 * ng + j3) * ng + k3) + cc, val);
+                           });
+        if (flags_next) mark_stencil_blocks(flags_next, B.base, (ng + 7) / 8);
+    }
 }
 
 __global__ void __launch_bounds__(kChunkWG) __attribute__((amdgpu_waves_per_eu(2, 2)))
     k_g2p2g_chunks(const float* planes, const float* cst, float* dst, int copy_const, int64_t S, int n,
                    const int4* __restrict__ chunks, const int* __restrict__ meta, MpmParams prm, float dt,
-                   const float4* __restrict__ gv, float* __restrict__ acc_next, int* __restrict__ status) {
+                   const float4* __restrict__ gv, float* __restrict__ acc_next, int* __restrict__ flags_next,
+                   int* __restrict__ status) {
     __shared__ float s_v[3 * kCT3];
     __shared__ double s_acc[4 * kCT3];
     const int c = blockIdx.x;
@@ -461,7 +480,7 @@ __global__ void __launch_bounds__(kChunkWG) __attribute__((amdgpu_waves_per_eu(2
             outside |= 1u << r;
             continue;
         }
-        g2p2g_flat_particle(cur, copy, dst, S, p, x, A, OrdNodesTile{s_v, ti}, prm, dt, s_acc, o2, acc_next,
+        g2p2g_flat_particle(cur, copy, dst, S, p, x, A, OrdNodesTile{s_v, ti}, prm, dt, s_acc, o2, acc_next, flags_next,
                             status);
     }
     for (int i = tid, r = 0; outside && i < n_per; i += kChunkWG, ++r) {
@@ -474,10 +493,11 @@ __global__ void __launch_bounds__(kChunkWG) __attribute__((amdgpu_waves_per_eu(2
         AxisWeights A;
         axis_weights(x, prm.inv_dx, A);
         const OrdNodesGrid V{gv, ((int64_t)A.base[0] * ng + A.base[1]) * ng + A.base[2], ng};
-        g2p2g_flat_particle(q, copy, dst, S, p, x, A, V, prm, dt, s_acc, o2, acc_next, status);
+        g2p2g_flat_particle(q, copy, dst, S, p, x, A, V, prm, dt, s_acc, o2, acc_next, flags_next, status);
     }
     __syncthreads();
     chunk_flush<4>(s_acc, o2, ng, acc_next);
+    if (flags_next) mark_chunk_blocks(flags_next, ch.z, (ng + 7) / 8, tid);
 }
 
 __global__ void __launch_bounds__(kB)
@@ -505,7 +525,7 @@ int check_flat(const MpmState* st) {
 namespace mpm {
 
 int p2g_ordered(const MpmParams* prm, const MpmState* st, const void* order, float* grid_acc, float dt,
-                void* stream) {
+                void* stream, int* flags) {
     int rc = check_sand(prm);
     if (rc) return rc;
     if ((rc = check_flat(st))) return rc;
@@ -516,13 +536,13 @@ int p2g_ordered(const MpmParams* prm, const MpmState* st, const void* order, flo
     const char* ob = (const char*)order;
     hipLaunchKernelGGL(k_p2g_chunks, dim3((unsigned)L.max_chunks), dim3(kChunkWG), 0, (hipStream_t)stream,
                        (const float*)st->planes, st->stride, (int)st->n, (const int4*)(ob + L.off_chunks),
-                       (const int*)(ob + L.off_meta), *prm, dt, grid_acc);
+                       (const int*)(ob + L.off_meta), *prm, dt, grid_acc, flags);
     return check_launch("k_p2g_chunks");
 }
 
 int g2p2g_ordered(const MpmParams* prm, const MpmState* src, const MpmState* dst, const void* order,
                   const float* grid_v, float* acc_next, float dt, const float* cst, bool copy_const,
-                  void* stream) {
+                  void* stream, int* flags_next) {
     int rc = check_sand(prm);
     if (rc) return rc;
     if ((rc = check_flat(src)) || (rc = check_flat(dst))) return rc;
@@ -536,7 +556,7 @@ int g2p2g_ordered(const MpmParams* prm, const MpmState* src, const MpmState* dst
     hipLaunchKernelGGL(k_g2p2g_chunks, dim3((unsigned)L.max_chunks), dim3(kChunkWG), 0, (hipStream_t)stream,
                        (const float*)src->planes, cst ? cst : (const float*)src->planes, (float*)dst->planes,
                        copy_const ? 1 : 0, src->stride, (int)src->n, (const int4*)(ob + L.off_chunks),
-                       (const int*)(ob + L.off_meta), *prm, dt, (const float4*)grid_v, acc_next, status);
+                       (const int*)(ob + L.off_meta), *prm, dt, (const float4*)grid_v, acc_next, flags_next, status);
     return check_launch("k_g2p2g_chunks");
 }
 
@@ -612,8 +632,7 @@ int mpm_step_ordered(const MpmParams* prm, const MpmState* st, const void* order
 
 int64_t mpm_run_ordered_workspace_bytes(const MpmParams* prm) {
     if (check_grid(prm)) return -1;
-    const int64_t nn = (int64_t)prm->n_grid * prm->n_grid * prm->n_grid;
-    return 3 * nn * 16;   // two P2G accumulators (this substep, the next) and the grid velocities
+    return 2 * mpm_step_grids_bytes(prm);   // two grid sets (this substep's, the next's)
 }
 
 int mpm_run_ordered(const MpmParams* prm, const MpmState* st, const void* order, const MpmCollider* colliders,
@@ -627,18 +646,22 @@ int mpm_run_ordered(const MpmParams* prm, const MpmState* st, const void* order,
         return set_err(-10, "workspace too small (mpm_run_ordered_workspace_bytes)");
     if (((uintptr_t)workspace) & 255) return set_err(-11, "workspace not 256-byte aligned");
     if (n_sub == 0) return 0;
-    const int64_t nn = (int64_t)prm->n_grid * prm->n_grid * prm->n_grid;
-    float* acc = (float*)workspace;
-    float* gv = acc + 4 * nn;
-    float* acc2 = gv + 4 * nn;
-    if ((rc = mpm_zero_grid(prm, acc, stream))) return rc;
-    if ((rc = p2g_ordered(prm, st, order, acc, (float)dt, stream))) return rc;
+    const int ng = prm->n_grid;
+    const int64_t nn = (int64_t)ng * ng * ng;
+    float* set = (float*)workspace;
+    float* set2 = set + mpm_step_grids_bytes(prm) / 4;
+    // both sets cleared whole (marks included); from then on each grid
+    // update clears the blocks the other set's earlier P2G marked
+    if ((rc = clear_grid_set(prm, set, stream)) || (rc = clear_grid_set(prm, set2, stream))) return rc;
+    if ((rc = p2g_ordered(prm, st, order, set, (float)dt, stream, grid_set_flags(set, ng)))) return rc;
     for (int j = 0; j < n_sub; ++j) {
-        if ((rc = grid_update_zero_next(prm, acc, gv, colliders, n_colliders, (float)((double)(k0 + j) * dt),
-                                        (float)dt, acc2, stream)))
+        if ((rc = grid_update_blocks(prm, set, colliders, n_colliders, (float)((double)(k0 + j) * dt), (float)dt, set2,
+                                     stream)))
             return rc;
-        if ((rc = g2p2g_ordered(prm, st, st, order, gv, acc2, (float)dt, nullptr, false, stream))) return rc;
-        float* t = acc; acc = acc2; acc2 = t;
+        if ((rc = g2p2g_ordered(prm, st, st, order, set + 4 * nn, set2, (float)dt, nullptr, false, stream,
+                                grid_set_flags(set2, ng))))
+            return rc;
+        float* t = set; set = set2; set2 = t;
     }
     return 0;
 }

@@ -96,7 +96,11 @@ def test_segment_entry_points_reject_bad_arguments():
     for k in range(2):
         st[k].n, st[k].stride, st[k].planes = 100, 128, 0x100000 * (k + 1)
     grids = (Ct.c_void_p * 2)(0x400000, 0x800000)
-    order, grid_bytes = 0x1000000, 16 * 32 ** 3
+    order, grid_bytes = 0x1000000, lib.mpm_step_grids_bytes(prm)
+    # accumulator and velocities (32^3 float4 each), 4^3 block marks padded to 256 bytes
+    assert grid_bytes == 2 * 16 * 32 ** 3 + 256
+    assert lib.mpm_adjoint_workspace_bytes(prm) == grid_bytes + 2 * 16 * 32 ** 3
+    assert lib.mpm_run_ordered_workspace_bytes(prm) == 2 * grid_bytes
     last = L.MpmState()
     last.n, last.stride, last.planes = 100, 128, st[1].planes
     rec = lambda n_sub, last_p, nxt, nbytes: lib.mpm_record_segment(prm, st, order, grids, cols, 0, 0, 1e-4, n_sub, 0,
