@@ -412,11 +412,40 @@ __device__ __forceinline__ void load_adj_in(const float* __restrict__ st, const 
     load9(an + p, SA, kAF, q.Fbn);
 }
 
-template <typename Nodes, typename Emit>
+// The x, v, C, tau adjoints of S_{n+1} as the G2P adjoint reads them: from
+// the adjoint planes (AdjPlanes: ap = an + p), or from registers (AdjRegs:
+// the fused kernel has just completed them with the P2G adjoint of
+// substep n+1).
+struct AdjPlanes {
+    const float* ap;
+    int64_t SA;
+    __device__ __forceinline__ void tau(float o[9]) const { load9(ap, SA, kAT, o); }
+    __device__ __forceinline__ void C(float o[9]) const { load9(ap, SA, kAC, o); }
+    __device__ __forceinline__ float x(int d) const { return ap[(kAX + d) * SA]; }
+    __device__ __forceinline__ float v(int d) const { return ap[(kAV + d) * SA]; }
+};
+struct AdjNext {
+    float x[3], v[3], C[9], tau[9];
+};
+struct AdjRegs {
+    const AdjNext& a;
+    __device__ __forceinline__ void tau(float o[9]) const {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) o[k] = a.tau[k];
+    }
+    __device__ __forceinline__ void C(float o[9]) const {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) o[k] = a.C[k];
+    }
+    __device__ __forceinline__ float x(int d) const { return a.x[d]; }
+    __device__ __forceinline__ float v(int d) const { return a.v[d]; }
+};
+
+template <typename Nodes, typename Src, typename Emit>
 __device__ __forceinline__ void g2p_adj_particle(const AdjIn& q, int p, const MpmParams& prm, float dt,
-                                                 const AxisWeights& Aw, const Nodes& V, const float* __restrict__ an,
-                                                 int64_t SA, float* __restrict__ ac, float& mub, float& lamb,
-                                                 float& alb, Emit&& emit) {
+                                                 const AxisWeights& Aw, const Nodes& V, const Src& an, int64_t SA,
+                                                 float* __restrict__ ac, float& mub, float& lamb, float& alb,
+                                                 Emit&& emit) {
     const float inv_dx = prm.inv_dx;
     const float* x = q.x;
     float G[9];
@@ -424,19 +453,18 @@ __device__ __forceinline__ void g2p_adj_particle(const AdjIn& q, int p, const Mp
         float v[3], Cm[9];
         gather_tile<9, 3>(V, 0, Aw, inv_dx, v, Cm, G);
     }
-    const float* ap = an + p;
     float xb[3], vt[3], Cb[9], Gb[9], Fb[9];
     {
         float Tb[9];
-        load9(ap, SA, kAT, Tb);
+        an.tau(Tb);
         sand_update_adj(G, q.F, dt, prm.mu, prm.lam, prm.alpha, q.Fbn, Tb, Gb, Fb, mub, lamb, alb);
     }
-    load9(ap, SA, kAC, Cb);
+    an.C(Cb);
     // x' = x + dt v'
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-        xb[d] = ap[(kAX + d) * SA];
-        vt[d] = ap[(kAV + d) * SA] + dt * xb[d];
+        xb[d] = an.x(d);
+        vt[d] = an.v(d) + dt * xb[d];
     }
     float fx[3];
 #pragma unroll
@@ -460,17 +488,92 @@ __device__ __forceinline__ void frozen_adj(const float* __restrict__ an, int64_t
     for (int k = 0; k < MPM_ADJ_NPLANES; ++k) ac[k * SA + p] = an[k * SA + p];
 }
 
+// P2G adjoint of one particle of S_n whose stencil base lies inside the
+// grid (defined with k_p2g_adj below): the v, C, tau adjoints of S_n and the
+// P2G part of its position adjoint.
+struct P2GAdjOut {
+    float v[3], C[9], tau[9], xg[3];
+};
+__device__ __forceinline__ void p2g_adj_particle(const float* __restrict__ sp, const float* __restrict__ kp,
+                                                 int64_t S, const float fx[3], int64_t b0, const MpmParams& prm,
+                                                 float dt, const float4* __restrict__ gb, P2GAdjOut& o);
+
+// position, stencil base and fractional offsets of particle p of S_n as the
+// P2G (and its adjoint) takes them; false when the stencil leaves the grid
+// (a frozen particle)
+__device__ __forceinline__ bool p2g_base(const float* __restrict__ st, const float* __restrict__ cst, int64_t S,
+                                         int p, const MpmParams& prm, float fx[3], int64_t* b0) {
+    const int ng = prm.n_grid;
+    int base[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const float gp = (cst[(MPM_P_X0 + d) * S + p] + st[(MPM_P_U + d) * S + p]) * prm.inv_dx;
+        base[d] = (int)(gp - 0.5f);
+        fx[d] = gp - (float)base[d];
+    }
+    *b0 = ((int64_t)base[0] * ng + base[1]) * ng + base[2];
+    return base[0] >= 0 && base[1] >= 0 && base[2] >= 0 && base[0] <= ng - 3 && base[1] <= ng - 3 &&
+           base[2] <= ng - 3;
+}
+
+// The fused adjoint of one particle: the P2G adjoint of substep n+1 (on
+// S_{n+1} = st_next, grid adjoint gb_next) completes the adjoint of S_{n+1},
+// of which the planes an hold the part the G2P adjoint of substep n+1 wrote
+// (x, F; every plane for a particle frozen at n+1); the G2P adjoint of
+// substep n then takes it from registers.
+template <typename Nodes, typename Emit>
+__device__ __forceinline__ void fused_adj_particle(const AdjIn& q, int p, const float* __restrict__ st_next,
+                                                   const float* __restrict__ cst, int64_t S,
+                                                   const float4* __restrict__ gb_next, const MpmParams& prm, float dt,
+                                                   const AxisWeights& Aw, const Nodes& V,
+                                                   const float* __restrict__ an, int64_t SA, float* __restrict__ ac,
+                                                   float& mub, float& lamb, float& alb, Emit&& emit) {
+    const float* ap = an + p;
+    AdjNext a;
+    float fx[3];
+    int64_t b0;
+    if (p2g_base(st_next, cst, S, p, prm, fx, &b0)) {
+        P2GAdjOut o;
+        p2g_adj_particle(st_next + p, cst + p, S, fx, b0, prm, dt, gb_next, o);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            a.x[d] = ap[(kAX + d) * SA] + o.xg[d];
+            a.v[d] = o.v[d];
+        }
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            a.C[k] = o.C[k];
+            a.tau[k] = o.tau[k];
+        }
+    } else {
+        // frozen at substep n+1: its G2P adjoint passed every plane through
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            a.x[d] = ap[(kAX + d) * SA];
+            a.v[d] = ap[(kAV + d) * SA];
+        }
+        load9(ap, SA, kAC, a.C);
+        load9(ap, SA, kAT, a.tau);
+    }
+    g2p_adj_particle(q, p, prm, dt, Aw, V, AdjRegs{a}, SA, ac, mub, lamb, alb, emit);
+}
+
 // G2P adjoint over the chunks of an ordered state (mpm_sort_by_bin): the
 // chunk's grid velocities are staged in an LDS tile at its origin
 // (chunk_origin); the grid-velocity adjoint is accumulated in an FP64 LDS tile there and
 // flushed once per node (chunk_flush); particles whose stencil leaves the
-// tile gather from and add to the global grids directly.
+// tile gather from and add to the global grids directly.  kFused: each
+// particle first runs the P2G adjoint of substep n+1 (fused_adj_particle;
+// st_next, gb_next), which spares k_p2g_adj's pass over the adjoint planes
+// between the substeps of a segment.
+template <bool kFused>
 __global__ void __launch_bounds__(kChunkWG)
     k_g2p_adj_chunks(const float* __restrict__ st, const float* __restrict__ cst, int64_t S, int n,
                      const int4* __restrict__ chunks,
                      const int* __restrict__ meta, MpmParams prm, float dt, const float4* __restrict__ gv,
                      const float* __restrict__ an, int64_t SA, float* __restrict__ vb, float* __restrict__ ac,
-                     double* __restrict__ pb) {
+                     double* __restrict__ pb, const float* __restrict__ st_next,
+                     const float4* __restrict__ gb_next) {
     __shared__ double s_t[3 * kCT3];
     __shared__ float s_v[3 * kCT3];
     __shared__ float s_par[3 * (kChunkWG / 64)];
@@ -515,10 +618,15 @@ __global__ void __launch_bounds__(kChunkWG)
             outside |= 1u << r;
             continue;
         }
-        g2p_adj_particle(cur, p, prm, dt, Aw, TileNodes{s_v, ti}, an, SA, ac, mub, lamb, alb,
-                         [&](int a, int i3, int j3, int k3, float val) {
-                             atomicAdd(&s_t[a * kCT3 + ti + (i3 * kCT + j3) * kCT + k3], (double)val);
-                         });
+        auto emit = [&](int a, int i3, int j3, int k3, float val) {
+            atomicAdd(&s_t[a * kCT3 + ti + (i3 * kCT + j3) * kCT + k3], (double)val);
+        };
+        if constexpr (kFused)
+            fused_adj_particle(cur, p, st_next, cst, S, gb_next, prm, dt, Aw, TileNodes{s_v, ti}, an, SA, ac, mub,
+                               lamb, alb, emit);
+        else
+            g2p_adj_particle(cur, p, prm, dt, Aw, TileNodes{s_v, ti}, AdjPlanes{an + p, SA}, SA, ac, mub, lamb, alb,
+                             emit);
     }
     for (int i = tid, r = 0; outside && i < n_per; i += kChunkWG, ++r) {
         if (!(outside & (1u << r))) continue;
@@ -527,10 +635,15 @@ __global__ void __launch_bounds__(kChunkWG)
         AxisWeights Aw;
         axis_weights(q.x, prm.inv_dx, Aw);
         const int64_t gb = ((int64_t)Aw.base[0] * ng + Aw.base[1]) * ng + Aw.base[2];
-        g2p_adj_particle(q, p, prm, dt, Aw, GridNodes{gv, gb, ng}, an, SA, ac, mub, lamb, alb,
-                         [&](int a, int i3, int j3, int k3, float val) {
-                             unsafeAtomicAdd(vb + 4 * (gb + ((int64_t)i3 * ng + j3) * ng + k3) + a, val);
-                         });
+        auto emit = [&](int a, int i3, int j3, int k3, float val) {
+            unsafeAtomicAdd(vb + 4 * (gb + ((int64_t)i3 * ng + j3) * ng + k3) + a, val);
+        };
+        if constexpr (kFused)
+            fused_adj_particle(q, p, st_next, cst, S, gb_next, prm, dt, Aw, GridNodes{gv, gb, ng}, an, SA, ac, mub,
+                               lamb, alb, emit);
+        else
+            g2p_adj_particle(q, p, prm, dt, Aw, GridNodes{gv, gb, ng}, AdjPlanes{an + p, SA}, SA, ac, mub, lamb, alb,
+                             emit);
     }
     block_stage_params(mub, lamb, alb, s_par);
     __syncthreads();
@@ -733,37 +846,20 @@ __device__ __forceinline__ void p2g_adj_sums(const float4* __restrict__ gb, int6
     }
 }
 
-__global__ void __launch_bounds__(kBlk)
-    k_p2g_adj(const float* __restrict__ st, const float* __restrict__ cst, int64_t S, int n, MpmParams prm, float dt,
-              const float4* __restrict__ gb, float* __restrict__ ac, int64_t SA) {
-    const int p = blockIdx.x * kBlk + threadIdx.x;
-    if (p >= n) return;
-    const float* sp = st + p;
-    const float* kp = cst + p;   // constant planes
+__device__ __forceinline__ void p2g_adj_particle(const float* __restrict__ sp, const float* __restrict__ kp,
+                                                 int64_t S, const float fx[3], int64_t b0, const MpmParams& prm,
+                                                 float dt, const float4* __restrict__ gb, P2GAdjOut& o) {
     const int ng = prm.n_grid;
     const float inv_dx = prm.inv_dx, dxp = prm.dx;
-    float x[3];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) x[d] = kp[(MPM_P_X0 + d) * S] + sp[(MPM_P_U + d) * S];
-    int base[3];
-    float fx[3];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        const float gp = x[d] * inv_dx;
-        base[d] = (int)(gp - 0.5f);
-        fx[d] = gp - (float)base[d];
-    }
-    if (!(base[0] >= 0 && base[1] >= 0 && base[2] >= 0 && base[0] <= ng - 3 && base[1] <= ng - 3 &&
-          base[2] <= ng - 3))
-        return;   // frozen: k_g2p_adj_chunks passed v, C, tau through
     const AxisF5 X = axis_f5(fx[0]), Y = axis_f5(fx[1]), Z = axis_f5(fx[2]);
-    const int64_t b0 = ((int64_t)base[0] * ng + base[1]) * ng + base[2];
     float r[3][kNSum], rm[3];
     p2g_adj_sums(gb, b0, ng, X, Y, Z, r, rm);
     const float m = kp[MPM_P_MASS * S], vol = kp[MPM_P_VOL * S];
     const float ct = dt * vol * inv_dx * inv_dx;
-    float* cp = ac + p;
-    float xg[3] = {m * inv_dx * rm[0], m * inv_dx * rm[1], m * inv_dx * rm[2]};   // the mass adjoint m_b
+    float* xg = o.xg;
+    xg[0] = m * inv_dx * rm[0];   // the mass adjoint m_b
+    xg[1] = m * inv_dx * rm[1];
+    xg[2] = m * inv_dx * rm[2];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         const float* ra = r[a];
@@ -774,13 +870,13 @@ __global__ void __launch_bounds__(kBlk)
             C[d] = sp[(MPM_P_C + a * 3 + d) * S];
             tau[d] = sp[(MPM_P_TAU + a * 3 + d) * S];
         }
-        cp[(kAV + a) * SA] = m * ra[kS0];
-        cp[(kAC + a * 3 + 0) * SA] = m * dxp * ra[kEx];
-        cp[(kAC + a * 3 + 1) * SA] = m * dxp * ra[kEy];
-        cp[(kAC + a * 3 + 2) * SA] = m * dxp * ra[kEz];
-        cp[(kAT + a * 3 + 0) * SA] = -dt * vol * inv_dx * ra[kDx];
-        cp[(kAT + a * 3 + 1) * SA] = -dt * vol * inv_dx * ra[kDy];
-        cp[(kAT + a * 3 + 2) * SA] = -dt * vol * inv_dx * ra[kDz];
+        o.v[a] = m * ra[kS0];
+        o.C[a * 3 + 0] = m * dxp * ra[kEx];
+        o.C[a * 3 + 1] = m * dxp * ra[kEy];
+        o.C[a * 3 + 2] = m * dxp * ra[kEz];
+        o.tau[a * 3 + 0] = -dt * vol * inv_dx * ra[kDx];
+        o.tau[a * 3 + 1] = -dt * vol * inv_dx * ra[kDy];
+        o.tau[a * 3 + 2] = -dt * vol * inv_dx * ra[kDz];
         const float mv = m * va * inv_dx;
         xg[0] += mv * ra[kDx] + m * (C[0] * ra[kPx] + C[1] * ra[kMxy] + C[2] * ra[kMxz] - C[0] * ra[kS0]) -
                  ct * (tau[0] * ra[kDDx] + tau[1] * ra[kHxy] + tau[2] * ra[kHxz]);
@@ -789,8 +885,30 @@ __global__ void __launch_bounds__(kBlk)
         xg[2] += mv * ra[kDz] + m * (C[0] * ra[kMzx] + C[1] * ra[kMzy] + C[2] * ra[kPz] - C[2] * ra[kS0]) -
                  ct * (tau[0] * ra[kHxz] + tau[1] * ra[kHyz] + tau[2] * ra[kDDz]);
     }
+}
+
+__global__ void __launch_bounds__(kBlk)
+    k_p2g_adj(const float* __restrict__ st, const float* __restrict__ cst, int64_t S, int n, MpmParams prm, float dt,
+              const float4* __restrict__ gb, float* __restrict__ ac, int64_t SA) {
+    const int p = blockIdx.x * kBlk + threadIdx.x;
+    if (p >= n) return;
+    float fx[3];
+    int64_t b0;
+    if (!p2g_base(st, cst, S, p, prm, fx, &b0)) return;   // frozen: k_g2p_adj_chunks passed v, C, tau through
+    P2GAdjOut o;
+    p2g_adj_particle(st + p, cst + p, S, fx, b0, prm, dt, gb, o);
+    float* cp = ac + p;
 #pragma unroll
-    for (int d = 0; d < 3; ++d) cp[(kAX + d) * SA] += xg[d];
+    for (int a = 0; a < 3; ++a) {
+        cp[(kAV + a) * SA] = o.v[a];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            cp[(kAC + a * 3 + d) * SA] = o.C[a * 3 + d];
+            cp[(kAT + a * 3 + d) * SA] = o.tau[a * 3 + d];
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) cp[(kAX + d) * SA] += o.xg[d];
 }
 
 __global__ void k_zero4(float4* __restrict__ g, int64_t nn) {
@@ -861,11 +979,11 @@ static int adjoint_substep(const MpmParams* prm, const MpmState* st, const void*
     if (n > 0) {
         const OrderLayout L = order_layout(*prm, n);
         const char* ob = (const char*)order;
-        hipLaunchKernelGGL(k_g2p_adj_chunks, dim3((unsigned)L.max_chunks), dim3(kChunkWG), 0, s,
+        hipLaunchKernelGGL(k_g2p_adj_chunks<false>, dim3((unsigned)L.max_chunks), dim3(kChunkWG), 0, s,
                            (const float*)st->planes, cst ? cst : (const float*)st->planes, st->stride, n,
                            (const int4*)(ob + L.off_chunks),
                            (const int*)(ob + L.off_meta), *prm, dt, (const float4*)gv, adj_next, adj_stride,
-                           (float*)vbg, adj_cur, param_adj);
+                           (float*)vbg, adj_cur, param_adj, (const float*)nullptr, (const float4*)nullptr);
         if ((rc = check_launch("k_g2p_adj_chunks"))) return rc;
     }
     if (flags) {
@@ -948,21 +1066,65 @@ static int record_segment(const MpmParams* prm, const MpmState* states, const vo
 
 // The reverse half: adjoint substeps k0+n_sub-1 .. k0 on the recorded states
 // and grids; adj_a holds the adjoint of S_{k0+n_sub} on entry and that of
-// S_k0 on return.
+// S_k0 on return.  Per substep j: the G2P adjoint (scattering the velocity
+// adjoint of grid j), the grid adjoint over the blocks grid j's P2G marked,
+// then the P2G adjoint of substep j -- which, for j > 0, runs inside the
+// G2P adjoint of substep j-1 (k_g2p_adj_chunks<true>): the adjoint of S_j
+// passes between the two in registers instead of through the adjoint planes.
 static int adjoint_recorded(const MpmParams* prm, const MpmState* states, const void* order, void* const* step_grids,
                             const MpmCollider* colliders, int n_colliders, void* workspace, int k0, double dt,
                             int n_sub, float* adj_a, float* adj_b, int64_t adj_stride, double* param_adj,
                             void* stream) {
-    int rc;
+    int rc = check_sand(prm);
+    if (rc) return rc;
+    const int64_t n64 = states[0].n;
+    if (n64 < 0 || states[0].stride < n64) return set_err(-4, "state invalid");
+    if (n64 > 0x7fffffffLL) return set_err(-6, "more than 2^31-1 particles");
+    if (!workspace || ((uintptr_t)workspace & 255)) return set_err(-11, "workspace NULL or not 256-byte aligned");
+    if (!order || ((uintptr_t)order & 255)) return set_err(-40, "order buffer NULL or not 256-byte aligned");
+    if (!param_adj || adj_stride < n64) return set_err(-30, "adjoint buffers invalid");
+    ColliderSet cs;
+    if ((rc = make_collider_set(colliders, n_colliders, &cs))) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    const int ng = prm->n_grid;
+    const int64_t nn = (int64_t)ng * ng * ng;
+    const int n = (int)n64;
+    const float fdt = (float)dt;
+    const float* cst = (const float*)states[0].planes;   // x0, m, vol, id of every state of the segment
+    float4* vbg = (float4*)((float*)workspace + mpm_step_grids_bytes(prm) / 4);
+    float4* gbg = vbg + nn;
+    const OrderLayout L = order_layout(*prm, n);
+    const char* ob = (const char*)order;
+    hipLaunchKernelGGL(k_zero4, dim3(blocks(nn)), dim3(kBlk), 0, s, vbg, nn);
+    if ((rc = check_launch("k_zero4"))) return rc;
     float* cur = adj_a;
     float* nxt = adj_b;
     for (int j = n_sub - 1; j >= 0; --j) {
-        if ((rc = adjoint_substep(prm, &states[j], order, step_grids[j],
-                                  grid_set_flags((float*)step_grids[j], prm->n_grid), colliders, n_colliders, workspace,
-                                  (float)((double)(k0 + j) * dt), (float)dt, cur, nxt, adj_stride, param_adj,
-                                  j < n_sub - 1, j > 0, (const float*)states[0].planes, stream)))
-            return rc;
+        float* set = (float*)step_grids[j];
+        const MpmState& st = states[j];
+        if (n > 0) {
+            // G2P adjoint of substep j (for j < n_sub-1 with the P2G adjoint of substep j+1 in front)
+            const bool fused = j < n_sub - 1;
+            auto kern = fused ? k_g2p_adj_chunks<true> : k_g2p_adj_chunks<false>;
+            hipLaunchKernelGGL(kern, dim3((unsigned)L.max_chunks), dim3(kChunkWG), 0, s, (const float*)st.planes,
+                               cst, st.stride, n, (const int4*)(ob + L.off_chunks), (const int*)(ob + L.off_meta),
+                               *prm, fdt, (const float4*)(set + 4 * nn), (const float*)cur, adj_stride, (float*)vbg,
+                               nxt, param_adj, fused ? (const float*)states[j + 1].planes : (const float*)nullptr,
+                               fused ? (const float4*)gbg : (const float4*)nullptr);
+            if ((rc = check_launch("k_g2p_adj_chunks"))) return rc;
+        }
+        // grid adjoint of substep j over its marked blocks; clears the velocity adjoint behind it
+        hipLaunchKernelGGL(k_grid_adj_blocks, dim3((unsigned)grid_blocks(ng)), dim3(512), 0, s, (const float4*)set,
+                           (const int*)grid_set_flags(set, ng), (float*)vbg, gbg, *prm, cs,
+                           (float)((double)(k0 + j) * dt), fdt, 1);
+        if ((rc = check_launch("k_grid_adj_blocks"))) return rc;
         float* t = cur; cur = nxt; nxt = t;
+    }
+    // P2G adjoint of the segment's first substep
+    if (n > 0) {
+        hipLaunchKernelGGL(k_p2g_adj, dim3(blocks(n)), dim3(kBlk), 0, s, (const float*)states[0].planes, cst,
+                           states[0].stride, n, *prm, fdt, (const float4*)gbg, cur, adj_stride);
+        if ((rc = check_launch("k_p2g_adj"))) return rc;
     }
     if (cur != adj_a && hipMemcpyAsync(adj_a, cur, (size_t)MPM_ADJ_NPLANES * adj_stride * sizeof(float),
                                        hipMemcpyDeviceToDevice, (hipStream_t)stream) != hipSuccess)
