@@ -78,7 +78,7 @@ def parse():
                     help="column: the 1M-particle headline; batch: 64 uneven scenes advanced together; "
                          "adjoint: forward + adjoint of a differentiable rollout")
     ap.add_argument("--scenes", type=int, default=64)
-    ap.add_argument("--wg-threads", type=int, default=0, choices=[0, 256, 512, 768],
+    ap.add_argument("--wg-threads", type=int, default=0, choices=[0, 256, 384, 512, 768],
                     help="threads per bin workgroup of the fused kernel (0 = library default)")
     ap.add_argument("--sort-period", type=int, default=0, help="substeps between bank-order sorts (0 = default)")
     ap.add_argument("--no-autotune", action="store_true",

@@ -120,10 +120,10 @@ class SceneBatch:
         return torch.cuda.current_stream(self.device).cuda_stream
 
     def set_launch(self, wg_threads: int = 0, sort_period: int = 0):
-        """Threads per bin workgroup (256, 512 or 768; 0 = 768) and substeps
+        """Threads per bin workgroup (256, 384, 512 or 768; 0 = 768) and substeps
         between bank-order sorts (0 = 32, < 0 = never); speed only."""
-        if wg_threads not in (0, 256, 512, 768):
-            raise ValueError("wg_threads must be 0, 256, 512 or 768")
+        if wg_threads not in (0, 256, 384, 512, 768):
+            raise ValueError("wg_threads must be 0, 256, 384, 512 or 768")
         self.bins.wg_threads, self.bins.sort_period = int(wg_threads), int(sort_period)
 
     def _flat(self) -> L.MpmState:

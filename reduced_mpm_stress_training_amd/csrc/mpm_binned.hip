@@ -847,6 +847,13 @@ __device__ __forceinline__ int bank_class(const int base[3], const int b0[3]) {
     return ((((base[0] - b0[0] + 1) * kTS + (base[1] - b0[1] + 1)) * kTS + (base[2] - b0[2] + 1)) & 31);
 }
 
+// Launch shapes of k_g2p2g held at three waves per SIMD (161 VGPRs, no
+// next-particle prefetch): one 768-thread workgroup per CU, or two of 384
+// (their 2 x 76 KB of LDS tiles fit the CU's 160 KB).  256 and 512 threads
+// keep two waves per SIMD and load the next particle ahead.
+template <int WG>
+constexpr bool kThreeWaves = WG == 768 || WG == 384;
+
 // One work item of k_g2p2g: slots [lo, hi) of `bin`, part `part` of
 // `parts` (the bin's passes dealt out in runs of ceil(passes / parts)), with
 // its own velocity and P2G tiles.  Called by every thread of the workgroup.
@@ -905,7 +912,7 @@ __device__ __forceinline__ void g2p2g_item(const SubstepArgs& a, int bin, int pa
         const int64_t offp = off0 + lo;   // the item's first slot
         const int n_in = hi - lo;
         const bool metal = sc.prm.material == 1;
-        if constexpr (WG == 768) {
+        if constexpr (kThreeWaves<WG>) {
             // three waves per SIMD leave no registers for loading the next
             // particle ahead
             for (int it = 0; it < n_in; it += WG) {
@@ -972,18 +979,19 @@ __device__ __forceinline__ void g2p2g_overflow(const SubstepArgs& a, int blk) {
     g2p2g_particle(a, sc, i < n_ovf, off, g.ovf_blk, g.n_bins, -1, nullptr, b0, nullptr, nullptr, q);
 }
 
-// WG: threads per workgroup (256, 512 or 768).  The 76 KB of LDS tiles
-// allow one 768- or 512-thread workgroup per CU, or two of 256 threads.  The
-// kernel needs ~161 VGPRs (its rarely taken paths -- the spill scatter and
-// the overflow run's global gather -- are written with small live sets so
-// that they do not set the budget), so 768 threads give three waves per
-// SIMD; 256 and 512 threads two.  Workgroup w < n_bins runs work item w of
+// WG: threads per workgroup (256, 384, 512 or 768).  The 76 KB of LDS tiles
+// allow one 768- or 512-thread workgroup per CU, or two of 256 or 384
+// threads.  The kernel needs ~161 VGPRs (its rarely taken paths -- the spill
+// scatter and the overflow run's global gather -- are written with small live
+// sets so that they do not set the budget), so 768 threads, or two
+// workgroups of 384, give three waves per SIMD; 256 and 512 threads two.
+// Workgroup w < n_bins runs work item w of
 // the schedule of this epoch (schedule_bins: at most n_bins items, largest
 // first; -1 past the last) or, without one, bin w whole; the workgroups
 // after them run the overflow run in blocks of WG particles.
 template <int WG>
 __global__ void __launch_bounds__(WG)
-    __attribute__((amdgpu_waves_per_eu(WG == 768 ? 3 : 2, WG == 768 ? 3 : 2))) k_g2p2g(SubstepArgs a) {
+    __attribute__((amdgpu_waves_per_eu(kThreeWaves<WG> ? 3 : 2, kThreeWaves<WG> ? 3 : 2))) k_g2p2g(SubstepArgs a) {
     __shared__ float s_vel[3 * kTS3];
     __shared__ double s_acc[4 * kTS3];
     __shared__ int s_stay, s_base, s_live;
@@ -1435,8 +1443,9 @@ int check_bins(const MpmBins* b, int64_t n_bins, int64_t n_slots, int max_cap, B
     if (b->bin_cap < 1 || b->overflow_cap < 0 || b->overflow_cap % b->bin_cap != 0)
         return set_err(-22, "bins: bin_cap < 1 or overflow_cap not a multiple of bin_cap");
     if (b->cur != 0 && b->cur != 1) return set_err(-23, "bins: cur must be 0 or 1");
-    if (b->wg_threads != 0 && b->wg_threads != 256 && b->wg_threads != 512 && b->wg_threads != 768)
-        return set_err(-29, "bins: wg_threads must be 0, 256, 512 or 768");
+    if (b->wg_threads != 0 && b->wg_threads != 256 && b->wg_threads != 384 && b->wg_threads != 512 &&
+        b->wg_threads != 768)
+        return set_err(-29, "bins: wg_threads must be 0, 256, 384, 512 or 768");
     if (n_slots + b->overflow_cap > 0x7fffffffLL) return set_err(-28, "bins: more than 2^31-1 slots");
     if (n_bins >= (1LL << 27)) return set_err(-28, "bins: 2^27 bins or more");
     if (max_cap > 16 * 1000 * 1000) return set_err(-22, "bins: more than 16e6 slots per bin");
@@ -1623,6 +1632,8 @@ int op_g2p2g(const Ctx& c, MpmBins* bins, float dt, hipStream_t s, bool store_al
     a.store_all = store_all ? 1 : 0;
     if (bins->wg_threads == 256)
         hipLaunchKernelGGL(k_g2p2g<256>, dim3(c.g.n_bins + ovf_wgs(c.g, 256)), dim3(256), 0, s, a);
+    else if (bins->wg_threads == 384)
+        hipLaunchKernelGGL(k_g2p2g<384>, dim3(c.g.n_bins + ovf_wgs(c.g, 384)), dim3(384), 0, s, a);
     else if (bins->wg_threads == 512)
         hipLaunchKernelGGL(k_g2p2g<512>, dim3(c.g.n_bins + ovf_wgs(c.g, 512)), dim3(512), 0, s, a);
     else

@@ -4,6 +4,7 @@
 //   svd3_rot          3x3 SVD (rotation U, V)       wp.svd3 at mpm_utils.py:92,297
 //   sand_return_map   Drucker-Prager projection     mpm_utils.py:87-112
 //   dp_stress_sym     Kirchhoff stress of F_e       mpm_utils.py:45-52
+//   sand_update_matfn the two above as functions of F_tr F_tr^T (no SVD)
 //   von_mises_return_map  metal projection          mpm_utils.py:59-84
 //   stvk_stress       StVK (Hencky) stress          mpm_utils.py:36-43
 //   fcr_stress        fixed-corotated stress        mpm_utils.py:29-34
@@ -241,26 +242,201 @@ __device__ __forceinline__ void dp_stress_sym(const float U[9], const float ls[3
         }
 }
 
-// Whole per-particle constitutive update of g2p_apic_and_stress
-// (mpm_utils.py:279-306), sand: F_tr = (I + dt gradv) F, return map, stress.
-__device__ __forceinline__ void sand_update(const float gradv[9], float dt, float F[9],
-                                            float mu, float lam, float alpha, float tau[9]) {
+// ---------------------------------------------------------------------------
+// The sand update without a decomposition.  With F_tr = U diag(s) V^T and
+// B = F_tr F_tr^T = U diag(s^2) U^T, every quantity the return map and the
+// stress take is a function of B alone:
+//   L = 1/2 ln B = U diag(ln s) U^T       (eps_i = ln s_i; tr L = sum eps)
+//   D = L - tr(L)/3 I = U diag(eps_hat) U^T, |eps_hat| = |D|_F
+//   branch 1: F_e = F_tr,                    H = L
+//   branch 2: F_e = U V^T = exp(-L) F_tr,    H = 0
+//   branch 3: F_e = U diag(e^H) V^T = exp(-r D) F_tr, H = L - r D, r = dg/|D|
+//   tau = U diag(2 mu H_i + lam sum H) U^T = 2 mu H + lam tr(H) I
+// (sand_return_mapping mpm_utils.py:87-112, kirchoff_stress_drucker_prager
+// :45-52).  L comes from the series 1/2 ln B = atanh(Z) = Z (1 + Z^2/3 +
+// Z^4/5 + ...), Z = X (2I + X)^-1, X = B - I, and exp from its Taylor series;
+// all products are of commuting symmetric matrices.  Each particle takes the
+// fewest terms that keep the truncation below 1/4 ulp for it (the bounds
+// below), so its result does not depend on the other particles of its wave.  Particles whose B is far from I (an
+// eigenvalue of B outside [1/3, 3]) or whose F_tr is inverted or singular
+// (det <= 0, where the reference's ln takes a negative or zero singular
+// value) return false and take the SVD path.
+// ---------------------------------------------------------------------------
+struct Sym3 {
+    float xx, xy, xz, yy, yz, zz;
+};
+
+// product of two commuting symmetric matrices (the product is symmetric)
+__device__ __forceinline__ Sym3 sym_mul(const Sym3& a, const Sym3& b) {
+    Sym3 c;
+    c.xx = a.xx * b.xx + a.xy * b.xy + a.xz * b.xz;
+    c.xy = a.xx * b.xy + a.xy * b.yy + a.xz * b.yz;
+    c.xz = a.xx * b.xz + a.xy * b.yz + a.xz * b.zz;
+    c.yy = a.xy * b.xy + a.yy * b.yy + a.yz * b.yz;
+    c.yz = a.xy * b.xz + a.yy * b.yz + a.yz * b.zz;
+    c.zz = a.xz * b.xz + a.yz * b.yz + a.zz * b.zz;
+    return c;
+}
+
+// c I + k M
+__device__ __forceinline__ Sym3 sym_axpi(float k, const Sym3& m, float c) {
+    return Sym3{fmaf(k, m.xx, c), k * m.xy, k * m.xz, fmaf(k, m.yy, c), k * m.yz, fmaf(k, m.zz, c)};
+}
+
+__device__ __forceinline__ float sym_norm2(const Sym3& m) {
+    return m.xx * m.xx + m.yy * m.yy + m.zz * m.zz + 2.0f * (m.xy * m.xy + m.xz * m.xz + m.yz * m.yz);
+}
+
+// Truncation bounds.  atanh: with K terms the first omitted one is
+// z^(2K+1)/(2K+1) for an eigenvalue z of Z, |z| <= 1/2; relative to z it is
+// below |Z|_F^(2K) / (0.75 (2K+1)); bound k-1 is the largest |Z|_F^2 at
+// which k terms keep that below 3e-8.  Taylor exp: with n terms after I the
+// first omitted one is |Y|^(n+1)/(n+1)!; bound n-1 is the largest |Y|_F^2
+// keeping it below 3e-8.
+constexpr int kLogTerms = 12;
+constexpr float kLogBound[kLogTerms - 1] = {6.75e-8f, 3.35e-4f, 5.40e-3f, 2.12e-2f, 4.75e-2f, 8.14e-2f,
+                                            1.19e-1f, 1.58e-1f, 1.96e-1f, 2.32e-1f, 2.68e-1f};
+constexpr float kLogMax = 0.25f;   // |Z|_F^2 beyond this takes the SVD path
+constexpr int kExpTerms = 10;
+constexpr float kExpBound[kExpTerms - 1] = {6.0e-8f, 3.19e-5f, 8.49e-4f, 6.62e-3f, 2.79e-2f,
+                                            8.06e-2f, 1.865e-1f, 3.66e-1f, 6.41e-1f};
+
+// Returns false (F, tau unspecified) when the SVD path must be taken.
+// mu, lam, alpha as sand_return_map.  F and tau receive F_e and tau.
+__device__ __forceinline__ bool sand_update_matfn(const float Ftr[9], float mu, float lam, float alpha, float F[9],
+                                                  float tau[9]) {
+    // X = F_tr F_tr^T - I; the diagonal as (F_ii - 1)(F_ii + 1) + ..., which
+    // keeps the small difference from I exact where F_ii is near 1
+    Sym3 X;
+    X.xx = fmaf(Ftr[1], Ftr[1], fmaf(Ftr[2], Ftr[2], (Ftr[0] - 1.0f) * (Ftr[0] + 1.0f)));
+    X.yy = fmaf(Ftr[3], Ftr[3], fmaf(Ftr[5], Ftr[5], (Ftr[4] - 1.0f) * (Ftr[4] + 1.0f)));
+    X.zz = fmaf(Ftr[6], Ftr[6], fmaf(Ftr[7], Ftr[7], (Ftr[8] - 1.0f) * (Ftr[8] + 1.0f)));
+    X.xy = Ftr[0] * Ftr[3] + Ftr[1] * Ftr[4] + Ftr[2] * Ftr[5];
+    X.xz = Ftr[0] * Ftr[6] + Ftr[1] * Ftr[7] + Ftr[2] * Ftr[8];
+    X.yz = Ftr[3] * Ftr[6] + Ftr[4] * Ftr[7] + Ftr[5] * Ftr[8];
+    const float det = Ftr[0] * (Ftr[4] * Ftr[8] - Ftr[5] * Ftr[7]) - Ftr[1] * (Ftr[3] * Ftr[8] - Ftr[5] * Ftr[6]) +
+                      Ftr[2] * (Ftr[3] * Ftr[7] - Ftr[4] * Ftr[6]);
+    // Z = X adj(M) / det(M), M = 2I + X
+    const float m00 = X.xx + 2.0f, m11 = X.yy + 2.0f, m22 = X.zz + 2.0f;
+    Sym3 Ad;
+    Ad.xx = m11 * m22 - X.yz * X.yz;
+    Ad.xy = X.xz * X.yz - X.xy * m22;
+    Ad.xz = X.xy * X.yz - X.xz * m11;
+    Ad.yy = m00 * m22 - X.xz * X.xz;
+    Ad.yz = X.xy * X.xz - m00 * X.yz;
+    Ad.zz = m00 * m11 - X.xy * X.xy;
+    const float idm = __builtin_amdgcn_rcpf(m00 * Ad.xx + X.xy * Ad.xy + X.xz * Ad.xz);
+    Sym3 Z = sym_mul(X, Ad);
+    Z = Sym3{Z.xx * idm, Z.xy * idm, Z.xz * idm, Z.yy * idm, Z.yz * idm, Z.zz * idm};
+    const Sym3 Z2 = sym_mul(Z, Z);
+    const float n2 = Z2.xx + Z2.yy + Z2.zz;   // |Z|_F^2
+    const bool ok = det > 0.0f && n2 <= kLogMax;
+    // terms of the series: each particle's own count (its result must not
+    // depend on the other particles of its wave); the loops run to the most
+    // any lane of the wave needs
+    int K = 1;
+    bool more = ok;
+#pragma unroll
+    for (int k = 0; k < kLogTerms - 1; ++k) {
+        more = more && n2 > kLogBound[k];
+        if (!__any(more)) break;
+        K += more ? 1 : 0;
+    }
+    // P = sum_j Z^(2j) / (2j+1), j < K, by Horner
+    Sym3 P = Sym3{1.0f, 0.0f, 0.0f, 1.0f, 0.0f, 1.0f};
+#pragma unroll
+    for (int j = kLogTerms - 1; j >= 1; --j) {
+        if (!__any(j < K)) continue;
+        const float c = 1.0f / (float)(2 * j + 1);
+        if (j == K - 1) P = Sym3{c, 0.0f, 0.0f, c, 0.0f, c};
+        else if (j < K - 1) P = sym_axpi(1.0f, sym_mul(Z2, P), c);
+    }
+    if (K > 1) P = sym_axpi(1.0f, sym_mul(Z2, P), 1.0f);
+    const Sym3 L = sym_mul(Z, P);
+    const float tr = L.xx + L.yy + L.zz;
+    const float t3 = tr * (1.0f / 3.0f);
+    const Sym3 D = Sym3{L.xx - t3, L.xy, L.xz, L.yy - t3, L.yz, L.zz - t3};
+    const float ehn = __builtin_amdgcn_sqrtf(sym_norm2(D));
+    const float dg = ehn + (3.0f * lam + 2.0f * mu) / (2.0f * mu) * tr * alpha;
+    Sym3 H = L;
+    if (dg > 0.0f) {
+        // branch 2 (tr > 0): H = 0, F_e = exp(-L) F_tr; branch 3: H = L - r D, F_e = exp(-r D) F_tr
+        Sym3 Y;
+        if (tr > 0.0f) {
+            H = Sym3{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+            Y = Sym3{-L.xx, -L.xy, -L.xz, -L.yy, -L.yz, -L.zz};
+        } else {
+            const float r = dg / ehn;
+            Y = Sym3{-r * D.xx, -r * D.xy, -r * D.xz, -r * D.yy, -r * D.yz, -r * D.zz};
+            H = Sym3{L.xx + Y.xx, L.xy + Y.xy, L.xz + Y.xz, L.yy + Y.yy, L.yz + Y.yz, L.zz + Y.zz};
+        }
+        const float ny2 = sym_norm2(Y);
+        int n = 1;   // this particle's terms, as K above
+        bool up = ok;
+#pragma unroll
+        for (int k = 0; k < kExpTerms - 1; ++k) {
+            up = up && ny2 > kExpBound[k];
+            if (!__any(up)) break;
+            n += up ? 1 : 0;
+        }
+        // E = I + Y (I + Y/2 (I + ... (I + Y/n)))
+        Sym3 E = Sym3{1.0f, 0.0f, 0.0f, 1.0f, 0.0f, 1.0f};
+#pragma unroll
+        for (int k = kExpTerms; k >= 1; --k) {
+            if (!__any(k <= n)) continue;
+            const float ik = 1.0f / (float)k;
+            if (k == n) E = sym_axpi(ik, Y, 1.0f);
+            else if (k < n) E = sym_axpi(ik, sym_mul(Y, E), 1.0f);
+        }
+        const float Ef[9] = {E.xx, E.xy, E.xz, E.xy, E.yy, E.yz, E.xz, E.yz, E.zz};
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                F[i * 3 + j] = Ef[i * 3 + 0] * Ftr[0 * 3 + j] + Ef[i * 3 + 1] * Ftr[1 * 3 + j] +
+                               Ef[i * 3 + 2] * Ftr[2 * 3 + j];
+    } else {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) F[k] = Ftr[k];
+    }
+    const float c2 = 2.0f * mu, cl = lam * (H.xx + H.yy + H.zz);
+    tau[0] = fmaf(c2, H.xx, cl);
+    tau[4] = fmaf(c2, H.yy, cl);
+    tau[8] = fmaf(c2, H.zz, cl);
+    tau[1] = tau[3] = c2 * H.xy;
+    tau[2] = tau[6] = c2 * H.xz;
+    tau[5] = tau[7] = c2 * H.yz;
+    return ok;
+}
+
+// F_tr = (I + dt gradv) F  (mpm_utils.py:279-282)
+__device__ __forceinline__ void trial_F(const float gradv[9], float dt, const float F[9], float Ftr[9]) {
     float A[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) A[k] = gradv[k] * dt;
     A[0] += 1.0f; A[4] += 1.0f; A[8] += 1.0f;
-    float Ftr[9];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j)
             Ftr[i * 3 + j] = A[i * 3 + 0] * F[0 * 3 + j] + A[i * 3 + 1] * F[1 * 3 + j] +
                              A[i * 3 + 2] * F[2 * 3 + j];
-    float U[9], s[3], V[9];
-    svd3_rot(Ftr, U, s, V);
-    float ls[3];
-    sand_return_map(Ftr, U, s, V, mu, lam, alpha, F, ls);
-    dp_stress_sym(U, ls, mu, lam, tau);
+}
+
+// Whole per-particle constitutive update of g2p_apic_and_stress
+// (mpm_utils.py:279-306), sand: F_tr = (I + dt gradv) F, return map, stress
+// (sand_update_matfn, the SVD form where it declines).
+__device__ __forceinline__ void sand_update(const float gradv[9], float dt, float F[9],
+                                            float mu, float lam, float alpha, float tau[9]) {
+    float Ftr[9];
+    trial_F(gradv, dt, F, Ftr);
+    if (!sand_update_matfn(Ftr, mu, lam, alpha, F, tau)) {
+        float U[9], s[3], V[9];
+        svd3_rot(Ftr, U, s, V);
+        float ls[3];
+        sand_return_map(Ftr, U, s, V, mu, lam, alpha, F, ls);
+        dp_stress_sym(U, ls, mu, lam, tau);
+    }
 }
 
 // von_mises_return_mapping (mpm_utils.py:59-84) on F_tr given its SVD, for
@@ -367,35 +543,32 @@ __device__ __forceinline__ void fcr_stress(const float F[9], const float U[9], c
 //   material 2 (sand):  F = Drucker-Prager(F_tr), stress of F_e;
 //   otherwise (jelly):  F = F_tr, FCR stress.
 // One SVD of F_tr serves the return map and the stress (the reference
-// recomputes it on F_e, mpm_utils.py:297).  `material` is uniform across a
-// workgroup, so the branches do not diverge.
+// recomputes it on F_e, mpm_utils.py:297); sand takes it only where
+// sand_update_matfn declines.  `material` is uniform across a workgroup, so
+// the material branches do not diverge.
 __device__ __forceinline__ void constitutive_update(const float gradv[9], float dt, float F[9], int material,
                                                     float mu, float lam, float alpha, int hardening, float xi,
                                                     float& ys, float tau[9]) {
-    float A[9];
-#pragma unroll
-    for (int k = 0; k < 9; ++k) A[k] = gradv[k] * dt;
-    A[0] += 1.0f; A[4] += 1.0f; A[8] += 1.0f;
     float Ftr[9];
+    trial_F(gradv, dt, F, Ftr);
+    // sand without a decomposition where it can (sand_update_matfn); one SVD
+    // serves the other materials and the sand particles it declines
+    const bool svd = material != 2 || !sand_update_matfn(Ftr, mu, lam, alpha, F, tau);
+    if (svd) {
+        float U[9], s[3], V[9];
+        svd3_rot(Ftr, U, s, V);
+        float ls[3];
+        if (material == 2) {
+            sand_return_map(Ftr, U, s, V, mu, lam, alpha, F, ls);
+            dp_stress_sym(U, ls, mu, lam, tau);
+        } else if (material == 1) {
+            von_mises_return_map(Ftr, U, s, V, mu, lam, ys, hardening, xi, F, ls);
+            stvk_stress(U, V, ls, mu, lam, tau);
+        } else {
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
-            Ftr[i * 3 + j] = A[i * 3 + 0] * F[0 * 3 + j] + A[i * 3 + 1] * F[1 * 3 + j] +
-                             A[i * 3 + 2] * F[2 * 3 + j];
-    float U[9], s[3], V[9];
-    svd3_rot(Ftr, U, s, V);
-    float ls[3];
-    if (material == 2) {
-        sand_return_map(Ftr, U, s, V, mu, lam, alpha, F, ls);
-        dp_stress_sym(U, ls, mu, lam, tau);
-    } else if (material == 1) {
-        von_mises_return_map(Ftr, U, s, V, mu, lam, ys, hardening, xi, F, ls);
-        stvk_stress(U, V, ls, mu, lam, tau);
-    } else {
-#pragma unroll
-        for (int k = 0; k < 9; ++k) F[k] = Ftr[k];
-        fcr_stress(F, U, V, det3(F), mu, lam, tau);
+            for (int k = 0; k < 9; ++k) F[k] = Ftr[k];
+            fcr_stress(F, U, V, det3(F), mu, lam, tau);
+        }
     }
 }
 

@@ -135,6 +135,66 @@ def test_binned_stage_parity(oracle):
     assert np.abs(out["u"] - (x_new - x_in)).max() < 2 * ulp
 
 
+def test_sand_update_over_strain_scales(oracle):
+    """The sand return map and stress of the G2P kernels (sand_return_mapping
+    mpm_utils.py:87-112, kirchoff_stress_drucker_prager :45-52), which the
+    GPU evaluates as functions of F_tr F_tr^T and hands to its SVD form for
+    large stretches and inverted or singular F_tr, particle by particle
+    against the FP64 oracle (its SVD, return map and stress of F_e as the
+    reference computes them).  A G2P from a zero velocity grid has
+    F_tr = F exactly, so every loaded F is one test case: rotations times
+    I + s N (N standard normal) for strain scales s from 1e-5 to 0.4, plus
+    inverted and nearly singular F.  Per group of cases the GPU's relative L2
+    error in tau is at most the FP32 oracle's own (the reference algorithm in
+    FP32) plus 1e-6, in F_e at most the larger of the FP32 oracle's and 1e-6;
+    stresses the FP64 oracle makes NaN (an
+    inverted F_tr the return map leaves elastic: ln of a negative singular
+    value) are NaN on the GPU too, and no others."""
+    sc = S.column_lattice(n_grid=32, per_axis=2, seed=3)
+    n = sc.n_particles
+    rng = np.random.default_rng(7)
+    scales = [1e-5, 1e-4, 1e-3, 1e-2, 5e-2, 1e-1, 2e-1, 4e-1]
+    groups = rng.integers(0, len(scales) + 2, size=n)
+    F = np.empty((n, 3, 3))
+    for p in range(n):
+        g = groups[p]
+        Q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+        Q[:, 0] *= np.sign(np.linalg.det(Q))
+        if g < len(scales):
+            F[p] = Q @ (np.eye(3) + scales[g] * rng.normal(size=(3, 3)))
+        elif g == len(scales):            # inverted
+            F[p] = Q @ (np.eye(3) + 0.01 * rng.normal(size=(3, 3))) @ np.diag([1.0, 1.0, -1.0])
+        else:                             # nearly singular
+            F[p] = Q @ np.diag([1.0, 1.0, 10.0 ** rng.uniform(-7, -3)]) @ (np.eye(3) + 0.01 * rng.normal(size=(3, 3)))
+    F = F.astype(np.float32)
+    sim = S.make_simulator(sc)
+    sim.use_bins = False
+    dev = sim.device
+    sim.load_F_from_tensor(torch.from_numpy(F.reshape(-1, 9)).to(dev))
+    lib = L.load()
+    gv = torch.zeros((sc.n_grid ** 3, 4), dtype=torch.float32, device=dev)
+    L.check(lib.mpm_g2p(sim._params(), sim._state(), gv.data_ptr(), 1e-4,
+                        torch.cuda.current_stream().cuda_stream), "g2p")
+    out = _gpu_state(sim)
+    prm = S.scene_params(sc)
+    mu, lam, alpha = prm["mu"], prm["lam"], prm["alpha"]
+    Fe64 = np.array([oracle.sand_return_map(f.astype(np.float64), mu, lam, alpha) for f in F])
+    tau64 = np.array([oracle.dp_stress(f, mu, lam) for f in Fe64])
+    f32 = np.float32
+    Fe32 = np.array([oracle.sand_return_map(f, f32(mu), f32(lam), f32(alpha)) for f in F])
+    tau32 = np.array([oracle.dp_stress(f, f32(mu), f32(lam)) for f in Fe32])
+    nan64 = np.isnan(tau64).any(axis=(1, 2))
+    assert np.array_equal(np.isnan(out["tau"]).any(axis=(1, 2)), nan64)
+    assert nan64.sum() > 0 and np.isfinite(out["F"]).all()
+    for g in range(len(scales) + 2):
+        m = (groups == g) & ~nan64
+        e_tau = rel_l2(out["tau"][m], tau64[m])
+        e32 = rel_l2(tau32[m], tau64[m])
+        assert e_tau <= e32 + 1e-6, (g, e_tau, e32)
+        e_F, e32_F = rel_l2(out["F"][m], Fe64[m]), rel_l2(Fe32[m], Fe64[m])
+        assert e_F <= max(e32_F, 1e-6), (g, e_F, e32_F)
+
+
 def _run_oracle(oracle, sc, n_steps, dtype, order=None, nthreads=1):
     P = S.scene_params(sc)
     st = S.scene_state(sc, np.float64)
