@@ -6,7 +6,7 @@
 //
 // One substep (MPM_Simulator_WARP.p2g2p, mpm_solver_warp.py:156-203) is two
 // launches:
-//   k_grid_bins   per 8^3 grid block: sums the P2G tiles of the (up to 27)
+//   k_grid_bins   per 8^3 grid block: sums the P2G tiles of the (up to 8)
 //                 overlapping bins and the spill grid, then
 //                 grid_normalization_and_gravity (mpm_utils.py:229-237) and
 //                 the colliders (mpm_solver_warp.py:429-449, 505-523).  Its
@@ -16,15 +16,17 @@
 //                 velocities, the sand return map and stress in registers,
 //                 update_u_from_x, and the P2G of the next substep
 //                 (mpm_utils.py:190-227) into an FP64 LDS tile (ds_add_f64;
-//                 FP32 LDS atomics run ~16x slower on gfx950), written out as
+//                 FP32 LDS atomics run ~8x slower on gfx950), written out as
 //                 one plain, coalesced tile per item; every particle is
 //                 written to the slot of its new bin in the other plane set.
 //                 Workgroup w takes item w of a list sorted largest first.
 // Bins' tiles carry an epoch stamp; the grid update uses the tiles of the
 // current epoch, so nothing needs clearing between substeps (zero_grid,
-// mpm_utils.py:115-123, becomes the epoch bump).  Particles whose next
-// stencil leaves their bin's tile (or that sit in the overflow run) scatter
-// with FP64 atomics into a spill grid that the grid update adds and clears.
+// mpm_utils.py:115-123, becomes the epoch bump).  A bin's tiles hold nodes
+// b0 .. b0+9 along each axis (b0 its first cell): the stencils of the bases
+// inside the bin.  Particles whose next base has left their bin (about one in
+// a hundred per substep), and those in the overflow run, scatter with FP64
+// atomics into a spill grid that the grid update adds and clears.
 // The P2G of the first substep (or after the state is replaced) is
 // k_p2g_bins.  So between substeps the tiles hold the next substep's P2G,
 // computed with that substep's dt.
@@ -43,10 +45,8 @@ namespace {
 
 constexpr int kWG = 512;          // threads per bin workgroup
 constexpr int kBinEdge = 8;           // cells per bin edge
-constexpr int kTG = kBinEdge + 2;     // gather tile edge (nodes b0 .. b0+9)
-constexpr int kTS = kBinEdge + 4;     // scatter tile edge (nodes b0-1 .. b0+10)
-constexpr int kTG3 = kTG * kTG * kTG;
-constexpr int kTS3 = kTS * kTS * kTS;
+constexpr int kTE = kBinEdge + 2;     // tile edge: nodes b0 .. b0+9, the stencils of the bin's bases
+constexpr int kTE3 = kTE * kTE * kTE;
 constexpr int kSortMax = 6144;        // bins up to this size are processed in bank order
 constexpr int kSortPeriod = 32;       // substeps between bank-order sorts of the bins
 constexpr int kStageFloats = 6144;    // LDS staging of the sort's copy (24 KB: two bins per CU)
@@ -385,34 +385,34 @@ __device__ __forceinline__ void scatter_tile(double* tile, int i0, const AxisWei
                                              const float v[3], const float C[9], const float tau[9], float dx,
                                              float inv_dx, float dt) {
     p2g_contrib(A, m, vol, v, C, tau, dx, inv_dx, dt, [&](int c, int i, int j, int k, float val) {
-        atomicAdd(&tile[c * kTS3 + i0 + (i * kTS + j) * kTS + k], (double)val);
+        atomicAdd(&tile[c * kTE3 + i0 + (i * kTE + j) * kTE + k], (double)val);
     });
 }
 
-// Writes a bin's LDS tile (nodes b0-1 .. b0+10) to its slot of the global
+// Writes a bin's LDS tile (nodes b0 .. b0+9) to its slot of the global
 // tile buffer as float4 (mv, m) and stamps it with the epoch.  tid: the
 // thread's index in the workgroup.
-// Marks the grid blocks that bin `bin`'s tile overlaps (nodes 8B-1 .. 8B+10
-// along each axis: blocks B-1 .. B+1, up to 27) with the tile's epoch, so
-// that the grid update passes over every other block after one load, without
-// the scene lookup and the 27 stamp loads.  tid: the thread's index in the
+// Marks the grid blocks that bin `bin`'s tile overlaps (nodes 8B .. 8B+9
+// along each axis: blocks B and B+1, up to 8) with the tile's epoch, so that
+// the grid update passes over every other block after one load, without the
+// scene lookup and the stamp loads.  tid: the thread's index in the
 // workgroup.
 __device__ __forceinline__ void mark_tile_blocks(int* __restrict__ blive, const SceneDesc& sc, int bin, int epoch,
                                                  int tid) {
-    if (tid < 27) {
+    if (tid < 8) {
         const int nb = sc.nb, lb = bin - sc.bin_base;
-        const int x = lb / (nb * nb) + tid / 9 - 1, y = (lb / nb) % nb + (tid / 3) % 3 - 1, z = lb % nb + tid % 3 - 1;
-        if (x >= 0 && y >= 0 && z >= 0 && x < nb && y < nb && z < nb) blive[sc.bin_base + (x * nb + y) * nb + z] = epoch;
+        const int x = lb / (nb * nb) + (tid >> 2), y = (lb / nb) % nb + ((tid >> 1) & 1), z = lb % nb + (tid & 1);
+        if (x < nb && y < nb && z < nb) blive[sc.bin_base + (x * nb + y) * nb + z] = epoch;
     }
 }
 
 __device__ __forceinline__ void write_tile(const double* s_acc, float4* __restrict__ tiles,
                                            int* __restrict__ stamps, int* __restrict__ blive, const SceneDesc& sc,
                                            int bin, int epoch, int tid) {
-    float4* out = tiles + (int64_t)bin * kTS3;
-    for (int t = tid; t < kTS3; t += blockDim.x)
-        out[t] = make_float4((float)s_acc[t], (float)s_acc[kTS3 + t], (float)s_acc[2 * kTS3 + t],
-                             (float)s_acc[3 * kTS3 + t]);
+    float4* out = tiles + (int64_t)bin * kTE3;
+    for (int t = tid; t < kTE3; t += blockDim.x)
+        out[t] = make_float4((float)s_acc[t], (float)s_acc[kTE3 + t], (float)s_acc[2 * kTE3 + t],
+                             (float)s_acc[3 * kTE3 + t]);
     if (tid == 0) stamps[bin] = epoch;
     mark_tile_blocks(blive, sc, bin, epoch, tid);
 }
@@ -575,9 +575,9 @@ __device__ __forceinline__ void g2p2g_particle(const SubstepArgs& a, const Scene
         const int lx = A.base[0] - b0[0], ly = A.base[1] - b0[1], lz = A.base[2] - b0[2];
         if (tile_vel != nullptr && lx >= 0 && ly >= 0 && lz >= 0 && lx < kBinEdge && ly < kBinEdge &&
             lz < kBinEdge) {
-            // the velocity tile has the scatter tile's layout (nodes b0-1 .. b0+10)
-            gather_tile<kTS * kTS, kTS>(SoANodes{tile_vel, kTS3}, ((lx + 1) * kTS + ly + 1) * kTS + lz + 1, A,
-                                        prm.inv_dx, v, Cm, G);
+            // the velocity tile holds nodes b0 .. b0+9
+            gather_tile<kTE * kTE, kTE>(SoANodes{tile_vel, kTE3}, (lx * kTE + ly) * kTE + lz, A, prm.inv_dx, v,
+                                        Cm, G);
         } else {
             // global gather: same contraction, nodes read from the grid
             const float4* vb = a.vel + sc.node_base + (((int64_t)A.base[0] * ng + A.base[1]) * ng + A.base[2]);
@@ -594,15 +594,20 @@ __device__ __forceinline__ void g2p2g_particle(const SubstepArgs& a, const Scene
         axis_weights(x, prm.inv_dx, A);     // weights of the next substep's P2G
         if (stencil_ok(A.base, ng)) {
             new_bin = sc.bin_base + bin_of_base(A.base, sc.nb);
-            const int sx = A.base[0] - (b0[0] - 1), sy = A.base[1] - (b0[1] - 1), sz = A.base[2] - (b0[2] - 1);
-            if (tile_acc != nullptr && sx >= 0 && sy >= 0 && sz >= 0 && sx <= kTS - 3 && sy <= kTS - 3 &&
-                sz <= kTS - 3) {
+            // the tile takes the stencils of bases inside the bin; a particle
+            // that has moved into another bin scatters into the spill grid
+            const int sx = A.base[0] - b0[0], sy = A.base[1] - b0[1], sz = A.base[2] - b0[2];
+            const bool in_tile = tile_acc != nullptr && sx >= 0 && sy >= 0 && sz >= 0 && sx < kBinEdge &&
+                                 sy < kBinEdge && sz < kBinEdge;
+            if (in_tile) {
                 mode = 1;
-                i0 = (sx * kTS + sy) * kTS + sz;
+                i0 = (sx * kTE + sy) * kTE + sz;
             } else {
                 mode = 2;
-                if (tile_acc != nullptr) atomicAdd(&a.status[1], 1);
             }
+            // count the bin's particles that left their tile, one atomic per wave
+            const unsigned long long out = __ballot(tile_acc != nullptr && !in_tile);
+            if (out && __lane_id() == __ffsll((long long)out) - 1) atomicAdd(&a.status[1], __popcll(out));
         } else {
             atomicAdd(&a.status[2], 1);  // left the grid: frozen from here on
             int bc[3];
@@ -646,10 +651,10 @@ __device__ __forceinline__ void g2p2g_particle(const SubstepArgs& a, const Scene
 
 // ---------------------------------------------------------------------------
 // Grid update of one 8^3 grid block: gathers the P2G tiles of the current
-// epoch from the bins overlapping it (the tile of bin B spans nodes 8B-1 ..
-// 8B+10, so a node x is covered by the tiles of bins (x+1)>>3 and, when
-// x+1-8((x+1)>>3) <= 3, the bin before) and the
-// spill grid, sums in FP64, then normalization, gravity and colliders.
+// epoch from the bins overlapping it (the tile of bin B spans nodes 8B ..
+// 8B+9, so a node x is covered by the tiles of bins x>>3 and, when
+// x mod 8 <= 1, the bin before) and the spill grid, sums in FP64, then
+// normalization, gravity and colliders.
 // Also resets the next plane set's bin counters.  clear_only: only drains
 // the spill grid.  Workgroup 0 builds the schedule of k_g2p2g from the
 // current bin counts (when `order` is given); workgroup 1 + b updates grid
@@ -692,7 +697,7 @@ __global__ void __launch_bounds__(kWG)
     stamps += sc.bin_base;
     pcount += sc.bin_base;
     pslot += sc.bin_base;
-    tiles += (int64_t)sc.bin_base * kTS3;
+    tiles += (int64_t)sc.bin_base * kTE3;
     const MpmCollider* cols = tab.dev ? tab.cols + sc.col_base : cs.c;
     const int n_cols = tab.dev ? sc.n_cols : cs.n;
     if (tid == 0) s_any = s_anyp = s_bad = 0;
@@ -745,12 +750,14 @@ __global__ void __launch_bounds__(kWG)
             sx = a4.x; sy = a4.y; sz = a4.z; sm = a4.w;
             acc[node] = make_double4(0.0, 0.0, 0.0, 0.0);
         }
+        // along each axis the tile of bin B holds nodes 8B .. 8B+9: node p is
+        // in the tile of bin p >> 3 and, when p mod 8 <= 1, of the bin before
         const int p[3] = {x, y, z};
         int hi[3], nlo[3];
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            hi[d] = (p[d] + 1) >> 3;
-            nlo[d] = (p[d] + 1 - 8 * hi[d] <= 3 && hi[d] >= 1) ? 2 : 1;
+            hi[d] = p[d] >> 3;
+            nlo[d] = ((p[d] & 7) <= 1 && hi[d] >= 1) ? 2 : 1;
         }
         const int bb[3] = {bx, by, bz};
         // the (up to 8) bins covering the node: their neighbour index and the
@@ -763,8 +770,8 @@ __global__ void __launch_bounds__(kWG)
             const int b[3] = {hi[0] - ix, hi[1] - iy, hi[2] - iz};
             cov[c] = ix < nlo[0] && iy < nlo[1] && iz < nlo[2] && b[0] < nb && b[1] < nb && b[2] < nb;
             li[c] = ((b[0] - bb[0] + 1) * 3 + (b[1] - bb[1] + 1)) * 3 + (b[2] - bb[2] + 1);
-            const int lx = p[0] - 8 * b[0] + 1, ly = p[1] - 8 * b[1] + 1, lz = p[2] - 8 * b[2] + 1;
-            lt[c] = (lx * kTS + ly) * kTS + lz;
+            const int lx = p[0] - 8 * b[0], ly = p[1] - 8 * b[1], lz = p[2] - 8 * b[2];
+            lt[c] = (lx * kTE + ly) * kTE + lz;
         }
         // whole-bin tiles, loaded all at once (absent ones read tile slot 0
         // and are dropped), summed in (ix, iy, iz) order
@@ -779,7 +786,7 @@ __global__ void __launch_bounds__(kWG)
                     const int ix = c >> 2, iy = (c >> 1) & 1, iz = c & 1;
                     use[cc] = cov[c] && s_live[li[c]] != 0;
                     const int64_t ti =
-                        (int64_t)(((hi[0] - ix) * nb + (hi[1] - iy)) * nb + (hi[2] - iz)) * kTS3 + lt[c];
+                        (int64_t)(((hi[0] - ix) * nb + (hi[1] - iy)) * nb + (hi[2] - iz)) * kTE3 + lt[c];
                     tv[cc] = tiles[use[cc] ? ti : 0];
                 }
 #pragma unroll
@@ -804,16 +811,15 @@ __global__ void __launch_bounds__(kWG)
                     bool cv = true;
 #pragma unroll
                     for (int e = 0; e < 3; ++e) {
-                        const int h = (q[e] + 1) >> 3;
+                        const int h = q[e] >> 3;
                         b[e] = h - d[e];
-                        cv = cv && b[e] < nb && (d[e] == 0 || (q[e] + 1 - 8 * h <= 3 && h >= 1));
+                        cv = cv && b[e] < nb && (d[e] == 0 || ((q[e] & 7) <= 1 && h >= 1));
                     }
                     const int slot =
                         cv ? s_pslot[part * 27 + ((b[0] - bx + 1) * 3 + (b[1] - by + 1)) * 3 + (b[2] - bz + 1)] : -1;
                     use[u] = slot >= 0;
-                    const int lt =
-                        ((q[0] - 8 * b[0] + 1) * kTS + (q[1] - 8 * b[1] + 1)) * kTS + (q[2] - 8 * b[2] + 1);
-                    off[u] = use[u] ? ((int64_t)slot * kTS3 + lt) * 4 : 0;
+                    const int lt = ((q[0] - 8 * b[0]) * kTE + (q[1] - 8 * b[1])) * kTE + (q[2] - 8 * b[2]);
+                    off[u] = use[u] ? ((int64_t)slot * kTE3 + lt) * 4 : 0;
                 }
                 if (!use[0] && !use[1]) continue;
                 const double2 a0 = *(const double2*)(ptiles + off[0]), a1 = *(const double2*)(ptiles + off[0] + 2);
@@ -839,20 +845,20 @@ __global__ void __launch_bounds__(kWG)
 // Fused G2P (this substep) + P2G (next substep), one workgroup per bin plus
 // workgroups over the overflow run.
 // ---------------------------------------------------------------------------
-// LDS bank class of a stencil base node in the 12^3 tiles: t mod 32, t the
-// tile index of the base (origin b0-1).  Lanes of a wave are given particles
+// LDS bank class of a stencil base node in the 10^3 tiles: t mod 32, t the
+// tile index of the base (origin b0).  Lanes of a wave are given particles
 // of distinct classes (see k_g2p2g), so that the 27-node gathers (4-byte
 // reads) and scatters (8-byte atomics) of a 32-lane half hit distinct banks.
 __device__ __forceinline__ int bank_class(const int base[3], const int b0[3]) {
-    return ((((base[0] - b0[0] + 1) * kTS + (base[1] - b0[1] + 1)) * kTS + (base[2] - b0[2] + 1)) & 31);
+    return ((((base[0] - b0[0]) * kTE + (base[1] - b0[1])) * kTE + (base[2] - b0[2])) & 31);
 }
 
-// Launch shapes of k_g2p2g held at three waves per SIMD (161 VGPRs, no
-// next-particle prefetch): one 768-thread workgroup per CU, or two of 384
-// (their 2 x 76 KB of LDS tiles fit the CU's 160 KB).  256 and 512 threads
+// Launch shapes of k_g2p2g held at three waves per SIMD (~165 VGPRs, no
+// next-particle prefetch): one 768-thread workgroup per CU, two of 384 or
+// three of 256 (3 x 44 KB of LDS tiles fit the CU's 160 KB).  512 threads
 // keep two waves per SIMD and load the next particle ahead.
 template <int WG>
-constexpr bool kThreeWaves = WG == 768 || WG == 384;
+constexpr bool kThreeWaves = WG == 768 || WG == 384 || WG == 256;
 
 // One work item of k_g2p2g: slots [lo, hi) of `bin`, part `part` of
 // `parts` (the bin's passes dealt out in runs of ceil(passes / parts)), with
@@ -887,15 +893,15 @@ __device__ __forceinline__ void g2p2g_item(const SubstepArgs& a, int bin, int pa
         for (int i = lo + tid; i < hi; i += WG) live += __float_as_int(a.src[off0 + (int64_t)MPM_P_PID * cap + i]) >= 0;
         if (live) atomicAdd(s_live, live);
     }
-    for (int t = tid; t < kTS3; t += WG) {
-        const int x = b0[0] - 1 + t / (kTS * kTS), y = b0[1] - 1 + (t / kTS) % kTS, z = b0[2] - 1 + t % kTS;
+    for (int t = tid; t < kTE3; t += WG) {
+        const int x = b0[0] + t / (kTE * kTE), y = b0[1] + (t / kTE) % kTE, z = b0[2] + t % kTE;
         float4 val = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (x >= 0 && y >= 0 && z >= 0 && x < ng && y < ng && z < ng) val = svel[((int64_t)x * ng + y) * ng + z];
+        if (x < ng && y < ng && z < ng) val = svel[((int64_t)x * ng + y) * ng + z];
         s_vel[t] = val.x;
-        s_vel[kTS3 + t] = val.y;
-        s_vel[2 * kTS3 + t] = val.z;
+        s_vel[kTE3 + t] = val.y;
+        s_vel[2 * kTE3 + t] = val.z;
     }
-    for (int t = tid; t < 4 * kTS3; t += WG) s_acc[t] = 0.0;
+    for (int t = tid; t < 4 * kTE3; t += WG) s_acc[t] = 0.0;
     __syncthreads();
     if (parts > 1) {
         if (tid == 0) *s_base = atomicAdd(&a.cnt_nxt[bin], *s_live);
@@ -945,10 +951,10 @@ __device__ __forceinline__ void g2p2g_item(const SubstepArgs& a, int bin, int pa
         // a part's sums stay FP64 until the grid update has added the parts:
         // rounded once, as a whole bin's tile, whichever particles fell
         // into which part
-        double* out = a.ptiles + (int64_t)blockIdx.x * kTS3 * 4;
-        for (int t = tid; t < kTS3; t += WG) {
-            *(double2*)(out + 4 * t) = make_double2(s_acc[t], s_acc[kTS3 + t]);
-            *(double2*)(out + 4 * t + 2) = make_double2(s_acc[2 * kTS3 + t], s_acc[3 * kTS3 + t]);
+        double* out = a.ptiles + (int64_t)blockIdx.x * kTE3 * 4;
+        for (int t = tid; t < kTE3; t += WG) {
+            *(double2*)(out + 4 * t) = make_double2(s_acc[t], s_acc[kTE3 + t]);
+            *(double2*)(out + 4 * t + 2) = make_double2(s_acc[2 * kTE3 + t], s_acc[3 * kTE3 + t]);
         }
         if (tid == 0) {
             a.pslot[part * g.n_bins + bin] = blockIdx.x;
@@ -992,8 +998,8 @@ __device__ __forceinline__ void g2p2g_overflow(const SubstepArgs& a, int blk) {
 template <int WG>
 __global__ void __launch_bounds__(WG)
     __attribute__((amdgpu_waves_per_eu(kThreeWaves<WG> ? 3 : 2, kThreeWaves<WG> ? 3 : 2))) k_g2p2g(SubstepArgs a) {
-    __shared__ float s_vel[3 * kTS3];
-    __shared__ double s_acc[4 * kTS3];
+    __shared__ float s_vel[3 * kTE3];
+    __shared__ double s_acc[4 * kTE3];
     __shared__ int s_stay, s_base, s_live;
     const BinGeom& g = a.g;
     const int wg = blockIdx.x;
@@ -1014,7 +1020,7 @@ __global__ void __launch_bounds__(kWG)
     k_p2g_bins(SceneTab tab, BinGeom g, float dt, const float* __restrict__ planes, const int* __restrict__ cnt,
                double* __restrict__ acc, int* __restrict__ gflag, float4* __restrict__ tiles,
                int* __restrict__ stamps, int* __restrict__ blive, int epoch_out) {
-    __shared__ double s_acc[4 * kTS3];
+    __shared__ double s_acc[4 * kTE3];
     const int tid = threadIdx.x;
     const int wg = blockIdx.x;
     const bool ovf = wg >= g.n_bins;
@@ -1034,7 +1040,7 @@ __global__ void __launch_bounds__(kWG)
     int b0[3] = {0, 0, 0};
     if (!ovf) {
         bin_origin(bin - sc_bin.bin_base, sc_bin.nb, b0);
-        for (int t = tid; t < 4 * kTS3; t += kWG) s_acc[t] = 0.0;
+        for (int t = tid; t < 4 * kTE3; t += kWG) s_acc[t] = 0.0;
         __syncthreads();
     }
     for (int it = 0; it < n_in; it += kWG) {
@@ -1059,9 +1065,9 @@ __global__ void __launch_bounds__(kWG)
         AxisWeights A;
         axis_weights(x, sc.prm.inv_dx, A);
         if (!stencil_ok(A.base, sc.prm.n_grid)) continue;
-        const int sx = A.base[0] - (b0[0] - 1), sy = A.base[1] - (b0[1] - 1), sz = A.base[2] - (b0[2] - 1);
-        if (!ovf && sx >= 0 && sy >= 0 && sz >= 0 && sx <= kTS - 3 && sy <= kTS - 3 && sz <= kTS - 3) {
-            scatter_tile(s_acc, (sx * kTS + sy) * kTS + sz, A, m, vol, v, Cm, tau, sc.prm.dx, sc.prm.inv_dx, dt);
+        const int sx = A.base[0] - b0[0], sy = A.base[1] - b0[1], sz = A.base[2] - b0[2];
+        if (!ovf && sx >= 0 && sy >= 0 && sz >= 0 && sx < kBinEdge && sy < kBinEdge && sz < kBinEdge) {
+            scatter_tile(s_acc, (sx * kTE + sy) * kTE + sz, A, m, vol, v, Cm, tau, sc.prm.dx, sc.prm.inv_dx, dt);
         } else {
             scatter_spill(acc, gflag, sc, A, m, vol, v, Cm, tau, dt);
         }
@@ -1364,8 +1370,8 @@ struct WsLayout {
     int* pslot;        // FP64 tile slot of part p of split bin b (p * nbins + b)
     int* blive;        // epoch of the newest tile overlapping each grid block (mark_tile_blocks)
     int* order;        // schedules of the sort and of k_g2p2g (schedule_bins)
-    float4* tiles;     // one tile of kTS^3 nodes per bin
-    double* ptiles;    // nbins FP64 tiles of kTS^3 nodes (4 doubles each) for the parts of split bins
+    float4* tiles;     // one tile of kTE^3 nodes per bin
+    double* ptiles;    // nbins FP64 tiles of kTE^3 nodes (4 doubles each) for the parts of split bins
     SceneDesc* table;  // batch: scene table
     MpmCollider* cols; // batch: colliders
     int64_t bytes;
@@ -1413,9 +1419,9 @@ WsLayout ws_layout(int64_t nodes, int64_t nbins, int n_scenes, int n_cols, void*
     w.order = (int*)(p + off);
     off += align256((2 * nbins + kItemsFirst) * 4);
     w.tiles = (float4*)(p + off);
-    off += align256(nbins * kTS3 * 16);
+    off += align256(nbins * kTE3 * 16);
     w.ptiles = (double*)(p + off);
-    off += align256(nbins * kTS3 * 32);
+    off += align256(nbins * kTE3 * 32);
     w.table = (SceneDesc*)(p + off);
     off += align256((int64_t)n_scenes * (int64_t)sizeof(SceneDesc));
     w.cols = (MpmCollider*)(p + off);

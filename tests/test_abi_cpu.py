@@ -148,4 +148,4 @@ def test_bins_reject_oversized_bins_and_size_the_part_tiles():
     nodes = 32 ** 3
     assert lib.mpm_bins_velocity_offset(prm) == nodes * 32
     assert lib.mpm_bins_status_offset(prm) >= nodes * 32 + nodes * 16
-    assert lib.mpm_bins_workspace_bytes(prm) >= nb ** 3 * 12 ** 3 * (16 + 32) + nodes * (32 + 16)
+    assert lib.mpm_bins_workspace_bytes(prm) >= nb ** 3 * 10 ** 3 * (16 + 32) + nodes * (32 + 16)
