@@ -49,7 +49,7 @@ constexpr int kTE = kBinEdge + 2;     // tile edge: nodes b0 .. b0+9, the stenci
 constexpr int kTE3 = kTE * kTE * kTE;
 constexpr int kSortMax = 6144;        // bins up to this size are processed in bank order
 constexpr int kSortPeriod = 32;       // substeps between bank-order sorts of the bins
-constexpr int kStageFloats = 6144;    // LDS staging of the sort's copy (24 KB: two bins per CU)
+constexpr int kStageFloats = 12288;   // LDS staging of the sort's copy (48 KB: two bins per CU)
 static_assert(kStageFloats >= kSortMax, "a bin's plane must fit the staging buffer");
 constexpr int kParts = 4;             // work items (each with its own P2G tile) a bin may be split into
 
@@ -1085,16 +1085,28 @@ __global__ void __launch_bounds__(kWG)
 // then hold sorted entries s apart, about one class apart, so any 32
 // consecutive slots span nearly all 32 classes.  Bins larger than kSortMax
 // and the overflow run are copied as they are.
+// Plane j of the planes a fused substep reads from a plane set: x0, u, F, m,
+// vol, pid, and yield_stress for metal (18 or 19).
+__device__ __forceinline__ int read_plane(int j) {
+    return j < 6 ? MPM_P_X0 + j
+                 : (j < 15 ? MPM_P_F + j - 6
+                           : (j == 15 ? MPM_P_MASS : (j == 16 ? MPM_P_VOL : (j == 17 ? MPM_P_PID : MPM_P_YIELD))));
+}
+
+// all_planes false (the periodic sort inside a run): only the planes the next
+// fused substep reads are copied -- v, C and tau of live particles are
+// rewritten by it before anything reads them -- plus v, C and tau of frozen
+// particles (stencil outside the grid), which the fused kernel copies forward.
 __device__ void interleave_bin(const float* __restrict__ src, const int* __restrict__ cnt_src,
                                float* __restrict__ dst, int* __restrict__ cnt_dst, const BinGeom& g,
-                               const SceneTab& tab, int item) {
+                               const SceneTab& tab, int item, bool all_planes) {
     __shared__ float s_stage[kStageFloats];      // planes of the bin being copied
     __shared__ unsigned short s_from[kSortMax];  // source slot of each destination slot
     __shared__ unsigned short s_rank[kSortMax];
     __shared__ unsigned char s_cls[kSortMax];
     __shared__ int s_ccnt[32];
     __shared__ int s_coff[32];
-    __shared__ int s_n, s_mul;
+    __shared__ int s_n, s_mul, s_frozen;
     const int tid = threadIdx.x;
     int64_t off0;
     int n_in;
@@ -1125,18 +1137,26 @@ __device__ void interleave_bin(const float* __restrict__ src, const int* __restr
         return;
     }
     if (tid < 32) s_ccnt[tid] = 0;
+    if (tid == 0) s_frozen = 0;
     __syncthreads();
     for (int i = tid; i < n_in; i += kWG) {
         const float* sp = src + off0 + i;
         if (__float_as_int(sp[(int64_t)MPM_P_PID * cap]) < 0) { s_cls[i] = 255; continue; }  // hole
-        int base[3];
+        int raw[3], base[3];
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             const float x = sp[(int64_t)(MPM_P_X0 + d) * cap] + sp[(int64_t)(MPM_P_U + d) * cap];
-            base[d] = min(max((int)(x * inv_dx - 0.5f), b0[d]), b0[d] + kBinEdge - 1);
+            raw[d] = (int)(x * inv_dx - 0.5f);
+            base[d] = min(max(raw[d], b0[d]), b0[d] + kBinEdge - 1);
         }
         const int c = bank_class(base, b0);
-        s_cls[i] = (unsigned char)c;
+        // frozen, or within a cell of it (a margin that keeps this test on the
+        // safe side of the fused kernel's own at cell boundaries)
+        const int ng = sc.prm.n_grid;
+        const bool frozen = !(raw[0] >= 1 && raw[1] >= 1 && raw[2] >= 1 && raw[0] <= ng - 4 && raw[1] <= ng - 4 &&
+                              raw[2] <= ng - 4);
+        if (frozen) s_frozen = 1;
+        s_cls[i] = (unsigned char)(c | (frozen ? 32 : 0));   // bit 5: frozen
         s_rank[i] = (unsigned short)atomicAdd(&s_ccnt[c], 1);
     }
     __syncthreads();
@@ -1159,25 +1179,37 @@ __device__ void interleave_bin(const float* __restrict__ src, const int* __restr
     __syncthreads();
     const unsigned n = (unsigned)s_n, mul = (unsigned)s_mul;
     for (int i = tid; i < n_in; i += kWG) {
-        const int c = s_cls[i];
-        if (c == 255) continue;
+        if (s_cls[i] == 255) continue;
+        const int c = s_cls[i] & 31;
         s_from[((unsigned)(s_coff[c] + s_rank[i]) * mul) % n] = (unsigned short)i;
     }
     // Permuted copy staged through LDS, as many planes at a time as fit:
     // coalesced reads of whole planes (many loads in flight per thread), then
     // coalesced writes gathering from LDS.
+    const int n_planes = all_planes ? MPM_NPLANES : (sc.prm.material == 1 ? 19 : 18);
     const int per = max(1, kStageFloats / max(n_in, 1));
-    for (int pl0 = 0; pl0 < MPM_NPLANES; pl0 += per) {
-        const int np = min(per, MPM_NPLANES - pl0);
+    for (int pl0 = 0; pl0 < n_planes; pl0 += per) {
+        const int np = min(per, n_planes - pl0);
         __syncthreads();  // s_from ready / the previous stage is written out
         for (int j = 0; j < np; ++j) {
-            const float* sp = src + off0 + (int64_t)(pl0 + j) * cap;
+            const int pl = all_planes ? pl0 + j : read_plane(pl0 + j);
+            const float* sp = src + off0 + (int64_t)pl * cap;
             for (int i = tid; i < n_in; i += kWG) s_stage[j * n_in + i] = sp[i];
         }
         __syncthreads();
         for (int j = 0; j < np; ++j) {
-            float* dp = dst + off0 + (int64_t)(pl0 + j) * cap;
+            const int pl = all_planes ? pl0 + j : read_plane(pl0 + j);
+            float* dp = dst + off0 + (int64_t)pl * cap;
             for (int q = tid; q < (int)n; q += kWG) dp[q] = s_stage[j * n_in + s_from[q]];
+        }
+    }
+    if (!all_planes && s_frozen) {
+        // v, C and tau of the frozen particles
+        for (int q = tid; q < (int)n; q += kWG) {
+            const int i = s_from[q];
+            if (!(s_cls[i] & 32)) continue;
+            for (int pl = MPM_P_V; pl < MPM_P_TAU + 9; ++pl)
+                dst[off0 + (int64_t)pl * cap + q] = src[off0 + (int64_t)pl * cap + i];
         }
     }
 }
@@ -1190,13 +1222,13 @@ __device__ void interleave_bin(const float* __restrict__ src, const int* __restr
 __global__ void __launch_bounds__(kWG)
     k_bins_interleave(const float* __restrict__ src, const int* __restrict__ cnt_src, float* __restrict__ dst,
                       int* __restrict__ cnt_dst, BinGeom g, SceneTab tab, const int* __restrict__ order,
-                      int sched_epoch) {
+                      int sched_epoch, int all_planes) {
     int item = blockIdx.x;
     if (item < g.n_bins && order != nullptr && order[g.n_bins] == sched_epoch) {
         item = order[item];
         if (item < 0) return;
     }
-    interleave_bin(src, cnt_src, dst, cnt_dst, g, tab, item);
+    interleave_bin(src, cnt_src, dst, cnt_dst, g, tab, item, all_planes != 0);
 }
 
 // fill n ints with v (bin counters; a kernel rather than a memset so that the
@@ -1516,10 +1548,10 @@ int* counts(const MpmBins* b, const BinGeom& g, int set) { return b->count + (in
 // k_bins_interleave over every bin and overflow block (order: a schedule or
 // nullptr)
 int launch_sort(const Ctx& c, const float* src, const int* cnt_src, float* dst, int* cnt_dst, const int* order,
-                int sched_epoch, hipStream_t s) {
+                int sched_epoch, hipStream_t s, bool all_planes = true) {
     const unsigned n_items = (unsigned)(c.g.n_bins + c.g.ovf_cap / c.g.ovf_blk);
     hipLaunchKernelGGL(k_bins_interleave, dim3(n_items), dim3(kWG), 0, s, src, cnt_src, dst, cnt_dst, c.g, c.tab,
-                       order, sched_epoch);
+                       order, sched_epoch, all_planes ? 1 : 0);
     return check_launch("k_bins_interleave");
 }
 
@@ -1651,14 +1683,16 @@ int op_g2p2g(const Ctx& c, MpmBins* bins, float dt, hipStream_t s, bool store_al
     return 0;
 }
 
-int op_sort(const Ctx& c, MpmBins* bins, hipStream_t s) {
+// all_planes false: between a grid update and the fused kernel that follows
+// it (the planes that kernel rewrites are not copied, see interleave_bin).
+int op_sort(const Ctx& c, MpmBins* bins, hipStream_t s, bool all_planes = true) {
     const int cur = bins->cur, nxt = 1 - cur;
     int* cnt_cur = counts(bins, c.g, cur);
     // With the schedule of this epoch's grid update (checked on the device),
     // only scheduled bins are copied; that grid update has zeroed cnt_nxt, so
     // the others read as empty.  Without one, every bin is copied.
     int rc = launch_sort(c, bins->planes[cur], cnt_cur, bins->planes[nxt], counts(bins, c.g, nxt), c.w.order,
-                         bins->epoch, s);
+                         bins->epoch, s, all_planes);
     if (rc) return rc;
     // the old set becomes the fused kernel's output: its counters start at zero
     const int cb = 2 * c.g.n_bins + 1;
@@ -1677,7 +1711,7 @@ int op_substep(const Ctx& c, MpmBins* bins, float time, float dt, hipStream_t s,
     // then, between the grid update (which schedules the bins) and the fused
     // kernel, which reads the sorted set
     const int period = bins->sort_period == 0 ? kSortPeriod : bins->sort_period;
-    if (++bins->since_sort >= period && period > 0 && (rc = op_sort(c, bins, s))) return rc;
+    if (++bins->since_sort >= period && period > 0 && (rc = op_sort(c, bins, s, false))) return rc;
     return op_g2p2g(c, bins, dt, s, store_all);
 }
 
@@ -1909,7 +1943,7 @@ int mpm_bins_profile(const MpmParams* prm, MpmBins* bins, const MpmCollider* col
         rc = op_grid(c, bins, (float)t, dt, s);
         (void)hipEventRecord(ev[1], s);
         const bool sorted = rc == 0 && ++bins->since_sort >= period && period > 0;
-        if (sorted) rc = op_sort(c, bins, s);
+        if (sorted) rc = op_sort(c, bins, s, false);
         (void)hipEventRecord(ev[2], s);
         if (rc == 0) rc = op_g2p2g(c, bins, dt, s, k == n_substeps - 1);
         (void)hipEventRecord(ev[3], s);
