@@ -845,12 +845,16 @@ __global__ void __launch_bounds__(kWG)
 // Fused G2P (this substep) + P2G (next substep), one workgroup per bin plus
 // workgroups over the overflow run.
 // ---------------------------------------------------------------------------
-// LDS bank class of a stencil base node in the 10^3 tiles: t mod 32, t the
-// tile index of the base (origin b0).  Lanes of a wave are given particles
-// of distinct classes (see k_g2p2g), so that the 27-node gathers (4-byte
-// reads) and scatters (8-byte atomics) of a 32-lane half hit distinct banks.
+// LDS bank class of a stencil base node in the 10^3 tiles: t mod 16, t the
+// tile index of the base (origin b0).  gfx950's LDS serves an 8-byte access
+// of 16 consecutive lanes at once from 32 four-byte banks: the scatter's
+// ds_add_f64 of such a group runs without conflicts when its 16 nodes differ
+// mod 16 -- which holds when the group's particles are of 16 distinct classes
+// (the offset of a stencil node from the base is the same for all lanes of
+// one atomic).  Lanes 16 apart do not conflict.  The bank-order sort lays
+// bins out so that aligned groups of 16 slots hold distinct classes.
 __device__ __forceinline__ int bank_class(const int base[3], const int b0[3]) {
-    return ((((base[0] - b0[0]) * kTE + (base[1] - b0[1])) * kTE + (base[2] - b0[2])) & 31);
+    return ((((base[0] - b0[0]) * kTE + (base[1] - b0[1])) * kTE + (base[2] - b0[2])) & 15);
 }
 
 // Launch shapes of k_g2p2g held at three waves per SIMD (~165 VGPRs, no
@@ -1079,12 +1083,15 @@ __global__ void __launch_bounds__(kWG)
 }
 
 // Copies every bin of plane set `src` into plane set `dst` in bank order and
-// without holes: a counting sort of the live particles by bank_class of
-// their base node (sorted index k), and entry k goes to slot
-// q = k * mul mod n, mul the inverse of s ~ n/32 modulo n.  Slots q and q+1
-// then hold sorted entries s apart, about one class apart, so any 32
-// consecutive slots span nearly all 32 classes.  Bins larger than kSortMax
-// and the overflow run are copied as they are.
+// without holes: the live particles are ranked within their bank_class
+// (rank r, class c), and laid out round by round -- round r holds the r-th
+// particle of every class that has one, in class order, so the particle goes
+// to slot sum_c' min(count_c', r) + #{c' < c : count_c' > r}.  While every
+// class still has particles a round is 16 slots, aligned to the 16-lane
+// groups of the fused kernel's waves, with 16 distinct classes; once classes
+// run out the rounds shorten and a group may repeat classes (few particles in
+// a full bin, whose classes hold near-equal counts).  Bins larger than
+// kSortMax and the overflow run are copied as they are.
 // Plane j of the planes a fused substep reads from a plane set: x0, u, F, m,
 // vol, pid, and yield_stress for metal (18 or 19).
 __device__ __forceinline__ int read_plane(int j) {
@@ -1104,9 +1111,8 @@ __device__ void interleave_bin(const float* __restrict__ src, const int* __restr
     __shared__ unsigned short s_from[kSortMax];  // source slot of each destination slot
     __shared__ unsigned short s_rank[kSortMax];
     __shared__ unsigned char s_cls[kSortMax];
-    __shared__ int s_ccnt[32];
-    __shared__ int s_coff[32];
-    __shared__ int s_n, s_mul, s_frozen;
+    __shared__ int s_ccnt[16];
+    __shared__ int s_n, s_frozen;
     const int tid = threadIdx.x;
     int64_t off0;
     int n_in;
@@ -1136,7 +1142,7 @@ __device__ void interleave_bin(const float* __restrict__ src, const int* __restr
         if (tid == 0) { cnt_dst[bin] = n_in; cnt_dst[g.n_bins + 1 + bin] = n_in; }
         return;
     }
-    if (tid < 32) s_ccnt[tid] = 0;
+    if (tid < 16) s_ccnt[tid] = 0;
     if (tid == 0) s_frozen = 0;
     __syncthreads();
     for (int i = tid; i < n_in; i += kWG) {
@@ -1156,32 +1162,29 @@ __device__ void interleave_bin(const float* __restrict__ src, const int* __restr
         const bool frozen = !(raw[0] >= 1 && raw[1] >= 1 && raw[2] >= 1 && raw[0] <= ng - 4 && raw[1] <= ng - 4 &&
                               raw[2] <= ng - 4);
         if (frozen) s_frozen = 1;
-        s_cls[i] = (unsigned char)(c | (frozen ? 32 : 0));   // bit 5: frozen
+        s_cls[i] = (unsigned char)(c | (frozen ? 32 : 0));   // class 0..15; bit 5: frozen
         s_rank[i] = (unsigned short)atomicAdd(&s_ccnt[c], 1);
     }
     __syncthreads();
-    if (tid < 64) {  // first wave: exclusive scan of the 32 class counts
-        const int h = tid < 32 ? s_ccnt[tid] : 0;
-        int incl = h;
-#pragma unroll
-        for (int o = 1; o < 32; o <<= 1) {
-            const int t = __shfl_up(incl, o);
-            if (tid >= o) incl += t;
-        }
-        if (tid < 32) s_coff[tid] = incl - h;
-        if (tid == 31) {
-            s_n = incl;
-            s_mul = interleave_mul(incl);
-            cnt_dst[bin] = incl;
-            cnt_dst[g.n_bins + 1 + bin] = incl;
-        }
+    if (tid == 0) {
+        int total = 0;
+        for (int c = 0; c < 16; ++c) total += s_ccnt[c];
+        s_n = total;
+        cnt_dst[bin] = total;
+        cnt_dst[g.n_bins + 1 + bin] = total;
     }
     __syncthreads();
-    const unsigned n = (unsigned)s_n, mul = (unsigned)s_mul;
+    const unsigned n = (unsigned)s_n;
     for (int i = tid; i < n_in; i += kWG) {
         if (s_cls[i] == 255) continue;
-        const int c = s_cls[i] & 31;
-        s_from[((unsigned)(s_coff[c] + s_rank[i]) * mul) % n] = (unsigned short)i;
+        const int c = s_cls[i] & 15, r = s_rank[i];
+        int pos = 0;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int ck = s_ccnt[k];
+            pos += min(ck, r) + ((k < c && ck > r) ? 1 : 0);
+        }
+        s_from[pos] = (unsigned short)i;
     }
     // Permuted copy staged through LDS, as many planes at a time as fit:
     // coalesced reads of whole planes (many loads in flight per thread), then
