@@ -512,7 +512,7 @@ def main():
 
     sim.run(args.warmup, dt)
     torch.cuda.synchronize()
-    period = args.sort_period or 32
+    period = args.sort_period or 16
     since0 = int(sim._bins["struct"].since_sort) if sim.use_bins and sim._bins is not None else 0
     t_start = torch.cuda.Event(enable_timing=True)
     t_end = torch.cuda.Event(enable_timing=True)
@@ -587,7 +587,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "sand column collapse, forward substep", "n_particles": n,
                    "path": "unbinned stage kernels" if args.unbinned else "binned G2P2G",
-                   "launch": {"wg_threads": args.wg_threads or 768, "sort_period": args.sort_period or 32},
+                   "launch": {"wg_threads": args.wg_threads or 768, "sort_period": args.sort_period or 16},
                    "n_grid": scene.n_grid, "box": "[0.3,0.7]^2x[0.05,0.6]", "dt": dt, "E": scene.E,
                    "nu": scene.nu, "friction_angle": scene.friction_angle, "density": scene.density,
                    "parallelism": "single GPU"},

@@ -253,7 +253,7 @@ typedef struct MpmBins {
     int32_t* count;       /* [2][2*nb^3 + 1]: per plane set, slots used per bin (nb^3),
                              slots used in the overflow run (1), live particles per bin (nb^3) */
     int32_t wg_threads;   /* threads per bin workgroup of the fused kernel: 256, 384, 512 or 768 (0 = 768) */
-    int32_t sort_period;  /* substeps between bank-order sorts (0 = 32, < 0 = never) */
+    int32_t sort_period;  /* substeps between bank-order sorts (0 = 16, < 0 = never) */
     float* record_acc;    /* NULL, or an n_grid^3 float4 grid that each grid update fills with the
                              substep's summed P2G (mv, m) -- the reference's grid_v_in and grid_m after
                              p2g2p (mpm_solver_warp.py:162); single scene only */

@@ -121,7 +121,7 @@ class SceneBatch:
 
     def set_launch(self, wg_threads: int = 0, sort_period: int = 0):
         """Threads per bin workgroup (256, 384, 512 or 768; 0 = 768) and substeps
-        between bank-order sorts (0 = 32, < 0 = never); speed only."""
+        between bank-order sorts (0 = 16, < 0 = never); speed only."""
         if wg_threads not in (0, 256, 384, 512, 768):
             raise ValueError("wg_threads must be 0, 256, 384, 512 or 768")
         self.bins.wg_threads, self.bins.sort_period = int(wg_threads), int(sort_period)
@@ -154,7 +154,7 @@ class SceneBatch:
         a HIP graph (mpm_batch_graph_create); run_captured replays it."""
         lib = L.load()
         if n_substeps is None:
-            period = self.bins.sort_period if self.bins.sort_period != 0 else 32
+            period = self.bins.sort_period if self.bins.sort_period != 0 else 16
             n_substeps = 2 * period if period > 0 else 64
         self.release_graph()
         stream = self._stream()

@@ -48,7 +48,7 @@ constexpr int kBinEdge = 8;           // cells per bin edge
 constexpr int kTE = kBinEdge + 2;     // tile edge: nodes b0 .. b0+9, the stencils of the bin's bases
 constexpr int kTE3 = kTE * kTE * kTE;
 constexpr int kSortMax = 6144;        // bins up to this size are processed in bank order
-constexpr int kSortPeriod = 32;       // substeps between bank-order sorts of the bins
+constexpr int kSortPeriod = 16;       // substeps between bank-order sorts of the bins
 constexpr int kStageFloats = 12288;   // LDS staging of the sort's copy (48 KB: two bins per CU)
 static_assert(kStageFloats >= kSortMax, "a bin's plane must fit the staging buffer");
 constexpr int kParts = 4;             // work items (each with its own P2G tile) a bin may be split into
@@ -845,16 +845,18 @@ __global__ void __launch_bounds__(kWG)
 // Fused G2P (this substep) + P2G (next substep), one workgroup per bin plus
 // workgroups over the overflow run.
 // ---------------------------------------------------------------------------
-// LDS bank class of a stencil base node in the 10^3 tiles: t mod 16, t the
+// LDS bank class of a stencil base node in the 10^3 tiles: t mod 32, t the
 // tile index of the base (origin b0).  gfx950's LDS serves an 8-byte access
 // of 16 consecutive lanes at once from 32 four-byte banks: the scatter's
 // ds_add_f64 of such a group runs without conflicts when its 16 nodes differ
-// mod 16 -- which holds when the group's particles are of 16 distinct classes
-// (the offset of a stencil node from the base is the same for all lanes of
-// one atomic).  Lanes 16 apart do not conflict.  The bank-order sort lays
-// bins out so that aligned groups of 16 slots hold distinct classes.
+// mod 16 (the offset of a stencil node from the base is the same for all
+// lanes of one atomic); lanes 16 apart do not conflict.  The gather's 4-byte
+// reads are served 32 lanes at a time and want the nodes to differ mod 32.
+// The bank-order sort lays bins out in rounds of the 32 classes in order, so
+// that aligned groups of 32 slots hold distinct classes and each aligned
+// half of them distinct classes mod 16.
 __device__ __forceinline__ int bank_class(const int base[3], const int b0[3]) {
-    return ((((base[0] - b0[0]) * kTE + (base[1] - b0[1])) * kTE + (base[2] - b0[2])) & 15);
+    return ((((base[0] - b0[0]) * kTE + (base[1] - b0[1])) * kTE + (base[2] - b0[2])) & 31);
 }
 
 // Launch shapes of k_g2p2g held at three waves per SIMD (~165 VGPRs, no
@@ -1087,8 +1089,8 @@ __global__ void __launch_bounds__(kWG)
 // (rank r, class c), and laid out round by round -- round r holds the r-th
 // particle of every class that has one, in class order, so the particle goes
 // to slot sum_c' min(count_c', r) + #{c' < c : count_c' > r}.  While every
-// class still has particles a round is 16 slots, aligned to the 16-lane
-// groups of the fused kernel's waves, with 16 distinct classes; once classes
+// class still has particles a round is 32 slots, aligned to the 32-lane
+// halves of the fused kernel's waves, with 32 distinct classes; once classes
 // run out the rounds shorten and a group may repeat classes (few particles in
 // a full bin, whose classes hold near-equal counts).  Bins larger than
 // kSortMax and the overflow run are copied as they are.
@@ -1111,7 +1113,7 @@ __device__ void interleave_bin(const float* __restrict__ src, const int* __restr
     __shared__ unsigned short s_from[kSortMax];  // source slot of each destination slot
     __shared__ unsigned short s_rank[kSortMax];
     __shared__ unsigned char s_cls[kSortMax];
-    __shared__ int s_ccnt[16];
+    __shared__ int s_ccnt[32];
     __shared__ int s_n, s_frozen;
     const int tid = threadIdx.x;
     int64_t off0;
@@ -1142,7 +1144,7 @@ __device__ void interleave_bin(const float* __restrict__ src, const int* __restr
         if (tid == 0) { cnt_dst[bin] = n_in; cnt_dst[g.n_bins + 1 + bin] = n_in; }
         return;
     }
-    if (tid < 16) s_ccnt[tid] = 0;
+    if (tid < 32) s_ccnt[tid] = 0;
     if (tid == 0) s_frozen = 0;
     __syncthreads();
     for (int i = tid; i < n_in; i += kWG) {
@@ -1162,13 +1164,13 @@ __device__ void interleave_bin(const float* __restrict__ src, const int* __restr
         const bool frozen = !(raw[0] >= 1 && raw[1] >= 1 && raw[2] >= 1 && raw[0] <= ng - 4 && raw[1] <= ng - 4 &&
                               raw[2] <= ng - 4);
         if (frozen) s_frozen = 1;
-        s_cls[i] = (unsigned char)(c | (frozen ? 32 : 0));   // class 0..15; bit 5: frozen
+        s_cls[i] = (unsigned char)(c | (frozen ? 64 : 0));   // class 0..31; bit 6: frozen
         s_rank[i] = (unsigned short)atomicAdd(&s_ccnt[c], 1);
     }
     __syncthreads();
     if (tid == 0) {
         int total = 0;
-        for (int c = 0; c < 16; ++c) total += s_ccnt[c];
+        for (int c = 0; c < 32; ++c) total += s_ccnt[c];
         s_n = total;
         cnt_dst[bin] = total;
         cnt_dst[g.n_bins + 1 + bin] = total;
@@ -1177,10 +1179,10 @@ __device__ void interleave_bin(const float* __restrict__ src, const int* __restr
     const unsigned n = (unsigned)s_n;
     for (int i = tid; i < n_in; i += kWG) {
         if (s_cls[i] == 255) continue;
-        const int c = s_cls[i] & 15, r = s_rank[i];
+        const int c = s_cls[i] & 31, r = s_rank[i];
         int pos = 0;
 #pragma unroll
-        for (int k = 0; k < 16; ++k) {
+        for (int k = 0; k < 32; ++k) {
             const int ck = s_ccnt[k];
             pos += min(ck, r) + ((k < c && ck > r) ? 1 : 0);
         }
@@ -1210,7 +1212,7 @@ __device__ void interleave_bin(const float* __restrict__ src, const int* __restr
         // v, C and tau of the frozen particles
         for (int q = tid; q < (int)n; q += kWG) {
             const int i = s_from[q];
-            if (!(s_cls[i] & 32)) continue;
+            if (!(s_cls[i] & 64)) continue;
             for (int pl = MPM_P_V; pl < MPM_P_TAU + 9; ++pl)
                 dst[off0 + (int64_t)pl * cap + q] = src[off0 + (int64_t)pl * cap + i];
         }

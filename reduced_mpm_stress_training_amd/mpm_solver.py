@@ -539,7 +539,7 @@ class MPM_Simulator_HIP:
 
     def set_launch(self, wg_threads: int = 0, sort_period: int = 0):
         """Launch parameters of the binned kernels: threads per bin workgroup
-        (256, 384, 512 or 768; 0 = 768) and substeps between bank-order sorts (0 = 32,
+        (256, 384, 512 or 768; 0 = 768) and substeps between bank-order sorts (0 = 16,
         < 0 = never).  They change speed only (see tune.py)."""
         if wg_threads not in (0, 256, 384, 512, 768):
             raise ValueError("wg_threads must be 0, 256, 384, 512 or 768")

@@ -23,7 +23,7 @@ import time
 
 import torch
 
-CANDIDATES = ((768, 32), (512, 32), (256, 32), (768, 64), (256, 64), (256, 16))
+CANDIDATES = ((768, 16), (768, 32), (512, 16), (256, 16), (256, 32), (256, 64))
 
 
 def autotune(factory, dt: float, trial_substeps: int = 32, warmup_substeps: int = 4,

@@ -101,7 +101,9 @@ def test_captured_run_matches_eager_run():
         sc.floor_start = 0.012
     dt = 1e-4
     eager, graph = B.SceneBatch(scenes), B.SceneBatch(scenes)
-    k = graph.capture(dt)
+    # 64 substeps per replay (four sort periods), so that the third replay
+    # crosses the colliders' start
+    k = graph.capture(dt, 64)
     assert k == 64
     for _ in range(3):
         eager.run(k, dt)
