@@ -100,15 +100,18 @@ __device__ __forceinline__ int chunk_tile_index(const int b[3], const int o[3]) 
 
 // Processing order of a chunk's particles (sorted slots first .. first+n_per,
 // whose particles sit at src[idx[slot]]): s_ord[q] = chunk index of the q-th
-// particle to process, in bank order (interleave_mul) of the LDS bank class
-// of each stencil base in the tile at origin o, so that the 32 lanes of a
-// half-wave gather and scatter through distinct banks.  Particles outside
-// the tile or the grid take class 0.  Scratch: s_cnt[64], s_mul; barriers
-// inside (call from every thread of the workgroup).
+// particle to process, in bank order of the LDS bank class t mod 32 of each
+// stencil base's tile index t in the tile at origin o: particle rank r of
+// class c goes to sum_c' min(count_c', r) + #{c' < c : count_c' > r}, so
+// that while every class has particles each aligned group of 32 holds the 32
+// classes in order -- its 4-byte gathers hit distinct banks, and each 16-lane
+// half, which gfx950 serves at once for 8-byte accesses, distinct banks for
+// the FP64 scatter.  Particles outside the tile or the grid take class 0.
+// Scratch: s_cnt[32]; barriers inside (call from every thread of the
+// workgroup).
 __device__ __forceinline__ void chunk_bank_order(const float* __restrict__ planes, int64_t S,
                                                  const int* __restrict__ idx, int first, int n_per, int n,
-                                                 float inv_dx, const int o[3], unsigned short* s_ord, int* s_cnt,
-                                                 int* s_mul) {
+                                                 float inv_dx, const int o[3], unsigned short* s_ord, int* s_cnt) {
     const int tid = threadIdx.x;
     if (tid < 32) s_cnt[tid] = 0;
     __syncthreads();
@@ -134,22 +137,18 @@ __device__ __forceinline__ void chunk_bank_order(const float* __restrict__ plane
         rank[r] = atomicAdd(&s_cnt[c], 1);
     }
     __syncthreads();
-    if (tid == 0) {
-        int acc = 0;
-        for (int c = 0; c < 32; ++c) {
-            s_cnt[32 + c] = acc;
-            acc += s_cnt[c];
-        }
-        *s_mul = interleave_mul(n_per);
-    }
-    __syncthreads();
-    const int mul = *s_mul;
 #pragma unroll
     for (int r = 0; r < kChunkMax / kChunkWG; ++r) {
         const int i = tid + r * kChunkWG;
         if (i >= n_per) continue;
-        const int k = s_cnt[32 + cls[r]] + rank[r];
-        s_ord[n_per > 1 ? (int)(((long long)k * mul) % n_per) : 0] = (unsigned short)i;
+        const int c = cls[r], rk = rank[r];
+        int pos = 0;
+#pragma unroll
+        for (int k = 0; k < 32; ++k) {
+            const int ck = s_cnt[k];
+            pos += min(ck, rk) + ((k < c && ck > rk) ? 1 : 0);
+        }
+        s_ord[pos] = (unsigned short)i;
     }
     __syncthreads();
 }

@@ -207,8 +207,7 @@ __global__ void __launch_bounds__(kChunkWG)
                        const int4* __restrict__ chunks, const int* __restrict__ meta, float inv_dx, int ng,
                        int* __restrict__ perm) {
     __shared__ unsigned short s_ord[kChunkMax];
-    __shared__ int s_cnt[64];
-    __shared__ int s_mul;
+    __shared__ int s_cnt[32];
     const int c = blockIdx.x;
     if (c >= meta[0]) return;
     const int4 ch = chunks[c];
@@ -217,7 +216,7 @@ __global__ void __launch_bounds__(kChunkWG)
     if (ch.x < 0 || ch.x + n_per > n || n_per < 1) return;
     int o[3];
     chunk_origin(ch.z, (ng + 7) / 8, o);
-    chunk_bank_order(src, SS, perm, ch.x, n_per, n, inv_dx, o, s_ord, s_cnt, &s_mul);   // barriers inside
+    chunk_bank_order(src, SS, perm, ch.x, n_per, n, inv_dx, o, s_ord, s_cnt);   // barriers inside
     constexpr int R = kChunkMax / kChunkWG;
     int from[R];
 #pragma unroll

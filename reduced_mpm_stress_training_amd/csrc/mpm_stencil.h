@@ -258,28 +258,4 @@ __device__ __forceinline__ void gather_global_rolled(const float4* __restrict__ 
     }
 }
 
-// Bank order.  Slots q of a run of n particles are filled with the
-// particles' counting sort by LDS bank class (sorted index k) as
-// q = k * mul mod n, mul the inverse of s ~ n/32 modulo n: slots q and q+1
-// then hold sorted entries s apart, about one class apart, so any 32
-// consecutive slots span nearly all 32 classes.
-__device__ __forceinline__ int gcd_int(int a, int b) {
-    while (b) { const int t = a % b; a = b; b = t; }
-    return a;
-}
-
-// mul = s^-1 mod n, s the smallest integer >= ceil(n/32) coprime to n
-__device__ __forceinline__ int interleave_mul(int n) {
-    if (n <= 1) return 0;
-    int st = max(1, (n + 31) >> 5);
-    while (gcd_int(st, n) != 1) ++st;
-    int r0 = n, r1 = st, t0 = 0, t1 = 1;  // extended Euclid on (n, st)
-    while (r1) {
-        const int qq = r0 / r1;
-        int t = r0 - qq * r1; r0 = r1; r1 = t;
-        t = t0 - qq * t1; t0 = t1; t1 = t;
-    }
-    return t0 < 0 ? t0 + n : t0;
-}
-
 }  // namespace mpm
