@@ -42,7 +42,7 @@
 extern "C" {
 #endif
 
-#define MPM_ABI_VERSION 14
+#define MPM_ABI_VERSION 15
 
 /* plane indices */
 #define MPM_P_X0 0
@@ -267,9 +267,11 @@ typedef struct MpmBins {
  * that left the grid (held at their last position), 3: grid nodes whose
  * P2G sum was NaN or Inf (a particle's state was not finite); all four
  * accumulate until the caller clears them; 48-49: the start time, as a double, of a captured run,
- * mpm_batch_graph_launch), tile stamps, the fused kernel's schedule, nb^3
- * P2G tiles of 12^3 float4 and nb^3 FP64 tiles of 12^3 x 4 doubles for the
- * parts of split bins. */
+ * mpm_batch_graph_launch), tile stamps, the lists of grid blocks each epoch's
+ * P2G wrote (the grid update visits only those) and per block the epoch of
+ * its last velocity update, the fused kernel's schedule, nb^3 P2G tiles of
+ * 10^3 float4 and nb^3 FP64 tiles of 10^3 x 4 doubles for the parts of split
+ * bins. */
 int64_t mpm_bins_workspace_bytes(const MpmParams* prm);
 
 /* Byte offset of the status counters in the workspace. */
@@ -277,8 +279,16 @@ int64_t mpm_bins_status_offset(const MpmParams* prm);
 
 /* Byte offset of the velocity grid (n_grid^3 float4: v after the grid
  * update, m) in the workspace: the reference's grid_v_out and grid_m
- * (mpm_solver_warp.py:65-67) of the last substep. */
+ * (mpm_solver_warp.py:65-67) of the last substep once mpm_bins_settle_grid
+ * has run.  The grid update writes only the 8^3 blocks that hold P2G sums;
+ * blocks the particles have left keep their last values until then. */
 int64_t mpm_bins_velocity_offset(const MpmParams* prm);
+
+/* Zeroes the velocity grid's blocks that the last grid update did not write
+ * (zero_grid, mpm_utils.py:115-123, clears every node each substep; the
+ * substeps here never read those nodes): call before reading the velocity
+ * grid as the reference's grid_v_out / grid_m.  No effect on the particles. */
+int mpm_bins_settle_grid(const MpmParams* prm, const MpmBins* bins, void* workspace, void* stream);
 
 /* Clears count[cur] and inserts every particle of a flat state (slot =
  * caller index, as MpmState) into the bins of plane set `cur`. */

@@ -10,7 +10,7 @@ import os
 
 from ._build import LIB_PATH
 
-MPM_ABI_VERSION = 14
+MPM_ABI_VERSION = 15
 MAX_COLLIDERS = 16
 
 # plane indices of the particle state buffer
@@ -99,6 +99,7 @@ _SIGNATURES = {
     "mpm_bins_status_offset": (C.c_int64, [C.POINTER(MpmParams)]),
     "mpm_bins_velocity_offset": (C.c_int64, [C.POINTER(MpmParams)]),
     "mpm_bins_clear_grid": (C.c_int, [C.POINTER(MpmParams), C.POINTER(MpmBins), C.c_void_p, C.c_void_p]),
+    "mpm_bins_settle_grid": (C.c_int, [C.POINTER(MpmParams), C.POINTER(MpmBins), C.c_void_p, C.c_void_p]),
     "mpm_bins_substep": (C.c_int, [C.POINTER(MpmParams), C.POINTER(MpmBins), C.POINTER(MpmCollider), C.c_int,
                                    C.c_void_p, C.c_float, C.c_float, C.c_void_p]),
     "mpm_bins_grid_update": (C.c_int, [C.POINTER(MpmParams), C.POINTER(MpmBins), C.POINTER(MpmCollider), C.c_int,

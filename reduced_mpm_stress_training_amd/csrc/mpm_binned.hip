@@ -210,17 +210,23 @@ __device__ __forceinline__ void load_passes(const int* __restrict__ cnt_cur, int
 }
 
 // prefix sums of s_hist in decreasing key order into s_cursor; s_total = the
-// count, s_max = the largest key present (0 if none)
+// count, s_max = the largest key present (0 if none).  Wave 0, lane l holding
+// key kSchedKeys - 1 - l: a 64-lane scan instead of a serial walk.
+static_assert(kSchedKeys == 64, "hist_cursors gives one key to each lane of a wave");
 __device__ __forceinline__ void hist_cursors(const int* s_hist, int* s_cursor, int* s_total, int* s_max) {
-    if (threadIdx.x == 0) {
-        int pos = 0, top = 0;
-        for (int k = kSchedKeys - 1; k >= 1; --k) {
-            s_cursor[k] = pos;
-            pos += s_hist[k];
-            if (top == 0 && s_hist[k] > 0) top = k;
+    if (threadIdx.x < 64) {
+        const int lane = threadIdx.x, k = kSchedKeys - 1 - lane;
+        const int h = k >= 1 ? s_hist[k] : 0;
+        int incl = h;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int o = __shfl_up(incl, d);
+            if (lane >= d) incl += o;
         }
-        *s_total = pos;
-        *s_max = top;
+        if (k >= 1) s_cursor[k] = incl - h;
+        const unsigned long long nz = __ballot(h > 0);
+        if (lane == 63) *s_total = incl;
+        if (lane == 0) *s_max = nz ? kSchedKeys - __ffsll((long long)nz) : 0;
     }
 }
 
@@ -353,8 +359,26 @@ __device__ void schedule_bins(const int* __restrict__ cnt_cur, const BinGeom& g,
     if (tid == 0) items[kItemsEpoch] = epoch;
 }
 
-__device__ __forceinline__ void mark_block(int* __restrict__ gflag, int nb, int x, int y, int z) {
-    gflag[((x >> 3) * nb + (y >> 3)) * nb + (z >> 3)] = 1;
+// The grid blocks that hold P2G sums of an epoch, listed once each by the
+// writer that marks them first (a tile flush or the spill scatter): the grid
+// update visits only these.  Two lists, by the epoch's parity: the grid
+// update of epoch e reads list e & 1 and empties list (e + 1) & 1 for the
+// next epoch's writers.
+struct BlockList {
+    int* __restrict__ mark;   // per grid block (global index): the newest epoch it was listed for
+    int* __restrict__ list;   // [2][n] grid blocks
+    int* __restrict__ head;   // [0], [1]: list lengths by epoch parity; [2]: epoch of the last grid update (0: none)
+    int n;                    // grid blocks of all scenes (= bins)
+};
+
+// The list append of grid block blk once its mark's previous value `old`
+// (atomicExch(&bl.mark[blk], epoch)) is known: the first writer of the epoch
+// appends.
+__device__ __forceinline__ void list_append(const BlockList& bl, int blk, int epoch, int old) {
+    if (old != epoch) {
+        const int i = atomicAdd(&bl.head[epoch & 1], 1);
+        if (i < bl.n) bl.list[(epoch & 1) * bl.n + i] = blk;
+    }
 }
 
 // Scatter of one particle straight into the spill grid with FP64 atomics
@@ -362,7 +386,7 @@ __device__ __forceinline__ void mark_block(int* __restrict__ gflag, int nb, int 
 // FP32 contributions sum in FP64 as in the LDS tiles, so the node's value
 // does not depend on the order the atomics land in unless the sum needs more
 // than 53 bits.
-__device__ void scatter_spill(double* __restrict__ acc, int* __restrict__ gflag, const SceneDesc& sc,
+__device__ void scatter_spill(double* __restrict__ acc, int* __restrict__ gflag, int epoch, const SceneDesc& sc,
                               const AxisWeights& A, float m, float vol, const float v[3], const float C[9],
                               const float tau[9], float dt) {
     const int ng = sc.prm.n_grid;
@@ -371,13 +395,17 @@ __device__ void scatter_spill(double* __restrict__ acc, int* __restrict__ gflag,
         const int x = A.base[0] + i, y = A.base[1] + j, z = A.base[2] + k;
         unsafeAtomicAdd(sacc + 4 * (((int64_t)x * ng + y) * ng + z) + c, (double)val);
     });
+    // the grid blocks of the stencil's nodes (base + 0, 2 per axis) hold
+    // spill contributions of this epoch: plain stores, nothing waits on them
+    // (the grid update finds blocks no tile listed by scanning these flags)
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j)
 #pragma unroll
             for (int k = 0; k < 2; ++k)
-                mark_block(gflag + sc.bin_base, sc.nb, A.base[0] + 2 * i, A.base[1] + 2 * j, A.base[2] + 2 * k);
+                gflag[sc.bin_base + (((A.base[0] + 2 * i) >> 3) * sc.nb + ((A.base[1] + 2 * j) >> 3)) * sc.nb +
+                      ((A.base[2] + 2 * k) >> 3)] = epoch;
 }
 
 // Scatter of one particle into a bin's FP64 LDS tile (SoA: mx, my, mz, m).
@@ -389,32 +417,32 @@ __device__ __forceinline__ void scatter_tile(double* tile, int i0, const AxisWei
     });
 }
 
-// Writes a bin's LDS tile (nodes b0 .. b0+9) to its slot of the global
-// tile buffer as float4 (mv, m) and stamps it with the epoch.  tid: the
-// thread's index in the workgroup.
-// Marks the grid blocks that bin `bin`'s tile overlaps (nodes 8B .. 8B+9
-// along each axis: blocks B and B+1, up to 8) with the tile's epoch, so that
-// the grid update passes over every other block after one load, without the
-// scene lookup and the stamp loads.  tid: the thread's index in the
-// workgroup.
-__device__ __forceinline__ void mark_tile_blocks(int* __restrict__ blive, const SceneDesc& sc, int bin, int epoch,
+// Lists the grid blocks that bin `bin`'s tile overlaps (nodes 8B .. 8B+9
+// along each axis: blocks B and B+1, up to 8) for the tile's epoch.  tid:
+// the thread's index in the workgroup.
+__device__ __forceinline__ void mark_tile_blocks(const BlockList& bl, const SceneDesc& sc, int bin, int epoch,
                                                  int tid) {
     if (tid < 8) {
         const int nb = sc.nb, lb = bin - sc.bin_base;
         const int x = lb / (nb * nb) + (tid >> 2), y = (lb / nb) % nb + ((tid >> 1) & 1), z = lb % nb + (tid & 1);
-        if (x < nb && y < nb && z < nb) blive[sc.bin_base + (x * nb + y) * nb + z] = epoch;
+        if (x < nb && y < nb && z < nb) {
+            const int blk = sc.bin_base + (x * nb + y) * nb + z;
+            list_append(bl, blk, epoch, atomicExch(&bl.mark[blk], epoch));
+        }
     }
 }
 
+// Writes a bin's LDS tile (nodes b0 .. b0+9) to its slot of the global
+// tile buffer as float4 (mv, m) and stamps it with the epoch (the writer
+// lists the grid blocks it overlaps: mark_tile_blocks).  tid: the thread's
+// index in the workgroup.
 __device__ __forceinline__ void write_tile(const double* s_acc, float4* __restrict__ tiles,
-                                           int* __restrict__ stamps, int* __restrict__ blive, const SceneDesc& sc,
-                                           int bin, int epoch, int tid) {
+                                           int* __restrict__ stamps, int bin, int epoch, int tid) {
     float4* out = tiles + (int64_t)bin * kTE3;
     for (int t = tid; t < kTE3; t += blockDim.x)
         out[t] = make_float4((float)s_acc[t], (float)s_acc[kTE3 + t], (float)s_acc[2 * kTE3 + t],
                              (float)s_acc[3 * kTE3 + t]);
     if (tid == 0) stamps[bin] = epoch;
-    mark_tile_blocks(blive, sc, bin, epoch, tid);
 }
 
 
@@ -433,7 +461,7 @@ struct SubstepArgs {
     int* __restrict__ status;
     float4* __restrict__ tiles;
     int* __restrict__ stamps;       // epoch of each bin's tile; -epoch: split, tiles in ptiles
-    int* __restrict__ blive;        // epoch of the newest tile overlapping each grid block
+    BlockList bl;                   // grid blocks written this epoch (tiles, spill)
     double* __restrict__ ptiles;    // FP64 tiles of split bins' parts, one slot per work item
     int* __restrict__ pcount;       // parts of each split bin
     int* __restrict__ pslot;        // ptiles slot of part p of bin b: pslot[p * n_bins + b]
@@ -643,22 +671,67 @@ __device__ __forceinline__ void g2p2g_particle(const SubstepArgs& a, const Scene
     if (mode == 1) {
         scatter_tile(tile_acc, i0, A, m, vol, v, Cm, tau, prm.dx, prm.inv_dx, dt);
     } else if (mode == 2) {
-        scatter_spill(a.acc, a.gflag, sc, A, m, vol, v, Cm, tau, dt);
+        scatter_spill(a.acc, a.gflag, a.epoch_out, sc, A, m, vol, v, Cm, tau, dt);
     }
 }
 
 }  // namespace
 
 // ---------------------------------------------------------------------------
-// Grid update of one 8^3 grid block: gathers the P2G tiles of the current
-// epoch from the bins overlapping it (the tile of bin B spans nodes 8B ..
-// 8B+9, so a node x is covered by the tiles of bins x>>3 and, when
-// x mod 8 <= 1, the bin before) and the spill grid, sums in FP64, then
-// normalization, gravity and colliders.
-// Also resets the next plane set's bin counters.  clear_only: only drains
-// the spill grid.  Workgroup 0 builds the schedule of k_g2p2g from the
-// current bin counts (when `order` is given); workgroup 1 + b updates grid
-// block b.
+// The tile entries a grid block sums, by source c = (ix, iy, iz) = the tile
+// of bin (bx - ix, by - iy, bz - iz): the tile of bin B spans nodes 8B ..
+// 8B+9, so source c covers the block's nodes whose local coordinate is < 2
+// along each axis it steps down -- 8 or 2 nodes per axis, 1000 entries in
+// all.  Source 0 (the block's own bin: 512 entries) is read straight into
+// registers; the halo sources 1..7 are staged in LDS from halo_off(c) on.
+constexpr int halo_size(int c) { return ((c & 4) ? 2 : 8) * ((c & 2) ? 2 : 8) * ((c & 1) ? 2 : 8); }
+constexpr int halo_off(int c) { return c == 0 ? 0 : halo_off(c - 1) + halo_size(c - 1); }
+constexpr int kHaloEnd = halo_off(8);
+constexpr int kHalo = kHaloEnd - 512;   // halo entries of a block
+static_assert(kHaloEnd == 1000, "the block's tile entries: 10^3");
+// Each half of a grid-update workgroup (kBlockThreads threads) takes one
+// block, two nodes per thread: four waves per block leave each thread the
+// registers of two nodes' loads in flight at four waves per SIMD, so that the
+// blocks of a 1M-particle column are all resident at once.
+constexpr int kBlockThreads = kWG / 2;
+static_assert(kWG == 512 && kBlockThreads == 256, "two 8^3 blocks per workgroup, two nodes per thread");
+
+// Stage halo entry e (kHaloEnd > e >= 512) of block (bx, by, bz) of a scene
+// with nb^3 bins whose tiles start at btiles.  Entries of sources outside the
+// grid read as zero; those of sources without a current tile are loaded all
+// the same (in bounds, dropped at the sum).
+__device__ __forceinline__ float4 halo_entry(const float4* __restrict__ btiles, int nb, int bx, int by, int bz,
+                                             int e) {
+    int c = 1, off = halo_off(1);
+#pragma unroll
+    for (int k = 2; k < 8; ++k)
+        if (e >= halo_off(k)) {
+            c = k;
+            off = halo_off(k);
+        }
+    const int ix = c >> 2, iy = (c >> 1) & 1, iz = c & 1;
+    const int ley = iy ? 1 : 3, lez = iz ? 1 : 3;
+    const int r = e - off;
+    const int u = r >> (ley + lez), v = (r >> lez) & ((1 << ley) - 1), q = r & ((1 << lez) - 1);
+    const int sbx = bx - ix, sby = by - iy, sbz = bz - iz;
+    if (sbx < 0 || sby < 0 || sbz < 0) return make_float4(0.f, 0.f, 0.f, 0.f);
+    return btiles[(int64_t)((sbx * nb + sby) * nb + sbz) * kTE3 + ((u + 8 * ix) * kTE + (v + 8 * iy)) * kTE +
+                  (q + 8 * iz)];
+}
+
+// Grid update over the grid blocks that hold P2G sums of this epoch: those
+// listed by the tile flushes (BlockList, this epoch's parity) and those only
+// the spill scatter wrote (spill flag gflag == epoch and not listed; each
+// workgroup scans a slice of the flags for them).  For each block: sums the
+// P2G tiles of the current epoch from the bins overlapping it and the spill
+// grid in FP64, then normalization, gravity and colliders.  Blocks holding
+// no P2G sum are not visited: G2P reads none of their nodes, and their
+// velocities are left as they are (k_settle_vel zeroes them for readers of
+// the whole grid).
+// Also resets the next plane set's bin counters and empties the next
+// epoch's list.  clear_only: only drains the spill grid.  Workgroup 0
+// builds the schedule of k_g2p2g from the current bin counts (when `order`
+// is given); workgroups 1 .. gridDim.x - 1 take the blocks two at a time.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kWG)
     k_grid_bins(double4* __restrict__ acc, float4* __restrict__ vel, int* __restrict__ gflag,
@@ -666,179 +739,205 @@ __global__ void __launch_bounds__(kWG)
                 const int* __restrict__ pslot, const double* __restrict__ ptiles, int* __restrict__ cnt_nxt,
                 const int* __restrict__ cnt_cur, int* __restrict__ order, BinGeom g, SceneTab tab,
                 ColliderSet cs, float time, float dt, int epoch, int clear_only, const double* __restrict__ t_base,
-                float4* __restrict__ rec, int* __restrict__ status, const int* __restrict__ blive) {
-    __shared__ int s_live[27];             // neighbour bin c has a whole-bin tile of this epoch
-    __shared__ int s_pslot[kParts * 27];   // FP64 tile slot of part p of neighbour bin c this epoch, or -1
-    __shared__ int s_any, s_anyp, s_bad;
+                float4* __restrict__ rec, int* __restrict__ status, BlockList bl, int* __restrict__ bvel) {
+    // per half: neighbour bin c has a whole-bin tile of this epoch; FP64 tile
+    // slot of part p of neighbour bin c this epoch, or -1; flags; halo
+    // entries (halo_off).  s_extra: the spill-only blocks of a slice.
+    __shared__ int s_live[2][27];
+    __shared__ int s_pslot[2][kParts * 27];
+    __shared__ int s_any[2], s_anyp[2], s_bad[2];
+    __shared__ float4 s_halo[2][kHalo];
+    __shared__ int s_extra[kWG], s_nextra;
+    const int tid = threadIdx.x;
     if (blockIdx.x == 0) {
+        if (tid == 0) {
+            bl.head[(epoch + 1) & 1] = 0;        // the next epoch's writers list from scratch
+            bl.head[2] = clear_only ? 0 : epoch;
+        }
         if (order) schedule_bins(cnt_cur, g, epoch, order);
         return;
     }
-    const int blk = blockIdx.x - 1;   // global block (= global bin index)
-    const int tid = threadIdx.x;
-    if (tid == 0 && cnt_nxt) {
-        cnt_nxt[blk] = 0;                 // slots
-        cnt_nxt[g.n_bins + 1 + blk] = 0;  // live particles
-        if (blk == 0) cnt_nxt[g.n_bins] = 0;
-    }
-    // most blocks hold nothing: no tile of this epoch overlaps them (the
-    // tiles' writers mark the blocks they overlap), no spill and no velocities
-    // of an earlier grid update to clear -- they leave here, before the scene
-    // lookup and the neighbours' stamps
-    if (gflag[blk] == 0 && (clear_only || blive[blk] != epoch)) return;
+    const int nw = gridDim.x - 1, w = blockIdx.x - 1;
+    // half (whole waves: wave-uniform), thread in it
+    const int hf = __builtin_amdgcn_readfirstlane(tid / kBlockThreads), ht = tid % kBlockThreads;
     // a captured run reads its start time from the device: `time` is then the offset
     const float tnow = t_base ? (float)(t_base[0] + (double)time) : time;
-    const SceneDesc sc = scene_of_bin(tab, blk);
-    const int nb = sc.nb, ng = sc.prm.n_grid;
-    const int lblk = blk - sc.bin_base;
-    const int bz = lblk % nb, by = (lblk / nb) % nb, bx = lblk / (nb * nb);
-    acc += sc.node_base;
-    vel += sc.node_base;
-    stamps += sc.bin_base;
-    pcount += sc.bin_base;
-    pslot += sc.bin_base;
-    tiles += (int64_t)sc.bin_base * kTE3;
-    const MpmCollider* cols = tab.dev ? tab.cols + sc.col_base : cs.c;
-    const int n_cols = tab.dev ? sc.n_cols : cs.n;
-    if (tid == 0) s_any = s_anyp = s_bad = 0;
-    __syncthreads();
-    if (tid < 27) {
-        const int c = tid;
-        const int x = bx + c / 9 - 1, y = by + (c / 3) % 3 - 1, z = bz + c % 3 - 1;
-        const bool in = !clear_only && x >= 0 && y >= 0 && z >= 0 && x < nb && y < nb && z < nb;
-        const int lb = in ? (x * nb + y) * nb + z : 0;
-        const int st = in ? stamps[lb] : 0;
-        const int live = in && st == epoch;
-        s_live[c] = live;
-        if (live) atomicOr(&s_any, 1);
-        // a split bin (stamp -epoch): its parts' FP64 tile slots
-        const bool split = in && st == -epoch && epoch > 0;
-        const int n_parts = split ? min(pcount[lb], kParts) : 0;
-#pragma unroll
-        for (int part = 0; part < kParts; ++part)
-            s_pslot[part * 27 + c] = part < n_parts ? pslot[(int64_t)part * g.n_bins + lb] : -1;
-        if (split) atomicOr(&s_anyp, 1);
+    const int* lst = bl.list + (epoch & 1) * bl.n;
+    // one round of loads: the list's length, this workgroup's first entry and
+    // the first chunk of its slice of spill flags and list marks
+    const int n_list = min(bl.head[epoch & 1], bl.n);
+    const int first = 2 * w + hf < bl.n ? lst[2 * w + hf] : 0;
+    const int slice = (g.n_bins + nw - 1) / nw, s0 = w * slice, s1 = min(g.n_bins, s0 + slice);
+    int sflag = 0, smark = 0;
+    if (s0 + tid < s1) {
+        sflag = gflag[s0 + tid];
+        smark = bl.mark[s0 + tid];
     }
-    // gflag bit 0: the spill grid holds contributions in this block; bit 1:
-    // the block's velocity nodes were written by an earlier grid update
-    const int gf = gflag[blk];
-    const int spill = gf & 1, written = gf & 2;
-    __syncthreads();
-    const bool live = spill || s_any || s_anyp;
-    if (!live && !written) return;
-    if (!live || clear_only) {
-        // drain the spill grid; a block that particles have left gets its
-        // velocity nodes zeroed once (zero_grid, mpm_utils.py:115-123)
-        for (int t = tid; t < 512; t += kWG) {
-            const int x = bx * 8 + (t >> 6), y = by * 8 + ((t >> 3) & 7), z = bz * 8 + (t & 7);
-            if (x >= ng || y >= ng || z >= ng) continue;
-            const int64_t node = ((int64_t)x * ng + y) * ng + z;
-            if (spill) acc[node] = make_double4(0.0, 0.0, 0.0, 0.0);
-            if (written) vel[node] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (ht == 0) s_any[hf] = s_anyp[hf] = s_bad[hf] = 0;
+    if (tid == 0) s_nextra = 0;
+    // thread ht: nodes (lx, ly, lz) and (lx + 4, ly, lz) of its half's block
+    const int lx = ht >> 6, ly = (ht >> 3) & 7, lz = ht & 7;
+
+    // Blocks blk (has) of the two halves: called by every thread.
+    auto update_pair = [&](bool has, int blk) {
+        __syncthreads();   // the previous blocks are done with the shared arrays; the flags are reset
+        blk = has ? __builtin_amdgcn_readfirstlane(blk) : 0;   // global block (= global bin index)
+        const SceneDesc sc = scene_of_bin(tab, blk);
+        const int nb = sc.nb, ng = sc.prm.n_grid;
+        const int lblk = blk - sc.bin_base;
+        const int bz = lblk % nb, by = (lblk / nb) % nb, bx = lblk / (nb * nb);
+        double4* const bacc = acc + sc.node_base;
+        const float4* const btiles = tiles + (int64_t)sc.bin_base * kTE3;
+        // one round of loads: the neighbours' stamps, the spill flag, the
+        // block's tile entries and its spill sums
+        if (has && ht < 27) {
+            const int c = ht;
+            const int x = bx + c / 9 - 1, y = by + (c / 3) % 3 - 1, z = bz + c % 3 - 1;
+            const bool in = !clear_only && x >= 0 && y >= 0 && z >= 0 && x < nb && y < nb && z < nb;
+            const int lb = in ? (x * nb + y) * nb + z : 0;
+            const int st = in ? stamps[sc.bin_base + lb] : 0;
+            const int live = in && st == epoch;
+            s_live[hf][c] = live;
+            if (live) atomicOr(&s_any[hf], 1);
+            // a split bin (stamp -epoch): its parts' FP64 tile slots
+            const bool split = in && st == -epoch && epoch > 0;
+            const int n_parts = split ? min(pcount[sc.bin_base + lb], kParts) : 0;
+#pragma unroll
+            for (int part = 0; part < kParts; ++part)
+                s_pslot[hf][part * 27 + c] =
+                    part < n_parts ? pslot[(int64_t)part * g.n_bins + sc.bin_base + lb] : -1;
+            if (split) atomicOr(&s_anyp[hf], 1);
         }
-        __syncthreads();
-        if (tid == 0) gflag[blk] = 0;
-        return;
-    }
-    for (int t = tid; t < 512; t += kWG) {
-        const int x = bx * 8 + (t >> 6), y = by * 8 + ((t >> 3) & 7), z = bz * 8 + (t & 7);
-        if (x >= ng || y >= ng || z >= ng) continue;
-        const int64_t node = ((int64_t)x * ng + y) * ng + z;
-        double sx = 0.0, sy = 0.0, sz = 0.0, sm = 0.0;
-        if (spill) {
-            const double4 a4 = acc[node];
-            sx = a4.x; sy = a4.y; sz = a4.z; sm = a4.w;
-            acc[node] = make_double4(0.0, 0.0, 0.0, 0.0);
-        }
-        // along each axis the tile of bin B holds nodes 8B .. 8B+9: node p is
-        // in the tile of bin p >> 3 and, when p mod 8 <= 1, of the bin before
-        const int p[3] = {x, y, z};
-        int hi[3], nlo[3];
+        // the spill grid holds contributions of this epoch in this block
+        const bool spill = has && gflag[blk] == epoch;
+        float4 own[2] = {make_float4(0.f, 0.f, 0.f, 0.f), make_float4(0.f, 0.f, 0.f, 0.f)};
+        double4 sp[2] = {make_double4(0.0, 0.0, 0.0, 0.0), make_double4(0.0, 0.0, 0.0, 0.0)};
+        if (has && !clear_only) {
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            hi[d] = p[d] >> 3;
-            nlo[d] = ((p[d] & 7) <= 1 && hi[d] >= 1) ? 2 : 1;
-        }
-        const int bb[3] = {bx, by, bz};
-        // the (up to 8) bins covering the node: their neighbour index and the
-        // node's index in their tiles
-        int li[8], lt[8];
-        bool cov[8];
+            for (int k = 0; k < 2; ++k)
+                own[k] = btiles[(int64_t)lblk * kTE3 + ((lx + 4 * k) * kTE + ly) * kTE + lz];
 #pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            const int ix = c >> 2, iy = (c >> 1) & 1, iz = c & 1;
-            const int b[3] = {hi[0] - ix, hi[1] - iy, hi[2] - iz};
-            cov[c] = ix < nlo[0] && iy < nlo[1] && iz < nlo[2] && b[0] < nb && b[1] < nb && b[2] < nb;
-            li[c] = ((b[0] - bb[0] + 1) * 3 + (b[1] - bb[1] + 1)) * 3 + (b[2] - bb[2] + 1);
-            const int lx = p[0] - 8 * b[0], ly = p[1] - 8 * b[1], lz = p[2] - 8 * b[2];
-            lt[c] = (lx * kTE + ly) * kTE + lz;
-        }
-        // whole-bin tiles, loaded all at once (absent ones read tile slot 0
-        // and are dropped), summed in (ix, iy, iz) order
-        if (s_any) {
-#pragma unroll
-            for (int h = 0; h < 8; h += 4) {
-                float4 tv[4];
-                bool use[4];
-#pragma unroll
-                for (int cc = 0; cc < 4; ++cc) {
-                    const int c = h + cc;
-                    const int ix = c >> 2, iy = (c >> 1) & 1, iz = c & 1;
-                    use[cc] = cov[c] && s_live[li[c]] != 0;
-                    const int64_t ti =
-                        (int64_t)(((hi[0] - ix) * nb + (hi[1] - iy)) * nb + (hi[2] - iz)) * kTE3 + lt[c];
-                    tv[cc] = tiles[use[cc] ? ti : 0];
-                }
-#pragma unroll
-                for (int cc = 0; cc < 4; ++cc)
-                    if (use[cc]) { sx += tv[cc].x; sy += tv[cc].y; sz += tv[cc].z; sm += tv[cc].w; }
+            for (int k = 0; k < 2; ++k) {
+                const int e = 512 + k * kBlockThreads + ht;
+                if (e < kHaloEnd) s_halo[hf][e - 512] = halo_entry(btiles, nb, bx, by, bz, e);
             }
         }
-        // the FP64 tiles of split bins' parts
-        if (s_anyp) {
-            // (two tiles at a time: this path is taken only next to split
-            // bins, and the registers of a wider batch would lower the
-            // occupancy of every grid block)
-#pragma unroll 1
-            for (int k = 0; k < 8 * kParts; k += 2) {
-                int64_t off[2];
-                bool use[2];
+        if (spill && !clear_only) {
 #pragma unroll
-                for (int u = 0; u < 2; ++u) {
-                    const int part = (k + u) >> 3, c = (k + u) & 7;
-                    const int q[3] = {x, y, z}, d[3] = {c >> 2, (c >> 1) & 1, c & 1};
-                    int b[3];
-                    bool cv = true;
+            for (int k = 0; k < 2; ++k) {
+                const int x = bx * 8 + lx + 4 * k, y = by * 8 + ly, z = bz * 8 + lz;
+                if (x < ng && y < ng && z < ng) sp[k] = bacc[((int64_t)x * ng + y) * ng + z];
+            }
+        }
+        __syncthreads();   // stamps and halo entries in LDS
+        const bool work = has && !clear_only && (spill || s_any[hf] || s_anyp[hf]);
+        if (work) {
+            // the two nodes' sums first, then their collider passes: the
+            // colliders' registers are not held beside the sums'
+            float4 a4[2];
 #pragma unroll
-                    for (int e = 0; e < 3; ++e) {
-                        const int h = q[e] >> 3;
-                        b[e] = h - d[e];
-                        cv = cv && b[e] < nb && (d[e] == 0 || ((q[e] & 7) <= 1 && h >= 1));
+            for (int k = 0; k < 2; ++k) {
+                const int nx = lx + 4 * k;
+                const int x = bx * 8 + nx, y = by * 8 + ly, z = bz * 8 + lz;
+                double sx = sp[k].x, sy = sp[k].y, sz = sp[k].z, sm = sp[k].w;
+                // whole-bin tiles in (ix, iy, iz) order
+                if (s_any[hf]) {
+#pragma unroll
+                    for (int c = 0; c < 8; ++c) {
+                        const int ix = c >> 2, iy = (c >> 1) & 1, iz = c & 1;
+                        const bool cov = (!ix || nx < 2) && (!iy || ly < 2) && (!iz || lz < 2);
+                        if (!cov || !s_live[hf][13 - 9 * ix - 3 * iy - iz]) continue;
+                        const float4 t4 = c == 0 ? own[k]
+                                                 : s_halo[hf][halo_off(c) - 512 +
+                                                              (nx * (iy ? 2 : 8) + ly) * (iz ? 2 : 8) + lz];
+                        sx += t4.x; sy += t4.y; sz += t4.z; sm += t4.w;
                     }
-                    const int slot =
-                        cv ? s_pslot[part * 27 + ((b[0] - bx + 1) * 3 + (b[1] - by + 1)) * 3 + (b[2] - bz + 1)] : -1;
-                    use[u] = slot >= 0;
-                    const int lt = ((q[0] - 8 * b[0]) * kTE + (q[1] - 8 * b[1])) * kTE + (q[2] - 8 * b[2]);
-                    off[u] = use[u] ? ((int64_t)slot * kTE3 + lt) * 4 : 0;
                 }
-                if (!use[0] && !use[1]) continue;
-                const double2 a0 = *(const double2*)(ptiles + off[0]), a1 = *(const double2*)(ptiles + off[0] + 2);
-                const double2 b0 = *(const double2*)(ptiles + off[1]), b1 = *(const double2*)(ptiles + off[1] + 2);
-                if (use[0]) { sx += a0.x; sy += a0.y; sz += a1.x; sm += a1.y; }
-                if (use[1]) { sx += b0.x; sy += b0.y; sz += b1.x; sm += b1.y; }
+                // the FP64 tiles of split bins' parts, one at a time (this
+                // path is taken only next to split bins)
+                if (s_anyp[hf]) {
+#pragma unroll 1
+                    for (int kk = 0; kk < 8 * kParts; ++kk) {
+                        const int part = kk >> 3, c = kk & 7;
+                        const int q[3] = {x, y, z}, d[3] = {c >> 2, (c >> 1) & 1, c & 1};
+                        int b[3];
+                        bool cv = true;
+#pragma unroll
+                        for (int e = 0; e < 3; ++e) {
+                            const int h = q[e] >> 3;
+                            b[e] = h - d[e];
+                            cv = cv && b[e] < nb && (d[e] == 0 || ((q[e] & 7) <= 1 && h >= 1));
+                        }
+                        const int slot =
+                            cv ? s_pslot[hf][part * 27 + ((b[0] - bx + 1) * 3 + (b[1] - by + 1)) * 3 + (b[2] - bz + 1)]
+                               : -1;
+                        if (slot < 0) continue;
+                        const int lt = ((q[0] - 8 * b[0]) * kTE + (q[1] - 8 * b[1])) * kTE + (q[2] - 8 * b[2]);
+                        const double* pt = ptiles + ((int64_t)slot * kTE3 + lt) * 4;
+                        const double2 a0 = *(const double2*)pt, a1 = *(const double2*)(pt + 2);
+                        sx += a0.x; sy += a0.y; sz += a1.x; sm += a1.y;
+                    }
+                }
+                a4[k] = make_float4((float)sx, (float)sy, (float)sz, (float)sm);
+            }
+            float4* const bvelg = vel + sc.node_base;
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                const int x = bx * 8 + lx + 4 * k, y = by * 8 + ly, z = bz * 8 + lz;
+                if (x >= ng || y >= ng || z >= ng) continue;
+                const int64_t node = ((int64_t)x * ng + y) * ng + z;
+                if (rec) rec[sc.node_base + node] = a4[k];   // grid_v_in and grid_m of the reference
+                // a NaN or Inf in any particle's state reaches its nodes' P2G sums
+                if (!isfinite(a4[k].x + a4[k].y + a4[k].z + a4[k].w)) atomicAdd(&s_bad[hf], 1);
+                // the colliders are wave-uniform: read through the scalar cache, from
+                // the scene table or from the kernel's arguments
+                bvelg[node] = tab.dev ? grid_node_update(a4[k], x, y, z, ng, sc.prm.dx, dt, sc.prm.gravity,
+                                                         tab.cols + sc.col_base, sc.n_cols, tnow)
+                                      : grid_node_update(a4[k], x, y, z, ng, sc.prm.dx, dt, sc.prm.gravity, cs, tnow);
             }
         }
-        const float4 a4 = make_float4((float)sx, (float)sy, (float)sz, (float)sm);
-        if (rec) rec[sc.node_base + node] = a4;   // grid_v_in and grid_m of the reference
-        // a NaN or Inf in any particle's state reaches its nodes' P2G sums
-        if (!isfinite(a4.x + a4.y + a4.z + a4.w)) atomicAdd(&s_bad, 1);
-        vel[node] = grid_node_update(a4, x, y, z, ng, sc.prm.dx, dt, sc.prm.gravity, cols, n_cols, tnow);
+        if (spill) {   // drained (after the loads above, off their path)
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                const int x = bx * 8 + lx + 4 * k, y = by * 8 + ly, z = bz * 8 + lz;
+                if (x < ng && y < ng && z < ng) bacc[((int64_t)x * ng + y) * ng + z] = make_double4(0.0, 0.0, 0.0, 0.0);
+            }
+        }
+        __syncthreads();   // every thread has read the flags and the spill flag
+        if (ht == 0 && has) {
+            if (spill) gflag[blk] = 0;
+            if (work) bvel[blk] = epoch;   // written by this epoch's update (k_settle_vel)
+            if (s_bad[hf]) atomicAdd(&status[3], s_bad[hf]);
+            s_any[hf] = s_anyp[hf] = s_bad[hf] = 0;
+        }
+    };
+
+    // the listed blocks: pairs w, w + nw, ... of the list
+    const int n_pairs = (n_list + 1) / 2;
+    for (int pi = w; pi < n_pairs; pi += nw) {
+        const int li = 2 * pi + hf;
+        update_pair(li < n_list, li < n_list ? (pi == w ? first : lst[li]) : 0);
     }
-    __syncthreads();
-    if (tid == 0) {
-        gflag[blk] = 2;
-        if (s_bad) atomicAdd(&status[3], s_bad);
+    // the blocks of this workgroup's slice that only the spill scatter wrote
+    // (rare: material moving into bins without a tile), a chunk of the slice
+    // at a time
+    __syncthreads();   // s_nextra is reset
+    for (int c0 = s0; c0 < s1; c0 += kWG) {
+        if (c0 > s0 && c0 + tid < s1) {
+            sflag = gflag[c0 + tid];
+            smark = bl.mark[c0 + tid];
+        }
+        if (c0 + tid < s1 && sflag == epoch && smark != epoch) s_extra[atomicAdd(&s_nextra, 1)] = c0 + tid;
+        __syncthreads();
+        const int n_extra = s_nextra;
+        for (int k = 0; k < n_extra; k += 2) update_pair(k + hf < n_extra, s_extra[min(k + hf, n_extra - 1)]);
+        __syncthreads();   // every thread has read s_nextra and s_extra
+        if (tid == 0) s_nextra = 0;
     }
+    // last, off the blocks' path: slots and live particles of every bin, and
+    // the overflow run's count, for the next fused kernel
+    if (cnt_nxt)
+        for (int i = w * kWG + tid; i < 2 * g.n_bins + 1; i += nw * kWG) cnt_nxt[i] = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -885,6 +984,19 @@ __device__ __forceinline__ void g2p2g_item(const SubstepArgs& a, int bin, int pa
     bin_origin(bin - sc.bin_base, sc.nb, b0);
     const float4* svel = a.vel + sc.node_base;
     const int64_t off0 = bin_off(sc, bin);
+    // the grid blocks the item's tile (or its bin's parts' tiles) will
+    // overlap are listed now: the exchanges are issued before the velocity
+    // tile's loads and their results used after them (lanes 0..7 of wave 1)
+    const int mt = tid - 64;
+    int mark_blk = -1, mark_old = 0;
+    if (part == 0 && mt >= 0 && mt < 8) {
+        const int nb = sc.nb, lb = bin - sc.bin_base;
+        const int x = lb / (nb * nb) + (mt >> 2), y = (lb / nb) % nb + ((mt >> 1) & 1), z = lb % nb + (mt & 1);
+        if (x < nb && y < nb && z < nb) {
+            mark_blk = sc.bin_base + (x * nb + y) * nb + z;
+            mark_old = atomicExch(&a.bl.mark[mark_blk], a.epoch_out);
+        }
+    }
     // the run stayers will fill: as many slots as the item has live
     // particles (a whole bin: the live count kept with the bin counts; a
     // part: counted here)
@@ -908,6 +1020,7 @@ __device__ __forceinline__ void g2p2g_item(const SubstepArgs& a, int bin, int pa
         s_vel[2 * kTE3 + t] = val.z;
     }
     for (int t = tid; t < 4 * kTE3; t += WG) s_acc[t] = 0.0;
+    if (mark_blk >= 0) list_append(a.bl, mark_blk, a.epoch_out, mark_old);
     __syncthreads();
     if (parts > 1) {
         if (tid == 0) *s_base = atomicAdd(&a.cnt_nxt[bin], *s_live);
@@ -952,7 +1065,7 @@ __device__ __forceinline__ void g2p2g_item(const SubstepArgs& a, int bin, int pa
         a.dst[off0 + (int64_t)MPM_P_PID * cap + h] = __int_as_float(-1);
     if (tid == 0) atomicAdd(&a.cnt_nxt[g.n_bins + 1 + bin], max(0, min(*s_stay, cap - stay_base)));
     if (parts == 1) {
-        write_tile(s_acc, a.tiles, a.stamps, a.blive, sc, bin, a.epoch_out, tid);
+        write_tile(s_acc, a.tiles, a.stamps, bin, a.epoch_out, tid);
     } else {
         // a part's sums stay FP64 until the grid update has added the parts:
         // rounded once, as a whole bin's tile, whichever particles fell
@@ -971,7 +1084,6 @@ __device__ __forceinline__ void g2p2g_item(const SubstepArgs& a, int bin, int pa
                 a.stamps[bin] = -a.epoch_out;
             }
         }
-        if (part == 0) mark_tile_blocks(a.blive, sc, bin, a.epoch_out, tid);
     }
 }
 
@@ -1025,7 +1137,7 @@ __global__ void __launch_bounds__(WG)
 __global__ void __launch_bounds__(kWG)
     k_p2g_bins(SceneTab tab, BinGeom g, float dt, const float* __restrict__ planes, const int* __restrict__ cnt,
                double* __restrict__ acc, int* __restrict__ gflag, float4* __restrict__ tiles,
-               int* __restrict__ stamps, int* __restrict__ blive, int epoch_out) {
+               int* __restrict__ stamps, BlockList bl, int epoch_out) {
     __shared__ double s_acc[4 * kTE3];
     const int tid = threadIdx.x;
     const int wg = blockIdx.x;
@@ -1075,12 +1187,13 @@ __global__ void __launch_bounds__(kWG)
         if (!ovf && sx >= 0 && sy >= 0 && sz >= 0 && sx < kBinEdge && sy < kBinEdge && sz < kBinEdge) {
             scatter_tile(s_acc, (sx * kTE + sy) * kTE + sz, A, m, vol, v, Cm, tau, sc.prm.dx, sc.prm.inv_dx, dt);
         } else {
-            scatter_spill(acc, gflag, sc, A, m, vol, v, Cm, tau, dt);
+            scatter_spill(acc, gflag, epoch_out, sc, A, m, vol, v, Cm, tau, dt);
         }
     }
     if (!ovf) {
         __syncthreads();
-        write_tile(s_acc, tiles, stamps, blive, sc_bin, bin, epoch_out, threadIdx.x);
+        write_tile(s_acc, tiles, stamps, bin, epoch_out, threadIdx.x);
+        mark_tile_blocks(bl, sc_bin, bin, epoch_out, threadIdx.x);
     }
 }
 
@@ -1241,14 +1354,49 @@ __global__ void __launch_bounds__(kWG)
 // Tile stamps at the start of a captured run: the pending tiles (stamp
 // `from`, or -from for a split bin) become the run's first epoch `to` (-to),
 // every other stamp is voided, so that no tile of an earlier replay matches
-// an epoch the run reuses.
-__global__ void k_stamp_rebase(int* __restrict__ stamps, int* __restrict__ blive, int n, int from, int to) {
+// an epoch the run reuses.  The block list of `from` becomes the list of
+// `to`, and velocity blocks written before are marked stale for
+// k_settle_vel (an epoch of an earlier replay could equal a later one's).
+__global__ void k_stamp_rebase(int* __restrict__ stamps, int* __restrict__ gflag, BlockList bl,
+                               int* __restrict__ bvel, int n, int from, int to) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
         const int st = stamps[i];
         stamps[i] = st == from ? to : st == -from ? -to : (int)0x80000000;
-        blive[i] = blive[i] == from ? to : 0;
+        bl.mark[i] = bl.mark[i] == from ? to : 0;
+        gflag[i] = gflag[i] == from ? to : 0;
+        if (bvel[i] != 0) bvel[i] = -1;
+        if ((from & 1) != (to & 1)) {
+            // the two parities' lengths are different words: read one, write the other
+            const int nf = min(bl.head[from & 1], n);
+            if (i < nf) bl.list[(to & 1) * n + i] = bl.list[(from & 1) * n + i];
+            if (i == 0) bl.head[to & 1] = nf;
+        }
     }
+}
+
+// Zeroes the velocity nodes of every grid block that the last grid update
+// did not write (its particles have left it; zero_grid, mpm_utils.py:115-123,
+// clears every node each substep).  The substeps never read such nodes, so
+// they leave them as they were; this runs before the grid is read from
+// outside.  One workgroup per grid block.
+__global__ void __launch_bounds__(kWG) k_settle_vel(float4* __restrict__ vel, int* __restrict__ bvel,
+                                                    const int* __restrict__ head, BinGeom g, SceneTab tab) {
+    const int blk = blockIdx.x;
+    if (blk >= g.n_bins) return;
+    const int e = bvel[blk];
+    if (e == 0 || e == head[2]) return;
+    const SceneDesc sc = scene_of_bin(tab, blk);
+    const int nb = sc.nb, ng = sc.prm.n_grid;
+    const int lblk = blk - sc.bin_base;
+    const int bz = lblk % nb, by = (lblk / nb) % nb, bx = lblk / (nb * nb);
+    for (int t = threadIdx.x; t < 512; t += kWG) {
+        const int x = bx * 8 + (t >> 6), y = by * 8 + ((t >> 3) & 7), z = bz * 8 + (t & 7);
+        if (x >= ng || y >= ng || z >= ng) continue;
+        vel[sc.node_base + ((int64_t)x * ng + y) * ng + z] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    __syncthreads();   // every thread has read bvel[blk]
+    if (threadIdx.x == 0) bvel[blk] = 0;
 }
 
 __global__ void k_time_set(double* __restrict__ t, double v, int add) {
@@ -1405,7 +1553,10 @@ struct WsLayout {
     int* stamps;       // epoch of each bin's tile (-epoch: the bin was split; see write_tile)
     int* pcount;       // parts of each split bin
     int* pslot;        // FP64 tile slot of part p of split bin b (p * nbins + b)
-    int* blive;        // epoch of the newest tile overlapping each grid block (mark_tile_blocks)
+    int* blive;        // per grid block: the newest epoch it was listed for (BlockList::mark)
+    int* bhead;        // BlockList::head (64 ints)
+    int* blist;        // BlockList::list, [2][nbins]
+    int* bvel;         // per grid block: epoch of the grid update that last wrote its velocities (0: zero, -1: stale)
     int* order;        // schedules of the sort and of k_g2p2g (schedule_bins)
     float4* tiles;     // one tile of kTE^3 nodes per bin
     double* ptiles;    // nbins FP64 tiles of kTE^3 nodes (4 doubles each) for the parts of split bins
@@ -1452,6 +1603,12 @@ WsLayout ws_layout(int64_t nodes, int64_t nbins, int n_scenes, int n_cols, void*
     w.pslot = (int*)(p + off);
     off += align256(kParts * nbins * 4);
     w.blive = (int*)(p + off);
+    off += align256(nbins * 4);
+    w.bhead = (int*)(p + off);
+    off += 256;
+    w.blist = (int*)(p + off);
+    off += align256(2 * nbins * 4);
+    w.bvel = (int*)(p + off);
     off += align256(nbins * 4);
     w.order = (int*)(p + off);
     off += align256((2 * nbins + kItemsFirst) * 4);
@@ -1605,12 +1762,29 @@ int op_extract(const Ctx& c, const MpmBins* bins, const MpmState* flat, hipStrea
     return check_launch("k_bins_gather");
 }
 
+BlockList block_list(const Ctx& c) {
+    BlockList bl;
+    bl.mark = c.w.blive;
+    bl.list = c.w.blist;
+    bl.head = c.w.bhead;
+    bl.n = (int)c.g.n_bins;
+    return bl;
+}
+
+// Workgroups of the grid update over the listed blocks, two blocks each (plus
+// the schedule's): as many as are resident at once (two per CU at four waves
+// per SIMD), which covers every listed block of a 1M-particle column; larger
+// lists are taken in turns.
+constexpr int kGridWGs = 512;
+
+unsigned grid_wgs(const Ctx& c) { return 1u + (unsigned)(c.g.n_bins < kGridWGs ? c.g.n_bins : kGridWGs); }
+
 int op_clear(const Ctx& c, MpmBins* bins, hipStream_t s) {
-    hipLaunchKernelGGL(k_grid_bins, dim3(c.g.n_bins + 1), dim3(kWG), 0, s, c.w.acc, c.w.vel, c.w.gflag,
+    hipLaunchKernelGGL(k_grid_bins, dim3(grid_wgs(c)), dim3(kWG), 0, s, c.w.acc, c.w.vel, c.w.gflag,
                        (const int*)c.w.stamps, (const float4*)c.w.tiles, (const int*)c.w.pcount,
                        (const int*)c.w.pslot, (const double*)c.w.ptiles, (int*)nullptr, (const int*)nullptr, (int*)nullptr, c.g, c.tab,
                        c.cs, 0.f, 0.f, bins->epoch, 1, (const double*)nullptr, (float4*)nullptr, c.w.status,
-                       (const int*)c.w.blive);
+                       block_list(c), c.w.bvel);
     int rc = check_launch("k_grid_bins(clear)");
     if (rc) return rc;
     bins->epoch += 1;  // the pending tiles are no longer current
@@ -1619,10 +1793,14 @@ int op_clear(const Ctx& c, MpmBins* bins, hipStream_t s) {
 
 int op_p2g(const Ctx& c, MpmBins* bins, float dt, hipStream_t s) {
     const int epoch_out = bins->epoch + 1;
+    // the list of epoch_out starts empty (no grid update of epoch_out - 1 need have run)
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, c.w.bhead + (epoch_out & 1), 1, 0);
+    int rc = check_launch("k_fill_int");
+    if (rc) return rc;
     hipLaunchKernelGGL(k_p2g_bins, dim3(c.g.n_bins + ovf_wgs(c.g)), dim3(kWG), 0, s, c.tab, c.g, dt,
                        bins->planes[bins->cur], (const int*)counts(bins, c.g, bins->cur), (double*)c.w.acc, c.w.gflag,
-                       c.w.tiles, c.w.stamps, c.w.blive, epoch_out);
-    int rc = check_launch("k_p2g_bins");
+                       c.w.tiles, c.w.stamps, block_list(c), epoch_out);
+    rc = check_launch("k_p2g_bins");
     if (rc) return rc;
     bins->epoch = epoch_out;
     return 0;
@@ -1639,11 +1817,11 @@ int op_grid(const Ctx& c, const MpmBins* bins, float time, float dt, hipStream_t
         int rc = check_launch("k_zero4");
         if (rc) return rc;
     }
-    hipLaunchKernelGGL(k_grid_bins, dim3(c.g.n_bins + 1), dim3(kWG), 0, s, c.w.acc, c.w.vel, c.w.gflag,
+    hipLaunchKernelGGL(k_grid_bins, dim3(grid_wgs(c)), dim3(kWG), 0, s, c.w.acc, c.w.vel, c.w.gflag,
                        (const int*)c.w.stamps, (const float4*)c.w.tiles, (const int*)c.w.pcount,
                        (const int*)c.w.pslot, (const double*)c.w.ptiles, counts(bins, c.g, 1 - bins->cur),
                        (const int*)counts(bins, c.g, bins->cur), c.w.order, c.g, c.tab, c.cs, time, dt, bins->epoch,
-                       0, t_base, rec, c.w.status, (const int*)c.w.blive);
+                       0, t_base, rec, c.w.status, block_list(c), c.w.bvel);
     return check_launch("k_grid_bins");
 }
 
@@ -1666,7 +1844,7 @@ int op_g2p2g(const Ctx& c, MpmBins* bins, float dt, hipStream_t s, bool store_al
     a.status = c.w.status;
     a.tiles = c.w.tiles;
     a.stamps = c.w.stamps;
-    a.blive = c.w.blive;
+    a.bl = block_list(c);
     a.ptiles = c.w.ptiles;
     a.pcount = c.w.pcount;
     a.pslot = c.w.pslot;
@@ -1764,7 +1942,7 @@ int op_graph_create(const Ctx& c, const void* ws_id, const MpmBins* bins, float 
         hipStreamDestroy(cs);
         return set_err(-53, "graph: begin capture failed");
     }
-    hipLaunchKernelGGL(k_stamp_rebase, dim3((c.g.n_bins + 255) / 256), dim3(256), 0, cs, c.w.stamps, c.w.blive,
+    hipLaunchKernelGGL(k_stamp_rebase, dim3((c.g.n_bins + 255) / 256), dim3(256), 0, cs, c.w.stamps, c.w.gflag, block_list(c), c.w.bvel,
                        c.g.n_bins, e_end, e0);
     rc = check_launch("k_stamp_rebase");
     double off = 0.0;
@@ -1817,7 +1995,7 @@ int op_graph_launch(const Ctx& c, const void* ws_id, MpmGraph* g, MpmBins* bins,
     if (n_replays < 0) return set_err(-26, "n_replays < 0");
     if (n_replays == 0) return 0;
     // the pending tiles take the stamp a replay's rebase expects; the time base
-    hipLaunchKernelGGL(k_stamp_rebase, dim3((c.g.n_bins + 255) / 256), dim3(256), 0, s, c.w.stamps, c.w.blive,
+    hipLaunchKernelGGL(k_stamp_rebase, dim3((c.g.n_bins + 255) / 256), dim3(256), 0, s, c.w.stamps, c.w.gflag, block_list(c), c.w.bvel,
                        c.g.n_bins, bins->epoch, g->epoch_end);
     int rc = check_launch("k_stamp_rebase");
     if (rc) return rc;
@@ -1875,6 +2053,17 @@ int mpm_bins_extract_planes(const MpmParams* prm, const MpmBins* bins, const Mpm
     if (rc) return rc;
     if (plane_mask >> MPM_NPLANES) return set_err(-26, "plane_mask names planes past MPM_NPLANES");
     return op_extract(c, bins, flat, (hipStream_t)stream, plane_mask);
+}
+
+int mpm_bins_settle_grid(const MpmParams* prm, const MpmBins* bins, void* workspace, void* stream) {
+    Ctx c;
+    int rc = ctx_single(prm, bins, workspace, nullptr, 0, false, &c);
+    if (rc) return rc;
+    if (!workspace) return set_err(-10, "workspace is NULL");
+    if (c.g.n_bins < 1) return 0;
+    hipLaunchKernelGGL(k_settle_vel, dim3((unsigned)c.g.n_bins), dim3(kWG), 0, (hipStream_t)stream, c.w.vel, c.w.bvel,
+                       (const int*)c.w.bhead, c.g, c.tab);
+    return check_launch("k_settle_vel");
 }
 
 int mpm_bins_clear_grid(const MpmParams* prm, MpmBins* bins, void* workspace, void* stream) {

@@ -812,7 +812,11 @@ class MPM_Simulator_HIP:
         if self._loc == "bins" and self._bins is not None:
             ws = self._bins["ws"]
             off = self._bins_ws_ptr() - ws.data_ptr()
-            vel_off = int(L.load().mpm_bins_velocity_offset(self._params()))
+            lib = L.load()
+            # blocks the particles have left still hold their last velocities
+            L.check(lib.mpm_bins_settle_grid(self._params(), self._bins["struct"], self._bins_ws_ptr(),
+                                             _stream_handle(self.device)), "mpm_bins_settle_grid")
+            vel_off = int(lib.mpm_bins_velocity_offset(self._params()))
             vel = ws[off + vel_off:off + vel_off + nn * 16].view(torch.float32).view(ng, ng, ng, 4)
             return self._record, vel
         else:

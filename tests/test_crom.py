@@ -307,9 +307,12 @@ def test_encoder_matches_torch():
 
 
 def test_crom_autoencoder_training_reduces_loss():
-    """CROMnet trained on a family of smooth displacement fields: the loss
-    falls by more than 10x and the encoder receives gradients through the
-    decoder's latent columns."""
+    """CROMnet trained on a family of smooth displacement fields: the
+    reconstruction loss over all frames falls by more than 10x (the
+    squared relative L2 error, measured on the same frames before and
+    after: a single minibatch's loss varies with the frames it holds and
+    with the order of the gradient atomics), and the encoder receives
+    gradients through the decoder's latent columns."""
     from reduced_mpm_stress_training_amd.crom import Crom
     torch.manual_seed(9)
     P, F = 4096, 24
@@ -322,15 +325,17 @@ def test_crom_autoencoder_training_reduces_loss():
     net.set_preprop(X.mean(0), X.std(0), q.reshape(-1, 3).mean(0), q.reshape(-1, 3).std(0))
     qd, xd = q.cuda(), x.cuda()
     e0 = net.encoder.params.clone()
-    first = float(net.train_step(qd[:8], xd[:8], lr=1e-3))
+    with torch.no_grad():
+        err0 = _rel(net.forward(qd, xd).cpu(), q)
     for it in range(600):
         b = (it * 8) % F
         last = net.train_step(qd[b:b + 8], xd[b:b + 8], lr=1e-3)
+    assert np.isfinite(float(last))
     with torch.no_grad():
         rec = net.forward(qd, xd).cpu()
     err = _rel(rec, q)
-    print(first, float(last), err)
-    assert float(last) < first / 10
+    print(err0, float(last), err)
+    assert err ** 2 < err0 ** 2 / 10
     assert err < 0.2
     assert not torch.equal(e0, net.encoder.params)
 
