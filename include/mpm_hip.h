@@ -269,9 +269,9 @@ typedef struct MpmBins {
  * accumulate until the caller clears them; 48-49: the start time, as a double, of a captured run,
  * mpm_batch_graph_launch), tile stamps, the lists of grid blocks each epoch's
  * P2G wrote (the grid update visits only those) and per block the epoch of
- * its last velocity update, the fused kernel's schedule, nb^3 P2G tiles of
- * 10^3 float4 and nb^3 FP64 tiles of 10^3 x 4 doubles for the parts of split
- * bins. */
+ * its last velocity update, the fused kernel's schedule and nb^3 P2G tiles
+ * of 10^3 float4 (the parts of a split bin add their sums into the spill
+ * grid). */
 int64_t mpm_bins_workspace_bytes(const MpmParams* prm);
 
 /* Byte offset of the status counters in the workspace. */

@@ -182,6 +182,13 @@ __device__ __forceinline__ int ceil_div_parts(int x, int k) {
     return k == 1 ? x : k == 2 ? (x + 1) >> 1 : k == 3 ? ((x + 2) * 21846) >> 16 : (x + 3) >> 2;
 }
 
+// parts of a bin of p passes when items should not exceed q passes (split_bin)
+__device__ __forceinline__ int split_parts(int p, int q) {
+    const int k = 1 + (p > q) + (p > 2 * q) + (p > 3 * q);
+    const int pp = ceil_div_parts(p, k);
+    return 1 + (p > pp) + (p > 2 * pp) + (p > 3 * pp);
+}
+
 // parts of a bin of p passes when items should not exceed q passes, and the
 // passes of each (the last one may have fewer): k = min(4, ceil(p / q)),
 // per = ceil(p / k), parts = ceil(p / per) <= k
@@ -279,12 +286,20 @@ __device__ void schedule_bins(const int* __restrict__ cnt_cur, const BinGeom& g,
     __syncthreads();
     hist_cursors(s_hist, s_cursor, &s_total, &s_max);
     __syncthreads();
-    // Split only where a bin exceeds a CU's share of all passes (whole bins
-    // then leave CUs idle); items then take at most half a share each, and
-    // all items must fit in n_bins slots.
+    // Split where a bin exceeds a CU's share of all passes (whole bins then
+    // leave CUs idle), and where more bins exceed half a share than there are
+    // CUs: the ones beyond the n_cu largest would run second on a CU after a
+    // large bin, so they are split (those the CUs start with stay whole).
+    // Items then take at most half a share each, and all items must fit in
+    // n_bins slots.
     const int share = s_passes / max(1, g.n_cu);
-    bool split = s_max > max(1, share) && s_max < kSchedKeys - 1;
     const int q = max(1, share / 2);
+    const int n_big = q < kSchedKeys ? s_cursor[q] : 0;   // bins whose key exceeds q (before the cursors move)
+    const bool split_all = s_max > max(1, share) && s_max < kSchedKeys - 1;
+    const bool split_late = !reload && n_big > g.n_cu && s_max < kSchedKeys - 1;
+    bool split = split_all || split_late;
+    __syncthreads();   // every thread has read n_big before the cursors move
+    int rank[kSchedRegs];   // each bin's place in the sort's list (without reload)
     // the sort's list (and k_g2p2g's when no bin is split)
     for (int c = 0; c < n_chunks; ++c) {
         if (reload) load_passes(cnt_cur, n_bins, cap, c, pv);
@@ -292,6 +307,7 @@ __device__ void schedule_bins(const int* __restrict__ cnt_cur, const BinGeom& g,
         for (int j = 0; j < kSchedRegs; ++j) {
             const int pos = wave_add(s_cursor, min(pv[j], kSchedKeys - 1), pv[j] > 0);
             const int b = c * kWG * kSchedRegs + j * kWG + tid;
+            rank[j] = pos;
             if (pv[j] > 0) {
                 order[pos] = b;
                 if (!split) items[kItemsFirst + pos] = b << 4;
@@ -302,15 +318,47 @@ __device__ void schedule_bins(const int* __restrict__ cnt_cur, const BinGeom& g,
         order[p] = -1;
         if (!split) items[kItemsFirst + p] = -1;
     }
-    if (split) {
+    if (split && !split_all) {
+        // Only the bins ranked [n_cu, n_big) split (all of them above q): the
+        // items are the n_cu largest bins, then those bins' parts, then the
+        // remaining bins in the list's order -- descending but for the parts'
+        // place among bins of q passes, without a second sort.
+        int my_parts = 0;
+#pragma unroll
+        for (int j = 0; j < kSchedRegs; ++j)
+            if (pv[j] > 0 && rank[j] >= g.n_cu && rank[j] < n_big) my_parts += split_parts(pv[j], q);
+        if (my_parts > 0) atomicAdd(&s_items, my_parts);
+        __syncthreads();
+        const int n_parts = s_items, n_split = n_big - g.n_cu, n_items = s_total + n_parts - n_split;
+        const bool fits = n_items <= n_bins;
+        if (tid == 0) s_passes = 0;   // from here the next free part slot
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kSchedRegs; ++j) {
+            const int b = j * kWG + tid, r = rank[j];
+            if (pv[j] <= 0) {
+            } else if (!fits || r < g.n_cu) {
+                items[kItemsFirst + r] = b << 4;
+            } else if (r < n_big) {
+                const int k = split_parts(pv[j], q);
+                const int base = g.n_cu + atomicAdd(&s_passes, k);
+#pragma unroll
+                for (int i = 0; i < kParts; ++i)
+                    if (i < k) items[kItemsFirst + base + i] = (b << 4) | (i << 2) | (k - 1);
+            } else {
+                items[kItemsFirst + r - n_big + g.n_cu + n_parts] = b << 4;
+            }
+        }
+        for (int p = (fits ? n_items : s_total) + tid; p < n_bins; p += kWG) items[kItemsFirst + p] = -1;
+    } else if (split) {
         int my_items = 0;
         for (int c = 0; c < n_chunks; ++c) {
             if (reload) load_passes(cnt_cur, n_bins, cap, c, pv);
 #pragma unroll
             for (int j = 0; j < kSchedRegs; ++j)
                 if (pv[j] > 0) {
-                    int parts, per;
-                    split_bin(pv[j], q, &parts, &per);
+                    int parts = 1, per = pv[j];
+                    if (split_all || rank[j] >= g.n_cu) split_bin(pv[j], q, &parts, &per);
                     my_items += parts;
                 }
         }
@@ -326,7 +374,8 @@ __device__ void schedule_bins(const int* __restrict__ cnt_cur, const BinGeom& g,
                 const int p = pv[j];
                 int parts = 0, per = 1;
                 if (p > 0) {
-                    if (split) split_bin(p, q, &parts, &per); else { parts = 1; per = p; }
+                    if (split && (split_all || rank[j] >= g.n_cu)) split_bin(p, q, &parts, &per);
+                    else { parts = 1; per = p; }
                 }
 #pragma unroll
                 for (int i = 0; i < kParts; ++i)
@@ -344,7 +393,8 @@ __device__ void schedule_bins(const int* __restrict__ cnt_cur, const BinGeom& g,
                 const int b = c * kWG * kSchedRegs + j * kWG + tid;
                 int parts = 0, per = 1;
                 if (p > 0) {
-                    if (split) split_bin(p, q, &parts, &per); else { parts = 1; per = p; }
+                    if (split && (split_all || rank[j] >= g.n_cu)) split_bin(p, q, &parts, &per);
+                    else { parts = 1; per = p; }
                 }
 #pragma unroll
                 for (int i = 0; i < kParts; ++i) {
@@ -460,11 +510,8 @@ struct SubstepArgs {
     int* __restrict__ gflag;
     int* __restrict__ status;
     float4* __restrict__ tiles;
-    int* __restrict__ stamps;       // epoch of each bin's tile; -epoch: split, tiles in ptiles
+    int* __restrict__ stamps;       // epoch of each bin's tile; -epoch: split, its parts in the spill grid
     BlockList bl;                   // grid blocks written this epoch (tiles, spill)
-    double* __restrict__ ptiles;    // FP64 tiles of split bins' parts, one slot per work item
-    int* __restrict__ pcount;       // parts of each split bin
-    int* __restrict__ pslot;        // ptiles slot of part p of bin b: pslot[p * n_bins + b]
     int* __restrict__ items;        // work items (schedule_bins): order + n_bins
     int sched_epoch;                // epoch the schedule must carry to be used
     int store_all;                  // 0: write only the planes the next substep reads (see g2p2g_particle)
@@ -735,17 +782,14 @@ __device__ __forceinline__ float4 halo_entry(const float4* __restrict__ btiles, 
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kWG)
     k_grid_bins(double4* __restrict__ acc, float4* __restrict__ vel, int* __restrict__ gflag,
-                const int* __restrict__ stamps, const float4* __restrict__ tiles, const int* __restrict__ pcount,
-                const int* __restrict__ pslot, const double* __restrict__ ptiles, int* __restrict__ cnt_nxt,
+                const int* __restrict__ stamps, const float4* __restrict__ tiles, int* __restrict__ cnt_nxt,
                 const int* __restrict__ cnt_cur, int* __restrict__ order, BinGeom g, SceneTab tab,
                 ColliderSet cs, float time, float dt, int epoch, int clear_only, const double* __restrict__ t_base,
                 float4* __restrict__ rec, int* __restrict__ status, BlockList bl, int* __restrict__ bvel) {
-    // per half: neighbour bin c has a whole-bin tile of this epoch; FP64 tile
-    // slot of part p of neighbour bin c this epoch, or -1; flags; halo
-    // entries (halo_off).  s_extra: the spill-only blocks of a slice.
+    // per half: neighbour bin c has a whole-bin tile of this epoch; flags;
+    // halo entries (halo_off).  s_extra: the spill-only blocks of a slice.
     __shared__ int s_live[2][27];
-    __shared__ int s_pslot[2][kParts * 27];
-    __shared__ int s_any[2], s_anyp[2], s_bad[2];
+    __shared__ int s_any[2], s_bad[2];
     __shared__ float4 s_halo[2][kHalo];
     __shared__ int s_extra[kWG], s_nextra;
     const int tid = threadIdx.x;
@@ -773,7 +817,7 @@ __global__ void __launch_bounds__(kWG)
         sflag = gflag[s0 + tid];
         smark = bl.mark[s0 + tid];
     }
-    if (ht == 0) s_any[hf] = s_anyp[hf] = s_bad[hf] = 0;
+    if (ht == 0) s_any[hf] = s_bad[hf] = 0;
     if (tid == 0) s_nextra = 0;
     // thread ht: nodes (lx, ly, lz) and (lx + 4, ly, lz) of its half's block
     const int lx = ht >> 6, ly = (ht >> 3) & 7, lz = ht & 7;
@@ -789,24 +833,16 @@ __global__ void __launch_bounds__(kWG)
         double4* const bacc = acc + sc.node_base;
         const float4* const btiles = tiles + (int64_t)sc.bin_base * kTE3;
         // one round of loads: the neighbours' stamps, the spill flag, the
-        // block's tile entries and its spill sums
+        // block's tile entries and its spill sums (the parts of split bins
+        // are in the spill grid)
         if (has && ht < 27) {
             const int c = ht;
             const int x = bx + c / 9 - 1, y = by + (c / 3) % 3 - 1, z = bz + c % 3 - 1;
             const bool in = !clear_only && x >= 0 && y >= 0 && z >= 0 && x < nb && y < nb && z < nb;
-            const int lb = in ? (x * nb + y) * nb + z : 0;
-            const int st = in ? stamps[sc.bin_base + lb] : 0;
+            const int st = in ? stamps[sc.bin_base + (x * nb + y) * nb + z] : 0;
             const int live = in && st == epoch;
             s_live[hf][c] = live;
             if (live) atomicOr(&s_any[hf], 1);
-            // a split bin (stamp -epoch): its parts' FP64 tile slots
-            const bool split = in && st == -epoch && epoch > 0;
-            const int n_parts = split ? min(pcount[sc.bin_base + lb], kParts) : 0;
-#pragma unroll
-            for (int part = 0; part < kParts; ++part)
-                s_pslot[hf][part * 27 + c] =
-                    part < n_parts ? pslot[(int64_t)part * g.n_bins + sc.bin_base + lb] : -1;
-            if (split) atomicOr(&s_anyp[hf], 1);
         }
         // the spill grid holds contributions of this epoch in this block
         const bool spill = has && gflag[blk] == epoch;
@@ -830,7 +866,7 @@ __global__ void __launch_bounds__(kWG)
             }
         }
         __syncthreads();   // stamps and halo entries in LDS
-        const bool work = has && !clear_only && (spill || s_any[hf] || s_anyp[hf]);
+        const bool work = has && !clear_only && (spill || s_any[hf]);
         if (work) {
             // the two nodes' sums first, then their collider passes: the
             // colliders' registers are not held beside the sums'
@@ -838,7 +874,6 @@ __global__ void __launch_bounds__(kWG)
 #pragma unroll
             for (int k = 0; k < 2; ++k) {
                 const int nx = lx + 4 * k;
-                const int x = bx * 8 + nx, y = by * 8 + ly, z = bz * 8 + lz;
                 double sx = sp[k].x, sy = sp[k].y, sz = sp[k].z, sm = sp[k].w;
                 // whole-bin tiles in (ix, iy, iz) order
                 if (s_any[hf]) {
@@ -851,31 +886,6 @@ __global__ void __launch_bounds__(kWG)
                                                  : s_halo[hf][halo_off(c) - 512 +
                                                               (nx * (iy ? 2 : 8) + ly) * (iz ? 2 : 8) + lz];
                         sx += t4.x; sy += t4.y; sz += t4.z; sm += t4.w;
-                    }
-                }
-                // the FP64 tiles of split bins' parts, one at a time (this
-                // path is taken only next to split bins)
-                if (s_anyp[hf]) {
-#pragma unroll 1
-                    for (int kk = 0; kk < 8 * kParts; ++kk) {
-                        const int part = kk >> 3, c = kk & 7;
-                        const int q[3] = {x, y, z}, d[3] = {c >> 2, (c >> 1) & 1, c & 1};
-                        int b[3];
-                        bool cv = true;
-#pragma unroll
-                        for (int e = 0; e < 3; ++e) {
-                            const int h = q[e] >> 3;
-                            b[e] = h - d[e];
-                            cv = cv && b[e] < nb && (d[e] == 0 || ((q[e] & 7) <= 1 && h >= 1));
-                        }
-                        const int slot =
-                            cv ? s_pslot[hf][part * 27 + ((b[0] - bx + 1) * 3 + (b[1] - by + 1)) * 3 + (b[2] - bz + 1)]
-                               : -1;
-                        if (slot < 0) continue;
-                        const int lt = ((q[0] - 8 * b[0]) * kTE + (q[1] - 8 * b[1])) * kTE + (q[2] - 8 * b[2]);
-                        const double* pt = ptiles + ((int64_t)slot * kTE3 + lt) * 4;
-                        const double2 a0 = *(const double2*)pt, a1 = *(const double2*)(pt + 2);
-                        sx += a0.x; sy += a0.y; sz += a1.x; sm += a1.y;
                     }
                 }
                 a4[k] = make_float4((float)sx, (float)sy, (float)sz, (float)sm);
@@ -908,7 +918,7 @@ __global__ void __launch_bounds__(kWG)
             if (spill) gflag[blk] = 0;
             if (work) bvel[blk] = epoch;   // written by this epoch's update (k_settle_vel)
             if (s_bad[hf]) atomicAdd(&status[3], s_bad[hf]);
-            s_any[hf] = s_anyp[hf] = s_bad[hf] = 0;
+            s_any[hf] = s_bad[hf] = 0;
         }
     };
 
@@ -1067,23 +1077,26 @@ __device__ __forceinline__ void g2p2g_item(const SubstepArgs& a, int bin, int pa
     if (parts == 1) {
         write_tile(s_acc, a.tiles, a.stamps, bin, a.epoch_out, tid);
     } else {
-        // a part's sums stay FP64 until the grid update has added the parts:
-        // rounded once, as a whole bin's tile, whichever particles fell
-        // into which part
-        double* out = a.ptiles + (int64_t)blockIdx.x * kTE3 * 4;
+        // a part adds its FP64 sums into the spill grid with FP64 atomics (the
+        // parts of a bin land in any order; FP64 sums of FP32 contributions do
+        // not depend on it unless they need more than 53 bits) and flags the
+        // tile's blocks; its bin has no whole tile this epoch (stamp -epoch)
+        double* sacc = a.acc + 4 * sc.node_base;
         for (int t = tid; t < kTE3; t += WG) {
-            *(double2*)(out + 4 * t) = make_double2(s_acc[t], s_acc[kTE3 + t]);
-            *(double2*)(out + 4 * t + 2) = make_double2(s_acc[2 * kTE3 + t], s_acc[3 * kTE3 + t]);
-        }
-        if (tid == 0) {
-            a.pslot[part * g.n_bins + bin] = blockIdx.x;
-            if (part == 0) {
-                // the parts with slots: those whose range starts below n_all
-                const int pb = bin_passes(n_all), pp = per / kWG;
-                a.pcount[bin] = 1 + (pb > pp) + (pb > 2 * pp) + (pb > 3 * pp);
-                a.stamps[bin] = -a.epoch_out;
+            const int x = b0[0] + t / (kTE * kTE), y = b0[1] + (t / kTE) % kTE, z = b0[2] + t % kTE;
+            if (x >= ng || y >= ng || z >= ng) continue;
+            double* nd = sacc + 4 * (((int64_t)x * ng + y) * ng + z);
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const double v = s_acc[c * kTE3 + t];
+                if (v != 0.0) unsafeAtomicAdd(nd + c, v);
             }
         }
+        if (tid < 8) {
+            const int x = (b0[0] >> 3) + (tid >> 2), y = (b0[1] >> 3) + ((tid >> 1) & 1), z = (b0[2] >> 3) + (tid & 1);
+            if (x < sc.nb && y < sc.nb && z < sc.nb) a.gflag[sc.bin_base + (x * sc.nb + y) * sc.nb + z] = a.epoch_out;
+        }
+        if (tid == 0 && part == 0) a.stamps[bin] = -a.epoch_out;
     }
 }
 
@@ -1550,16 +1563,13 @@ struct WsLayout {
     float4* vel;       // grid velocities (v, m)
     int* gflag;        // spill flags per grid block
     int* status;       // 64 counters
-    int* stamps;       // epoch of each bin's tile (-epoch: the bin was split; see write_tile)
-    int* pcount;       // parts of each split bin
-    int* pslot;        // FP64 tile slot of part p of split bin b (p * nbins + b)
+    int* stamps;       // epoch of each bin's tile (-epoch: the bin was split, its parts are in the spill grid)
     int* blive;        // per grid block: the newest epoch it was listed for (BlockList::mark)
     int* bhead;        // BlockList::head (64 ints)
     int* blist;        // BlockList::list, [2][nbins]
     int* bvel;         // per grid block: epoch of the grid update that last wrote its velocities (0: zero, -1: stale)
     int* order;        // schedules of the sort and of k_g2p2g (schedule_bins)
     float4* tiles;     // one tile of kTE^3 nodes per bin
-    double* ptiles;    // nbins FP64 tiles of kTE^3 nodes (4 doubles each) for the parts of split bins
     SceneDesc* table;  // batch: scene table
     MpmCollider* cols; // batch: colliders
     int64_t bytes;
@@ -1598,10 +1608,6 @@ WsLayout ws_layout(int64_t nodes, int64_t nbins, int n_scenes, int n_cols, void*
     off += 256;
     w.stamps = (int*)(p + off);
     off += align256(nbins * 4);
-    w.pcount = (int*)(p + off);
-    off += align256(nbins * 4);
-    w.pslot = (int*)(p + off);
-    off += align256(kParts * nbins * 4);
     w.blive = (int*)(p + off);
     off += align256(nbins * 4);
     w.bhead = (int*)(p + off);
@@ -1614,8 +1620,6 @@ WsLayout ws_layout(int64_t nodes, int64_t nbins, int n_scenes, int n_cols, void*
     off += align256((2 * nbins + kItemsFirst) * 4);
     w.tiles = (float4*)(p + off);
     off += align256(nbins * kTE3 * 16);
-    w.ptiles = (double*)(p + off);
-    off += align256(nbins * kTE3 * 32);
     w.table = (SceneDesc*)(p + off);
     off += align256((int64_t)n_scenes * (int64_t)sizeof(SceneDesc));
     w.cols = (MpmCollider*)(p + off);
@@ -1781,8 +1785,7 @@ unsigned grid_wgs(const Ctx& c) { return 1u + (unsigned)(c.g.n_bins < kGridWGs ?
 
 int op_clear(const Ctx& c, MpmBins* bins, hipStream_t s) {
     hipLaunchKernelGGL(k_grid_bins, dim3(grid_wgs(c)), dim3(kWG), 0, s, c.w.acc, c.w.vel, c.w.gflag,
-                       (const int*)c.w.stamps, (const float4*)c.w.tiles, (const int*)c.w.pcount,
-                       (const int*)c.w.pslot, (const double*)c.w.ptiles, (int*)nullptr, (const int*)nullptr, (int*)nullptr, c.g, c.tab,
+                       (const int*)c.w.stamps, (const float4*)c.w.tiles, (int*)nullptr, (const int*)nullptr, (int*)nullptr, c.g, c.tab,
                        c.cs, 0.f, 0.f, bins->epoch, 1, (const double*)nullptr, (float4*)nullptr, c.w.status,
                        block_list(c), c.w.bvel);
     int rc = check_launch("k_grid_bins(clear)");
@@ -1818,8 +1821,7 @@ int op_grid(const Ctx& c, const MpmBins* bins, float time, float dt, hipStream_t
         if (rc) return rc;
     }
     hipLaunchKernelGGL(k_grid_bins, dim3(grid_wgs(c)), dim3(kWG), 0, s, c.w.acc, c.w.vel, c.w.gflag,
-                       (const int*)c.w.stamps, (const float4*)c.w.tiles, (const int*)c.w.pcount,
-                       (const int*)c.w.pslot, (const double*)c.w.ptiles, counts(bins, c.g, 1 - bins->cur),
+                       (const int*)c.w.stamps, (const float4*)c.w.tiles, counts(bins, c.g, 1 - bins->cur),
                        (const int*)counts(bins, c.g, bins->cur), c.w.order, c.g, c.tab, c.cs, time, dt, bins->epoch,
                        0, t_base, rec, c.w.status, block_list(c), c.w.bvel);
     return check_launch("k_grid_bins");
@@ -1845,9 +1847,6 @@ int op_g2p2g(const Ctx& c, MpmBins* bins, float dt, hipStream_t s, bool store_al
     a.tiles = c.w.tiles;
     a.stamps = c.w.stamps;
     a.bl = block_list(c);
-    a.ptiles = c.w.ptiles;
-    a.pcount = c.w.pcount;
-    a.pslot = c.w.pslot;
     a.items = c.w.order + c.g.n_bins;
     a.sched_epoch = bins->epoch;
     a.store_all = store_all ? 1 : 0;

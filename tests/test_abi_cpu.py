@@ -126,13 +126,13 @@ def test_segment_entry_points_reject_bad_arguments():
     assert b"adjoint buffers" in lib.mpm_last_error()                     # adj_a == adj_b
 
 
-def test_bins_reject_oversized_bins_and_size_the_part_tiles():
+def test_bins_reject_oversized_bins_and_size_the_tiles():
     """The binned entry points check MpmBins on the host before any launch
     (the pointers here are never dereferenced): more than 16e6 slots per bin
     is refused (the fused kernel's split arithmetic counts passes of 512 in
-    15 bits).  The workspace holds, per bin, a float4 tile and an FP64 tile
-    of 12^3 nodes (the latter for the parts of split bins), after the FP64
-    spill grid and the velocity grid."""
+    15 bits).  The workspace holds, per bin, a float4 tile of 10^3 nodes and
+    the block lists (the parts of split bins add into the FP64 spill grid),
+    after the spill grid and the velocity grid."""
     lib = L.load()
     prm = L.MpmParams()
     prm.n_grid, prm.material, prm.dx, prm.inv_dx = 32, 2, 1.0 / 32, 32.0
@@ -148,4 +148,4 @@ def test_bins_reject_oversized_bins_and_size_the_part_tiles():
     nodes = 32 ** 3
     assert lib.mpm_bins_velocity_offset(prm) == nodes * 32
     assert lib.mpm_bins_status_offset(prm) >= nodes * 32 + nodes * 16
-    assert lib.mpm_bins_workspace_bytes(prm) >= nb ** 3 * 10 ** 3 * (16 + 32) + nodes * (32 + 16)
+    assert lib.mpm_bins_workspace_bytes(prm) >= nb ** 3 * (10 ** 3 * 16 + 4 * 4) + nodes * (32 + 16)
