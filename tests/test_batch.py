@@ -121,3 +121,26 @@ def test_captured_run_matches_eager_run():
     _compare_batches(eager, graph, len(scenes))
     assert int(graph.status()[0]) == 0
     graph.release_graph()
+
+
+def test_captured_run_of_odd_length_matches_eager_run():
+    """A graph of 17 substeps (one sort; 17 + 1 launches of the plane sets'
+    swap keep the state on its plane set): each replay renumbers the pending
+    epoch from odd to even, so the grid-block list of the pending P2G moves to
+    the other parity's list at every replay.  Every replay against the eager
+    run."""
+    scenes = _scenes()
+    dt = 1e-4
+    eager, graph = B.SceneBatch(scenes), B.SceneBatch(scenes)
+    k = graph.capture(dt, 17)
+    assert k == 17
+    for _ in range(3):
+        eager.run(k, dt)
+        graph.run_captured(1)
+        torch.cuda.synchronize()
+        _compare_batches(eager, graph, len(scenes))
+    eager.run(5, dt)              # eager substeps continue from a replay's state
+    graph.run(5, dt)
+    _compare_batches(eager, graph, len(scenes))
+    assert int(graph.status()[0]) == 0
+    graph.release_graph()

@@ -260,18 +260,29 @@ def test_free_fall_gpu():
     assert np.abs(out["tau"]).max() < 1e-2
 
 
+def _split_bins(sim):
+    """Bins whose tile stamp is negative: the last fused kernel ran them as
+    parts.  The stamps follow the 64 status words in the workspace."""
+    B = sim._bins
+    off = sim._bins_ws_ptr() - B["ws"].data_ptr() + int(L.load().mpm_bins_status_offset(sim._params())) + 256
+    return int((B["ws"][off:off + 4 * B["nb"] ** 3].view(torch.int32) < 0).sum())
+
+
 @pytest.mark.parametrize("n_particles, n_grid", [(1_000_000, 128), (250_000, 64)])
 def test_headline_size_parity(oracle, n_particles, n_grid):
-    """1M particles on 128^3 (the headline workload), and 250k on 64^3, where
-    the fused kernel splits every full bin into three parts with FP64 tiles
-    (80 bins for 256 CUs), for 3 substeps against the FP32 and FP64 oracles
-    (OpenMP), plus mass conservation of the P2G grid.  Positions are compared
-    as x = x0 + u (u itself is ~3e-5 m after 3 substeps, where one ulp of x is
-    2e-3 of u)."""
+    """1M particles on 128^3 (the headline workload: more full bins than CUs,
+    so the ones ranked past the CU count run in two parts), and 250k on 64^3,
+    where every full bin runs in three parts (80 bins for 256 CUs; the parts
+    add their FP64 sums into the spill grid), for 3 substeps against the FP32
+    and FP64 oracles (OpenMP), plus mass conservation of the P2G grid.
+    Positions are compared as x = x0 + u (u itself is ~3e-5 m after 3
+    substeps, where one ulp of x is 2e-3 of u)."""
     sc = S.column_uniform(n_particles=n_particles, n_grid=n_grid)
     sim = S.make_simulator(sc)
     n_steps = 3
     sim.run(n_steps, sc.dt)
+    torch.cuda.synchronize()
+    assert _split_bins(sim) > 0   # the split path ran
     g = {k: v.reshape(sc.n_particles, -1) for k, v in _gpu_state(sim).items()}
     m_grid = float(sim.grid_m.sum())
     assert abs(m_grid - float(sc.mass.astype(np.float64).sum())) < 1e-4 * float(sc.mass.sum())
