@@ -115,6 +115,7 @@ class Rollout:
         planes[L.P_VOL] = t(st["vol"])
         planes[L.P_PID] = torch.arange(self.n, dtype=torch.int32, device=self.device).view(torch.float32)
         self._init = planes
+        self._init_sorted = None   # (planes, order) of the initial state's sort, made once
         self._checkpoints = {}
         self._final = None
         self._ws_seg = []
@@ -160,6 +161,24 @@ class Rollout:
                                          self._stream()), "mpm_sort_by_bin")
         return dst, order
 
+    def _sorted_start(self, v0):
+        """The initial state in the sort's slot order with v0 in place, and the
+        sort's order buffer.  The sort keys are positions, which no call
+        changes, so the caller-order initial state is sorted once; each call
+        gathers v0 into the sorted slots by the pid plane."""
+        if self._init_sorted is None:
+            self._init_sorted = self._sort(self._init)
+        sp, so = self._init_sorted
+        planes = sp.clone()
+        v = v0.to(self.device, torch.float32).t().contiguous()
+        planes[L.P_V:L.P_V + 3] = self._permute(v, 3, sp[L.P_PID].data_ptr(), scatter=False)
+        return planes, so
+
+    def _copy_order(self, dst, src):
+        """An order buffer's contents (from its 256-byte aligned start) into another."""
+        a, b = self._aligned(src) - src.data_ptr(), self._aligned(dst) - dst.data_ptr()
+        dst[b:b + self._order_bytes].copy_(src[a:a + self._order_bytes])
+
     def _step(self, planes, k, order, ws=None):
         ws = self._ws_step if ws is None else ws
         L.check(L.load().mpm_step_ordered(self._prm, self._state(planes), self._aligned(order), self._cols,
@@ -179,16 +198,21 @@ class Rollout:
         order; keeps the checkpoints (state and slot order)."""
         self._set_material(E, nu, phi)
         self._material_args = (float(E), float(nu), float(phi))
-        planes = self._init.clone()
-        planes[L.P_V:L.P_V + 3] = v0.to(self.device, torch.float32).t()
+        start, order0 = self._sorted_start(v0)
         self._checkpoints = {}
         lib = L.load()
         self._recorded = self._records()
         if self._recorded:
-            return self._forward_recorded(planes)
+            return self._forward_recorded(start, order0)
         self._release_record()
+        planes = start
         for c0 in range(0, self.n_substeps, self.K):
-            planes, order = self._sort(planes)
+            if c0 == 0:
+                # a copy: the substeps count their failures into the order buffer
+                order = torch.empty_like(order0)
+                self._copy_order(order, order0)
+            else:
+                planes, order = self._sort(planes)
             self._checkpoints[c0] = (planes.clone(), order)
             L.check(lib.mpm_run_ordered(self._prm, self._state(planes), self._aligned(order), self._cols,
                                         self._ncol, self._aligned(self._ws_run), self._ws_run_bytes, c0, self.dt,
@@ -205,9 +229,10 @@ class Rollout:
         self._tape_states, self._tape_grids, self._tape_orders = [], [], []
         self._tape_last = None
 
-    def _forward_recorded(self, planes):
+    def _forward_recorded(self, planes, order0):
         """forward() keeping every substep: per segment the sort into the
-        record's state of its first substep, then mpm_record_segment."""
+        record's state of its first substep (the first segment's state and
+        order: the sorted start, copied), then mpm_record_segment."""
         lib = L.load()
         n_seg = (self.n_substeps + self.K - 1) // self.K
         while len(self._tape_states) < self.n_substeps:
@@ -221,7 +246,11 @@ class Rollout:
         last = self._tape_last
         for s_i, c0 in enumerate(range(0, self.n_substeps, self.K)):
             nk = min(self.K, self.n_substeps - c0)
-            self._sort(planes, dst=self._tape_states[c0], order=self._tape_orders[s_i])
+            if s_i == 0:
+                self._tape_states[0].copy_(planes)
+                self._copy_order(self._tape_orders[0], order0)
+            else:
+                self._sort(planes, dst=self._tape_states[c0], order=self._tape_orders[s_i])
             states, grids = self._segment_args(c0, nk)
             # the last substep's P2G of the next state goes straight into the
             # next segment's first grid (the sort does not change the grid)
